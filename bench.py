@@ -1,0 +1,152 @@
+#!/usr/bin/env python3
+"""Benchmark of the SBP hot path on one MI355X.
+
+A step is one full partition() (GSAP/partition.hpp:852-954, timed like GSAP.cu:24-30) of
+the 200,000-vertex directed DC-SBM graph (71 planted blocks, Dirichlet(1) sizes,
+intra/inter ratio 1.5, ~4.5M edges, seed 3) with the graph already resident in HBM.
+
+value = nodal-move edges/s = E * sweeps / sum of sweep wall time, where a sweep is the body
+of partition.hpp:912-935 (4 nodal batches, each with its block-matrix rebuild, plus the
+description length).  The roofline object is computed for the dominant kernel from HIP
+events recorded on its stream inside the timed steps.  The CPU baseline times the oracle
+(single thread) on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import gsap_amd  # noqa: E402
+
+CONFIGS = {
+    "c1": dict(N=1000, K=11, alpha=0.0, rho=4.0, mean_deg=20.0, seed=1),
+    "c2": dict(N=20000, K=33, alpha=1.0, rho=1.5, mean_deg=20.0, seed=2),
+    "c3": dict(N=200000, K=71, alpha=1.0, rho=1.5, mean_deg=22.5, seed=3),
+}
+HBM_PEAK_GBS = 8000.0
+KCLASS = ["nodal", "rebuild", "block_terms", "merge", "mdl", "accept_sum"]
+
+
+def algorithmic_bytes(kclass, g, nb, B, nnz):
+    """Algorithmic HBM bytes of one launch (int32 ids and weights)."""
+    E, N = g.E, g.N
+    if kclass == "nodal":       # one batch: its vertices' out+in adjacency + per-vertex words
+        return (16 * E + 24 * N) / nb
+    if kclass == "rebuild":     # out-adjacency + labels read, CSR+CSC+degrees written
+        return 8 * E + 8 * N + 16 * nnz + 16 * B
+    return None
+
+
+def cpu_baseline(g, labels, B, seconds):
+    """Single-thread oracle: nodal batches (each with its rebuild) from the GPU's final
+    labelling, for about `seconds`."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    opts = gsap_amd.Options(seed=1)
+    lab = labels.copy()
+    t0 = time.perf_counter()
+    nbat = 0
+    while True:
+        O.nodal_batch(g, opts, lab, B, nbat % opts.num_batches, 100000 + nbat)
+        nbat += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or nbat >= 64:
+            break
+    sweeps = nbat / opts.num_batches
+    return dict(value=g.E * sweeps / el, unit="edges/s", cores=1, kind="port",
+                sample=f"{nbat} nodal batches (each with its block-matrix rebuild) of the CPU "
+                       f"oracle on the same 200k-vertex graph from the final labelling "
+                       f"(B={B}), {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c3", choices=list(CONFIGS))
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU: this build partitions one graph per GPU and has no "
+              f"multi-GPU path (asked for --gpus {a.gpus})", file=sys.stderr)
+        sys.exit(2)
+    cfg = CONFIGS[a.config]
+    t0 = time.perf_counter()
+    g = gsap_amd.generate_dcsbm(**cfg)
+    t_gen = time.perf_counter() - t0
+    prec = gsap_amd.PRECISION_FP64 if a.precision == "fp64" else gsap_amd.PRECISION_MIXED
+    ctx = gsap_amd.Context(0)
+    for w in range(a.warmup):
+        ctx.partition(g, gsap_amd.Options(seed=1, precision=prec))
+        print(f"warmup {w} done", file=sys.stderr, flush=True)
+    walls, reps = [], []
+    for k in range(a.steps):
+        t1 = time.perf_counter()
+        lab, rep = ctx.partition(g, gsap_amd.Options(seed=1, precision=prec, profile=1))
+        walls.append(time.perf_counter() - t1)
+        reps.append(rep)
+        print(f"step {k}: {walls[-1]:.3f} s, B={rep.B}, sweeps={rep.sweeps}", file=sys.stderr,
+              flush=True)
+    sweep_edges = sum(r.sweep_edges for r in reps)
+    t_sweep = sum(r.t_sweep for r in reps)
+    value = sweep_edges / t_sweep
+    kms = np.sum([r.kernel_ms for r in reps], axis=0)
+    kcalls = np.sum([r.kernel_calls for r in reps], axis=0)
+    dom = int(np.argmax(kms[:len(KCLASS)]))
+    dname = KCLASS[dom]
+    avg_ms = kms[dom] / max(kcalls[dom], 1)
+    final = ctx.blockmodel(g, lab, rep.B)
+    abytes = algorithmic_bytes(dname, g, 4, rep.B, final["nnz"])
+    if abytes is None:
+        dname, dom = "nodal", 0
+        avg_ms = kms[0] / max(kcalls[0], 1)
+        abytes = algorithmic_bytes("nodal", g, 4, rep.B, final["nnz"])
+    achieved = abytes / (avg_ms * 1e-3) / 1e9
+    p, r = gsap_amd.pairwise(g.truth, lab)
+    out = {
+        "metric": "nodal-move edges/s (full partition of the 200k-vertex graph); full-partition "
+                  "seconds and % HBM peak reported alongside",
+        "value": value,
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * float(np.mean(walls)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == gsap_amd.PRECISION_FP64 else "f32+f64",
+        "data": "synthetic (seeded DC-SBM generator)",
+        "config": {"workload": f"partition {a.config}: N={g.N}, E={g.E}, K={cfg['K']}, "
+                               f"alpha={cfg['alpha']}, rho={cfg['rho']}, seed={cfg['seed']}",
+                   "precision": a.precision, "parallelism": "single GPU"},
+        "full_partition_s": float(np.mean(walls)),
+        "final_B": rep.B, "final_S": rep.S, "sweeps": rep.sweeps, "levels": rep.levels,
+        "pairwise_precision": p, "pairwise_recall": r,
+        "t_merge_s": float(np.mean([x.t_merge for x in reps])),
+        "t_sweep_s": float(np.mean([x.t_sweep for x in reps])),
+        "graph_generation_s": t_gen,
+        "kernel_ms_per_step": {KCLASS[i]: float(kms[i] / a.steps) for i in range(len(KCLASS))},
+        "kernel_calls_per_step": {KCLASS[i]: int(kcalls[i] // a.steps) for i in range(len(KCLASS))},
+        "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "avg_launch_ms": avg_ms,
+                     "algorithmic_bytes_per_launch": abytes},
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(g, lab, rep.B, a.cpu_seconds)
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,252 @@
+// Device/host primitives shared by every kernel of the SBP hot path.
+//
+//  * Philox4x32-10 counter-based RNG (Salmon et al., SC'11) replacing the clock-seeded
+//    curand streams of GSAP/kernels.hpp:645-676, 859-874.
+//  * Integer sampling helpers (mulhi, the random-vs-neighbour coin of kernels.hpp:720,
+//    847, 1024 as an exact integer comparison).
+//  * The canonical FP64 log2 and the MDL term w*(log2 w - (log2 a + log2 b)) with the
+//    operand grouping of kernels.hpp:173-177.
+//  * The canonical 64-way summation: 64 strided partials (position k goes to partial
+//    k mod 64, ascending k), reduced by a binary tree with offsets 32..1.  A group of G
+//    lanes (G | 64) evaluates it with 64/G partials per lane, so every group width
+//    gives the same bits.
+#pragma once
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "log2_table.h"
+
+#define GSAP_HD __host__ __device__ __forceinline__
+#define GSAP_D __device__ __forceinline__
+
+namespace gsap {
+
+// ------------------------------------------------------------------ Philox
+constexpr uint32_t PH_M0 = 0xD2511F53u, PH_M1 = 0xCD9E8D57u;
+constexpr uint32_t PH_W0 = 0x9E3779B9u, PH_W1 = 0xBB67AE85u;
+// counter word 3: purpose of the draw
+constexpr uint32_t TAG_MERGE = 0x4D455247u;  // block-merge proposal (r, i)
+constexpr uint32_t TAG_NODE = 0x4E4F4445u;   // nodal proposal of one vertex
+constexpr uint32_t TAG_BLOCK = 0x424C4B53u;  // nodal per-block sample and coin
+constexpr uint32_t TAG_GEN = 0x47454E52u;    // graph generator
+
+struct Philox {
+  uint32_t x0, x1, x2, x3;
+};
+
+GSAP_HD Philox philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                      uint32_t k1) {
+#pragma unroll
+  for (int round = 0; round < 10; ++round) {
+    uint64_t p0 = (uint64_t)PH_M0 * c0;
+    uint64_t p1 = (uint64_t)PH_M1 * c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    uint32_t n1 = (uint32_t)p1;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    uint32_t n3 = (uint32_t)p0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    k0 += PH_W0; k1 += PH_W1;
+  }
+  return Philox{c0, c1, c2, c3};
+}
+
+// uniform integer in [0, n) from a 32-bit draw (n < 2^32)
+GSAP_HD uint32_t mulhi(uint32_t x, uint64_t n) { return (uint32_t)(((uint64_t)x * n) >> 32); }
+
+// true with probability B/(deg+B): kernels.hpp:720 + 847 / 1024 as integers
+GSAP_HD bool coin_random(uint32_t x, int64_t deg, int64_t B) {
+  return (uint64_t)x * (uint64_t)(deg + B) < ((uint64_t)B << 32);
+}
+
+// ---------------------------------------------------------- canonical log2
+// log2(x) = e + log2(c_j) + P(r),  x = 2^e m,  c_j = 1 + j/128 (top 7 mantissa bits),
+// r = fma(m, 1/c_j, -1),  P(r) = sum_{k=1..9} (-1)^(k+1) r^k / (k ln 2) by Horner with fma.
+static __constant__ double d_log2_inv[128] = {GSAP_LOG2_INV_VALUES};
+static __constant__ double d_log2_lg[128] = {GSAP_LOG2_LG_VALUES};
+static __constant__ double d_log2_poly[9] = {GSAP_LOG2_POLY_VALUES};
+static const double h_log2_inv[128] = {GSAP_LOG2_INV_VALUES};
+static const double h_log2_lg[128] = {GSAP_LOG2_LG_VALUES};
+static const double h_log2_poly[9] = {GSAP_LOG2_POLY_VALUES};
+
+GSAP_HD double clog2_core(double x, const double *inv, const double *lg, const double *poly) {
+  if (x == 0.0) return -__builtin_huge_val();
+  if (!(x > 0.0)) return __builtin_nan("");
+  if (x == __builtin_huge_val()) return x;
+  uint64_t bits = __builtin_bit_cast(uint64_t, x);
+  int e = (int)((bits >> 52) & 0x7ff);
+  if (e == 0) {
+    double y = x * 18014398509481984.0;  // 2^54
+    bits = __builtin_bit_cast(uint64_t, y);
+    e = (int)((bits >> 52) & 0x7ff) - 54;
+  }
+  e -= 1023;
+  int j = (int)((bits >> 45) & 127);
+  double m = __builtin_bit_cast(double, (bits & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
+  double r = __builtin_fma(m, inv[j], -1.0);
+  double p = poly[8];
+  p = __builtin_fma(p, r, poly[7]);
+  p = __builtin_fma(p, r, poly[6]);
+  p = __builtin_fma(p, r, poly[5]);
+  p = __builtin_fma(p, r, poly[4]);
+  p = __builtin_fma(p, r, poly[3]);
+  p = __builtin_fma(p, r, poly[2]);
+  p = __builtin_fma(p, r, poly[1]);
+  p = __builtin_fma(p, r, poly[0]);
+  p = p * r;
+  return ((double)e + lg[j]) + p;
+}
+
+GSAP_D double clog2(double x) { return clog2_core(x, d_log2_inv, d_log2_lg, d_log2_poly); }
+inline double clog2_host(double x) { return clog2_core(x, h_log2_inv, h_log2_lg, h_log2_poly); }
+
+// w * (log2 w - (la + lb)), the term of kernels.hpp:173-177
+GSAP_D double term(int64_t w, double la, double lb) {
+  double dw = (double)w;
+  return dw * (clog2(dw) - (la + lb));
+}
+
+// ------------------------------------------------------- canonical group sum
+template <int G>
+struct CAcc {
+  static constexpr int S = 64 / G;
+  double p[S];
+  GSAP_D void init() {
+#pragma unroll
+    for (int j = 0; j < S; ++j) p[j] = 0.0;
+  }
+  // valid in the group's lane 0
+  GSAP_D double reduce() {
+#pragma unroll
+    for (int off = 32; off >= G; off >>= 1) {
+      constexpr int dummy = 0;
+      (void)dummy;
+      const int d = off / G;
+#pragma unroll
+      for (int j = 0; j < S; ++j)
+        if (j < d) p[j] += p[j + d];
+    }
+    double x = p[0];
+#pragma unroll
+    for (int off = G / 2; off >= 1; off >>= 1) x += __shfl_down(x, off, G);
+    return x;
+  }
+};
+
+// lanes of a group, lane index inside the group
+template <int G>
+GSAP_D int glane() {
+  return (int)(threadIdx.x & (G - 1));
+}
+
+// ------------------------------------------------------------- hash tables
+// Open addressing, linear probing; key -1 = empty.  Works on LDS or global
+// memory through generic pointers; reads use atomic loads so that global
+// tables never see stale L1 lines.
+struct HTab {
+  int32_t *key;
+  int32_t *w;
+  int32_t *first;
+  uint32_t bits;
+};
+
+GSAP_D uint32_t ht_slot(int32_t t, uint32_t bits) {
+  return bits == 0 ? 0u : (((uint32_t)t * 2654435761u) >> (32 - bits));
+}
+GSAP_D int32_t ld_relaxed(const int32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+GSAP_D void ht_clear(const HTab &h, int lane, int nl) {
+  uint32_t cap = 1u << h.bits;
+  for (uint32_t i = lane; i < cap; i += nl) {
+    __hip_atomic_store(&h.key[i], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&h.w[i], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&h.first[i], 0x7fffffff, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+GSAP_D void ht_insert(const HTab &h, int32_t t, int32_t w, int32_t pos) {
+  uint32_t mask = (1u << h.bits) - 1u;
+  uint32_t s = ht_slot(t, h.bits);
+  for (;;) {
+    int32_t old = atomicCAS(&h.key[s], -1, t);
+    if (old == -1 || old == t) {
+      atomicAdd(&h.w[s], w);
+      atomicMin(&h.first[s], pos);
+      return;
+    }
+    s = (s + 1) & mask;
+  }
+}
+// slot of key t, or -1
+GSAP_D int32_t ht_find(const HTab &h, int32_t t) {
+  uint32_t mask = (1u << h.bits) - 1u;
+  uint32_t s = ht_slot(t, h.bits);
+  for (;;) {
+    int32_t k = ld_relaxed(&h.key[s]);
+    if (k == t) return (int32_t)s;
+    if (k == -1) return -1;
+    s = (s + 1) & mask;
+  }
+}
+GSAP_D int32_t ht_weight(const HTab &h, int32_t t) {
+  int32_t s = ht_find(h, t);
+  return s < 0 ? 0 : ld_relaxed(&h.w[s]);
+}
+
+// ------------------------------------------------------------ block matrix
+// Device view of the block matrix M (CSR + CSC, degrees, cached logs).
+struct BMView {
+  int32_t B;
+  const int32_t *rowptr, *col, *val;     // CSR, columns ascending
+  const int32_t *colptr, *row, *cval;    // CSC, rows ascending
+  const int32_t *dout, *din;
+  const double *ldout, *ldin;
+  const int32_t *dense;                  // B*B row-major, or nullptr
+};
+
+// M[r][t] (0 when absent)
+GSAP_D int32_t bm_row_get(const BMView &m, int32_t r, int32_t t) {
+  if (m.dense) return m.dense[(int64_t)r * m.B + t];
+  int32_t lo = m.rowptr[r], hi = m.rowptr[r + 1];
+  while (lo < hi) {
+    int32_t mid = (lo + hi) >> 1;
+    if (m.col[mid] < t) lo = mid + 1; else hi = mid;
+  }
+  return (lo < m.rowptr[r + 1] && m.col[lo] == t) ? m.val[lo] : 0;
+}
+// M[t][s] via column s (0 when absent)
+GSAP_D int32_t bm_col_get(const BMView &m, int32_t s, int32_t t) {
+  if (m.dense) return m.dense[(int64_t)t * m.B + s];
+  int32_t lo = m.colptr[s], hi = m.colptr[s + 1];
+  while (lo < hi) {
+    int32_t mid = (lo + hi) >> 1;
+    if (m.row[mid] < t) lo = mid + 1; else hi = mid;
+  }
+  return (lo < m.colptr[s + 1] && m.row[lo] == t) ? m.cval[lo] : 0;
+}
+
+// Weighted pick over row x then column x: first entry whose running weight
+// exceeds target, skipping id `skip` (its mass carries forward); `fallback` if
+// none qualifies.  Sequential (kernels.hpp:767-816, 978-995 as integers).
+GSAP_D int32_t bm_walk(const BMView &m, int32_t x, uint64_t target, int32_t skip,
+                       int32_t fallback) {
+  uint64_t cum = 0;
+  for (int32_t k = m.rowptr[x]; k < m.rowptr[x + 1]; ++k) {
+    cum += (uint64_t)m.val[k];
+    if (cum > target && m.col[k] != skip) return m.col[k];
+  }
+  for (int32_t k = m.colptr[x]; k < m.colptr[x + 1]; ++k) {
+    cum += (uint64_t)m.cval[k];
+    if (cum > target && m.row[k] != skip) return m.row[k];
+  }
+  return fallback;
+}
+
+// Total order on doubles as unsigned keys (a radix sort on float bits orders so).
+GSAP_HD uint64_t order_key(double d) {
+  uint64_t u = __builtin_bit_cast(uint64_t, d);
+  return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+
+}  // namespace gsap

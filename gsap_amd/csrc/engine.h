@@ -1,0 +1,81 @@
+// Device state of one partition run and the kernel entry points the host
+// orchestrator (engine.hip) drives.
+#pragma once
+#include "common.h"
+#include "prims.h"
+
+namespace gsap {
+
+// Graph in HBM: out-CSR, in-CSR (N+1 pointers), weighted degrees and the COO source
+// of every out-edge (so edge-parallel kernels read it coalesced).
+struct GraphDev {
+  int64_t N = 0, E = 0;
+  int32_t *optr = nullptr, *oadj = nullptr, *ow = nullptr;
+  int32_t *iptr = nullptr, *iadj = nullptr, *iw = nullptr;
+  int32_t *kout = nullptr, *kin = nullptr, *esrc = nullptr;
+  // nodal hash tables for vertices whose adjacency does not fit the LDS table
+  int64_t *hoff = nullptr;       // per vertex: offset into hpool (words), -1 when light
+  uint8_t *hbits_o = nullptr;    // per vertex: log2 capacity of out table (0 = light)
+  uint8_t *hbits_i = nullptr;
+  int32_t *hpool = nullptr;      // 3 words per slot, out table then in table
+};
+
+// Block matrix of the current labelling.  rowptr[B] holds nnz on the device.
+struct BMBuf {
+  int32_t B = 0;
+  int32_t *rowptr = nullptr, *colptr = nullptr;   // Bmax+1
+  int32_t *col = nullptr, *val = nullptr, *erow = nullptr;  // E
+  int32_t *row = nullptr, *cval = nullptr;        // E
+  int32_t *dout = nullptr, *din = nullptr;        // Bmax
+  double *ldout = nullptr, *ldin = nullptr;       // Bmax
+  int32_t *dense = nullptr;                       // dense_cap words
+  int64_t dense_cap = 0;
+  bool dense_valid = false;
+  BMView view() const {
+    BMView v;
+    v.B = B; v.rowptr = rowptr; v.col = col; v.val = val;
+    v.colptr = colptr; v.row = row; v.cval = cval;
+    v.dout = dout; v.din = din; v.ldout = ldout; v.ldin = ldin;
+    v.dense = dense_valid ? dense : nullptr;
+    return v;
+  }
+};
+
+struct RebuildScratch {
+  void *k0 = nullptr, *k1 = nullptr;        // E x 8 bytes
+  uint32_t *v0 = nullptr, *v1 = nullptr;    // E
+  uint32_t *flags = nullptr, *uidx = nullptr;  // E
+  uint32_t *tmp = nullptr;                  // radix / scan temporary
+  uint32_t *cnt = nullptr;                  // Bmax + 1
+  int32_t *partials = nullptr;              // LDS-histogram partials
+  int64_t partials_cap = 0;
+};
+
+// ---- bm.hip
+void rebuild_blockmodel(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm,
+                        RebuildScratch &rs, int32_t dense_max_B, hipStream_t st);
+
+// ---- sweep.hip
+struct RngKey {
+  uint32_t k0, k1, chain;
+};
+void launch_block_terms(const BMView &m, double *OUT, int32_t *sof, uint8_t *coin, RngKey key,
+                        uint32_t phase, bool sample, int32_t avg_row, hipStream_t st);
+void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const double *OUT,
+                  const int32_t *sof, const uint8_t *coin, RngKey key, uint32_t phase, int64_t lo,
+                  int64_t BS, int32_t *prop, double *dS, hipStream_t st);
+void launch_apply(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo, int64_t BS,
+                  hipStream_t st);
+// canonical array sum: tiles of 1024 (n read from n_dev when non-null, else n_host)
+void launch_csum_array(const double *x, const int32_t *n_dev, int64_t n_host, int64_t n_max,
+                       double *tiles, double *out, hipStream_t st);
+void launch_mdl_terms(const BMView &m, const int32_t *erow, int64_t n_max, double *terms,
+                      hipStream_t st);
+void launch_merge(const BMView &m, const double *OUT, RngKey key, uint32_t phase, int32_t np,
+                  int32_t *prop, double *dS, int32_t avg_row, hipStream_t st);
+void launch_argmin(const double *dS, const int32_t *prop, int32_t B, int32_t np, int32_t *best,
+                   double *dS_best, hipStream_t st);
+void launch_relabel(int32_t *lab, const int32_t *map, int64_t N, hipStream_t st);
+void launch_iota(int32_t *x, int64_t n, hipStream_t st);
+
+}  // namespace gsap

@@ -1,0 +1,775 @@
+// Host orchestration of the SBP hot path on one MI355X and the C ABI of gsap.h.
+//
+// Control flow follows partition() of GSAP/partition.hpp:852-954:
+//   level loop: block merge (propose_block_merge + carry_out_best_merges, :900-904)
+//               -> up to max_nodal_itr sweeps of num_batches nodal batches, each followed
+//                  by a block-matrix rebuild (:912-916)
+//               -> total description length + convergence test (:918-934)
+//               -> golden-section bracket update over B (:938, :80-322)
+// One host thread, one HIP stream, one host<->device sync per sweep (the convergence test
+// needs S) and one per merge phase (the greedy merge order is a sequential union-find on
+// the host, like the reference's host loop).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/gsap/gsap.h"
+#include "engine.h"
+
+using namespace gsap;
+
+namespace {
+
+struct Err : std::runtime_error {
+  int code;
+  Err(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPX(call)                                                                        \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      throw Err(GSAP_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+uint64_t fnv(const void *p, int64_t n, uint64_t h = 1469598103934665603ull) {
+  const unsigned char *c = (const unsigned char *)p;
+  for (int64_t i = 0; i < n; ++i) { h ^= c[i]; h *= 1099511628211ull; }
+  return h;
+}
+
+// Device allocation that remembers its capacity.
+struct DBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  void ensure(size_t bytes) {
+    if (bytes <= cap) return;
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (bytes == 0) return;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      p = nullptr;
+      throw Err(GSAP_ERR_MEMORY, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    cap = bytes;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <class T>
+  T *as() const { return (T *)p; }
+};
+
+}  // namespace
+
+// Optional per-kernel-class device timing with HIP events on the work stream.
+struct Prof {
+  bool on = false;
+  struct Span { int k; hipEvent_t a, b; };
+  std::vector<hipEvent_t> pool;
+  std::vector<Span> pend;
+  double ms[8] = {};
+  int64_t calls[8] = {};
+  hipEvent_t ev() {
+    if (pool.empty()) {
+      hipEvent_t e;
+      HIPX(hipEventCreate(&e));
+      return e;
+    }
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  int begin(int k, hipStream_t st) {
+    if (!on) return -1;
+    Span s{k, ev(), nullptr};
+    HIPX(hipEventRecord(s.a, st));
+    pend.push_back(s);
+    return (int)pend.size() - 1;
+  }
+  void end(int i, hipStream_t st) {
+    if (i < 0) return;
+    pend[i].b = ev();
+    HIPX(hipEventRecord(pend[i].b, st));
+  }
+  // after a stream synchronisation
+  void flush() {
+    for (Span &s : pend) {
+      float t = 0;
+      if (s.b) {
+        HIPX(hipEventElapsedTime(&t, s.a, s.b));
+        ms[s.k] += t;
+        calls[s.k]++;
+        pool.push_back(s.b);
+      }
+      pool.push_back(s.a);
+    }
+    pend.clear();
+  }
+  void reset(bool enable) {
+    flush();
+    on = enable;
+    for (int k = 0; k < 8; ++k) { ms[k] = 0; calls[k] = 0; }
+  }
+  void destroy() {
+    flush();
+    for (hipEvent_t e : pool) hipEventDestroy(e);
+    pool.clear();
+  }
+};
+
+struct gsap_ctx {
+  Prof prof;
+  int device = 0;
+  hipStream_t st = nullptr;
+  std::string err;
+  std::vector<gsap_trace_rec> trace;
+  // graph
+  DBuf optr, oadj, ow, iptr, iadj, iw, kout, kin, esrc, hoff, hbo, hbi, hpool;
+  // labels and golden-section snapshots
+  DBuf lab, snap[3];
+  // block matrix
+  DBuf rowptr, colptr, col, val, erow, row, cval, dout, din, ldout, ldin, dense;
+  // rebuild scratch
+  DBuf k0, k1, v0, v1, flags, uidx, tmp, cnt, partials;
+  // sweep / merge
+  DBuf OUT, sof, coin, prop, dS, best, dSbest, map, terms, tiles, res;
+  GraphDev g;
+  BMBuf bm;
+  RebuildScratch rs;
+  int64_t N = 0, E = 0;
+  int32_t dense_max_B = 1024;
+};
+
+namespace {
+
+void set_device(gsap_ctx *c) { HIPX(hipSetDevice(c->device)); }
+
+// --------------------------------------------------------------- graph upload
+void validate(const gsap_graph_i32 *g) {
+  if (!g) throw Err(GSAP_ERR_INVALID, "graph is NULL");
+  if (g->N <= 0 || g->E < 0) throw Err(GSAP_ERR_INVALID, "N must be > 0 and E >= 0");
+  if (g->N >= (int64_t)1 << 31 || g->E >= (int64_t)1 << 31)
+    throw Err(GSAP_ERR_INVALID, "N and E must fit int32");
+  if (!g->out_ptr || !g->in_ptr || (g->E > 0 && (!g->out_adj || !g->out_w || !g->in_adj || !g->in_w)))
+    throw Err(GSAP_ERR_INVALID, "graph arrays must not be NULL");
+  if (g->out_ptr[0] != 0 || g->in_ptr[0] != 0 || g->out_ptr[g->N] != g->E || g->in_ptr[g->N] != g->E)
+    throw Err(GSAP_ERR_INVALID, "ptr arrays must start at 0 and end at E");
+  for (int64_t v = 0; v < g->N; ++v)
+    if (g->out_ptr[v + 1] < g->out_ptr[v] || g->in_ptr[v + 1] < g->in_ptr[v])
+      throw Err(GSAP_ERR_INVALID, "ptr arrays must be non-decreasing");
+  int64_t wsum = 0;
+  for (int64_t e = 0; e < g->E; ++e) {
+    if (g->out_adj[e] < 0 || g->out_adj[e] >= g->N || g->in_adj[e] < 0 || g->in_adj[e] >= g->N)
+      throw Err(GSAP_ERR_INVALID, "adjacency id out of range");
+    if (g->out_w[e] <= 0 || g->in_w[e] <= 0) throw Err(GSAP_ERR_INVALID, "weights must be > 0");
+    wsum += g->out_w[e];
+  }
+  if (2 * wsum >= ((int64_t)1 << 31) - 1)
+    throw Err(GSAP_ERR_INVALID, "total edge weight must stay below 2^30");
+}
+
+template <class T>
+void h2d(DBuf &b, const T *src, int64_t n, hipStream_t st) {
+  b.ensure(sizeof(T) * (size_t)std::max<int64_t>(n, 1));
+  if (n > 0) HIPX(hipMemcpyAsync(b.p, src, sizeof(T) * n, hipMemcpyHostToDevice, st));
+}
+
+void upload_graph(gsap_ctx *c, const gsap_graph_i32 *g) {
+  validate(g);
+  const int64_t N = g->N, E = g->E;
+  c->N = N; c->E = E;
+  hipStream_t st = c->st;
+  std::vector<int32_t> kout(N, 0), kin(N, 0), esrc(E);
+  for (int64_t v = 0; v < N; ++v) {
+    for (int32_t j = g->out_ptr[v]; j < g->out_ptr[v + 1]; ++j) { kout[v] += g->out_w[j]; esrc[j] = (int32_t)v; }
+    for (int32_t j = g->in_ptr[v]; j < g->in_ptr[v + 1]; ++j) kin[v] += g->in_w[j];
+  }
+  // vertices whose adjacency exceeds half the LDS hash table get global tables
+  std::vector<int64_t> hoff(N, -1);
+  std::vector<uint8_t> hbo(N, 0), hbi(N, 0);
+  int64_t pool = 0;
+  auto bits_for = [](int64_t n) { uint8_t b = 1; while (((int64_t)1 << b) < 2 * n) ++b; return b; };
+  for (int64_t v = 0; v < N; ++v) {
+    int64_t no = g->out_ptr[v + 1] - g->out_ptr[v], ni = g->in_ptr[v + 1] - g->in_ptr[v];
+    if (no > 128 || ni > 128) {
+      hbo[v] = bits_for(std::max<int64_t>(no, 1));
+      hbi[v] = bits_for(std::max<int64_t>(ni, 1));
+      hoff[v] = pool;
+      pool += 3 * (((int64_t)1 << hbo[v]) + ((int64_t)1 << hbi[v]));
+    }
+  }
+  h2d(c->optr, g->out_ptr, N + 1, st);
+  h2d(c->iptr, g->in_ptr, N + 1, st);
+  h2d(c->oadj, g->out_adj, E, st);
+  h2d(c->ow, g->out_w, E, st);
+  h2d(c->iadj, g->in_adj, E, st);
+  h2d(c->iw, g->in_w, E, st);
+  h2d(c->kout, kout.data(), N, st);
+  h2d(c->kin, kin.data(), N, st);
+  h2d(c->esrc, esrc.data(), E, st);
+  h2d(c->hoff, hoff.data(), N, st);
+  h2d(c->hbo, hbo.data(), N, st);
+  h2d(c->hbi, hbi.data(), N, st);
+  c->hpool.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(pool, 1));
+  HIPX(hipStreamSynchronize(st));  // host vectors go out of scope
+  GraphDev &d = c->g;
+  d.N = N; d.E = E;
+  d.optr = c->optr.as<int32_t>(); d.oadj = c->oadj.as<int32_t>(); d.ow = c->ow.as<int32_t>();
+  d.iptr = c->iptr.as<int32_t>(); d.iadj = c->iadj.as<int32_t>(); d.iw = c->iw.as<int32_t>();
+  d.kout = c->kout.as<int32_t>(); d.kin = c->kin.as<int32_t>(); d.esrc = c->esrc.as<int32_t>();
+  d.hoff = c->hoff.as<int64_t>(); d.hbits_o = c->hbo.as<uint8_t>(); d.hbits_i = c->hbi.as<uint8_t>();
+  d.hpool = c->hpool.as<int32_t>();
+}
+
+void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
+  const int64_t N = c->N, E = std::max<int64_t>(c->E, 1);
+  c->dense_max_B = dense_max_B;
+  c->lab.ensure(4 * N);
+  for (auto &s : c->snap) s.ensure(4 * N);
+  c->rowptr.ensure(4 * (N + 1)); c->colptr.ensure(4 * (N + 1));
+  c->col.ensure(4 * E); c->val.ensure(4 * E); c->erow.ensure(4 * E);
+  c->row.ensure(4 * E); c->cval.ensure(4 * E);
+  c->dout.ensure(4 * N); c->din.ensure(4 * N);
+  c->ldout.ensure(8 * N); c->ldin.ensure(8 * N);
+  const int64_t dB = std::min<int64_t>(dense_max_B, N);
+  c->dense.ensure(4 * (size_t)std::max<int64_t>(dB * dB, 1));
+  c->k0.ensure(8 * E); c->k1.ensure(8 * E);
+  c->v0.ensure(4 * E); c->v1.ensure(4 * E); c->flags.ensure(4 * E); c->uidx.ensure(4 * E);
+  size_t tw = std::max(radix_temp_words(E), scan_temp_words(N + 1));
+  c->tmp.ensure(4 * tw);
+  c->cnt.ensure(4 * (N + 1));
+  const int64_t pc = 256 * 128 * 128;
+  c->partials.ensure(4 * pc);
+  c->OUT.ensure(8 * N); c->sof.ensure(4 * N); c->coin.ensure(N);
+  c->prop.ensure(4 * (size_t)np * N); c->dS.ensure(8 * (size_t)np * N);
+  c->best.ensure(4 * N); c->dSbest.ensure(8 * N); c->map.ensure(4 * N);
+  c->terms.ensure(8 * E); c->tiles.ensure(8 * (E / 1024 + 2));
+  c->res.ensure(8 * 64);
+  BMBuf &b = c->bm;
+  b.rowptr = c->rowptr.as<int32_t>(); b.colptr = c->colptr.as<int32_t>();
+  b.col = c->col.as<int32_t>(); b.val = c->val.as<int32_t>(); b.erow = c->erow.as<int32_t>();
+  b.row = c->row.as<int32_t>(); b.cval = c->cval.as<int32_t>();
+  b.dout = c->dout.as<int32_t>(); b.din = c->din.as<int32_t>();
+  b.ldout = c->ldout.as<double>(); b.ldin = c->ldin.as<double>();
+  b.dense = c->dense.as<int32_t>(); b.dense_cap = (int64_t)(c->dense.cap / 4);
+  RebuildScratch &r = c->rs;
+  r.k0 = c->k0.p; r.k1 = c->k1.p; r.v0 = c->v0.as<uint32_t>(); r.v1 = c->v1.as<uint32_t>();
+  r.flags = c->flags.as<uint32_t>(); r.uidx = c->uidx.as<uint32_t>(); r.tmp = c->tmp.as<uint32_t>();
+  r.cnt = c->cnt.as<uint32_t>(); r.partials = c->partials.as<int32_t>(); r.partials_cap = pc;
+}
+
+int32_t resolve_dense(const gsap_options &o) { return o.dense_max_B > 0 ? o.dense_max_B : 1024; }
+
+RngKey rng_key(const gsap_options &o) {
+  return RngKey{(uint32_t)o.seed, (uint32_t)(o.seed >> 32), o.chain};
+}
+
+void rebuild(gsap_ctx *c, int32_t B) {
+  int p = c->prof.begin(1, c->st);
+  rebuild_blockmodel(c->g, c->lab.as<int32_t>(), B, c->bm, c->rs, c->dense_max_B, c->st);
+  c->prof.end(p, c->st);
+  HIPX(hipGetLastError());
+}
+
+int32_t read_nnz(gsap_ctx *c) {
+  int32_t nnz = 0;
+  HIPX(hipMemcpyAsync(&nnz, c->bm.rowptr + c->bm.B, 4, hipMemcpyDeviceToHost, c->st));
+  HIPX(hipStreamSynchronize(c->st));
+  return nnz;
+}
+
+int32_t avg_row(int64_t nnz, int32_t B) { return (int32_t)(B > 0 ? (nnz + B - 1) / B : 0); }
+
+// total description length from the canonical data sum (partition.hpp:665-704, FP64)
+double model_S(double data_S, int64_t N, int64_t E, int32_t B) {
+  const double LN2 = 0.6931471805599453094;
+  double b = (double)B, dE = (double)E;
+  double x = (b * b) / dE;
+  double model = dE * (1.0 + x) * (clog2_host(1.0 + x) * LN2) - x * (clog2_host(x) * LN2) +
+                 (double)N * (clog2_host(b) * LN2);
+  return model - data_S;
+}
+
+void launch_mdl(gsap_ctx *c, double *out) {
+  int p = c->prof.begin(4, c->st);
+  BMView m = c->bm.view();
+  launch_mdl_terms(m, c->bm.erow, c->E, c->terms.as<double>(), c->st);
+  launch_csum_array(c->terms.as<double>(), c->bm.rowptr + c->bm.B, 0, c->E,
+                    c->tiles.as<double>(), out, c->st);
+  c->prof.end(p, c->st);
+}
+
+double mdl_now(gsap_ctx *c) {
+  double *res = c->res.as<double>();
+  launch_mdl(c, res);
+  double h = 0;
+  HIPX(hipMemcpyAsync(&h, res, 8, hipMemcpyDeviceToHost, c->st));
+  HIPX(hipStreamSynchronize(c->st));
+  return model_S(h, c->N, c->E, c->bm.B);
+}
+
+int32_t find_root(std::vector<int32_t> &par, int32_t x) {
+  int32_t r = x;
+  while (par[r] != r) r = par[r];
+  while (par[x] != r) { int32_t n = par[x]; par[x] = r; x = n; }
+  return r;
+}
+
+// One merge phase: proposals + delta-S + argmin on the device, greedy merge order on the
+// host (carry_out_best_merges, partition.hpp:26-78, as union-find), relabel on the device.
+int32_t merge_phase(gsap_ctx *c, const gsap_options &o, int32_t B_to_merge, uint32_t phase,
+                    std::vector<int32_t> *prop_out, std::vector<double> *dS_out,
+                    std::vector<int32_t> *best_out, std::vector<double> *dSb_out) {
+  const int32_t B = c->bm.B, np = o.num_proposals;
+  hipStream_t st = c->st;
+  BMView m = c->bm.view();
+  int32_t ar = avg_row(read_nnz(c), B);
+  int p = c->prof.begin(2, st);
+  launch_block_terms(m, c->OUT.as<double>(), nullptr, nullptr, rng_key(o), phase, false, ar, st);
+  c->prof.end(p, st);
+  p = c->prof.begin(3, st);
+  launch_merge(m, c->OUT.as<double>(), rng_key(o), phase, np, c->prop.as<int32_t>(),
+               c->dS.as<double>(), ar, st);
+  c->prof.end(p, st);
+  launch_argmin(c->dS.as<double>(), c->prop.as<int32_t>(), B, np, c->best.as<int32_t>(),
+                c->dSbest.as<double>(), st);
+  HIPX(hipGetLastError());
+  std::vector<int32_t> best(B);
+  std::vector<double> dsb(B);
+  HIPX(hipMemcpyAsync(best.data(), c->best.p, 4 * (size_t)B, hipMemcpyDeviceToHost, st));
+  HIPX(hipMemcpyAsync(dsb.data(), c->dSbest.p, 8 * (size_t)B, hipMemcpyDeviceToHost, st));
+  if (prop_out) {
+    prop_out->resize((size_t)B * np);
+    dS_out->resize((size_t)B * np);
+    HIPX(hipMemcpyAsync(prop_out->data(), c->prop.p, 4 * (size_t)B * np, hipMemcpyDeviceToHost, st));
+    HIPX(hipMemcpyAsync(dS_out->data(), c->dS.p, 8 * (size_t)B * np, hipMemcpyDeviceToHost, st));
+  }
+  HIPX(hipStreamSynchronize(st));
+  c->prof.flush();
+  if (best_out) *best_out = best;
+  if (dSb_out) *dSb_out = dsb;
+  // stable ascending order of the best delta-S (thrust::sort_by_key, partition.hpp:442-445)
+  std::vector<uint64_t> key(B);
+  for (int32_t r = 0; r < B; ++r) key[r] = order_key(dsb[r]);
+  std::vector<int32_t> order(B);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+  std::vector<int32_t> par(B);
+  std::iota(par.begin(), par.end(), 0);
+  int32_t merged = 0, cidx = 0;
+  while (merged < B_to_merge && cidx < B) {
+    int32_t from = order[cidx++];
+    int32_t to = find_root(par, best[from]);
+    if (to != from) { par[from] = to; ++merged; }
+  }
+  std::vector<int32_t> rank(B, -1), map(B);
+  int32_t nb = 0;
+  for (int32_t x = 0; x < B; ++x)
+    if (find_root(par, x) == x) rank[x] = nb++;
+  for (int32_t x = 0; x < B; ++x) map[x] = rank[find_root(par, x)];
+  HIPX(hipMemcpyAsync(c->map.p, map.data(), 4 * (size_t)B, hipMemcpyHostToDevice, st));
+  launch_relabel(c->lab.as<int32_t>(), c->map.as<int32_t>(), c->N, st);
+  HIPX(hipGetLastError());
+  HIPX(hipStreamSynchronize(st));  // map goes out of scope
+  return B - merged;
+}
+
+// One nodal batch on the current block matrix; the accepted delta-S sum lands in *dE_dev.
+void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, uint32_t phase, int32_t ar,
+                 double *dE_dev) {
+  const int64_t BS = c->N / o.num_batches, lo = (int64_t)b * BS;
+  hipStream_t st = c->st;
+  BMView m = c->bm.view();
+  int p = c->prof.begin(2, st);
+  launch_block_terms(m, c->OUT.as<double>(), c->sof.as<int32_t>(), c->coin.as<uint8_t>(),
+                     rng_key(o), phase, true, ar, st);
+  c->prof.end(p, st);
+  p = c->prof.begin(0, st);
+  launch_nodal(c->g, c->lab.as<int32_t>(), m, c->OUT.as<double>(), c->sof.as<int32_t>(),
+               c->coin.as<uint8_t>(), rng_key(o), phase, lo, BS, c->prop.as<int32_t>(),
+               c->dS.as<double>(), st);
+  c->prof.end(p, st);
+  p = c->prof.begin(5, st);
+  launch_apply(c->lab.as<int32_t>(), c->prop.as<int32_t>(), c->dS.as<double>(), lo, BS, st);
+  launch_csum_array(c->dS.as<double>(), nullptr, BS, BS, c->tiles.as<double>(), dE_dev, st);
+  c->prof.end(p, st);
+  HIPX(hipGetLastError());
+}
+
+void check_options(const gsap_options &o) {
+  if (o.num_proposals < 1 || o.num_proposals > 32)
+    throw Err(GSAP_ERR_INVALID, "num_proposals must be in [1, 32]");
+  if (o.num_batches < 1 || o.num_batches > 63)
+    throw Err(GSAP_ERR_INVALID, "num_batches must be in [1, 63]");
+  if (o.max_nodal_itr < 1) throw Err(GSAP_ERR_INVALID, "max_nodal_itr must be >= 1");
+  if (o.moving_avg_window < 2) throw Err(GSAP_ERR_INVALID, "moving_avg_window must be >= 2");
+  if (o.precision != GSAP_PRECISION_FP64 && o.precision != GSAP_PRECISION_MIXED)
+    throw Err(GSAP_ERR_INVALID, "unknown precision");
+}
+
+struct Slot {
+  int32_t B = 0, B_to_merge = 0;
+  double S = std::numeric_limits<double>::infinity();
+  int idx = 0;  // snapshot buffer
+};
+
+void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &o, int32_t *out,
+                    gsap_report *rep) {
+  check_options(o);
+  const double t0 = now();
+  set_device(c);
+  upload_graph(c, gin);
+  alloc_work(c, o.num_proposals, resolve_dense(o));
+  c->trace.clear();
+  c->prof.reset(o.profile != 0);
+  const bool tr = o.trace != 0;
+  const int64_t N = c->N;
+  hipStream_t st = c->st;
+  int32_t *lab = c->lab.as<int32_t>();
+  double *res = c->res.as<double>();
+  launch_iota(lab, N, st);
+  int32_t B = (int32_t)N;
+  int32_t B_to_merge = (int32_t)((float)B * o.reduction_rate);
+  rebuild(c, B);
+  Slot large, med, small;
+  large.idx = 0; med.idx = 1; small.idx = 2;
+  uint32_t merge_ctr = 0, batch_ctr = 0;
+  int32_t levels = 0, sweeps = 0;
+  int64_t batches = 0;
+  double S = 0.0, t_merge = 0.0, t_sweep = 0.0;
+  bool optimal = false;
+  const int64_t BS = N / o.num_batches;
+  std::vector<int32_t> hprop, hlab;
+  std::vector<double> hdS;
+  auto rec = [&](int32_t kind, int32_t Bv, int32_t a, int32_t b, uint64_t h1, uint64_t h2,
+                 double x, double y) {
+    c->trace.push_back(gsap_trace_rec{kind, Bv, a, b, h1, h2, x, y});
+  };
+  auto labels_hash = [&]() {
+    hlab.resize(N);
+    HIPX(hipMemcpyAsync(hlab.data(), lab, 4 * (size_t)N, hipMemcpyDeviceToHost, st));
+    HIPX(hipStreamSynchronize(st));
+    return fnv(hlab.data(), 4 * N);
+  };
+  while (!optimal) {
+    const double tm = now();
+    int32_t nB = merge_phase(c, o, B_to_merge, merge_ctr, tr ? &hprop : nullptr,
+                             tr ? &hdS : nullptr, nullptr, nullptr);
+    if (tr) rec(1, B, nB, (int32_t)merge_ctr, fnv(hprop.data(), 4 * (int64_t)hprop.size()),
+                labels_hash(), 0.0, 0.0);
+    ++merge_ctr;
+    B = nB;
+    rebuild(c, B);
+    int32_t ar = avg_row(read_nnz(c), B);
+    t_merge += now() - tm;
+    std::vector<double> dE(o.max_nodal_itr, 0.0);
+    for (int itr = 0; itr < o.max_nodal_itr; ++itr) {
+      const double ts = now();
+      for (int b = 0; b < o.num_batches; ++b) {
+        nodal_batch(c, o, b, batch_ctr, ar, res + 1 + b);
+        rebuild(c, B);
+        if (tr) {
+          double e = 0;
+          hprop.resize(BS);
+          HIPX(hipMemcpyAsync(hprop.data(), c->prop.p, 4 * (size_t)BS, hipMemcpyDeviceToHost, st));
+          HIPX(hipMemcpyAsync(&e, res + 1 + b, 8, hipMemcpyDeviceToHost, st));
+          HIPX(hipStreamSynchronize(st));
+          rec(2, B, itr, b, fnv(hprop.data(), 4 * BS), labels_hash(), e, 0.0);
+        }
+        ++batch_ctr;
+        ++batches;
+      }
+      launch_mdl(c, res);
+      double h[64];
+      int32_t nnz = 0;
+      HIPX(hipMemcpyAsync(h, res, 8 * (size_t)(1 + o.num_batches), hipMemcpyDeviceToHost, st));
+      HIPX(hipMemcpyAsync(&nnz, c->bm.rowptr + B, 4, hipMemcpyDeviceToHost, st));
+      HIPX(hipStreamSynchronize(st));
+      c->prof.flush();
+      ar = avg_row(nnz, B);
+      for (int b = 0; b < o.num_batches; ++b) dE[itr] = dE[itr] + h[1 + b];
+      S = model_S(h[0], N, c->E, B);
+      ++sweeps;
+      t_sweep += now() - ts;
+      if (tr) {
+        std::vector<int32_t> col(nnz), val(nnz);
+        HIPX(hipMemcpyAsync(col.data(), c->bm.col, 4 * (size_t)nnz, hipMemcpyDeviceToHost, st));
+        HIPX(hipMemcpyAsync(val.data(), c->bm.val, 4 * (size_t)nnz, hipMemcpyDeviceToHost, st));
+        HIPX(hipStreamSynchronize(st));
+        rec(3, B, itr, 0, fnv(col.data(), 4 * (int64_t)nnz), fnv(val.data(), 4 * (int64_t)nnz), S,
+            dE[itr]);
+      }
+      const int W = o.moving_avg_window;
+      if (itr >= W - 1) {
+        bool isf = std::isfinite(large.S) && std::isfinite(med.S) && std::isfinite(small.S);
+        double mean = 0.0;
+        for (int i = itr - W + 1; i < itr; ++i) mean += dE[i];
+        mean /= (double)(W - 1);
+        if (!isf) { if (-mean < o.threshold1 * S) break; }
+        else { if (-mean < o.threshold2 * S) break; }
+      }
+    }
+    // golden section (partition.hpp:80-322); snapshots hold labels only, the block
+    // matrix of a restored snapshot is rebuilt from them
+    int32_t PB = B, PBm = B_to_merge;
+    double PS = S;
+    auto save = [&](Slot &s) {
+      s.B = PB; s.B_to_merge = PBm; s.S = PS;
+      HIPX(hipMemcpyAsync(c->snap[s.idx].p, lab, 4 * (size_t)N, hipMemcpyDeviceToDevice, st));
+    };
+    if (PS <= med.S) {
+      if (med.B > PB) std::swap(large, med);
+      else std::swap(small, med);
+      save(med);
+    } else {
+      save((med.B > PB) ? small : large);
+    }
+    const Slot *from;
+    if (std::isinf(small.S)) {
+      from = &med;
+      B_to_merge = (int32_t)((float)med.B * o.reduction_rate);
+      if (B_to_merge == 0) optimal = true;
+    } else if (large.B - small.B == 2) {
+      optimal = true;
+      from = &med;
+      B_to_merge = med.B_to_merge;
+    } else if ((large.B - med.B) >= (med.B - small.B)) {
+      int32_t next_B = med.B + (int32_t)std::round((double)(large.B - med.B) * 0.618);
+      from = &large;
+      B_to_merge = large.B - next_B;
+    } else {
+      int32_t next_B = small.B + (int32_t)std::round((double)(med.B - small.B) * 0.618);
+      from = &med;
+      B_to_merge = med.B - next_B;
+    }
+    if (B_to_merge <= 0 && !optimal) optimal = true;
+    B = from->B;
+    S = from->S;
+    HIPX(hipMemcpyAsync(lab, c->snap[from->idx].p, 4 * (size_t)N, hipMemcpyDeviceToDevice, st));
+    rebuild(c, B);
+    ++levels;
+    if (tr) rec(4, B, B_to_merge, optimal ? 1 : 0, labels_hash(), 0, S, 0.0);
+    if (o.max_levels > 0 && levels >= o.max_levels) break;
+  }
+  HIPX(hipMemcpyAsync(out, lab, 4 * (size_t)N, hipMemcpyDeviceToHost, st));
+  HIPX(hipStreamSynchronize(st));
+  c->prof.flush();
+  if (rep) {
+    std::memset(rep, 0, sizeof(*rep));
+    rep->B = B; rep->levels = levels; rep->merge_phases = (int32_t)merge_ctr;
+    rep->sweeps = sweeps; rep->batches = batches; rep->S = S;
+    rep->t_total = now() - t0; rep->t_merge = t_merge; rep->t_sweep = t_sweep;
+    rep->sweep_edges = c->E * (int64_t)sweeps;
+    rep->n_trace = (int64_t)c->trace.size();
+    for (int k = 0; k < 8; ++k) { rep->kernel_ms[k] = c->prof.ms[k]; rep->kernel_calls[k] = c->prof.calls[k]; }
+  }
+}
+
+gsap_options defaults() {
+  gsap_options o;
+  std::memset(&o, 0, sizeof(o));
+  o.seed = 1;
+  o.num_proposals = 10;
+  o.reduction_rate = 0.35f;
+  o.max_nodal_itr = 100;
+  o.num_batches = 4;
+  o.threshold1 = 5e-4;
+  o.threshold2 = 1e-4;
+  o.moving_avg_window = 3;
+  o.max_levels = 0;
+  o.precision = GSAP_PRECISION_FP64;
+  o.chain = 0;
+  o.trace = 0;
+  o.dense_max_B = 0;
+  o.profile = 0;
+  return o;
+}
+
+template <class F>
+int guarded(gsap_ctx *c, F &&f) {
+  if (!c) return GSAP_ERR_INVALID;
+  try {
+    c->err.clear();
+    f();
+    return GSAP_OK;
+  } catch (const Err &e) {
+    c->err = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    c->err = e.what();
+    return GSAP_ERR_INTERNAL;
+  }
+}
+
+void setup_unit(gsap_ctx *c, const gsap_graph_i32 *g, const gsap_options *o, const int32_t *labels,
+                int32_t B) {
+  gsap_options oo = o ? *o : defaults();
+  check_options(oo);
+  set_device(c);
+  upload_graph(c, g);
+  alloc_work(c, oo.num_proposals, resolve_dense(oo));
+  if (B <= 0 || B > c->N) throw Err(GSAP_ERR_INVALID, "B must be in [1, N]");
+  for (int64_t v = 0; v < c->N; ++v)
+    if (labels[v] < 0 || labels[v] >= B) throw Err(GSAP_ERR_INVALID, "label out of range");
+  HIPX(hipMemcpyAsync(c->lab.p, labels, 4 * (size_t)c->N, hipMemcpyHostToDevice, c->st));
+  rebuild(c, B);
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+void gsap_options_default(gsap_options *o) {
+  if (o) *o = defaults();
+}
+
+int gsap_create(gsap_ctx **ctx, int device) {
+  if (!ctx) return GSAP_ERR_INVALID;
+  *ctx = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return GSAP_ERR_DEVICE;
+  if (device < 0 || device >= n) return GSAP_ERR_INVALID;
+  gsap_ctx *c = new gsap_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess) {
+    delete c;
+    return GSAP_ERR_DEVICE;
+  }
+  *ctx = c;
+  return GSAP_OK;
+}
+
+int gsap_destroy(gsap_ctx *c) {
+  if (!c) return GSAP_OK;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->st);
+  DBuf *all[] = {&c->optr, &c->oadj, &c->ow, &c->iptr, &c->iadj, &c->iw, &c->kout, &c->kin,
+                 &c->esrc, &c->hoff, &c->hbo, &c->hbi, &c->hpool, &c->lab, &c->snap[0],
+                 &c->snap[1], &c->snap[2], &c->rowptr, &c->colptr, &c->col, &c->val, &c->erow,
+                 &c->row, &c->cval, &c->dout, &c->din, &c->ldout, &c->ldin, &c->dense, &c->k0,
+                 &c->k1, &c->v0, &c->v1, &c->flags, &c->uidx, &c->tmp, &c->cnt, &c->partials,
+                 &c->OUT, &c->sof, &c->coin, &c->prop, &c->dS, &c->best, &c->dSbest, &c->map,
+                 &c->terms, &c->tiles, &c->res};
+  for (DBuf *b : all) b->release();
+  try { c->prof.destroy(); } catch (...) {}
+  hipStreamDestroy(c->st);
+  delete c;
+  return GSAP_OK;
+}
+
+const char *gsap_last_error(const gsap_ctx *c) { return c ? c->err.c_str() : "no context"; }
+
+int gsap_partition_i32(gsap_ctx *c, const gsap_graph_i32 *g, const gsap_options *o,
+                       int32_t *partitions_out, gsap_report *report) {
+  return guarded(c, [&] {
+    if (!partitions_out) throw Err(GSAP_ERR_INVALID, "partitions_out is NULL");
+    gsap_options oo = o ? *o : defaults();
+    partition_impl(c, g, oo, partitions_out, report);
+  });
+}
+
+int gsap_get_trace(const gsap_ctx *c, gsap_trace_rec *out, int64_t cap, int64_t *n) {
+  if (!c || !n) return GSAP_ERR_INVALID;
+  *n = (int64_t)c->trace.size();
+  if (out)
+    for (int64_t i = 0; i < *n && i < cap; ++i) out[i] = c->trace[i];
+  return GSAP_OK;
+}
+
+int gsap_blockmodel_i32(gsap_ctx *c, const gsap_graph_i32 *g, const int32_t *labels, int32_t B,
+                        int64_t *rowptr, int32_t *col, int64_t *val, int64_t *colptr, int32_t *row,
+                        int64_t *cval, int64_t *dout, int64_t *din, int64_t *nnz_out,
+                        const gsap_options *o) {
+  return guarded(c, [&] {
+    if (!labels || !nnz_out) throw Err(GSAP_ERR_INVALID, "NULL argument");
+    setup_unit(c, g, o, labels, B);
+    const int32_t nnz = read_nnz(c);
+    *nnz_out = nnz;
+    std::vector<int32_t> rp(B + 1), cp(B + 1), v(nnz), cv(nnz), dO(B), dI(B);
+    hipStream_t st = c->st;
+    HIPX(hipMemcpyAsync(rp.data(), c->bm.rowptr, 4 * (size_t)(B + 1), hipMemcpyDeviceToHost, st));
+    HIPX(hipMemcpyAsync(cp.data(), c->bm.colptr, 4 * (size_t)(B + 1), hipMemcpyDeviceToHost, st));
+    if (nnz > 0) {
+      HIPX(hipMemcpyAsync(col, c->bm.col, 4 * (size_t)nnz, hipMemcpyDeviceToHost, st));
+      HIPX(hipMemcpyAsync(row, c->bm.row, 4 * (size_t)nnz, hipMemcpyDeviceToHost, st));
+      HIPX(hipMemcpyAsync(v.data(), c->bm.val, 4 * (size_t)nnz, hipMemcpyDeviceToHost, st));
+      HIPX(hipMemcpyAsync(cv.data(), c->bm.cval, 4 * (size_t)nnz, hipMemcpyDeviceToHost, st));
+    }
+    HIPX(hipMemcpyAsync(dO.data(), c->bm.dout, 4 * (size_t)B, hipMemcpyDeviceToHost, st));
+    HIPX(hipMemcpyAsync(dI.data(), c->bm.din, 4 * (size_t)B, hipMemcpyDeviceToHost, st));
+    HIPX(hipStreamSynchronize(st));
+    for (int32_t x = 0; x <= B; ++x) { rowptr[x] = rp[x]; colptr[x] = cp[x]; }
+    for (int32_t k = 0; k < nnz; ++k) { val[k] = v[k]; cval[k] = cv[k]; }
+    for (int32_t x = 0; x < B; ++x) { dout[x] = dO[x]; din[x] = dI[x]; }
+  });
+}
+
+int gsap_merge_phase_i32(gsap_ctx *c, const gsap_graph_i32 *g, const gsap_options *o,
+                         int32_t *labels, int32_t B, int32_t B_to_merge, uint32_t m, int32_t *prop,
+                         double *dS, int32_t *best, double *dS_best, int32_t *B_new) {
+  return guarded(c, [&] {
+    if (!labels || !B_new) throw Err(GSAP_ERR_INVALID, "NULL argument");
+    gsap_options oo = o ? *o : defaults();
+    setup_unit(c, g, &oo, labels, B);
+    std::vector<int32_t> hp, hb;
+    std::vector<double> hd, hdb;
+    *B_new = merge_phase(c, oo, B_to_merge, m, &hp, &hd, &hb, &hdb);
+    HIPX(hipMemcpy(labels, c->lab.p, 4 * (size_t)c->N, hipMemcpyDeviceToHost));
+    if (prop) std::copy(hp.begin(), hp.end(), prop);
+    if (dS) std::copy(hd.begin(), hd.end(), dS);
+    if (best) std::copy(hb.begin(), hb.end(), best);
+    if (dS_best) std::copy(hdb.begin(), hdb.end(), dS_best);
+  });
+}
+
+int gsap_nodal_batch_i32(gsap_ctx *c, const gsap_graph_i32 *g, const gsap_options *o,
+                         int32_t *labels, int32_t B, int32_t b, uint32_t q, int32_t *prop,
+                         double *dS, double *dE) {
+  return guarded(c, [&] {
+    if (!labels || !dE) throw Err(GSAP_ERR_INVALID, "NULL argument");
+    gsap_options oo = o ? *o : defaults();
+    if (b < 0 || b >= oo.num_batches) throw Err(GSAP_ERR_INVALID, "batch index out of range");
+    setup_unit(c, g, &oo, labels, B);
+    const int64_t BS = c->N / oo.num_batches;
+    int32_t ar = avg_row(read_nnz(c), B);
+    double *res = c->res.as<double>();
+    nodal_batch(c, oo, b, q, ar, res);
+    hipStream_t st = c->st;
+    HIPX(hipMemcpyAsync(dE, res, 8, hipMemcpyDeviceToHost, st));
+    if (prop && BS > 0)
+      HIPX(hipMemcpyAsync(prop, c->prop.p, 4 * (size_t)BS, hipMemcpyDeviceToHost, st));
+    if (dS && BS > 0) HIPX(hipMemcpyAsync(dS, c->dS.p, 8 * (size_t)BS, hipMemcpyDeviceToHost, st));
+    HIPX(hipMemcpyAsync(labels, c->lab.p, 4 * (size_t)c->N, hipMemcpyDeviceToHost, st));
+    HIPX(hipStreamSynchronize(st));
+  });
+}
+
+int gsap_mdl_i32(gsap_ctx *c, const gsap_graph_i32 *g, const int32_t *labels, int32_t B,
+                 double *S, const gsap_options *o) {
+  return guarded(c, [&] {
+    if (!labels || !S) throw Err(GSAP_ERR_INVALID, "NULL argument");
+    setup_unit(c, g, o, labels, B);
+    *S = mdl_now(c);
+  });
+}
+
+}  // extern "C"

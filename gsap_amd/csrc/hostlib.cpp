@@ -1,0 +1,292 @@
+// Host-side helpers of the C ABI: the seeded DC-SBM benchmark generator, the
+// GraphChallenge TSV reader (GSAP/load.hpp:6-133), pairwise precision / recall
+// (GSAP/evaluate/evaluate.hpp:130-181) and the canonical log2 on the host.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <numeric>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/gsap/gsap.h"
+#include "common.h"
+
+using namespace gsap;
+
+namespace {
+
+struct Gen {
+  uint32_t k0, k1;
+  Philox at(uint32_t idx, uint32_t stage, uint32_t sub) const {
+    return philox(idx, stage, sub, TAG_GEN, k0, k1);
+  }
+};
+
+inline double unit(uint32_t x) { return ((double)x + 0.5) * (1.0 / 4294967296.0); }
+
+// Gamma(alpha, 1) by Marsaglia & Tsang (2000); alpha < 1 via Gamma(alpha+1) * U^(1/alpha).
+double gamma_draw(const Gen &g, uint32_t k, double alpha) {
+  double boost = 1.0;
+  if (alpha < 1.0) {
+    Philox u = g.at(k, 1, 0xFFFFFFFFu);
+    boost = std::pow(unit(u.x0), 1.0 / alpha);
+    alpha += 1.0;
+  }
+  const double d = alpha - 1.0 / 3.0, c = 1.0 / std::sqrt(9.0 * d);
+  for (uint32_t attempt = 0;; ++attempt) {
+    Philox u = g.at(k, 1, attempt);
+    double z = std::sqrt(-2.0 * std::log(unit(u.x0))) * std::cos(6.283185307179586 * unit(u.x1));
+    double v = 1.0 + c * z;
+    if (v <= 0.0) continue;
+    v = v * v * v;
+    double uu = unit(u.x2);
+    if (std::log(uu) < 0.5 * z * z + d - d * v + d * std::log(v)) return d * v * boost;
+  }
+}
+
+// first index with cdf[i] > target (cdf strictly cumulative)
+int64_t pick_cdf(const uint64_t *cdf, int64_t n, uint64_t target) {
+  return std::upper_bound(cdf, cdf + n, target) - cdf;
+}
+
+uint64_t mulhi64(uint32_t x, uint64_t n) {  // floor(x * n / 2^32), n < 2^32
+  return ((uint64_t)x * n) >> 32;
+}
+
+struct PairSet {
+  std::vector<uint64_t> t;
+  uint64_t mask;
+  explicit PairSet(int64_t n) {
+    uint64_t cap = 1024;
+    while (cap < (uint64_t)(2 * n + 16)) cap <<= 1;
+    t.assign(cap, ~0ull);
+    mask = cap - 1;
+  }
+  bool insert(uint64_t k) {
+    uint64_t h = (k * 0x9E3779B97F4A7C15ull) >> 20;
+    for (uint64_t i = h & mask;; i = (i + 1) & mask) {
+      if (t[i] == k) return false;
+      if (t[i] == ~0ull) { t[i] = k; return true; }
+    }
+  }
+};
+
+void build_csr(int64_t N, const std::vector<int32_t> &src, const std::vector<int32_t> &dst,
+               const std::vector<int32_t> &w, int32_t *ptr, int32_t *adj, int32_t *wout) {
+  // stable counting sort by src: keeps input order inside each row
+  std::vector<int64_t> cnt(N + 1, 0);
+  for (int32_t s : src) cnt[s + 1]++;
+  for (int64_t v = 0; v < N; ++v) cnt[v + 1] += cnt[v];
+  for (int64_t v = 0; v <= N; ++v) ptr[v] = (int32_t)cnt[v];
+  std::vector<int64_t> cur(cnt.begin(), cnt.end() - 1);
+  for (size_t e = 0; e < src.size(); ++e) {
+    int64_t p = cur[src[e]]++;
+    adj[p] = dst[e];
+    wout[p] = w[e];
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+double gsap_log2(double x) { return clog2_host(x); }
+
+void gsap_log2_tables(double *inv_c, double *log2_c, double *poly) {
+  std::memcpy(inv_c, h_log2_inv, sizeof h_log2_inv);
+  std::memcpy(log2_c, h_log2_lg, sizeof h_log2_lg);
+  std::memcpy(poly, h_log2_poly, sizeof h_log2_poly);
+}
+
+int gsap_generate_dcsbm_i32(const gsap_dcsbm_params *p, int64_t *E_out, int32_t *out_ptr,
+                            int32_t *out_adj, int32_t *out_w, int32_t *in_ptr, int32_t *in_adj,
+                            int32_t *in_w, int32_t *truth) {
+  if (!p || !E_out) return GSAP_ERR_INVALID;
+  const int64_t N = p->N;
+  const int32_t K = p->K;
+  if (N <= 1 || N >= ((int64_t)1 << 31) || K < 1 || K > N || p->mean_deg <= 0 || p->rho < 0 ||
+      p->deg_min < 1 || p->deg_max < p->deg_min)
+    return GSAP_ERR_INVALID;
+  if ((uint64_t)N * (uint64_t)p->deg_max >= ((uint64_t)1 << 32)) return GSAP_ERR_INVALID;
+  const int64_t E = std::llround((double)N * p->mean_deg);
+  if (E <= 0 || E > N * (N - 1) / 4 || E >= ((int64_t)1 << 30)) return GSAP_ERR_INVALID;
+  *E_out = E;
+  if (!out_ptr) return GSAP_OK;
+  if (!out_adj || !out_w || !in_ptr || !in_adj || !in_w) return GSAP_ERR_INVALID;
+  const Gen g{(uint32_t)p->seed, (uint32_t)(p->seed >> 32)};
+
+  // 1. block sizes: near-equal, or Dirichlet(alpha) with every block non-empty
+  std::vector<int64_t> size(K);
+  if (p->alpha <= 0) {
+    for (int32_t k = 0; k < K; ++k) size[k] = N / K + (k < N % K ? 1 : 0);
+  } else {
+    std::vector<double> gk(K);
+    double tot = 0;
+    for (int32_t k = 0; k < K; ++k) { gk[k] = gamma_draw(g, (uint32_t)k, p->alpha); tot += gk[k]; }
+    const int64_t spare = N - K;
+    std::vector<double> frac(K);
+    int64_t used = 0;
+    for (int32_t k = 0; k < K; ++k) {
+      double x = gk[k] / tot * (double)spare;
+      size[k] = 1 + (int64_t)std::floor(x);
+      frac[k] = x - std::floor(x);
+      used += size[k];
+    }
+    std::vector<int32_t> ord(K);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return frac[a] > frac[b]; });
+    for (int64_t i = 0; used < N; ++i, ++used) size[ord[i % K]]++;
+  }
+  // 2. vertex -> block, shuffled (Fisher-Yates)
+  std::vector<int32_t> blk(N);
+  {
+    int64_t v = 0;
+    for (int32_t k = 0; k < K; ++k)
+      for (int64_t i = 0; i < size[k]; ++i) blk[v++] = k;
+    for (int64_t i = N - 1; i > 0; --i) {
+      Philox u = g.at((uint32_t)i, 2, 0);
+      int64_t j = (int64_t)mulhi64(u.x0, (uint64_t)(i + 1));
+      std::swap(blk[i], blk[j]);
+    }
+  }
+  // 3. degree propensities: discrete power law k^-exponent on [deg_min, deg_max]
+  const int32_t nk = p->deg_max - p->deg_min + 1;
+  std::vector<double> pcdf(nk);
+  {
+    double s = 0;
+    for (int32_t i = 0; i < nk; ++i) { s += std::pow((double)(p->deg_min + i), -p->exponent); pcdf[i] = s; }
+    for (int32_t i = 0; i < nk; ++i) pcdf[i] /= s;
+  }
+  auto theta = [&](uint32_t x) {
+    double u = unit(x);
+    int32_t i = (int32_t)(std::upper_bound(pcdf.begin(), pcdf.end(), u) - pcdf.begin());
+    return (uint64_t)(p->deg_min + std::min(i, nk - 1));
+  };
+  std::vector<uint64_t> cdf_out(N);
+  std::vector<std::vector<int32_t>> members(K);
+  std::vector<std::vector<uint64_t>> cdf_in(K);
+  {
+    uint64_t acc = 0;
+    for (int64_t v = 0; v < N; ++v) {
+      Philox u = g.at((uint32_t)v, 3, 0);
+      acc += theta(u.x0);
+      cdf_out[v] = acc;
+      members[blk[v]].push_back((int32_t)v);
+      uint64_t prev = cdf_in[blk[v]].empty() ? 0 : cdf_in[blk[v]].back();
+      cdf_in[blk[v]].push_back(prev + theta(u.x1));
+    }
+  }
+  const uint64_t tot_out = cdf_out[N - 1];
+  const uint32_t thr = (uint32_t)std::min(4294967295.0, std::floor(p->rho / (1.0 + p->rho) * 4294967296.0));
+  // 4. edges: source ~ theta_out, destination block intra w.p. rho/(1+rho) else uniform
+  //    over the other blocks, destination ~ theta_in inside it; no self-loops, no duplicates
+  std::vector<int32_t> src, dst;
+  src.reserve(E); dst.reserve(E);
+  PairSet seen(E);
+  const uint64_t max_attempts = (uint64_t)E * 200 + 100000;
+  for (uint64_t a = 0; (int64_t)src.size() < E; ++a) {
+    if (a >= max_attempts) return GSAP_ERR_INVALID;
+    Philox u = g.at((uint32_t)a, 4, (uint32_t)(a >> 32));
+    int64_t s = pick_cdf(cdf_out.data(), N, mulhi64(u.x0, tot_out));
+    int32_t bs = blk[s], bd = bs;
+    if (K > 1 && u.x1 >= thr) {
+      bd = (int32_t)mulhi64(u.x2, (uint64_t)(K - 1));
+      if (bd >= bs) ++bd;
+    }
+    const std::vector<uint64_t> &ci = cdf_in[bd];
+    int64_t t = members[bd][pick_cdf(ci.data(), (int64_t)ci.size(), mulhi64(u.x3, ci.back()))];
+    if (t == s) continue;
+    if (!seen.insert(((uint64_t)s << 32) | (uint64_t)t)) continue;
+    src.push_back((int32_t)s);
+    dst.push_back((int32_t)t);
+  }
+  // 5. CSR views, each row ordered by neighbour id
+  {
+    std::vector<int64_t> ord(E);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
+      return src[a] != src[b] ? src[a] < src[b] : dst[a] < dst[b];
+    });
+    std::vector<int32_t> s2(E), d2(E), w1(E, 1);
+    for (int64_t i = 0; i < E; ++i) { s2[i] = src[ord[i]]; d2[i] = dst[ord[i]]; }
+    build_csr(N, s2, d2, w1, out_ptr, out_adj, out_w);
+    std::sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
+      return dst[a] != dst[b] ? dst[a] < dst[b] : src[a] < src[b];
+    });
+    for (int64_t i = 0; i < E; ++i) { s2[i] = src[ord[i]]; d2[i] = dst[ord[i]]; }
+    build_csr(N, d2, s2, w1, in_ptr, in_adj, in_w);
+  }
+  if (truth) std::copy(blk.begin(), blk.end(), truth);
+  return GSAP_OK;
+}
+
+int gsap_read_tsv_i32(const char *path, int64_t *N_out, int64_t *E_out, int32_t *out_ptr,
+                      int32_t *out_adj, int32_t *out_w, int32_t *in_ptr, int32_t *in_adj,
+                      int32_t *in_w) {
+  if (!path || !N_out || !E_out) return GSAP_ERR_INVALID;
+  std::ifstream f(path);
+  if (!f.is_open()) return GSAP_ERR_IO;
+  std::vector<int32_t> src, dst, w;
+  std::string line;
+  int64_t N = 0;
+  while (std::getline(f, line)) {
+    std::istringstream is(line);
+    long long a, b;
+    double x = 1.0;
+    if (!(is >> a)) continue;  // blank line
+    if (!(is >> b)) return GSAP_ERR_IO;
+    if (!(is >> x)) x = 1.0;
+    if (a < 1 || b < 1 || a >= ((long long)1 << 31) || b >= ((long long)1 << 31)) return GSAP_ERR_IO;
+    src.push_back((int32_t)(a - 1));
+    dst.push_back((int32_t)(b - 1));
+    w.push_back((int32_t)x);  // load.hpp:72-73: stof then integer cast
+    N = std::max<int64_t>(N, std::max(a, b));
+  }
+  *N_out = N;
+  *E_out = (int64_t)src.size();
+  if (!out_ptr) return GSAP_OK;
+  if (!out_adj || !out_w || !in_ptr || !in_adj || !in_w) return GSAP_ERR_INVALID;
+  build_csr(N, src, dst, w, out_ptr, out_adj, out_w);
+  build_csr(N, dst, src, w, in_ptr, in_adj, in_w);
+  return GSAP_OK;
+}
+
+int gsap_pairwise_i32(int64_t N, const int32_t *truth, const int32_t *given, double *precision,
+                      double *recall) {
+  if (N <= 0 || !truth || !given || !precision || !recall) return GSAP_ERR_INVALID;
+  int32_t Bt = 0, Bg = 0;
+  for (int64_t i = 0; i < N; ++i) {
+    if (given[i] < 0) return GSAP_ERR_INVALID;
+    Bt = std::max(Bt, truth[i] + 1);
+    Bg = std::max(Bg, given[i] + 1);
+  }
+  std::vector<int64_t> a(Bt, 0), b(Bg, 0);
+  std::vector<uint64_t> pairs;
+  pairs.reserve(N);
+  for (int64_t i = 0; i < N; ++i) {
+    if (truth[i] < 0) continue;  // unknown truth, evaluate.hpp:43-48
+    a[truth[i]]++;
+    b[given[i]]++;
+    pairs.push_back(((uint64_t)truth[i] << 32) | (uint32_t)given[i]);
+  }
+  std::sort(pairs.begin(), pairs.end());
+  int64_t same = 0;
+  for (size_t i = 0; i < pairs.size();) {
+    size_t j = i;
+    while (j < pairs.size() && pairs[j] == pairs[i]) ++j;
+    int64_t n = (int64_t)(j - i);
+    same += n * (n - 1) / 2;
+    i = j;
+  }
+  int64_t sa = 0, sb = 0;
+  for (int64_t x : a) sa += x * (x - 1) / 2;
+  for (int64_t x : b) sb += x * (x - 1) / 2;
+  *recall = sa ? (double)same / (double)sa : 0.0;
+  *precision = sb ? (double)same / (double)sb : 0.0;
+  return GSAP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,523 @@
+// Kernels of one SBP level: per-block terms and samples, nodal moves, block merges,
+// reductions.  Every FP64 sum follows the canonical 64-way order of common.h, so the
+// results are bit-identical for any group width and match the CPU oracle.
+#include "engine.h"
+
+namespace gsap {
+
+// --------------------------------------------------------------- helpers
+GSAP_D double wave_sum(double x) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x += __shfl_down(x, off, 64);
+  return x;  // lane 0
+}
+
+GSAP_D uint32_t wave_incl_scan_u32(uint32_t x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  return x;
+}
+
+// Weighted pick over v's out list then in list: the first entry whose running weight
+// exceeds target (kernels.hpp:916-936 as integers).  Wave-wide; every lane returns it.
+GSAP_D int32_t wave_pick(const int32_t *w1, const int32_t *id1, int32_t n1, const int32_t *w2,
+                         const int32_t *id2, int32_t n2, uint32_t target) {
+  const int lane = threadIdx.x & 63;
+  uint32_t base = 0;
+  const int32_t n = n1 + n2;
+  for (int32_t c = 0; c < n; c += 64) {
+    const int32_t k = c + lane;
+    uint32_t w = 0;
+    if (k < n1) w = (uint32_t)w1[k];
+    else if (k < n) w = (uint32_t)w2[k - n1];
+    const uint32_t incl = wave_incl_scan_u32(w);
+    const uint64_t mask = __ballot(base + incl > target);
+    if (mask) {
+      const int32_t kk = c + (int32_t)__ffsll((long long)mask) - 1;
+      return kk < n1 ? id1[kk] : id2[kk - n1];
+    }
+    base += __shfl(incl, 63, 64);
+  }
+  return id1 ? id1[0] : id2[0];  // unreachable when target < total weight
+}
+
+template <int G, class F>
+GSAP_D void seg_for(int32_t n, F &&f) {
+  const int q = threadIdx.x & (G - 1);
+  for (int32_t base = 0; base < n; base += 64) {
+#pragma unroll
+    for (int j = 0; j < 64 / G; ++j) {
+      const int32_t k = base + q + j * G;
+      if (k < n) f(k, j);
+    }
+  }
+}
+
+static unsigned grid_for(int64_t n, int threads) {
+  int64_t g = (n + threads - 1) / threads;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
+// ------------------------------------------------ per-block terms and samples
+// OUT[x] = sum_t M_xt (log2 M_xt - (log2 d_out_x + log2 d_in_t))  (kernels.hpp:143-182)
+// and, when sampling, the nodal per-block neighbour sample and coin (kernels.hpp:942-997,
+// 1024-1029; one draw per block per batch, shared by its vertices).
+template <int G>
+__global__ __launch_bounds__(256) void k_block_terms(BMView m, double *OUT, int32_t *sof,
+                                                     uint8_t *coin, RngKey key, uint32_t phase,
+                                                     int sample) {
+  const int32_t x = (int32_t)(blockIdx.x * (256 / G) + threadIdx.x / G);
+  if (x >= m.B) return;
+  const int q = threadIdx.x & (G - 1);
+  CAcc<G> acc;
+  acc.init();
+  const int32_t b0 = m.rowptr[x], n = m.rowptr[x + 1] - b0;
+  const double lo = m.ldout[x];
+  seg_for<G>(n, [&](int32_t k, int j) {
+    acc.p[j] += term(m.val[b0 + k], lo, m.ldin[m.col[b0 + k]]);
+  });
+  const double out = acc.reduce();
+  if (q == 0) {
+    OUT[x] = out;
+    if (sample) {
+      const Philox y = philox((uint32_t)x, phase, key.chain, TAG_BLOCK, key.k0, key.k1);
+      const int64_t deg = (int64_t)m.dout[x] + m.din[x];
+      sof[x] = deg == 0 ? 0 : bm_walk(m, x, mulhi(y.x0, deg), -1, 0);
+      coin[x] = coin_random(y.x1, deg, m.B) ? 1 : 0;
+    }
+  }
+}
+
+void launch_block_terms(const BMView &m, double *OUT, int32_t *sof, uint8_t *coin, RngKey key,
+                        uint32_t phase, bool sample, int32_t avg_row, hipStream_t st) {
+  const int s = sample ? 1 : 0;
+  if (avg_row <= 8)
+    k_block_terms<8><<<grid_for(m.B, 32), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s);
+  else if (avg_row <= 16)
+    k_block_terms<16><<<grid_for(m.B, 16), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s);
+  else
+    k_block_terms<64><<<grid_for(m.B, 4), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s);
+}
+
+// ------------------------------------------------------------- nodal moves
+constexpr int LBITS = 8, LCAP = 1 << LBITS;
+
+struct NodalArgs {
+  int64_t lo, BS;
+  const int32_t *optr, *oadj, *ow, *iptr, *iadj, *iw, *kout, *kin, *lab;
+  BMView m;
+  const double *OUT;
+  const int32_t *sof;
+  const uint8_t *coin;
+  RngKey key;
+  uint32_t phase;
+  int32_t *prop;
+  double *dS;
+  const int64_t *hoff;
+  const uint8_t *hbits_o, *hbits_i;
+  int32_t *hpool;
+};
+
+// One wave per vertex v of the batch: proposal (kernels.hpp:877-939, 1000-1032) and
+// delta-S against the batch-start block matrix (kernels.hpp:494-643, 1153-1726 with the
+// node-only entries run-merged).  Labels are not written here (batch-Jacobi).
+__global__ __launch_bounds__(64) void k_nodal(NodalArgs a) {
+  __shared__ int32_t lko[LCAP], lwo[LCAP], lfo[LCAP], lki[LCAP], lwi[LCAP], lfi[LCAP];
+  const int lane = threadIdx.x;
+  const int64_t i = blockIdx.x;
+  const int64_t v = a.lo + i;
+  const BMView &m = a.m;
+  const int32_t B = m.B;
+  const int32_t r = a.lab[v];
+  const Philox x = philox((uint32_t)v, a.phase, a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
+  const int32_t o0 = a.optr[v], no = a.optr[v + 1] - o0;
+  const int32_t i0 = a.iptr[v], ni = a.iptr[v + 1] - i0;
+  const int64_t kout = a.kout[v], kin = a.kin[v];
+  const int64_t tot = kout + kin;
+  int32_t s;
+  if (tot == 0) {
+    s = (int32_t)mulhi(x.x0, B);
+  } else {
+    const int32_t nbr = wave_pick(a.ow + o0, a.oadj + o0, no, a.iw + i0, a.iadj + i0, ni,
+                                  mulhi(x.x1, tot));
+    const int32_t u = a.lab[nbr];
+    s = a.coin[u] ? (int32_t)mulhi(x.x0, B) : a.sof[u];
+  }
+  if (lane == 0) a.prop[i] = s;
+  if (s == r) {
+    if (lane == 0) a.dS[i] = 0.0;
+    return;
+  }
+
+  // a_t (out weight of v into block t) and c_t (in weight from block t), with the
+  // first adjacency position of each block
+  HTab ho, hi;
+  const uint32_t bo = a.hbits_o[v];
+  if (bo) {
+    const uint32_t bi = a.hbits_i[v];
+    int32_t *p = a.hpool + a.hoff[v];
+    const uint32_t co = 1u << bo, ci = 1u << bi;
+    ho = HTab{p, p + co, p + 2 * co, bo};
+    p += 3 * co;
+    hi = HTab{p, p + ci, p + 2 * ci, bi};
+  } else {
+    ho = HTab{lko, lwo, lfo, (uint32_t)LBITS};
+    hi = HTab{lki, lwi, lfi, (uint32_t)LBITS};
+  }
+  ht_clear(ho, lane, 64);
+  ht_clear(hi, lane, 64);
+  __syncthreads();
+  for (int32_t j = lane; j < no; j += 64) ht_insert(ho, a.lab[a.oadj[o0 + j]], a.ow[o0 + j], j);
+  for (int32_t j = lane; j < ni; j += 64) ht_insert(hi, a.lab[a.iadj[i0 + j]], a.iw[i0 + j], j);
+  __syncthreads();
+  const int64_t c_r = ht_weight(hi, r), c_s = ht_weight(hi, s);
+
+  // new row r
+  const double l_dr = clog2((double)((int64_t)m.dout[r] - kout));
+  double p_ro = 0.0;
+  bool f = false;
+  {
+    const int32_t b0 = m.rowptr[r], n = m.rowptr[r + 1] - b0;
+    for (int32_t k = lane; k < n; k += 64) {
+      const int32_t t = m.col[b0 + k];
+      const int64_t R = (int64_t)m.val[b0 + k] - ht_weight(ho, t) - (t == r ? c_r : 0) +
+                        (t == s ? c_r : 0);
+      if (t == s) f = true;
+      if (R > 0) p_ro += term(R, m.ldin[t], l_dr);
+    }
+  }
+  const bool s_in_r = __ballot(f) != 0;
+  double nro = wave_sum(p_ro);
+  if (!s_in_r && c_r > 0) nro = nro + term(c_r, m.ldin[s], l_dr);
+
+  // new row s (row entries, then v's out-blocks absent from row s)
+  const double l_ds = clog2((double)((int64_t)m.dout[s] + kout));
+  double p_sa = 0.0, p_sb = 0.0;
+  f = false;
+  {
+    const int32_t b0 = m.rowptr[s], n = m.rowptr[s + 1] - b0;
+    for (int32_t k = lane; k < n; k += 64) {
+      const int32_t t = m.col[b0 + k];
+      const int64_t S = (int64_t)m.val[b0 + k] + ht_weight(ho, t) - (t == r ? c_s : 0) +
+                        (t == s ? c_s : 0);
+      if (t == s) f = true;
+      if (S > 0) p_sa += term(S, m.ldin[t], l_ds);
+    }
+  }
+  for (int32_t j = lane; j < no; j += 64) {
+    const int32_t t = a.lab[a.oadj[o0 + j]];
+    const int32_t slot = ht_find(ho, t);
+    if (ld_relaxed(&ho.first[slot]) != j) continue;
+    if (bm_row_get(m, s, t) != 0) continue;
+    if (t == s) f = true;
+    const int64_t S = (int64_t)ld_relaxed(&ho.w[slot]) - (t == r ? c_s : 0) + (t == s ? c_s : 0);
+    if (S > 0) p_sb += term(S, m.ldin[t], l_ds);
+  }
+  const bool found = __ballot(f) != 0;
+  double nso = wave_sum(p_sa) + wave_sum(p_sb);
+  if (!found && c_s > 0) nso = nso + term(c_s, m.ldin[s], l_ds);
+
+  // old / new column r
+  const double l_ir = clog2((double)((int64_t)m.din[r] - kin));
+  double p_inr = 0.0, p_nri = 0.0;
+  {
+    const int32_t b0 = m.colptr[r], n = m.colptr[r + 1] - b0;
+    for (int32_t k = lane; k < n; k += 64) {
+      const int32_t t = m.row[b0 + k];
+      if (t == r || t == s) continue;
+      const int32_t M = m.cval[b0 + k];
+      p_inr += term(M, m.ldout[t], m.ldin[r]);
+      const int64_t w = (int64_t)M - ht_weight(hi, t);
+      if (w > 0) p_nri += term(w, m.ldout[t], l_ir);
+    }
+  }
+  // old / new column s (column entries, then v's in-blocks absent from column s)
+  const double l_is = clog2((double)((int64_t)m.din[s] + kin));
+  double p_ins = 0.0, p_nsa = 0.0, p_nsb = 0.0;
+  {
+    const int32_t b0 = m.colptr[s], n = m.colptr[s + 1] - b0;
+    for (int32_t k = lane; k < n; k += 64) {
+      const int32_t t = m.row[b0 + k];
+      if (t == r || t == s) continue;
+      const int32_t M = m.cval[b0 + k];
+      p_ins += term(M, m.ldout[t], m.ldin[s]);
+      const int64_t w = (int64_t)M + ht_weight(hi, t);
+      if (w > 0) p_nsa += term(w, m.ldout[t], l_is);
+    }
+  }
+  for (int32_t j = lane; j < ni; j += 64) {
+    const int32_t t = a.lab[a.iadj[i0 + j]];
+    const int32_t slot = ht_find(hi, t);
+    if (ld_relaxed(&hi.first[slot]) != j) continue;
+    if (t == r || t == s) continue;
+    if (bm_col_get(m, s, t) != 0) continue;
+    const int64_t w = ld_relaxed(&hi.w[slot]);
+    if (w > 0) p_nsb += term(w, m.ldout[t], l_is);
+  }
+  const double inr = wave_sum(p_inr), ins = wave_sum(p_ins), nri = wave_sum(p_nri);
+  const double nsi = wave_sum(p_nsa) + wave_sum(p_nsb);
+  // calculate_dS_overall_nodal (kernels.hpp:600-643): accept iff dS < 0
+  double d = a.OUT[r] + a.OUT[s];
+  d = d + (inr + ins);
+  d = d - nro;
+  d = d - nso;
+  d = d - nri;
+  d = d - nsi;
+  if (lane == 0) a.dS[i] = d < 0.0 ? d : 0.0;
+}
+
+void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const double *OUT,
+                  const int32_t *sof, const uint8_t *coin, RngKey key, uint32_t phase, int64_t lo,
+                  int64_t BS, int32_t *prop, double *dS, hipStream_t st) {
+  if (BS <= 0) return;
+  NodalArgs a;
+  a.lo = lo; a.BS = BS;
+  a.optr = g.optr; a.oadj = g.oadj; a.ow = g.ow;
+  a.iptr = g.iptr; a.iadj = g.iadj; a.iw = g.iw;
+  a.kout = g.kout; a.kin = g.kin; a.lab = lab;
+  a.m = m; a.OUT = OUT; a.sof = sof; a.coin = coin; a.key = key; a.phase = phase;
+  a.prop = prop; a.dS = dS;
+  a.hoff = g.hoff; a.hbits_o = g.hbits_o; a.hbits_i = g.hbits_i; a.hpool = g.hpool;
+  k_nodal<<<(unsigned)BS, 64, 0, st>>>(a);
+}
+
+__global__ void k_apply(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo,
+                        int64_t BS) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < BS && dS[i] < 0.0) lab[lo + i] = prop[i];
+}
+
+void launch_apply(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo, int64_t BS,
+                  hipStream_t st) {
+  if (BS <= 0) return;
+  k_apply<<<grid_for(BS, 256), 256, 0, st>>>(lab, prop, dS, lo, BS);
+}
+
+// ---------------------------------------------------------- canonical sums
+__global__ __launch_bounds__(64) void k_tile_sums(const double *x, const int32_t *n_dev,
+                                                  int64_t n_host, double *tiles) {
+  const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
+  const int64_t t0 = (int64_t)blockIdx.x * 1024;
+  if (t0 >= n) return;
+  const int lane = threadIdx.x;
+  double p = 0.0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int64_t i = t0 + lane + 64 * j;
+    if (i < n) p += x[i];
+  }
+  p = wave_sum(p);
+  if (lane == 0) tiles[blockIdx.x] = p;
+}
+
+__global__ __launch_bounds__(64) void k_final_sum(const double *tiles, const int32_t *n_dev,
+                                                  int64_t n_host, double *out) {
+  const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
+  const int64_t nt = (n + 1023) / 1024;
+  const int lane = threadIdx.x;
+  double p = 0.0;
+  for (int64_t k = lane; k < nt; k += 64) p += tiles[k];
+  p = wave_sum(p);
+  if (lane == 0) *out = p;
+}
+
+void launch_csum_array(const double *x, const int32_t *n_dev, int64_t n_host, int64_t n_max,
+                       double *tiles, double *out, hipStream_t st) {
+  const int64_t nt = (n_max + 1023) / 1024;
+  if (nt > 0) k_tile_sums<<<(unsigned)nt, 64, 0, st>>>(x, n_dev, n_host, tiles);
+  k_final_sum<<<1, 64, 0, st>>>(tiles, n_dev, n_host, out);
+}
+
+// data term of every block-matrix entry, CSR order (kernels.hpp:1066-1093)
+__global__ void k_mdl_terms(BMView m, const int32_t *erow, double *terms) {
+  const int32_t nnz = m.rowptr[m.B];
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
+       k += (int64_t)gridDim.x * blockDim.x)
+    terms[k] = term(m.val[k], m.ldout[erow[k]], m.ldin[m.col[k]]);
+}
+
+void launch_mdl_terms(const BMView &m, const int32_t *erow, int64_t n_max, double *terms,
+                      hipStream_t st) {
+  int64_t g = grid_for(n_max, 256);
+  if (g > 16384) g = 16384;
+  k_mdl_terms<<<(unsigned)g, 256, 0, st>>>(m, erow, terms);
+}
+
+// ------------------------------------------------------------- block merges
+// One group of G lanes per proposal (r, i): the proposal (kernels.hpp:726-856, every lane
+// computes it) and the merge delta-S (kernels.hpp:143-449, s-side column fixed).
+template <int G>
+__global__ __launch_bounds__(256) void k_merge(BMView m, const double *OUT, RngKey key,
+                                               uint32_t phase, int32_t np, int32_t *prop,
+                                               double *dSout) {
+  const int64_t idx = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
+  const int32_t B = m.B;
+  if (idx >= (int64_t)B * np) return;
+  const int q = threadIdx.x & (G - 1);
+  const int32_t r = (int32_t)(idx / np);
+  const Philox x = philox((uint32_t)idx, phase, key.chain, TAG_MERGE, key.k0, key.k1);
+  int32_t s;
+  const int64_t deg = (int64_t)m.dout[r] + m.din[r];
+  if (deg == 0) {
+    s = (int32_t)mulhi(x.x0, B);
+  } else {
+    const int32_t u = bm_walk(m, r, mulhi(x.x1, deg), -1, 0);
+    const int64_t degu = (int64_t)m.dout[u] + m.din[u];
+    if (coin_random(x.x3, degu, B)) s = (int32_t)mulhi(x.x0, B);
+    else s = bm_walk(m, u, mulhi(x.x2, degu), r, 0);
+  }
+  // old column terms excluding t in {r, s}
+  double inx_r, inx_s;
+  {
+    CAcc<G> acc;
+    acc.init();
+    const int32_t b0 = m.colptr[r], n = m.colptr[r + 1] - b0;
+    const double li = m.ldin[r];
+    seg_for<G>(n, [&](int32_t k, int j) {
+      const int32_t t = m.row[b0 + k];
+      if (t != r && t != s) acc.p[j] += term(m.cval[b0 + k], m.ldout[t], li);
+    });
+    inx_r = acc.reduce();
+  }
+  {
+    CAcc<G> acc;
+    acc.init();
+    const int32_t b0 = m.colptr[s], n = m.colptr[s + 1] - b0;
+    const double li = m.ldin[s];
+    seg_for<G>(n, [&](int32_t k, int j) {
+      const int32_t t = m.row[b0 + k];
+      if (t != r && t != s) acc.p[j] += term(m.cval[b0 + k], m.ldout[t], li);
+    });
+    inx_s = acc.reduce();
+  }
+  // merged row: row r entries (M_rt + M_st), then row s entries absent from row r
+  const double ldo = clog2((double)((int64_t)m.dout[r] + m.dout[s]));
+  double new_out;
+  {
+    CAcc<G> na, nb;
+    na.init();
+    nb.init();
+    const int32_t b0 = m.rowptr[r], n = m.rowptr[r + 1] - b0;
+    seg_for<G>(n, [&](int32_t k, int j) {
+      const int32_t t = m.col[b0 + k];
+      na.p[j] += term((int64_t)m.val[b0 + k] + bm_row_get(m, s, t), m.ldin[t], ldo);
+    });
+    if (s != r) {
+      const int32_t c0 = m.rowptr[s], c = m.rowptr[s + 1] - c0;
+      seg_for<G>(c, [&](int32_t k, int j) {
+        const int32_t t = m.col[c0 + k];
+        if (bm_row_get(m, r, t) == 0) nb.p[j] += term(m.val[c0 + k], m.ldin[t], ldo);
+      });
+    }
+    const double a_ = na.reduce();
+    const double b_ = nb.reduce();
+    new_out = a_ + b_;
+  }
+  // merged column, t not in {r, s}
+  const double ldi = clog2((double)((int64_t)m.din[r] + m.din[s]));
+  double new_in;
+  {
+    CAcc<G> ia, ib;
+    ia.init();
+    ib.init();
+    const int32_t b0 = m.colptr[r], n = m.colptr[r + 1] - b0;
+    seg_for<G>(n, [&](int32_t k, int j) {
+      const int32_t t = m.row[b0 + k];
+      if (t != r && t != s)
+        ia.p[j] += term((int64_t)m.cval[b0 + k] + bm_col_get(m, s, t), m.ldout[t], ldi);
+    });
+    if (s != r) {
+      const int32_t c0 = m.colptr[s], c = m.colptr[s + 1] - c0;
+      seg_for<G>(c, [&](int32_t k, int j) {
+        const int32_t t = m.row[c0 + k];
+        if (t != r && t != s && bm_col_get(m, r, t) == 0)
+          ib.p[j] += term(m.cval[c0 + k], m.ldout[t], ldi);
+      });
+    }
+    const double a_ = ia.reduce();
+    const double b_ = ib.reduce();
+    new_in = a_ + b_;
+  }
+  if (q == 0) {
+    double d = OUT[r] + OUT[s];
+    d = d + (inx_r + inx_s);
+    d = d - new_out;
+    d = d - new_in;
+    prop[idx] = s;
+    dSout[idx] = d;
+  }
+}
+
+void launch_merge(const BMView &m, const double *OUT, RngKey key, uint32_t phase, int32_t np,
+                  int32_t *prop, double *dS, int32_t avg_row, hipStream_t st) {
+  const int64_t n = (int64_t)m.B * np;
+  if (avg_row <= 8)
+    k_merge<8><<<grid_for(n, 32), 256, 0, st>>>(m, OUT, key, phase, np, prop, dS);
+  else if (avg_row <= 16)
+    k_merge<16><<<grid_for(n, 16), 256, 0, st>>>(m, OUT, key, phase, np, prop, dS);
+  else
+    k_merge<64><<<grid_for(n, 4), 256, 0, st>>>(m, OUT, key, phase, np, prop, dS);
+}
+
+// find_best_warp (kernels.hpp:452-491): 32 lanes padded with (+MAX, 0), shuffle-down
+// tree, a tie takes the other lane's proposal.
+__global__ void k_argmin(const double *dS, const int32_t *prop, int32_t B, int32_t np,
+                         int32_t *best, double *dS_best) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= B) return;
+  double v[32];
+  int32_t b[32];
+#pragma unroll
+  for (int l = 0; l < 32; ++l) {
+    if (l < np) {
+      v[l] = dS[(int64_t)r * np + l];
+      b[l] = prop[(int64_t)r * np + l];
+    } else {
+      v[l] = 1.7976931348623157e308;
+      b[l] = 0;
+    }
+  }
+#pragma unroll
+  for (int off = 16; off >= 1; off >>= 1) {
+#pragma unroll
+    for (int l = 0; l < 16; ++l) {
+      if (l < off) {
+        const double a_ = v[l], o = v[l + off];
+        const double mn = (a_ != a_) ? o : (o < a_ ? o : a_);
+        if (mn == o) b[l] = b[l + off];
+        v[l] = mn;
+      }
+    }
+  }
+  best[r] = b[0];
+  dS_best[r] = v[0];
+}
+
+void launch_argmin(const double *dS, const int32_t *prop, int32_t B, int32_t np, int32_t *best,
+                   double *dS_best, hipStream_t st) {
+  k_argmin<<<grid_for(B, 128), 128, 0, st>>>(dS, prop, B, np, best, dS_best);
+}
+
+__global__ void k_relabel(int32_t *lab, const int32_t *map, int64_t N) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < N) lab[v] = map[lab[v]];
+}
+
+void launch_relabel(int32_t *lab, const int32_t *map, int64_t N, hipStream_t st) {
+  k_relabel<<<grid_for(N, 256), 256, 0, st>>>(lab, map, N);
+}
+
+__global__ void k_iota(int32_t *x, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = (int32_t)i;
+}
+
+void launch_iota(int32_t *x, int64_t n, hipStream_t st) {
+  k_iota<<<grid_for(n, 256), 256, 0, st>>>(x, n);
+}
+
+}  // namespace gsap

@@ -1,0 +1,187 @@
+/* gsap.h -- C ABI of the MI355X-native stochastic-block-partitioning hot path.
+ *
+ * Plain pointers and sizes only; no exceptions cross this boundary.  Every call
+ * returns an int status (GSAP_OK == 0) and leaves a message readable through
+ * gsap_last_error().  A context owns all device memory; it is not thread-safe
+ * (use one context per host thread).
+ *
+ * Reference interfaces these entry points replace (paths relative to the
+ * reference checkout):
+ *   gsap_partition_i32      <- template<T,V,W> void partition(Graph<T,V,W>&)
+ *                              GSAP/partition.hpp:852-954, instantiated as
+ *                              <int,int,unsigned> by GSAP.cu:20,25
+ *   gsap_blockmodel_i32     <- calculate_new_csr, GSAP/partition.hpp:707-849
+ *   gsap_merge_phase_i32    <- propose_block_merge + carry_out_best_merges,
+ *                              GSAP/partition.hpp:324-454 and 26-78
+ *   gsap_nodal_batch_i32    <- propose_nodal_move, GSAP/partition.hpp:456-662
+ *   gsap_mdl_i32            <- compute_overall_entropy_cuda, GSAP/partition.hpp:665-704
+ *   gsap_read_tsv_i32       <- readFile, GSAP/load.hpp:6-133 (whitespace tolerant)
+ *   gsap_pairwise_i32       <- bf::evaluate pair counts, GSAP/evaluate/evaluate.hpp:130-181
+ *   gsap_generate_dcsbm_i32 -- no counterpart: the seeded DC-SBM generator that
+ *                              produces the benchmark graphs.
+ */
+#ifndef GSAP_GSAP_H
+#define GSAP_GSAP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GSAP_OK 0
+#define GSAP_ERR_INVALID 1   /* bad argument or malformed graph */
+#define GSAP_ERR_DEVICE 2    /* HIP runtime error */
+#define GSAP_ERR_MEMORY 3    /* device allocation failed */
+#define GSAP_ERR_INTERNAL 4  /* invariant broken inside the library */
+#define GSAP_ERR_IO 5        /* file could not be read */
+
+#define GSAP_PRECISION_FP64 0  /* plain form: FP64 delta-S everywhere */
+#define GSAP_PRECISION_MIXED 1 /* FP32 bulk delta-S + FP64 near-threshold recheck */
+
+typedef struct gsap_ctx gsap_ctx;
+
+/* A directed graph in two CSR views (out-adjacency and in-adjacency).  Ids are
+ * 0-based int32, weights are positive int32, ptr arrays have N+1 entries (the
+ * reference's Csr::adj_ptr has N; the C++ wrapper appends the end).  All
+ * pointers are host pointers. */
+typedef struct gsap_graph_i32 {
+  int64_t N;
+  int64_t E;
+  const int32_t *out_ptr; /* N+1 */
+  const int32_t *out_adj; /* E   */
+  const int32_t *out_w;   /* E   */
+  const int32_t *in_ptr;  /* N+1 */
+  const int32_t *in_adj;  /* E   */
+  const int32_t *in_w;    /* E   */
+} gsap_graph_i32;
+
+/* Algorithm options.  Defaults (gsap_options_default) are the hyper-parameters
+ * hard-coded at GSAP/partition.hpp:855-861. */
+typedef struct gsap_options {
+  uint64_t seed;              /* Philox key */
+  int32_t num_proposals;      /* merge proposals per block (10), 1..32 */
+  float reduction_rate;       /* block reduction rate (0.35) */
+  int32_t max_nodal_itr;      /* nodal sweeps per level (100) */
+  int32_t num_batches;        /* nodal batches per sweep (4) */
+  double threshold1;          /* convergence threshold before bracketing (5e-4) */
+  double threshold2;          /* convergence threshold after bracketing (1e-4) */
+  int32_t moving_avg_window;  /* moving-average window (3) */
+  int32_t max_levels;         /* safety cap on outer iterations (0 = none) */
+  int32_t precision;          /* GSAP_PRECISION_* */
+  uint32_t chain;             /* Philox stream id: distinct chains, distinct streams */
+  int32_t trace;              /* 1: record per-step digests (slow, for tests) */
+  int32_t dense_max_B;        /* use the dense block matrix when B <= this (0 = auto) */
+  int32_t profile;            /* 1: time kernel classes with HIP events (report.kernel_ms) */
+  int32_t reserved[7];
+} gsap_options;
+
+/* Per-call report.  Times are seconds of host wall clock around device work. */
+typedef struct gsap_report {
+  int32_t B;              /* final number of blocks (ids may have gaps) */
+  int32_t levels;         /* outer iterations */
+  int32_t merge_phases;
+  int32_t sweeps;         /* nodal sweeps (4 batches + MDL each) */
+  int64_t batches;
+  double S;               /* final description length (FP64) */
+  double t_total;
+  double t_merge;
+  double t_sweep;         /* sum over sweeps of sweep wall time */
+  double t_rebuild;       /* block-matrix rebuild time inside sweeps (if profiled) */
+  int64_t sweep_edges;    /* E * sweeps */
+  double max_abs_err_fp32;/* mixed form: max |dS32 - dS64| seen on rechecked moves */
+  int64_t rechecks;       /* mixed form: decisions recomputed in FP64 */
+  int64_t n_trace;        /* trace records produced (may exceed capacity) */
+  double kernel_ms[8];    /* device time per kernel class when options.profile = 1:     */
+  int64_t kernel_calls[8];/* 0 nodal moves, 1 block-matrix rebuild, 2 per-block terms,   */
+                          /* 3 merge proposals+delta-S, 4 MDL, 5 accept+batch sum        */
+} gsap_report;
+
+/* One trace record; identical layout to the oracle's. */
+typedef struct gsap_trace_rec {
+  int32_t kind; /* 1 merge, 2 batch, 3 sweep, 4 golden */
+  int32_t B;
+  int32_t a, b;
+  uint64_t h1, h2;
+  double x, y;
+} gsap_trace_rec;
+
+void gsap_options_default(gsap_options *o);
+
+int gsap_create(gsap_ctx **ctx, int device);
+int gsap_destroy(gsap_ctx *ctx);
+const char *gsap_last_error(const gsap_ctx *ctx);
+
+/* Whole partition (GSAP/partition.hpp:852-954).  partitions_out[N] receives
+ * 0-based block ids.  report may be NULL. */
+int gsap_partition_i32(gsap_ctx *ctx, const gsap_graph_i32 *g, const gsap_options *o,
+                       int32_t *partitions_out, gsap_report *report);
+
+/* Trace records of the last gsap_partition_i32 call with o->trace = 1. */
+int gsap_get_trace(const gsap_ctx *ctx, gsap_trace_rec *out, int64_t cap, int64_t *n);
+
+/* Block matrix of a labelling (GSAP/partition.hpp:707-849).  Caller-allocated
+ * outputs: rowptr[B+1], col[E], val[E], colptr[B+1], row[E], cval[E], dout[B],
+ * din[B].  *nnz receives the number of nonzero blocks pairs. */
+int gsap_blockmodel_i32(gsap_ctx *ctx, const gsap_graph_i32 *g, const int32_t *labels, int32_t B,
+                        int64_t *rowptr, int32_t *col, int64_t *val, int64_t *colptr, int32_t *row,
+                        int64_t *cval, int64_t *dout, int64_t *din, int64_t *nnz,
+                        const gsap_options *o);
+
+/* One block-merge phase at merge counter m (GSAP/partition.hpp:324-454, 26-78).
+ * labels[N] is updated in place; prop[np*B], dS[np*B], best[B], dS_best[B] may
+ * be NULL.  *B_new receives the block count after merging. */
+int gsap_merge_phase_i32(gsap_ctx *ctx, const gsap_graph_i32 *g, const gsap_options *o,
+                         int32_t *labels, int32_t B, int32_t B_to_merge, uint32_t m, int32_t *prop,
+                         double *dS, int32_t *best, double *dS_best, int32_t *B_new);
+
+/* One nodal batch b at batch counter q (GSAP/partition.hpp:456-662).  labels[N]
+ * updated in place; prop[BS], dS[BS] may be NULL; *dE receives the sum of the
+ * accepted delta-S. */
+int gsap_nodal_batch_i32(gsap_ctx *ctx, const gsap_graph_i32 *g, const gsap_options *o,
+                         int32_t *labels, int32_t B, int32_t b, uint32_t q, int32_t *prop,
+                         double *dS, double *dE);
+
+/* Description length of a labelling (GSAP/partition.hpp:665-704), FP64. */
+int gsap_mdl_i32(gsap_ctx *ctx, const gsap_graph_i32 *g, const int32_t *labels, int32_t B,
+                 double *S, const gsap_options *o);
+
+/* ---- host-side helpers (no device needed) ---- */
+
+/* Seeded degree-corrected SBM.  Two-phase: call with out arrays NULL to learn
+ * E (= llround(N*mean_deg)), then again with arrays of the right sizes.
+ * alpha <= 0 gives near-equal block sizes, else Dirichlet(alpha) sizes. */
+typedef struct gsap_dcsbm_params {
+  int64_t N;
+  int32_t K;
+  double alpha;
+  double rho;       /* intra / inter edge ratio */
+  double mean_deg;
+  int32_t deg_min;  /* 5 */
+  int32_t deg_max;  /* 100 */
+  double exponent;  /* 2.5 */
+  uint64_t seed;
+} gsap_dcsbm_params;
+
+int gsap_generate_dcsbm_i32(const gsap_dcsbm_params *p, int64_t *E, int32_t *out_ptr,
+                            int32_t *out_adj, int32_t *out_w, int32_t *in_ptr, int32_t *in_adj,
+                            int32_t *in_w, int32_t *truth);
+
+/* GraphChallenge TSV (1-based "src dst w" lines).  Two-phase like the
+ * generator: first call with arrays NULL to get N and E. */
+int gsap_read_tsv_i32(const char *path, int64_t *N, int64_t *E, int32_t *out_ptr, int32_t *out_adj,
+                      int32_t *out_w, int32_t *in_ptr, int32_t *in_adj, int32_t *in_w);
+
+/* Pairwise precision / recall against a truth labelling (int64 counts). */
+int gsap_pairwise_i32(int64_t N, const int32_t *truth, const int32_t *given, double *precision,
+                      double *recall);
+
+/* Canonical log2 used by every FP64 delta-S / MDL term (host evaluation). */
+double gsap_log2(double x);
+/* Its constants: inv_c[128], log2_c[128], poly[9]. */
+void gsap_log2_tables(double *inv_c, double *log2_c, double *poly);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GSAP_GSAP_H */

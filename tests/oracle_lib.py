@@ -1,0 +1,202 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) for the tests.
+
+The oracle is the checker: tests compare the HIP path against it; nothing in the
+product imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class OGraph(C.Structure):
+    _fields_ = [("N", C.c_int64), ("E", C.c_int64),
+                ("out_ptr", C.c_void_p), ("out_adj", C.c_void_p), ("out_w", C.c_void_p),
+                ("in_ptr", C.c_void_p), ("in_adj", C.c_void_p), ("in_w", C.c_void_p)]
+
+
+class OOptions(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("num_proposals", C.c_int32),
+                ("reduction_rate", C.c_float), ("max_nodal_itr", C.c_int32),
+                ("num_batches", C.c_int32), ("threshold1", C.c_double),
+                ("threshold2", C.c_double), ("moving_avg_window", C.c_int32),
+                ("max_levels", C.c_int32), ("chain", C.c_uint32)]
+
+
+class OTrace(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("B", C.c_int32), ("a", C.c_int32), ("b", C.c_int32),
+                ("h1", C.c_uint64), ("h2", C.c_uint64), ("x", C.c_double), ("y", C.c_double)]
+
+
+class OReport(C.Structure):
+    _fields_ = [("B", C.c_int32), ("levels", C.c_int32), ("merge_phases", C.c_int32),
+                ("sweeps", C.c_int32), ("batches", C.c_int64), ("S", C.c_double),
+                ("n_trace", C.c_int64)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.run(["make", "-C", ORACLE_DIR], check=True, capture_output=True)
+        L = C.CDLL(ORACLE_SO)
+        L.orc_log2.restype = C.c_double
+        L.orc_log2.argtypes = [C.c_double]
+        L.orc_log2_tables.argtypes = [_f64p, _f64p, _f64p]
+        L.orc_csum64.restype = C.c_double
+        L.orc_csum64.argtypes = [_f64p, C.c_int64]
+        L.orc_csum_array.restype = C.c_double
+        L.orc_csum_array.argtypes = [_f64p, C.c_int64]
+        L.orc_blockmodel.restype = C.c_int64
+        L.orc_blockmodel.argtypes = [C.c_int64, C.c_int64, _i32p, _i32p, _i32p, _i32p, C.c_int32,
+                                     _i64p, _i32p, _i64p, _i64p, _i32p, _i64p, _i64p, _i64p]
+        L.orc_merge_phase.restype = C.c_int32
+        L.orc_merge_phase.argtypes = [C.POINTER(OGraph), C.POINTER(OOptions), _i32p, C.c_int32,
+                                      C.c_int32, C.c_uint32, _i32p, _f64p, _i32p, _f64p]
+        L.orc_nodal_batch.restype = None
+        L.orc_nodal_batch.argtypes = [C.POINTER(OGraph), C.POINTER(OOptions), _i32p, C.c_int32,
+                                      C.c_int32, C.c_uint32, _i32p, _f64p,
+                                      C.POINTER(C.c_double)]
+        L.orc_mdl.restype = C.c_double
+        L.orc_mdl.argtypes = [C.POINTER(OGraph), _i32p, C.c_int32]
+        L.orc_partition.restype = C.c_int
+        L.orc_partition.argtypes = [C.POINTER(OGraph), C.POINTER(OOptions), _i32p,
+                                    C.POINTER(OReport), C.POINTER(OTrace), C.c_int64]
+        L.orc_carry_out_merges.restype = C.c_int32
+        L.orc_carry_out_merges.argtypes = [_i32p, _f64p, C.c_int32, C.c_int32, _i32p]
+        L.orc_argmin_tree.restype = None
+        L.orc_argmin_tree.argtypes = [_f64p, _i32p, C.c_int32, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_int32)]
+        L.orc_pairwise.restype = None
+        L.orc_pairwise.argtypes = [C.c_int64, _i32p, _i32p, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]
+        L.orc_philox4x32_10.restype = None
+        L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                        C.POINTER(C.c_uint32)]
+        _lib = L
+    return _lib
+
+
+def ograph(g):
+    return OGraph(g.N, g.E, g.out_ptr.ctypes.data, g.out_adj.ctypes.data, g.out_w.ctypes.data,
+                  g.in_ptr.ctypes.data, g.in_adj.ctypes.data, g.in_w.ctypes.data)
+
+
+def ooptions(o):
+    return OOptions(o.seed, o.num_proposals, o.reduction_rate, o.max_nodal_itr, o.num_batches,
+                    o.threshold1, o.threshold2, o.moving_avg_window, o.max_levels, o.chain)
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def log2(x):
+    return lib().orc_log2(float(x))
+
+
+def log2_tables():
+    inv = np.zeros(128); lg = np.zeros(128); poly = np.zeros(9)
+    lib().orc_log2_tables(inv, lg, poly)
+    return inv, lg, poly
+
+
+def csum64(x):
+    x = np.ascontiguousarray(x, np.float64)
+    return lib().orc_csum64(x, len(x))
+
+
+def csum_array(x):
+    x = np.ascontiguousarray(x, np.float64)
+    return lib().orc_csum_array(x, len(x))
+
+
+def blockmodel(g, labels, B):
+    E = g.E
+    rowptr = np.zeros(B + 1, np.int64); colptr = np.zeros(B + 1, np.int64)
+    col = np.zeros(max(E, 1), np.int32); row = np.zeros(max(E, 1), np.int32)
+    val = np.zeros(max(E, 1), np.int64); cval = np.zeros(max(E, 1), np.int64)
+    dout = np.zeros(B, np.int64); din = np.zeros(B, np.int64)
+    lab = np.ascontiguousarray(labels, np.int32)
+    k = lib().orc_blockmodel(g.N, E, g.out_ptr, g.out_adj, g.out_w, lab, B, rowptr, col, val,
+                             colptr, row, cval, dout, din)
+    assert k >= 0
+    return dict(rowptr=rowptr, col=col[:k], val=val[:k], colptr=colptr, row=row[:k],
+                cval=cval[:k], dout=dout, din=din, nnz=k)
+
+
+def merge_phase(g, opts, labels, B, B_to_merge, m):
+    np_ = opts.num_proposals
+    lab = np.ascontiguousarray(labels, np.int32).copy()
+    prop = np.zeros(np_ * B, np.int32); dS = np.zeros(np_ * B, np.float64)
+    best = np.zeros(B, np.int32); dSb = np.zeros(B, np.float64)
+    og, oo = ograph(g), ooptions(opts)
+    nb = lib().orc_merge_phase(C.byref(og), C.byref(oo), lab, B, B_to_merge, m, prop, dS, best,
+                               dSb)
+    return dict(labels=lab, prop=prop, dS=dS, best=best, dS_best=dSb, B=nb)
+
+
+def nodal_batch(g, opts, labels, B, b, q):
+    BS = g.N // opts.num_batches
+    lab = np.ascontiguousarray(labels, np.int32).copy()
+    prop = np.zeros(max(BS, 1), np.int32); dS = np.zeros(max(BS, 1), np.float64)
+    dE = C.c_double()
+    og, oo = ograph(g), ooptions(opts)
+    lib().orc_nodal_batch(C.byref(og), C.byref(oo), lab, B, b, q, prop, dS, C.byref(dE))
+    return dict(labels=lab, prop=prop[:BS], dS=dS[:BS], dE=dE.value)
+
+
+def mdl(g, labels, B):
+    og = ograph(g)
+    return lib().orc_mdl(C.byref(og), np.ascontiguousarray(labels, np.int32), B)
+
+
+def partition(g, opts, trace_cap=0):
+    out = np.zeros(g.N, np.int32)
+    rep = OReport()
+    og, oo = ograph(g), ooptions(opts)
+    buf = (OTrace * max(trace_cap, 1))()
+    rc = lib().orc_partition(C.byref(og), C.byref(oo), out, C.byref(rep),
+                             buf if trace_cap else None, trace_cap)
+    assert rc == 0
+    n = min(rep.n_trace, trace_cap)
+    tr = [(t.kind, t.B, t.a, t.b, t.h1, t.h2, t.x, t.y) for t in buf[:n]]
+    return out, rep, tr
+
+
+def carry_out_merges(best, dS_best, B, B_to_merge):
+    m = np.zeros(B, np.int32)
+    merged = lib().orc_carry_out_merges(np.ascontiguousarray(best, np.int32),
+                                        np.ascontiguousarray(dS_best, np.float64), B, B_to_merge,
+                                        m)
+    return merged, m
+
+
+def argmin_tree(dS, prop):
+    bd, b = C.c_double(), C.c_int32()
+    lib().orc_argmin_tree(np.ascontiguousarray(dS, np.float64),
+                          np.ascontiguousarray(prop, np.int32), len(dS), C.byref(bd), C.byref(b))
+    return bd.value, b.value
+
+
+def pairwise(truth, given):
+    p, r = C.c_double(), C.c_double()
+    lib().orc_pairwise(len(truth), np.ascontiguousarray(truth, np.int32),
+                       np.ascontiguousarray(given, np.int32), C.byref(p), C.byref(r))
+    return p.value, r.value

@@ -1,0 +1,180 @@
+"""GPU parity: the HIP path against the CPU oracle, bit-exact.
+
+Integer outputs (block matrix, proposals, labels, merges) must be identical; FP64 delta-S,
+batch sums and description lengths must be identical bit for bit (same canonical log2 and
+summation order on both sides)."""
+import numpy as np
+import pytest
+
+import gsap_amd
+import oracle_lib as O
+from conftest import random_graph, tsv_graph
+from gsap_amd import Options
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = gsap_amd.Context(0)
+    yield c
+    c.close()
+
+
+def hub_graph(seed=5):
+    """Random graph plus two hubs with > 128 in/out neighbours (global hash tables)."""
+    rng = np.random.default_rng(seed)
+    N = 600
+    src, dst = [], []
+    seen = set()
+    def add(a, b):
+        if a != b and (a, b) not in seen:
+            seen.add((a, b)); src.append(a); dst.append(b)
+    for _ in range(3000):
+        add(int(rng.integers(N)), int(rng.integers(N)))
+    for h in (3, 250):
+        for t in rng.choice(N, 300, replace=False):
+            add(h, int(t)); add(int(t), h)
+    return gsap_amd.Graph.from_edges(N, src, dst)
+
+
+CASES = {
+    "tiny_w": lambda: random_graph(64, 400, 1, wmax=4),
+    "rand": lambda: random_graph(700, 6000, 2),
+    "hubs": hub_graph,
+    "LL1k": lambda: tsv_graph("LL"),
+    "HH5k": lambda: tsv_graph("HH", "5k"),
+}
+_graphs = {}
+
+
+def graph(name):
+    if name not in _graphs:
+        _graphs[name] = CASES[name]()
+    return _graphs[name]
+
+
+def eq_bits(a, b):
+    return np.array_equal(np.asarray(a, np.float64).view(np.uint64),
+                          np.asarray(b, np.float64).view(np.uint64))
+
+
+def labels_for(g, B, seed):
+    lab = np.random.default_rng(seed).integers(0, B, g.N).astype(np.int32)
+    return lab
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("Bfrac,dense", [(1.0, 0), (0.25, 0), (0.01, 0), (0.01, 1), (0.25, 100000)])
+def test_blockmodel(ctx, name, Bfrac, dense):
+    g = graph(name)
+    B = max(1, int(g.N * Bfrac))
+    lab = labels_for(g, B, 3) if B < g.N else np.arange(g.N, dtype=np.int32)
+    print(f"blockmodel {name} B={B} dense_max_B={dense}", flush=True)
+    got = ctx.blockmodel(g, lab, B, Options(dense_max_B=dense))
+    ref = O.blockmodel(g, lab, B)
+    for k in ("nnz", "rowptr", "col", "val", "colptr", "row", "cval", "dout", "din"):
+        assert np.array_equal(got[k], ref[k]), k
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("Bfrac", [1.0, 0.3, 0.02])
+def test_mdl(ctx, name, Bfrac):
+    g = graph(name)
+    B = max(1, int(g.N * Bfrac))
+    lab = labels_for(g, B, 4) if B < g.N else np.arange(g.N, dtype=np.int32)
+    assert eq_bits(ctx.mdl(g, lab, B), O.mdl(g, lab, B))
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("Bfrac,dense", [(1.0, 0), (0.2, 0), (0.03, 0), (0.03, 1)])
+def test_merge_phase(ctx, name, Bfrac, dense):
+    g = graph(name)
+    B = max(2, int(g.N * Bfrac))
+    lab = labels_for(g, B, 6) if B < g.N else np.arange(g.N, dtype=np.int32)
+    o = Options(seed=17, dense_max_B=dense)
+    print(f"merge {name} B={B}", flush=True)
+    got = ctx.merge_phase(g, o, lab, B, int(B * 0.35), 3)
+    ref = O.merge_phase(g, o, lab, B, int(B * 0.35), 3)
+    assert np.array_equal(got["prop"], ref["prop"])
+    assert eq_bits(got["dS"], ref["dS"])
+    assert np.array_equal(got["best"], ref["best"])
+    assert eq_bits(got["dS_best"], ref["dS_best"])
+    assert got["B"] == ref["B"]
+    assert np.array_equal(got["labels"], ref["labels"])
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("Bfrac,dense", [(1.0, 0), (0.15, 0), (0.02, 0), (0.02, 1)])
+def test_nodal_batch(ctx, name, Bfrac, dense):
+    g = graph(name)
+    B = max(2, int(g.N * Bfrac))
+    lab = labels_for(g, B, 8) if B < g.N else np.arange(g.N, dtype=np.int32)
+    o = Options(seed=23, dense_max_B=dense)
+    for b in range(o.num_batches):
+        print(f"nodal {name} B={B} batch={b}", flush=True)
+        got = ctx.nodal_batch(g, o, lab, B, b, 11 + b)
+        ref = O.nodal_batch(g, o, lab, B, b, 11 + b)
+        assert np.array_equal(got["prop"], ref["prop"])
+        assert eq_bits(got["dS"], ref["dS"])
+        assert eq_bits(got["dE"], ref["dE"])
+        assert np.array_equal(got["labels"], ref["labels"])
+
+
+@pytest.mark.parametrize("name", ["tiny_w", "hubs", "LL1k"])
+def test_partition_trace(ctx, name):
+    g = graph(name)
+    o = Options(seed=3, trace=1)
+    print(f"partition {name}", flush=True)
+    lab, rep = ctx.partition(g, o)
+    tr = ctx.trace()
+    rlab, rrep, rtr = O.partition(g, o, trace_cap=len(tr) + 10)
+    assert len(tr) == rrep.n_trace
+    for k, (a, b) in enumerate(zip(tr, rtr)):
+        assert a[:6] == b[:6], (k, a, b)
+        assert eq_bits(a[6], b[6]) and eq_bits(a[7], b[7]), (k, a, b)
+    assert np.array_equal(lab, rlab)
+    assert rep.B == rrep.B and eq_bits(rep.S, rrep.S)
+
+
+def test_partition_generated_c1(ctx):
+    g = gsap_amd.generate_dcsbm(1000, 11, 0.0, 4.0, 20.0, seed=1)
+    o = Options(seed=1)
+    lab, rep = ctx.partition(g, o)
+    rlab, rrep, _ = O.partition(g, o)
+    assert np.array_equal(lab, rlab) and eq_bits(rep.S, rrep.S)
+    assert rep.S == ctx.mdl(g, lab, rep.B)
+
+
+def test_partition_generated_c2(ctx):
+    g = gsap_amd.generate_dcsbm(20000, 33, 1.0, 1.5, 20.0, seed=2)
+    o = Options(seed=1)
+    print("partition C2", flush=True)
+    lab, rep = ctx.partition(g, o)
+    rlab, rrep, _ = O.partition(g, o)
+    assert rep.B == rrep.B and eq_bits(rep.S, rrep.S)
+    assert np.array_equal(lab, rlab)
+
+
+@pytest.fixture(scope="module")
+def c3():
+    return gsap_amd.generate_dcsbm(200000, 71, 1.0, 1.5, 22.5, seed=3)
+
+
+@pytest.mark.parametrize("B", [71, 5000, 200000])
+def test_c3_blockmodel_and_batch(ctx, c3, B):
+    g = c3
+    lab = labels_for(g, B, 9) if B < g.N else np.arange(g.N, dtype=np.int32)
+    print(f"C3 blockmodel B={B}", flush=True)
+    got = ctx.blockmodel(g, lab, B)
+    ref = O.blockmodel(g, lab, B)
+    for k in ("nnz", "rowptr", "col", "val", "colptr", "row", "cval", "dout", "din"):
+        assert np.array_equal(got[k], ref[k]), k
+    # size-independent invariants
+    assert got["val"].sum() == g.E and got["dout"].sum() == g.E and got["din"].sum() == g.E
+    o = Options(seed=5)
+    print(f"C3 nodal B={B}", flush=True)
+    gb = ctx.nodal_batch(g, o, lab, B, 1, 40)
+    rb = O.nodal_batch(g, o, lab, B, 1, 40)
+    assert np.array_equal(gb["prop"], rb["prop"]) and eq_bits(gb["dS"], rb["dS"])
+    assert eq_bits(gb["dE"], rb["dE"])
