@@ -260,7 +260,7 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   c->OUT.ensure(8 * N); c->sof.ensure(4 * N); c->coin.ensure(N);
   c->prop.ensure(4 * (size_t)np * N); c->dS.ensure(8 * (size_t)np * N);
   c->best.ensure(4 * N); c->dSbest.ensure(8 * N); c->map.ensure(4 * N);
-  c->terms.ensure(8 * E); c->tiles.ensure(8 * (E / 1024 + 2));
+  c->terms.ensure(8 * E); c->tiles.ensure(8 * (std::max(E, N) / 1024 + 2));
   c->res.ensure(8 * 64);
   BMBuf &b = c->bm;
   b.rowptr = c->rowptr.as<int32_t>(); b.colptr = c->colptr.as<int32_t>();
