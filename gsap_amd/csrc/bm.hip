@@ -147,7 +147,9 @@ __global__ void k_rle(const K *keys, const uint32_t *vals, const uint32_t *flags
                       int32_t *erow) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
        e += (int64_t)gridDim.x * blockDim.x) {
-    uint32_t i = uidx[e];
+    // uidx is the exclusive scan of the run-head flags: a run's index is
+    // uidx + flag - 1 for every element of the run
+    uint32_t i = uidx[e] + flags[e] - 1u;
     atomicAdd(&val[i], (int32_t)vals[e]);
     if (flags[e]) {
       K k = keys[e];
