@@ -70,21 +70,6 @@ __global__ __launch_bounds__(64) void k_dense_rows(const int32_t *D, int32_t B, 
   if (lane == 0) { rcnt[r] = c; dout[r] = s; }
 }
 
-// one thread per column: nonzero count and column sum (coalesced across threads)
-__global__ void k_dense_cols(const int32_t *D, int32_t B, uint32_t *ccnt, int32_t *din) {
-  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= B) return;
-  uint32_t c = 0;
-  int32_t sum = 0;
-  for (int32_t t = 0; t < B; ++t) {
-    int32_t x = D[(int64_t)t * B + s];
-    c += x != 0;
-    sum += x;
-  }
-  ccnt[s] = c;
-  din[s] = sum;
-}
-
 __global__ __launch_bounds__(64) void k_dense_write_rows(const int32_t *D, int32_t B,
                                                          const int32_t *rowptr, int32_t *col,
                                                          int32_t *val, int32_t *erow) {
@@ -98,20 +83,38 @@ __global__ __launch_bounds__(64) void k_dense_write_rows(const int32_t *D, int32
     uint64_t m = __ballot(x != 0);
     if (x != 0) {
       int32_t pos = run + (int32_t)__popcll(m & lt);
-      col[pos] = t; val[pos] = x; erow[pos] = r;
+      col[pos] = t; val[pos] = x;
+      if (erow) erow[pos] = r;
     }
     run += (int32_t)__popcll(m);
   }
 }
 
-__global__ void k_dense_write_cols(const int32_t *D, int32_t B, const int32_t *colptr,
-                                   int32_t *row, int32_t *cval) {
-  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= B) return;
-  int32_t pos = colptr[s];
-  for (int32_t t = 0; t < B; ++t) {
-    int32_t x = D[(int64_t)t * B + s];
-    if (x != 0) { row[pos] = t; cval[pos] = x; ++pos; }
+// one wave per segment: out[x] = sum of vals[ptr[x] .. ptr[x+1])
+__global__ __launch_bounds__(256) void k_seg_sum(const int32_t *ptr, const int32_t *vals,
+                                                 int32_t B, int32_t *out) {
+  const int32_t x = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (x >= B) return;
+  const int lane = threadIdx.x & 63;
+  int32_t s = 0;
+  for (int32_t k = ptr[x] + lane; k < ptr[x + 1]; k += 64) s += vals[k];
+  for (int off = 32; off >= 1; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) out[x] = s;
+}
+
+// DT = D^T through 32x32 LDS tiles (coalesced reads and writes)
+__global__ __launch_bounds__(256) void k_transpose(const int32_t *D, int32_t B, int32_t *DT) {
+  __shared__ int32_t tile[32][33];
+  const int32_t bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int k = ty; k < 32; k += 8) {
+    const int32_t r = by + k, c = bx + tx;
+    if (r < B && c < B) tile[k][tx] = D[(int64_t)r * B + c];
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int32_t r = bx + k, c = by + tx;
+    if (r < B && c < B) DT[(int64_t)r * B + c] = tile[tx][k];
   }
 }
 
@@ -177,16 +180,6 @@ __global__ void k_ptr_from_major(const int32_t *maj, const int32_t *nnz_dev, int
   }
 }
 
-__global__ void k_degrees(const int32_t *erow, const int32_t *col, const int32_t *val,
-                          const int32_t *nnz_dev, int32_t *dout, int32_t *din) {
-  const int32_t nnz = *nnz_dev;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
-       k += (int64_t)gridDim.x * blockDim.x) {
-    atomicAdd(&dout[erow[k]], val[k]);
-    atomicAdd(&din[col[k]], val[k]);
-  }
-}
-
 template <class K>
 __global__ void k_make_tkeys(const int32_t *col, const int32_t *erow, const int32_t *val,
                              const int32_t *nnz_dev, int64_t n, int32_t B, K pad, K *keys,
@@ -249,9 +242,7 @@ static void sparse_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf 
   k_rle<K><<<grid_for(E, T), T, 0, st>>>(k0, v0, rs.flags, rs.uidx, E, B, bm.col, bm.val,
                                          bm.erow);
   k_ptr_from_major<<<grid_for(E, T), T, 0, st>>>(bm.erow, nnz_dev, B, bm.rowptr);
-  k_fill_i32<<<grid_for(B, T), T, 0, st>>>(bm.dout, B, 0);
-  k_fill_i32<<<grid_for(B, T), T, 0, st>>>(bm.din, B, 0);
-  k_degrees<<<grid_for(E, T), T, 0, st>>>(bm.erow, bm.col, bm.val, nnz_dev, bm.dout, bm.din);
+  k_seg_sum<<<(unsigned)((B + 3) / 4), 256, 0, st>>>(bm.rowptr, bm.val, B, bm.dout);
   // CSC: sort (col, row) keys of the nnz entries; padding sorts behind them
   const K pad = (bits >= (int)(8 * sizeof(K))) ? (K)~(K)0 : (K)(((K)1 << bits) - 1);
   k0 = (K *)rs.k0; k1 = (K *)rs.k1; v0 = rs.v0; v1 = rs.v1;
@@ -262,6 +253,7 @@ static void sparse_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf 
   k_csc_unpack<K><<<grid_for(E, T), T, 0, st>>>(k0, v0, nnz_dev, B, bm.row, bm.cval, ccol);
   k_copy_i32<<<1, 1, 0, st>>>(nnz_dev, bm.colptr + B);
   k_ptr_from_major<<<grid_for(E, T), T, 0, st>>>(ccol, nnz_dev, B, bm.colptr);
+  k_seg_sum<<<(unsigned)((B + 3) / 4), 256, 0, st>>>(bm.colptr, bm.cval, B, bm.din);
 }
 
 static void dense_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm,
@@ -282,15 +274,20 @@ static void dense_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &
   k_dense_rows<<<B, 64, 0, st>>>(D, B, rs.cnt, bm.dout);
   scan_exclusive(rs.cnt, (uint32_t *)bm.rowptr, B, (uint32_t *)(bm.rowptr + B), rs.tmp, st);
   k_dense_write_rows<<<B, 64, 0, st>>>(D, B, bm.rowptr, bm.col, bm.val, bm.erow);
-  k_dense_cols<<<grid_for(B, 64), 64, 0, st>>>(D, B, rs.cnt, bm.din);
+  // columns: the same row kernels on the transpose (partials are free again here)
+  int32_t *DT = rs.partials;
+  const unsigned tb = (unsigned)((B + 31) / 32);
+  k_transpose<<<dim3(tb, tb), 256, 0, st>>>(D, B, DT);
+  k_dense_rows<<<B, 64, 0, st>>>(DT, B, rs.cnt, bm.din);
   scan_exclusive(rs.cnt, (uint32_t *)bm.colptr, B, (uint32_t *)(bm.colptr + B), rs.tmp, st);
-  k_dense_write_cols<<<grid_for(B, 64), 64, 0, st>>>(D, B, bm.colptr, bm.row, bm.cval);
+  k_dense_write_rows<<<B, 64, 0, st>>>(DT, B, bm.colptr, bm.row, bm.cval, nullptr);
 }
 
 void rebuild_blockmodel(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm,
                         RebuildScratch &rs, int32_t dense_max_B, hipStream_t st) {
   bm.B = B;
-  const bool dense = B <= dense_max_B && (int64_t)B * B <= bm.dense_cap;
+  const bool dense = B <= dense_max_B && (int64_t)B * B <= bm.dense_cap &&
+                     (int64_t)B * B <= rs.partials_cap;
   if (dense) {
     dense_path(g, lab, B, bm, rs, st);
   } else if ((uint64_t)B * (uint64_t)B <= 0xFFFFFFFFull) {

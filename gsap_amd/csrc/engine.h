@@ -61,8 +61,7 @@ struct RngKey {
 };
 void launch_block_terms(const BMView &m, double *OUT, int32_t *sof, uint8_t *coin, RngKey key,
                         uint32_t phase, bool sample, int32_t avg_row, hipStream_t st);
-void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const double *OUT,
-                  const int32_t *sof, const uint8_t *coin, RngKey key, uint32_t phase, int64_t lo,
+void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const int32_t *sof, const uint8_t *coin, RngKey key, uint32_t phase, int64_t lo,
                   int64_t BS, int32_t *prop, double *dS, hipStream_t st);
 void launch_apply(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo, int64_t BS,
                   hipStream_t st);

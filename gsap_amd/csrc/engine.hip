@@ -255,7 +255,7 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   size_t tw = std::max(radix_temp_words(E), scan_temp_words(N + 1));
   c->tmp.ensure(4 * tw);
   c->cnt.ensure(4 * (N + 1));
-  const int64_t pc = 256 * 128 * 128;
+  const int64_t pc = std::max<int64_t>(256 * 128 * 128, dB * dB);
   c->partials.ensure(4 * pc);
   c->OUT.ensure(8 * N); c->sof.ensure(4 * N); c->coin.ensure(N);
   c->prop.ensure(4 * (size_t)np * N); c->dS.ensure(8 * (size_t)np * N);
@@ -399,12 +399,11 @@ void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, uint32_t phase, 
   hipStream_t st = c->st;
   BMView m = c->bm.view();
   int p = c->prof.begin(2, st);
-  launch_block_terms(m, c->OUT.as<double>(), c->sof.as<int32_t>(), c->coin.as<uint8_t>(),
-                     rng_key(o), phase, true, ar, st);
+  launch_block_terms(m, nullptr, c->sof.as<int32_t>(), c->coin.as<uint8_t>(), rng_key(o), phase,
+                     true, ar, st);
   c->prof.end(p, st);
   p = c->prof.begin(0, st);
-  launch_nodal(c->g, c->lab.as<int32_t>(), m, c->OUT.as<double>(), c->sof.as<int32_t>(),
-               c->coin.as<uint8_t>(), rng_key(o), phase, lo, BS, c->prop.as<int32_t>(),
+  launch_nodal(c->g, c->lab.as<int32_t>(), m, c->sof.as<int32_t>(), c->coin.as<uint8_t>(), rng_key(o), phase, lo, BS, c->prop.as<int32_t>(),
                c->dS.as<double>(), st);
   c->prof.end(p, st);
   p = c->prof.begin(5, st);

@@ -57,14 +57,54 @@ GSAP_D void seg_for(int32_t n, F &&f) {
   }
 }
 
+// Group-parallel form of bm_walk: entries of row x then column x in chunks of G lanes, an
+// inclusive prefix sum of the weights inside the group, and the first lane whose running
+// weight exceeds target with an id other than skip.  Same result as the sequential walk.
+template <int G>
+GSAP_D int32_t group_walk(const BMView &m, int32_t x, uint64_t target, int32_t skip,
+                          int32_t fallback) {
+  const int q = threadIdx.x & (G - 1);
+  const int gbase = (threadIdx.x & 63) & ~(G - 1);
+  const int32_t r0 = m.rowptr[x], nr = m.rowptr[x + 1] - r0;
+  const int32_t c0 = m.colptr[x], n = nr + (m.colptr[x + 1] - c0);
+  uint64_t base = 0;
+  for (int32_t c = 0; c < n; c += G) {
+    const int32_t k = c + q;
+    uint64_t w = 0;
+    int32_t id = skip;
+    if (k < nr) { w = (uint64_t)m.val[r0 + k]; id = m.col[r0 + k]; }
+    else if (k < n) { w = (uint64_t)m.cval[c0 + k - nr]; id = m.row[c0 + k - nr]; }
+    uint64_t incl = w;
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+      uint64_t y = __shfl_up(incl, off, G);
+      if (q >= off) incl += y;
+    }
+    const bool pred = k < n && base + incl > target && id != skip;
+    const uint64_t bal = (__ballot(pred) >> gbase) & (G == 64 ? ~0ull : ((1ull << G) - 1ull));
+    if (bal) {
+      const int first = __ffsll((long long)bal) - 1;
+      return __shfl(id, first, G);
+    }
+    base += __shfl(incl, G - 1, G);
+  }
+  return fallback;
+}
+
+GSAP_D int64_t wave_sum_i64(int64_t x) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x += __shfl_down(x, off, 64);
+  return x;  // lane 0
+}
+
 static unsigned grid_for(int64_t n, int threads) {
   int64_t g = (n + threads - 1) / threads;
   return (unsigned)(g < 1 ? 1 : g);
 }
 
 // ------------------------------------------------ per-block terms and samples
-// OUT[x] = sum_t M_xt (log2 M_xt - (log2 d_out_x + log2 d_in_t))  (kernels.hpp:143-182)
-// and, when sampling, the nodal per-block neighbour sample and coin (kernels.hpp:942-997,
+// OUT[x] = sum_t M_xt (log2 M_xt - (log2 d_out_x + log2 d_in_t))  (kernels.hpp:143-182), used
+// by the merge delta-S, and the nodal per-block neighbour sample and coin (kernels.hpp:942-997,
 // 1024-1029; one draw per block per batch, shared by its vertices).
 template <int G>
 __global__ __launch_bounds__(256) void k_block_terms(BMView m, double *OUT, int32_t *sof,
@@ -73,20 +113,23 @@ __global__ __launch_bounds__(256) void k_block_terms(BMView m, double *OUT, int3
   const int32_t x = (int32_t)(blockIdx.x * (256 / G) + threadIdx.x / G);
   if (x >= m.B) return;
   const int q = threadIdx.x & (G - 1);
-  CAcc<G> acc;
-  acc.init();
-  const int32_t b0 = m.rowptr[x], n = m.rowptr[x + 1] - b0;
-  const double lo = m.ldout[x];
-  seg_for<G>(n, [&](int32_t k, int j) {
-    acc.p[j] += term(m.val[b0 + k], lo, m.ldin[m.col[b0 + k]]);
-  });
-  const double out = acc.reduce();
-  if (q == 0) {
-    OUT[x] = out;
-    if (sample) {
-      const Philox y = philox((uint32_t)x, phase, key.chain, TAG_BLOCK, key.k0, key.k1);
-      const int64_t deg = (int64_t)m.dout[x] + m.din[x];
-      sof[x] = deg == 0 ? 0 : bm_walk(m, x, mulhi(y.x0, deg), -1, 0);
+  if (OUT) {
+    CAcc<G> acc;
+    acc.init();
+    const int32_t b0 = m.rowptr[x], n = m.rowptr[x + 1] - b0;
+    const double lo = m.ldout[x];
+    seg_for<G>(n, [&](int32_t k, int j) {
+      acc.p[j] += term(m.val[b0 + k], lo, m.ldin[m.col[b0 + k]]);
+    });
+    const double out = acc.reduce();
+    if (q == 0) OUT[x] = out;
+  }
+  if (sample) {
+    const Philox y = philox((uint32_t)x, phase, key.chain, TAG_BLOCK, key.k0, key.k1);
+    const int64_t deg = (int64_t)m.dout[x] + m.din[x];
+    const int32_t so = deg == 0 ? 0 : group_walk<G>(m, x, mulhi(y.x0, deg), -1, 0);
+    if (q == 0) {
+      sof[x] = so;
       coin[x] = coin_random(y.x1, deg, m.B) ? 1 : 0;
     }
   }
@@ -110,7 +153,6 @@ struct NodalArgs {
   int64_t lo, BS;
   const int32_t *optr, *oadj, *ow, *iptr, *iadj, *iw, *kout, *kin, *lab;
   BMView m;
-  const double *OUT;
   const int32_t *sof;
   const uint8_t *coin;
   RngKey key;
@@ -122,9 +164,12 @@ struct NodalArgs {
   int32_t *hpool;
 };
 
-// One wave per vertex v of the batch: proposal (kernels.hpp:877-939, 1000-1032) and
-// delta-S against the batch-start block matrix (kernels.hpp:494-643, 1153-1726 with the
-// node-only entries run-merged).  Labels are not written here (batch-Jacobi).
+// One wave per vertex v of the batch: proposal (kernels.hpp:877-939, 1000-1032) and delta-S
+// against the batch-start block matrix (kernels.hpp:494-643, 1153-1726, node-only entries
+// run-merged).  Only the entries v touches are evaluated term by term ("slots": v's
+// out-blocks, r and s for rows r and s; v's in-blocks outside {r, s} for columns r and s);
+// the untouched entries contribute (log2 d_old - log2 d_new) * (their summed count).
+// Labels are not written here (batch-Jacobi).
 __global__ __launch_bounds__(64) void k_nodal(NodalArgs a) {
   __shared__ int32_t lko[LCAP], lwo[LCAP], lfo[LCAP], lki[LCAP], lwi[LCAP], lfi[LCAP];
   const int lane = threadIdx.x;
@@ -175,103 +220,85 @@ __global__ __launch_bounds__(64) void k_nodal(NodalArgs a) {
   for (int32_t j = lane; j < ni; j += 64) ht_insert(hi, a.lab[a.iadj[i0 + j]], a.iw[i0 + j], j);
   __syncthreads();
   const int64_t c_r = ht_weight(hi, r), c_s = ht_weight(hi, s);
-
-  // new row r
+  const bool r_in_a = ht_find(ho, r) >= 0, s_in_a = ht_find(ho, s) >= 0;
   const double l_dr = clog2((double)((int64_t)m.dout[r] - kout));
-  double p_ro = 0.0;
-  bool f = false;
-  {
-    const int32_t b0 = m.rowptr[r], n = m.rowptr[r + 1] - b0;
-    for (int32_t k = lane; k < n; k += 64) {
-      const int32_t t = m.col[b0 + k];
-      const int64_t R = (int64_t)m.val[b0 + k] - ht_weight(ho, t) - (t == r ? c_r : 0) +
-                        (t == s ? c_r : 0);
-      if (t == s) f = true;
-      if (R > 0) p_ro += term(R, m.ldin[t], l_dr);
-    }
-  }
-  const bool s_in_r = __ballot(f) != 0;
-  double nro = wave_sum(p_ro);
-  if (!s_in_r && c_r > 0) nro = nro + term(c_r, m.ldin[s], l_dr);
-
-  // new row s (row entries, then v's out-blocks absent from row s)
   const double l_ds = clog2((double)((int64_t)m.dout[s] + kout));
-  double p_sa = 0.0, p_sb = 0.0;
-  f = false;
-  {
-    const int32_t b0 = m.rowptr[s], n = m.rowptr[s + 1] - b0;
-    for (int32_t k = lane; k < n; k += 64) {
-      const int32_t t = m.col[b0 + k];
-      const int64_t S = (int64_t)m.val[b0 + k] + ht_weight(ho, t) - (t == r ? c_s : 0) +
-                        (t == s ? c_s : 0);
-      if (t == s) f = true;
-      if (S > 0) p_sa += term(S, m.ldin[t], l_ds);
-    }
-  }
-  for (int32_t j = lane; j < no; j += 64) {
-    const int32_t t = a.lab[a.oadj[o0 + j]];
-    const int32_t slot = ht_find(ho, t);
-    if (ld_relaxed(&ho.first[slot]) != j) continue;
-    if (bm_row_get(m, s, t) != 0) continue;
-    if (t == s) f = true;
-    const int64_t S = (int64_t)ld_relaxed(&ho.w[slot]) - (t == r ? c_s : 0) + (t == s ? c_s : 0);
-    if (S > 0) p_sb += term(S, m.ldin[t], l_ds);
-  }
-  const bool found = __ballot(f) != 0;
-  double nso = wave_sum(p_sa) + wave_sum(p_sb);
-  if (!found && c_s > 0) nso = nso + term(c_s, m.ldin[s], l_ds);
-
-  // old / new column r
   const double l_ir = clog2((double)((int64_t)m.din[r] - kin));
-  double p_inr = 0.0, p_nri = 0.0;
-  {
-    const int32_t b0 = m.colptr[r], n = m.colptr[r + 1] - b0;
-    for (int32_t k = lane; k < n; k += 64) {
-      const int32_t t = m.row[b0 + k];
-      if (t == r || t == s) continue;
-      const int32_t M = m.cval[b0 + k];
-      p_inr += term(M, m.ldout[t], m.ldin[r]);
-      const int64_t w = (int64_t)M - ht_weight(hi, t);
-      if (w > 0) p_nri += term(w, m.ldout[t], l_ir);
-    }
-  }
-  // old / new column s (column entries, then v's in-blocks absent from column s)
   const double l_is = clog2((double)((int64_t)m.din[s] + kin));
-  double p_ins = 0.0, p_nsa = 0.0, p_nsb = 0.0;
-  {
-    const int32_t b0 = m.colptr[s], n = m.colptr[s + 1] - b0;
-    for (int32_t k = lane; k < n; k += 64) {
-      const int32_t t = m.row[b0 + k];
-      if (t == r || t == s) continue;
-      const int32_t M = m.cval[b0 + k];
-      p_ins += term(M, m.ldout[t], m.ldin[s]);
-      const int64_t w = (int64_t)M + ht_weight(hi, t);
-      if (w > 0) p_nsa += term(w, m.ldout[t], l_is);
+
+  // rows r and s over the row slots
+  double p_row = 0.0;
+  int64_t sum_r = 0, sum_s = 0;
+  for (int32_t j = lane; j < no + 2; j += 64) {
+    int32_t t;
+    int64_t av = 0;
+    bool use;
+    if (j < no) {
+      t = a.lab[a.oadj[o0 + j]];
+      const int32_t slot = ht_find(ho, t);
+      use = ld_relaxed(&ho.first[slot]) == j;
+      av = ld_relaxed(&ho.w[slot]);
+    } else if (j == no) {
+      t = r;
+      use = !r_in_a;
+    } else {
+      t = s;
+      use = !s_in_a;
     }
+    if (!use) continue;
+    const int64_t Mr = bm_row_get(m, r, t), Ms = bm_row_get(m, s, t);
+    const double or_ = Mr > 0 ? term(Mr, m.ldout[r], m.ldin[t]) : 0.0;
+    const double os_ = Ms > 0 ? term(Ms, m.ldout[s], m.ldin[t]) : 0.0;
+    const int64_t R = Mr - av - (t == r ? c_r : 0) + (t == s ? c_r : 0);
+    const int64_t S = Ms + av - (t == r ? c_s : 0) + (t == s ? c_s : 0);
+    const double nr = R > 0 ? term(R, m.ldin[t], l_dr) : 0.0;
+    const double ns = S > 0 ? term(S, m.ldin[t], l_ds) : 0.0;
+    p_row += ((or_ + os_) - nr) - ns;
+    sum_r += Mr;
+    sum_s += Ms;
   }
+  // columns r and s over the column slots (t not in {r, s})
+  double p_col = 0.0;
+  int64_t csum_r = 0, csum_s = 0;
   for (int32_t j = lane; j < ni; j += 64) {
     const int32_t t = a.lab[a.iadj[i0 + j]];
     const int32_t slot = ht_find(hi, t);
-    if (ld_relaxed(&hi.first[slot]) != j) continue;
-    if (t == r || t == s) continue;
-    if (bm_col_get(m, s, t) != 0) continue;
-    const int64_t w = ld_relaxed(&hi.w[slot]);
-    if (w > 0) p_nsb += term(w, m.ldout[t], l_is);
+    if (ld_relaxed(&hi.first[slot]) != j || t == r || t == s) continue;
+    const int64_t c = ld_relaxed(&hi.w[slot]);
+    const int64_t Mr = bm_col_get(m, r, t), Ms = bm_col_get(m, s, t);
+    const double ir = Mr > 0 ? term(Mr, m.ldout[t], m.ldin[r]) : 0.0;
+    const double is = Ms > 0 ? term(Ms, m.ldout[t], m.ldin[s]) : 0.0;
+    const int64_t wr = Mr - c, ws = Ms + c;
+    const double nr = wr > 0 ? term(wr, m.ldout[t], l_ir) : 0.0;
+    const double ns = ws > 0 ? term(ws, m.ldout[t], l_is) : 0.0;
+    p_col += ((ir + is) - nr) - ns;
+    csum_r += Mr;
+    csum_s += Ms;
   }
-  const double inr = wave_sum(p_inr), ins = wave_sum(p_ins), nri = wave_sum(p_nri);
-  const double nsi = wave_sum(p_nsa) + wave_sum(p_nsb);
-  // calculate_dS_overall_nodal (kernels.hpp:600-643): accept iff dS < 0
-  double d = a.OUT[r] + a.OUT[s];
-  d = d + (inr + ins);
-  d = d - nro;
-  d = d - nso;
-  d = d - nri;
-  d = d - nsi;
-  if (lane == 0) a.dS[i] = d < 0.0 ? d : 0.0;
+  const double drow = wave_sum(p_row), dcol = wave_sum(p_col);
+  sum_r = wave_sum_i64(sum_r);
+  sum_s = wave_sum_i64(sum_s);
+  csum_r = wave_sum_i64(csum_r);
+  csum_s = wave_sum_i64(csum_s);
+  if (lane == 0) {
+    const int64_t W_r = (int64_t)m.dout[r] - sum_r, W_s = (int64_t)m.dout[s] - sum_s;
+    const double Y_r = W_r > 0 ? (m.ldout[r] - l_dr) * (double)W_r : 0.0;
+    const double Y_s = W_s > 0 ? (m.ldout[s] - l_ds) * (double)W_s : 0.0;
+    const int64_t W2_r = (int64_t)m.din[r] - bm_col_get(m, r, r) - bm_col_get(m, r, s) - csum_r;
+    const int64_t W2_s = (int64_t)m.din[s] - bm_col_get(m, s, r) - bm_col_get(m, s, s) - csum_s;
+    const double Y2_r = W2_r > 0 ? (m.ldin[r] - l_ir) * (double)W2_r : 0.0;
+    const double Y2_s = W2_s > 0 ? (m.ldin[s] - l_is) * (double)W2_s : 0.0;
+    // calculate_dS_overall_nodal (kernels.hpp:600-643): accept iff dS < 0
+    double d = drow - Y_r;
+    d = d - Y_s;
+    d = d + dcol;
+    d = d - Y2_r;
+    d = d - Y2_s;
+    a.dS[i] = d < 0.0 ? d : 0.0;
+  }
 }
 
-void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const double *OUT,
-                  const int32_t *sof, const uint8_t *coin, RngKey key, uint32_t phase, int64_t lo,
+void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const int32_t *sof, const uint8_t *coin, RngKey key, uint32_t phase, int64_t lo,
                   int64_t BS, int32_t *prop, double *dS, hipStream_t st) {
   if (BS <= 0) return;
   NodalArgs a;
@@ -279,7 +306,7 @@ void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const 
   a.optr = g.optr; a.oadj = g.oadj; a.ow = g.ow;
   a.iptr = g.iptr; a.iadj = g.iadj; a.iw = g.iw;
   a.kout = g.kout; a.kin = g.kin; a.lab = lab;
-  a.m = m; a.OUT = OUT; a.sof = sof; a.coin = coin; a.key = key; a.phase = phase;
+  a.m = m; a.sof = sof; a.coin = coin; a.key = key; a.phase = phase;
   a.prop = prop; a.dS = dS;
   a.hoff = g.hoff; a.hbits_o = g.hbits_o; a.hbits_i = g.hbits_i; a.hpool = g.hpool;
   k_nodal<<<(unsigned)BS, 64, 0, st>>>(a);
@@ -365,10 +392,10 @@ __global__ __launch_bounds__(256) void k_merge(BMView m, const double *OUT, RngK
   if (deg == 0) {
     s = (int32_t)mulhi(x.x0, B);
   } else {
-    const int32_t u = bm_walk(m, r, mulhi(x.x1, deg), -1, 0);
+    const int32_t u = group_walk<G>(m, r, mulhi(x.x1, deg), -1, 0);
     const int64_t degu = (int64_t)m.dout[u] + m.din[u];
     if (coin_random(x.x3, degu, B)) s = (int32_t)mulhi(x.x0, B);
-    else s = bm_walk(m, u, mulhi(x.x2, degu), r, 0);
+    else s = group_walk<G>(m, u, mulhi(x.x2, degu), r, 0);
   }
   // old column terms excluding t in {r, s}
   double inx_r, inx_s;

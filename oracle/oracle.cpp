@@ -436,10 +436,16 @@ struct Scratch {
   }
 };
 
-// Proposal + delta-S of one vertex against the batch-start state.
-// Formulas: kernels.hpp:494-643 (old terms, accept), 1153-1726 (new rows / columns).
-double nodal_vertex(const G &g, const BM &m, const std::vector<double> &OUT, const int32_t *lab,
-                    int64_t v, int32_t r, int32_t s, Scratch &sc) {
+// Delta-S of moving one vertex v from r to s against the batch-start state.
+// Formulas: kernels.hpp:494-643 (old terms, accept), 1153-1726 (new rows / columns), with the
+// node-only entries run-merged.  Entries of rows r, s and columns r, s that v does not touch
+// keep their count and change only through the row / column degree, so their old-minus-new
+// difference is (log2 d_old - log2 d_new) * W, W = their summed count; the remaining entries
+// ("slots": v's out-blocks, r and s for the rows; v's in-blocks outside {r, s} for the
+// columns) are evaluated term by term.  Slot order is the first-occurrence position in v's
+// out (in) adjacency, then r, then s.
+double nodal_vertex(const G &g, const BM &m, const int32_t *lab, int64_t v, int32_t r, int32_t s,
+                    Scratch &sc) {
   if (r == s) return 0.0;
   ++sc.stamp;
   const int64_t st = sc.stamp;
@@ -458,78 +464,65 @@ double nodal_vertex(const G &g, const BM &m, const std::vector<double> &OUT, con
   }
   const int64_t c_r = C(r), c_s = C(s);
   const int64_t kout = g.kout[v], kin = g.kin[v];
+  const double l_dr = clog2((double)(m.dout[r] - kout)), l_ds = clog2((double)(m.dout[s] + kout));
+  const double l_ir = clog2((double)(m.din[r] - kin)), l_is = clog2((double)(m.din[s] + kin));
 
-  // new row r: R_t = M_rt - a_t - [t=r]c_r + [t=s]c_r   (kernels.hpp:1444-1581)
-  double l_dr = clog2((double)(m.dout[r] - kout));
-  CSum ro;
-  bool s_in_r = false;
-  for (int64_t k = m.rowptr[r]; k < m.rowptr[r + 1]; ++k) {
-    int32_t t = m.col[k];
-    int64_t R = m.val[k] - A(t) - (t == r ? c_r : 0) + (t == s ? c_r : 0);
-    if (t == s) s_in_r = true;
-    if (R > 0) ro.add(k - m.rowptr[r], term(R, m.ldin[t], l_dr));
+  // rows r and s: old terms minus new terms of the touched columns t
+  //   new row r: R_t = M_rt - a_t - [t=r]c_r + [t=s]c_r        (kernels.hpp:1444-1581)
+  //   new row s: S_t = M_st + a_t - [t=r]c_s + [t=s]c_s        (kernels.hpp:1584-1726)
+  CSum drow;
+  int64_t sum_r = 0, sum_s = 0;
+  auto row_slot = [&](int64_t k, int32_t t) {
+    const int64_t Mr = m.row_get(r, t), Ms = m.row_get(s, t), a = A(t);
+    const double or_ = Mr > 0 ? term(Mr, m.ldout[r], m.ldin[t]) : 0.0;
+    const double os_ = Ms > 0 ? term(Ms, m.ldout[s], m.ldin[t]) : 0.0;
+    const int64_t R = Mr - a - (t == r ? c_r : 0) + (t == s ? c_r : 0);
+    const int64_t S = Ms + a - (t == r ? c_s : 0) + (t == s ? c_s : 0);
+    const double nr = R > 0 ? term(R, m.ldin[t], l_dr) : 0.0;
+    const double ns = S > 0 ? term(S, m.ldin[t], l_ds) : 0.0;
+    drow.add(k, ((or_ + os_) - nr) - ns);
+    sum_r += Mr;
+    sum_s += Ms;
+  };
+  const int64_t no = o1 - o0, ni = i1 - i0;
+  for (int64_t j = 0; j < no; ++j) {
+    int32_t t = lab[g.oadj[o0 + j]];
+    if (sc.fa[t] == j) row_slot(j, t);
   }
-  double nro = ro.result();
-  if (!s_in_r && c_r > 0) nro = nro + term(c_r, m.ldin[s], l_dr);
+  if (A(r) == 0) row_slot(no, r);
+  if (A(s) == 0) row_slot(no + 1, s);
+  const int64_t W_r = m.dout[r] - sum_r, W_s = m.dout[s] - sum_s;
+  const double Y_r = W_r > 0 ? (m.ldout[r] - l_dr) * (double)W_r : 0.0;
+  const double Y_s = W_s > 0 ? (m.ldout[s] - l_ds) * (double)W_s : 0.0;
 
-  // new row s: S_t = M_st + a_t - [t=r]c_s + [t=s]c_s   (kernels.hpp:1584-1726, Q13 fixed)
-  double l_ds = clog2((double)(m.dout[s] + kout));
-  CSum sa, sb;
-  bool found = false;
-  for (int64_t k = m.rowptr[s]; k < m.rowptr[s + 1]; ++k) {
-    int32_t t = m.col[k];
-    int64_t S = m.val[k] + A(t) - (t == r ? c_s : 0) + (t == s ? c_s : 0);
-    if (t == s) found = true;
-    if (S > 0) sa.add(k - m.rowptr[s], term(S, m.ldin[t], l_ds));
+  // columns r and s, t not in {r, s}
+  //   new column r: M_tr - c_t   (kernels.hpp:1153-1250);  new column s: M_ts + c_t (1253-1362)
+  CSum dcol;
+  int64_t csum_r = 0, csum_s = 0;
+  for (int64_t j = 0; j < ni; ++j) {
+    int32_t t = lab[g.iadj[i0 + j]];
+    if (sc.fc[t] != j || t == r || t == s) continue;
+    const int64_t Mr = m.col_get(r, t), Ms = m.col_get(s, t), c = C(t);
+    const double ir = Mr > 0 ? term(Mr, m.ldout[t], m.ldin[r]) : 0.0;
+    const double is = Ms > 0 ? term(Ms, m.ldout[t], m.ldin[s]) : 0.0;
+    const int64_t wr = Mr - c, ws = Ms + c;
+    const double nr = wr > 0 ? term(wr, m.ldout[t], l_ir) : 0.0;
+    const double ns = ws > 0 ? term(ws, m.ldout[t], l_is) : 0.0;
+    dcol.add(j, ((ir + is) - nr) - ns);
+    csum_r += Mr;
+    csum_s += Ms;
   }
-  for (int64_t j = o0; j < o1; ++j) {
-    int32_t t = lab[g.oadj[j]];
-    if (sc.fa[t] != j - o0) continue;  // first occurrence of block t only
-    if (m.row_get(s, t) != 0) continue;
-    if (t == s) found = true;
-    int64_t S = A(t) - (t == r ? c_s : 0) + (t == s ? c_s : 0);
-    if (S > 0) sb.add(j - o0, term(S, m.ldin[t], l_ds));
-  }
-  double nso = sa.result() + sb.result();
-  if (!found && c_s > 0) nso = nso + term(c_s, m.ldin[s], l_ds);
+  const int64_t W2_r = m.din[r] - m.col_get(r, r) - m.col_get(r, s) - csum_r;
+  const int64_t W2_s = m.din[s] - m.col_get(s, r) - m.col_get(s, s) - csum_s;
+  const double Y2_r = W2_r > 0 ? (m.ldin[r] - l_ir) * (double)W2_r : 0.0;
+  const double Y2_s = W2_s > 0 ? (m.ldin[s] - l_is) * (double)W2_s : 0.0;
 
-  // old and new column r, t not in {r, s}   (kernels.hpp:494-544, 1153-1250)
-  double l_ir = clog2((double)(m.din[r] - kin));
-  CSum inr, nri;
-  for (int64_t k = m.colptr[r]; k < m.colptr[r + 1]; ++k) {
-    int32_t t = m.row[k];
-    if (t == r || t == s) continue;
-    inr.add(k - m.colptr[r], term(m.cval[k], m.ldout[t], m.ldin[r]));
-    int64_t w = m.cval[k] - C(t);
-    if (w > 0) nri.add(k - m.colptr[r], term(w, m.ldout[t], l_ir));
-  }
-  // old and new column s   (kernels.hpp:546-597, 1253-1362, Q12/Q13 fixed)
-  double l_is = clog2((double)(m.din[s] + kin));
-  CSum ins, nsa, nsb;
-  for (int64_t k = m.colptr[s]; k < m.colptr[s + 1]; ++k) {
-    int32_t t = m.row[k];
-    if (t == r || t == s) continue;
-    ins.add(k - m.colptr[s], term(m.cval[k], m.ldout[t], m.ldin[s]));
-    int64_t w = m.cval[k] + C(t);
-    if (w > 0) nsa.add(k - m.colptr[s], term(w, m.ldout[t], l_is));
-  }
-  for (int64_t j = i0; j < i1; ++j) {
-    int32_t t = lab[g.iadj[j]];
-    if (sc.fc[t] != j - i0) continue;
-    if (t == r || t == s) continue;
-    if (m.col_get(s, t) != 0) continue;
-    int64_t w = C(t);
-    if (w > 0) nsb.add(j - i0, term(w, m.ldout[t], l_is));
-  }
-  double nsi = nsa.result() + nsb.result();
-
-  // calculate_dS_overall_nodal (kernels.hpp:600-643)
-  double d = OUT[r] + OUT[s];
-  d = d + (inr.result() + ins.result());
-  d = d - nro;
-  d = d - nso;
-  d = d - nri.result();
-  d = d - nsi;
+  // calculate_dS_overall_nodal (kernels.hpp:600-643): accept iff dS < 0
+  double d = drow.result() - Y_r;
+  d = d - Y_s;
+  d = d + dcol.result();
+  d = d - Y2_r;
+  d = d - Y2_s;
   return d;
 }
 
@@ -537,7 +530,6 @@ void nodal_batch(const G &g, const orc_options &o, const Rng &rng, int32_t *lab,
                  int32_t b, uint32_t phase, int32_t *prop, double *dS, double *dE, Scratch &sc) {
   const int64_t N = g.N, BS = N / o.num_batches, lo = (int64_t)b * BS;
   const int32_t B = m.B;
-  std::vector<double> OUT = row_terms(m);
   // per-block neighbour sample and coin, shared by all vertices (Q5, partition.hpp:485-523)
   std::vector<int32_t> sof(B);
   std::vector<uint8_t> coin(B);
@@ -574,7 +566,7 @@ void nodal_batch(const G &g, const orc_options &o, const Rng &rng, int32_t *lab,
       s = coin[u] ? (int32_t)mulhi(x[0], B) : sof[u];
     }
     ps[i] = s;
-    double d = nodal_vertex(g, m, OUT, lab, v, lab[v], s, sc);
+    double d = nodal_vertex(g, m, lab, v, lab[v], s, sc);
     ds[i] = d < 0.0 ? d : 0.0;
   }
   for (int64_t i = 0; i < BS; ++i)
