@@ -74,7 +74,19 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 512 MiB device copy (PMC byte-counter calibration)")
     a = ap.parse_args()
+    if a.calibrate_copy:
+        import torch
+        x = torch.ones(128 << 20, dtype=torch.int32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4,
+                          "launches": 3}))
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU: this build partitions one graph per GPU and has no "
               f"multi-GPU path (asked for --gpus {a.gpus})", file=sys.stderr)

@@ -20,35 +20,67 @@ __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
     p[i] = v;
 }
 
-__global__ void k_hist_lds(const int32_t *esrc, const int32_t *oadj, const int32_t *ow,
-                           const int32_t *lab, int64_t E, int32_t B, int32_t *partials) {
-  extern __shared__ int32_t h[];
-  const int32_t BB = B * B;
-  for (int32_t i = threadIdx.x; i < BB; i += blockDim.x) h[i] = 0;
+// ---- vertices grouped by block (counting sort; order inside a block is irrelevant)
+constexpr int VT = 2048;  // vertices per workgroup
+
+__global__ __launch_bounds__(256) void k_label_hist(const int32_t *lab, int64_t N, int32_t B,
+                                                    uint32_t *H, int32_t W) {
+  extern __shared__ uint32_t cnt[];
+  for (int32_t b = threadIdx.x; b < B; b += blockDim.x) cnt[b] = 0;
   __syncthreads();
-  const int64_t chunk = (E + gridDim.x - 1) / gridDim.x;
-  const int64_t e0 = (int64_t)blockIdx.x * chunk;
-  const int64_t e1 = e0 + chunk < E ? e0 + chunk : E;
-  for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x)
-    atomicAdd(&h[lab[esrc[e]] * B + lab[oadj[e]]], ow[e]);
+  const int64_t v0 = (int64_t)blockIdx.x * VT;
+  for (int64_t v = v0 + threadIdx.x; v < v0 + VT && v < N; v += blockDim.x)
+    atomicAdd(&cnt[lab[v]], 1u);
   __syncthreads();
-  int32_t *out = partials + (int64_t)blockIdx.x * BB;
-  for (int32_t i = threadIdx.x; i < BB; i += blockDim.x) out[i] = h[i];
+  for (int32_t b = threadIdx.x; b < B; b += blockDim.x) H[(int64_t)b * W + blockIdx.x] = cnt[b];
 }
 
-__global__ void k_hist_reduce(const int32_t *partials, int32_t nparts, int32_t BB, int32_t *D) {
-  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_label_scatter(const int32_t *lab, int64_t N, int32_t B,
+                                                       const uint32_t *H, int32_t W,
+                                                       int32_t *vperm) {
+  extern __shared__ uint32_t cur[];
+  for (int32_t b = threadIdx.x; b < B; b += blockDim.x) cur[b] = H[(int64_t)b * W + blockIdx.x];
+  __syncthreads();
+  const int64_t v0 = (int64_t)blockIdx.x * VT;
+  for (int64_t v = v0 + threadIdx.x; v < v0 + VT && v < N; v += blockDim.x)
+    vperm[atomicAdd(&cur[lab[v]], 1u)] = (int32_t)v;
+}
+
+// ---- M by row tiles in LDS: workgroup (tile, chunk) owns rows [r0, r0+R) and one chunk of
+// those rows' vertices; each wave takes a vertex and its lanes add its out-edges into the
+// LDS tile; the tile goes out as a plane of partial counts (no global atomics).
+__global__ __launch_bounds__(512) void k_tile_hist(const int32_t *optr, const int32_t *oadj,
+                                                   const int32_t *ow, const int32_t *lab,
+                                                   const int32_t *vperm, const uint32_t *H,
+                                                   int32_t W, int64_t N, int32_t B, int32_t R,
+                                                   int32_t C, int32_t *partial) {
+  extern __shared__ int32_t T[];
+  const int32_t r0 = blockIdx.x * R;
+  const int32_t r1 = r0 + R < B ? r0 + R : B;
+  const int32_t nt = (r1 - r0) * B;
+  for (int32_t i = threadIdx.x; i < nt; i += blockDim.x) T[i] = 0;
+  __syncthreads();
+  const int64_t vs = H[(int64_t)r0 * W];
+  const int64_t ve = r1 < B ? (int64_t)H[(int64_t)r1 * W] : N;
+  const int64_t c = blockIdx.y;
+  const int64_t a0 = vs + (ve - vs) * c / C, a1 = vs + (ve - vs) * (c + 1) / C;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int64_t k = a0 + w; k < a1; k += nw) {
+    const int32_t v = vperm[k];
+    int32_t *row = T + (lab[v] - r0) * B;
+    for (int32_t j = optr[v] + lane; j < optr[v + 1]; j += 64) atomicAdd(&row[lab[oadj[j]]], ow[j]);
+  }
+  __syncthreads();
+  int32_t *out = partial + c * (int64_t)B * B + (int64_t)r0 * B;
+  for (int32_t i = threadIdx.x; i < nt; i += blockDim.x) out[i] = T[i];
+}
+
+__global__ void k_hist_reduce(const int32_t *partials, int32_t nparts, int64_t BB, int32_t *D) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= BB) return;
   int32_t s = 0;
   for (int32_t p = 0; p < nparts; ++p) s += partials[(int64_t)p * BB + i];
   D[i] = s;
-}
-
-__global__ void k_hist_global(const int32_t *esrc, const int32_t *oadj, const int32_t *ow,
-                              const int32_t *lab, int64_t E, int32_t B, int32_t *D) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
-       e += (int64_t)gridDim.x * blockDim.x)
-    atomicAdd(&D[(int64_t)lab[esrc[e]] * B + lab[oadj[e]]], ow[e]);
 }
 
 // one wave per row: nonzero count and row sum
@@ -258,19 +290,25 @@ static void sparse_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf 
 
 static void dense_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm,
                        RebuildScratch &rs, hipStream_t st) {
-  const int64_t E = g.E;
-  const int32_t BB = B * B;
+  const int64_t N = g.N;
+  const int64_t BB = (int64_t)B * B;
   const int T = 256;
   int32_t *D = bm.dense;
-  const int nparts = 256;
-  if ((int64_t)BB * 4 <= 65536 && (int64_t)nparts * BB <= rs.partials_cap) {
-    k_hist_lds<<<nparts, 512, (size_t)BB * 4, st>>>(g.esrc, g.oadj, g.ow, lab, E, B,
-                                                     rs.partials);
-    k_hist_reduce<<<grid_for(BB, T), T, 0, st>>>(rs.partials, nparts, BB, D);
-  } else {
-    k_fill_i32<<<grid_for(BB, T), T, 0, st>>>(D, BB, 0);
-    k_hist_global<<<grid_for(E, T, 8192), T, 0, st>>>(g.esrc, g.oadj, g.ow, lab, E, B, D);
-  }
+  // vertices grouped by block
+  const int32_t W = (int32_t)((N + VT - 1) / VT);
+  k_label_hist<<<W, 256, (size_t)B * 4, st>>>(lab, N, B, rs.vhist, W);
+  scan_exclusive(rs.vhist, rs.vhist, (int64_t)B * W, nullptr, rs.tmp, st);
+  k_label_scatter<<<W, 256, (size_t)B * 4, st>>>(lab, N, B, rs.vhist, W, rs.vperm);
+  // row tiles of at most 16384 counters (64 KB of LDS), chunks so that ~2048 workgroups run
+  const int32_t R = B >= 16384 ? 1 : 16384 / B;
+  const int32_t ntiles = (B + R - 1) / R;
+  int32_t C = 2048 / ntiles;
+  if ((int64_t)C * BB > ((int64_t)8 << 20)) C = (int32_t)(((int64_t)8 << 20) / BB);
+  if ((int64_t)C * BB > rs.partials_cap) C = (int32_t)(rs.partials_cap / BB);
+  if (C < 1) C = 1;
+  k_tile_hist<<<dim3(ntiles, C), 512, (size_t)R * B * 4, st>>>(
+      g.optr, g.oadj, g.ow, lab, rs.vperm, rs.vhist, W, N, B, R, C, rs.partials);
+  k_hist_reduce<<<grid_for(BB, T), T, 0, st>>>(rs.partials, C, BB, D);
   k_dense_rows<<<B, 64, 0, st>>>(D, B, rs.cnt, bm.dout);
   scan_exclusive(rs.cnt, (uint32_t *)bm.rowptr, B, (uint32_t *)(bm.rowptr + B), rs.tmp, st);
   k_dense_write_rows<<<B, 64, 0, st>>>(D, B, bm.rowptr, bm.col, bm.val, bm.erow);
@@ -287,7 +325,7 @@ void rebuild_blockmodel(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf 
                         RebuildScratch &rs, int32_t dense_max_B, hipStream_t st) {
   bm.B = B;
   const bool dense = B <= dense_max_B && (int64_t)B * B <= bm.dense_cap &&
-                     (int64_t)B * B <= rs.partials_cap;
+                     (int64_t)B * B <= rs.partials_cap && B <= rs.vhist_B;
   if (dense) {
     dense_path(g, lab, B, bm, rs, st);
   } else if ((uint64_t)B * (uint64_t)B <= 0xFFFFFFFFull) {

@@ -47,8 +47,11 @@ struct RebuildScratch {
   uint32_t *flags = nullptr, *uidx = nullptr;  // E
   uint32_t *tmp = nullptr;                  // radix / scan temporary
   uint32_t *cnt = nullptr;                  // Bmax + 1
-  int32_t *partials = nullptr;              // LDS-histogram partials
-  int64_t partials_cap = 0;
+  int32_t *partials = nullptr;              // partial planes of the tiled histogram
+  int64_t partials_cap = 0;                 // words
+  uint32_t *vhist = nullptr;                // per-(block, vertex tile) counts, block-major
+  int32_t *vperm = nullptr;                 // vertices grouped by block
+  int32_t vhist_B = 0;                      // largest B the vhist buffer holds
 };
 
 // ---- bm.hip

@@ -57,7 +57,7 @@ struct DBuf {
   size_t cap = 0;
   void ensure(size_t bytes) {
     if (bytes <= cap) return;
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
     if (bytes == 0) return;
@@ -68,7 +68,7 @@ struct DBuf {
     cap = bytes;
   }
   void release() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
   }
@@ -129,7 +129,7 @@ struct Prof {
   }
   void destroy() {
     flush();
-    for (hipEvent_t e : pool) hipEventDestroy(e);
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
     pool.clear();
   }
 };
@@ -147,7 +147,7 @@ struct gsap_ctx {
   // block matrix
   DBuf rowptr, colptr, col, val, erow, row, cval, dout, din, ldout, ldin, dense;
   // rebuild scratch
-  DBuf k0, k1, v0, v1, flags, uidx, tmp, cnt, partials;
+  DBuf k0, k1, v0, v1, flags, uidx, tmp, cnt, partials, vhist, vperm;
   // sweep / merge
   DBuf OUT, sof, coin, prop, dS, best, dSbest, map, terms, tiles, res;
   GraphDev g;
@@ -252,11 +252,15 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   c->dense.ensure(4 * (size_t)std::max<int64_t>(dB * dB, 1));
   c->k0.ensure(8 * E); c->k1.ensure(8 * E);
   c->v0.ensure(4 * E); c->v1.ensure(4 * E); c->flags.ensure(4 * E); c->uidx.ensure(4 * E);
-  size_t tw = std::max(radix_temp_words(E), scan_temp_words(N + 1));
+  const int64_t W = (N + 2047) / 2048;
+  size_t tw = std::max(radix_temp_words(E), scan_temp_words(std::max<int64_t>(N + 1, dB * W)));
   c->tmp.ensure(4 * tw);
   c->cnt.ensure(4 * (N + 1));
-  const int64_t pc = std::max<int64_t>(256 * 128 * 128, dB * dB);
+  // partial planes: up to 2048 / ceil(B / (16384 / B)) planes of B*B counters
+  const int64_t pc = std::max<int64_t>(4 * dB * dB, 2048 * 16384);
   c->partials.ensure(4 * pc);
+  c->vhist.ensure(4 * (size_t)std::max<int64_t>(dB * W, 1));
+  c->vperm.ensure(4 * N);
   c->OUT.ensure(8 * N); c->sof.ensure(4 * N); c->coin.ensure(N);
   c->prop.ensure(4 * (size_t)np * N); c->dS.ensure(8 * (size_t)np * N);
   c->best.ensure(4 * N); c->dSbest.ensure(8 * N); c->map.ensure(4 * N);
@@ -273,6 +277,7 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   r.k0 = c->k0.p; r.k1 = c->k1.p; r.v0 = c->v0.as<uint32_t>(); r.v1 = c->v1.as<uint32_t>();
   r.flags = c->flags.as<uint32_t>(); r.uidx = c->uidx.as<uint32_t>(); r.tmp = c->tmp.as<uint32_t>();
   r.cnt = c->cnt.as<uint32_t>(); r.partials = c->partials.as<int32_t>(); r.partials_cap = pc;
+  r.vhist = c->vhist.as<uint32_t>(); r.vperm = c->vperm.as<int32_t>(); r.vhist_B = (int32_t)dB;
 }
 
 int32_t resolve_dense(const gsap_options &o) { return o.dense_max_B > 0 ? o.dense_max_B : 1024; }
@@ -659,18 +664,19 @@ int gsap_create(gsap_ctx **ctx, int device) {
 
 int gsap_destroy(gsap_ctx *c) {
   if (!c) return GSAP_OK;
-  hipSetDevice(c->device);
-  hipStreamSynchronize(c->st);
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->st);
   DBuf *all[] = {&c->optr, &c->oadj, &c->ow, &c->iptr, &c->iadj, &c->iw, &c->kout, &c->kin,
                  &c->esrc, &c->hoff, &c->hbo, &c->hbi, &c->hpool, &c->lab, &c->snap[0],
                  &c->snap[1], &c->snap[2], &c->rowptr, &c->colptr, &c->col, &c->val, &c->erow,
                  &c->row, &c->cval, &c->dout, &c->din, &c->ldout, &c->ldin, &c->dense, &c->k0,
                  &c->k1, &c->v0, &c->v1, &c->flags, &c->uidx, &c->tmp, &c->cnt, &c->partials,
+                 &c->vhist, &c->vperm,
                  &c->OUT, &c->sof, &c->coin, &c->prop, &c->dS, &c->best, &c->dSbest, &c->map,
                  &c->terms, &c->tiles, &c->res};
   for (DBuf *b : all) b->release();
   try { c->prof.destroy(); } catch (...) {}
-  hipStreamDestroy(c->st);
+  (void)hipStreamDestroy(c->st);
   delete c;
   return GSAP_OK;
 }

@@ -138,10 +138,12 @@ __global__ __launch_bounds__(256) void k_block_terms(BMView m, double *OUT, int3
 void launch_block_terms(const BMView &m, double *OUT, int32_t *sof, uint8_t *coin, RngKey key,
                         uint32_t phase, bool sample, int32_t avg_row, hipStream_t st) {
   const int s = sample ? 1 : 0;
-  if (avg_row <= 8)
+  if (avg_row <= 12)
     k_block_terms<8><<<grid_for(m.B, 32), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s);
-  else if (avg_row <= 16)
+  else if (avg_row <= 24)
     k_block_terms<16><<<grid_for(m.B, 16), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s);
+  else if (avg_row <= 48)
+    k_block_terms<32><<<grid_for(m.B, 8), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s);
   else
     k_block_terms<64><<<grid_for(m.B, 4), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s);
 }
@@ -482,10 +484,12 @@ __global__ __launch_bounds__(256) void k_merge(BMView m, const double *OUT, RngK
 void launch_merge(const BMView &m, const double *OUT, RngKey key, uint32_t phase, int32_t np,
                   int32_t *prop, double *dS, int32_t avg_row, hipStream_t st) {
   const int64_t n = (int64_t)m.B * np;
-  if (avg_row <= 8)
+  if (avg_row <= 12)
     k_merge<8><<<grid_for(n, 32), 256, 0, st>>>(m, OUT, key, phase, np, prop, dS);
-  else if (avg_row <= 16)
+  else if (avg_row <= 24)
     k_merge<16><<<grid_for(n, 16), 256, 0, st>>>(m, OUT, key, phase, np, prop, dS);
+  else if (avg_row <= 48)
+    k_merge<32><<<grid_for(n, 8), 256, 0, st>>>(m, OUT, key, phase, np, prop, dS);
   else
     k_merge<64><<<grid_for(n, 4), 256, 0, st>>>(m, OUT, key, phase, np, prop, dS);
 }
