@@ -64,23 +64,31 @@ __global__ __launch_bounds__(512) void k_tile_hist(const int32_t *optr, const in
   const int64_t ve = r1 < B ? (int64_t)H[(int64_t)r1 * W] : N;
   const int64_t c = blockIdx.y;
   const int64_t a0 = vs + (ve - vs) * c / C, a1 = vs + (ve - vs) * (c + 1) / C;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int64_t k = a0 + w; k < a1; k += nw) {
+  // a group of 16 lanes per vertex: 4 vertices in flight per wave
+  const int q = threadIdx.x & 15, grp = threadIdx.x >> 4, ng = blockDim.x >> 4;
+  for (int64_t k = a0 + grp; k < a1; k += ng) {
     const int32_t v = vperm[k];
     int32_t *row = T + (lab[v] - r0) * B;
-    for (int32_t j = optr[v] + lane; j < optr[v + 1]; j += 64) atomicAdd(&row[lab[oadj[j]]], ow[j]);
+    for (int32_t j = optr[v] + q; j < optr[v + 1]; j += 16) atomicAdd(&row[lab[oadj[j]]], ow[j]);
   }
   __syncthreads();
   int32_t *out = partial + c * (int64_t)B * B + (int64_t)r0 * B;
   for (int32_t i = threadIdx.x; i < nt; i += blockDim.x) out[i] = T[i];
 }
 
-__global__ void k_hist_reduce(const int32_t *partials, int32_t nparts, int64_t BB, int32_t *D) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// D[i] = sum over the C partial planes; a group of L lanes per counter (L = 1 when C is
+// small, up to 64 when a few counters have many planes)
+template <int L>
+__global__ __launch_bounds__(256) void k_hist_reduce(const int32_t *partials, int32_t nparts,
+                                                     int64_t BB, int32_t *D) {
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / L;
+  const int q = threadIdx.x & (L - 1);
   if (i >= BB) return;
   int32_t s = 0;
-  for (int32_t p = 0; p < nparts; ++p) s += partials[(int64_t)p * BB + i];
-  D[i] = s;
+  for (int32_t p = q; p < nparts; p += L) s += partials[(int64_t)p * BB + i];
+#pragma unroll
+  for (int off = L / 2; off >= 1; off >>= 1) s += __shfl_down(s, off, L);
+  if (q == 0) D[i] = s;
 }
 
 // one wave per row: nonzero count and row sum
@@ -306,9 +314,16 @@ static void dense_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &
   if ((int64_t)C * BB > ((int64_t)8 << 20)) C = (int32_t)(((int64_t)8 << 20) / BB);
   if ((int64_t)C * BB > rs.partials_cap) C = (int32_t)(rs.partials_cap / BB);
   if (C < 1) C = 1;
+  // a single chunk writes its tiles straight into D
   k_tile_hist<<<dim3(ntiles, C), 512, (size_t)R * B * 4, st>>>(
-      g.optr, g.oadj, g.ow, lab, rs.vperm, rs.vhist, W, N, B, R, C, rs.partials);
-  k_hist_reduce<<<grid_for(BB, T), T, 0, st>>>(rs.partials, C, BB, D);
+      g.optr, g.oadj, g.ow, lab, rs.vperm, rs.vhist, W, N, B, R, C, C == 1 ? D : rs.partials);
+  if (C == 1) {
+  } else if (C <= 4)
+    k_hist_reduce<1><<<grid_for(BB, T), T, 0, st>>>(rs.partials, C, BB, D);
+  else if (C <= 32)
+    k_hist_reduce<8><<<grid_for(BB * 8, T), T, 0, st>>>(rs.partials, C, BB, D);
+  else
+    k_hist_reduce<64><<<grid_for(BB * 64, T), T, 0, st>>>(rs.partials, C, BB, D);
   k_dense_rows<<<B, 64, 0, st>>>(D, B, rs.cnt, bm.dout);
   scan_exclusive(rs.cnt, (uint32_t *)bm.rowptr, B, (uint32_t *)(bm.rowptr + B), rs.tmp, st);
   k_dense_write_rows<<<B, 64, 0, st>>>(D, B, bm.rowptr, bm.col, bm.val, bm.erow);

@@ -120,8 +120,6 @@ struct CAcc {
   GSAP_D double reduce() {
 #pragma unroll
     for (int off = 32; off >= G; off >>= 1) {
-      constexpr int dummy = 0;
-      (void)dummy;
       const int d = off / G;
 #pragma unroll
       for (int j = 0; j < S; ++j)
@@ -224,6 +222,54 @@ GSAP_D int32_t bm_col_get(const BMView &m, int32_t s, int32_t t) {
     if (m.row[mid] < t) lo = mid + 1; else hi = mid;
   }
   return (lo < m.colptr[s + 1] && m.row[lo] == t) ? m.cval[lo] : 0;
+}
+
+// M[r][t] and M[s][t] together (independent probes interleaved)
+GSAP_D void bm_row_get2(const BMView &m, int32_t r, int32_t s, int32_t t, int32_t &Mr,
+                        int32_t &Ms) {
+  if (m.dense) {
+    Mr = m.dense[(int64_t)r * m.B + t];
+    Ms = m.dense[(int64_t)s * m.B + t];
+    return;
+  }
+  int32_t lo1 = m.rowptr[r], hi1 = m.rowptr[r + 1], e1 = hi1;
+  int32_t lo2 = m.rowptr[s], hi2 = m.rowptr[s + 1], e2 = hi2;
+  while (lo1 < hi1 || lo2 < hi2) {
+    if (lo1 < hi1) {
+      int32_t mid = (lo1 + hi1) >> 1;
+      if (m.col[mid] < t) lo1 = mid + 1; else hi1 = mid;
+    }
+    if (lo2 < hi2) {
+      int32_t mid = (lo2 + hi2) >> 1;
+      if (m.col[mid] < t) lo2 = mid + 1; else hi2 = mid;
+    }
+  }
+  Mr = (lo1 < e1 && m.col[lo1] == t) ? m.val[lo1] : 0;
+  Ms = (lo2 < e2 && m.col[lo2] == t) ? m.val[lo2] : 0;
+}
+
+// M[t][r] and M[t][s] together via columns r and s
+GSAP_D void bm_col_get2(const BMView &m, int32_t r, int32_t s, int32_t t, int32_t &Mr,
+                        int32_t &Ms) {
+  if (m.dense) {
+    Mr = m.dense[(int64_t)t * m.B + r];
+    Ms = m.dense[(int64_t)t * m.B + s];
+    return;
+  }
+  int32_t lo1 = m.colptr[r], hi1 = m.colptr[r + 1], e1 = hi1;
+  int32_t lo2 = m.colptr[s], hi2 = m.colptr[s + 1], e2 = hi2;
+  while (lo1 < hi1 || lo2 < hi2) {
+    if (lo1 < hi1) {
+      int32_t mid = (lo1 + hi1) >> 1;
+      if (m.row[mid] < t) lo1 = mid + 1; else hi1 = mid;
+    }
+    if (lo2 < hi2) {
+      int32_t mid = (lo2 + hi2) >> 1;
+      if (m.row[mid] < t) lo2 = mid + 1; else hi2 = mid;
+    }
+  }
+  Mr = (lo1 < e1 && m.row[lo1] == t) ? m.cval[lo1] : 0;
+  Ms = (lo2 < e2 && m.row[lo2] == t) ? m.cval[lo2] : 0;
 }
 
 // Weighted pick over row x then column x: first entry whose running weight

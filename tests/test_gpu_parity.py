@@ -21,10 +21,10 @@ def ctx():
     c.close()
 
 
-def hub_graph(seed=5):
-    """Random graph plus two hubs with > 128 in/out neighbours (global hash tables)."""
+def hub_graph(seed=5, N=600, hub_deg=300):
+    """Random graph plus two hubs with many in/out neighbours (large LDS tables; beyond 1024
+    neighbours, the global hash-table pool)."""
     rng = np.random.default_rng(seed)
-    N = 600
     src, dst = [], []
     seen = set()
     def add(a, b):
@@ -33,7 +33,7 @@ def hub_graph(seed=5):
     for _ in range(3000):
         add(int(rng.integers(N)), int(rng.integers(N)))
     for h in (3, 250):
-        for t in rng.choice(N, 300, replace=False):
+        for t in rng.choice(N, hub_deg, replace=False):
             add(h, int(t)); add(int(t), h)
     return gsap_amd.Graph.from_edges(N, src, dst)
 
@@ -42,6 +42,7 @@ CASES = {
     "tiny_w": lambda: random_graph(64, 400, 1, wmax=4),
     "rand": lambda: random_graph(700, 6000, 2),
     "hubs": hub_graph,
+    "bighubs": lambda: hub_graph(6, N=4000, hub_deg=1500),
     "LL1k": lambda: tsv_graph("LL"),
     "HH5k": lambda: tsv_graph("HH", "5k"),
 }
