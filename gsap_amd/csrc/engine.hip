@@ -150,6 +150,11 @@ struct gsap_ctx {
   DBuf k0, k1, v0, v1, flags, uidx, tmp, cnt, partials, vhist, vperm;
   // sweep / merge
   DBuf OUT, sof, coin, prop, dS, best, dSbest, map, terms, tiles, res;
+  // per batch, the batch's vertices split by neighbour count (<= 16, <= 32, <= 128, larger)
+  DBuf vlist;
+  std::vector<int64_t> list_off;  // [batch * 4 + class] -> offset into vlist; last = total
+  std::vector<uint32_t> big_bits; // per batch: LDS table bits of the largest class
+  int32_t lists_nb = 0;
   GraphDev g;
   BMBuf bm;
   RebuildScratch rs;
@@ -238,6 +243,38 @@ void upload_graph(gsap_ctx *c, const gsap_graph_i32 *g) {
   d.hpool = c->hpool.as<int32_t>();
 }
 
+// Vertex lists of every nodal batch by class of max(out, in) neighbour count: <= 16, <= 32,
+// <= 128, larger.  The last class gets LDS tables sized for its largest vertex (at most 2048
+// slots; beyond that the kernel uses the global pool).
+void build_batch_lists(gsap_ctx *c, const gsap_graph_i32 *g, int32_t nb) {
+  const int64_t N = g->N, BS = N / nb;
+  std::vector<int32_t> lst;
+  lst.reserve((size_t)(BS * nb));
+  c->list_off.assign((size_t)nb * 4 + 1, 0);
+  c->big_bits.assign((size_t)nb, 8);
+  for (int32_t b = 0; b < nb; ++b)
+    for (int k = 0; k < 4; ++k) {
+      c->list_off[(size_t)b * 4 + k] = (int64_t)lst.size();
+      for (int64_t v = (int64_t)b * BS; v < (int64_t)(b + 1) * BS; ++v) {
+        const int64_t d = std::max<int64_t>(g->out_ptr[v + 1] - g->out_ptr[v], g->in_ptr[v + 1] - g->in_ptr[v]);
+        const int cls = d <= 16 ? 0 : (d <= 32 ? 1 : (d <= 128 ? 2 : 3));
+        if (cls != k) continue;
+        lst.push_back((int32_t)v);
+        if (cls == 3) {
+          uint32_t bits = 8;
+          while (bits < 11 && ((int64_t)1 << bits) < 2 * d) ++bits;
+          c->big_bits[b] = std::max(c->big_bits[b], bits);
+        }
+      }
+    }
+  c->list_off[(size_t)nb * 4] = (int64_t)lst.size();
+  c->vlist.ensure(4 * std::max<size_t>(lst.size(), 1));
+  if (!lst.empty())
+    HIPX(hipMemcpyAsync(c->vlist.p, lst.data(), 4 * lst.size(), hipMemcpyHostToDevice, c->st));
+  HIPX(hipStreamSynchronize(c->st));
+  c->lists_nb = nb;
+}
+
 void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   const int64_t N = c->N, E = std::max<int64_t>(c->E, 1);
   c->dense_max_B = dense_max_B;
@@ -280,7 +317,7 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   r.vhist = c->vhist.as<uint32_t>(); r.vperm = c->vperm.as<int32_t>(); r.vhist_B = (int32_t)dB;
 }
 
-int32_t resolve_dense(const gsap_options &o) { return o.dense_max_B > 0 ? o.dense_max_B : 1024; }
+int32_t resolve_dense(const gsap_options &o) { return o.dense_max_B > 0 ? o.dense_max_B : 8192; }
 
 RngKey rng_key(const gsap_options &o) {
   return RngKey{(uint32_t)o.seed, (uint32_t)(o.seed >> 32), o.chain};
@@ -408,7 +445,15 @@ void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, uint32_t phase, 
                      true, ar, st);
   c->prof.end(p, st);
   p = c->prof.begin(0, st);
-  launch_nodal(c->g, c->lab.as<int32_t>(), m, c->sof.as<int32_t>(), c->coin.as<uint8_t>(), rng_key(o), phase, lo, BS, c->prop.as<int32_t>(),
+  const int32_t *lists[4];
+  int32_t counts[4];
+  for (int k = 0; k < 4; ++k) {
+    const int64_t off = c->list_off[(size_t)b * 4 + k], end = c->list_off[(size_t)b * 4 + k + 1];
+    lists[k] = c->vlist.as<int32_t>() + off;
+    counts[k] = (int32_t)(end - off);
+  }
+  launch_nodal(c->g, c->lab.as<int32_t>(), m, c->sof.as<int32_t>(), c->coin.as<uint8_t>(),
+               rng_key(o), phase, lo, BS, lists, counts, c->big_bits[b], c->prop.as<int32_t>(),
                c->dS.as<double>(), st);
   c->prof.end(p, st);
   p = c->prof.begin(5, st);
@@ -441,6 +486,7 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
   const double t0 = now();
   set_device(c);
   upload_graph(c, gin);
+  build_batch_lists(c, gin, o.num_batches);
   alloc_work(c, o.num_proposals, resolve_dense(o));
   c->trace.clear();
   c->prof.reset(o.profile != 0);
@@ -629,6 +675,7 @@ void setup_unit(gsap_ctx *c, const gsap_graph_i32 *g, const gsap_options *o, con
   check_options(oo);
   set_device(c);
   upload_graph(c, g);
+  build_batch_lists(c, g, oo.num_batches);
   alloc_work(c, oo.num_proposals, resolve_dense(oo));
   if (B <= 0 || B > c->N) throw Err(GSAP_ERR_INVALID, "B must be in [1, N]");
   for (int64_t v = 0; v < c->N; ++v)
@@ -671,7 +718,7 @@ int gsap_destroy(gsap_ctx *c) {
                  &c->snap[1], &c->snap[2], &c->rowptr, &c->colptr, &c->col, &c->val, &c->erow,
                  &c->row, &c->cval, &c->dout, &c->din, &c->ldout, &c->ldin, &c->dense, &c->k0,
                  &c->k1, &c->v0, &c->v1, &c->flags, &c->uidx, &c->tmp, &c->cnt, &c->partials,
-                 &c->vhist, &c->vperm,
+                 &c->vhist, &c->vperm, &c->vlist,
                  &c->OUT, &c->sof, &c->coin, &c->prop, &c->dS, &c->best, &c->dSbest, &c->map,
                  &c->terms, &c->tiles, &c->res};
   for (DBuf *b : all) b->release();

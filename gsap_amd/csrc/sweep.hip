@@ -171,12 +171,17 @@ struct NodalArgs {
 // run-merged).  Only the entries v touches are evaluated term by term ("slots": v's
 // out-blocks, r and s for rows r and s; v's in-blocks outside {r, s} for columns r and s);
 // the untouched entries contribute (log2 d_old - log2 d_new) * (their summed count).
-// Labels are not written here (batch-Jacobi).
-__global__ __launch_bounds__(64) void k_nodal(NodalArgs a) {
-  __shared__ int32_t lko[LCAP], lwo[LCAP], lfo[LCAP], lki[LCAP], lwi[LCAP], lfi[LCAP];
+// Labels are not written here (batch-Jacobi).  Hash tables of 2^lbits slots live in LDS;
+// a vertex whose adjacency needs more uses its tables in the global pool.
+__global__ __launch_bounds__(64, 6) void k_nodal(NodalArgs a, const int32_t *list,
+                                                 uint32_t lbits) {
+  extern __shared__ int32_t lds[];
+  const uint32_t lcap = 1u << lbits;
+  int32_t *lko = lds, *lwo = lds + lcap, *lfo = lds + 2 * lcap;
+  int32_t *lki = lds + 3 * lcap, *lwi = lds + 4 * lcap, *lfi = lds + 5 * lcap;
   const int lane = threadIdx.x;
-  const int64_t i = blockIdx.x;
-  const int64_t v = a.lo + i;
+  const int64_t v = list[blockIdx.x];
+  const int64_t i = v - a.lo;
   const BMView &m = a.m;
   const int32_t B = m.B;
   const int32_t r = a.lab[v];
@@ -203,30 +208,45 @@ __global__ __launch_bounds__(64) void k_nodal(NodalArgs a) {
   // a_t (out weight of v into block t) and c_t (in weight from block t), with the
   // first adjacency position of each block
   HTab ho, hi;
-  const uint32_t bo = a.hbits_o[v];
-  if (bo) {
-    const uint32_t bi = a.hbits_i[v];
+  if (2 * (no > ni ? no : ni) > (int32_t)lcap) {
+    const uint32_t bo = a.hbits_o[v], bi = a.hbits_i[v];
     int32_t *p = a.hpool + a.hoff[v];
     const uint32_t co = 1u << bo, ci = 1u << bi;
     ho = HTab{p, p + co, p + 2 * co, bo};
     p += 3 * co;
     hi = HTab{p, p + ci, p + 2 * ci, bi};
   } else {
-    ho = HTab{lko, lwo, lfo, (uint32_t)LBITS};
-    hi = HTab{lki, lwi, lfi, (uint32_t)LBITS};
+    ho = HTab{lko, lwo, lfo, lbits};
+    hi = HTab{lki, lwi, lfi, lbits};
   }
   ht_clear(ho, lane, 64);
   ht_clear(hi, lane, 64);
+  // first chunk of both adjacency lists: gather the neighbour blocks once, keep them
+  const int32_t t_o0 = lane < no ? a.lab[a.oadj[o0 + lane]] : -1;
+  const int32_t t_i0 = lane < ni ? a.lab[a.iadj[i0 + lane]] : -1;
+  const int32_t w_o0 = lane < no ? a.ow[o0 + lane] : 0;
+  const int32_t w_i0 = lane < ni ? a.iw[i0 + lane] : 0;
+  // block degrees of r and s, needed after the tables
+  const int64_t dout_r = m.dout[r], dout_s = m.dout[s], din_r = m.din[r], din_s = m.din[s];
+  const double ldo_r = m.ldout[r], ldo_s = m.ldout[s], ldi_r = m.ldin[r], ldi_s = m.ldin[s];
   __syncthreads();
-  for (int32_t j = lane; j < no; j += 64) ht_insert(ho, a.lab[a.oadj[o0 + j]], a.ow[o0 + j], j);
-  for (int32_t j = lane; j < ni; j += 64) ht_insert(hi, a.lab[a.iadj[i0 + j]], a.iw[i0 + j], j);
+  if (lane < no) ht_insert(ho, t_o0, w_o0, lane);
+  if (lane < ni) ht_insert(hi, t_i0, w_i0, lane);
+  for (int32_t j = lane + 64; j < no; j += 64) ht_insert(ho, a.lab[a.oadj[o0 + j]], a.ow[o0 + j], j);
+  for (int32_t j = lane + 64; j < ni; j += 64) ht_insert(hi, a.lab[a.iadj[i0 + j]], a.iw[i0 + j], j);
   __syncthreads();
   const int64_t c_r = ht_weight(hi, r), c_s = ht_weight(hi, s);
   const bool r_in_a = ht_find(ho, r) >= 0, s_in_a = ht_find(ho, s) >= 0;
-  const double l_dr = clog2((double)((int64_t)m.dout[r] - kout));
-  const double l_ds = clog2((double)((int64_t)m.dout[s] + kout));
-  const double l_ir = clog2((double)((int64_t)m.din[r] - kin));
-  const double l_is = clog2((double)((int64_t)m.din[s] + kin));
+  const double l_dr = clog2((double)(dout_r - kout));
+  const double l_ds = clog2((double)(dout_s + kout));
+  const double l_ir = clog2((double)(din_r - kin));
+  const double l_is = clog2((double)(din_s + kin));
+  // M[r][r], M[s][r], M[r][s], M[s][s] for the untouched column counts, one lane each
+  int64_t corner = 0;
+  if (lane < 4) {
+    const int32_t cc = lane < 2 ? r : s, rr = (lane & 1) ? s : r;
+    corner = bm_col_get(m, cc, rr);
+  }
 
   // rows r and s over the row slots
   double p_row = 0.0;
@@ -236,7 +256,7 @@ __global__ __launch_bounds__(64) void k_nodal(NodalArgs a) {
     int64_t av = 0;
     bool use;
     if (j < no) {
-      t = a.lab[a.oadj[o0 + j]];
+      t = j < 64 ? t_o0 : a.lab[a.oadj[o0 + j]];
       const int32_t slot = ht_find(ho, t);
       use = ld_relaxed(&ho.first[slot]) == j;
       av = ld_relaxed(&ho.w[slot]);
@@ -248,13 +268,16 @@ __global__ __launch_bounds__(64) void k_nodal(NodalArgs a) {
       use = !s_in_a;
     }
     if (!use) continue;
-    const int64_t Mr = bm_row_get(m, r, t), Ms = bm_row_get(m, s, t);
-    const double or_ = Mr > 0 ? term(Mr, m.ldout[r], m.ldin[t]) : 0.0;
-    const double os_ = Ms > 0 ? term(Ms, m.ldout[s], m.ldin[t]) : 0.0;
+    int32_t Mr32, Ms32;
+    bm_row_get2(m, r, s, t, Mr32, Ms32);
+    const int64_t Mr = Mr32, Ms = Ms32;
+    const double ldt = m.ldin[t];
+    const double or_ = Mr > 0 ? term(Mr, ldo_r, ldt) : 0.0;
+    const double os_ = Ms > 0 ? term(Ms, ldo_s, ldt) : 0.0;
     const int64_t R = Mr - av - (t == r ? c_r : 0) + (t == s ? c_r : 0);
     const int64_t S = Ms + av - (t == r ? c_s : 0) + (t == s ? c_s : 0);
-    const double nr = R > 0 ? term(R, m.ldin[t], l_dr) : 0.0;
-    const double ns = S > 0 ? term(S, m.ldin[t], l_ds) : 0.0;
+    const double nr = R > 0 ? term(R, ldt, l_dr) : 0.0;
+    const double ns = S > 0 ? term(S, ldt, l_ds) : 0.0;
     p_row += ((or_ + os_) - nr) - ns;
     sum_r += Mr;
     sum_s += Ms;
@@ -263,16 +286,19 @@ __global__ __launch_bounds__(64) void k_nodal(NodalArgs a) {
   double p_col = 0.0;
   int64_t csum_r = 0, csum_s = 0;
   for (int32_t j = lane; j < ni; j += 64) {
-    const int32_t t = a.lab[a.iadj[i0 + j]];
+    const int32_t t = j < 64 ? t_i0 : a.lab[a.iadj[i0 + j]];
     const int32_t slot = ht_find(hi, t);
     if (ld_relaxed(&hi.first[slot]) != j || t == r || t == s) continue;
     const int64_t c = ld_relaxed(&hi.w[slot]);
-    const int64_t Mr = bm_col_get(m, r, t), Ms = bm_col_get(m, s, t);
-    const double ir = Mr > 0 ? term(Mr, m.ldout[t], m.ldin[r]) : 0.0;
-    const double is = Ms > 0 ? term(Ms, m.ldout[t], m.ldin[s]) : 0.0;
+    int32_t Mr32, Ms32;
+    bm_col_get2(m, r, s, t, Mr32, Ms32);
+    const int64_t Mr = Mr32, Ms = Ms32;
+    const double ldt = m.ldout[t];
+    const double ir = Mr > 0 ? term(Mr, ldt, ldi_r) : 0.0;
+    const double is = Ms > 0 ? term(Ms, ldt, ldi_s) : 0.0;
     const int64_t wr = Mr - c, ws = Ms + c;
-    const double nr = wr > 0 ? term(wr, m.ldout[t], l_ir) : 0.0;
-    const double ns = ws > 0 ? term(ws, m.ldout[t], l_is) : 0.0;
+    const double nr = wr > 0 ? term(wr, ldt, l_ir) : 0.0;
+    const double ns = ws > 0 ? term(ws, ldt, l_is) : 0.0;
     p_col += ((ir + is) - nr) - ns;
     csum_r += Mr;
     csum_s += Ms;
@@ -282,14 +308,16 @@ __global__ __launch_bounds__(64) void k_nodal(NodalArgs a) {
   sum_s = wave_sum_i64(sum_s);
   csum_r = wave_sum_i64(csum_r);
   csum_s = wave_sum_i64(csum_s);
+  const int64_t M_rr = __shfl(corner, 0, 64), M_sr = __shfl(corner, 1, 64);
+  const int64_t M_rs = __shfl(corner, 2, 64), M_ss = __shfl(corner, 3, 64);
   if (lane == 0) {
-    const int64_t W_r = (int64_t)m.dout[r] - sum_r, W_s = (int64_t)m.dout[s] - sum_s;
-    const double Y_r = W_r > 0 ? (m.ldout[r] - l_dr) * (double)W_r : 0.0;
-    const double Y_s = W_s > 0 ? (m.ldout[s] - l_ds) * (double)W_s : 0.0;
-    const int64_t W2_r = (int64_t)m.din[r] - bm_col_get(m, r, r) - bm_col_get(m, r, s) - csum_r;
-    const int64_t W2_s = (int64_t)m.din[s] - bm_col_get(m, s, r) - bm_col_get(m, s, s) - csum_s;
-    const double Y2_r = W2_r > 0 ? (m.ldin[r] - l_ir) * (double)W2_r : 0.0;
-    const double Y2_s = W2_s > 0 ? (m.ldin[s] - l_is) * (double)W2_s : 0.0;
+    const int64_t W_r = dout_r - sum_r, W_s = dout_s - sum_s;
+    const double Y_r = W_r > 0 ? (ldo_r - l_dr) * (double)W_r : 0.0;
+    const double Y_s = W_s > 0 ? (ldo_s - l_ds) * (double)W_s : 0.0;
+    const int64_t W2_r = din_r - M_rr - M_sr - csum_r;
+    const int64_t W2_s = din_s - M_rs - M_ss - csum_s;
+    const double Y2_r = W2_r > 0 ? (ldi_r - l_ir) * (double)W2_r : 0.0;
+    const double Y2_s = W2_s > 0 ? (ldi_s - l_is) * (double)W2_s : 0.0;
     // calculate_dS_overall_nodal (kernels.hpp:600-643): accept iff dS < 0
     double d = drow - Y_r;
     d = d - Y_s;
@@ -300,8 +328,177 @@ __global__ __launch_bounds__(64) void k_nodal(NodalArgs a) {
   }
 }
 
-void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const int32_t *sof, const uint8_t *coin, RngKey key, uint32_t phase, int64_t lo,
-                  int64_t BS, int32_t *prop, double *dS, hipStream_t st) {
+// Group sums over the G lanes of a group (every lane gets the result).
+template <int G>
+GSAP_D int64_t group_allsum(int64_t x) {
+#pragma unroll
+  for (int off = G / 2; off >= 1; off >>= 1) x += __shfl_xor(x, off, G);
+  return x;
+}
+
+// Same computation as k_nodal for a vertex with at most G out- and G in-neighbours, G lanes
+// per vertex: lane q holds out-entry q and in-entry q; the neighbour blocks are deduplicated
+// in registers (shuffles inside the group) instead of hash tables, and the block of the
+// sampled neighbour comes from the same gathered values.
+template <int G>
+__global__ __launch_bounds__(256, 5) void k_nodal_small(NodalArgs a, const int32_t *list,
+                                                        int32_t n) {
+  const int32_t gid = (int32_t)(blockIdx.x * (256 / G) + threadIdx.x / G);
+  if (gid >= n) return;
+  const int q = threadIdx.x & (G - 1);
+  const int gbase = (threadIdx.x & 63) & ~(G - 1);
+  const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+  const int64_t v = list[gid];
+  const int64_t i = v - a.lo;
+  const BMView &m = a.m;
+  const int32_t B = m.B;
+  const int32_t r = a.lab[v];
+  const Philox x = philox((uint32_t)v, a.phase, a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
+  const int32_t o0 = a.optr[v], no = a.optr[v + 1] - o0;
+  const int32_t i0 = a.iptr[v], ni = a.iptr[v + 1] - i0;
+  const int64_t kout = a.kout[v], kin = a.kin[v];
+  const int64_t tot = kout + kin;
+  int32_t t_o = -1, w_o = 0, t_i = -1, w_i = 0;
+  if (q < no) { w_o = a.ow[o0 + q]; t_o = a.lab[a.oadj[o0 + q]]; }
+  if (q < ni) { w_i = a.iw[i0 + q]; t_i = a.lab[a.iadj[i0 + q]]; }
+  // proposal: first entry of (out list, in list) whose running weight exceeds the target
+  int32_t s;
+  if (tot == 0) {
+    s = (int32_t)mulhi(x.x0, B);
+  } else {
+    const uint32_t target = mulhi(x.x1, tot);
+    uint32_t io = (uint32_t)w_o, ii = (uint32_t)w_i;
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+      const uint32_t yo = __shfl_up(io, off, G), yi = __shfl_up(ii, off, G);
+      if (q >= off) { io += yo; ii += yi; }
+    }
+    const uint32_t tot_o = __shfl(io, G - 1, G);
+    const uint64_t mo = (__ballot(q < no && io > target) >> gbase) & gmask;
+    int32_t u;
+    if (mo) {
+      u = __shfl(t_o, __ffsll((long long)mo) - 1, G);
+    } else {
+      const uint64_t mi = (__ballot(q < ni && tot_o + ii > target) >> gbase) & gmask;
+      u = __shfl(t_i, __ffsll((long long)mi) - 1, G);
+    }
+    s = a.coin[u] ? (int32_t)mulhi(x.x0, B) : a.sof[u];
+  }
+  if (q == 0) a.prop[i] = s;
+  if (s == r) {
+    if (q == 0) a.dS[i] = 0.0;
+    return;
+  }
+  // per-lane block weight and first-occurrence flag
+  int64_t a_sum = 0, c_sum = 0;
+  bool first_o = q < no, first_i = q < ni;
+  const int pend = no > ni ? no : ni;
+#pragma unroll 1
+  for (int p = 0; p < pend; ++p) {
+    const int32_t tpo = __shfl(t_o, p, G), wpo = __shfl(w_o, p, G);
+    const int32_t tpi = __shfl(t_i, p, G), wpi = __shfl(w_i, p, G);
+    if (tpo == t_o) { a_sum += wpo; if (p < q) first_o = false; }
+    if (tpi == t_i) { c_sum += wpi; if (p < q) first_i = false; }
+  }
+  const int64_t c_r = group_allsum<G>((q < ni && t_i == r) ? w_i : 0);
+  const int64_t c_s = group_allsum<G>((q < ni && t_i == s) ? w_i : 0);
+  const bool r_in_a = ((__ballot(q < no && t_o == r) >> gbase) & gmask) != 0;
+  const bool s_in_a = ((__ballot(q < no && t_o == s) >> gbase) & gmask) != 0;
+  const int64_t dout_r = m.dout[r], dout_s = m.dout[s], din_r = m.din[r], din_s = m.din[s];
+  const double ldo_r = m.ldout[r], ldo_s = m.ldout[s], ldi_r = m.ldin[r], ldi_s = m.ldin[s];
+  const double l_dr = clog2((double)(dout_r - kout));
+  const double l_ds = clog2((double)(dout_s + kout));
+  const double l_ir = clog2((double)(din_r - kin));
+  const double l_is = clog2((double)(din_s + kin));
+  int64_t corner = 0;
+  if (q < 4) {
+    const int32_t cc = q < 2 ? r : s, rr = (q & 1) ? s : r;
+    corner = bm_col_get(m, cc, rr);
+  }
+  // rows r and s over the row slots: positions 0..no-1 (first occurrences), no (r), no+1 (s)
+  CAcc<G> prow;
+  prow.init();
+  int64_t sum_r = 0, sum_s = 0;
+#pragma unroll
+  for (int jj = 0; jj < 64 / G; ++jj) {
+    const int32_t j = q + jj * G;
+    if (j >= no + 2) break;
+    int32_t t;
+    int64_t av = 0;
+    bool use;
+    if (j < no) {
+      t = t_o;
+      av = a_sum;
+      use = first_o;
+    } else if (j == no) {
+      t = r;
+      use = !r_in_a;
+    } else {
+      t = s;
+      use = !s_in_a;
+    }
+    if (!use) continue;
+    int32_t Mr32, Ms32;
+    bm_row_get2(m, r, s, t, Mr32, Ms32);
+    const int64_t Mr = Mr32, Ms = Ms32;
+    const double ldt = m.ldin[t];
+    const double or_ = Mr > 0 ? term(Mr, ldo_r, ldt) : 0.0;
+    const double os_ = Ms > 0 ? term(Ms, ldo_s, ldt) : 0.0;
+    const int64_t R = Mr - av - (t == r ? c_r : 0) + (t == s ? c_r : 0);
+    const int64_t S = Ms + av - (t == r ? c_s : 0) + (t == s ? c_s : 0);
+    const double nr = R > 0 ? term(R, ldt, l_dr) : 0.0;
+    const double ns = S > 0 ? term(S, ldt, l_ds) : 0.0;
+    prow.p[jj] += ((or_ + os_) - nr) - ns;
+    sum_r += Mr;
+    sum_s += Ms;
+  }
+  // columns r and s over the column slots (position q, t not in {r, s})
+  CAcc<G> pcol;
+  pcol.init();
+  int64_t csum_r = 0, csum_s = 0;
+  if (first_i && t_i != r && t_i != s) {
+    const int32_t t = t_i;
+    int32_t Mr32, Ms32;
+    bm_col_get2(m, r, s, t, Mr32, Ms32);
+    const int64_t Mr = Mr32, Ms = Ms32;
+    const double ldt = m.ldout[t];
+    const double ir = Mr > 0 ? term(Mr, ldt, ldi_r) : 0.0;
+    const double is = Ms > 0 ? term(Ms, ldt, ldi_s) : 0.0;
+    const int64_t wr = Mr - c_sum, ws = Ms + c_sum;
+    const double nr = wr > 0 ? term(wr, ldt, l_ir) : 0.0;
+    const double ns = ws > 0 ? term(ws, ldt, l_is) : 0.0;
+    pcol.p[0] += ((ir + is) - nr) - ns;
+    csum_r += Mr;
+    csum_s += Ms;
+  }
+  const double drow = prow.reduce(), dcol = pcol.reduce();
+  sum_r = group_allsum<G>(sum_r);
+  sum_s = group_allsum<G>(sum_s);
+  csum_r = group_allsum<G>(csum_r);
+  csum_s = group_allsum<G>(csum_s);
+  const int64_t M_rr = __shfl(corner, 0, G), M_sr = __shfl(corner, 1, G);
+  const int64_t M_rs = __shfl(corner, 2, G), M_ss = __shfl(corner, 3, G);
+  if (q == 0) {
+    const int64_t W_r = dout_r - sum_r, W_s = dout_s - sum_s;
+    const double Y_r = W_r > 0 ? (ldo_r - l_dr) * (double)W_r : 0.0;
+    const double Y_s = W_s > 0 ? (ldo_s - l_ds) * (double)W_s : 0.0;
+    const int64_t W2_r = din_r - M_rr - M_sr - csum_r;
+    const int64_t W2_s = din_s - M_rs - M_ss - csum_s;
+    const double Y2_r = W2_r > 0 ? (ldi_r - l_ir) * (double)W2_r : 0.0;
+    const double Y2_s = W2_s > 0 ? (ldi_s - l_is) * (double)W2_s : 0.0;
+    double d = drow - Y_r;
+    d = d - Y_s;
+    d = d + dcol;
+    d = d - Y2_r;
+    d = d - Y2_s;
+    a.dS[i] = d < 0.0 ? d : 0.0;
+  }
+}
+
+void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const int32_t *sof,
+                  const uint8_t *coin, RngKey key, uint32_t phase, int64_t lo, int64_t BS,
+                  const int32_t *const lists[4], const int32_t counts[4], uint32_t big_bits,
+                  int32_t *prop, double *dS, hipStream_t st) {
   if (BS <= 0) return;
   NodalArgs a;
   a.lo = lo; a.BS = BS;
@@ -311,7 +508,15 @@ void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const 
   a.m = m; a.sof = sof; a.coin = coin; a.key = key; a.phase = phase;
   a.prop = prop; a.dS = dS;
   a.hoff = g.hoff; a.hbits_o = g.hbits_o; a.hbits_i = g.hbits_i; a.hpool = g.hpool;
-  k_nodal<<<(unsigned)BS, 64, 0, st>>>(a);
+  if (counts[0] > 0)
+    k_nodal_small<16><<<grid_for(counts[0], 16), 256, 0, st>>>(a, lists[0], counts[0]);
+  if (counts[1] > 0)
+    k_nodal_small<32><<<grid_for(counts[1], 8), 256, 0, st>>>(a, lists[1], counts[1]);
+  if (counts[2] > 0)
+    k_nodal<<<(unsigned)counts[2], 64, 6 * 4 * (size_t)LCAP, st>>>(a, lists[2], (uint32_t)LBITS);
+  if (counts[3] > 0)
+    k_nodal<<<(unsigned)counts[3], 64, 6 * 4 * ((size_t)1 << big_bits), st>>>(a, lists[3],
+                                                                             big_bits);
 }
 
 __global__ void k_apply(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo,
