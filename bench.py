@@ -31,6 +31,81 @@ CONFIGS = {
 }
 HBM_PEAK_GBS = 8000.0
 KCLASS = ["nodal", "rebuild", "block_terms", "merge", "mdl", "accept_sum"]
+# device kernels of each class (the library times a class with HIP events around all of them)
+KERNELS_OF = {
+    "nodal": ["k_nodal"],
+    "rebuild": ["k_label_", "k_tile_hist", "k_hist_reduce", "k_dense_", "k_transpose", "k_logs",
+                "k_make_keys", "k_make_tkeys", "k_radix_", "k_run_flags", "k_rle", "k_scan_",
+                "k_fill_i32", "k_ptr_from_major", "k_seg_sum", "k_csc_unpack", "k_copy_i32"],
+    "block_terms": ["k_block_terms"],
+    "merge": ["k_merge", "k_argmin"],
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c3_fp64_traffic.json")
+
+
+def kernel_class(name):
+    short = name.split("(")[0].replace("void ", "").replace("gsap::", "")
+    for k, pats in KERNELS_OF.items():
+        if any(short.startswith(pt) for pt in pats):
+            return k
+    return "other"
+
+
+def summarize_profile(stats_csv, fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, bench_log,
+                      out_md, out_json, command):
+    """rocprofv3 CSVs -> a short markdown summary + per-class HBM bytes per launch."""
+    import csv
+    from collections import defaultdict
+    stats = list(csv.DictReader(open(stats_csv)))
+
+    def per_dispatch(path):
+        rows = list(csv.DictReader(open(path)))
+        agg = defaultdict(float)
+        for r in rows:
+            agg[r["Kernel_Name"]] += float(r["Counter_Value"])
+        return agg, rows
+
+    fetch, _ = per_dispatch(fetch_csv)
+    write, _ = per_dispatch(write_csv)
+    cf, crow = per_dispatch(cal_fetch_csv)
+    cw, _ = per_dispatch(cal_write_csv)
+    copy = [k for k in cf if "copyBuffer" in k][0]
+    ncopy = sum(1 for r in crow if r["Kernel_Name"] == copy)
+    copy_bytes = 512 * 1024 * 1024 * ncopy  # 3 copies of a 512 MiB buffer (bench --calibrate-copy)
+    f_fac = copy_bytes / (cf[copy] * 1024)
+    w_fac = copy_bytes / (cw[copy] * 1024)
+    tot = sum(float(r["TotalDurationNs"]) for r in stats)
+    lines = [f"# Kernel profile: full partition of the 200k-vertex graph (FP64 form)", "",
+             f"Command: `{command}`", "",
+             f"PMC calibration on {ncopy} plain 512 MiB device copies (bench.py --calibrate-copy): "
+             f"bytes read = {f_fac:.3f} x FETCH_SIZE KiB x 1024, "
+             f"bytes written = {w_fac:.3f} x WRITE_SIZE KiB x 1024.", "",
+             "| kernel | class | calls | total ms | avg us | % | HBM read MB/call | HBM write MB/call |",
+             "|---|---|---|---|---|---|---|---|"]
+    cls_bytes = defaultdict(lambda: [0.0, 0.0])
+    calls_of = {r["Name"]: int(r["Calls"]) for r in stats}
+    for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"])):
+        n = r["Name"]
+        c = kernel_class(n)
+        calls = int(r["Calls"])
+        rb = fetch.get(n, 0.0) * 1024 * f_fac
+        wb = write.get(n, 0.0) * 1024 * w_fac
+        cls_bytes[c][0] += rb
+        cls_bytes[c][1] += wb
+        if float(r["TotalDurationNs"]) / tot >= 0.003:
+            lines.append(f"| {n.split('(')[0][:48]} | {c} | {calls} | "
+                         f"{float(r['TotalDurationNs']) / 1e6:.2f} | {float(r['AverageNs']) / 1e3:.1f} | "
+                         f"{100 * float(r['TotalDurationNs']) / tot:.1f} | {rb / calls / 1e6:.2f} | "
+                         f"{wb / calls / 1e6:.2f} |")
+    lines += ["", f"Total kernel time {tot / 1e6:.1f} ms.", ""]
+    open(out_md, "w").write("\n".join(lines) + "\n")
+    bench = json.loads([l for l in open(bench_log) if l.startswith("{")][-1])
+    json.dump({"command": command, "fetch_factor": f_fac, "write_factor": w_fac,
+               "class_launches": bench["kernel_calls_per_step"],
+               "class_bytes_total": {k: v[0] + v[1] for k, v in cls_bytes.items()},
+               "class_read_total": {k: v[0] for k, v in cls_bytes.items()},
+               "class_write_total": {k: v[1] for k, v in cls_bytes.items()}},
+              open(out_json, "w"), indent=1)
 
 
 def algorithmic_bytes(kclass, g, nb, B, nnz):
@@ -76,7 +151,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 512 MiB device copy (PMC byte-counter calibration)")
+    ap.add_argument("--summarize-profile", nargs=8,
+                    metavar=("STATS", "FETCH", "WRITE", "CALF", "CALW", "BENCH_LOG", "OUT_MD",
+                             "OUT_JSON"),
+                    help="turn rocprofv3 CSVs into the profiles/ summary")
+    ap.add_argument("--profile-command", default="")
     a = ap.parse_args()
+    if a.summarize_profile:
+        summarize_profile(*a.summarize_profile, command=a.profile_command)
+        return
     if a.calibrate_copy:
         import torch
         x = torch.ones(128 << 20, dtype=torch.int32, device="cuda")
@@ -123,6 +206,12 @@ def main():
         avg_ms = kms[0] / max(kcalls[0], 1)
         abytes = algorithmic_bytes("nodal", g, 4, rep.B, final["nnz"])
     achieved = abytes / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        tf = json.load(open(TRAFFIC_FILE))
+        launches = tf.get("class_launches", {}).get(dname)
+        if launches:
+            traffic = tf["class_bytes_total"].get(dname, 0.0) / launches
     p, r = gsap_amd.pairwise(g.truth, lab)
     out = {
         "metric": "nodal-move edges/s (full partition of the 200k-vertex graph); full-partition "
@@ -151,7 +240,7 @@ def main():
         "kernel_calls_per_step": {KCLASS[i]: int(kcalls[i] // a.steps) for i in range(len(KCLASS))},
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "avg_launch_ms": avg_ms,
+                     "traffic": traffic, "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": abytes},
     }
     if not a.no_cpu_baseline:
