@@ -49,7 +49,8 @@ class COptions(C.Structure):
                 ("threshold2", C.c_double), ("moving_avg_window", C.c_int32),
                 ("max_levels", C.c_int32), ("precision", C.c_int32), ("chain", C.c_uint32),
                 ("trace", C.c_int32), ("dense_max_B", C.c_int32), ("profile", C.c_int32),
-                ("reserved", C.c_int32 * 7)]
+                ("model", C.c_int32), ("workers", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
 
 
 class CReport(C.Structure):
@@ -77,7 +78,8 @@ EXPORTS = [
     "gsap_options_default", "gsap_create", "gsap_destroy", "gsap_last_error",
     "gsap_partition_i32", "gsap_get_trace", "gsap_blockmodel_i32", "gsap_merge_phase_i32",
     "gsap_nodal_batch_i32", "gsap_mdl_i32", "gsap_generate_dcsbm_i32", "gsap_read_tsv_i32",
-    "gsap_pairwise_i32", "gsap_log2", "gsap_log2_tables",
+    "gsap_pairwise_i32", "gsap_log2", "gsap_log2_tables", "gsap_partition_batch_i32",
+    "gsap_partition_chains_i32",
 ]
 
 _lib = None
@@ -101,6 +103,10 @@ def lib() -> C.CDLL:
     L.gsap_partition_i32.argtypes = [vp, C.POINTER(CGraph), C.POINTER(COptions), _i32p,
                                      C.POINTER(CReport)]
     L.gsap_get_trace.argtypes = [vp, C.POINTER(CTrace), i64, C.POINTER(i64)]
+    L.gsap_partition_batch_i32.argtypes = [vp, C.POINTER(CGraph), i32, C.POINTER(COptions),
+                                           C.POINTER(C.c_void_p), C.POINTER(CReport)]
+    L.gsap_partition_chains_i32.argtypes = [vp, C.POINTER(CGraph), C.POINTER(COptions), i32,
+                                            _i32p, C.POINTER(CReport), _f64p, C.POINTER(i32)]
     L.gsap_blockmodel_i32.argtypes = [vp, C.POINTER(CGraph), _i32p, i32, _i64p, _i32p, _i64p,
                                       _i64p, _i32p, _i64p, _i64p, _i64p, C.POINTER(i64),
                                       C.POINTER(COptions)]
@@ -184,12 +190,14 @@ class Options:
     trace: int = 0
     dense_max_B: int = 0
     profile: int = 0
+    model: int = 0  # 0: the reference's model term, 1: corrected (x ln x scaled by E)
+    workers: int = 0  # batch / chain calls: concurrent sub-contexts (0 = 8)
 
     def cstruct(self) -> COptions:
         o = COptions()
         for f in ("seed", "num_proposals", "reduction_rate", "max_nodal_itr", "num_batches",
                   "threshold1", "threshold2", "moving_avg_window", "max_levels", "precision",
-                  "chain", "trace", "dense_max_B", "profile"):
+                  "chain", "trace", "dense_max_B", "profile", "model", "workers"):
             setattr(o, f, getattr(self, f))
         return o
 
@@ -258,6 +266,28 @@ class Context:
         _check(lib().gsap_partition_i32(self._h, C.byref(cg), C.byref(co), out, C.byref(rep)),
                self._h)
         return out, Report.from_c(rep)
+
+    def partition_batch(self, graphs, opts: Options = Options()):
+        """Batched form: independent graphs on concurrent sub-contexts."""
+        n = len(graphs)
+        cg = (CGraph * max(n, 1))(*[g.cstruct() for g in graphs])
+        outs = [np.empty(g.N, dtype=np.int32) for g in graphs]
+        ptrs = (C.c_void_p * max(n, 1))(*[o.ctypes.data for o in outs])
+        reps = (CReport * max(n, 1))()
+        co = opts.cstruct()
+        _check(lib().gsap_partition_batch_i32(self._h, cg, n, C.byref(co), ptrs, reps), self._h)
+        return outs, [Report.from_c(reps[k]) for k in range(n)]
+
+    def partition_chains(self, g: Graph, n_chains: int, opts: Options = Options()):
+        """n_chains chains (stream ids opts.chain + k); returns the lowest-S labelling."""
+        out = np.empty(g.N, dtype=np.int32)
+        rep = CReport()
+        S = np.zeros(n_chains, np.float64)
+        best = C.c_int32()
+        cg, co = g.cstruct(), opts.cstruct()
+        _check(lib().gsap_partition_chains_i32(self._h, C.byref(cg), C.byref(co), n_chains, out,
+                                               C.byref(rep), S, C.byref(best)), self._h)
+        return out, Report.from_c(rep), S, best.value
 
     def trace(self):
         n = C.c_int64()

@@ -19,9 +19,9 @@ CFLAGS = [
     "-x", "hip", "-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
     # bit-exact FP64 terms: no contraction of a*b+c into fma behind our back
     "-ffp-contract=off",
-    # packed FP32 instructions stay off (environment notes)
+    # no packed FP32 instructions (some of their forms have given wrong results on this part)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+    "-Wall", "-pthread", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
 
@@ -40,7 +40,7 @@ def build(verbose: bool = True) -> str:
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs]
+    cmd = [HIPCC, "-shared", "-fPIC", "-pthread", f"--offload-arch={ARCH}", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

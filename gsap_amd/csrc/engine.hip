@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -19,6 +20,7 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/gsap/gsap.h"
@@ -160,6 +162,8 @@ struct gsap_ctx {
   RebuildScratch rs;
   int64_t N = 0, E = 0;
   int32_t dense_max_B = 1024;
+  // sub-contexts of batch / chain calls (one per worker thread)
+  std::vector<gsap_ctx *> workers;
 };
 
 namespace {
@@ -340,12 +344,17 @@ int32_t read_nnz(gsap_ctx *c) {
 int32_t avg_row(int64_t nnz, int32_t B) { return (int32_t)(B > 0 ? (nnz + B - 1) / B : 0); }
 
 // total description length from the canonical data sum (partition.hpp:665-704, FP64)
-double model_S(double data_S, int64_t N, int64_t E, int32_t B) {
+double model_S(double data_S, int64_t N, int64_t E, int32_t B, int32_t model_kind) {
   const double LN2 = 0.6931471805599453094;
   double b = (double)B, dE = (double)E;
   double x = (b * b) / dE;
-  double model = dE * (1.0 + x) * (clog2_host(1.0 + x) * LN2) - x * (clog2_host(x) * LN2) +
-                 (double)N * (clog2_host(b) * LN2);
+  double model;
+  if (model_kind == GSAP_MODEL_CORRECTED)
+    model = dE * ((1.0 + x) * (clog2_host(1.0 + x) * LN2) - x * (clog2_host(x) * LN2)) +
+            (double)N * (clog2_host(b) * LN2);
+  else
+    model = dE * (1.0 + x) * (clog2_host(1.0 + x) * LN2) - x * (clog2_host(x) * LN2) +
+            (double)N * (clog2_host(b) * LN2);
   return model - data_S;
 }
 
@@ -358,13 +367,13 @@ void launch_mdl(gsap_ctx *c, double *out) {
   c->prof.end(p, c->st);
 }
 
-double mdl_now(gsap_ctx *c) {
+double mdl_now(gsap_ctx *c, int32_t model_kind) {
   double *res = c->res.as<double>();
   launch_mdl(c, res);
   double h = 0;
   HIPX(hipMemcpyAsync(&h, res, 8, hipMemcpyDeviceToHost, c->st));
   HIPX(hipStreamSynchronize(c->st));
-  return model_S(h, c->N, c->E, c->bm.B);
+  return model_S(h, c->N, c->E, c->bm.B, model_kind);
 }
 
 int32_t find_root(std::vector<int32_t> &par, int32_t x) {
@@ -472,6 +481,8 @@ void check_options(const gsap_options &o) {
   if (o.moving_avg_window < 2) throw Err(GSAP_ERR_INVALID, "moving_avg_window must be >= 2");
   if (o.precision != GSAP_PRECISION_FP64 && o.precision != GSAP_PRECISION_MIXED)
     throw Err(GSAP_ERR_INVALID, "unknown precision");
+  if (o.model != GSAP_MODEL_REFERENCE && o.model != GSAP_MODEL_CORRECTED)
+    throw Err(GSAP_ERR_INVALID, "unknown model");
 }
 
 struct Slot {
@@ -556,7 +567,7 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
       c->prof.flush();
       ar = avg_row(nnz, B);
       for (int b = 0; b < o.num_batches; ++b) dE[itr] = dE[itr] + h[1 + b];
-      S = model_S(h[0], N, c->E, B);
+      S = model_S(h[0], N, c->E, B, o.model);
       ++sweeps;
       t_sweep += now() - ts;
       if (tr) {
@@ -650,6 +661,8 @@ gsap_options defaults() {
   o.trace = 0;
   o.dense_max_B = 0;
   o.profile = 0;
+  o.model = GSAP_MODEL_REFERENCE;
+  o.workers = 0;
   return o;
 }
 
@@ -684,6 +697,33 @@ void setup_unit(gsap_ctx *c, const gsap_graph_i32 *g, const gsap_options *o, con
   rebuild(c, B);
 }
 
+// Runs jobs 0..n-1 on up to `want` worker sub-contexts, one host thread each.
+template <class F>
+void run_pool(gsap_ctx *c, int32_t n, int32_t want, F &&job) {
+  const int32_t W = std::max(1, std::min(n, want > 0 ? want : 8));
+  while ((int32_t)c->workers.size() < W) {
+    gsap_ctx *w = nullptr;
+    if (gsap_create(&w, c->device) != GSAP_OK)
+      throw Err(GSAP_ERR_DEVICE, "could not create a worker context");
+    c->workers.push_back(w);
+  }
+  std::atomic<int32_t> next(0);
+  std::vector<std::string> errs(n);
+  std::vector<int> codes(n, GSAP_OK);
+  std::vector<std::thread> th;
+  for (int32_t w = 0; w < W; ++w)
+    th.emplace_back([&, w] {
+      for (int32_t k; (k = next++) < n;) {
+        codes[k] = job(c->workers[w], k);
+        if (codes[k] != GSAP_OK) errs[k] = gsap_last_error(c->workers[w]);
+      }
+    });
+  for (auto &t : th) t.join();
+  for (int32_t k = 0; k < n; ++k)
+    if (codes[k] != GSAP_OK)
+      throw Err(codes[k], "job " + std::to_string(k) + ": " + errs[k]);
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -711,6 +751,8 @@ int gsap_create(gsap_ctx **ctx, int device) {
 
 int gsap_destroy(gsap_ctx *c) {
   if (!c) return GSAP_OK;
+  for (gsap_ctx *w : c->workers) gsap_destroy(w);
+  c->workers.clear();
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->st);
   DBuf *all[] = {&c->optr, &c->oadj, &c->ow, &c->iptr, &c->iadj, &c->iw, &c->kout, &c->kin,
@@ -736,6 +778,44 @@ int gsap_partition_i32(gsap_ctx *c, const gsap_graph_i32 *g, const gsap_options 
     if (!partitions_out) throw Err(GSAP_ERR_INVALID, "partitions_out is NULL");
     gsap_options oo = o ? *o : defaults();
     partition_impl(c, g, oo, partitions_out, report);
+  });
+}
+
+int gsap_partition_batch_i32(gsap_ctx *c, const gsap_graph_i32 *graphs, int32_t n_graphs,
+                             const gsap_options *o, int32_t *const *outs, gsap_report *reports) {
+  return guarded(c, [&] {
+    if (!graphs || !outs || n_graphs < 0) throw Err(GSAP_ERR_INVALID, "NULL argument");
+    const gsap_options oo = o ? *o : defaults();
+    check_options(oo);
+    run_pool(c, n_graphs, oo.workers, [&](gsap_ctx *w, int32_t k) {
+      return gsap_partition_i32(w, &graphs[k], &oo, outs[k], reports ? &reports[k] : nullptr);
+    });
+  });
+}
+
+int gsap_partition_chains_i32(gsap_ctx *c, const gsap_graph_i32 *g, const gsap_options *o,
+                              int32_t n_chains, int32_t *out, gsap_report *report,
+                              double *S_chain, int32_t *best_chain) {
+  return guarded(c, [&] {
+    if (!g || !out || n_chains < 1) throw Err(GSAP_ERR_INVALID, "bad argument");
+    const gsap_options oo = o ? *o : defaults();
+    check_options(oo);
+    validate(g);
+    std::vector<std::vector<int32_t>> labs(n_chains, std::vector<int32_t>((size_t)g->N));
+    std::vector<gsap_report> reps(n_chains);
+    run_pool(c, n_chains, oo.workers, [&](gsap_ctx *w, int32_t k) {
+      gsap_options ok = oo;
+      ok.chain = oo.chain + (uint32_t)k;
+      return gsap_partition_i32(w, g, &ok, labs[k].data(), &reps[k]);
+    });
+    int32_t best = 0;
+    for (int32_t k = 1; k < n_chains; ++k)
+      if (reps[k].S < reps[best].S) best = k;
+    std::copy(labs[best].begin(), labs[best].end(), out);
+    if (report) *report = reps[best];
+    if (S_chain)
+      for (int32_t k = 0; k < n_chains; ++k) S_chain[k] = reps[k].S;
+    if (best_chain) *best_chain = best;
   });
 }
 
@@ -820,7 +900,7 @@ int gsap_mdl_i32(gsap_ctx *c, const gsap_graph_i32 *g, const int32_t *labels, in
   return guarded(c, [&] {
     if (!labels || !S) throw Err(GSAP_ERR_INVALID, "NULL argument");
     setup_unit(c, g, o, labels, B);
-    *S = mdl_now(c);
+    *S = mdl_now(c, o ? o->model : GSAP_MODEL_REFERENCE);
   });
 }
 

@@ -39,6 +39,12 @@ extern "C" {
 #define GSAP_PRECISION_FP64 0  /* plain form: FP64 delta-S everywhere */
 #define GSAP_PRECISION_MIXED 1 /* FP32 bulk delta-S + FP64 near-threshold recheck */
 
+/* Model term of the description length (partition.hpp:696-699):
+ *   REFERENCE  E(1+x)ln(1+x) - x ln x + N ln B,   x = B^2/E  (as the reference computes it)
+ *   CORRECTED  E[(1+x)ln(1+x) - x ln x] + N ln B              (the x ln x term scaled by E) */
+#define GSAP_MODEL_REFERENCE 0
+#define GSAP_MODEL_CORRECTED 1
+
 typedef struct gsap_ctx gsap_ctx;
 
 /* A directed graph in two CSR views (out-adjacency and in-adjacency).  Ids are
@@ -73,7 +79,9 @@ typedef struct gsap_options {
   int32_t trace;              /* 1: record per-step digests (slow, for tests) */
   int32_t dense_max_B;        /* use the dense block matrix when B <= this (0 = auto) */
   int32_t profile;            /* 1: time kernel classes with HIP events (report.kernel_ms) */
-  int32_t reserved[7];
+  int32_t model;              /* GSAP_MODEL_*: description-length model term */
+  int32_t workers;            /* batch / chain calls: concurrent sub-contexts (0 = auto, 8) */
+  int32_t reserved[5];
 } gsap_options;
 
 /* Per-call report.  Times are seconds of host wall clock around device work. */
@@ -116,6 +124,21 @@ const char *gsap_last_error(const gsap_ctx *ctx);
  * 0-based block ids.  report may be NULL. */
 int gsap_partition_i32(gsap_ctx *ctx, const gsap_graph_i32 *g, const gsap_options *o,
                        int32_t *partitions_out, gsap_report *report);
+
+/* Batched form: n_graphs independent graphs (uneven sizes allowed).  They run on a pool of
+ * o->workers sub-contexts, each with its own HIP stream and workspace, so the kernels of
+ * different graphs overlap on the device.  outs[k] receives graph k's labels (N_k int32),
+ * reports[k] its report (reports may be NULL).  Graph k's result equals
+ * gsap_partition_i32 on it alone.  No counterpart in the reference (one graph per run). */
+int gsap_partition_batch_i32(gsap_ctx *ctx, const gsap_graph_i32 *graphs, int32_t n_graphs,
+                             const gsap_options *o, int32_t *const *outs, gsap_report *reports);
+
+/* Parallel chains: n_chains partitions of one graph with Philox stream ids o->chain + k;
+ * out receives the labels of the chain with the lowest S (ties: lowest k), *best_chain its
+ * index, S_chain[k] every chain's S (may be NULL).  No counterpart in the reference. */
+int gsap_partition_chains_i32(gsap_ctx *ctx, const gsap_graph_i32 *g, const gsap_options *o,
+                              int32_t n_chains, int32_t *out, gsap_report *report,
+                              double *S_chain, int32_t *best_chain);
 
 /* Trace records of the last gsap_partition_i32 call with o->trace = 1. */
 int gsap_get_trace(const gsap_ctx *ctx, gsap_trace_rec *out, int64_t cap, int64_t *n);

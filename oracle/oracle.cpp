@@ -576,8 +576,9 @@ void nodal_batch(const G &g, const orc_options &o, const Rng &rng, int32_t *lab,
   *dE = csum_array(ds.data(), BS);
 }
 
-// compute_overall_entropy_cuda (partition.hpp:665-704) in FP64, canonical order.
-double mdl(const BM &m, int64_t N, int64_t E) {
+// compute_overall_entropy_cuda (partition.hpp:665-704) in FP64, canonical order.  model 1
+// scales the x ln x term by E (the description length of Peixoto's degree-corrected SBM).
+double mdl(const BM &m, int64_t N, int64_t E, int32_t model) {
   std::vector<double> t(m.nnz);
   for (int32_t r = 0; r < m.B; ++r)
     for (int64_t k = m.rowptr[r]; k < m.rowptr[r + 1]; ++k)
@@ -586,9 +587,14 @@ double mdl(const BM &m, int64_t N, int64_t E) {
   const double LN2 = 0.6931471805599453094;
   double B = (double)m.B, dE = (double)E;
   double x = (B * B) / dE;
-  double model = dE * (1.0 + x) * (clog2(1.0 + x) * LN2) - x * (clog2(x) * LN2) +
-                 (double)N * (clog2(B) * LN2);
-  return model - data_S;
+  double mt;
+  if (model == 1)
+    mt = dE * ((1.0 + x) * (clog2(1.0 + x) * LN2) - x * (clog2(x) * LN2)) +
+         (double)N * (clog2(B) * LN2);
+  else
+    mt = dE * (1.0 + x) * (clog2(1.0 + x) * LN2) - x * (clog2(x) * LN2) +
+         (double)N * (clog2(B) * LN2);
+  return mt - data_S;
 }
 
 uint64_t fnv(const void *p, int64_t n, uint64_t h = 1469598103934665603ull) {
@@ -677,10 +683,10 @@ void orc_nodal_batch(const orc_graph *gg, const orc_options *o, int32_t *labels,
   nodal_batch(g, *o, rng, labels, m, b, q, prop, dS, dE, sc);
 }
 
-double orc_mdl(const orc_graph *gg, const int32_t *labels, int32_t B) {
+double orc_mdl(const orc_graph *gg, const int32_t *labels, int32_t B, int32_t model) {
   BM m;
   build_bm(gg->N, gg->out_ptr, gg->out_adj, gg->out_w, labels, B, m);
-  return mdl(m, gg->N, gg->E);
+  return mdl(m, gg->N, gg->E, model);
 }
 
 int orc_partition(const orc_graph *gg, const orc_options *o, int32_t *labels_out, orc_report *rep,
@@ -741,7 +747,7 @@ int orc_partition(const orc_graph *gg, const orc_options *o, int32_t *labels_out
         ++batch_ctr; ++batches;
         dE[itr] = dE[itr] + e;
       }
-      S = mdl(m, N, g.E);
+      S = mdl(m, N, g.E, o->model);
       ++sweeps;
       std::vector<int32_t> v32(m.val.begin(), m.val.end());
       rec(3, B, itr, 0, fnv(m.col.data(), 4 * m.nnz), fnv(v32.data(), 4 * m.nnz), S, dE[itr]);

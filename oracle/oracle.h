@@ -67,6 +67,7 @@ typedef struct orc_options {
   int32_t moving_avg_window; /* 3    partition.hpp:861 */
   int32_t max_levels;        /* safety cap on outer iterations */
   uint32_t chain;            /* Philox stream id (counter word 2) */
+  int32_t model;             /* 0: model term as partition.hpp:696-699, 1: x ln x scaled by E */
 } orc_options;
 
 typedef struct orc_trace_rec {
@@ -101,7 +102,7 @@ void orc_nodal_batch(const orc_graph *g, const orc_options *o, int32_t *labels, 
                      int32_t b, uint32_t q, int32_t *prop, double *dS, double *dE);
 
 /* Total description length of a labelling. */
-double orc_mdl(const orc_graph *g, const int32_t *labels, int32_t B);
+double orc_mdl(const orc_graph *g, const int32_t *labels, int32_t B, int32_t model);
 
 /* Whole partition().  labels_out[N]; trace may be NULL (cap = max records). */
 int orc_partition(const orc_graph *g, const orc_options *o, int32_t *labels_out,

@@ -29,7 +29,7 @@ class OOptions(C.Structure):
                 ("reduction_rate", C.c_float), ("max_nodal_itr", C.c_int32),
                 ("num_batches", C.c_int32), ("threshold1", C.c_double),
                 ("threshold2", C.c_double), ("moving_avg_window", C.c_int32),
-                ("max_levels", C.c_int32), ("chain", C.c_uint32)]
+                ("max_levels", C.c_int32), ("chain", C.c_uint32), ("model", C.c_int32)]
 
 
 class OTrace(C.Structure):
@@ -70,7 +70,7 @@ def lib():
                                       C.c_int32, C.c_uint32, _i32p, _f64p,
                                       C.POINTER(C.c_double)]
         L.orc_mdl.restype = C.c_double
-        L.orc_mdl.argtypes = [C.POINTER(OGraph), _i32p, C.c_int32]
+        L.orc_mdl.argtypes = [C.POINTER(OGraph), _i32p, C.c_int32, C.c_int32]
         L.orc_partition.restype = C.c_int
         L.orc_partition.argtypes = [C.POINTER(OGraph), C.POINTER(OOptions), _i32p,
                                     C.POINTER(OReport), C.POINTER(OTrace), C.c_int64]
@@ -96,7 +96,8 @@ def ograph(g):
 
 def ooptions(o):
     return OOptions(o.seed, o.num_proposals, o.reduction_rate, o.max_nodal_itr, o.num_batches,
-                    o.threshold1, o.threshold2, o.moving_avg_window, o.max_levels, o.chain)
+                    o.threshold1, o.threshold2, o.moving_avg_window, o.max_levels, o.chain,
+                    o.model)
 
 
 def philox(ctr, key):
@@ -162,9 +163,9 @@ def nodal_batch(g, opts, labels, B, b, q):
     return dict(labels=lab, prop=prop[:BS], dS=dS[:BS], dE=dE.value)
 
 
-def mdl(g, labels, B):
+def mdl(g, labels, B, model=0):
     og = ograph(g)
-    return lib().orc_mdl(C.byref(og), np.ascontiguousarray(labels, np.int32), B)
+    return lib().orc_mdl(C.byref(og), np.ascontiguousarray(labels, np.int32), B, model)
 
 
 def partition(g, opts, trace_cap=0):

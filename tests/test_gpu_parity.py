@@ -122,10 +122,11 @@ def test_nodal_batch(ctx, name, Bfrac, dense):
         assert np.array_equal(got["labels"], ref["labels"])
 
 
-@pytest.mark.parametrize("name", ["tiny_w", "hubs", "LL1k"])
-def test_partition_trace(ctx, name):
+@pytest.mark.parametrize("name,model", [("tiny_w", 0), ("hubs", 0), ("LL1k", 0), ("LL1k", 1),
+                                        ("HH5k", 1)])
+def test_partition_trace(ctx, name, model):
     g = graph(name)
-    o = Options(seed=3, trace=1)
+    o = Options(seed=3, trace=1, model=model)
     print(f"partition {name}", flush=True)
     lab, rep = ctx.partition(g, o)
     tr = ctx.trace()
@@ -136,6 +137,7 @@ def test_partition_trace(ctx, name):
         assert eq_bits(a[6], b[6]) and eq_bits(a[7], b[7]), (k, a, b)
     assert np.array_equal(lab, rlab)
     assert rep.B == rrep.B and eq_bits(rep.S, rrep.S)
+    assert eq_bits(rep.S, ctx.mdl(g, lab, rep.B, Options(model=model)))
 
 
 def test_partition_generated_c1(ctx):
@@ -179,3 +181,24 @@ def test_c3_blockmodel_and_batch(ctx, c3, B):
     rb = O.nodal_batch(g, o, lab, B, 1, 40)
     assert np.array_equal(gb["prop"], rb["prop"]) and eq_bits(gb["dS"], rb["dS"])
     assert eq_bits(gb["dE"], rb["dE"])
+
+
+def test_partition_batch(ctx):
+    graphs = [graph("tiny_w"), graph("hubs"), graph("LL1k"),
+              gsap_amd.generate_dcsbm(1000, 11, 0.0, 4.0, 20.0, seed=1), graph("rand")]
+    o = Options(seed=9, workers=3)
+    outs, reps = ctx.partition_batch(graphs, o)
+    for g, lab, rep in zip(graphs, outs, reps):
+        rlab, rrep, _ = O.partition(g, o)
+        assert np.array_equal(lab, rlab) and eq_bits(rep.S, rrep.S)
+
+
+def test_partition_chains(ctx):
+    g = graph("LL1k")
+    o = Options(seed=2, chain=5, workers=2)
+    lab, rep, S, best = ctx.partition_chains(g, 4, o)
+    refs = [O.partition(g, Options(seed=2, chain=5 + k)) for k in range(4)]
+    for k in range(4):
+        assert eq_bits(S[k], refs[k][1].S)
+    assert best == int(np.argmin([r[1].S for r in refs]))
+    assert np.array_equal(lab, refs[best][0]) and eq_bits(rep.S, S[best])

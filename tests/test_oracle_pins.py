@@ -118,6 +118,10 @@ def test_mdl_closed_form():
     model = 5 * (1 + x) * math.log(1 + x) - x * math.log(x) + 4 * math.log(2)
     assert abs(data + 6 * math.log2(3)) < 1e-12
     assert abs(S - (model - data)) < 1e-12
+    # corrected model term: E [(1+x) ln(1+x) - x ln x] + N ln B
+    S1 = O.mdl(g, np.array([0, 0, 1, 1], np.int32), 2, model=1)
+    model1 = 5 * ((1 + x) * math.log(1 + x) - x * math.log(x)) + 4 * math.log(2)
+    assert abs(S1 - (model1 - data)) < 1e-12
 
 
 # --------------------------------------------------------------------------- delta-S
