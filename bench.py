@@ -140,12 +140,75 @@ def cpu_baseline(g, labels, B, seconds):
                        f"(B={B}), {el:.1f} s")
 
 
+def c4_graphs(n=256):
+    """256 independent DC-SBM graphs, N log-uniform in [500, 50000], K = round(N^(1/3)),
+    Dirichlet(1) sizes, intra/inter ratio 4 (even g) or 1.5 (odd g), seed 4 + g."""
+    rng = np.random.default_rng(4)
+    Ns = np.exp(rng.uniform(np.log(500), np.log(50000), n)).astype(int)
+    out = []
+    for g, N in enumerate(Ns):
+        K = max(2, int(round(N ** (1 / 3))))
+        out.append(gsap_amd.generate_dcsbm(int(N), K, 1.0, 4.0 if g % 2 == 0 else 1.5, 20.0,
+                                           seed=4 + g))
+    return out
+
+
+def bench_multi(a, prec):
+    """C4 (batched graphs) and C5 (parallel chains): aggregate nodal-move edges/s over the
+    wall time of the whole call."""
+    t0 = time.perf_counter()
+    if a.config == "c4":
+        graphs = c4_graphs()
+        desc = (f"batched: {len(graphs)} DC-SBM graphs, N log-uniform [500, 50000] "
+                f"(sum N={sum(g.N for g in graphs)}, sum E={sum(g.E for g in graphs)}), "
+                f"K=round(N^(1/3)), rho 4/1.5, seeds 4+g")
+    else:
+        g5 = gsap_amd.generate_dcsbm(2000000, 150, 0.5, 2.0, 22.5, seed=5)
+        graphs = [g5]
+        desc = (f"chains: {a.chains} chains on N={g5.N}, E={g5.E}, K=150, alpha=0.5, rho=2, "
+                f"seed 5; lowest S kept")
+    t_gen = time.perf_counter() - t0
+    ctx = gsap_amd.Context(0)
+    opts = gsap_amd.Options(seed=1, precision=prec, workers=a.workers)
+    walls, edges = [], []
+    for k in range(a.warmup + a.steps):
+        t1 = time.perf_counter()
+        if a.config == "c4":
+            _, reps = ctx.partition_batch(graphs, opts)
+            e = sum(r.sweep_edges for r in reps)
+            extra = {}
+        else:
+            _, rep, S, best = ctx.partition_chains(graphs[0], a.chains, opts)
+            e = None
+            extra = {"best_chain": best, "S_best": rep.S, "S_chains": [float(x) for x in S]}
+        w = time.perf_counter() - t1
+        print(f"{'warmup' if k < a.warmup else 'step'} {k}: {w:.2f} s", file=sys.stderr,
+              flush=True)
+        if k >= a.warmup:
+            walls.append(w)
+            edges.append(e)
+    ctx.close()
+    out = {"metric": "aggregate nodal-move edges/s over the whole call" if a.config == "c4"
+                     else "partition seconds of the parallel-chain run",
+           "value": (sum(edges) / sum(walls)) if a.config == "c4" else float(np.mean(walls)),
+           "unit": "edges/s" if a.config == "c4" else "s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": 1e3 * float(np.mean(walls)),
+           "higher_is_better": a.config == "c4", "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic (seeded DC-SBM generator)",
+           "config": {"workload": desc, "workers": a.workers or 8},
+           "graph_generation_s": t_gen}
+    out.update(extra)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c3", choices=list(CONFIGS))
+    ap.add_argument("--config", default="c3", choices=list(CONFIGS) + ["c4", "c5"])
+    ap.add_argument("--chains", type=int, default=16)
+    ap.add_argument("--workers", type=int, default=0)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -174,11 +237,14 @@ def main():
         print(f"bench.py runs on one GPU: this build partitions one graph per GPU and has no "
               f"multi-GPU path (asked for --gpus {a.gpus})", file=sys.stderr)
         sys.exit(2)
+    prec = gsap_amd.PRECISION_FP64 if a.precision == "fp64" else gsap_amd.PRECISION_MIXED
+    if a.config in ("c4", "c5"):
+        bench_multi(a, prec)
+        return
     cfg = CONFIGS[a.config]
     t0 = time.perf_counter()
     g = gsap_amd.generate_dcsbm(**cfg)
     t_gen = time.perf_counter() - t0
-    prec = gsap_amd.PRECISION_FP64 if a.precision == "fp64" else gsap_amd.PRECISION_MIXED
     ctx = gsap_amd.Context(0)
     for w in range(a.warmup):
         ctx.partition(g, gsap_amd.Options(seed=1, precision=prec))

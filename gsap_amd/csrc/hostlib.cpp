@@ -9,6 +9,7 @@
 #include <numeric>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/gsap/gsap.h"
@@ -86,6 +87,78 @@ void build_csr(int64_t N, const std::vector<int32_t> &src, const std::vector<int
     int64_t p = cur[src[e]]++;
     adj[p] = dst[e];
     wout[p] = w[e];
+  }
+}
+
+int host_threads() {
+  unsigned h = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(16u, h));
+}
+
+// f(t) on nthr threads, t = 0 .. nthr-1
+template <class F>
+void run_threads(int nthr, F &&f) {
+  std::vector<std::thread> ts;
+  for (int t = 1; t < nthr; ++t) ts.emplace_back([&f, t] { f(t); });
+  f(0);
+  for (auto &t : ts) t.join();
+}
+
+// f(lo, hi) over [0, n) split into nthr contiguous ranges
+template <class F>
+void parallel_for(int nthr, uint64_t n, F &&f) {
+  if (nthr <= 1 || n < 4096) { f(0, n); return; }
+  std::vector<std::thread> ts;
+  const uint64_t per = (n + nthr - 1) / nthr;
+  for (int t = 0; t < nthr; ++t) {
+    const uint64_t lo = std::min(n, per * t), hi = std::min(n, lo + per);
+    if (lo < hi) ts.emplace_back([&f, lo, hi] { f(lo, hi); });
+  }
+  for (auto &t : ts) t.join();
+}
+
+// LSD radix sort of 64-bit keys, 16-bit digits; digits that are equal in every key are
+// skipped. Per-thread histograms keep the scatter stable.
+void radix_sort_u64(std::vector<uint64_t> &k, std::vector<uint64_t> &tmp, int nthr) {
+  const uint64_t n = k.size();
+  if (n < 2) return;
+  tmp.resize(n);
+  uint64_t all_or = 0, all_and = ~0ull;
+  for (uint64_t x : k) { all_or |= x; all_and &= x; }
+  const uint64_t per = (n + nthr - 1) / nthr;
+  std::vector<uint64_t> hist((size_t)nthr << 16);
+  for (int sh = 0; sh < 64; sh += 16) {
+    if ((((all_or ^ all_and) >> sh) & 0xFFFF) == 0) continue;
+    std::fill(hist.begin(), hist.end(), 0);
+    run_threads(nthr, [&](uint64_t t) {
+      uint64_t *h = &hist[t << 16];
+      for (uint64_t i = per * t; i < std::min(n, per * (t + 1)); ++i) h[(k[i] >> sh) & 0xFFFF]++;
+    });
+    uint64_t run = 0;
+    for (uint32_t d = 0; d < 65536; ++d)
+      for (int t = 0; t < nthr; ++t) {
+        uint64_t c = hist[((size_t)t << 16) + d];
+        hist[((size_t)t << 16) + d] = run;
+        run += c;
+      }
+    run_threads(nthr, [&](uint64_t t) {
+      uint64_t *h = &hist[t << 16];
+      for (uint64_t i = per * t; i < std::min(n, per * (t + 1)); ++i)
+        tmp[h[(k[i] >> sh) & 0xFFFF]++] = k[i];
+    });
+    k.swap(tmp);
+  }
+}
+
+// keys sorted ascending as (major << 32 | minor), all distinct -> CSR with unit weights
+void csr_from_sorted_keys(int64_t N, const std::vector<uint64_t> &k, int32_t *ptr, int32_t *adj,
+                          int32_t *w) {
+  std::fill(ptr, ptr + N + 1, 0);
+  for (uint64_t x : k) ptr[(x >> 32) + 1]++;
+  for (int64_t v = 0; v < N; ++v) ptr[v + 1] += ptr[v];
+  for (size_t i = 0; i < k.size(); ++i) {
+    adj[i] = (int32_t)(uint32_t)k[i];
+    w[i] = 1;
   }
 }
 
@@ -182,13 +255,10 @@ int gsap_generate_dcsbm_i32(const gsap_dcsbm_params *p, int64_t *E_out, int32_t 
   const uint64_t tot_out = cdf_out[N - 1];
   const uint32_t thr = (uint32_t)std::min(4294967295.0, std::floor(p->rho / (1.0 + p->rho) * 4294967296.0));
   // 4. edges: source ~ theta_out, destination block intra w.p. rho/(1+rho) else uniform
-  //    over the other blocks, destination ~ theta_in inside it; no self-loops, no duplicates
-  std::vector<int32_t> src, dst;
-  src.reserve(E); dst.reserve(E);
-  PairSet seen(E);
-  const uint64_t max_attempts = (uint64_t)E * 200 + 100000;
-  for (uint64_t a = 0; (int64_t)src.size() < E; ++a) {
-    if (a >= max_attempts) return GSAP_ERR_INVALID;
+  //    over the other blocks, destination ~ theta_in inside it; no self-loops, no duplicates.
+  //    Attempt a depends only on (seed, a), so candidates are drawn in parallel chunks and
+  //    then accepted in attempt order: the first E distinct non-loop pairs.
+  auto candidate = [&](uint64_t a) -> uint64_t {
     Philox u = g.at((uint32_t)a, 4, (uint32_t)(a >> 32));
     int64_t s = pick_cdf(cdf_out.data(), N, mulhi64(u.x0, tot_out));
     int32_t bs = blk[s], bd = bs;
@@ -198,26 +268,36 @@ int gsap_generate_dcsbm_i32(const gsap_dcsbm_params *p, int64_t *E_out, int32_t 
     }
     const std::vector<uint64_t> &ci = cdf_in[bd];
     int64_t t = members[bd][pick_cdf(ci.data(), (int64_t)ci.size(), mulhi64(u.x3, ci.back()))];
-    if (t == s) continue;
-    if (!seen.insert(((uint64_t)s << 32) | (uint64_t)t)) continue;
-    src.push_back((int32_t)s);
-    dst.push_back((int32_t)t);
+    return t == s ? ~0ull : (((uint64_t)s << 32) | (uint64_t)t);
+  };
+  const int nthr = host_threads();
+  std::vector<uint64_t> keys;
+  keys.reserve(E);
+  {
+    PairSet seen(E);
+    const uint64_t max_attempts = (uint64_t)E * 200 + 100000;
+    const uint64_t chunk = std::max<uint64_t>(1 << 16, (uint64_t)E / 8);
+    std::vector<uint64_t> cand(chunk);
+    for (uint64_t a0 = 0; (int64_t)keys.size() < E; a0 += chunk) {
+      if (a0 >= max_attempts) return GSAP_ERR_INVALID;
+      parallel_for(nthr, chunk, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t i = lo; i < hi; ++i) cand[i] = candidate(a0 + i);
+      });
+      for (uint64_t i = 0; i < chunk && (int64_t)keys.size() < E; ++i) {
+        if (a0 + i >= max_attempts) return GSAP_ERR_INVALID;
+        if (cand[i] == ~0ull || !seen.insert(cand[i])) continue;
+        keys.push_back(cand[i]);
+      }
+    }
   }
   // 5. CSR views, each row ordered by neighbour id
   {
-    std::vector<int64_t> ord(E);
-    std::iota(ord.begin(), ord.end(), 0);
-    std::sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
-      return src[a] != src[b] ? src[a] < src[b] : dst[a] < dst[b];
-    });
-    std::vector<int32_t> s2(E), d2(E), w1(E, 1);
-    for (int64_t i = 0; i < E; ++i) { s2[i] = src[ord[i]]; d2[i] = dst[ord[i]]; }
-    build_csr(N, s2, d2, w1, out_ptr, out_adj, out_w);
-    std::sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
-      return dst[a] != dst[b] ? dst[a] < dst[b] : src[a] < src[b];
-    });
-    for (int64_t i = 0; i < E; ++i) { s2[i] = src[ord[i]]; d2[i] = dst[ord[i]]; }
-    build_csr(N, d2, s2, w1, in_ptr, in_adj, in_w);
+    std::vector<uint64_t> tmp(E);
+    radix_sort_u64(keys, tmp, nthr);
+    csr_from_sorted_keys(N, keys, out_ptr, out_adj, out_w);
+    for (int64_t i = 0; i < E; ++i) keys[i] = (keys[i] << 32) | (keys[i] >> 32);
+    radix_sort_u64(keys, tmp, nthr);
+    csr_from_sorted_keys(N, keys, in_ptr, in_adj, in_w);
   }
   if (truth) std::copy(blk.begin(), blk.end(), truth);
   return GSAP_OK;
