@@ -140,6 +140,30 @@ def cpu_baseline(g, labels, B, seconds):
                        f"(B={B}), {el:.1f} s")
 
 
+def compare_forms(ctx, g, lab, prec):
+    """Plain FP64 form against the mixed form (FP32 delta-S with FP64 for accepted and
+    near-threshold proposals): one untimed and one timed partition of the other form, no
+    event profiling in either timed run."""
+    other = gsap_amd.PRECISION_MIXED if prec == gsap_amd.PRECISION_FP64 else gsap_amd.PRECISION_FP64
+    walls, res = {}, {}
+    for form in (prec, other):
+        if form == other:
+            ctx.partition(g, gsap_amd.Options(seed=1, precision=form))
+        t = time.perf_counter()
+        res[form] = ctx.partition(g, gsap_amd.Options(seed=1, precision=form))
+        walls[form] = time.perf_counter() - t
+        print(f"form {'fp64' if form == 0 else 'mixed'}: {walls[form]:.3f} s", file=sys.stderr,
+              flush=True)
+    F, M = gsap_amd.PRECISION_FP64, gsap_amd.PRECISION_MIXED
+    (l64, r64), (l32, r32) = res[F], res[M]
+    return {"fp64_s": walls[F], "mixed_s": walls[M], "mixed_over_fp64": walls[M] / walls[F],
+            "identical_partition": bool(np.array_equal(l64, l32)),
+            "identical_S": bool(np.float64(r64.S).tobytes() == np.float64(r32.S).tobytes()),
+            "mixed_max_abs_err_fp32": r32.max_abs_err_fp32, "mixed_rechecks": r32.rechecks,
+            "mixed_fp64_evals": r32.fp64_evals,
+            "nodal_proposals": int(r32.batches * (g.N // 4))}
+
+
 def c4_graphs(n=256):
     """256 independent DC-SBM graphs, N log-uniform in [500, 50000], K = round(N^(1/3)),
     Dirichlet(1) sizes, intra/inter ratio 4 (even g) or 1.5 (odd g), seed 4 + g."""
@@ -208,6 +232,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3", choices=list(CONFIGS) + ["c4", "c5"])
     ap.add_argument("--chains", type=int, default=16)
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the FP64-vs-mixed comparison run")
     ap.add_argument("--workers", type=int, default=0)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -309,6 +335,8 @@ def main():
                      "traffic": traffic, "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": abytes},
     }
+    if not a.no_compare:
+        out["precision_forms"] = compare_forms(ctx, g, lab, prec)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, lab, rep.B, a.cpu_seconds)
     ctx.close()

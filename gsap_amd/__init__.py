@@ -34,6 +34,7 @@ class GsapError(RuntimeError):
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 
 
 class CGraph(C.Structure):
@@ -60,7 +61,7 @@ class CReport(C.Structure):
                 ("t_rebuild", C.c_double), ("sweep_edges", C.c_int64),
                 ("max_abs_err_fp32", C.c_double), ("rechecks", C.c_int64),
                 ("n_trace", C.c_int64), ("kernel_ms", C.c_double * 8),
-                ("kernel_calls", C.c_int64 * 8)]
+                ("kernel_calls", C.c_int64 * 8), ("fp64_evals", C.c_int64)]
 
 
 class CTrace(C.Structure):
@@ -79,7 +80,7 @@ EXPORTS = [
     "gsap_partition_i32", "gsap_get_trace", "gsap_blockmodel_i32", "gsap_merge_phase_i32",
     "gsap_nodal_batch_i32", "gsap_mdl_i32", "gsap_generate_dcsbm_i32", "gsap_read_tsv_i32",
     "gsap_pairwise_i32", "gsap_log2", "gsap_log2_tables", "gsap_partition_batch_i32",
-    "gsap_partition_chains_i32",
+    "gsap_partition_chains_i32", "gsap_log2f_n",
 ]
 
 _lib = None
@@ -126,6 +127,8 @@ def lib() -> C.CDLL:
     L.gsap_log2.restype = C.c_double
     L.gsap_log2_tables.argtypes = [_f64p, _f64p, _f64p]
     L.gsap_log2_tables.restype = None
+    L.gsap_log2f_n.argtypes = [_f32p, _f32p, C.c_int64]
+    L.gsap_log2f_n.restype = None
     _lib = L
     return L
 
@@ -219,12 +222,13 @@ class Report:
     n_trace: int
     kernel_ms: list = field(default_factory=list)
     kernel_calls: list = field(default_factory=list)
+    fp64_evals: int = 0
 
     @staticmethod
     def from_c(r: CReport) -> "Report":
         return Report(r.B, r.levels, r.merge_phases, r.sweeps, r.batches, r.S, r.t_total,
                       r.t_merge, r.t_sweep, r.sweep_edges, r.max_abs_err_fp32, r.rechecks,
-                      r.n_trace, list(r.kernel_ms), list(r.kernel_calls))
+                      r.n_trace, list(r.kernel_ms), list(r.kernel_calls), r.fp64_evals)
 
 
 def _check(rc: int, ctx=None):
@@ -391,6 +395,14 @@ def pairwise(truth: np.ndarray, given: np.ndarray):
 
 def log2(x: float) -> float:
     return lib().gsap_log2(float(x))
+
+
+def log2f(x) -> np.ndarray:
+    """FP32 log2 of the mixed form (x >= 1), evaluated on the host by the device code."""
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(x)
+    lib().gsap_log2f_n(x, y, len(x))
+    return y
 
 
 def log2_tables():

@@ -107,6 +107,61 @@ GSAP_D double term(int64_t w, double la, double lb) {
   return dw * (clog2(dw) - (la + lb));
 }
 
+// ----------------------------------------------------- FP32 log2 and terms
+// Mixed-precision form: the same table split in float (entries rounded to nearest),
+// log2(1+r) = r/ln2 (1 - r/2 + r^2/3 - r^3/4), |r| < 2^-7 (truncation < 2^-36).
+// For x >= 1 (integer counts) the error against the exact log2 is at most
+// (2 |log2 x| + 6) 2^-24, including the rounding of an integer x to float.
+static __constant__ float d_log2_inv32[128] = {GSAP_LOG2_INV_VALUES};
+static __constant__ float d_log2_lg32[128] = {GSAP_LOG2_LG_VALUES};
+static const float h_log2_inv32[128] = {GSAP_LOG2_INV_VALUES};
+static const float h_log2_lg32[128] = {GSAP_LOG2_LG_VALUES};
+
+GSAP_HD float flog2_core(float x, const float *inv, const float *lg) {
+  const uint32_t bits = __builtin_bit_cast(uint32_t, x);
+  const int e = (int)((bits >> 23) & 0xff) - 127;  // x >= 1: normal
+  const int j = (int)((bits >> 16) & 127);
+  const float m = __builtin_bit_cast(float, (bits & 0x007FFFFFu) | 0x3F800000u);
+  const float r = __builtin_fmaf(m, inv[j], -1.0f);
+  float p = __builtin_fmaf(r, -0.25f, 0.33333334f);
+  p = __builtin_fmaf(p, r, -0.5f);
+  p = __builtin_fmaf(p, r, 1.0f);
+  p = (p * r) * 1.44269504f;  // 1/ln 2
+  return ((float)e + lg[j]) + p;
+}
+GSAP_D float flog2(float x) { return flog2_core(x, d_log2_inv32, d_log2_lg32); }
+inline float flog2_host(float x) { return flog2_core(x, h_log2_inv32, h_log2_lg32); }
+
+// FP32 term w * (log2 w - (la + lb)) and its error weight: with u = 2^-24 the rounding
+// error of the returned value is at most 6 u w (|log2 w| + |la| + |lb| + 1); eb
+// accumulates w (|log2 w| + |la| + |lb| + 1).
+GSAP_D float term32(int64_t w, float la, float lb, float &eb) {
+  const float fw = (float)w;
+  const float L = flog2(fw);
+  eb += fw * (((__builtin_fabsf(L) + __builtin_fabsf(la)) + __builtin_fabsf(lb)) + 1.0f);
+  return fw * (L - (la + lb));
+}
+
+// Kahan-compensated float accumulator; combine() merges two of them through TwoSum.
+struct Kahan {
+  float s, c;
+  GSAP_HD void init() { s = 0.0f; c = 0.0f; }
+  GSAP_HD void add(float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  GSAP_HD void combine(float s2, float c2) {
+    const float t = s + s2;
+    const float bp = t - s;
+    const float err = (s - (t - bp)) + (s2 - bp);  // s + s2 = t + err exactly
+    s = t;
+    c = (c + c2) - err;
+  }
+  GSAP_HD float value() const { return s - c; }
+};
+
 // ------------------------------------------------------- canonical group sum
 template <int G>
 struct CAcc {

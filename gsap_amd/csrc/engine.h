@@ -69,7 +69,9 @@ void launch_block_terms(const BMView &m, double *OUT, int32_t *sof, uint8_t *coi
 void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const int32_t *sof,
                   const uint8_t *coin, RngKey key, uint32_t phase, int64_t lo, int64_t BS,
                   const int32_t *const lists[4], const int32_t counts[4], uint32_t big_bits,
-                  int32_t *prop, double *dS, hipStream_t st);
+                  int32_t *prop, double *dS, float *err, uint8_t *near, hipStream_t st);
+void launch_mix_stats(const float *err, const uint8_t *near, int64_t n, uint32_t *max_bits,
+                      unsigned long long *count, hipStream_t st);
 void launch_apply(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo, int64_t BS,
                   hipStream_t st);
 // canonical array sum: tiles of 1024 (n read from n_dev when non-null, else n_host)

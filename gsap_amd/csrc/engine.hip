@@ -152,6 +152,8 @@ struct gsap_ctx {
   DBuf k0, k1, v0, v1, flags, uidx, tmp, cnt, partials, vhist, vperm;
   // sweep / merge
   DBuf OUT, sof, coin, prop, dS, best, dSbest, map, terms, tiles, res;
+  // mixed form: per-vertex |dS32 - dS64| and evaluation class; batch statistics
+  DBuf errv, evcls, mixstat;
   // per batch, the batch's vertices split by neighbour count (<= 16, <= 32, <= 128, larger)
   DBuf vlist;
   std::vector<int64_t> list_off;  // [batch * 4 + class] -> offset into vlist; last = total
@@ -307,6 +309,7 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   c->best.ensure(4 * N); c->dSbest.ensure(8 * N); c->map.ensure(4 * N);
   c->terms.ensure(8 * E); c->tiles.ensure(8 * (std::max(E, N) / 1024 + 2));
   c->res.ensure(8 * 64);
+  c->errv.ensure(4 * N); c->evcls.ensure(N); c->mixstat.ensure(32);
   BMBuf &b = c->bm;
   b.rowptr = c->rowptr.as<int32_t>(); b.colptr = c->colptr.as<int32_t>();
   b.col = c->col.as<int32_t>(); b.val = c->val.as<int32_t>(); b.erow = c->erow.as<int32_t>();
@@ -461,9 +464,16 @@ void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, uint32_t phase, 
     lists[k] = c->vlist.as<int32_t>() + off;
     counts[k] = (int32_t)(end - off);
   }
+  const bool mixed = o.precision == GSAP_PRECISION_MIXED;
+  float *err = mixed ? c->errv.as<float>() : nullptr;
+  uint8_t *evcls = mixed ? c->evcls.as<uint8_t>() : nullptr;
   launch_nodal(c->g, c->lab.as<int32_t>(), m, c->sof.as<int32_t>(), c->coin.as<uint8_t>(),
                rng_key(o), phase, lo, BS, lists, counts, c->big_bits[b], c->prop.as<int32_t>(),
-               c->dS.as<double>(), st);
+               c->dS.as<double>(), err, evcls, st);
+  if (mixed) {
+    uint32_t *ms = c->mixstat.as<uint32_t>();
+    launch_mix_stats(err, evcls, BS, ms, reinterpret_cast<unsigned long long *>(ms + 2), st);
+  }
   c->prof.end(p, st);
   p = c->prof.begin(5, st);
   launch_apply(c->lab.as<int32_t>(), c->prop.as<int32_t>(), c->dS.as<double>(), lo, BS, st);
@@ -507,6 +517,7 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
   int32_t *lab = c->lab.as<int32_t>();
   double *res = c->res.as<double>();
   launch_iota(lab, N, st);
+  HIPX(hipMemsetAsync(c->mixstat.p, 0, 32, st));
   int32_t B = (int32_t)N;
   int32_t B_to_merge = (int32_t)((float)B * o.reduction_rate);
   rebuild(c, B);
@@ -631,6 +642,8 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
     if (o.max_levels > 0 && levels >= o.max_levels) break;
   }
   HIPX(hipMemcpyAsync(out, lab, 4 * (size_t)N, hipMemcpyDeviceToHost, st));
+  uint32_t mix[8] = {0};
+  HIPX(hipMemcpyAsync(mix, c->mixstat.p, 32, hipMemcpyDeviceToHost, st));
   HIPX(hipStreamSynchronize(st));
   c->prof.flush();
   if (rep) {
@@ -641,6 +654,13 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
     rep->sweep_edges = c->E * (int64_t)sweeps;
     rep->n_trace = (int64_t)c->trace.size();
     for (int k = 0; k < 8; ++k) { rep->kernel_ms[k] = c->prof.ms[k]; rep->kernel_calls[k] = c->prof.calls[k]; }
+    float mx;
+    std::memcpy(&mx, &mix[0], 4);
+    uint64_t cnt[2];
+    std::memcpy(cnt, &mix[2], 16);
+    rep->max_abs_err_fp32 = (double)mx;
+    rep->rechecks = (int64_t)cnt[0];
+    rep->fp64_evals = (int64_t)cnt[1];
   }
 }
 
@@ -762,7 +782,7 @@ int gsap_destroy(gsap_ctx *c) {
                  &c->k1, &c->v0, &c->v1, &c->flags, &c->uidx, &c->tmp, &c->cnt, &c->partials,
                  &c->vhist, &c->vperm, &c->vlist,
                  &c->OUT, &c->sof, &c->coin, &c->prop, &c->dS, &c->best, &c->dSbest, &c->map,
-                 &c->terms, &c->tiles, &c->res};
+                 &c->terms, &c->tiles, &c->res, &c->errv, &c->evcls, &c->mixstat};
   for (DBuf *b : all) b->release();
   try { c->prof.destroy(); } catch (...) {}
   (void)hipStreamDestroy(c->st);

@@ -174,6 +174,10 @@ void gsap_log2_tables(double *inv_c, double *log2_c, double *poly) {
   std::memcpy(poly, h_log2_poly, sizeof h_log2_poly);
 }
 
+void gsap_log2f_n(const float *x, float *y, int64_t n) {
+  for (int64_t k = 0; k < n; ++k) y[k] = flog2_host(x[k]);
+}
+
 int gsap_generate_dcsbm_i32(const gsap_dcsbm_params *p, int64_t *E_out, int32_t *out_ptr,
                             int32_t *out_adj, int32_t *out_w, int32_t *in_ptr, int32_t *in_adj,
                             int32_t *in_w, int32_t *truth) {

@@ -164,7 +164,35 @@ struct NodalArgs {
   const int64_t *hoff;
   const uint8_t *hbits_o, *hbits_i;
   int32_t *hpool;
+  float *err;      // mixed form: |dS32 - dS64| where FP64 was evaluated, else 0
+  uint8_t *near;   // mixed form: 0 FP32 rejection, 1 FP64 evaluated, 2 |dS32| within bound
 };
+
+constexpr float U32 = 5.9604645e-08f;  // 2^-24
+
+// Final delta-S of one proposal (calculate_dS_overall_nodal, kernels.hpp:600-643): the
+// row and column sums minus the untouched-entry terms, in this order.
+GSAP_D double nodal_total(double drow, double dcol, double Y_r, double Y_s, double Y2_r,
+                          double Y2_s) {
+  double d = drow - Y_r;
+  d = d - Y_s;
+  d = d + dcol;
+  d = d - Y2_r;
+  d = d - Y2_s;
+  return d;
+}
+
+// Mixed form, first pass: true when the FP32 delta-S proves a rejection (dS64 >= 0).
+// eb is the summed error weight of the FP32 terms; 16 u eb bounds |dS32 - dS64| with
+// margin (the FP32 term, slot and Kahan roundings need 11 u eb), plus the FP64
+// roundings of the final combination.
+GSAP_D bool mixed_rejects(double d32, float eb, double Y_r, double Y_s, double Y2_r,
+                          double Y2_s, double &bound) {
+  bound = 16.0 * (double)U32 * (double)eb +
+          1.8e-15 * ((((__builtin_fabs(d32) + __builtin_fabs(Y_r)) + __builtin_fabs(Y_s)) +
+                      __builtin_fabs(Y2_r)) + __builtin_fabs(Y2_s));
+  return d32 >= bound;
+}
 
 // One wave per vertex v of the batch: proposal (kernels.hpp:877-939, 1000-1032) and delta-S
 // against the batch-start block matrix (kernels.hpp:494-643, 1153-1726, node-only entries
@@ -173,6 +201,7 @@ struct NodalArgs {
 // the untouched entries contribute (log2 d_old - log2 d_new) * (their summed count).
 // Labels are not written here (batch-Jacobi).  Hash tables of 2^lbits slots live in LDS;
 // a vertex whose adjacency needs more uses its tables in the global pool.
+template <bool MIX>
 __global__ __launch_bounds__(64, 6) void k_nodal(NodalArgs a, const int32_t *list,
                                                  uint32_t lbits) {
   extern __shared__ int32_t lds[];
@@ -201,7 +230,13 @@ __global__ __launch_bounds__(64, 6) void k_nodal(NodalArgs a, const int32_t *lis
   }
   if (lane == 0) a.prop[i] = s;
   if (s == r) {
-    if (lane == 0) a.dS[i] = 0.0;
+    if (lane == 0) {
+      a.dS[i] = 0.0;
+      if constexpr (MIX) {
+        a.err[i] = 0.0f;
+        a.near[i] = 0;
+      }
+    }
     return;
   }
 
@@ -248,11 +283,9 @@ __global__ __launch_bounds__(64, 6) void k_nodal(NodalArgs a, const int32_t *lis
     corner = bm_col_get(m, cc, rr);
   }
 
-  // rows r and s over the row slots
-  double p_row = 0.0;
-  int64_t sum_r = 0, sum_s = 0;
-  for (int32_t j = lane; j < no + 2; j += 64) {
-    int32_t t;
+  // rows r and s over the row slots; gathers shared by the FP32 and FP64 passes
+  auto row_slot = [&](int32_t j, int32_t &t, int64_t &Mr, int64_t &Ms, int64_t &R,
+                      int64_t &S) -> bool {
     int64_t av = 0;
     bool use;
     if (j < no) {
@@ -267,64 +300,149 @@ __global__ __launch_bounds__(64, 6) void k_nodal(NodalArgs a, const int32_t *lis
       t = s;
       use = !s_in_a;
     }
-    if (!use) continue;
+    if (!use) return false;
     int32_t Mr32, Ms32;
     bm_row_get2(m, r, s, t, Mr32, Ms32);
-    const int64_t Mr = Mr32, Ms = Ms32;
-    const double ldt = m.ldin[t];
-    const double or_ = Mr > 0 ? term(Mr, ldo_r, ldt) : 0.0;
-    const double os_ = Ms > 0 ? term(Ms, ldo_s, ldt) : 0.0;
-    const int64_t R = Mr - av - (t == r ? c_r : 0) + (t == s ? c_r : 0);
-    const int64_t S = Ms + av - (t == r ? c_s : 0) + (t == s ? c_s : 0);
-    const double nr = R > 0 ? term(R, ldt, l_dr) : 0.0;
-    const double ns = S > 0 ? term(S, ldt, l_ds) : 0.0;
-    p_row += ((or_ + os_) - nr) - ns;
-    sum_r += Mr;
-    sum_s += Ms;
-  }
+    Mr = Mr32;
+    Ms = Ms32;
+    R = Mr - av - (t == r ? c_r : 0) + (t == s ? c_r : 0);
+    S = Ms + av - (t == r ? c_s : 0) + (t == s ? c_s : 0);
+    return true;
+  };
   // columns r and s over the column slots (t not in {r, s})
-  double p_col = 0.0;
-  int64_t csum_r = 0, csum_s = 0;
-  for (int32_t j = lane; j < ni; j += 64) {
-    const int32_t t = j < 64 ? t_i0 : a.lab[a.iadj[i0 + j]];
+  auto col_slot = [&](int32_t j, int32_t &t, int64_t &Mr, int64_t &Ms, int64_t &c) -> bool {
+    t = j < 64 ? t_i0 : a.lab[a.iadj[i0 + j]];
     const int32_t slot = ht_find(hi, t);
-    if (ld_relaxed(&hi.first[slot]) != j || t == r || t == s) continue;
-    const int64_t c = ld_relaxed(&hi.w[slot]);
+    if (ld_relaxed(&hi.first[slot]) != j || t == r || t == s) return false;
+    c = ld_relaxed(&hi.w[slot]);
     int32_t Mr32, Ms32;
     bm_col_get2(m, r, s, t, Mr32, Ms32);
-    const int64_t Mr = Mr32, Ms = Ms32;
-    const double ldt = m.ldout[t];
-    const double ir = Mr > 0 ? term(Mr, ldt, ldi_r) : 0.0;
-    const double is = Ms > 0 ? term(Ms, ldt, ldi_s) : 0.0;
-    const int64_t wr = Mr - c, ws = Ms + c;
-    const double nr = wr > 0 ? term(wr, ldt, l_ir) : 0.0;
-    const double ns = ws > 0 ? term(ws, ldt, l_is) : 0.0;
-    p_col += ((ir + is) - nr) - ns;
-    csum_r += Mr;
-    csum_s += Ms;
+    Mr = Mr32;
+    Ms = Ms32;
+    return true;
+  };
+  int64_t sum_r = 0, sum_s = 0, csum_r = 0, csum_s = 0;
+  // FP64 terms (and, when sums is set, the integer sums of the touched entries)
+  auto fp64_pass = [&](double &drow, double &dcol, bool sums) {
+    double p_row = 0.0;
+    for (int32_t j = lane; j < no + 2; j += 64) {
+      int32_t t;
+      int64_t Mr, Ms, R, S;
+      if (!row_slot(j, t, Mr, Ms, R, S)) continue;
+      const double ldt = m.ldin[t];
+      const double or_ = Mr > 0 ? term(Mr, ldo_r, ldt) : 0.0;
+      const double os_ = Ms > 0 ? term(Ms, ldo_s, ldt) : 0.0;
+      const double nr = R > 0 ? term(R, ldt, l_dr) : 0.0;
+      const double ns = S > 0 ? term(S, ldt, l_ds) : 0.0;
+      p_row += ((or_ + os_) - nr) - ns;
+      if (sums) {
+        sum_r += Mr;
+        sum_s += Ms;
+      }
+    }
+    double p_col = 0.0;
+    for (int32_t j = lane; j < ni; j += 64) {
+      int32_t t;
+      int64_t Mr, Ms, c;
+      if (!col_slot(j, t, Mr, Ms, c)) continue;
+      const double ldt = m.ldout[t];
+      const double ir = Mr > 0 ? term(Mr, ldt, ldi_r) : 0.0;
+      const double is = Ms > 0 ? term(Ms, ldt, ldi_s) : 0.0;
+      const int64_t wr = Mr - c, ws = Ms + c;
+      const double nr = wr > 0 ? term(wr, ldt, l_ir) : 0.0;
+      const double ns = ws > 0 ? term(ws, ldt, l_is) : 0.0;
+      p_col += ((ir + is) - nr) - ns;
+      if (sums) {
+        csum_r += Mr;
+        csum_s += Ms;
+      }
+    }
+    drow = wave_sum(p_row);
+    dcol = wave_sum(p_col);
+  };
+  // plain form: one FP64 pass; mixed form: an FP32 pass first (with the integer sums)
+  double drow = 0.0, dcol = 0.0;
+  Kahan krow, kcol;
+  krow.init();
+  kcol.init();
+  float eb = 0.0f;
+  if constexpr (!MIX) {
+    fp64_pass(drow, dcol, true);
+  } else {
+    for (int32_t j = lane; j < no + 2; j += 64) {
+      int32_t t;
+      int64_t Mr, Ms, R, S;
+      if (!row_slot(j, t, Mr, Ms, R, S)) continue;
+      sum_r += Mr;
+      sum_s += Ms;
+      const float ldt = (float)m.ldin[t];
+      const float or_ = Mr > 0 ? term32(Mr, (float)ldo_r, ldt, eb) : 0.0f;
+      const float os_ = Ms > 0 ? term32(Ms, (float)ldo_s, ldt, eb) : 0.0f;
+      const float nr = R > 0 ? term32(R, ldt, (float)l_dr, eb) : 0.0f;
+      const float ns = S > 0 ? term32(S, ldt, (float)l_ds, eb) : 0.0f;
+      krow.add(((or_ + os_) - nr) - ns);
+    }
+    for (int32_t j = lane; j < ni; j += 64) {
+      int32_t t;
+      int64_t Mr, Ms, c;
+      if (!col_slot(j, t, Mr, Ms, c)) continue;
+      csum_r += Mr;
+      csum_s += Ms;
+      const float ldt = (float)m.ldout[t];
+      const float ir = Mr > 0 ? term32(Mr, ldt, (float)ldi_r, eb) : 0.0f;
+      const float is = Ms > 0 ? term32(Ms, ldt, (float)ldi_s, eb) : 0.0f;
+      const int64_t wr = Mr - c, ws = Ms + c;
+      const float nr = wr > 0 ? term32(wr, ldt, (float)l_ir, eb) : 0.0f;
+      const float ns = ws > 0 ? term32(ws, ldt, (float)l_is, eb) : 0.0f;
+      kcol.add(((ir + is) - nr) - ns);
+    }
   }
-  const double drow = wave_sum(p_row), dcol = wave_sum(p_col);
   sum_r = wave_sum_i64(sum_r);
   sum_s = wave_sum_i64(sum_s);
   csum_r = wave_sum_i64(csum_r);
   csum_s = wave_sum_i64(csum_s);
   const int64_t M_rr = __shfl(corner, 0, 64), M_sr = __shfl(corner, 1, 64);
   const int64_t M_rs = __shfl(corner, 2, 64), M_ss = __shfl(corner, 3, 64);
+  const int64_t W_r = dout_r - sum_r, W_s = dout_s - sum_s;
+  const double Y_r = W_r > 0 ? (ldo_r - l_dr) * (double)W_r : 0.0;
+  const double Y_s = W_s > 0 ? (ldo_s - l_ds) * (double)W_s : 0.0;
+  const int64_t W2_r = din_r - M_rr - M_sr - csum_r;
+  const int64_t W2_s = din_s - M_rs - M_ss - csum_s;
+  const double Y2_r = W2_r > 0 ? (ldi_r - l_ir) * (double)W2_r : 0.0;
+  const double Y2_s = W2_s > 0 ? (ldi_s - l_is) * (double)W2_s : 0.0;
+  double d32 = 0.0, bound = 0.0;
+  if constexpr (MIX) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      krow.combine(__shfl_down(krow.s, off, 64), __shfl_down(krow.c, off, 64));
+      kcol.combine(__shfl_down(kcol.s, off, 64), __shfl_down(kcol.c, off, 64));
+      eb += __shfl_down(eb, off, 64);
+    }
+    int rej = 0;
+    if (lane == 0) {
+      d32 = nodal_total((double)krow.value(), (double)kcol.value(), Y_r, Y_s, Y2_r, Y2_s);
+      rej = mixed_rejects(d32, eb, Y_r, Y_s, Y2_r, Y2_s, bound);
+    }
+    if (__shfl(rej, 0, 64)) {
+      if (lane == 0) {
+        a.dS[i] = 0.0;
+        a.err[i] = 0.0f;
+        a.near[i] = 0;
+      }
+      return;
+    }
+    fp64_pass(drow, dcol, false);
+  }
   if (lane == 0) {
-    const int64_t W_r = dout_r - sum_r, W_s = dout_s - sum_s;
-    const double Y_r = W_r > 0 ? (ldo_r - l_dr) * (double)W_r : 0.0;
-    const double Y_s = W_s > 0 ? (ldo_s - l_ds) * (double)W_s : 0.0;
-    const int64_t W2_r = din_r - M_rr - M_sr - csum_r;
-    const int64_t W2_s = din_s - M_rs - M_ss - csum_s;
-    const double Y2_r = W2_r > 0 ? (ldi_r - l_ir) * (double)W2_r : 0.0;
-    const double Y2_s = W2_s > 0 ? (ldi_s - l_is) * (double)W2_s : 0.0;
-    // calculate_dS_overall_nodal (kernels.hpp:600-643): accept iff dS < 0
-    double d = drow - Y_r;
-    d = d - Y_s;
-    d = d + dcol;
-    d = d - Y2_r;
-    d = d - Y2_s;
-    a.dS[i] = d < 0.0 ? d : 0.0;
+    const double d = nodal_total(drow, dcol, Y_r, Y_s, Y2_r, Y2_s);
+    a.dS[i] = d < 0.0 ? d : 0.0;  // accept iff dS < 0
+    if constexpr (MIX) {
+      // non-finite delta-S (an empty block's -inf log, the reference's quirk) only ever
+      // sends a proposal to the FP64 pass; it is left out of the error statistic
+      const double e = __builtin_fabs(d32 - d);
+      a.err[i] = e <= 1e30 ? (float)e : 0.0f;
+      a.near[i] = d32 > -bound ? 2 : 1;
+    }
   }
 }
 
@@ -340,7 +458,7 @@ GSAP_D int64_t group_allsum(int64_t x) {
 // per vertex: lane q holds out-entry q and in-entry q; the neighbour blocks are deduplicated
 // in registers (shuffles inside the group) instead of hash tables, and the block of the
 // sampled neighbour comes from the same gathered values.
-template <int G>
+template <int G, bool MIX>
 __global__ __launch_bounds__(256, 5) void k_nodal_small(NodalArgs a, const int32_t *list,
                                                         int32_t n) {
   const int32_t gid = (int32_t)(blockIdx.x * (256 / G) + threadIdx.x / G);
@@ -386,7 +504,13 @@ __global__ __launch_bounds__(256, 5) void k_nodal_small(NodalArgs a, const int32
   }
   if (q == 0) a.prop[i] = s;
   if (s == r) {
-    if (q == 0) a.dS[i] = 0.0;
+    if (q == 0) {
+      a.dS[i] = 0.0;
+      if constexpr (MIX) {
+        a.err[i] = 0.0f;
+        a.near[i] = 0;
+      }
+    }
     return;
   }
   // per-lane block weight and first-occurrence flag
@@ -416,14 +540,8 @@ __global__ __launch_bounds__(256, 5) void k_nodal_small(NodalArgs a, const int32
     corner = bm_col_get(m, cc, rr);
   }
   // rows r and s over the row slots: positions 0..no-1 (first occurrences), no (r), no+1 (s)
-  CAcc<G> prow;
-  prow.init();
-  int64_t sum_r = 0, sum_s = 0;
-#pragma unroll
-  for (int jj = 0; jj < 64 / G; ++jj) {
-    const int32_t j = q + jj * G;
-    if (j >= no + 2) break;
-    int32_t t;
+  auto row_slot = [&](int32_t j, int32_t &t, int64_t &Mr, int64_t &Ms, int64_t &R,
+                      int64_t &S) -> bool {
     int64_t av = 0;
     bool use;
     if (j < no) {
@@ -437,68 +555,156 @@ __global__ __launch_bounds__(256, 5) void k_nodal_small(NodalArgs a, const int32
       t = s;
       use = !s_in_a;
     }
-    if (!use) continue;
+    if (!use) return false;
     int32_t Mr32, Ms32;
     bm_row_get2(m, r, s, t, Mr32, Ms32);
-    const int64_t Mr = Mr32, Ms = Ms32;
-    const double ldt = m.ldin[t];
-    const double or_ = Mr > 0 ? term(Mr, ldo_r, ldt) : 0.0;
-    const double os_ = Ms > 0 ? term(Ms, ldo_s, ldt) : 0.0;
-    const int64_t R = Mr - av - (t == r ? c_r : 0) + (t == s ? c_r : 0);
-    const int64_t S = Ms + av - (t == r ? c_s : 0) + (t == s ? c_s : 0);
-    const double nr = R > 0 ? term(R, ldt, l_dr) : 0.0;
-    const double ns = S > 0 ? term(S, ldt, l_ds) : 0.0;
-    prow.p[jj] += ((or_ + os_) - nr) - ns;
-    sum_r += Mr;
-    sum_s += Ms;
-  }
-  // columns r and s over the column slots (position q, t not in {r, s})
-  CAcc<G> pcol;
-  pcol.init();
-  int64_t csum_r = 0, csum_s = 0;
-  if (first_i && t_i != r && t_i != s) {
-    const int32_t t = t_i;
+    Mr = Mr32;
+    Ms = Ms32;
+    R = Mr - av - (t == r ? c_r : 0) + (t == s ? c_r : 0);
+    S = Ms + av - (t == r ? c_s : 0) + (t == s ? c_s : 0);
+    return true;
+  };
+  // column slot of lane q (t not in {r, s})
+  const bool col_use = first_i && t_i != r && t_i != s;
+  auto col_get = [&](int64_t &Mr, int64_t &Ms) {
     int32_t Mr32, Ms32;
-    bm_col_get2(m, r, s, t, Mr32, Ms32);
-    const int64_t Mr = Mr32, Ms = Ms32;
-    const double ldt = m.ldout[t];
-    const double ir = Mr > 0 ? term(Mr, ldt, ldi_r) : 0.0;
-    const double is = Ms > 0 ? term(Ms, ldt, ldi_s) : 0.0;
-    const int64_t wr = Mr - c_sum, ws = Ms + c_sum;
-    const double nr = wr > 0 ? term(wr, ldt, l_ir) : 0.0;
-    const double ns = ws > 0 ? term(ws, ldt, l_is) : 0.0;
-    pcol.p[0] += ((ir + is) - nr) - ns;
-    csum_r += Mr;
-    csum_s += Ms;
+    bm_col_get2(m, r, s, t_i, Mr32, Ms32);
+    Mr = Mr32;
+    Ms = Ms32;
+  };
+  int64_t sum_r = 0, sum_s = 0, csum_r = 0, csum_s = 0;
+  auto fp64_pass = [&](double &drow, double &dcol, bool sums) {
+    CAcc<G> prow;
+    prow.init();
+#pragma unroll
+    for (int jj = 0; jj < 64 / G; ++jj) {
+      const int32_t j = q + jj * G;
+      if (j >= no + 2) break;
+      int32_t t;
+      int64_t Mr, Ms, R, S;
+      if (!row_slot(j, t, Mr, Ms, R, S)) continue;
+      const double ldt = m.ldin[t];
+      const double or_ = Mr > 0 ? term(Mr, ldo_r, ldt) : 0.0;
+      const double os_ = Ms > 0 ? term(Ms, ldo_s, ldt) : 0.0;
+      const double nr = R > 0 ? term(R, ldt, l_dr) : 0.0;
+      const double ns = S > 0 ? term(S, ldt, l_ds) : 0.0;
+      prow.p[jj] += ((or_ + os_) - nr) - ns;
+      if (sums) {
+        sum_r += Mr;
+        sum_s += Ms;
+      }
+    }
+    CAcc<G> pcol;
+    pcol.init();
+    if (col_use) {
+      int64_t Mr, Ms;
+      col_get(Mr, Ms);
+      const double ldt = m.ldout[t_i];
+      const double ir = Mr > 0 ? term(Mr, ldt, ldi_r) : 0.0;
+      const double is = Ms > 0 ? term(Ms, ldt, ldi_s) : 0.0;
+      const int64_t wr = Mr - c_sum, ws = Ms + c_sum;
+      const double nr = wr > 0 ? term(wr, ldt, l_ir) : 0.0;
+      const double ns = ws > 0 ? term(ws, ldt, l_is) : 0.0;
+      pcol.p[0] += ((ir + is) - nr) - ns;
+      if (sums) {
+        csum_r += Mr;
+        csum_s += Ms;
+      }
+    }
+    drow = prow.reduce();
+    dcol = pcol.reduce();
+  };
+  double drow = 0.0, dcol = 0.0;
+  Kahan krow, kcol;
+  krow.init();
+  kcol.init();
+  float eb = 0.0f;
+  if constexpr (!MIX) {
+    fp64_pass(drow, dcol, true);
+  } else {
+#pragma unroll
+    for (int jj = 0; jj < 64 / G; ++jj) {
+      const int32_t j = q + jj * G;
+      if (j >= no + 2) break;
+      int32_t t;
+      int64_t Mr, Ms, R, S;
+      if (!row_slot(j, t, Mr, Ms, R, S)) continue;
+      sum_r += Mr;
+      sum_s += Ms;
+      const float ldt = (float)m.ldin[t];
+      const float or_ = Mr > 0 ? term32(Mr, (float)ldo_r, ldt, eb) : 0.0f;
+      const float os_ = Ms > 0 ? term32(Ms, (float)ldo_s, ldt, eb) : 0.0f;
+      const float nr = R > 0 ? term32(R, ldt, (float)l_dr, eb) : 0.0f;
+      const float ns = S > 0 ? term32(S, ldt, (float)l_ds, eb) : 0.0f;
+      krow.add(((or_ + os_) - nr) - ns);
+    }
+    if (col_use) {
+      int64_t Mr, Ms;
+      col_get(Mr, Ms);
+      csum_r += Mr;
+      csum_s += Ms;
+      const float ldt = (float)m.ldout[t_i];
+      const float ir = Mr > 0 ? term32(Mr, ldt, (float)ldi_r, eb) : 0.0f;
+      const float is = Ms > 0 ? term32(Ms, ldt, (float)ldi_s, eb) : 0.0f;
+      const int64_t wr = Mr - c_sum, ws = Ms + c_sum;
+      const float nr = wr > 0 ? term32(wr, ldt, (float)l_ir, eb) : 0.0f;
+      const float ns = ws > 0 ? term32(ws, ldt, (float)l_is, eb) : 0.0f;
+      kcol.add(((ir + is) - nr) - ns);
+    }
   }
-  const double drow = prow.reduce(), dcol = pcol.reduce();
   sum_r = group_allsum<G>(sum_r);
   sum_s = group_allsum<G>(sum_s);
   csum_r = group_allsum<G>(csum_r);
   csum_s = group_allsum<G>(csum_s);
   const int64_t M_rr = __shfl(corner, 0, G), M_sr = __shfl(corner, 1, G);
   const int64_t M_rs = __shfl(corner, 2, G), M_ss = __shfl(corner, 3, G);
+  const int64_t W_r = dout_r - sum_r, W_s = dout_s - sum_s;
+  const double Y_r = W_r > 0 ? (ldo_r - l_dr) * (double)W_r : 0.0;
+  const double Y_s = W_s > 0 ? (ldo_s - l_ds) * (double)W_s : 0.0;
+  const int64_t W2_r = din_r - M_rr - M_sr - csum_r;
+  const int64_t W2_s = din_s - M_rs - M_ss - csum_s;
+  const double Y2_r = W2_r > 0 ? (ldi_r - l_ir) * (double)W2_r : 0.0;
+  const double Y2_s = W2_s > 0 ? (ldi_s - l_is) * (double)W2_s : 0.0;
+  double d32 = 0.0, bound = 0.0;
+  if constexpr (MIX) {
+#pragma unroll
+    for (int off = G / 2; off >= 1; off >>= 1) {
+      krow.combine(__shfl_down(krow.s, off, G), __shfl_down(krow.c, off, G));
+      kcol.combine(__shfl_down(kcol.s, off, G), __shfl_down(kcol.c, off, G));
+      eb += __shfl_down(eb, off, G);
+    }
+    int rej = 0;
+    if (q == 0) {
+      d32 = nodal_total((double)krow.value(), (double)kcol.value(), Y_r, Y_s, Y2_r, Y2_s);
+      rej = mixed_rejects(d32, eb, Y_r, Y_s, Y2_r, Y2_s, bound);
+    }
+    if (__shfl(rej, 0, G)) {
+      if (q == 0) {
+        a.dS[i] = 0.0;
+        a.err[i] = 0.0f;
+        a.near[i] = 0;
+      }
+      return;
+    }
+    fp64_pass(drow, dcol, false);
+  }
   if (q == 0) {
-    const int64_t W_r = dout_r - sum_r, W_s = dout_s - sum_s;
-    const double Y_r = W_r > 0 ? (ldo_r - l_dr) * (double)W_r : 0.0;
-    const double Y_s = W_s > 0 ? (ldo_s - l_ds) * (double)W_s : 0.0;
-    const int64_t W2_r = din_r - M_rr - M_sr - csum_r;
-    const int64_t W2_s = din_s - M_rs - M_ss - csum_s;
-    const double Y2_r = W2_r > 0 ? (ldi_r - l_ir) * (double)W2_r : 0.0;
-    const double Y2_s = W2_s > 0 ? (ldi_s - l_is) * (double)W2_s : 0.0;
-    double d = drow - Y_r;
-    d = d - Y_s;
-    d = d + dcol;
-    d = d - Y2_r;
-    d = d - Y2_s;
+    const double d = nodal_total(drow, dcol, Y_r, Y_s, Y2_r, Y2_s);
     a.dS[i] = d < 0.0 ? d : 0.0;
+    if constexpr (MIX) {
+      // non-finite delta-S (an empty block's -inf log, the reference's quirk) only ever
+      // sends a proposal to the FP64 pass; it is left out of the error statistic
+      const double e = __builtin_fabs(d32 - d);
+      a.err[i] = e <= 1e30 ? (float)e : 0.0f;
+      a.near[i] = d32 > -bound ? 2 : 1;
+    }
   }
 }
 
 void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const int32_t *sof,
                   const uint8_t *coin, RngKey key, uint32_t phase, int64_t lo, int64_t BS,
                   const int32_t *const lists[4], const int32_t counts[4], uint32_t big_bits,
-                  int32_t *prop, double *dS, hipStream_t st) {
+                  int32_t *prop, double *dS, float *err, uint8_t *near, hipStream_t st) {
   if (BS <= 0) return;
   NodalArgs a;
   a.lo = lo; a.BS = BS;
@@ -508,15 +714,66 @@ void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const 
   a.m = m; a.sof = sof; a.coin = coin; a.key = key; a.phase = phase;
   a.prop = prop; a.dS = dS;
   a.hoff = g.hoff; a.hbits_o = g.hbits_o; a.hbits_i = g.hbits_i; a.hpool = g.hpool;
+  a.err = err; a.near = near;
+  if (err) {
+    if (counts[0] > 0)
+      k_nodal_small<16, true><<<grid_for(counts[0], 16), 256, 0, st>>>(a, lists[0], counts[0]);
+    if (counts[1] > 0)
+      k_nodal_small<32, true><<<grid_for(counts[1], 8), 256, 0, st>>>(a, lists[1], counts[1]);
+    if (counts[2] > 0)
+      k_nodal<true><<<(unsigned)counts[2], 64, 6 * 4 * (size_t)LCAP, st>>>(a, lists[2],
+                                                                          (uint32_t)LBITS);
+    if (counts[3] > 0)
+      k_nodal<true><<<(unsigned)counts[3], 64, 6 * 4 * ((size_t)1 << big_bits), st>>>(
+          a, lists[3], big_bits);
+    return;
+  }
   if (counts[0] > 0)
-    k_nodal_small<16><<<grid_for(counts[0], 16), 256, 0, st>>>(a, lists[0], counts[0]);
+    k_nodal_small<16, false><<<grid_for(counts[0], 16), 256, 0, st>>>(a, lists[0], counts[0]);
   if (counts[1] > 0)
-    k_nodal_small<32><<<grid_for(counts[1], 8), 256, 0, st>>>(a, lists[1], counts[1]);
+    k_nodal_small<32, false><<<grid_for(counts[1], 8), 256, 0, st>>>(a, lists[1], counts[1]);
   if (counts[2] > 0)
-    k_nodal<<<(unsigned)counts[2], 64, 6 * 4 * (size_t)LCAP, st>>>(a, lists[2], (uint32_t)LBITS);
+    k_nodal<false><<<(unsigned)counts[2], 64, 6 * 4 * (size_t)LCAP, st>>>(a, lists[2],
+                                                                         (uint32_t)LBITS);
   if (counts[3] > 0)
-    k_nodal<<<(unsigned)counts[3], 64, 6 * 4 * ((size_t)1 << big_bits), st>>>(a, lists[3],
-                                                                             big_bits);
+    k_nodal<false><<<(unsigned)counts[3], 64, 6 * 4 * ((size_t)1 << big_bits), st>>>(
+        a, lists[3], big_bits);
+}
+
+// Mixed-form statistics of one batch: max |dS32 - dS64| (float bits, order-preserving for
+// values >= 0), the count of near-threshold decisions (class 2) and of FP64 evaluations
+// (classes 1 and 2).
+__global__ __launch_bounds__(256) void k_mix_stats(const float *err, const uint8_t *cls,
+                                                   int64_t n, uint32_t *max_bits,
+                                                   unsigned long long *count) {
+  uint32_t mx = 0;
+  uint32_t c_near = 0, c_eval = 0;
+  for (int64_t k = blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256) {
+    const uint32_t b = __builtin_bit_cast(uint32_t, err[k]);
+    mx = b > mx ? b : mx;
+    c_near += cls[k] == 2;
+    c_eval += cls[k] != 0;
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const uint32_t y = __shfl_down(mx, off, 64);
+    mx = y > mx ? y : mx;
+    c_near += __shfl_down(c_near, off, 64);
+    c_eval += __shfl_down(c_eval, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (mx) atomicMax(max_bits, mx);
+    if (c_near) atomicAdd(count, (unsigned long long)c_near);
+    if (c_eval) atomicAdd(count + 1, (unsigned long long)c_eval);
+  }
+}
+
+void launch_mix_stats(const float *err, const uint8_t *near, int64_t n, uint32_t *max_bits,
+                      unsigned long long *count, hipStream_t st) {
+  if (n <= 0) return;
+  unsigned grid = (unsigned)((n + 255) / 256);
+  if (grid > 1024) grid = 1024;
+  k_mix_stats<<<grid, 256, 0, st>>>(err, near, n, max_bits, count);
 }
 
 __global__ void k_apply(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo,

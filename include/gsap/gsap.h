@@ -97,12 +97,15 @@ typedef struct gsap_report {
   double t_sweep;         /* sum over sweeps of sweep wall time */
   double t_rebuild;       /* block-matrix rebuild time inside sweeps (if profiled) */
   int64_t sweep_edges;    /* E * sweeps */
-  double max_abs_err_fp32;/* mixed form: max |dS32 - dS64| seen on rechecked moves */
-  int64_t rechecks;       /* mixed form: decisions recomputed in FP64 */
+  double max_abs_err_fp32;/* mixed form: max |dS32 - dS64| over proposals evaluated in FP64 */
+  int64_t rechecks;       /* mixed form: decisions whose FP32 delta-S lay within its error
+                             bound of 0, taken from the FP64 recomputation */
   int64_t n_trace;        /* trace records produced (may exceed capacity) */
   double kernel_ms[8];    /* device time per kernel class when options.profile = 1:     */
   int64_t kernel_calls[8];/* 0 nodal moves, 1 block-matrix rebuild, 2 per-block terms,   */
                           /* 3 merge proposals+delta-S, 4 MDL, 5 accept+batch sum        */
+  int64_t fp64_evals;     /* mixed form: nodal proposals evaluated in FP64 (rechecks plus
+                             every accepted move, whose exact delta-S enters the batch sum) */
 } gsap_report;
 
 /* One trace record; identical layout to the oracle's. */
@@ -203,6 +206,9 @@ int gsap_pairwise_i32(int64_t N, const int32_t *truth, const int32_t *given, dou
 double gsap_log2(double x);
 /* Its constants: inv_c[128], log2_c[128], poly[9]. */
 void gsap_log2_tables(double *inv_c, double *log2_c, double *poly);
+/* FP32 log2 of the mixed form (host evaluation of the device code), y[k] = log2(x[k]) for
+ * x[k] >= 1. */
+void gsap_log2f_n(const float *x, float *y, int64_t n);
 
 #ifdef __cplusplus
 }

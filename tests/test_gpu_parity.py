@@ -202,3 +202,57 @@ def test_partition_chains(ctx):
         assert eq_bits(S[k], refs[k][1].S)
     assert best == int(np.argmin([r[1].S for r in refs]))
     assert np.array_equal(lab, refs[best][0]) and eq_bits(rep.S, S[best])
+
+
+MIXED = gsap_amd.PRECISION_MIXED
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("Bfrac,dense", [(1.0, 0), (0.15, 0), (0.02, 0), (0.02, 1)])
+def test_nodal_batch_mixed(ctx, name, Bfrac, dense):
+    """Mixed form (FP32 delta-S, FP64 for accepted and near-threshold proposals): same
+    proposals, delta-S bits, batch sum and labels as the FP64 oracle."""
+    g = graph(name)
+    B = max(2, int(g.N * Bfrac))
+    lab = labels_for(g, B, 8) if B < g.N else np.arange(g.N, dtype=np.int32)
+    o = Options(seed=23, dense_max_B=dense, precision=MIXED)
+    for b in range(o.num_batches):
+        print(f"nodal mixed {name} B={B} batch={b}", flush=True)
+        got = ctx.nodal_batch(g, o, lab, B, b, 11 + b)
+        ref = O.nodal_batch(g, o, lab, B, b, 11 + b)
+        assert np.array_equal(got["prop"], ref["prop"])
+        assert eq_bits(got["dS"], ref["dS"])
+        assert eq_bits(got["dE"], ref["dE"])
+        assert np.array_equal(got["labels"], ref["labels"])
+
+
+@pytest.mark.parametrize("name", ["hubs", "LL1k", "HH5k"])
+def test_partition_trace_mixed(ctx, name):
+    g = graph(name)
+    o = Options(seed=3, trace=1, precision=MIXED)
+    print(f"partition mixed {name}", flush=True)
+    lab, rep = ctx.partition(g, o)
+    tr = ctx.trace()
+    rlab, rrep, rtr = O.partition(g, o, trace_cap=len(tr) + 10)
+    assert len(tr) == rrep.n_trace
+    for k, (a, b) in enumerate(zip(tr, rtr)):
+        assert a[:6] == b[:6], (k, a, b)
+        assert eq_bits(a[6], b[6]) and eq_bits(a[7], b[7]), (k, a, b)
+    assert np.array_equal(lab, rlab) and eq_bits(rep.S, rrep.S)
+    # every accepted move was evaluated in FP64; the FP32 error is a fraction of a bit
+    assert 0 < rep.fp64_evals <= rep.batches * (g.N // 4)
+    assert 0 <= rep.rechecks <= rep.fp64_evals
+    assert 0 < rep.max_abs_err_fp32 < 1.0
+
+
+def test_partition_c3_mixed_equals_fp64(ctx, c3):
+    """200k-vertex graph: the mixed form partitions bit-identically to the plain FP64 form."""
+    print("C3 fp64", flush=True)
+    lab64, rep64 = ctx.partition(c3, Options(seed=1))
+    print("C3 mixed", flush=True)
+    lab32, rep32 = ctx.partition(c3, Options(seed=1, precision=MIXED))
+    assert np.array_equal(lab64, lab32)
+    assert eq_bits(rep64.S, rep32.S) and rep64.B == rep32.B and rep64.sweeps == rep32.sweeps
+    assert rep32.fp64_evals > 0 and 0 < rep32.max_abs_err_fp32 < 1.0
+    print(f"C3 fp64 {rep64.t_total:.3f}s mixed {rep32.t_total:.3f}s rechecks {rep32.rechecks} "
+          f"fp64_evals {rep32.fp64_evals} err {rep32.max_abs_err_fp32:.3g}", flush=True)

@@ -81,3 +81,21 @@ def test_generator_pinned_digest():
     d = h(g.truth.ctypes.data, 4 * g.N, d)
     with open(os.path.join(GOLDEN, "dcsbm_c1_digest.txt")) as f:
         assert int(f.read().split()[0], 16) == d
+
+
+def test_fp32_log2_error_bound():
+    """The mixed form's FP32 log2 stays within (2 |log2 x| + 6) 2^-24 of the exact log2 for
+    every integer count up to 2^24 and a sample up to 2^31 (the bound its delta-S error
+    bound is built on)."""
+    u = 2.0 ** -24
+    x = np.arange(1, 2 ** 24 + 1, dtype=np.float64)
+    L = np.log2(x)
+    y = gsap_amd.log2f(x.astype(np.float32)).astype(np.float64)
+    assert np.all(np.abs(y - L) <= (2 * np.abs(L) + 6) * u)
+    xi = np.random.default_rng(0).integers(2 ** 24, 2 ** 31, 1_000_000).astype(np.float64)
+    L = np.log2(xi)
+    y = gsap_amd.log2f(xi.astype(np.float32)).astype(np.float64)
+    assert np.all(np.abs(y - L) <= (2 * np.abs(L) + 6) * u)
+    # powers of two are exact
+    p = 2.0 ** np.arange(0, 31)
+    assert np.array_equal(gsap_amd.log2f(p.astype(np.float32)), np.arange(0, 31, dtype=np.float32))
