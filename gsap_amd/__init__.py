@@ -51,7 +51,7 @@ class COptions(C.Structure):
                 ("max_levels", C.c_int32), ("precision", C.c_int32), ("chain", C.c_uint32),
                 ("trace", C.c_int32), ("dense_max_B", C.c_int32), ("profile", C.c_int32),
                 ("model", C.c_int32), ("workers", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("profile_classes", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class CReport(C.Structure):
@@ -193,6 +193,7 @@ class Options:
     trace: int = 0
     dense_max_B: int = 0
     profile: int = 0
+    profile_classes: int = 0
     model: int = 0  # 0: the reference's model term, 1: corrected (x ln x scaled by E)
     workers: int = 0  # batch / chain calls: concurrent sub-contexts (0 = 8)
 
@@ -200,7 +201,8 @@ class Options:
         o = COptions()
         for f in ("seed", "num_proposals", "reduction_rate", "max_nodal_itr", "num_batches",
                   "threshold1", "threshold2", "moving_avg_window", "max_levels", "precision",
-                  "chain", "trace", "dense_max_B", "profile", "model", "workers"):
+                  "chain", "trace", "dense_max_B", "profile", "model", "workers",
+                  "profile_classes"):
             setattr(o, f, getattr(self, f))
         return o
 

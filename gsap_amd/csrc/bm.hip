@@ -10,6 +10,8 @@
 //           The dense matrix stays valid for O(1) M[r][t] lookups by the sweep kernels.
 //   sparse  (larger B): key = b(u)*B + b(v) per edge, stable radix sort, run-length encode
 //           (integer atomics, exact), then a second radix sort of (col, row) keys for the CSC.
+#include <algorithm>
+
 #include "engine.h"
 
 namespace gsap {
@@ -60,14 +62,15 @@ __global__ __launch_bounds__(512) void k_tile_hist(const int32_t *optr, const in
   const int32_t nt = (r1 - r0) * B;
   for (int32_t i = threadIdx.x; i < nt; i += blockDim.x) T[i] = 0;
   __syncthreads();
-  const int64_t vs = H[(int64_t)r0 * W];
-  const int64_t ve = r1 < B ? (int64_t)H[(int64_t)r1 * W] : N;
+  // one tile holding every row (vperm == nullptr): chunks of the vertices in id order
+  const int64_t vs = vperm ? (int64_t)H[(int64_t)r0 * W] : 0;
+  const int64_t ve = (vperm && r1 < B) ? (int64_t)H[(int64_t)r1 * W] : N;
   const int64_t c = blockIdx.y;
   const int64_t a0 = vs + (ve - vs) * c / C, a1 = vs + (ve - vs) * (c + 1) / C;
   // a group of 16 lanes per vertex: 4 vertices in flight per wave
   const int q = threadIdx.x & 15, grp = threadIdx.x >> 4, ng = blockDim.x >> 4;
   for (int64_t k = a0 + grp; k < a1; k += ng) {
-    const int32_t v = vperm[k];
+    const int32_t v = vperm ? vperm[k] : (int32_t)k;
     int32_t *row = T + (lab[v] - r0) * B;
     for (int32_t j = optr[v] + q; j < optr[v + 1]; j += 16) atomicAdd(&row[lab[oadj[j]]], ow[j]);
   }
@@ -164,6 +167,81 @@ __global__ void k_logs(const int32_t *dout, const int32_t *din, int32_t B, doubl
   if (x >= B) return;
   ldout[x] = clog2((double)dout[x]);
   ldin[x] = clog2((double)din[x]);
+}
+
+// Small block matrices (B <= 128): CSR, CSC, degrees and their logs from D in one
+// workgroup of 1024 threads (16 waves).  Row r (column c) is one wave's ballot compaction
+// of D's row (column) in ascending order, as k_dense_write_rows does.
+constexpr int SMALL_B = 128;
+__global__ __launch_bounds__(1024) void k_small_bm(const int32_t *D, int32_t B, int32_t *rowptr,
+                                                   int32_t *col, int32_t *val, int32_t *erow,
+                                                   int32_t *colptr, int32_t *row, int32_t *cval,
+                                                   int32_t *dout, int32_t *din, double *ldout,
+                                                   double *ldin) {
+  __shared__ int32_t rcnt[SMALL_B], ccnt[SMALL_B], roff[SMALL_B + 1], coff[SMALL_B + 1];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  // counts and sums: waves 0..15 take rows, then columns
+  for (int32_t x = w; x < 2 * B; x += 16) {
+    const bool is_row = x < B;
+    const int32_t y = is_row ? x : x - B;
+    uint32_t n = 0;
+    int32_t sum = 0;
+    for (int32_t t = lane; t < B; t += 64) {
+      const int32_t d = is_row ? D[y * B + t] : D[t * B + y];
+      n += d != 0;
+      sum += d;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      n += __shfl_down(n, off, 64);
+      sum += __shfl_down(sum, off, 64);
+    }
+    if (lane == 0) {
+      if (is_row) { rcnt[y] = (int32_t)n; dout[y] = sum; ldout[y] = clog2((double)sum); }
+      else { ccnt[y] = (int32_t)n; din[y] = sum; ldin[y] = clog2((double)sum); }
+    }
+  }
+  __syncthreads();
+  // exclusive scans of the counts (B <= 128: two entries per lane)
+  if (w < 2) {
+    const int32_t *cnt = w == 0 ? rcnt : ccnt;
+    int32_t *off = w == 0 ? roff : coff;
+    const int32_t a0 = 2 * lane < B ? cnt[2 * lane] : 0;
+    const int32_t a1 = 2 * lane + 1 < B ? cnt[2 * lane + 1] : 0;
+    int32_t incl = a0 + a1;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int32_t yv = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += yv;
+    }
+    const int32_t ex = incl - (a0 + a1);
+    if (2 * lane < B) off[2 * lane] = ex;
+    if (2 * lane + 1 < B) off[2 * lane + 1] = ex + a0;
+    if (lane == 63) off[B] = incl;
+  }
+  __syncthreads();
+  for (int32_t x = threadIdx.x; x <= B; x += 1024) {
+    rowptr[x] = roff[x];
+    colptr[x] = coff[x];
+  }
+  // entries
+  for (int32_t x = w; x < 2 * B; x += 16) {
+    const bool is_row = x < B;
+    const int32_t y = is_row ? x : x - B;
+    int32_t run = is_row ? roff[y] : coff[y];
+    for (int32_t base = 0; base < B; base += 64) {
+      const int32_t t = base + lane;
+      const int32_t d = t < B ? (is_row ? D[y * B + t] : D[t * B + y]) : 0;
+      const uint64_t m = __ballot(d != 0);
+      if (d != 0) {
+        const int32_t pos = run + (int32_t)__popcll(m & lt);
+        if (is_row) { col[pos] = t; val[pos] = d; if (erow) erow[pos] = y; }
+        else { row[pos] = t; cval[pos] = d; }
+      }
+      run += (int32_t)__popcll(m);
+    }
+  }
 }
 
 // ---------------------------------------------------------------- sparse path
@@ -302,21 +380,25 @@ static void dense_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &
   const int64_t BB = (int64_t)B * B;
   const int T = 256;
   int32_t *D = bm.dense;
-  // vertices grouped by block
-  const int32_t W = (int32_t)((N + VT - 1) / VT);
-  k_label_hist<<<W, 256, (size_t)B * 4, st>>>(lab, N, B, rs.vhist, W);
-  scan_exclusive(rs.vhist, rs.vhist, (int64_t)B * W, nullptr, rs.tmp, st);
-  k_label_scatter<<<W, 256, (size_t)B * 4, st>>>(lab, N, B, rs.vhist, W, rs.vperm);
   // row tiles of at most 16384 counters (64 KB of LDS), chunks so that ~2048 workgroups run
   const int32_t R = B >= 16384 ? 1 : 16384 / B;
   const int32_t ntiles = (B + R - 1) / R;
-  int32_t C = 2048 / ntiles;
+  const int32_t W = (int32_t)((N + VT - 1) / VT);
+  const bool one_tile = ntiles == 1;
+  if (!one_tile) {  // vertices grouped by block, so that a tile reads only its rows' vertices
+    k_label_hist<<<W, 256, (size_t)B * 4, st>>>(lab, N, B, rs.vhist, W);
+    scan_exclusive(rs.vhist, rs.vhist, (int64_t)B * W, nullptr, rs.tmp, st);
+    k_label_scatter<<<W, 256, (size_t)B * 4, st>>>(lab, N, B, rs.vhist, W, rs.vperm);
+  }
+  // one tile: 256 chunks of the vertex range (one workgroup per CU, few partial planes)
+  int32_t C = one_tile ? 256 : 2048 / ntiles;
   if ((int64_t)C * BB > ((int64_t)8 << 20)) C = (int32_t)(((int64_t)8 << 20) / BB);
   if ((int64_t)C * BB > rs.partials_cap) C = (int32_t)(rs.partials_cap / BB);
   if (C < 1) C = 1;
   // a single chunk writes its tiles straight into D
-  k_tile_hist<<<dim3(ntiles, C), 512, (size_t)R * B * 4, st>>>(
-      g.optr, g.oadj, g.ow, lab, rs.vperm, rs.vhist, W, N, B, R, C, C == 1 ? D : rs.partials);
+  k_tile_hist<<<dim3(ntiles, C), 512, (size_t)std::min(R, B) * B * 4, st>>>(
+      g.optr, g.oadj, g.ow, lab, one_tile ? nullptr : rs.vperm, rs.vhist, W, N, B, R, C,
+      C == 1 ? D : rs.partials);
   if (C == 1) {
   } else if (C <= 4)
     k_hist_reduce<1><<<grid_for(BB, T), T, 0, st>>>(rs.partials, C, BB, D);
@@ -324,6 +406,11 @@ static void dense_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &
     k_hist_reduce<8><<<grid_for(BB * 8, T), T, 0, st>>>(rs.partials, C, BB, D);
   else
     k_hist_reduce<64><<<grid_for(BB * 64, T), T, 0, st>>>(rs.partials, C, BB, D);
+  if (B <= SMALL_B) {
+    k_small_bm<<<1, 1024, 0, st>>>(D, B, bm.rowptr, bm.col, bm.val, bm.erow, bm.colptr, bm.row,
+                                   bm.cval, bm.dout, bm.din, bm.ldout, bm.ldin);
+    return;
+  }
   k_dense_rows<<<B, 64, 0, st>>>(D, B, rs.cnt, bm.dout);
   scan_exclusive(rs.cnt, (uint32_t *)bm.rowptr, B, (uint32_t *)(bm.rowptr + B), rs.tmp, st);
   k_dense_write_rows<<<B, 64, 0, st>>>(D, B, bm.rowptr, bm.col, bm.val, bm.erow);
@@ -343,6 +430,10 @@ void rebuild_blockmodel(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf 
                      (int64_t)B * B <= rs.partials_cap && B <= rs.vhist_B;
   if (dense) {
     dense_path(g, lab, B, bm, rs, st);
+    if (B <= SMALL_B) {  // k_small_bm wrote the logs
+      bm.dense_valid = true;
+      return;
+    }
   } else if ((uint64_t)B * (uint64_t)B <= 0xFFFFFFFFull) {
     sparse_path<uint32_t>(g, lab, B, bm, rs, st);
   } else {

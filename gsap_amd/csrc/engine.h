@@ -64,11 +64,27 @@ struct RngKey {
 };
 void launch_block_terms(const BMView &m, double *OUT, int32_t *sof, uint8_t *coin, RngKey key,
                         uint32_t phase, bool sample, int32_t avg_row, hipStream_t st);
-// vertices of the batch in four lists by max(out, in) neighbour count: <= 16, <= 32, <= 128,
-// larger (LDS tables of 2^big_bits slots)
+// Nodal kernels by the larger of a vertex's out- and in-neighbour counts d: d <= 16 and 32
+// use register deduplication in groups of 16 and 32 lanes; d <= 128 a wave with 256-slot
+// LDS hash tables; larger d a 256-thread workgroup with tables sized for the vertex (LDS
+// up to big_lds bytes, else its slice of the global pool).
+constexpr int NCLS = 4;
+inline int nodal_class(int64_t d) { return d <= 16 ? 0 : d <= 32 ? 1 : d <= 128 ? 2 : 3; }
+// Space (int32 words) a class-3 vertex needs: three int32 arrays per hash table of 2^bits
+// slots (bits = smallest with 2^bits >= 2 n) and one double per row and column slot
+inline int64_t big_table_bits(int64_t n) {
+  int64_t b = 1;
+  while (((int64_t)1 << b) < 2 * (n > 1 ? n : 1)) ++b;
+  return b;
+}
+inline int64_t big_words(int64_t no, int64_t ni) {
+  return 3 * (((int64_t)1 << big_table_bits(no)) + ((int64_t)1 << big_table_bits(ni))) +
+         2 * (no + 2 + ni);
+}
+constexpr int64_t BIG_LDS_CAP = 40 * 1024;
 void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const int32_t *sof,
                   const uint8_t *coin, RngKey key, uint32_t phase, int64_t lo, int64_t BS,
-                  const int32_t *const lists[4], const int32_t counts[4], uint32_t big_bits,
+                  const int32_t *const lists[NCLS], const int32_t counts[NCLS], uint32_t big_lds,
                   int32_t *prop, double *dS, float *err, uint8_t *near, hipStream_t st);
 void launch_mix_stats(const float *err, const uint8_t *near, int64_t n, uint32_t *max_bits,
                       unsigned long long *count, hipStream_t st);

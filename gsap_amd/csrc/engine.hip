@@ -98,8 +98,9 @@ struct Prof {
     pool.pop_back();
     return e;
   }
+  uint32_t mask = 0xFFu;
   int begin(int k, hipStream_t st) {
-    if (!on) return -1;
+    if (!on || !((mask >> k) & 1u)) return -1;
     Span s{k, ev(), nullptr};
     HIPX(hipEventRecord(s.a, st));
     pend.push_back(s);
@@ -154,10 +155,10 @@ struct gsap_ctx {
   DBuf OUT, sof, coin, prop, dS, best, dSbest, map, terms, tiles, res;
   // mixed form: per-vertex |dS32 - dS64| and evaluation class; batch statistics
   DBuf errv, evcls, mixstat;
-  // per batch, the batch's vertices split by neighbour count (<= 16, <= 32, <= 128, larger)
+  // per batch, the batch's vertices split by neighbour-count class (nodal_class)
   DBuf vlist;
   std::vector<int64_t> list_off;  // [batch * 4 + class] -> offset into vlist; last = total
-  std::vector<uint32_t> big_bits; // per batch: LDS table bits of the largest class
+  std::vector<uint32_t> big_lds;  // per batch: LDS bytes of the largest class's launch
   int32_t lists_nb = 0;
   GraphDev g;
   BMBuf bm;
@@ -212,18 +213,18 @@ void upload_graph(gsap_ctx *c, const gsap_graph_i32 *g) {
     for (int32_t j = g->out_ptr[v]; j < g->out_ptr[v + 1]; ++j) { kout[v] += g->out_w[j]; esrc[j] = (int32_t)v; }
     for (int32_t j = g->in_ptr[v]; j < g->in_ptr[v + 1]; ++j) kin[v] += g->in_w[j];
   }
-  // vertices whose adjacency exceeds half the LDS hash table get global tables
+  // vertices of the largest nodal class get tables and slot arrays in the global pool
+  // (used when they do not fit the launch's LDS); offsets in int32 words, even
   std::vector<int64_t> hoff(N, -1);
   std::vector<uint8_t> hbo(N, 0), hbi(N, 0);
   int64_t pool = 0;
-  auto bits_for = [](int64_t n) { uint8_t b = 1; while (((int64_t)1 << b) < 2 * n) ++b; return b; };
   for (int64_t v = 0; v < N; ++v) {
     int64_t no = g->out_ptr[v + 1] - g->out_ptr[v], ni = g->in_ptr[v + 1] - g->in_ptr[v];
-    if (no > 128 || ni > 128) {
-      hbo[v] = bits_for(std::max<int64_t>(no, 1));
-      hbi[v] = bits_for(std::max<int64_t>(ni, 1));
+    if (nodal_class(std::max(no, ni)) == NCLS - 1) {
+      hbo[v] = (uint8_t)big_table_bits(no);
+      hbi[v] = (uint8_t)big_table_bits(ni);
       hoff[v] = pool;
-      pool += 3 * (((int64_t)1 << hbo[v]) + ((int64_t)1 << hbi[v]));
+      pool += (big_words(no, ni) + 1) & ~(int64_t)1;
     }
   }
   h2d(c->optr, g->out_ptr, N + 1, st);
@@ -256,24 +257,24 @@ void build_batch_lists(gsap_ctx *c, const gsap_graph_i32 *g, int32_t nb) {
   const int64_t N = g->N, BS = N / nb;
   std::vector<int32_t> lst;
   lst.reserve((size_t)(BS * nb));
-  c->list_off.assign((size_t)nb * 4 + 1, 0);
-  c->big_bits.assign((size_t)nb, 8);
+  c->list_off.assign((size_t)nb * NCLS + 1, 0);
+  c->big_lds.assign((size_t)nb, 0);
   for (int32_t b = 0; b < nb; ++b)
-    for (int k = 0; k < 4; ++k) {
-      c->list_off[(size_t)b * 4 + k] = (int64_t)lst.size();
+    for (int k = 0; k < NCLS; ++k) {
+      c->list_off[(size_t)b * NCLS + k] = (int64_t)lst.size();
       for (int64_t v = (int64_t)b * BS; v < (int64_t)(b + 1) * BS; ++v) {
-        const int64_t d = std::max<int64_t>(g->out_ptr[v + 1] - g->out_ptr[v], g->in_ptr[v + 1] - g->in_ptr[v]);
-        const int cls = d <= 16 ? 0 : (d <= 32 ? 1 : (d <= 128 ? 2 : 3));
-        if (cls != k) continue;
+        const int64_t no = g->out_ptr[v + 1] - g->out_ptr[v];
+        const int64_t ni = g->in_ptr[v + 1] - g->in_ptr[v];
+        if (nodal_class(std::max(no, ni)) != k) continue;
         lst.push_back((int32_t)v);
-        if (cls == 3) {
-          uint32_t bits = 8;
-          while (bits < 11 && ((int64_t)1 << bits) < 2 * d) ++bits;
-          c->big_bits[b] = std::max(c->big_bits[b], bits);
+        if (k == NCLS - 1) {
+          // LDS of the launch: enough for its largest vertex, within the cap
+          const int64_t need = 4 * big_words(no, ni);
+          if (need <= BIG_LDS_CAP) c->big_lds[b] = std::max<uint32_t>(c->big_lds[b], (uint32_t)need);
         }
       }
     }
-  c->list_off[(size_t)nb * 4] = (int64_t)lst.size();
+  c->list_off[(size_t)nb * NCLS] = (int64_t)lst.size();
   c->vlist.ensure(4 * std::max<size_t>(lst.size(), 1));
   if (!lst.empty())
     HIPX(hipMemcpyAsync(c->vlist.p, lst.data(), 4 * lst.size(), hipMemcpyHostToDevice, c->st));
@@ -457,10 +458,11 @@ void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, uint32_t phase, 
                      true, ar, st);
   c->prof.end(p, st);
   p = c->prof.begin(0, st);
-  const int32_t *lists[4];
-  int32_t counts[4];
-  for (int k = 0; k < 4; ++k) {
-    const int64_t off = c->list_off[(size_t)b * 4 + k], end = c->list_off[(size_t)b * 4 + k + 1];
+  const int32_t *lists[NCLS];
+  int32_t counts[NCLS];
+  for (int k = 0; k < NCLS; ++k) {
+    const int64_t off = c->list_off[(size_t)b * NCLS + k];
+    const int64_t end = c->list_off[(size_t)b * NCLS + k + 1];
     lists[k] = c->vlist.as<int32_t>() + off;
     counts[k] = (int32_t)(end - off);
   }
@@ -468,7 +470,7 @@ void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, uint32_t phase, 
   float *err = mixed ? c->errv.as<float>() : nullptr;
   uint8_t *evcls = mixed ? c->evcls.as<uint8_t>() : nullptr;
   launch_nodal(c->g, c->lab.as<int32_t>(), m, c->sof.as<int32_t>(), c->coin.as<uint8_t>(),
-               rng_key(o), phase, lo, BS, lists, counts, c->big_bits[b], c->prop.as<int32_t>(),
+               rng_key(o), phase, lo, BS, lists, counts, c->big_lds[b], c->prop.as<int32_t>(),
                c->dS.as<double>(), err, evcls, st);
   if (mixed) {
     uint32_t *ms = c->mixstat.as<uint32_t>();
@@ -511,6 +513,7 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
   alloc_work(c, o.num_proposals, resolve_dense(o));
   c->trace.clear();
   c->prof.reset(o.profile != 0);
+  c->prof.mask = o.profile_classes ? (uint32_t)o.profile_classes : 0xFFu;
   const bool tr = o.trace != 0;
   const int64_t N = c->N;
   hipStream_t st = c->st;

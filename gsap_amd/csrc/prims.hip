@@ -90,6 +90,32 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_apply(const uint32_t *in, uint3
   }
 }
 
+// single block, whole array: tiles in order with a running carry (small n)
+__global__ __launch_bounds__(SCAN_T) void k_scan_single(const uint32_t *in, uint32_t *out,
+                                                        int64_t n, uint32_t *total) {
+  __shared__ uint32_t lds4[4];
+  uint32_t carry = 0;
+  for (int64_t t0 = 0; t0 < n; t0 += SCAN_TILE) {
+    const int64_t base = t0 + (int64_t)threadIdx.x * SCAN_I;
+    uint32_t v[SCAN_I];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_I; ++i) {
+      v[i] = (base + i < n) ? in[base + i] : 0u;
+      s += v[i];
+    }
+    uint32_t tot;
+    uint32_t ex = block_excl_scan(s, lds4, &tot) + carry;
+#pragma unroll
+    for (int i = 0; i < SCAN_I; ++i) {
+      if (base + i < n) out[base + i] = ex;
+      ex += v[i];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
 size_t scan_temp_words(int64_t n) { return (size_t)((n + SCAN_TILE - 1) / SCAN_TILE) + 1; }
 
 void scan_exclusive(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *total, uint32_t *tmp,
@@ -97,6 +123,10 @@ void scan_exclusive(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *tota
   int64_t nt = (n + SCAN_TILE - 1) / SCAN_TILE;
   if (nt == 0) {
     if (total) hipMemsetAsync(total, 0, 4, st);
+    return;
+  }
+  if (nt <= 16) {  // up to 32768 elements: one launch
+    k_scan_single<<<1, SCAN_T, 0, st>>>(in, out, n, total);
     return;
   }
   k_scan_reduce<<<(unsigned)nt, SCAN_T, 0, st>>>(in, n, tmp);
@@ -123,15 +153,32 @@ __global__ __launch_bounds__(RS_T) void k_radix_hist(const K *keys, int64_t n, i
   hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
+// Stable scatter of one tile: items are ranked in rounds of 256 (one per thread; inside a
+// wave64 from 8 ballots + popcount, across the four waves from an LDS prefix, across rounds
+// from running digit bases), placed in LDS in tile-local digit order, then written out in
+// that order so that each digit's run goes to consecutive addresses.  The tile's count of
+// digit d is the step between its scanned offset and the next entry of the digit-major
+// offset array.
 template <class K>
 __global__ __launch_bounds__(RS_T) void k_radix_scatter(const K *kin, const uint32_t *vin,
                                                         K *kout, uint32_t *vout, int64_t n,
                                                         int shift, const uint32_t *offs,
                                                         int64_t ntiles) {
-  __shared__ uint32_t base[256];
+  __shared__ K sk[RS_TILE];
+  __shared__ uint32_t sv[RS_TILE];
+  __shared__ uint32_t goff[256], loff[256], run[256];
   __shared__ uint32_t wc[4][256];
+  __shared__ uint32_t lds4[4];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  base[tid] = offs[(int64_t)tid * ntiles + blockIdx.x];
+  {
+    const int64_t idx = (int64_t)tid * ntiles + blockIdx.x;
+    const uint32_t g = offs[idx];
+    const uint32_t nx = (idx + 1 < 256 * ntiles) ? offs[idx + 1] : (uint32_t)n;
+    goff[tid] = g;
+    const uint32_t ex = block_excl_scan(nx - g, lds4, nullptr);
+    loff[tid] = ex;
+    run[tid] = ex;
+  }
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int64_t tbase = (int64_t)blockIdx.x * RS_TILE;
   for (int it = 0; it < RS_I; ++it) {
@@ -161,14 +208,23 @@ __global__ __launch_bounds__(RS_T) void k_radix_scatter(const K *kin, const uint
       wc[3][tid] = c0 + c1 + c2;
       __syncthreads();
       if (valid) {
-        uint32_t pos = base[d] + wc[w][d] + rank;
-        kout[pos] = k;
-        vout[pos] = v;
+        const uint32_t pos = run[d] + wc[w][d] + rank;
+        sk[pos] = k;
+        sv[pos] = v;
       }
       __syncthreads();
-      base[tid] += c0 + c1 + c2 + c3;
+      run[tid] += c0 + c1 + c2 + c3;
     }
     __syncthreads();
+  }
+  const int64_t left = n - tbase;
+  const int32_t tn = left < RS_TILE ? (int32_t)left : RS_TILE;
+  for (int32_t i = tid; i < tn; i += RS_T) {
+    const K k = sk[i];
+    const uint32_t d = (uint32_t)(k >> shift) & 255u;
+    const uint32_t pos = goff[d] + ((uint32_t)i - loff[d]);
+    kout[pos] = k;
+    vout[pos] = sv[i];
   }
 }
 

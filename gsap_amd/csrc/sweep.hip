@@ -199,8 +199,7 @@ GSAP_D bool mixed_rejects(double d32, float eb, double Y_r, double Y_s, double Y
 // run-merged).  Only the entries v touches are evaluated term by term ("slots": v's
 // out-blocks, r and s for rows r and s; v's in-blocks outside {r, s} for columns r and s);
 // the untouched entries contribute (log2 d_old - log2 d_new) * (their summed count).
-// Labels are not written here (batch-Jacobi).  Hash tables of 2^lbits slots live in LDS;
-// a vertex whose adjacency needs more uses its tables in the global pool.
+// Labels are not written here (batch-Jacobi).  Hash tables of 2^lbits slots live in LDS.
 template <bool MIX>
 __global__ __launch_bounds__(64, 6) void k_nodal(NodalArgs a, const int32_t *list,
                                                  uint32_t lbits) {
@@ -242,18 +241,9 @@ __global__ __launch_bounds__(64, 6) void k_nodal(NodalArgs a, const int32_t *lis
 
   // a_t (out weight of v into block t) and c_t (in weight from block t), with the
   // first adjacency position of each block
-  HTab ho, hi;
-  if (2 * (no > ni ? no : ni) > (int32_t)lcap) {
-    const uint32_t bo = a.hbits_o[v], bi = a.hbits_i[v];
-    int32_t *p = a.hpool + a.hoff[v];
-    const uint32_t co = 1u << bo, ci = 1u << bi;
-    ho = HTab{p, p + co, p + 2 * co, bo};
-    p += 3 * co;
-    hi = HTab{p, p + ci, p + 2 * ci, bi};
-  } else {
-    ho = HTab{lko, lwo, lfo, lbits};
-    hi = HTab{lki, lwi, lfi, lbits};
-  }
+  // (vertices here have at most lcap / 2 out- and in-neighbours)
+  const HTab ho{lko, lwo, lfo, lbits};
+  const HTab hi{lki, lwi, lfi, lbits};
   ht_clear(ho, lane, 64);
   ht_clear(hi, lane, 64);
   // first chunk of both adjacency lists: gather the neighbour blocks once, keep them
@@ -446,6 +436,172 @@ __global__ __launch_bounds__(64, 6) void k_nodal(NodalArgs a, const int32_t *lis
   }
 }
 
+// One 256-thread workgroup per vertex with more than 128 out- or in-neighbours: the
+// computation of k_nodal with the adjacency spread over four waves.  Hash tables are sized
+// for the vertex (in LDS when they fit the launch's allocation, else in the vertex's slice
+// of the global pool); every row and column slot value is stored by position and wave 0
+// sums them in the canonical order, so the bits equal the one-wave form.  The mixed form
+// evaluates these few vertices in FP64 directly.
+template <bool MIX>
+__global__ __launch_bounds__(256) void k_nodal_big(NodalArgs a, const int32_t *list,
+                                                   uint32_t lds_bytes) {
+  extern __shared__ double lds_d[];
+  __shared__ int64_t red[4][4];
+  __shared__ int64_t corner_s[4];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int64_t v = list[blockIdx.x];
+  const int64_t i = v - a.lo;
+  const BMView &m = a.m;
+  const int32_t B = m.B;
+  const int32_t r = a.lab[v];
+  const Philox x = philox((uint32_t)v, a.phase, a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
+  const int32_t o0 = a.optr[v], no = a.optr[v + 1] - o0;
+  const int32_t i0 = a.iptr[v], ni = a.iptr[v + 1] - i0;
+  const int64_t kout = a.kout[v], kin = a.kin[v];
+  const int64_t tot = kout + kin;
+  int32_t s;  // every wave computes the same proposal
+  if (tot == 0) {
+    s = (int32_t)mulhi(x.x0, B);
+  } else {
+    const int32_t nbr = wave_pick(a.ow + o0, a.oadj + o0, no, a.iw + i0, a.iadj + i0, ni,
+                                  mulhi(x.x1, tot));
+    const int32_t u = a.lab[nbr];
+    s = a.coin[u] ? (int32_t)mulhi(x.x0, B) : a.sof[u];
+  }
+  if (tid == 0) a.prop[i] = s;
+  if (s == r) {
+    if (tid == 0) {
+      a.dS[i] = 0.0;
+      if constexpr (MIX) {
+        a.err[i] = 0.0f;
+        a.near[i] = 0;
+      }
+    }
+    return;
+  }
+  const uint32_t bo = a.hbits_o[v], bi = a.hbits_i[v];
+  const int32_t co = 1 << bo, ci = 1 << bi;
+  const int64_t words = 3 * (int64_t)(co + ci) + 2 * (int64_t)(no + 2 + ni);
+  int32_t *base = 4 * words <= (int64_t)lds_bytes ? reinterpret_cast<int32_t *>(lds_d)
+                                                  : a.hpool + a.hoff[v];
+  const HTab ho{base, base + co, base + 2 * co, bo};
+  const HTab hi{base + 3 * co, base + 3 * co + ci, base + 3 * co + 2 * ci, bi};
+  double *xs = reinterpret_cast<double *>(base + 3 * (co + ci));
+  ht_clear(ho, tid, 256);
+  ht_clear(hi, tid, 256);
+  const int64_t dout_r = m.dout[r], dout_s = m.dout[s], din_r = m.din[r], din_s = m.din[s];
+  const double ldo_r = m.ldout[r], ldo_s = m.ldout[s], ldi_r = m.ldin[r], ldi_s = m.ldin[s];
+  if (tid < 4) {
+    const int32_t cc = tid < 2 ? r : s, rr = (tid & 1) ? s : r;
+    corner_s[tid] = bm_col_get(m, cc, rr);
+  }
+  __syncthreads();
+  for (int32_t j = tid; j < no; j += 256) ht_insert(ho, a.lab[a.oadj[o0 + j]], a.ow[o0 + j], j);
+  for (int32_t j = tid; j < ni; j += 256) ht_insert(hi, a.lab[a.iadj[i0 + j]], a.iw[i0 + j], j);
+  __syncthreads();
+  const int64_t c_r = ht_weight(hi, r), c_s = ht_weight(hi, s);
+  const bool r_in_a = ht_find(ho, r) >= 0, s_in_a = ht_find(ho, s) >= 0;
+  const double l_dr = clog2((double)(dout_r - kout));
+  const double l_ds = clog2((double)(dout_s + kout));
+  const double l_ir = clog2((double)(din_r - kin));
+  const double l_is = clog2((double)(din_s + kin));
+  int64_t sum_r = 0, sum_s = 0, csum_r = 0, csum_s = 0;
+  // row slots: positions 0..no-1 (first occurrences), no (r), no+1 (s)
+  for (int32_t j = tid; j < no + 2; j += 256) {
+    int32_t t;
+    int64_t av = 0;
+    bool use;
+    if (j < no) {
+      t = a.lab[a.oadj[o0 + j]];
+      const int32_t slot = ht_find(ho, t);
+      use = ld_relaxed(&ho.first[slot]) == j;
+      av = ld_relaxed(&ho.w[slot]);
+    } else if (j == no) {
+      t = r;
+      use = !r_in_a;
+    } else {
+      t = s;
+      use = !s_in_a;
+    }
+    double xv = 0.0;
+    if (use) {
+      int32_t Mr32, Ms32;
+      bm_row_get2(m, r, s, t, Mr32, Ms32);
+      const int64_t Mr = Mr32, Ms = Ms32;
+      const int64_t R = Mr - av - (t == r ? c_r : 0) + (t == s ? c_r : 0);
+      const int64_t S = Ms + av - (t == r ? c_s : 0) + (t == s ? c_s : 0);
+      const double ldt = m.ldin[t];
+      const double or_ = Mr > 0 ? term(Mr, ldo_r, ldt) : 0.0;
+      const double os_ = Ms > 0 ? term(Ms, ldo_s, ldt) : 0.0;
+      const double nr = R > 0 ? term(R, ldt, l_dr) : 0.0;
+      const double ns = S > 0 ? term(S, ldt, l_ds) : 0.0;
+      xv = ((or_ + os_) - nr) - ns;
+      sum_r += Mr;
+      sum_s += Ms;
+    }
+    xs[j] = xv;
+  }
+  // column slots (t not in {r, s})
+  for (int32_t j = tid; j < ni; j += 256) {
+    const int32_t t = a.lab[a.iadj[i0 + j]];
+    const int32_t slot = ht_find(hi, t);
+    double xv = 0.0;
+    if (ld_relaxed(&hi.first[slot]) == j && t != r && t != s) {
+      const int64_t c = ld_relaxed(&hi.w[slot]);
+      int32_t Mr32, Ms32;
+      bm_col_get2(m, r, s, t, Mr32, Ms32);
+      const int64_t Mr = Mr32, Ms = Ms32;
+      const double ldt = m.ldout[t];
+      const double ir = Mr > 0 ? term(Mr, ldt, ldi_r) : 0.0;
+      const double is = Ms > 0 ? term(Ms, ldt, ldi_s) : 0.0;
+      const int64_t wr = Mr - c, ws = Ms + c;
+      const double nr = wr > 0 ? term(wr, ldt, l_ir) : 0.0;
+      const double ns = ws > 0 ? term(ws, ldt, l_is) : 0.0;
+      xv = ((ir + is) - nr) - ns;
+      csum_r += Mr;
+      csum_s += Ms;
+    }
+    xs[no + 2 + j] = xv;
+  }
+  sum_r = wave_sum_i64(sum_r);
+  sum_s = wave_sum_i64(sum_s);
+  csum_r = wave_sum_i64(csum_r);
+  csum_s = wave_sum_i64(csum_s);
+  if (lane == 0) {
+    red[wv][0] = sum_r;
+    red[wv][1] = sum_s;
+    red[wv][2] = csum_r;
+    red[wv][3] = csum_s;
+  }
+  __syncthreads();
+  if (wv != 0) return;
+  // canonical sums: position k into partial k mod 64, ascending; unused slots hold +0.0
+  double p_row = 0.0, p_col = 0.0;
+  for (int32_t k = lane; k < no + 2; k += 64) p_row += xs[k];
+  for (int32_t k = lane; k < ni; k += 64) p_col += xs[no + 2 + k];
+  const double drow = wave_sum(p_row), dcol = wave_sum(p_col);
+  if (lane == 0) {
+    const int64_t Sr = red[0][0] + red[1][0] + red[2][0] + red[3][0];
+    const int64_t Ss = red[0][1] + red[1][1] + red[2][1] + red[3][1];
+    const int64_t Cr = red[0][2] + red[1][2] + red[2][2] + red[3][2];
+    const int64_t Cs = red[0][3] + red[1][3] + red[2][3] + red[3][3];
+    const int64_t M_rr = corner_s[0], M_sr = corner_s[1], M_rs = corner_s[2], M_ss = corner_s[3];
+    const int64_t W_r = dout_r - Sr, W_s = dout_s - Ss;
+    const double Y_r = W_r > 0 ? (ldo_r - l_dr) * (double)W_r : 0.0;
+    const double Y_s = W_s > 0 ? (ldo_s - l_ds) * (double)W_s : 0.0;
+    const int64_t W2_r = din_r - M_rr - M_sr - Cr;
+    const int64_t W2_s = din_s - M_rs - M_ss - Cs;
+    const double Y2_r = W2_r > 0 ? (ldi_r - l_ir) * (double)W2_r : 0.0;
+    const double Y2_s = W2_s > 0 ? (ldi_s - l_is) * (double)W2_s : 0.0;
+    const double d = nodal_total(drow, dcol, Y_r, Y_s, Y2_r, Y2_s);
+    a.dS[i] = d < 0.0 ? d : 0.0;
+    if constexpr (MIX) {
+      a.err[i] = 0.0f;
+      a.near[i] = 1;
+    }
+  }
+}
+
 // Group sums over the G lanes of a group (every lane gets the result).
 template <int G>
 GSAP_D int64_t group_allsum(int64_t x) {
@@ -573,11 +729,14 @@ __global__ __launch_bounds__(256, 5) void k_nodal_small(NodalArgs a, const int32
     Ms = Ms32;
   };
   int64_t sum_r = 0, sum_s = 0, csum_r = 0, csum_s = 0;
+  // positions j = q + jj G (jj < JJ cover 0 .. no + 1 <= G + 1); position j belongs to
+  // canonical partial j mod 64, which is this lane's p[jj mod (64 / G)]
+  constexpr int JJ = G == 64 ? 2 : 64 / G;
   auto fp64_pass = [&](double &drow, double &dcol, bool sums) {
     CAcc<G> prow;
     prow.init();
 #pragma unroll
-    for (int jj = 0; jj < 64 / G; ++jj) {
+    for (int jj = 0; jj < JJ; ++jj) {
       const int32_t j = q + jj * G;
       if (j >= no + 2) break;
       int32_t t;
@@ -588,7 +747,7 @@ __global__ __launch_bounds__(256, 5) void k_nodal_small(NodalArgs a, const int32
       const double os_ = Ms > 0 ? term(Ms, ldo_s, ldt) : 0.0;
       const double nr = R > 0 ? term(R, ldt, l_dr) : 0.0;
       const double ns = S > 0 ? term(S, ldt, l_ds) : 0.0;
-      prow.p[jj] += ((or_ + os_) - nr) - ns;
+      prow.p[jj % CAcc<G>::S] += ((or_ + os_) - nr) - ns;
       if (sums) {
         sum_r += Mr;
         sum_s += Ms;
@@ -623,7 +782,7 @@ __global__ __launch_bounds__(256, 5) void k_nodal_small(NodalArgs a, const int32
     fp64_pass(drow, dcol, true);
   } else {
 #pragma unroll
-    for (int jj = 0; jj < 64 / G; ++jj) {
+    for (int jj = 0; jj < JJ; ++jj) {
       const int32_t j = q + jj * G;
       if (j >= no + 2) break;
       int32_t t;
@@ -703,7 +862,7 @@ __global__ __launch_bounds__(256, 5) void k_nodal_small(NodalArgs a, const int32
 
 void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const int32_t *sof,
                   const uint8_t *coin, RngKey key, uint32_t phase, int64_t lo, int64_t BS,
-                  const int32_t *const lists[4], const int32_t counts[4], uint32_t big_bits,
+                  const int32_t *const lists[NCLS], const int32_t counts[NCLS], uint32_t big_lds,
                   int32_t *prop, double *dS, float *err, uint8_t *near, hipStream_t st) {
   if (BS <= 0) return;
   NodalArgs a;
@@ -724,8 +883,7 @@ void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const 
       k_nodal<true><<<(unsigned)counts[2], 64, 6 * 4 * (size_t)LCAP, st>>>(a, lists[2],
                                                                           (uint32_t)LBITS);
     if (counts[3] > 0)
-      k_nodal<true><<<(unsigned)counts[3], 64, 6 * 4 * ((size_t)1 << big_bits), st>>>(
-          a, lists[3], big_bits);
+      k_nodal_big<true><<<(unsigned)counts[3], 256, big_lds, st>>>(a, lists[3], big_lds);
     return;
   }
   if (counts[0] > 0)
@@ -736,8 +894,7 @@ void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const 
     k_nodal<false><<<(unsigned)counts[2], 64, 6 * 4 * (size_t)LCAP, st>>>(a, lists[2],
                                                                          (uint32_t)LBITS);
   if (counts[3] > 0)
-    k_nodal<false><<<(unsigned)counts[3], 64, 6 * 4 * ((size_t)1 << big_bits), st>>>(
-        a, lists[3], big_bits);
+    k_nodal_big<false><<<(unsigned)counts[3], 256, big_lds, st>>>(a, lists[3], big_lds);
 }
 
 // Mixed-form statistics of one batch: max |dS32 - dS64| (float bits, order-preserving for

@@ -81,7 +81,8 @@ typedef struct gsap_options {
   int32_t profile;            /* 1: time kernel classes with HIP events (report.kernel_ms) */
   int32_t model;              /* GSAP_MODEL_*: description-length model term */
   int32_t workers;            /* batch / chain calls: concurrent sub-contexts (0 = auto, 8) */
-  int32_t reserved[5];
+  int32_t profile_classes;    /* with profile = 1: bit k times kernel class k (0 = all) */
+  int32_t reserved[4];
 } gsap_options;
 
 /* Per-call report.  Times are seconds of host wall clock around device work. */

@@ -239,10 +239,10 @@ def test_partition_trace_mixed(ctx, name):
         assert a[:6] == b[:6], (k, a, b)
         assert eq_bits(a[6], b[6]) and eq_bits(a[7], b[7]), (k, a, b)
     assert np.array_equal(lab, rlab) and eq_bits(rep.S, rrep.S)
-    # every accepted move was evaluated in FP64; the FP32 error is a fraction of a bit
+    # every accepted move was evaluated in FP64; the FP32 error is tiny next to S
     assert 0 < rep.fp64_evals <= rep.batches * (g.N // 4)
     assert 0 <= rep.rechecks <= rep.fp64_evals
-    assert 0 < rep.max_abs_err_fp32 < 1.0
+    assert 0 < rep.max_abs_err_fp32 < 1e-6 * abs(rep.S)
 
 
 def test_partition_c3_mixed_equals_fp64(ctx, c3):
@@ -253,6 +253,6 @@ def test_partition_c3_mixed_equals_fp64(ctx, c3):
     lab32, rep32 = ctx.partition(c3, Options(seed=1, precision=MIXED))
     assert np.array_equal(lab64, lab32)
     assert eq_bits(rep64.S, rep32.S) and rep64.B == rep32.B and rep64.sweeps == rep32.sweeps
-    assert rep32.fp64_evals > 0 and 0 < rep32.max_abs_err_fp32 < 1.0
+    assert rep32.fp64_evals > 0 and 0 < rep32.max_abs_err_fp32 < 1e-6 * abs(rep32.S)
     print(f"C3 fp64 {rep64.t_total:.3f}s mixed {rep32.t_total:.3f}s rechecks {rep32.rechecks} "
           f"fp64_evals {rep32.fp64_evals} err {rep32.max_abs_err_fp32:.3g}", flush=True)
