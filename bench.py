@@ -9,7 +9,7 @@ value = nodal-move edges/s = E * sweeps / sum of sweep wall time, where a sweep 
 of partition.hpp:912-935 (4 nodal batches, each with its block-matrix rebuild, plus the
 description length).  The roofline object is computed for the dominant kernel from HIP
 events recorded on its stream inside the timed steps.  The CPU baseline times the oracle
-(single thread) on a bounded sample of the same workload.
+on the host's cores (and on one thread) on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -118,9 +118,15 @@ def algorithmic_bytes(kclass, g, nb, B, nnz):
     return None
 
 
-def cpu_baseline(g, labels, B, seconds):
-    """Single-thread oracle: nodal batches (each with its rebuild) from the GPU's final
-    labelling, for about `seconds`."""
+def cpu_threads():
+    """Host threads of the all-cores oracle: the box's share (OMP_NUM_THREADS), at most 16."""
+    n = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    return max(1, min(16, n))
+
+
+def cpu_baseline(g, labels, B, seconds, threads):
+    """The oracle's nodal batches (each with its block-matrix rebuild) from the GPU's final
+    labelling, for about `seconds` or 64 batches, on `threads` host threads."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     opts = gsap_amd.Options(seed=1)
@@ -128,16 +134,19 @@ def cpu_baseline(g, labels, B, seconds):
     t0 = time.perf_counter()
     nbat = 0
     while True:
-        O.nodal_batch(g, opts, lab, B, nbat % opts.num_batches, 100000 + nbat)
+        if threads == 1:
+            O.nodal_batch(g, opts, lab, B, nbat % opts.num_batches, 100000 + nbat)
+        else:
+            O.nodal_batch_mt(g, opts, lab, B, nbat % opts.num_batches, 100000 + nbat, threads)
         nbat += 1
         el = time.perf_counter() - t0
         if el >= seconds or nbat >= 64:
             break
     sweeps = nbat / opts.num_batches
-    return dict(value=g.E * sweeps / el, unit="edges/s", cores=1, kind="port",
+    return dict(value=g.E * sweeps / el, unit="edges/s", cores=threads, kind="port",
                 sample=f"{nbat} nodal batches (each with its block-matrix rebuild) of the CPU "
-                       f"oracle on the same 200k-vertex graph from the final labelling "
-                       f"(B={B}), {el:.1f} s")
+                       f"oracle on {threads} thread(s), same 200k-vertex graph from the final "
+                       f"labelling (B={B}), {el:.1f} s")
 
 
 def compare_forms(ctx, g, lab, prec):
@@ -345,7 +354,9 @@ def main():
     if not a.no_compare:
         out["precision_forms"] = compare_forms(ctx, g, lab, prec)
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(g, lab, rep.B, a.cpu_seconds)
+        nt = cpu_threads()
+        out["cpu_baseline"] = cpu_baseline(g, lab, rep.B, a.cpu_seconds, nt)
+        out["cpu_baseline_1thread"] = cpu_baseline(g, lab, rep.B, a.cpu_seconds, 1)
     ctx.close()
     print(json.dumps(out))
 

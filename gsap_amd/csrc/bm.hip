@@ -62,20 +62,39 @@ __global__ __launch_bounds__(512) void k_tile_hist(const int32_t *optr, const in
   const int32_t nt = (r1 - r0) * B;
   for (int32_t i = threadIdx.x; i < nt; i += blockDim.x) T[i] = 0;
   __syncthreads();
-  // one tile holding every row (vperm == nullptr): chunks of the vertices in id order
-  const int64_t vs = vperm ? (int64_t)H[(int64_t)r0 * W] : 0;
-  const int64_t ve = (vperm && r1 < B) ? (int64_t)H[(int64_t)r1 * W] : N;
+  const int64_t vs = H[(int64_t)r0 * W];
+  const int64_t ve = r1 < B ? (int64_t)H[(int64_t)r1 * W] : N;
   const int64_t c = blockIdx.y;
   const int64_t a0 = vs + (ve - vs) * c / C, a1 = vs + (ve - vs) * (c + 1) / C;
   // a group of 16 lanes per vertex: 4 vertices in flight per wave
   const int q = threadIdx.x & 15, grp = threadIdx.x >> 4, ng = blockDim.x >> 4;
   for (int64_t k = a0 + grp; k < a1; k += ng) {
-    const int32_t v = vperm ? vperm[k] : (int32_t)k;
+    const int32_t v = vperm[k];
     int32_t *row = T + (lab[v] - r0) * B;
     for (int32_t j = optr[v] + q; j < optr[v + 1]; j += 16) atomicAdd(&row[lab[oadj[j]]], ow[j]);
   }
   __syncthreads();
   int32_t *out = partial + c * (int64_t)B * B + (int64_t)r0 * B;
+  for (int32_t i = threadIdx.x; i < nt; i += blockDim.x) out[i] = T[i];
+}
+
+// B <= 128: the whole B x B histogram in LDS; workgroup c takes a contiguous chunk of the
+// edges (id order, one edge per thread per step, so that many label gathers are in flight)
+// and writes its plane of partial counts (or D itself when C == 1).
+__global__ __launch_bounds__(512) void k_edge_hist(const int32_t *esrc, const int32_t *oadj,
+                                                   const int32_t *ow, const int32_t *lab,
+                                                   int64_t E, int32_t B, int32_t C,
+                                                   int32_t *partial) {
+  extern __shared__ int32_t T[];
+  const int32_t nt = B * B;
+  for (int32_t i = threadIdx.x; i < nt; i += blockDim.x) T[i] = 0;
+  __syncthreads();
+  const int64_t c = blockIdx.x;
+  const int64_t a0 = E * c / C, a1 = E * (c + 1) / C;
+  for (int64_t e = a0 + threadIdx.x; e < a1; e += blockDim.x)
+    atomicAdd(&T[lab[esrc[e]] * B + lab[oadj[e]]], ow[e]);
+  __syncthreads();
+  int32_t *out = partial + c * (int64_t)nt;
   for (int32_t i = threadIdx.x; i < nt; i += blockDim.x) out[i] = T[i];
 }
 
@@ -390,15 +409,23 @@ static void dense_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &
     scan_exclusive(rs.vhist, rs.vhist, (int64_t)B * W, nullptr, rs.tmp, st);
     k_label_scatter<<<W, 256, (size_t)B * 4, st>>>(lab, N, B, rs.vhist, W, rs.vperm);
   }
-  // one tile: 256 chunks of the vertex range (one workgroup per CU, few partial planes)
-  int32_t C = one_tile ? 256 : 2048 / ntiles;
-  if ((int64_t)C * BB > ((int64_t)8 << 20)) C = (int32_t)(((int64_t)8 << 20) / BB);
-  if ((int64_t)C * BB > rs.partials_cap) C = (int32_t)(rs.partials_cap / BB);
-  if (C < 1) C = 1;
-  // a single chunk writes its tiles straight into D
-  k_tile_hist<<<dim3(ntiles, C), 512, (size_t)std::min(R, B) * B * 4, st>>>(
-      g.optr, g.oadj, g.ow, lab, one_tile ? nullptr : rs.vperm, rs.vhist, W, N, B, R, C,
-      C == 1 ? D : rs.partials);
+  int32_t C;
+  if (one_tile) {
+    // edge chunks; about 2M partial counters in all (128 planes at B = 128)
+    C = (int32_t)std::min<int64_t>(1024, std::max<int64_t>(128, ((int64_t)2 << 20) / BB));
+    if ((int64_t)C * BB > rs.partials_cap) C = (int32_t)(rs.partials_cap / BB);
+    if ((int64_t)C > g.E) C = (int32_t)std::max<int64_t>(g.E, 1);
+    k_edge_hist<<<C, 512, (size_t)BB * 4, st>>>(g.esrc, g.oadj, g.ow, lab, g.E, B, C,
+                                               C == 1 ? D : rs.partials);
+  } else {
+    C = 2048 / ntiles;
+    if ((int64_t)C * BB > ((int64_t)8 << 20)) C = (int32_t)(((int64_t)8 << 20) / BB);
+    if ((int64_t)C * BB > rs.partials_cap) C = (int32_t)(rs.partials_cap / BB);
+    if (C < 1) C = 1;
+    // a single chunk writes its tiles straight into D
+    k_tile_hist<<<dim3(ntiles, C), 512, (size_t)R * B * 4, st>>>(
+        g.optr, g.oadj, g.ow, lab, rs.vperm, rs.vhist, W, N, B, R, C, C == 1 ? D : rs.partials);
+  }
   if (C == 1) {
   } else if (C <= 4)
     k_hist_reduce<1><<<grid_for(BB, T), T, 0, st>>>(rs.partials, C, BB, D);

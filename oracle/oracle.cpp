@@ -40,6 +40,7 @@
 #include <cstring>
 #include <limits>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -205,8 +206,24 @@ struct BM {
   }
 };
 
+// f(t, lo, hi) on nthr threads over [0, n) in contiguous ranges (results are identical for
+// any thread count: every index is computed by the same code, independently)
+template <class F>
+void par_ranges(int nthr, int64_t n, F &&f) {
+  if (nthr <= 1 || n < 2 * nthr) {
+    f(0, (int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> ts;
+  for (int t = 0; t < nthr; ++t) {
+    const int64_t lo = n * t / nthr, hi = n * (t + 1) / nthr;
+    ts.emplace_back([&f, t, lo, hi] { f(t, lo, hi); });
+  }
+  for (auto &t : ts) t.join();
+}
+
 void build_bm(int64_t N, const int32_t *optr, const int32_t *oadj, const int32_t *ow,
-              const int32_t *lab, int32_t B, BM &m) {
+              const int32_t *lab, int32_t B, BM &m, int nthr = 1) {
   m.B = B;
   // bucket edges by source block (stable), then sort each row by destination block
   std::vector<int64_t> cnt(B + 1, 0);
@@ -221,11 +238,15 @@ void build_bm(int64_t N, const int32_t *optr, const int32_t *oadj, const int32_t
   m.rowptr.assign(B + 1, 0);
   m.col.clear(); m.val.clear();
   m.dout.assign(B, 0); m.din.assign(B, 0);
+  par_ranges(nthr, B, [&](int, int64_t r0, int64_t r1) {
+    for (int64_t r = r0; r < r1; ++r)
+      std::sort(ent.begin() + cnt[r], ent.begin() + cnt[r + 1],
+                [](const std::pair<int32_t, int32_t> &x, const std::pair<int32_t, int32_t> &y) {
+                  return x.first < y.first;
+                });
+  });
   for (int32_t r = 0; r < B; ++r) {
     auto b = ent.begin() + cnt[r], e = ent.begin() + cnt[r + 1];
-    std::sort(b, e, [](const std::pair<int32_t, int32_t> &x, const std::pair<int32_t, int32_t> &y) {
-      return x.first < y.first;
-    });
     for (auto it = b; it != e;) {
       int32_t t = it->first;
       int64_t s = 0;
@@ -527,7 +548,8 @@ double nodal_vertex(const G &g, const BM &m, const int32_t *lab, int64_t v, int3
 }
 
 void nodal_batch(const G &g, const orc_options &o, const Rng &rng, int32_t *lab, const BM &m,
-                 int32_t b, uint32_t phase, int32_t *prop, double *dS, double *dE, Scratch &sc) {
+                 int32_t b, uint32_t phase, int32_t *prop, double *dS, double *dE, Scratch &sc,
+                 int nthr = 1) {
   const int64_t N = g.N, BS = N / o.num_batches, lo = (int64_t)b * BS;
   const int32_t B = m.B;
   // per-block neighbour sample and coin, shared by all vertices (Q5, partition.hpp:485-523)
@@ -543,7 +565,11 @@ void nodal_batch(const G &g, const orc_options &o, const Rng &rng, int32_t *lab,
   sc.ensure(B);
   std::vector<int32_t> ps(BS);
   std::vector<double> ds(BS);
-  for (int64_t i = 0; i < BS; ++i) {
+  std::vector<Scratch> extra(nthr > 1 ? nthr - 1 : 0);
+  par_ranges(nthr, BS, [&](int t, int64_t i0, int64_t i1) {
+  Scratch &tsc = t == 0 ? sc : extra[t - 1];
+  tsc.ensure(B);
+  for (int64_t i = i0; i < i1; ++i) {
     int64_t v = lo + i;
     uint32_t x[4];
     rng.draw((uint32_t)v, phase, TAG_NODE, x);
@@ -566,9 +592,10 @@ void nodal_batch(const G &g, const orc_options &o, const Rng &rng, int32_t *lab,
       s = coin[u] ? (int32_t)mulhi(x[0], B) : sof[u];
     }
     ps[i] = s;
-    double d = nodal_vertex(g, m, lab, v, lab[v], s, sc);
+    double d = nodal_vertex(g, m, lab, v, lab[v], s, tsc);
     ds[i] = d < 0.0 ? d : 0.0;
   }
+  });
   for (int64_t i = 0; i < BS; ++i)
     if (ds[i] < 0.0) lab[lo + i] = ps[i];
   if (prop) std::copy(ps.begin(), ps.end(), prop);
@@ -681,6 +708,17 @@ void orc_nodal_batch(const orc_graph *gg, const orc_options *o, int32_t *labels,
   build_bm(g.N, g.optr, g.oadj, g.ow, labels, B, m);
   Scratch sc;
   nodal_batch(g, *o, rng, labels, m, b, q, prop, dS, dE, sc);
+}
+
+void orc_nodal_batch_mt(const orc_graph *gg, const orc_options *o, int32_t *labels, int32_t B,
+                        int32_t b, uint32_t q, int32_t *prop, double *dS, double *dE,
+                        int32_t threads) {
+  G g(gg);
+  Rng rng{(uint32_t)o->seed, (uint32_t)(o->seed >> 32), o->chain};
+  BM m;
+  build_bm(g.N, g.optr, g.oadj, g.ow, labels, B, m, threads);
+  Scratch sc;
+  nodal_batch(g, *o, rng, labels, m, b, q, prop, dS, dE, sc, threads);
 }
 
 double orc_mdl(const orc_graph *gg, const int32_t *labels, int32_t B, int32_t model) {

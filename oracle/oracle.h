@@ -100,6 +100,11 @@ int32_t orc_merge_phase(const orc_graph *g, const orc_options *o, int32_t *label
  * returns the canonical sum of accepted delta-S in *dE. */
 void orc_nodal_batch(const orc_graph *g, const orc_options *o, int32_t *labels, int32_t B,
                      int32_t b, uint32_t q, int32_t *prop, double *dS, double *dE);
+/* The same batch with the per-vertex proposals / delta-S and the per-row sorts of the
+ * block-matrix build spread over `threads` host threads (identical results). */
+void orc_nodal_batch_mt(const orc_graph *g, const orc_options *o, int32_t *labels, int32_t B,
+                        int32_t b, uint32_t q, int32_t *prop, double *dS, double *dE,
+                        int32_t threads);
 
 /* Total description length of a labelling. */
 double orc_mdl(const orc_graph *g, const int32_t *labels, int32_t B, int32_t model);

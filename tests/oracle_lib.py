@@ -69,6 +69,10 @@ def lib():
         L.orc_nodal_batch.argtypes = [C.POINTER(OGraph), C.POINTER(OOptions), _i32p, C.c_int32,
                                       C.c_int32, C.c_uint32, _i32p, _f64p,
                                       C.POINTER(C.c_double)]
+        L.orc_nodal_batch_mt.restype = None
+        L.orc_nodal_batch_mt.argtypes = [C.POINTER(OGraph), C.POINTER(OOptions), _i32p, C.c_int32,
+                                         C.c_int32, C.c_uint32, _i32p, _f64p,
+                                         C.POINTER(C.c_double), C.c_int32]
         L.orc_mdl.restype = C.c_double
         L.orc_mdl.argtypes = [C.POINTER(OGraph), _i32p, C.c_int32, C.c_int32]
         L.orc_partition.restype = C.c_int
@@ -160,6 +164,17 @@ def nodal_batch(g, opts, labels, B, b, q):
     dE = C.c_double()
     og, oo = ograph(g), ooptions(opts)
     lib().orc_nodal_batch(C.byref(og), C.byref(oo), lab, B, b, q, prop, dS, C.byref(dE))
+    return dict(labels=lab, prop=prop[:BS], dS=dS[:BS], dE=dE.value)
+
+
+def nodal_batch_mt(g, opts, labels, B, b, q, threads):
+    BS = g.N // opts.num_batches
+    lab = np.ascontiguousarray(labels, np.int32).copy()
+    prop = np.zeros(max(BS, 1), np.int32); dS = np.zeros(max(BS, 1), np.float64)
+    dE = C.c_double()
+    og, oo = ograph(g), ooptions(opts)
+    lib().orc_nodal_batch_mt(C.byref(og), C.byref(oo), lab, B, b, q, prop, dS, C.byref(dE),
+                             threads)
     return dict(labels=lab, prop=prop[:BS], dS=dS[:BS], dE=dE.value)
 
 

@@ -337,3 +337,19 @@ def test_oracle_partition_quality(cat):
     p, r = O.pairwise(g.truth, labels)
     if cat == "LL":
         assert p > 0.95 and 8 <= rep.B <= 30
+
+
+def test_multithreaded_batch_is_identical():
+    """The all-cores oracle batch (the CPU baseline's second mode) gives the single-thread
+    results bit for bit: proposals and delta-S are independent within a batch."""
+    g = tsv_graph("HH")
+    rng = np.random.default_rng(12)
+    for B in (g.N, 40, 7):
+        lab = np.arange(g.N, dtype=np.int32) if B == g.N else rng.integers(0, B, g.N).astype(np.int32)
+        o = Options(seed=31)
+        for b in range(o.num_batches):
+            a = O.nodal_batch(g, o, lab, B, b, 5 + b)
+            m = O.nodal_batch_mt(g, o, lab, B, b, 5 + b, 6)
+            assert np.array_equal(a["prop"], m["prop"]) and np.array_equal(a["labels"], m["labels"])
+            assert a["dS"].tobytes() == m["dS"].tobytes()
+            assert np.float64(a["dE"]).tobytes() == np.float64(m["dE"]).tobytes()
