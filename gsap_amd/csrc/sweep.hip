@@ -200,15 +200,18 @@ GSAP_D bool mixed_rejects(double d32, float eb, double Y_r, double Y_s, double Y
 // out-blocks, r and s for rows r and s; v's in-blocks outside {r, s} for columns r and s);
 // the untouched entries contribute (log2 d_old - log2 d_new) * (their summed count).
 // Labels are not written here (batch-Jacobi).  Hash tables of 2^lbits slots live in LDS.
+// Orders the LDS operations of one wave (its tables are private to it).
+GSAP_D void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 template <bool MIX>
-__global__ __launch_bounds__(64, 6) void k_nodal(NodalArgs a, const int32_t *list,
-                                                 uint32_t lbits) {
-  extern __shared__ int32_t lds[];
+GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbits) {
   const uint32_t lcap = 1u << lbits;
   int32_t *lko = lds, *lwo = lds + lcap, *lfo = lds + 2 * lcap;
   int32_t *lki = lds + 3 * lcap, *lwi = lds + 4 * lcap, *lfi = lds + 5 * lcap;
-  const int lane = threadIdx.x;
-  const int64_t v = list[blockIdx.x];
+  const int lane = threadIdx.x & 63;
   const int64_t i = v - a.lo;
   const BMView &m = a.m;
   const int32_t B = m.B;
@@ -254,12 +257,12 @@ __global__ __launch_bounds__(64, 6) void k_nodal(NodalArgs a, const int32_t *lis
   // block degrees of r and s, needed after the tables
   const int64_t dout_r = m.dout[r], dout_s = m.dout[s], din_r = m.din[r], din_s = m.din[s];
   const double ldo_r = m.ldout[r], ldo_s = m.ldout[s], ldi_r = m.ldin[r], ldi_s = m.ldin[s];
-  __syncthreads();
+  wave_lds_sync();
   if (lane < no) ht_insert(ho, t_o0, w_o0, lane);
   if (lane < ni) ht_insert(hi, t_i0, w_i0, lane);
   for (int32_t j = lane + 64; j < no; j += 64) ht_insert(ho, a.lab[a.oadj[o0 + j]], a.ow[o0 + j], j);
   for (int32_t j = lane + 64; j < ni; j += 64) ht_insert(hi, a.lab[a.iadj[i0 + j]], a.iw[i0 + j], j);
-  __syncthreads();
+  wave_lds_sync();
   const int64_t c_r = ht_weight(hi, r), c_s = ht_weight(hi, s);
   const bool r_in_a = ht_find(ho, r) >= 0, s_in_a = ht_find(ho, s) >= 0;
   const double l_dr = clog2((double)(dout_r - kout));
@@ -443,13 +446,10 @@ __global__ __launch_bounds__(64, 6) void k_nodal(NodalArgs a, const int32_t *lis
 // sums them in the canonical order, so the bits equal the one-wave form.  The mixed form
 // evaluates these few vertices in FP64 directly.
 template <bool MIX>
-__global__ __launch_bounds__(256) void k_nodal_big(NodalArgs a, const int32_t *list,
-                                                   uint32_t lds_bytes) {
-  extern __shared__ double lds_d[];
+GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds_bytes) {
   __shared__ int64_t red[4][4];
   __shared__ int64_t corner_s[4];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int64_t v = list[blockIdx.x];
   const int64_t i = v - a.lo;
   const BMView &m = a.m;
   const int32_t B = m.B;
@@ -615,9 +615,8 @@ GSAP_D int64_t group_allsum(int64_t x) {
 // in registers (shuffles inside the group) instead of hash tables, and the block of the
 // sampled neighbour comes from the same gathered values.
 template <int G, bool MIX>
-__global__ __launch_bounds__(256, 5) void k_nodal_small(NodalArgs a, const int32_t *list,
-                                                        int32_t n) {
-  const int32_t gid = (int32_t)(blockIdx.x * (256 / G) + threadIdx.x / G);
+GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int32_t bid) {
+  const int32_t gid = (int32_t)(bid * (256 / G) + threadIdx.x / G);
   if (gid >= n) return;
   const int q = threadIdx.x & (G - 1);
   const int gbase = (threadIdx.x & 63) & ~(G - 1);
@@ -860,6 +859,42 @@ __global__ __launch_bounds__(256, 5) void k_nodal_small(NodalArgs a, const int32
   }
 }
 
+// Vertex lists of one batch by class, and the workgroups of each class in the fused launch
+struct NodalLists {
+  const int32_t *list[NCLS];
+  int32_t n[NCLS];
+  int32_t nb[NCLS];
+  uint32_t big_lds;
+};
+
+// One launch per batch: the largest vertices first (one workgroup each), then 4 vertices
+// per workgroup (a wave each, 6 KB of LDS tables per wave), then groups of 32 and 16 lanes.
+template <bool MIX>
+__global__ __launch_bounds__(256) void k_nodal_fused(NodalArgs a, NodalLists L) {
+  extern __shared__ double lds_d[];
+  int32_t b = (int32_t)blockIdx.x;
+  if (b < L.nb[3]) {
+    nodal_big<MIX>(a, L.list[3][b], lds_d, L.big_lds);
+    return;
+  }
+  b -= L.nb[3];
+  if (b < L.nb[2]) {
+    const int32_t k = b * 4 + (int32_t)(threadIdx.x >> 6);
+    if (k < L.n[2])
+      nodal_wave<MIX>(a, L.list[2][k],
+                      reinterpret_cast<int32_t *>(lds_d) + (threadIdx.x >> 6) * 6 * LCAP,
+                      (uint32_t)LBITS);
+    return;
+  }
+  b -= L.nb[2];
+  if (b < L.nb[1]) {
+    nodal_small<32, MIX>(a, L.list[1], L.n[1], b);
+    return;
+  }
+  b -= L.nb[1];
+  nodal_small<16, MIX>(a, L.list[0], L.n[0], b);
+}
+
 void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const int32_t *sof,
                   const uint8_t *coin, RngKey key, uint32_t phase, int64_t lo, int64_t BS,
                   const int32_t *const lists[NCLS], const int32_t counts[NCLS], uint32_t big_lds,
@@ -874,27 +909,23 @@ void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const 
   a.prop = prop; a.dS = dS;
   a.hoff = g.hoff; a.hbits_o = g.hbits_o; a.hbits_i = g.hbits_i; a.hpool = g.hpool;
   a.err = err; a.near = near;
-  if (err) {
-    if (counts[0] > 0)
-      k_nodal_small<16, true><<<grid_for(counts[0], 16), 256, 0, st>>>(a, lists[0], counts[0]);
-    if (counts[1] > 0)
-      k_nodal_small<32, true><<<grid_for(counts[1], 8), 256, 0, st>>>(a, lists[1], counts[1]);
-    if (counts[2] > 0)
-      k_nodal<true><<<(unsigned)counts[2], 64, 6 * 4 * (size_t)LCAP, st>>>(a, lists[2],
-                                                                          (uint32_t)LBITS);
-    if (counts[3] > 0)
-      k_nodal_big<true><<<(unsigned)counts[3], 256, big_lds, st>>>(a, lists[3], big_lds);
-    return;
+  NodalLists L;
+  const int per_block[NCLS] = {16, 8, 4, 1};
+  int64_t total = 0;
+  for (int k = 0; k < NCLS; ++k) {
+    L.list[k] = lists[k];
+    L.n[k] = counts[k];
+    L.nb[k] = (counts[k] + per_block[k] - 1) / per_block[k];
+    total += L.nb[k];
   }
-  if (counts[0] > 0)
-    k_nodal_small<16, false><<<grid_for(counts[0], 16), 256, 0, st>>>(a, lists[0], counts[0]);
-  if (counts[1] > 0)
-    k_nodal_small<32, false><<<grid_for(counts[1], 8), 256, 0, st>>>(a, lists[1], counts[1]);
-  if (counts[2] > 0)
-    k_nodal<false><<<(unsigned)counts[2], 64, 6 * 4 * (size_t)LCAP, st>>>(a, lists[2],
-                                                                         (uint32_t)LBITS);
-  if (counts[3] > 0)
-    k_nodal_big<false><<<(unsigned)counts[3], 256, big_lds, st>>>(a, lists[3], big_lds);
+  L.big_lds = big_lds;
+  if (total == 0) return;
+  size_t lds = (size_t)4 * 6 * 4 * LCAP;  // four waves' tables
+  if (L.nb[3] > 0 && big_lds > lds) lds = big_lds;
+  if (err)
+    k_nodal_fused<true><<<(unsigned)total, 256, lds, st>>>(a, L);
+  else
+    k_nodal_fused<false><<<(unsigned)total, 256, lds, st>>>(a, L);
 }
 
 // Mixed-form statistics of one batch: max |dS32 - dS64| (float bits, order-preserving for
