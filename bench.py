@@ -34,9 +34,10 @@ KCLASS = ["nodal", "rebuild", "block_terms", "merge", "mdl", "accept_sum"]
 # device kernels of each class (the library times a class with HIP events around all of them)
 KERNELS_OF = {
     "nodal": ["k_nodal"],
-    "rebuild": ["k_label_", "k_tile_hist", "k_hist_reduce", "k_dense_", "k_transpose", "k_logs",
-                "k_make_keys", "k_make_tkeys", "k_radix_", "k_run_flags", "k_rle", "k_scan_",
-                "k_fill_i32", "k_ptr_from_major", "k_seg_sum", "k_csc_unpack", "k_copy_i32"],
+    "rebuild": ["k_label_", "k_tile_hist", "k_edge_hist", "k_hist_reduce", "k_dense_",
+                "k_small_bm", "k_transpose", "k_logs", "k_make_keys", "k_make_tkeys", "k_radix_",
+                "k_run_flags", "k_rle", "k_scan_", "k_fill_i32", "k_ptr_from_major", "k_seg_sum",
+                "k_csc_unpack", "k_copy_i32"],
     "block_terms": ["k_block_terms"],
     "merge": ["k_merge", "k_argmin"],
 }
@@ -65,8 +66,10 @@ def summarize_profile(stats_csv, fetch_csv, write_csv, cal_fetch_csv, cal_write_
             agg[r["Kernel_Name"]] += float(r["Counter_Value"])
         return agg, rows
 
-    fetch, _ = per_dispatch(fetch_csv)
+    fetch, frows = per_dispatch(fetch_csv)
     write, _ = per_dispatch(write_csv)
+    # partitions in the profiled process (one k_iota dispatch each)
+    n_part = max(1, sum(1 for r in frows if "k_iota" in r["Kernel_Name"]))
     cf, crow = per_dispatch(cal_fetch_csv)
     cw, _ = per_dispatch(cal_write_csv)
     copy = [k for k in cf if "copyBuffer" in k][0]
@@ -101,7 +104,8 @@ def summarize_profile(stats_csv, fetch_csv, write_csv, cal_fetch_csv, cal_write_
     open(out_md, "w").write("\n".join(lines) + "\n")
     bench = json.loads([l for l in open(bench_log) if l.startswith("{")][-1])
     json.dump({"command": command, "fetch_factor": f_fac, "write_factor": w_fac,
-               "class_launches": bench["kernel_calls_per_step"],
+               "partitions_profiled": n_part,
+               "class_launches": {k: n_part * v for k, v in bench["kernel_calls_per_step"].items()},
                "class_bytes_total": {k: v[0] + v[1] for k, v in cls_bytes.items()},
                "class_read_total": {k: v[0] for k, v in cls_bytes.items()},
                "class_write_total": {k: v[1] for k, v in cls_bytes.items()}},

@@ -317,33 +317,29 @@ __global__ void k_ptr_from_major(const int32_t *maj, const int32_t *nnz_dev, int
   }
 }
 
-template <class K>
-__global__ void k_make_tkeys(const int32_t *col, const int32_t *erow, const int32_t *val,
-                             const int32_t *nnz_dev, int64_t n, int32_t B, K pad, K *keys,
-                             uint32_t *vals) {
+// CSC keys: the column of every CSR entry (entries past nnz get B and sort last); the
+// value is the CSR position.  A stable sort by column of the row-major CSR order leaves
+// rows ascending inside each column.
+__global__ void k_make_ckeys(const int32_t *col, const int32_t *nnz_dev, int64_t n, int32_t B,
+                             uint32_t *keys, uint32_t *vals) {
   const int32_t nnz = *nnz_dev;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += (int64_t)gridDim.x * blockDim.x) {
-    if (k < nnz) {
-      keys[k] = (K)col[k] * (K)B + (K)erow[k];
-      vals[k] = (uint32_t)val[k];
-    } else {
-      keys[k] = pad;
-      vals[k] = 0;
-    }
+    keys[k] = k < nnz ? (uint32_t)col[k] : (uint32_t)B;
+    vals[k] = (uint32_t)k;
   }
 }
 
-template <class K>
-__global__ void k_csc_unpack(const K *keys, const uint32_t *vals, const int32_t *nnz_dev,
-                             int32_t B, int32_t *row, int32_t *cval, int32_t *ccol) {
+__global__ void k_csc_gather(const uint32_t *keys, const uint32_t *vals, const int32_t *nnz_dev,
+                             const int32_t *erow, const int32_t *val, int32_t *row,
+                             int32_t *cval, int32_t *ccol) {
   const int32_t nnz = *nnz_dev;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
        k += (int64_t)gridDim.x * blockDim.x) {
-    K key = keys[k];
-    row[k] = (int32_t)(key % (K)B);
-    ccol[k] = (int32_t)(key / (K)B);
-    cval[k] = (int32_t)vals[k];
+    const uint32_t i = vals[k];
+    row[k] = erow[i];
+    cval[k] = val[i];
+    ccol[k] = (int32_t)keys[k];
   }
 }
 
@@ -380,14 +376,14 @@ static void sparse_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf 
                                          bm.erow);
   k_ptr_from_major<<<grid_for(E, T), T, 0, st>>>(bm.erow, nnz_dev, B, bm.rowptr);
   k_seg_sum<<<(unsigned)((B + 3) / 4), 256, 0, st>>>(bm.rowptr, bm.val, B, bm.dout);
-  // CSC: sort (col, row) keys of the nnz entries; padding sorts behind them
-  const K pad = (bits >= (int)(8 * sizeof(K))) ? (K)~(K)0 : (K)(((K)1 << bits) - 1);
-  k0 = (K *)rs.k0; k1 = (K *)rs.k1; v0 = rs.v0; v1 = rs.v1;
-  k_make_tkeys<K><<<grid_for(E, T), T, 0, st>>>(bm.col, bm.erow, bm.val, nnz_dev, E, B, pad, k0,
-                                                v0);
-  radix_sort_pairs<K>(k0, v0, k1, v1, E, bits, rs.tmp, st);
+  // CSC: stable sort of the CSR entries by column alone
+  uint32_t *c0 = (uint32_t *)rs.k0, *c1 = (uint32_t *)rs.k1;
+  v0 = rs.v0; v1 = rs.v1;
+  k_make_ckeys<<<grid_for(E, T), T, 0, st>>>(bm.col, nnz_dev, E, B, c0, v0);
+  radix_sort_pairs<uint32_t>(c0, v0, c1, v1, E, bits_for((uint64_t)B + 1), rs.tmp, st);
   int32_t *ccol = (int32_t *)rs.flags;  // reuse: column of each CSC entry
-  k_csc_unpack<K><<<grid_for(E, T), T, 0, st>>>(k0, v0, nnz_dev, B, bm.row, bm.cval, ccol);
+  k_csc_gather<<<grid_for(E, T), T, 0, st>>>(c0, v0, nnz_dev, bm.erow, bm.val, bm.row, bm.cval,
+                                            ccol);
   k_copy_i32<<<1, 1, 0, st>>>(nnz_dev, bm.colptr + B);
   k_ptr_from_major<<<grid_for(E, T), T, 0, st>>>(ccol, nnz_dev, B, bm.colptr);
   k_seg_sum<<<(unsigned)((B + 3) / 4), 256, 0, st>>>(bm.colptr, bm.cval, B, bm.din);
