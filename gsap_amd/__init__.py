@@ -51,7 +51,8 @@ class COptions(C.Structure):
                 ("max_levels", C.c_int32), ("precision", C.c_int32), ("chain", C.c_uint32),
                 ("trace", C.c_int32), ("dense_max_B", C.c_int32), ("profile", C.c_int32),
                 ("model", C.c_int32), ("workers", C.c_int32),
-                ("profile_classes", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("profile_classes", C.c_int32), ("graph", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
 
 
 class CReport(C.Structure):
@@ -196,13 +197,14 @@ class Options:
     profile_classes: int = 0
     model: int = 0  # 0: the reference's model term, 1: corrected (x ln x scaled by E)
     workers: int = 0  # batch / chain calls: concurrent sub-contexts (0 = 8)
+    graph: int = 0  # 1: each level's sweep captured once and replayed as a HIP graph
 
     def cstruct(self) -> COptions:
         o = COptions()
         for f in ("seed", "num_proposals", "reduction_rate", "max_nodal_itr", "num_batches",
                   "threshold1", "threshold2", "moving_avg_window", "max_levels", "precision",
                   "chain", "trace", "dense_max_B", "profile", "model", "workers",
-                  "profile_classes"):
+                  "profile_classes", "graph"):
             setattr(o, f, getattr(self, f))
         return o
 

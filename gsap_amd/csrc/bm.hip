@@ -4,12 +4,20 @@
 // calls, run flags, remove_if, scans and mgpu::segreduce).
 //
 // Two paths, chosen by B:
-//   dense   (B <= dense_max_B): M is histogrammed into a B*B int32 matrix -- per-workgroup
-//           LDS histograms + a column-parallel reduction when B*B fits 64 KB of LDS, global
-//           atomics otherwise -- then compacted row-wise (wave ballots) and column-wise.
-//           The dense matrix stays valid for O(1) M[r][t] lookups by the sweep kernels.
-//   sparse  (larger B): key = b(u)*B + b(v) per edge, stable radix sort, run-length encode
-//           (integer atomics, exact), then a second radix sort of (col, row) keys for the CSC.
+//   small    (B <= 128 with the dense matrix): the whole B x B histogram in LDS per
+//            workgroup over a chunk of the edges, a reduction of the partial planes, then
+//            CSR + CSC + degrees from D in one workgroup.
+//   segmented (every other B): segment x < B holds the out-edges of the vertices of block
+//            x keyed by the target's block (row x of M), segment B + x the in-edges of
+//            block x keyed by the source's block (column x).  Vertices reserve their slice
+//            of a segment with one atomic; each segment is then sorted and run-length
+//            reduced by a team sized to it (registers for <= 64 entries, one wave's LDS for
+//            <= 512, one workgroup's LDS for <= 4096, LDS counters over column ranges
+//            above that).  The integer results do not depend on the order in which the
+//            vertices reserved their slices.
+// The dense B x B copy of M (kept when B <= dense_max_B, for O(1) lookups by the sweep
+// kernels) is maintained in O(nnz): every rebuild first clears the previous nonzeros, so
+// the buffer holds zeros everywhere else.
 #include <algorithm>
 
 #include "engine.h"
@@ -22,62 +30,16 @@ __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
     p[i] = v;
 }
 
-// ---- vertices grouped by block (counting sort; order inside a block is irrelevant)
-constexpr int VT = 2048;  // vertices per workgroup
-
-__global__ __launch_bounds__(256) void k_label_hist(const int32_t *lab, int64_t N, int32_t B,
-                                                    uint32_t *H, int32_t W) {
-  extern __shared__ uint32_t cnt[];
-  for (int32_t b = threadIdx.x; b < B; b += blockDim.x) cnt[b] = 0;
-  __syncthreads();
-  const int64_t v0 = (int64_t)blockIdx.x * VT;
-  for (int64_t v = v0 + threadIdx.x; v < v0 + VT && v < N; v += blockDim.x)
-    atomicAdd(&cnt[lab[v]], 1u);
-  __syncthreads();
-  for (int32_t b = threadIdx.x; b < B; b += blockDim.x) H[(int64_t)b * W + blockIdx.x] = cnt[b];
+// dense[r * B + c] = 0 for every entry (r, c) of the previous CSR (nnz on the device)
+__global__ void k_dense_clear(const int32_t *erow, const int32_t *col, const int32_t *nnz_dev,
+                              int32_t B, int32_t *dense) {
+  const int32_t nnz = *nnz_dev;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
+       k += (int64_t)gridDim.x * blockDim.x)
+    dense[(int64_t)erow[k] * B + col[k]] = 0;
 }
 
-__global__ __launch_bounds__(256) void k_label_scatter(const int32_t *lab, int64_t N, int32_t B,
-                                                       const uint32_t *H, int32_t W,
-                                                       int32_t *vperm) {
-  extern __shared__ uint32_t cur[];
-  for (int32_t b = threadIdx.x; b < B; b += blockDim.x) cur[b] = H[(int64_t)b * W + blockIdx.x];
-  __syncthreads();
-  const int64_t v0 = (int64_t)blockIdx.x * VT;
-  for (int64_t v = v0 + threadIdx.x; v < v0 + VT && v < N; v += blockDim.x)
-    vperm[atomicAdd(&cur[lab[v]], 1u)] = (int32_t)v;
-}
-
-// ---- M by row tiles in LDS: workgroup (tile, chunk) owns rows [r0, r0+R) and one chunk of
-// those rows' vertices; each wave takes a vertex and its lanes add its out-edges into the
-// LDS tile; the tile goes out as a plane of partial counts (no global atomics).
-__global__ __launch_bounds__(512) void k_tile_hist(const int32_t *optr, const int32_t *oadj,
-                                                   const int32_t *ow, const int32_t *lab,
-                                                   const int32_t *vperm, const uint32_t *H,
-                                                   int32_t W, int64_t N, int32_t B, int32_t R,
-                                                   int32_t C, int32_t *partial) {
-  extern __shared__ int32_t T[];
-  const int32_t r0 = blockIdx.x * R;
-  const int32_t r1 = r0 + R < B ? r0 + R : B;
-  const int32_t nt = (r1 - r0) * B;
-  for (int32_t i = threadIdx.x; i < nt; i += blockDim.x) T[i] = 0;
-  __syncthreads();
-  const int64_t vs = H[(int64_t)r0 * W];
-  const int64_t ve = r1 < B ? (int64_t)H[(int64_t)r1 * W] : N;
-  const int64_t c = blockIdx.y;
-  const int64_t a0 = vs + (ve - vs) * c / C, a1 = vs + (ve - vs) * (c + 1) / C;
-  // a group of 16 lanes per vertex: 4 vertices in flight per wave
-  const int q = threadIdx.x & 15, grp = threadIdx.x >> 4, ng = blockDim.x >> 4;
-  for (int64_t k = a0 + grp; k < a1; k += ng) {
-    const int32_t v = vperm[k];
-    int32_t *row = T + (lab[v] - r0) * B;
-    for (int32_t j = optr[v] + q; j < optr[v + 1]; j += 16) atomicAdd(&row[lab[oadj[j]]], ow[j]);
-  }
-  __syncthreads();
-  int32_t *out = partial + c * (int64_t)B * B + (int64_t)r0 * B;
-  for (int32_t i = threadIdx.x; i < nt; i += blockDim.x) out[i] = T[i];
-}
-
+// ------------------------------------------------------------------ small path
 // B <= 128: the whole B x B histogram in LDS; workgroup c takes a contiguous chunk of the
 // edges (id order, one edge per thread per step, so that many label gathers are in flight)
 // and writes its plane of partial counts (or D itself when C == 1).
@@ -113,84 +75,18 @@ __global__ __launch_bounds__(256) void k_hist_reduce(const int32_t *partials, in
   if (q == 0) D[i] = s;
 }
 
-// one wave per row: nonzero count and row sum
-__global__ __launch_bounds__(64) void k_dense_rows(const int32_t *D, int32_t B, uint32_t *rcnt,
-                                                   int32_t *dout) {
-  const int32_t r = blockIdx.x;
-  const int lane = threadIdx.x;
-  uint32_t c = 0;
-  int32_t s = 0;
-  for (int32_t t = lane; t < B; t += 64) {
-    int32_t x = D[(int64_t)r * B + t];
-    c += x != 0;
-    s += x;
-  }
-  for (int off = 32; off >= 1; off >>= 1) {
-    c += __shfl_down(c, off, 64);
-    s += __shfl_down(s, off, 64);
-  }
-  if (lane == 0) { rcnt[r] = c; dout[r] = s; }
-}
-
-__global__ __launch_bounds__(64) void k_dense_write_rows(const int32_t *D, int32_t B,
-                                                         const int32_t *rowptr, int32_t *col,
-                                                         int32_t *val, int32_t *erow) {
-  const int32_t r = blockIdx.x;
-  const int lane = threadIdx.x;
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  int32_t run = rowptr[r];
-  for (int32_t base = 0; base < B; base += 64) {
-    int32_t t = base + lane;
-    int32_t x = t < B ? D[(int64_t)r * B + t] : 0;
-    uint64_t m = __ballot(x != 0);
-    if (x != 0) {
-      int32_t pos = run + (int32_t)__popcll(m & lt);
-      col[pos] = t; val[pos] = x;
-      if (erow) erow[pos] = r;
-    }
-    run += (int32_t)__popcll(m);
-  }
-}
-
-// one wave per segment: out[x] = sum of vals[ptr[x] .. ptr[x+1])
-__global__ __launch_bounds__(256) void k_seg_sum(const int32_t *ptr, const int32_t *vals,
-                                                 int32_t B, int32_t *out) {
-  const int32_t x = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (x >= B) return;
-  const int lane = threadIdx.x & 63;
-  int32_t s = 0;
-  for (int32_t k = ptr[x] + lane; k < ptr[x + 1]; k += 64) s += vals[k];
-  for (int off = 32; off >= 1; off >>= 1) s += __shfl_down(s, off, 64);
-  if (lane == 0) out[x] = s;
-}
-
-// DT = D^T through 32x32 LDS tiles (coalesced reads and writes)
-__global__ __launch_bounds__(256) void k_transpose(const int32_t *D, int32_t B, int32_t *DT) {
-  __shared__ int32_t tile[32][33];
-  const int32_t bx = blockIdx.x * 32, by = blockIdx.y * 32;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  for (int k = ty; k < 32; k += 8) {
-    const int32_t r = by + k, c = bx + tx;
-    if (r < B && c < B) tile[k][tx] = D[(int64_t)r * B + c];
-  }
-  __syncthreads();
-  for (int k = ty; k < 32; k += 8) {
-    const int32_t r = bx + k, c = by + tx;
-    if (r < B && c < B) DT[(int64_t)r * B + c] = tile[tx][k];
-  }
-}
-
 __global__ void k_logs(const int32_t *dout, const int32_t *din, int32_t B, double *ldout,
                        double *ldin) {
-  const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= B) return;
-  ldout[x] = clog2((double)dout[x]);
-  ldin[x] = clog2((double)din[x]);
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < B;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    ldout[x] = clog2((double)dout[x]);
+    ldin[x] = clog2((double)din[x]);
+  }
 }
 
 // Small block matrices (B <= 128): CSR, CSC, degrees and their logs from D in one
 // workgroup of 1024 threads (16 waves).  Row r (column c) is one wave's ballot compaction
-// of D's row (column) in ascending order, as k_dense_write_rows does.
+// of D's row (column) in ascending order.
 constexpr int SMALL_B = 128;
 __global__ __launch_bounds__(1024) void k_small_bm(const int32_t *D, int32_t B, int32_t *rowptr,
                                                    int32_t *col, int32_t *val, int32_t *erow,
@@ -263,87 +159,729 @@ __global__ __launch_bounds__(1024) void k_small_bm(const int32_t *D, int32_t B, 
   }
 }
 
-// ---------------------------------------------------------------- sparse path
-template <class K>
-__global__ void k_make_keys(const int32_t *esrc, const int32_t *oadj, const int32_t *ow,
-                            const int32_t *lab, int64_t E, int32_t B, K *keys, uint32_t *vals) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    keys[e] = (K)lab[esrc[e]] * (K)B + (K)lab[oadj[e]];
-    vals[e] = (uint32_t)ow[e];
+// ------------------------------------------------------------------- tile path
+// ---- vertices grouped by block (counting sort; order inside a block is irrelevant)
+constexpr int VT = 2048;  // vertices per workgroup
+
+__global__ __launch_bounds__(256) void k_label_hist(const int32_t *lab, int64_t N, int32_t B,
+                                                    uint32_t *H, int32_t W) {
+  extern __shared__ uint32_t cnt[];
+  for (int32_t b = threadIdx.x; b < B; b += blockDim.x) cnt[b] = 0;
+  __syncthreads();
+  const int64_t v0 = (int64_t)blockIdx.x * VT;
+  for (int64_t v = v0 + threadIdx.x; v < v0 + VT && v < N; v += blockDim.x)
+    atomicAdd(&cnt[lab[v]], 1u);
+  __syncthreads();
+  for (int32_t b = threadIdx.x; b < B; b += blockDim.x) H[(int64_t)b * W + blockIdx.x] = cnt[b];
+}
+
+__global__ __launch_bounds__(256) void k_label_scatter(const int32_t *lab, int64_t N, int32_t B,
+                                                       const uint32_t *H, int32_t W,
+                                                       int32_t *vperm) {
+  extern __shared__ uint32_t cur[];
+  for (int32_t b = threadIdx.x; b < B; b += blockDim.x) cur[b] = H[(int64_t)b * W + blockIdx.x];
+  __syncthreads();
+  const int64_t v0 = (int64_t)blockIdx.x * VT;
+  for (int64_t v = v0 + threadIdx.x; v < v0 + VT && v < N; v += blockDim.x)
+    vperm[atomicAdd(&cur[lab[v]], 1u)] = (int32_t)v;
+}
+
+// ---- M by row tiles in LDS: workgroup (tile, chunk) owns rows [r0, r0+R) and one chunk of
+// those rows' vertices; each wave takes a vertex and its lanes add its out-edges into the
+// LDS tile; the tile goes out as a plane of partial counts (no global atomics).
+__global__ __launch_bounds__(512) void k_tile_hist(const int32_t *optr, const int32_t *oadj,
+                                                   const int32_t *ow, const int32_t *lab,
+                                                   const int32_t *vperm, const uint32_t *H,
+                                                   int32_t W, int64_t N, int32_t B, int32_t R,
+                                                   int32_t C, int32_t *partial) {
+  extern __shared__ int32_t T[];
+  const int32_t r0 = blockIdx.x * R;
+  const int32_t r1 = r0 + R < B ? r0 + R : B;
+  const int32_t nt = (r1 - r0) * B;
+  for (int32_t i = threadIdx.x; i < nt; i += blockDim.x) T[i] = 0;
+  __syncthreads();
+  const int64_t vs = H[(int64_t)r0 * W];
+  const int64_t ve = r1 < B ? (int64_t)H[(int64_t)r1 * W] : N;
+  const int64_t c = blockIdx.y;
+  const int64_t a0 = vs + (ve - vs) * c / C, a1 = vs + (ve - vs) * (c + 1) / C;
+  // a group of 16 lanes per vertex: 4 vertices in flight per wave
+  const int q = threadIdx.x & 15, grp = threadIdx.x >> 4, ng = blockDim.x >> 4;
+  for (int64_t k = a0 + grp; k < a1; k += ng) {
+    const int32_t v = vperm[k];
+    int32_t *row = T + (lab[v] - r0) * B;
+    for (int32_t j = optr[v] + q; j < optr[v + 1]; j += 16) atomicAdd(&row[lab[oadj[j]]], ow[j]);
+  }
+  __syncthreads();
+  int32_t *out = partial + c * (int64_t)B * B + (int64_t)r0 * B;
+  for (int32_t i = threadIdx.x; i < nt; i += blockDim.x) out[i] = T[i];
+}
+
+// one wave per row: nonzero count and row sum
+__global__ __launch_bounds__(64) void k_dense_rows(const int32_t *D, int32_t B, uint32_t *rcnt,
+                                                   int32_t *dout) {
+  const int32_t r = blockIdx.x;
+  const int lane = threadIdx.x;
+  uint32_t c = 0;
+  int32_t s = 0;
+  for (int32_t t = lane; t < B; t += 64) {
+    int32_t x = D[(int64_t)r * B + t];
+    c += x != 0;
+    s += x;
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    c += __shfl_down(c, off, 64);
+    s += __shfl_down(s, off, 64);
+  }
+  if (lane == 0) { rcnt[r] = c; dout[r] = s; }
+}
+
+__global__ __launch_bounds__(64) void k_dense_write_rows(const int32_t *D, int32_t B,
+                                                         const int32_t *rowptr, int32_t *col,
+                                                         int32_t *val, int32_t *erow) {
+  const int32_t r = blockIdx.x;
+  const int lane = threadIdx.x;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int32_t run = rowptr[r];
+  for (int32_t base = 0; base < B; base += 64) {
+    int32_t t = base + lane;
+    int32_t x = t < B ? D[(int64_t)r * B + t] : 0;
+    uint64_t m = __ballot(x != 0);
+    if (x != 0) {
+      int32_t pos = run + (int32_t)__popcll(m & lt);
+      col[pos] = t; val[pos] = x;
+      if (erow) erow[pos] = r;
+    }
+    run += (int32_t)__popcll(m);
   }
 }
 
-template <class K>
-__global__ void k_run_flags(const K *keys, int64_t E, uint32_t *flags) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
-       e += (int64_t)gridDim.x * blockDim.x)
-    flags[e] = (e == 0 || keys[e] != keys[e - 1]) ? 1u : 0u;
+// DT = D^T through 32x32 LDS tiles (coalesced reads and writes)
+__global__ __launch_bounds__(256) void k_transpose(const int32_t *D, int32_t B, int32_t *DT) {
+  __shared__ int32_t tile[32][33];
+  const int32_t bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int k = ty; k < 32; k += 8) {
+    const int32_t r = by + k, c = bx + tx;
+    if (r < B && c < B) tile[k][tx] = D[(int64_t)r * B + c];
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int32_t r = bx + k, c = by + tx;
+    if (r < B && c < B) DT[(int64_t)r * B + c] = tile[tx][k];
+  }
 }
 
-template <class K>
-__global__ void k_rle(const K *keys, const uint32_t *vals, const uint32_t *flags,
-                      const uint32_t *uidx, int64_t E, int32_t B, int32_t *col, int32_t *val,
-                      int32_t *erow) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    // uidx is the exclusive scan of the run-head flags: a run's index is
-    // uidx + flag - 1 for every element of the run
-    uint32_t i = uidx[e] + flags[e] - 1u;
-    atomicAdd(&val[i], (int32_t)vals[e]);
-    if (flags[e]) {
-      K k = keys[e];
-      col[i] = (int32_t)(k % (K)B);
-      erow[i] = (int32_t)(k / (K)B);
+// -------------------------------------------------------------- segmented path
+// Segment classes by entry count n: registers in groups of 16 lanes (n <= 16) or one wave
+// (n <= 64), one wave's LDS (n <= 512).  Larger segments: when B fits one column range
+// (B <= SEG_RC) a workgroup counts the segment's weights per key in LDS and writes the
+// nonzero keys in order over the segment's start (one pass); otherwise one workgroup's LDS
+// sort takes n <= 4096 and larger segments are counted per (segment, column range) in two
+// passes (counts, then writes at the final position).
+enum { CL_A = 0, CL_A2 = 1, CL_W = 2, CL_G = 3, CL_C = 4 };
+// SEG_RC counters of a column range plus the workgroup's reduction words stay under 64 KB
+constexpr int SEG_W = 512, SEG_G = 4096, SEG_RC = 16000;
+constexpr unsigned SEG_GRID = 1024;  // workgroups of the persistent segment kernels
+constexpr int SEG_PRIV_B = 4096;     // up to this B, reservations are counted in LDS first
+constexpr int SEG_VT = 2048;         // vertices per workgroup of the LDS-counted reservation
+GSAP_HD int seg_class(uint32_t n, bool one_range) {
+  if (n <= 16) return CL_A;
+  if (n <= 64) return CL_A2;
+  if (n <= (uint32_t)SEG_W) return CL_W;
+  if (one_range) return CL_C;
+  return n <= (uint32_t)SEG_G ? CL_G : CL_C;
+}
+
+__global__ void k_seg_zero(uint32_t *off, uint32_t *u, uint32_t *ncl, int32_t *dout,
+                           int32_t *din, int32_t B) {
+  const int64_t n2 = 2 * (int64_t)B;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n2;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    off[i] = 0;
+    if (i < n2) u[i] = 0;
+    if (i < B) { dout[i] = 0; din[i] = 0; }
+  }
+  if (blockIdx.x == 0 && threadIdx.x < SEG_NCL) ncl[threadIdx.x] = 0;
+}
+
+// Segment sizes, each vertex's slice start inside its row segment (vrel[v]) and column
+// segment (vrel[N + v]), and the weighted block degrees.  PRIV: a workgroup counts its
+// SEG_VT vertices per block in LDS and reserves its share of each segment (and adds its
+// degree sums) with one global atomic per block, so that blocks of many vertices do not
+// serialize on one address.
+template <bool PRIV>
+__global__ __launch_bounds__(256) void k_seg_reserve(const int32_t *lab, const int32_t *optr,
+                                                     const int32_t *iptr, const int32_t *kout,
+                                                     const int32_t *kin, int64_t N, int32_t B,
+                                                     uint32_t *cnt, int32_t *dout,
+                                                     int32_t *din, uint32_t *vrel) {
+  if constexpr (!PRIV) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < N;
+         v += (int64_t)gridDim.x * blockDim.x) {
+      const int32_t r = lab[v];
+      const int32_t no = optr[v + 1] - optr[v], ni = iptr[v + 1] - iptr[v];
+      vrel[v] = no ? atomicAdd(&cnt[r], (uint32_t)no) : 0u;
+      vrel[N + v] = ni ? atomicAdd(&cnt[B + r], (uint32_t)ni) : 0u;
+      if (kout[v]) atomicAdd(&dout[r], kout[v]);
+      if (kin[v]) atomicAdd(&din[r], kin[v]);
+    }
+  } else {
+    extern __shared__ uint32_t sh[];
+    uint32_t *cr = sh, *cc = sh + B, *so = sh + 2 * B, *si = sh + 3 * B;
+    for (int32_t i = threadIdx.x; i < 4 * B; i += 256) sh[i] = 0;
+    __syncthreads();
+    constexpr int PER = SEG_VT / 256;
+    const int64_t v0 = (int64_t)blockIdx.x * SEG_VT + threadIdx.x;
+    int32_t rr[PER];
+    uint32_t ro[PER], ri[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int64_t v = v0 + k * 256;
+      rr[k] = -1;
+      ro[k] = ri[k] = 0;
+      if (v < N) {
+        const int32_t r = lab[v];
+        const int32_t no = optr[v + 1] - optr[v], ni = iptr[v + 1] - iptr[v];
+        rr[k] = r;
+        if (no) ro[k] = atomicAdd(&cr[r], (uint32_t)no);
+        if (ni) ri[k] = atomicAdd(&cc[r], (uint32_t)ni);
+        if (kout[v]) atomicAdd(&so[r], (uint32_t)kout[v]);
+        if (kin[v]) atomicAdd(&si[r], (uint32_t)kin[v]);
+      }
+    }
+    __syncthreads();
+    for (int32_t b = threadIdx.x; b < B; b += 256) {
+      if (cr[b]) cr[b] = atomicAdd(&cnt[b], cr[b]);
+      if (cc[b]) cc[b] = atomicAdd(&cnt[B + b], cc[b]);
+      if (so[b]) atomicAdd(&dout[b], (int32_t)so[b]);
+      if (si[b]) atomicAdd(&din[b], (int32_t)si[b]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int64_t v = v0 + k * 256;
+      if (rr[k] >= 0) {
+        vrel[v] = cr[rr[k]] + ro[k];
+        vrel[N + v] = cc[rr[k]] + ri[k];
+      }
     }
   }
 }
 
-// ptr[x] = first entry whose major index (maj[i]) >= x; ptr[B] already holds nnz
-__global__ void k_ptr_from_major(const int32_t *maj, const int32_t *nnz_dev, int32_t B,
-                                 int32_t *ptr) {
-  const int32_t nnz = *nnz_dev;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (nnz == 0) {
-    for (int64_t x = i; x < B; x += (int64_t)gridDim.x * blockDim.x) ptr[x] = 0;
-    return;
-  }
-  for (int64_t k = i; k < nnz; k += (int64_t)gridDim.x * blockDim.x) {
-    int32_t r = maj[k];
-    int32_t prev = k == 0 ? -1 : maj[k - 1];
-    for (int32_t x = prev + 1; x <= r; ++x) ptr[x] = (int32_t)k;
-    if (k == nnz - 1)
-      for (int32_t x = r + 1; x < B; ++x) ptr[x] = nnz;
-  }
-}
-
-// CSC keys: the column of every CSR entry (entries past nnz get B and sort last); the
-// value is the CSR position.  A stable sort by column of the row-major CSR order leaves
-// rows ascending inside each column.
-__global__ void k_make_ckeys(const int32_t *col, const int32_t *nnz_dev, int64_t n, int32_t B,
-                             uint32_t *keys, uint32_t *vals) {
-  const int32_t nnz = *nnz_dev;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-       k += (int64_t)gridDim.x * blockDim.x) {
-    keys[k] = k < nnz ? (uint32_t)col[k] : (uint32_t)B;
-    vals[k] = (uint32_t)k;
+// Edge-parallel: out-edge e goes to its source's slice of the source block's row segment,
+// keyed by the target's block; in-edge f to its target's slice of the target block's column
+// segment, keyed by the source's block.
+__global__ __launch_bounds__(256) void k_seg_fill(const int32_t *lab, const int32_t *esrc,
+                                                  const int32_t *optr, const int32_t *oadj,
+                                                  const int32_t *ow, const int32_t *idst,
+                                                  const int32_t *iptr, const int32_t *iadj,
+                                                  const int32_t *iw, int64_t N, int64_t E,
+                                                  int32_t B, const uint32_t *off,
+                                                  const uint32_t *vrel, uint32_t *ekey,
+                                                  uint32_t *ew) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < 2 * E;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    if (e < E) {
+      const int32_t v = esrc[e];
+      const uint32_t pos = off[lab[v]] + vrel[v] + (uint32_t)(e - optr[v]);
+      ekey[pos] = (uint32_t)lab[oadj[e]];
+      ew[pos] = (uint32_t)ow[e];
+    } else {
+      const int64_t f = e - E;
+      const int32_t v = idst[f];
+      const uint32_t pos = off[B + lab[v]] + vrel[N + v] + (uint32_t)(f - iptr[v]);
+      ekey[pos] = (uint32_t)lab[iadj[f]];
+      ew[pos] = (uint32_t)iw[f];
+    }
   }
 }
 
-__global__ void k_csc_gather(const uint32_t *keys, const uint32_t *vals, const int32_t *nnz_dev,
-                             const int32_t *erow, const int32_t *val, int32_t *row,
-                             int32_t *cval, int32_t *ccol) {
-  const int32_t nnz = *nnz_dev;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
-       k += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t i = vals[k];
-    row[k] = erow[i];
-    cval[k] = val[i];
-    ccol[k] = (int32_t)keys[k];
+// Nonempty segments into per-class lists: per-wave ballots, per-workgroup LDS counts, one
+// global atomic per class and workgroup round (1024 segments).
+__global__ __launch_bounds__(256) void k_seg_classify(const uint32_t *off, int64_t nseg,
+                                                      int one_range, uint32_t *lists,
+                                                      int64_t lcap, uint32_t *ncl) {
+  __shared__ uint32_t wcnt[SEG_NCL], wbase[SEG_NCL];
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int64_t base = (int64_t)blockIdx.x * 1024; base < nseg; base += (int64_t)gridDim.x * 1024) {
+    if (threadIdx.x < SEG_NCL) wcnt[threadIdx.x] = 0;
+    __syncthreads();
+    int c[4];
+    uint32_t woff[4][SEG_NCL];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t x = base + j * 256 + threadIdx.x;
+      c[j] = -1;
+      if (x < nseg) {
+        const uint32_t n = off[x + 1] - off[x];
+        if (n) c[j] = seg_class(n, one_range != 0);
+      }
+#pragma unroll
+      for (int k = 0; k < SEG_NCL; ++k) {
+        const uint64_t m = __ballot(c[j] == k);
+        uint32_t o = 0;
+        if (m && lane == 0) o = atomicAdd(&wcnt[k], (uint32_t)__popcll(m));
+        woff[j][k] = __shfl(o, 0, 64) + (uint32_t)__popcll(m & lt);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < SEG_NCL)
+      wbase[threadIdx.x] = wcnt[threadIdx.x] ? atomicAdd(&ncl[threadIdx.x], wcnt[threadIdx.x]) : 0u;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t x = base + j * 256 + threadIdx.x;
+#pragma unroll
+      for (int k = 0; k < SEG_NCL; ++k)
+        if (c[j] == k) lists[k * lcap + wbase[k] + woff[j][k]] = (uint32_t)x;
+    }
+    __syncthreads();
   }
 }
 
-__global__ void k_copy_i32(const int32_t *a, int32_t *b) { *b = *a; }
+// Sort a segment of n <= G entries held one per lane of a G-lane group (bitonic network on
+// shuffles), then reduce runs of equal keys: run totals by a segmented suffix sum, heads
+// compacted by ballot.  Writes the runs in place at the segment start and their count.
+template <int G>
+GSAP_D void seg_sort_group(uint32_t *ekey, uint32_t *ew, uint32_t s0, int n, uint32_t *uout) {
+  const int q = threadIdx.x & (G - 1);
+  const int gbase = (threadIdx.x & 63) & ~(G - 1);
+  const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+  uint32_t key = q < n ? ekey[s0 + q] : 0xFFFFFFFFu;
+  uint32_t val = q < n ? ew[s0 + q] : 0u;
+#pragma unroll
+  for (int k = 2; k <= G; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const uint32_t pk = __shfl_xor(key, j, G), pv = __shfl_xor(val, j, G);
+      const uint32_t mmin = ((q & k) == 0) == ((q & j) == 0) ? ~0u : 0u;
+      const uint32_t nk = (min(pk, key) & mmin) | (max(pk, key) & ~mmin);
+      val = nk != key ? pv : val;
+      key = nk;
+    }
+  }
+  uint32_t acc = val;
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1) {
+    const uint32_t y = __shfl_down(acc, o, G);
+    const uint32_t ky = __shfl_down(key, o, G);
+    if (q + o < G && ky == key) acc += y;
+  }
+  const uint32_t kp = __shfl_up(key, 1, G);
+  const bool head = q < n && (q == 0 || kp != key);
+  const uint64_t hm = (__ballot(head) >> gbase) & gmask;
+  if (head) {
+    const uint32_t pos = (uint32_t)__popcll(hm & ((1ull << q) - 1ull));
+    ekey[s0 + pos] = key;
+    ew[s0 + pos] = acc;
+  }
+  if (q == 0) *uout = (uint32_t)__popcll(hm);
+}
+
+__global__ __launch_bounds__(256) void k_seg_sort_reg(const uint32_t *off, uint32_t *ekey,
+                                                      uint32_t *ew, const uint32_t *lists,
+                                                      int64_t lcap, const uint32_t *ncl,
+                                                      uint32_t *u) {
+  const int64_t nA2 = ncl[CL_A2], nA = ncl[CL_A];
+  const int64_t tasks = nA2 + (nA + 3) / 4;
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < tasks; t += nw) {
+    if (t < nA2) {
+      const uint32_t x = lists[CL_A2 * lcap + t];
+      seg_sort_group<64>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x]);
+    } else {
+      const int64_t k = (t - nA2) * 4 + ((threadIdx.x & 63) >> 4);
+      if (k < nA) {
+        const uint32_t x = lists[CL_A * lcap + k];
+        seg_sort_group<16>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x]);
+      }
+    }
+  }
+}
+
+// inclusive scan of one value per thread over a team of T threads (one wave, or a 256-thread
+// workgroup using sc4)
+template <int T>
+GSAP_D uint32_t team_incl_scan(uint32_t x, uint32_t *sc4, uint32_t &total) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if constexpr (T == 64) {
+    total = __shfl(x, 63, 64);
+    return x;
+  } else {
+    const int w = threadIdx.x >> 6;
+    if (lane == 63) sc4[w] = x;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < T / 64; ++i) {
+      const uint32_t c = sc4[i];
+      if (i < w) pre += c;
+      tot += c;
+    }
+    __syncthreads();
+    total = tot;
+    return pre + x;
+  }
+}
+
+// Sort a segment of n <= T*R entries held in registers, R per thread in blocked order
+// (element g = thread * R + i), with a bitonic network: steps with j < R compare registers of
+// one thread, R <= j < 64R exchange with lane (lane ^ j/R) by shuffles, larger j (T = 256)
+// go through LDS (xk, xv: T*R words each).  Then runs of equal keys are reduced: run heads
+// record the exclusive prefix of the weights in xv[run], run tails write key and total to
+// the segment's start.  Padding keys are 0xFFFFFFFF (above every block id).
+template <int T, int R>
+GSAP_D void team_sort_regs(uint32_t *ekey, uint32_t *ew, uint32_t s0, int n, uint32_t *uout,
+                           uint32_t *xk, uint32_t *xv, uint32_t *sc4) {
+  const int tid = T == 64 ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+  constexpr int P = T * R;
+  uint32_t key[R], val[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int g = tid * R + i;
+    key[i] = g < n ? ekey[s0 + g] : 0xFFFFFFFFu;
+    val[i] = g < n ? ew[s0 + g] : 0u;
+  }
+#pragma unroll
+  for (int k = 2; k <= P; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j < R) {
+        // branch-free compare-exchange of registers i < l = i ^ j
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          const int l = i ^ j;
+          if (l > i) {
+            const bool asc = ((tid * R + i) & k) == 0;
+            const uint32_t a = key[i], b = key[l], va = val[i], vb = val[l];
+            const bool sw = (a > b) == asc;
+            key[i] = sw ? b : a;
+            key[l] = sw ? a : b;
+            val[i] = sw ? vb : va;
+            val[l] = sw ? va : vb;
+          }
+        }
+      } else if (j < 64 * R) {
+        // all of the step's shuffles first, then branch-free selects; which side keeps the
+        // minimum depends on lane bits only (j, k >= R)
+        const int lm = j / R;
+        uint32_t pk[R], pv[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          pk[i] = __shfl_xor(key[i], lm, 64);
+          pv[i] = __shfl_xor(val[i], lm, 64);
+        }
+        const int g0 = tid * R;
+        const uint32_t mmin = ((g0 & j) == 0) == ((g0 & k) == 0) ? ~0u : 0u;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          const uint32_t nk = (min(pk[i], key[i]) & mmin) | (max(pk[i], key[i]) & ~mmin);
+          val[i] = nk != key[i] ? pv[i] : val[i];
+          key[i] = nk;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          xk[tid * R + i] = key[i];
+          xv[tid * R + i] = val[i];
+        }
+        __syncthreads();
+        const int g0 = tid * R;
+        const uint32_t mmin = ((g0 & j) == 0) == ((g0 & k) == 0) ? ~0u : 0u;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          const uint32_t pk = xk[(g0 + i) ^ j], pv = xv[(g0 + i) ^ j];
+          const uint32_t nk = (min(pk, key[i]) & mmin) | (max(pk, key[i]) & ~mmin);
+          val[i] = nk != key[i] ? pv : val[i];
+          key[i] = nk;
+        }
+        __syncthreads();
+      }
+    }
+  }
+  // neighbours of the thread's first and last element
+  uint32_t kprev, knext;
+  if constexpr (T == 64) {
+    kprev = __shfl_up(key[R - 1], 1, 64);
+    knext = __shfl_down(key[0], 1, 64);
+  } else {
+    xk[tid] = key[R - 1];
+    xk[T + tid] = key[0];
+    __syncthreads();
+    kprev = tid > 0 ? xk[tid - 1] : 0u;
+    knext = tid + 1 < T ? xk[T + tid + 1] : 0xFFFFFFFFu;
+    __syncthreads();
+  }
+  uint32_t hsum = 0, wsum = 0;
+  bool head[R], tail[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int g = tid * R + i;
+    const uint32_t kp = i > 0 ? key[i - 1] : kprev;
+    const uint32_t kn = i + 1 < R ? key[i + 1] : knext;
+    head[i] = g < n && (g == 0 || kp != key[i]);
+    tail[i] = g < n && (g == n - 1 || kn != key[i]);
+    hsum += head[i] ? 1u : 0u;
+    wsum += val[i];
+  }
+  uint32_t th, tw;
+  const uint32_t hi = team_incl_scan<T>(hsum, sc4, th);
+  const uint32_t wi = team_incl_scan<T>(wsum, sc4, tw);
+  uint32_t run = hi - hsum, ex = wi - wsum;  // heads and weight before this thread
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    if (head[i]) xv[run] = ex;
+    run += head[i] ? 1u : 0u;
+    ex += val[i];
+  }
+  if constexpr (T == 64) wave_lds_sync();
+  else __syncthreads();
+  run = hi - hsum;
+  ex = wi - wsum;
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    run += head[i] ? 1u : 0u;
+    if (tail[i]) {
+      ekey[s0 + run - 1] = key[i];
+      ew[s0 + run - 1] = (ex + val[i]) - xv[run - 1];
+    }
+    ex += val[i];
+  }
+  if (tid == 0) *uout = th;
+  if constexpr (T == 64) wave_lds_sync();
+  else __syncthreads();
+}
+
+// one wave per segment of class W (n <= 512): 8 entries per lane, 2 KB of LDS per wave
+__global__ __launch_bounds__(256) void k_seg_sort_w(const uint32_t *off, uint32_t *ekey,
+                                                    uint32_t *ew, const uint32_t *lists,
+                                                    int64_t lcap, const uint32_t *ncl,
+                                                    uint32_t *u) {
+  __shared__ uint32_t lds[4][SEG_W];
+  const int w = threadIdx.x >> 6;
+  const int64_t n = ncl[CL_W];
+  for (int64_t t = (int64_t)blockIdx.x * 4 + w; t < n; t += (int64_t)gridDim.x * 4) {
+    const uint32_t x = lists[CL_W * lcap + t];
+    team_sort_regs<64, SEG_W / 64>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x],
+                                   nullptr, lds[w], nullptr);
+  }
+}
+
+// one workgroup per segment of class G (n <= 4096): 16 entries per thread, 32 KB of LDS
+__global__ __launch_bounds__(256) void k_seg_sort_g(const uint32_t *off, uint32_t *ekey,
+                                                    uint32_t *ew, const uint32_t *lists,
+                                                    int64_t lcap, const uint32_t *ncl,
+                                                    uint32_t *u) {
+  __shared__ uint32_t xk[SEG_G], xv[SEG_G];
+  __shared__ uint32_t sc4[4];
+  const int64_t n = ncl[CL_G];
+  for (int64_t t = blockIdx.x; t < n; t += gridDim.x) {
+    const uint32_t x = lists[CL_G * lcap + t];
+    team_sort_regs<256, SEG_G / 256>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], xk,
+                                     xv, sc4);
+  }
+}
+
+// Class C: weights of the segment's keys in [c0, c0 + cn) counted in LDS by a workgroup of
+// 512 threads (four entries in flight per thread).  Ends with a barrier: every read of the
+// segment is complete.
+GSAP_D void seg_range_count(uint32_t *cnt, const uint32_t *ekey, const uint32_t *ew,
+                            uint32_t s0, uint32_t s1, uint32_t c0, uint32_t cn) {
+  for (uint32_t i = threadIdx.x; i < cn; i += 512) cnt[i] = 0;
+  __syncthreads();
+  for (uint64_t e0 = (uint64_t)s0 + threadIdx.x; e0 < s1; e0 += 4 * 512) {
+    uint32_t k[4], w[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint64_t e = e0 + (uint64_t)j * 512;
+      k[j] = e < s1 ? ekey[e] - c0 : 0xFFFFFFFFu;
+      w[j] = e < s1 ? ew[e] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (k[j] < cn) atomicAdd(&cnt[k[j]], w[j]);
+  }
+  __syncthreads();
+}
+
+GSAP_D uint32_t block_sum512(uint32_t x, uint32_t *red8) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+  if (lane == 0) red8[w] = x;
+  __syncthreads();
+  uint32_t t = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) t += red8[i];
+  __syncthreads();
+  return t;
+}
+
+// Nonzero counters cnt[0..cn) in ascending order to keys c0 + i / weights at out + pos
+// (and the matching dense row when given); returns the number written.
+GSAP_D uint32_t seg_emit(const uint32_t *cnt, uint32_t cn, uint32_t c0, uint32_t pos,
+                         uint32_t *red8, uint32_t *okey, uint32_t *ow_, int32_t *col,
+                         int32_t *val, int32_t *erow, int32_t x_row, int32_t *dense_row) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint32_t start = pos;
+  for (uint32_t base = 0; base < cn; base += 512) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t c = i < cn ? cnt[i] : 0u;
+    const uint64_t m = __ballot(c != 0);
+    if (lane == 0) red8[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t wpre = 0, tot = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t cj = red8[j];
+      if (j < w) wpre += cj;
+      tot += cj;
+    }
+    __syncthreads();
+    if (c != 0) {
+      const uint32_t p = pos + wpre + (uint32_t)__popcll(m & lt);
+      if (okey) {
+        okey[p] = c0 + i;
+        ow_[p] = c;
+      } else {
+        col[p] = (int32_t)(c0 + i);
+        val[p] = (int32_t)c;
+        if (erow) erow[p] = x_row;
+      }
+      if (dense_row) dense_row[c0 + i] = (int32_t)c;
+    }
+    pos += tot;
+  }
+  return pos - start;
+}
+
+// B <= SEG_RC: one workgroup per class-C segment, all B keys in one LDS range; the runs
+// overwrite the segment's start (its reads are complete after the count).
+__global__ __launch_bounds__(512) void k_seg_c1(const uint32_t *off, uint32_t *ekey,
+                                                uint32_t *ew, const uint32_t *lists,
+                                                int64_t lcap, const uint32_t *ncl, int32_t B,
+                                                uint32_t *u) {
+  extern __shared__ uint32_t cnt[];
+  __shared__ uint32_t red8[8];
+  const int64_t n = ncl[CL_C];
+  for (int64_t t = blockIdx.x; t < n; t += gridDim.x) {
+    const uint32_t x = lists[CL_C * lcap + t];
+    const uint32_t s0 = off[x];
+    seg_range_count(cnt, ekey, ew, s0, off[x + 1], 0, (uint32_t)B);
+    const uint32_t m = seg_emit(cnt, (uint32_t)B, 0, s0, red8, ekey, ew, nullptr, nullptr,
+                                nullptr, 0, nullptr);
+    if (threadIdx.x == 0) u[x] = m;
+  }
+}
+
+// B > SEG_RC, pass 1: nonzero count of every (class-C segment, column range)
+__global__ __launch_bounds__(512) void k_seg_c_count(const uint32_t *off, const uint32_t *ekey,
+                                                     const uint32_t *ew, const uint32_t *lists,
+                                                     int64_t lcap, const uint32_t *ncl,
+                                                     int32_t B, int32_t nr, uint32_t *cntc,
+                                                     uint32_t *u) {
+  extern __shared__ uint32_t cnt[];
+  __shared__ uint32_t red8[8];
+  const uint32_t RC = B < SEG_RC ? (uint32_t)B : (uint32_t)SEG_RC;
+  const int64_t items = (int64_t)ncl[CL_C] * nr;
+  for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const int64_t k = it / nr;
+    const uint32_t q = (uint32_t)(it - k * nr);
+    const uint32_t x = lists[CL_C * lcap + k];
+    const uint32_t c0 = q * RC, cn = min(RC, (uint32_t)B - c0);
+    seg_range_count(cnt, ekey, ew, off[x], off[x + 1], c0, cn);
+    uint32_t nz = 0;
+    for (uint32_t i = threadIdx.x; i < cn; i += 512) nz += cnt[i] != 0;
+    nz = block_sum512(nz, red8);
+    if (threadIdx.x == 0) {
+      cntc[it] = nz;
+      if (nz) atomicAdd(&u[x], nz);
+    }
+  }
+}
+
+// B > SEG_RC, pass 2: recount each (segment, range) and write its nonzeros at the
+// segment's final position plus the earlier ranges' counts
+__global__ __launch_bounds__(512) void k_seg_c_write(const uint32_t *off, const uint32_t *ekey,
+                                                     const uint32_t *ew, const uint32_t *lists,
+                                                     int64_t lcap, const uint32_t *ncl,
+                                                     int32_t B, int32_t nr,
+                                                     const uint32_t *cntc, const uint32_t *uoff,
+                                                     int32_t *col, int32_t *val, int32_t *erow,
+                                                     int32_t *row, int32_t *cval,
+                                                     int32_t *dense) {
+  extern __shared__ uint32_t cnt[];
+  __shared__ uint32_t red8[8];
+  const uint32_t RC = B < SEG_RC ? (uint32_t)B : (uint32_t)SEG_RC;
+  const int64_t items = (int64_t)ncl[CL_C] * nr;
+  const uint32_t nnz = uoff[B];
+  for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const int64_t k = it / nr;
+    const uint32_t q = (uint32_t)(it - k * nr);
+    if (cntc[it] == 0) continue;  // uniform across the workgroup
+    const uint32_t x = lists[CL_C * lcap + k];
+    const bool isrow = x < (uint32_t)B;
+    uint32_t pre = 0;
+    for (uint32_t qq = threadIdx.x; qq < q; qq += 512) pre += cntc[k * nr + qq];
+    pre = block_sum512(pre, red8);
+    const uint32_t pos = (isrow ? uoff[x] : uoff[x] - nnz) + pre;
+    const uint32_t c0 = q * RC, cn = min(RC, (uint32_t)B - c0);
+    seg_range_count(cnt, ekey, ew, off[x], off[x + 1], c0, cn);
+    if (isrow)
+      seg_emit(cnt, cn, c0, pos, red8, nullptr, nullptr, col, val, erow, (int32_t)x,
+               dense ? dense + (int64_t)x * B : nullptr);
+    else
+      seg_emit(cnt, cn, c0, pos, red8, nullptr, nullptr, row, cval, nullptr, 0, nullptr);
+  }
+}
+
+// A group of 16 lanes per segment: row/column pointers from the scanned run counts, and the
+// runs of every segment (except two-pass class C) copied from its start to its final place.
+__global__ __launch_bounds__(256) void k_seg_compact(const uint32_t *off, const uint32_t *ekey,
+                                                     const uint32_t *ew, const uint32_t *uoff,
+                                                     int32_t B, int one_range, int32_t *rowptr,
+                                                     int32_t *col, int32_t *val, int32_t *erow,
+                                                     int32_t *colptr, int32_t *row,
+                                                     int32_t *cval, int32_t *dense) {
+  const int q = threadIdx.x & 15;
+  const int64_t ng = ((int64_t)gridDim.x * blockDim.x) >> 4;
+  const uint32_t nnz = uoff[B];
+  for (int64_t x = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; x < 2 * (int64_t)B;
+       x += ng) {
+    const bool isrow = x < B;
+    const uint32_t p = isrow ? uoff[x] : uoff[x] - nnz;
+    if (q == 0) {
+      if (isrow) rowptr[x] = (int32_t)p;
+      else colptr[x - B] = (int32_t)p;
+    }
+    const uint32_t n = off[x + 1] - off[x];
+    if (n == 0 || (!one_range && seg_class(n, false) == CL_C)) continue;
+    const uint32_t cnt = uoff[x + 1] - uoff[x], s0 = off[x];
+    for (uint32_t k = q; k < cnt; k += 16) {
+      const int32_t key = (int32_t)ekey[s0 + k], w = (int32_t)ew[s0 + k];
+      if (isrow) {
+        col[p + k] = key;
+        val[p + k] = w;
+        erow[p + k] = (int32_t)x;
+        if (dense) dense[x * B + key] = w;
+      } else {
+        row[p + k] = key;
+        cval[p + k] = w;
+      }
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    rowptr[B] = (int32_t)nnz;
+    colptr[B] = (int32_t)nnz;
+  }
+}
 
 static unsigned grid_for(int64_t n, int threads, unsigned cap = 65535u * 4u) {
   int64_t g = (n + threads - 1) / threads;
@@ -352,76 +890,78 @@ static unsigned grid_for(int64_t n, int threads, unsigned cap = 65535u * 4u) {
   return (unsigned)g;
 }
 
-static int bits_for(uint64_t x) {  // number of bits needed to represent values < x
-  int b = 0;
-  while (b < 64 && (x - 1) >> b) ++b;
-  return b;
+int64_t seg_cntc_words(int64_t N, int64_t E) {
+  const int64_t nr = (N + SEG_RC - 1) / SEG_RC;
+  return (2 * E / (SEG_G + 1) + 1) * nr;
 }
 
-template <class K>
-static void sparse_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm,
-                        RebuildScratch &rs, hipStream_t st) {
-  const int64_t E = g.E;
+static void seg_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm,
+                     RebuildScratch &rs, bool dense, hipStream_t st) {
+  const int64_t N = g.N, E = g.E, nseg = 2 * (int64_t)B;
   const int T = 256;
-  K *k0 = (K *)rs.k0, *k1 = (K *)rs.k1;
-  uint32_t *v0 = rs.v0, *v1 = rs.v1;
-  const int bits = bits_for((uint64_t)B * (uint64_t)B);
-  int32_t *nnz_dev = bm.rowptr + B;
-  k_make_keys<K><<<grid_for(E, T), T, 0, st>>>(g.esrc, g.oadj, g.ow, lab, E, B, k0, v0);
-  radix_sort_pairs<K>(k0, v0, k1, v1, E, bits, rs.tmp, st);
-  k_run_flags<K><<<grid_for(E, T), T, 0, st>>>(k0, E, rs.flags);
-  scan_exclusive(rs.flags, rs.uidx, E, (uint32_t *)nnz_dev, rs.tmp, st);
-  k_fill_i32<<<grid_for(E, T), T, 0, st>>>(bm.val, E, 0);
-  k_rle<K><<<grid_for(E, T), T, 0, st>>>(k0, v0, rs.flags, rs.uidx, E, B, bm.col, bm.val,
-                                         bm.erow);
-  k_ptr_from_major<<<grid_for(E, T), T, 0, st>>>(bm.erow, nnz_dev, B, bm.rowptr);
-  k_seg_sum<<<(unsigned)((B + 3) / 4), 256, 0, st>>>(bm.rowptr, bm.val, B, bm.dout);
-  // CSC: stable sort of the CSR entries by column alone
-  uint32_t *c0 = (uint32_t *)rs.k0, *c1 = (uint32_t *)rs.k1;
-  v0 = rs.v0; v1 = rs.v1;
-  k_make_ckeys<<<grid_for(E, T), T, 0, st>>>(bm.col, nnz_dev, E, B, c0, v0);
-  radix_sort_pairs<uint32_t>(c0, v0, c1, v1, E, bits_for((uint64_t)B + 1), rs.tmp, st);
-  int32_t *ccol = (int32_t *)rs.flags;  // reuse: column of each CSC entry
-  k_csc_gather<<<grid_for(E, T), T, 0, st>>>(c0, v0, nnz_dev, bm.erow, bm.val, bm.row, bm.cval,
-                                            ccol);
-  k_copy_i32<<<1, 1, 0, st>>>(nnz_dev, bm.colptr + B);
-  k_ptr_from_major<<<grid_for(E, T), T, 0, st>>>(ccol, nnz_dev, B, bm.colptr);
-  k_seg_sum<<<(unsigned)((B + 3) / 4), 256, 0, st>>>(bm.colptr, bm.cval, B, bm.din);
+  uint32_t *ekey = (uint32_t *)rs.k0, *ew = (uint32_t *)rs.k1;
+  const bool one = B <= SEG_RC;
+  const int32_t nr = (int32_t)((B + SEG_RC - 1) / SEG_RC);
+  const uint32_t RC = one ? (uint32_t)B : (uint32_t)SEG_RC;
+  int32_t *D = dense ? bm.dense : nullptr;
+  k_seg_zero<<<grid_for(nseg + 1, T), T, 0, st>>>(rs.soff, rs.su, rs.ncl, bm.dout, bm.din, B);
+  if (B <= SEG_PRIV_B)
+    k_seg_reserve<true><<<(unsigned)((N + SEG_VT - 1) / SEG_VT), 256, (size_t)16 * B, st>>>(
+        lab, g.optr, g.iptr, g.kout, g.kin, N, B, rs.soff, bm.dout, bm.din, rs.vrel);
+  else
+    k_seg_reserve<false><<<grid_for(N, T), T, 0, st>>>(lab, g.optr, g.iptr, g.kout, g.kin, N, B,
+                                                       rs.soff, bm.dout, bm.din, rs.vrel);
+  scan_exclusive(rs.soff, rs.soff, nseg, rs.soff + nseg, rs.tmp, st);
+  k_seg_fill<<<grid_for(2 * E, T), T, 0, st>>>(lab, g.esrc, g.optr, g.oadj, g.ow, g.idst, g.iptr,
+                                               g.iadj, g.iw, N, E, B, rs.soff, rs.vrel, ekey, ew);
+  k_seg_classify<<<grid_for(nseg, T, SEG_GRID), T, 0, st>>>(rs.soff, nseg, one ? 1 : 0, rs.lists,
+                                                            rs.lcap, rs.ncl);
+  if (one) {
+    k_seg_c1<<<SEG_GRID, 512, (size_t)RC * 4, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl,
+                                                    B, rs.su);
+  } else {
+    k_seg_sort_g<<<SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl, rs.su);
+    k_seg_c_count<<<SEG_GRID / 2, 512, (size_t)RC * 4, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
+                                                             rs.ncl, B, nr, rs.cntc, rs.su);
+  }
+  k_seg_sort_w<<<SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl, rs.su);
+  k_seg_sort_reg<<<SEG_GRID * 2, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl,
+                                               rs.su);
+  scan_exclusive(rs.su, rs.suoff, nseg, rs.suoff + nseg, rs.tmp, st);
+  k_seg_compact<<<grid_for(nseg * 16, T, SEG_GRID * 2), T, 0, st>>>(
+      rs.soff, ekey, ew, rs.suoff, B, one ? 1 : 0, bm.rowptr, bm.col, bm.val, bm.erow, bm.colptr,
+      bm.row, bm.cval, D);
+  if (!one)
+    k_seg_c_write<<<SEG_GRID / 2, 512, (size_t)RC * 4, st>>>(
+        rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl, B, nr, rs.cntc, rs.suoff, bm.col, bm.val,
+        bm.erow, bm.row, bm.cval, D);
+  k_logs<<<grid_for(B, 256), 256, 0, st>>>(bm.dout, bm.din, B, bm.ldout, bm.ldin);
 }
 
-static void dense_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm,
-                       RebuildScratch &rs, hipStream_t st) {
+// 128 < B <= TILE_B (dense): vertices grouped by block; workgroup (row tile, chunk) counts
+// the out-edges of its rows' vertices in LDS and writes a partial plane; the planes are
+// summed into D; CSR from D's rows, CSC from the rows of D's transpose.
+constexpr int TILE_B = 1024;
+static void tile_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm,
+                      RebuildScratch &rs, hipStream_t st) {
   const int64_t N = g.N;
   const int64_t BB = (int64_t)B * B;
   const int T = 256;
   int32_t *D = bm.dense;
-  // row tiles of at most 16384 counters (64 KB of LDS), chunks so that ~2048 workgroups run
-  const int32_t R = B >= 16384 ? 1 : 16384 / B;
+  uint32_t *vhist = rs.lists, *cnt = rs.su;
+  int32_t *vperm = (int32_t *)rs.vrel;
+  const int32_t R = 16384 / B;
   const int32_t ntiles = (B + R - 1) / R;
   const int32_t W = (int32_t)((N + VT - 1) / VT);
-  const bool one_tile = ntiles == 1;
-  if (!one_tile) {  // vertices grouped by block, so that a tile reads only its rows' vertices
-    k_label_hist<<<W, 256, (size_t)B * 4, st>>>(lab, N, B, rs.vhist, W);
-    scan_exclusive(rs.vhist, rs.vhist, (int64_t)B * W, nullptr, rs.tmp, st);
-    k_label_scatter<<<W, 256, (size_t)B * 4, st>>>(lab, N, B, rs.vhist, W, rs.vperm);
-  }
-  int32_t C;
-  if (one_tile) {
-    // edge chunks; about 2M partial counters in all (128 planes at B = 128)
-    C = (int32_t)std::min<int64_t>(1024, std::max<int64_t>(128, ((int64_t)2 << 20) / BB));
-    if ((int64_t)C * BB > rs.partials_cap) C = (int32_t)(rs.partials_cap / BB);
-    if ((int64_t)C > g.E) C = (int32_t)std::max<int64_t>(g.E, 1);
-    k_edge_hist<<<C, 512, (size_t)BB * 4, st>>>(g.esrc, g.oadj, g.ow, lab, g.E, B, C,
-                                               C == 1 ? D : rs.partials);
-  } else {
-    C = 2048 / ntiles;
-    if ((int64_t)C * BB > ((int64_t)8 << 20)) C = (int32_t)(((int64_t)8 << 20) / BB);
-    if ((int64_t)C * BB > rs.partials_cap) C = (int32_t)(rs.partials_cap / BB);
-    if (C < 1) C = 1;
-    // a single chunk writes its tiles straight into D
-    k_tile_hist<<<dim3(ntiles, C), 512, (size_t)R * B * 4, st>>>(
-        g.optr, g.oadj, g.ow, lab, rs.vperm, rs.vhist, W, N, B, R, C, C == 1 ? D : rs.partials);
-  }
+  k_label_hist<<<W, 256, (size_t)B * 4, st>>>(lab, N, B, vhist, W);
+  scan_exclusive(vhist, vhist, (int64_t)B * W, nullptr, rs.tmp, st);
+  k_label_scatter<<<W, 256, (size_t)B * 4, st>>>(lab, N, B, vhist, W, vperm);
+  int32_t C = 2048 / ntiles;
+  if ((int64_t)C * BB > ((int64_t)8 << 20)) C = (int32_t)(((int64_t)8 << 20) / BB);
+  if ((int64_t)C * BB > rs.partials_cap) C = (int32_t)(rs.partials_cap / BB);
+  if (C < 1) C = 1;
+  k_tile_hist<<<dim3(ntiles, C), 512, (size_t)R * B * 4, st>>>(
+      g.optr, g.oadj, g.ow, lab, vperm, vhist, W, N, B, R, C, C == 1 ? D : rs.partials);
   if (C == 1) {
   } else if (C <= 4)
     k_hist_reduce<1><<<grid_for(BB, T), T, 0, st>>>(rs.partials, C, BB, D);
@@ -429,41 +969,58 @@ static void dense_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &
     k_hist_reduce<8><<<grid_for(BB * 8, T), T, 0, st>>>(rs.partials, C, BB, D);
   else
     k_hist_reduce<64><<<grid_for(BB * 64, T), T, 0, st>>>(rs.partials, C, BB, D);
-  if (B <= SMALL_B) {
-    k_small_bm<<<1, 1024, 0, st>>>(D, B, bm.rowptr, bm.col, bm.val, bm.erow, bm.colptr, bm.row,
-                                   bm.cval, bm.dout, bm.din, bm.ldout, bm.ldin);
-    return;
-  }
-  k_dense_rows<<<B, 64, 0, st>>>(D, B, rs.cnt, bm.dout);
-  scan_exclusive(rs.cnt, (uint32_t *)bm.rowptr, B, (uint32_t *)(bm.rowptr + B), rs.tmp, st);
+  k_dense_rows<<<B, 64, 0, st>>>(D, B, cnt, bm.dout);
+  scan_exclusive(cnt, (uint32_t *)bm.rowptr, B, (uint32_t *)(bm.rowptr + B), rs.tmp, st);
   k_dense_write_rows<<<B, 64, 0, st>>>(D, B, bm.rowptr, bm.col, bm.val, bm.erow);
-  // columns: the same row kernels on the transpose (partials are free again here)
   int32_t *DT = rs.partials;
   const unsigned tb = (unsigned)((B + 31) / 32);
   k_transpose<<<dim3(tb, tb), 256, 0, st>>>(D, B, DT);
-  k_dense_rows<<<B, 64, 0, st>>>(DT, B, rs.cnt, bm.din);
-  scan_exclusive(rs.cnt, (uint32_t *)bm.colptr, B, (uint32_t *)(bm.colptr + B), rs.tmp, st);
+  k_dense_rows<<<B, 64, 0, st>>>(DT, B, cnt, bm.din);
+  scan_exclusive(cnt, (uint32_t *)bm.colptr, B, (uint32_t *)(bm.colptr + B), rs.tmp, st);
   k_dense_write_rows<<<B, 64, 0, st>>>(DT, B, bm.colptr, bm.row, bm.cval, nullptr);
+  k_logs<<<grid_for(B, 256), 256, 0, st>>>(bm.dout, bm.din, B, bm.ldout, bm.ldin);
+}
+
+static void small_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm,
+                       RebuildScratch &rs, hipStream_t st) {
+  const int64_t BB = (int64_t)B * B;
+  const int T = 256;
+  int32_t *D = bm.dense;
+  // edge chunks; about 2M partial counters in all (128 planes at B = 128)
+  int32_t C = (int32_t)std::min<int64_t>(1024, std::max<int64_t>(128, ((int64_t)2 << 20) / BB));
+  if ((int64_t)C * BB > rs.partials_cap) C = (int32_t)(rs.partials_cap / BB);
+  if ((int64_t)C > g.E) C = (int32_t)std::max<int64_t>(g.E, 1);
+  k_edge_hist<<<C, 512, (size_t)BB * 4, st>>>(g.esrc, g.oadj, g.ow, lab, g.E, B, C,
+                                             C == 1 ? D : rs.partials);
+  if (C == 1) {
+  } else if (C <= 4)
+    k_hist_reduce<1><<<grid_for(BB, T), T, 0, st>>>(rs.partials, C, BB, D);
+  else if (C <= 32)
+    k_hist_reduce<8><<<grid_for(BB * 8, T), T, 0, st>>>(rs.partials, C, BB, D);
+  else
+    k_hist_reduce<64><<<grid_for(BB * 64, T), T, 0, st>>>(rs.partials, C, BB, D);
+  k_small_bm<<<1, 1024, 0, st>>>(D, B, bm.rowptr, bm.col, bm.val, bm.erow, bm.colptr, bm.row,
+                                 bm.cval, bm.dout, bm.din, bm.ldout, bm.ldin);
 }
 
 void rebuild_blockmodel(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm,
                         RebuildScratch &rs, int32_t dense_max_B, hipStream_t st) {
-  bm.B = B;
-  const bool dense = B <= dense_max_B && (int64_t)B * B <= bm.dense_cap &&
-                     (int64_t)B * B <= rs.partials_cap && B <= rs.vhist_B;
-  if (dense) {
-    dense_path(g, lab, B, bm, rs, st);
-    if (B <= SMALL_B) {  // k_small_bm wrote the logs
-      bm.dense_valid = true;
-      return;
-    }
-  } else if ((uint64_t)B * (uint64_t)B <= 0xFFFFFFFFull) {
-    sparse_path<uint32_t>(g, lab, B, bm, rs, st);
-  } else {
-    sparse_path<uint64_t>(g, lab, B, bm, rs, st);
+  const bool dense = B <= dense_max_B && (int64_t)B * B <= bm.dense_cap;
+  const bool small = dense && B <= SMALL_B && (int64_t)B * B * 128 <= rs.partials_cap;
+  const bool tile = !small && dense && B <= TILE_B && (int64_t)B * B <= rs.partials_cap;
+  // restore the all-zero dense buffer, unless the small or tile path overwrites the same
+  // B x B region in full
+  if (bm.dense_B > 0 && !((small || tile) && bm.dense_B == B)) {
+    k_dense_clear<<<grid_for(g.E, 256, SEG_GRID), 256, 0, st>>>(
+        bm.erow, bm.col, bm.rowptr + bm.dense_B, bm.dense_B, bm.dense);
+    bm.dense_B = 0;
   }
+  bm.B = B;
+  if (small) small_path(g, lab, B, bm, rs, st);
+  else if (tile) tile_path(g, lab, B, bm, rs, st);
+  else seg_path(g, lab, B, bm, rs, dense, st);
   bm.dense_valid = dense;
-  k_logs<<<grid_for(B, 256), 256, 0, st>>>(bm.dout, bm.din, B, bm.ldout, bm.ldin);
+  bm.dense_B = dense ? B : 0;
 }
 
 }  // namespace gsap

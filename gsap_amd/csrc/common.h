@@ -344,6 +344,12 @@ GSAP_D int32_t bm_walk(const BMView &m, int32_t x, uint64_t target, int32_t skip
   return fallback;
 }
 
+// Orders the LDS operations of one wave (LDS that is private to the wave).
+GSAP_D void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // Total order on doubles as unsigned keys (a radix sort on float bits orders so).
 GSAP_HD uint64_t order_key(double d) {
   uint64_t u = __builtin_bit_cast(uint64_t, d);

@@ -13,6 +13,7 @@ struct GraphDev {
   int32_t *optr = nullptr, *oadj = nullptr, *ow = nullptr;
   int32_t *iptr = nullptr, *iadj = nullptr, *iw = nullptr;
   int32_t *kout = nullptr, *kin = nullptr, *esrc = nullptr;
+  int32_t *idst = nullptr;  // owner (target vertex) of every in-CSR entry
   // nodal hash tables for vertices whose adjacency does not fit the LDS table
   int64_t *hoff = nullptr;       // per vertex: offset into hpool (words), -1 when light
   uint8_t *hbits_o = nullptr;    // per vertex: log2 capacity of out table (0 = light)
@@ -31,6 +32,8 @@ struct BMBuf {
   int32_t *dense = nullptr;                       // dense_cap words
   int64_t dense_cap = 0;
   bool dense_valid = false;
+  // B of the entries the dense buffer holds (the current CSR's); 0 = all zero
+  int32_t dense_B = 0;
   BMView view() const {
     BMView v;
     v.B = B; v.rowptr = rowptr; v.col = col; v.val = val;
@@ -41,18 +44,24 @@ struct BMBuf {
   }
 };
 
+constexpr int SEG_NCL = 5;  // segment classes of the segmented rebuild
+
 struct RebuildScratch {
-  void *k0 = nullptr, *k1 = nullptr;        // E x 8 bytes
-  uint32_t *v0 = nullptr, *v1 = nullptr;    // E
-  uint32_t *flags = nullptr, *uidx = nullptr;  // E
-  uint32_t *tmp = nullptr;                  // radix / scan temporary
-  uint32_t *cnt = nullptr;                  // Bmax + 1
-  int32_t *partials = nullptr;              // partial planes of the tiled histogram
-  int64_t partials_cap = 0;                 // words
-  uint32_t *vhist = nullptr;                // per-(block, vertex tile) counts, block-major
-  int32_t *vperm = nullptr;                 // vertices grouped by block
-  int32_t vhist_B = 0;                      // largest B the vhist buffer holds
+  void *k0 = nullptr, *k1 = nullptr;  // 2E words each: segment keys and weights
+  uint32_t *tmp = nullptr;            // scan temporary
+  int32_t *partials = nullptr;        // partial planes of the small-B histogram
+  int64_t partials_cap = 0;           // words
+  uint32_t *soff = nullptr;           // 2 Bmax + 1: segment sizes, then offsets
+  uint32_t *su = nullptr;             // 2 Bmax: runs per segment
+  uint32_t *vrel = nullptr;           // 2N: each vertex's slice start inside its row / column segment
+  uint32_t *suoff = nullptr;          // 2 Bmax + 1: offsets of the runs
+  uint32_t *lists = nullptr;          // SEG_NCL lists of lcap segment ids
+  int64_t lcap = 0;
+  uint32_t *ncl = nullptr;            // SEG_NCL list lengths
+  uint32_t *cntc = nullptr;           // runs per (large segment, column range)
 };
+// words of cntc for a graph of N vertices and E edges
+int64_t seg_cntc_words(int64_t N, int64_t E);
 
 // ---- bm.hip
 void rebuild_blockmodel(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm,
@@ -62,8 +71,14 @@ void rebuild_blockmodel(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf 
 struct RngKey {
   uint32_t k0, k1, chain;
 };
+// Philox phase counter of a nodal batch: *dev + add (dev may be null).  Kept in device
+// memory so that a captured sweep replays with the current batch counter.
+struct PhaseRef {
+  const uint32_t *dev;
+  uint32_t add;
+};
 void launch_block_terms(const BMView &m, double *OUT, int32_t *sof, uint8_t *coin, RngKey key,
-                        uint32_t phase, bool sample, int32_t avg_row, hipStream_t st);
+                        PhaseRef phase, bool sample, int32_t avg_row, hipStream_t st);
 // Nodal kernels by the larger of a vertex's out- and in-neighbour counts d: d <= 16 and 32
 // use register deduplication in groups of 16 and 32 lanes; d <= 128 a wave with 256-slot
 // LDS hash tables; larger d a 256-thread workgroup with tables sized for the vertex (LDS
@@ -83,7 +98,7 @@ inline int64_t big_words(int64_t no, int64_t ni) {
 }
 constexpr int64_t BIG_LDS_CAP = 40 * 1024;
 void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const int32_t *sof,
-                  const uint8_t *coin, RngKey key, uint32_t phase, int64_t lo, int64_t BS,
+                  const uint8_t *coin, RngKey key, PhaseRef phase, int64_t lo, int64_t BS,
                   const int32_t *const lists[NCLS], const int32_t counts[NCLS], uint32_t big_lds,
                   int32_t *prop, double *dS, float *err, uint8_t *near, hipStream_t st);
 void launch_mix_stats(const float *err, const uint8_t *near, int64_t n, uint32_t *max_bits,
@@ -101,5 +116,9 @@ void launch_argmin(const double *dS, const int32_t *prop, int32_t B, int32_t np,
                    double *dS_best, hipStream_t st);
 void launch_relabel(int32_t *lab, const int32_t *map, int64_t N, hipStream_t st);
 void launch_iota(int32_t *x, int64_t n, hipStream_t st);
+void launch_add_u32(uint32_t *x, uint32_t v, hipStream_t st);
+// keys[r] = order key of the double x[r], vals[r] = r
+void launch_order_keys(const double *x, int32_t n, uint64_t *keys, uint32_t *vals,
+                       hipStream_t st);
 
 }  // namespace gsap

@@ -144,15 +144,17 @@ struct gsap_ctx {
   std::string err;
   std::vector<gsap_trace_rec> trace;
   // graph
-  DBuf optr, oadj, ow, iptr, iadj, iw, kout, kin, esrc, hoff, hbo, hbi, hpool;
+  DBuf optr, oadj, ow, iptr, iadj, iw, kout, kin, esrc, idst, hoff, hbo, hbi, hpool;
   // labels and golden-section snapshots
   DBuf lab, snap[3];
   // block matrix
   DBuf rowptr, colptr, col, val, erow, row, cval, dout, din, ldout, ldin, dense;
   // rebuild scratch
-  DBuf k0, k1, v0, v1, flags, uidx, tmp, cnt, partials, vhist, vperm;
+  DBuf k0, k1, tmp, partials, soff, su, suoff, lists, ncl, cntc, vrel;
   // sweep / merge
   DBuf OUT, sof, coin, prop, dS, best, dSbest, map, terms, tiles, res;
+  // device batch counter (Philox phase of the sweep's first batch); merge-order sort
+  DBuf ctr, mk0, mk1, mv0, mv1;
   // mixed form: per-vertex |dS32 - dS64| and evaluation class; batch statistics
   DBuf errv, evcls, mixstat;
   // per batch, the batch's vertices split by neighbour-count class (nodal_class)
@@ -195,6 +197,22 @@ void validate(const gsap_graph_i32 *g) {
   }
   if (2 * wsum >= ((int64_t)1 << 31) - 1)
     throw Err(GSAP_ERR_INVALID, "total edge weight must stay below 2^30");
+  // the in-CSR must hold the same weighted edges as the out-CSR (row and column segments
+  // of the block matrix are built from one and the other): order-independent hash sums
+  auto mix = [](uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+  };
+  uint64_t ho = 0, hi = 0;
+  for (int64_t v = 0; v < g->N; ++v) {
+    for (int32_t j = g->out_ptr[v]; j < g->out_ptr[v + 1]; ++j)
+      ho += mix(((uint64_t)v << 32 | (uint32_t)g->out_adj[j]) ^ mix((uint64_t)g->out_w[j]));
+    for (int32_t j = g->in_ptr[v]; j < g->in_ptr[v + 1]; ++j)
+      hi += mix(((uint64_t)(uint32_t)g->in_adj[j] << 32 | (uint64_t)v) ^ mix((uint64_t)g->in_w[j]));
+  }
+  if (ho != hi) throw Err(GSAP_ERR_INVALID, "in-CSR and out-CSR hold different edges");
 }
 
 template <class T>
@@ -208,10 +226,10 @@ void upload_graph(gsap_ctx *c, const gsap_graph_i32 *g) {
   const int64_t N = g->N, E = g->E;
   c->N = N; c->E = E;
   hipStream_t st = c->st;
-  std::vector<int32_t> kout(N, 0), kin(N, 0), esrc(E);
+  std::vector<int32_t> kout(N, 0), kin(N, 0), esrc(E), idst(E);
   for (int64_t v = 0; v < N; ++v) {
     for (int32_t j = g->out_ptr[v]; j < g->out_ptr[v + 1]; ++j) { kout[v] += g->out_w[j]; esrc[j] = (int32_t)v; }
-    for (int32_t j = g->in_ptr[v]; j < g->in_ptr[v + 1]; ++j) kin[v] += g->in_w[j];
+    for (int32_t j = g->in_ptr[v]; j < g->in_ptr[v + 1]; ++j) { kin[v] += g->in_w[j]; idst[j] = (int32_t)v; }
   }
   // vertices of the largest nodal class get tables and slot arrays in the global pool
   // (used when they do not fit the launch's LDS); offsets in int32 words, even
@@ -236,6 +254,7 @@ void upload_graph(gsap_ctx *c, const gsap_graph_i32 *g) {
   h2d(c->kout, kout.data(), N, st);
   h2d(c->kin, kin.data(), N, st);
   h2d(c->esrc, esrc.data(), E, st);
+  h2d(c->idst, idst.data(), E, st);
   h2d(c->hoff, hoff.data(), N, st);
   h2d(c->hbo, hbo.data(), N, st);
   h2d(c->hbi, hbi.data(), N, st);
@@ -246,6 +265,7 @@ void upload_graph(gsap_ctx *c, const gsap_graph_i32 *g) {
   d.optr = c->optr.as<int32_t>(); d.oadj = c->oadj.as<int32_t>(); d.ow = c->ow.as<int32_t>();
   d.iptr = c->iptr.as<int32_t>(); d.iadj = c->iadj.as<int32_t>(); d.iw = c->iw.as<int32_t>();
   d.kout = c->kout.as<int32_t>(); d.kin = c->kin.as<int32_t>(); d.esrc = c->esrc.as<int32_t>();
+  d.idst = c->idst.as<int32_t>();
   d.hoff = c->hoff.as<int64_t>(); d.hbits_o = c->hbo.as<uint8_t>(); d.hbits_i = c->hbi.as<uint8_t>();
   d.hpool = c->hpool.as<int32_t>();
 }
@@ -287,6 +307,9 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   c->dense_max_B = dense_max_B;
   c->lab.ensure(4 * N);
   for (auto &s : c->snap) s.ensure(4 * N);
+  // the dense buffer's zero-except-current-entries state survives only if neither it nor
+  // the arrays that describe its entries move
+  const void *before[4] = {c->dense.p, c->erow.p, c->col.p, c->rowptr.p};
   c->rowptr.ensure(4 * (N + 1)); c->colptr.ensure(4 * (N + 1));
   c->col.ensure(4 * E); c->val.ensure(4 * E); c->erow.ensure(4 * E);
   c->row.ensure(4 * E); c->cval.ensure(4 * E);
@@ -294,22 +317,29 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   c->ldout.ensure(8 * N); c->ldin.ensure(8 * N);
   const int64_t dB = std::min<int64_t>(dense_max_B, N);
   c->dense.ensure(4 * (size_t)std::max<int64_t>(dB * dB, 1));
+  const void *after[4] = {c->dense.p, c->erow.p, c->col.p, c->rowptr.p};
+  if (!std::equal(before, before + 4, after)) {
+    HIPX(hipMemsetAsync(c->dense.p, 0, c->dense.cap, c->st));
+    c->bm.dense_B = 0;
+  }
   c->k0.ensure(8 * E); c->k1.ensure(8 * E);
-  c->v0.ensure(4 * E); c->v1.ensure(4 * E); c->flags.ensure(4 * E); c->uidx.ensure(4 * E);
-  const int64_t W = (N + 2047) / 2048;
-  size_t tw = std::max(radix_temp_words(E), scan_temp_words(std::max<int64_t>(N + 1, dB * W)));
-  c->tmp.ensure(4 * tw);
-  c->cnt.ensure(4 * (N + 1));
-  // partial planes: up to 2048 / ceil(B / (16384 / B)) planes of B*B counters
-  const int64_t pc = std::max<int64_t>(4 * dB * dB, 2048 * 16384);
+  c->tmp.ensure(4 * std::max(scan_temp_words(2 * N + 1), radix_temp_words(N)));
+  // partial planes of the small-B histogram (up to 1024 planes of at most 128 x 128
+  // counters) and of the row-tile histogram (up to 8M counters), then the transpose
+  const int64_t sb = std::min<int64_t>(dB, 128);
+  const int64_t pc = std::max<int64_t>(1024 * sb * sb, 1);
   c->partials.ensure(4 * pc);
-  c->vhist.ensure(4 * (size_t)std::max<int64_t>(dB * W, 1));
-  c->vperm.ensure(4 * N);
+  c->soff.ensure(4 * (2 * N + 1));
+  c->su.ensure(4 * 2 * N); c->suoff.ensure(4 * (2 * N + 1)); c->vrel.ensure(4 * 2 * N);
+  c->lists.ensure(4 * (size_t)SEG_NCL * 2 * N); c->ncl.ensure(4 * 8);
+  c->cntc.ensure(4 * (size_t)std::max<int64_t>(seg_cntc_words(N, E), 1));
   c->OUT.ensure(8 * N); c->sof.ensure(4 * N); c->coin.ensure(N);
   c->prop.ensure(4 * (size_t)np * N); c->dS.ensure(8 * (size_t)np * N);
   c->best.ensure(4 * N); c->dSbest.ensure(8 * N); c->map.ensure(4 * N);
   c->terms.ensure(8 * E); c->tiles.ensure(8 * (std::max(E, N) / 1024 + 2));
   c->res.ensure(8 * 64);
+  c->ctr.ensure(8);
+  c->mk0.ensure(8 * N); c->mk1.ensure(8 * N); c->mv0.ensure(4 * N); c->mv1.ensure(4 * N);
   c->errv.ensure(4 * N); c->evcls.ensure(N); c->mixstat.ensure(32);
   BMBuf &b = c->bm;
   b.rowptr = c->rowptr.as<int32_t>(); b.colptr = c->colptr.as<int32_t>();
@@ -319,10 +349,12 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   b.ldout = c->ldout.as<double>(); b.ldin = c->ldin.as<double>();
   b.dense = c->dense.as<int32_t>(); b.dense_cap = (int64_t)(c->dense.cap / 4);
   RebuildScratch &r = c->rs;
-  r.k0 = c->k0.p; r.k1 = c->k1.p; r.v0 = c->v0.as<uint32_t>(); r.v1 = c->v1.as<uint32_t>();
-  r.flags = c->flags.as<uint32_t>(); r.uidx = c->uidx.as<uint32_t>(); r.tmp = c->tmp.as<uint32_t>();
-  r.cnt = c->cnt.as<uint32_t>(); r.partials = c->partials.as<int32_t>(); r.partials_cap = pc;
-  r.vhist = c->vhist.as<uint32_t>(); r.vperm = c->vperm.as<int32_t>(); r.vhist_B = (int32_t)dB;
+  r.k0 = c->k0.p; r.k1 = c->k1.p; r.tmp = c->tmp.as<uint32_t>();
+  r.partials = c->partials.as<int32_t>(); r.partials_cap = pc;
+  r.soff = c->soff.as<uint32_t>();
+  r.su = c->su.as<uint32_t>(); r.suoff = c->suoff.as<uint32_t>(); r.vrel = c->vrel.as<uint32_t>();
+  r.lists = c->lists.as<uint32_t>(); r.lcap = 2 * N; r.ncl = c->ncl.as<uint32_t>();
+  r.cntc = c->cntc.as<uint32_t>();
 }
 
 int32_t resolve_dense(const gsap_options &o) { return o.dense_max_B > 0 ? o.dense_max_B : 8192; }
@@ -397,7 +429,8 @@ int32_t merge_phase(gsap_ctx *c, const gsap_options &o, int32_t B_to_merge, uint
   BMView m = c->bm.view();
   int32_t ar = avg_row(read_nnz(c), B);
   int p = c->prof.begin(2, st);
-  launch_block_terms(m, c->OUT.as<double>(), nullptr, nullptr, rng_key(o), phase, false, ar, st);
+  launch_block_terms(m, c->OUT.as<double>(), nullptr, nullptr, rng_key(o), PhaseRef{nullptr, phase},
+                     false, ar, st);
   c->prof.end(p, st);
   p = c->prof.begin(3, st);
   launch_merge(m, c->OUT.as<double>(), rng_key(o), phase, np, c->prop.as<int32_t>(),
@@ -406,10 +439,25 @@ int32_t merge_phase(gsap_ctx *c, const gsap_options &o, int32_t B_to_merge, uint
   launch_argmin(c->dS.as<double>(), c->prop.as<int32_t>(), B, np, c->best.as<int32_t>(),
                 c->dSbest.as<double>(), st);
   HIPX(hipGetLastError());
-  std::vector<int32_t> best(B);
-  std::vector<double> dsb(B);
+  // stable ascending order of the best delta-S (thrust::sort_by_key, partition.hpp:442-445):
+  // a stable LSD radix sort of the order-preserving 64-bit keys on the device for large B,
+  // std::stable_sort of the same keys on the host otherwise (identical order)
+  const bool dev_sort = B >= 4096;
+  std::vector<int32_t> best(B), order(B);
+  std::vector<double> dsb;
   HIPX(hipMemcpyAsync(best.data(), c->best.p, 4 * (size_t)B, hipMemcpyDeviceToHost, st));
-  HIPX(hipMemcpyAsync(dsb.data(), c->dSbest.p, 8 * (size_t)B, hipMemcpyDeviceToHost, st));
+  if (dev_sort) {
+    uint64_t *k0 = c->mk0.as<uint64_t>(), *k1 = c->mk1.as<uint64_t>();
+    uint32_t *v0 = c->mv0.as<uint32_t>(), *v1 = c->mv1.as<uint32_t>();
+    launch_order_keys(c->dSbest.as<double>(), B, k0, v0, st);
+    radix_sort_pairs<uint64_t>(k0, v0, k1, v1, B, 64, c->tmp.as<uint32_t>(), st);
+    HIPX(hipGetLastError());
+    HIPX(hipMemcpyAsync(order.data(), v0, 4 * (size_t)B, hipMemcpyDeviceToHost, st));
+  }
+  if (!dev_sort || dSb_out) {
+    dsb.resize(B);
+    HIPX(hipMemcpyAsync(dsb.data(), c->dSbest.p, 8 * (size_t)B, hipMemcpyDeviceToHost, st));
+  }
   if (prop_out) {
     prop_out->resize((size_t)B * np);
     dS_out->resize((size_t)B * np);
@@ -420,13 +468,13 @@ int32_t merge_phase(gsap_ctx *c, const gsap_options &o, int32_t B_to_merge, uint
   c->prof.flush();
   if (best_out) *best_out = best;
   if (dSb_out) *dSb_out = dsb;
-  // stable ascending order of the best delta-S (thrust::sort_by_key, partition.hpp:442-445)
-  std::vector<uint64_t> key(B);
-  for (int32_t r = 0; r < B; ++r) key[r] = order_key(dsb[r]);
-  std::vector<int32_t> order(B);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+  if (!dev_sort) {
+    std::vector<uint64_t> key(B);
+    for (int32_t r = 0; r < B; ++r) key[r] = order_key(dsb[r]);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+  }
   std::vector<int32_t> par(B);
   std::iota(par.begin(), par.end(), 0);
   int32_t merged = 0, cidx = 0;
@@ -448,7 +496,7 @@ int32_t merge_phase(gsap_ctx *c, const gsap_options &o, int32_t B_to_merge, uint
 }
 
 // One nodal batch on the current block matrix; the accepted delta-S sum lands in *dE_dev.
-void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, uint32_t phase, int32_t ar,
+void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, PhaseRef phase, int32_t ar,
                  double *dE_dev) {
   const int64_t BS = c->N / o.num_batches, lo = (int64_t)b * BS;
   hipStream_t st = c->st;
@@ -497,6 +545,39 @@ void check_options(const gsap_options &o) {
     throw Err(GSAP_ERR_INVALID, "unknown model");
 }
 
+// A level's captured sweep: the executable graph, destroyed when the level ends or an error
+// unwinds.
+struct GraphGuard {
+  gsap_ctx *c;
+  hipGraphExec_t exec = nullptr;
+  explicit GraphGuard(gsap_ctx *ctx) : c(ctx) {}
+  template <class F>
+  void capture(F &&body) {
+    HIPX(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    hipGraph_t g = nullptr;
+    try {
+      body();
+    } catch (...) {
+      (void)hipStreamEndCapture(c->st, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPX(hipStreamEndCapture(c->st, &g));
+    const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      exec = nullptr;
+      throw Err(GSAP_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+  }
+  ~GraphGuard() {
+    if (exec) {
+      (void)hipStreamSynchronize(c->st);
+      (void)hipGraphExecDestroy(exec);
+    }
+  }
+};
+
 struct Slot {
   int32_t B = 0, B_to_merge = 0;
   double S = std::numeric_limits<double>::infinity();
@@ -521,6 +602,8 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
   double *res = c->res.as<double>();
   launch_iota(lab, N, st);
   HIPX(hipMemsetAsync(c->mixstat.p, 0, 32, st));
+  uint32_t *ctr = c->ctr.as<uint32_t>();
+  HIPX(hipMemsetAsync(ctr, 0, 4, st));
   int32_t B = (int32_t)N;
   int32_t B_to_merge = (int32_t)((float)B * o.reduction_rate);
   rebuild(c, B);
@@ -556,23 +639,50 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
     int32_t ar = avg_row(read_nnz(c), B);
     t_merge += now() - tm;
     std::vector<double> dE(o.max_nodal_itr, 0.0);
+    // one sweep: num_batches x (nodal batch + rebuild), the description length, and the
+    // device batch counter advanced; with options.graph (and no tracing or event timing:
+    // HIP cannot time events recorded inside a graph) it is captured once per level (B and
+    // every launch shape are fixed within a level) and replayed as a HIP graph
+    const bool use_graph = o.graph && !tr && !c->prof.on;
+    GraphGuard gg(c);
+    const int32_t ar_level = ar;
     for (int itr = 0; itr < o.max_nodal_itr; ++itr) {
       const double ts = now();
-      for (int b = 0; b < o.num_batches; ++b) {
-        nodal_batch(c, o, b, batch_ctr, ar, res + 1 + b);
+      if (use_graph) {
+        if (!gg.exec) gg.capture([&] {
+          for (int b = 0; b < o.num_batches; ++b) {
+            nodal_batch(c, o, b, PhaseRef{ctr, (uint32_t)b}, ar_level, res + 1 + b);
+            rebuild(c, B);
+          }
+          launch_mdl(c, res);
+          launch_add_u32(ctr, (uint32_t)o.num_batches, st);
+        });
+        HIPX(hipGraphLaunch(gg.exec, st));
+        batch_ctr += (uint32_t)o.num_batches;
+        batches += o.num_batches;
+      }
+      for (int b = 0; !use_graph && b < o.num_batches; ++b) {
+        nodal_batch(c, o, b, PhaseRef{ctr, (uint32_t)b}, ar, res + 1 + b);
         rebuild(c, B);
         if (tr) {
           double e = 0;
           hprop.resize(BS);
+          hdS.resize(BS);
           HIPX(hipMemcpyAsync(hprop.data(), c->prop.p, 4 * (size_t)BS, hipMemcpyDeviceToHost, st));
+          HIPX(hipMemcpyAsync(hdS.data(), c->dS.p, 8 * (size_t)BS, hipMemcpyDeviceToHost, st));
           HIPX(hipMemcpyAsync(&e, res + 1 + b, 8, hipMemcpyDeviceToHost, st));
           HIPX(hipStreamSynchronize(st));
-          rec(2, B, itr, b, fnv(hprop.data(), 4 * BS), labels_hash(), e, 0.0);
+          int64_t moved = 0;  // accepted moves of the batch
+          for (int64_t i = 0; i < BS; ++i) moved += hdS[i] < 0.0;
+          rec(2, B, itr, b, fnv(hprop.data(), 4 * BS), labels_hash(), e, (double)moved);
         }
         ++batch_ctr;
         ++batches;
       }
-      launch_mdl(c, res);
+      if (!use_graph) {
+        launch_mdl(c, res);
+        launch_add_u32(ctr, (uint32_t)o.num_batches, st);
+      }
       double h[64];
       int32_t nnz = 0;
       HIPX(hipMemcpyAsync(h, res, 8 * (size_t)(1 + o.num_batches), hipMemcpyDeviceToHost, st));
@@ -779,13 +889,14 @@ int gsap_destroy(gsap_ctx *c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->st);
   DBuf *all[] = {&c->optr, &c->oadj, &c->ow, &c->iptr, &c->iadj, &c->iw, &c->kout, &c->kin,
-                 &c->esrc, &c->hoff, &c->hbo, &c->hbi, &c->hpool, &c->lab, &c->snap[0],
+                 &c->esrc, &c->idst, &c->hoff, &c->hbo, &c->hbi, &c->hpool, &c->lab, &c->snap[0],
                  &c->snap[1], &c->snap[2], &c->rowptr, &c->colptr, &c->col, &c->val, &c->erow,
                  &c->row, &c->cval, &c->dout, &c->din, &c->ldout, &c->ldin, &c->dense, &c->k0,
-                 &c->k1, &c->v0, &c->v1, &c->flags, &c->uidx, &c->tmp, &c->cnt, &c->partials,
-                 &c->vhist, &c->vperm, &c->vlist,
+                 &c->k1, &c->tmp, &c->partials, &c->soff, &c->su, &c->suoff,
+                 &c->lists, &c->ncl, &c->cntc, &c->vrel, &c->vlist,
                  &c->OUT, &c->sof, &c->coin, &c->prop, &c->dS, &c->best, &c->dSbest, &c->map,
-                 &c->terms, &c->tiles, &c->res, &c->errv, &c->evcls, &c->mixstat};
+                 &c->terms, &c->tiles, &c->res, &c->ctr, &c->mk0, &c->mk1, &c->mv0,
+                 &c->mv1, &c->errv, &c->evcls, &c->mixstat};
   for (DBuf *b : all) b->release();
   try { c->prof.destroy(); } catch (...) {}
   (void)hipStreamDestroy(c->st);
@@ -907,7 +1018,8 @@ int gsap_nodal_batch_i32(gsap_ctx *c, const gsap_graph_i32 *g, const gsap_option
     const int64_t BS = c->N / oo.num_batches;
     int32_t ar = avg_row(read_nnz(c), B);
     double *res = c->res.as<double>();
-    nodal_batch(c, oo, b, q, ar, res);
+    HIPX(hipMemcpyAsync(c->ctr.p, &q, 4, hipMemcpyHostToDevice, c->st));
+    nodal_batch(c, oo, b, PhaseRef{c->ctr.as<uint32_t>(), 0}, ar, res);
     hipStream_t st = c->st;
     HIPX(hipMemcpyAsync(dE, res, 8, hipMemcpyDeviceToHost, st));
     if (prop && BS > 0)

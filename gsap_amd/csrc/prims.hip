@@ -125,7 +125,7 @@ void scan_exclusive(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *tota
     if (total) hipMemsetAsync(total, 0, 4, st);
     return;
   }
-  if (nt <= 16) {  // up to 32768 elements: one launch
+  if (nt <= 2) {  // up to 4096 elements: one launch
     k_scan_single<<<1, SCAN_T, 0, st>>>(in, out, n, total);
     return;
   }

@@ -108,7 +108,7 @@ static unsigned grid_for(int64_t n, int threads) {
 // 1024-1029; one draw per block per batch, shared by its vertices).
 template <int G>
 __global__ __launch_bounds__(256) void k_block_terms(BMView m, double *OUT, int32_t *sof,
-                                                     uint8_t *coin, RngKey key, uint32_t phase,
+                                                     uint8_t *coin, RngKey key, PhaseRef pr,
                                                      int sample) {
   const int32_t x = (int32_t)(blockIdx.x * (256 / G) + threadIdx.x / G);
   if (x >= m.B) return;
@@ -125,6 +125,7 @@ __global__ __launch_bounds__(256) void k_block_terms(BMView m, double *OUT, int3
     if (q == 0) OUT[x] = out;
   }
   if (sample) {
+    const uint32_t phase = (pr.dev ? *pr.dev : 0u) + pr.add;
     const Philox y = philox((uint32_t)x, phase, key.chain, TAG_BLOCK, key.k0, key.k1);
     const int64_t deg = (int64_t)m.dout[x] + m.din[x];
     const int32_t so = deg == 0 ? 0 : group_walk<G>(m, x, mulhi(y.x0, deg), -1, 0);
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(256) void k_block_terms(BMView m, double *OUT, int3
 }
 
 void launch_block_terms(const BMView &m, double *OUT, int32_t *sof, uint8_t *coin, RngKey key,
-                        uint32_t phase, bool sample, int32_t avg_row, hipStream_t st) {
+                        PhaseRef phase, bool sample, int32_t avg_row, hipStream_t st) {
   const int s = sample ? 1 : 0;
   if (avg_row <= 12)
     k_block_terms<8><<<grid_for(m.B, 32), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s);
@@ -158,7 +159,7 @@ struct NodalArgs {
   const int32_t *sof;
   const uint8_t *coin;
   RngKey key;
-  uint32_t phase;
+  PhaseRef phase;
   int32_t *prop;
   double *dS;
   const int64_t *hoff;
@@ -167,6 +168,10 @@ struct NodalArgs {
   float *err;      // mixed form: |dS32 - dS64| where FP64 was evaluated, else 0
   uint8_t *near;   // mixed form: 0 FP32 rejection, 1 FP64 evaluated, 2 |dS32| within bound
 };
+
+GSAP_D uint32_t nodal_phase(const NodalArgs &a) {
+  return (a.phase.dev ? *a.phase.dev : 0u) + a.phase.add;
+}
 
 constexpr float U32 = 5.9604645e-08f;  // 2^-24
 
@@ -200,11 +205,6 @@ GSAP_D bool mixed_rejects(double d32, float eb, double Y_r, double Y_s, double Y
 // out-blocks, r and s for rows r and s; v's in-blocks outside {r, s} for columns r and s);
 // the untouched entries contribute (log2 d_old - log2 d_new) * (their summed count).
 // Labels are not written here (batch-Jacobi).  Hash tables of 2^lbits slots live in LDS.
-// Orders the LDS operations of one wave (its tables are private to it).
-GSAP_D void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
 
 template <bool MIX>
 GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbits) {
@@ -216,7 +216,7 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
   const BMView &m = a.m;
   const int32_t B = m.B;
   const int32_t r = a.lab[v];
-  const Philox x = philox((uint32_t)v, a.phase, a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
+  const Philox x = philox((uint32_t)v, nodal_phase(a), a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
   const int32_t o0 = a.optr[v], no = a.optr[v + 1] - o0;
   const int32_t i0 = a.iptr[v], ni = a.iptr[v + 1] - i0;
   const int64_t kout = a.kout[v], kin = a.kin[v];
@@ -454,7 +454,7 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
   const BMView &m = a.m;
   const int32_t B = m.B;
   const int32_t r = a.lab[v];
-  const Philox x = philox((uint32_t)v, a.phase, a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
+  const Philox x = philox((uint32_t)v, nodal_phase(a), a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
   const int32_t o0 = a.optr[v], no = a.optr[v + 1] - o0;
   const int32_t i0 = a.iptr[v], ni = a.iptr[v + 1] - i0;
   const int64_t kout = a.kout[v], kin = a.kin[v];
@@ -626,7 +626,7 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
   const BMView &m = a.m;
   const int32_t B = m.B;
   const int32_t r = a.lab[v];
-  const Philox x = philox((uint32_t)v, a.phase, a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
+  const Philox x = philox((uint32_t)v, nodal_phase(a), a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
   const int32_t o0 = a.optr[v], no = a.optr[v + 1] - o0;
   const int32_t i0 = a.iptr[v], ni = a.iptr[v + 1] - i0;
   const int64_t kout = a.kout[v], kin = a.kin[v];
@@ -896,7 +896,7 @@ __global__ __launch_bounds__(256) void k_nodal_fused(NodalArgs a, NodalLists L) 
 }
 
 void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const int32_t *sof,
-                  const uint8_t *coin, RngKey key, uint32_t phase, int64_t lo, int64_t BS,
+                  const uint8_t *coin, RngKey key, PhaseRef phase, int64_t lo, int64_t BS,
                   const int32_t *const lists[NCLS], const int32_t counts[NCLS], uint32_t big_lds,
                   int32_t *prop, double *dS, float *err, uint8_t *near, hipStream_t st) {
   if (BS <= 0) return;
@@ -1195,6 +1195,20 @@ void launch_relabel(int32_t *lab, const int32_t *map, int64_t N, hipStream_t st)
 __global__ void k_iota(int32_t *x, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = (int32_t)i;
+}
+
+__global__ void k_add_u32(uint32_t *x, uint32_t v) { *x += v; }
+void launch_add_u32(uint32_t *x, uint32_t v, hipStream_t st) { k_add_u32<<<1, 1, 0, st>>>(x, v); }
+
+__global__ void k_order_keys(const double *x, int32_t n, uint64_t *keys, uint32_t *vals) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  keys[r] = order_key(x[r]);
+  vals[r] = (uint32_t)r;
+}
+void launch_order_keys(const double *x, int32_t n, uint64_t *keys, uint32_t *vals,
+                       hipStream_t st) {
+  k_order_keys<<<grid_for(n, 256), 256, 0, st>>>(x, n, keys, vals);
 }
 
 void launch_iota(int32_t *x, int64_t n, hipStream_t st) {

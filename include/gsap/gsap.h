@@ -82,7 +82,10 @@ typedef struct gsap_options {
   int32_t model;              /* GSAP_MODEL_*: description-length model term */
   int32_t workers;            /* batch / chain calls: concurrent sub-contexts (0 = auto, 8) */
   int32_t profile_classes;    /* with profile = 1: bit k times kernel class k (0 = all) */
-  int32_t reserved[4];
+  int32_t graph;              /* 1: capture a level's sweep once as a HIP graph and replay it
+                                 (same results; ignored with trace or profile); 0: launch
+                                 the sweep kernel by kernel */
+  int32_t reserved[3];
 } gsap_options;
 
 /* Per-call report.  Times are seconds of host wall clock around device work. */
@@ -109,9 +112,14 @@ typedef struct gsap_report {
                              every accepted move, whose exact delta-S enters the batch sum) */
 } gsap_report;
 
-/* One trace record; identical layout to the oracle's. */
+/* One trace record; identical layout to the oracle's.
+ *   kind 1 merge:  B, a = B after merging, b = merge counter, h1 = proposals, h2 = labels
+ *   kind 2 batch:  B, a = sweep, b = batch, h1 = proposals, h2 = labels, x = accepted delta-S
+ *                  sum, y = accepted moves
+ *   kind 3 sweep:  B, a = sweep, h1 = CSR columns, h2 = CSR values, x = S, y = sweep delta-S
+ *   kind 4 golden: B, a = next B_to_merge, b = 1 when optimal, h1 = labels, x = S */
 typedef struct gsap_trace_rec {
-  int32_t kind; /* 1 merge, 2 batch, 3 sweep, 4 golden */
+  int32_t kind;
   int32_t B;
   int32_t a, b;
   uint64_t h1, h2;

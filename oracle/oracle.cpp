@@ -781,7 +781,9 @@ int orc_partition(const orc_graph *gg, const orc_options *o, int32_t *labels_out
         double e = 0.0;
         nodal_batch(g, *o, rng, lab.data(), m, b, batch_ctr, bp.data(), bd.data(), &e, sc);
         build_bm(N, g.optr, g.oadj, g.ow, lab.data(), B, m);
-        rec(2, B, itr, b, fnv(bp.data(), 4 * BS), fnv(lab.data(), 4 * N), e, 0.0);
+        int64_t moved = 0;  // accepted moves of the batch
+        for (int64_t i = 0; i < BS; ++i) moved += bd[i] < 0.0;
+        rec(2, B, itr, b, fnv(bp.data(), 4 * BS), fnv(lab.data(), 4 * N), e, (double)moved);
         ++batch_ctr; ++batches;
         dE[itr] = dE[itr] + e;
       }

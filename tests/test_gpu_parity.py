@@ -183,6 +183,63 @@ def test_c3_blockmodel_and_batch(ctx, c3, B):
     assert eq_bits(gb["dE"], rb["dE"])
 
 
+@pytest.mark.parametrize("B,big,dense", [(20000, 0.3, 0), (20000, 0.3, 30000), (3000, 0.05, 0),
+                                         (300, 0.0, 0), (300, 0.0, 1), (129, 0.5, 0)])
+def test_c3_blockmodel_skewed(ctx, c3, B, big, dense):
+    """Labellings whose segments fall in every size class of the segmented rebuild: one
+    block holding a fraction `big` of the vertices (segments of ~1M entries, counted over
+    several column ranges when B > 16000), the rest uniform."""
+    g = c3
+    rng = np.random.default_rng(11)
+    lab = rng.integers(1, B, g.N).astype(np.int32)
+    lab[rng.random(g.N) < big] = 0
+    print(f"C3 skewed blockmodel B={B} big={big} dense={dense}", flush=True)
+    got = ctx.blockmodel(g, lab, B, Options(dense_max_B=dense))
+    ref = O.blockmodel(g, lab, B)
+    for k in ("nnz", "rowptr", "col", "val", "colptr", "row", "cval", "dout", "din"):
+        assert np.array_equal(got[k], ref[k]), k
+
+
+@pytest.mark.parametrize("name", ["hubs", "LL1k", "HH5k"])
+def test_partition_graph_replay(ctx, name):
+    """options.graph: every level's sweep captured once and replayed (at least three times
+    per level, with the level's eager merge, rebuild and golden-section work in between);
+    labels and S must equal the oracle's, and a second call must reproduce the first."""
+    g = graph(name)
+    o = Options(seed=4, graph=1)
+    print(f"partition graph replay {name}", flush=True)
+    lab, rep = ctx.partition(g, o)
+    rlab, rrep, _ = O.partition(g, Options(seed=4))
+    assert np.array_equal(lab, rlab) and eq_bits(rep.S, rrep.S)
+    assert rep.sweeps == rrep.sweeps and rep.B == rrep.B
+    lab2, rep2 = ctx.partition(g, o)
+    assert np.array_equal(lab2, lab) and eq_bits(rep2.S, rep.S)
+
+
+def test_rejects_malformed_graphs(ctx):
+    """Host-side validation before any device work: an in-CSR that does not hold the
+    out-CSR's edges (a moved or reweighted in-edge), and ids out of range."""
+    g = graph("rand")
+    bad = gsap_amd.Graph(g.N, g.out_ptr, g.out_adj, g.out_w, g.in_ptr, g.in_adj.copy(),
+                         g.in_w)
+    bad.in_adj[0] = (bad.in_adj[0] + 1) % g.N
+    with pytest.raises(gsap_amd.GsapError, match="different edges"):
+        ctx.partition(bad, Options(seed=1))
+    bad = gsap_amd.Graph(g.N, g.out_ptr, g.out_adj, g.out_w, g.in_ptr, g.in_adj,
+                         g.in_w.copy())
+    bad.in_w[3] += 1
+    with pytest.raises(gsap_amd.GsapError, match="different edges"):
+        ctx.partition(bad, Options(seed=1))
+    bad = gsap_amd.Graph(g.N, g.out_ptr, g.out_adj.copy(), g.out_w, g.in_ptr, g.in_adj, g.in_w)
+    bad.out_adj[5] = g.N
+    with pytest.raises(gsap_amd.GsapError, match="out of range"):
+        ctx.partition(bad, Options(seed=1))
+    # the context stays usable
+    lab, rep = ctx.partition(g, Options(seed=1))
+    rlab, rrep, _ = O.partition(g, Options(seed=1))
+    assert np.array_equal(lab, rlab)
+
+
 def test_partition_batch(ctx):
     graphs = [graph("tiny_w"), graph("hubs"), graph("LL1k"),
               gsap_amd.generate_dcsbm(1000, 11, 0.0, 4.0, 20.0, seed=1), graph("rand")]
