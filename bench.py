@@ -360,6 +360,7 @@ def main():
         "pairwise_precision": p, "pairwise_recall": r,
         "t_merge_s": float(np.mean([x.t_merge for x in reps])),
         "t_sweep_s": float(np.mean([x.t_sweep for x in reps])),
+        "t_upload_s": float(np.mean([x.t_upload for x in reps])),
         "graph_generation_s": t_gen,
         "kernel_ms_per_step": {KCLASS[i]: float(kms[i]) for i in range(len(KCLASS))},
         "kernel_calls_per_step": {KCLASS[i]: int(kcalls[i]) for i in range(len(KCLASS))},

@@ -272,22 +272,26 @@ __global__ __launch_bounds__(256) void k_transpose(const int32_t *D, int32_t B, 
 }
 
 // -------------------------------------------------------------- segmented path
-// Segment classes by entry count n: registers in groups of 16 lanes (n <= 16) or one wave
-// (n <= 64), one wave's LDS (n <= 512).  Larger segments: when B fits one column range
+// Segment classes by entry count n: registers in groups of 16 or 32 lanes or one wave (one
+// entry per lane, n <= 64), one wave with 2, 4 or 8 entries per lane (n <= 512).  Larger segments: when B fits one column range
 // (B <= SEG_RC) a workgroup counts the segment's weights per key in LDS and writes the
 // nonzero keys in order over the segment's start (one pass); otherwise one workgroup's LDS
 // sort takes n <= 4096 and larger segments are counted per (segment, column range) in two
 // passes (counts, then writes at the final position).
-enum { CL_A = 0, CL_A2 = 1, CL_W = 2, CL_G = 3, CL_C = 4 };
+enum { CL_A16 = 0, CL_A32 = 1, CL_A64 = 2, CL_W128 = 3, CL_W256 = 4, CL_W512 = 5, CL_G = 6,
+       CL_C = 7 };
 // SEG_RC counters of a column range plus the workgroup's reduction words stay under 64 KB
 constexpr int SEG_W = 512, SEG_G = 4096, SEG_RC = 16000;
 constexpr unsigned SEG_GRID = 1024;  // workgroups of the persistent segment kernels
 constexpr int SEG_PRIV_B = 4096;     // up to this B, reservations are counted in LDS first
 constexpr int SEG_VT = 2048;         // vertices per workgroup of the LDS-counted reservation
 GSAP_HD int seg_class(uint32_t n, bool one_range) {
-  if (n <= 16) return CL_A;
-  if (n <= 64) return CL_A2;
-  if (n <= (uint32_t)SEG_W) return CL_W;
+  if (n <= 16) return CL_A16;
+  if (n <= 32) return CL_A32;
+  if (n <= 64) return CL_A64;
+  if (n <= 128) return CL_W128;
+  if (n <= 256) return CL_W256;
+  if (n <= (uint32_t)SEG_W) return CL_W512;
   if (one_range) return CL_C;
   return n <= (uint32_t)SEG_G ? CL_G : CL_C;
 }
@@ -370,7 +374,7 @@ __global__ __launch_bounds__(256) void k_seg_reserve(const int32_t *lab, const i
 
 // Edge-parallel: out-edge e goes to its source's slice of the source block's row segment,
 // keyed by the target's block; in-edge f to its target's slice of the target block's column
-// segment, keyed by the source's block.
+// segment, keyed by the source's block; seg[pos] records the segment of every entry.
 __global__ __launch_bounds__(256) void k_seg_fill(const int32_t *lab, const int32_t *esrc,
                                                   const int32_t *optr, const int32_t *oadj,
                                                   const int32_t *ow, const int32_t *idst,
@@ -378,20 +382,24 @@ __global__ __launch_bounds__(256) void k_seg_fill(const int32_t *lab, const int3
                                                   const int32_t *iw, int64_t N, int64_t E,
                                                   int32_t B, const uint32_t *off,
                                                   const uint32_t *vrel, uint32_t *ekey,
-                                                  uint32_t *ew) {
+                                                  uint32_t *ew, uint32_t *seg) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < 2 * E;
        e += (int64_t)gridDim.x * blockDim.x) {
     if (e < E) {
       const int32_t v = esrc[e];
-      const uint32_t pos = off[lab[v]] + vrel[v] + (uint32_t)(e - optr[v]);
+      const uint32_t x = (uint32_t)lab[v];
+      const uint32_t pos = off[x] + vrel[v] + (uint32_t)(e - optr[v]);
       ekey[pos] = (uint32_t)lab[oadj[e]];
       ew[pos] = (uint32_t)ow[e];
+      seg[pos] = x;
     } else {
       const int64_t f = e - E;
       const int32_t v = idst[f];
-      const uint32_t pos = off[B + lab[v]] + vrel[N + v] + (uint32_t)(f - iptr[v]);
+      const uint32_t x = (uint32_t)(B + lab[v]);
+      const uint32_t pos = off[x] + vrel[N + v] + (uint32_t)(f - iptr[v]);
       ekey[pos] = (uint32_t)lab[iadj[f]];
       ew[pos] = (uint32_t)iw[f];
+      seg[pos] = x;
     }
   }
 }
@@ -479,21 +487,30 @@ GSAP_D void seg_sort_group(uint32_t *ekey, uint32_t *ew, uint32_t s0, int n, uin
   if (q == 0) *uout = (uint32_t)__popcll(hm);
 }
 
+// Wave tasks: one segment of <= 64 entries, two of <= 32 or four of <= 16.
 __global__ __launch_bounds__(256) void k_seg_sort_reg(const uint32_t *off, uint32_t *ekey,
                                                       uint32_t *ew, const uint32_t *lists,
                                                       int64_t lcap, const uint32_t *ncl,
                                                       uint32_t *u) {
-  const int64_t nA2 = ncl[CL_A2], nA = ncl[CL_A];
-  const int64_t tasks = nA2 + (nA + 3) / 4;
+  const int64_t n64 = ncl[CL_A64], n32 = ncl[CL_A32], n16 = ncl[CL_A16];
+  const int64_t t32 = (n32 + 1) / 2, t16 = (n16 + 3) / 4;
+  const int64_t tasks = n64 + t32 + t16;
   const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  const int lane = threadIdx.x & 63;
   for (int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < tasks; t += nw) {
-    if (t < nA2) {
-      const uint32_t x = lists[CL_A2 * lcap + t];
+    if (t < n64) {
+      const uint32_t x = lists[CL_A64 * lcap + t];
       seg_sort_group<64>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x]);
+    } else if (t < n64 + t32) {
+      const int64_t k = (t - n64) * 2 + (lane >> 5);
+      if (k < n32) {
+        const uint32_t x = lists[CL_A32 * lcap + k];
+        seg_sort_group<32>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x]);
+      }
     } else {
-      const int64_t k = (t - nA2) * 4 + ((threadIdx.x & 63) >> 4);
-      if (k < nA) {
-        const uint32_t x = lists[CL_A * lcap + k];
+      const int64_t k = (t - n64 - t32) * 4 + (lane >> 4);
+      if (k < n16) {
+        const uint32_t x = lists[CL_A16 * lcap + k];
         seg_sort_group<16>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x]);
       }
     }
@@ -658,18 +675,30 @@ GSAP_D void team_sort_regs(uint32_t *ekey, uint32_t *ew, uint32_t s0, int n, uin
   else __syncthreads();
 }
 
-// one wave per segment of class W (n <= 512): 8 entries per lane, 2 KB of LDS per wave
+// one wave per segment of 65..512 entries: 2, 4 or 8 entries per lane (networks of 128,
+// 256 or 512), 2 KB of LDS per wave
 __global__ __launch_bounds__(256) void k_seg_sort_w(const uint32_t *off, uint32_t *ekey,
                                                     uint32_t *ew, const uint32_t *lists,
                                                     int64_t lcap, const uint32_t *ncl,
                                                     uint32_t *u) {
   __shared__ uint32_t lds[4][SEG_W];
   const int w = threadIdx.x >> 6;
-  const int64_t n = ncl[CL_W];
-  for (int64_t t = (int64_t)blockIdx.x * 4 + w; t < n; t += (int64_t)gridDim.x * 4) {
-    const uint32_t x = lists[CL_W * lcap + t];
-    team_sort_regs<64, SEG_W / 64>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x],
-                                   nullptr, lds[w], nullptr);
+  const int64_t n512 = ncl[CL_W512], n256 = ncl[CL_W256], n128 = ncl[CL_W128];
+  const int64_t tasks = n512 + n256 + n128;
+  for (int64_t t = (int64_t)blockIdx.x * 4 + w; t < tasks; t += (int64_t)gridDim.x * 4) {
+    if (t < n512) {
+      const uint32_t x = lists[CL_W512 * lcap + t];
+      team_sort_regs<64, 8>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], nullptr,
+                            lds[w], nullptr);
+    } else if (t < n512 + n256) {
+      const uint32_t x = lists[CL_W256 * lcap + (t - n512)];
+      team_sort_regs<64, 4>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], nullptr,
+                            lds[w], nullptr);
+    } else {
+      const uint32_t x = lists[CL_W128 * lcap + (t - n512 - n256)];
+      team_sort_regs<64, 2>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], nullptr,
+                            lds[w], nullptr);
+    }
   }
 }
 
@@ -842,44 +871,56 @@ __global__ __launch_bounds__(512) void k_seg_c_write(const uint32_t *off, const 
   }
 }
 
-// A group of 16 lanes per segment: row/column pointers from the scanned run counts, and the
-// runs of every segment (except two-pass class C) copied from its start to its final place.
+// Row/column pointers from the scanned run counts, then entry-parallel: the runs of every
+// segment (except two-pass class C, written at their place already) copied from the
+// segment's start to its final position, four entries per thread.
 __global__ __launch_bounds__(256) void k_seg_compact(const uint32_t *off, const uint32_t *ekey,
-                                                     const uint32_t *ew, const uint32_t *uoff,
-                                                     int32_t B, int one_range, int32_t *rowptr,
-                                                     int32_t *col, int32_t *val, int32_t *erow,
+                                                     const uint32_t *ew, const uint32_t *seg,
+                                                     const uint32_t *uoff, int32_t B,
+                                                     int one_range, int64_t n_ent,
+                                                     int32_t *rowptr, int32_t *col,
+                                                     int32_t *val, int32_t *erow,
                                                      int32_t *colptr, int32_t *row,
                                                      int32_t *cval, int32_t *dense) {
-  const int q = threadIdx.x & 15;
-  const int64_t ng = ((int64_t)gridDim.x * blockDim.x) >> 4;
   const uint32_t nnz = uoff[B];
-  for (int64_t x = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; x < 2 * (int64_t)B;
-       x += ng) {
-    const bool isrow = x < B;
-    const uint32_t p = isrow ? uoff[x] : uoff[x] - nnz;
-    if (q == 0) {
-      if (isrow) rowptr[x] = (int32_t)p;
-      else colptr[x - B] = (int32_t)p;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t x = tid; x <= 2 * (int64_t)B; x += stride) {
+    if (x < B) rowptr[x] = (int32_t)uoff[x];
+    else if (x < 2 * (int64_t)B) colptr[x - B] = (int32_t)(uoff[x] - nnz);
+    else { rowptr[B] = (int32_t)nnz; colptr[B] = (int32_t)nnz; }
+  }
+  for (int64_t e0 = tid; e0 < n_ent; e0 += 4 * stride) {
+    uint32_t x[4], key[4], w[4], k[4], cnt[4], p[4], n[4];
+    bool ok[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t e = e0 + j * stride;
+      ok[j] = e < n_ent;
+      if (!ok[j]) continue;
+      x[j] = seg[e];
+      key[j] = ekey[e];
+      w[j] = ew[e];
+      const uint32_t s0 = off[x[j]];
+      n[j] = off[x[j] + 1] - s0;
+      k[j] = (uint32_t)e - s0;
+      cnt[j] = uoff[x[j] + 1] - uoff[x[j]];
+      p[j] = uoff[x[j]] - (x[j] < (uint32_t)B ? 0u : nnz);
     }
-    const uint32_t n = off[x + 1] - off[x];
-    if (n == 0 || (!one_range && seg_class(n, false) == CL_C)) continue;
-    const uint32_t cnt = uoff[x + 1] - uoff[x], s0 = off[x];
-    for (uint32_t k = q; k < cnt; k += 16) {
-      const int32_t key = (int32_t)ekey[s0 + k], w = (int32_t)ew[s0 + k];
-      if (isrow) {
-        col[p + k] = key;
-        val[p + k] = w;
-        erow[p + k] = (int32_t)x;
-        if (dense) dense[x * B + key] = w;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (!ok[j] || k[j] >= cnt[j] || (!one_range && seg_class(n[j], false) == CL_C)) continue;
+      const uint32_t q = p[j] + k[j];
+      if (x[j] < (uint32_t)B) {
+        col[q] = (int32_t)key[j];
+        val[q] = (int32_t)w[j];
+        erow[q] = (int32_t)x[j];
+        if (dense) dense[(int64_t)x[j] * B + key[j]] = (int32_t)w[j];
       } else {
-        row[p + k] = key;
-        cval[p + k] = w;
+        row[q] = (int32_t)key[j];
+        cval[q] = (int32_t)w[j];
       }
     }
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    rowptr[B] = (int32_t)nnz;
-    colptr[B] = (int32_t)nnz;
   }
 }
 
@@ -913,7 +954,8 @@ static void seg_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm
                                                        rs.soff, bm.dout, bm.din, rs.vrel);
   scan_exclusive(rs.soff, rs.soff, nseg, rs.soff + nseg, rs.tmp, st);
   k_seg_fill<<<grid_for(2 * E, T), T, 0, st>>>(lab, g.esrc, g.optr, g.oadj, g.ow, g.idst, g.iptr,
-                                               g.iadj, g.iw, N, E, B, rs.soff, rs.vrel, ekey, ew);
+                                               g.iadj, g.iw, N, E, B, rs.soff, rs.vrel, ekey, ew,
+                                               rs.sseg);
   k_seg_classify<<<grid_for(nseg, T, SEG_GRID), T, 0, st>>>(rs.soff, nseg, one ? 1 : 0, rs.lists,
                                                             rs.lcap, rs.ncl);
   if (one) {
@@ -928,9 +970,9 @@ static void seg_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm
   k_seg_sort_reg<<<SEG_GRID * 2, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl,
                                                rs.su);
   scan_exclusive(rs.su, rs.suoff, nseg, rs.suoff + nseg, rs.tmp, st);
-  k_seg_compact<<<grid_for(nseg * 16, T, SEG_GRID * 2), T, 0, st>>>(
-      rs.soff, ekey, ew, rs.suoff, B, one ? 1 : 0, bm.rowptr, bm.col, bm.val, bm.erow, bm.colptr,
-      bm.row, bm.cval, D);
+  k_seg_compact<<<grid_for((2 * E + 3) / 4, T), T, 0, st>>>(
+      rs.soff, ekey, ew, rs.sseg, rs.suoff, B, one ? 1 : 0, 2 * E, bm.rowptr, bm.col, bm.val,
+      bm.erow, bm.colptr, bm.row, bm.cval, D);
   if (!one)
     k_seg_c_write<<<SEG_GRID / 2, 512, (size_t)RC * 4, st>>>(
         rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl, B, nr, rs.cntc, rs.suoff, bm.col, bm.val,
@@ -941,7 +983,7 @@ static void seg_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm
 // 128 < B <= TILE_B (dense): vertices grouped by block; workgroup (row tile, chunk) counts
 // the out-edges of its rows' vertices in LDS and writes a partial plane; the planes are
 // summed into D; CSR from D's rows, CSC from the rows of D's transpose.
-constexpr int TILE_B = 1024;
+constexpr int TILE_B = 2048;
 static void tile_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm,
                       RebuildScratch &rs, hipStream_t st) {
   const int64_t N = g.N;
@@ -956,8 +998,9 @@ static void tile_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &b
   k_label_hist<<<W, 256, (size_t)B * 4, st>>>(lab, N, B, vhist, W);
   scan_exclusive(vhist, vhist, (int64_t)B * W, nullptr, rs.tmp, st);
   k_label_scatter<<<W, 256, (size_t)B * 4, st>>>(lab, N, B, vhist, W, vperm);
+  // about 2048 workgroups, within a budget of 4M partial counters
   int32_t C = 2048 / ntiles;
-  if ((int64_t)C * BB > ((int64_t)8 << 20)) C = (int32_t)(((int64_t)8 << 20) / BB);
+  if ((int64_t)C * BB > ((int64_t)4 << 20)) C = (int32_t)(((int64_t)4 << 20) / BB);
   if ((int64_t)C * BB > rs.partials_cap) C = (int32_t)(rs.partials_cap / BB);
   if (C < 1) C = 1;
   k_tile_hist<<<dim3(ntiles, C), 512, (size_t)R * B * 4, st>>>(

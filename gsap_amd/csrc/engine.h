@@ -44,7 +44,7 @@ struct BMBuf {
   }
 };
 
-constexpr int SEG_NCL = 5;  // segment classes of the segmented rebuild
+constexpr int SEG_NCL = 8;  // segment classes of the segmented rebuild
 
 struct RebuildScratch {
   void *k0 = nullptr, *k1 = nullptr;  // 2E words each: segment keys and weights
@@ -54,6 +54,7 @@ struct RebuildScratch {
   uint32_t *soff = nullptr;           // 2 Bmax + 1: segment sizes, then offsets
   uint32_t *su = nullptr;             // 2 Bmax: runs per segment
   uint32_t *vrel = nullptr;           // 2N: each vertex's slice start inside its row / column segment
+  uint32_t *sseg = nullptr;           // 2E: segment of every segment entry
   uint32_t *suoff = nullptr;          // 2 Bmax + 1: offsets of the runs
   uint32_t *lists = nullptr;          // SEG_NCL lists of lcap segment ids
   int64_t lcap = 0;

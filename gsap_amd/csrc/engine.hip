@@ -150,7 +150,7 @@ struct gsap_ctx {
   // block matrix
   DBuf rowptr, colptr, col, val, erow, row, cval, dout, din, ldout, ldin, dense;
   // rebuild scratch
-  DBuf k0, k1, tmp, partials, soff, su, suoff, lists, ncl, cntc, vrel;
+  DBuf k0, k1, tmp, partials, soff, su, suoff, lists, ncl, cntc, vrel, sseg;
   // sweep / merge
   DBuf OUT, sof, coin, prop, dS, best, dSbest, map, terms, tiles, res;
   // device batch counter (Philox phase of the sweep's first batch); merge-order sort
@@ -176,6 +176,33 @@ namespace {
 void set_device(gsap_ctx *c) { HIPX(hipSetDevice(c->device)); }
 
 // --------------------------------------------------------------- graph upload
+// f(lo, hi, t) over [0, n) in contiguous chunks on up to 16 host threads (t = chunk index)
+template <class F>
+void parallel_chunks(int64_t n, int nt, F &&f) {
+  if (nt <= 1 || n < 65536) {
+    f((int64_t)0, n, 0);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t) {
+    const int64_t lo = n * t / nt, hi = n * (t + 1) / nt;
+    th.emplace_back([&f, lo, hi, t] { f(lo, hi, t); });
+  }
+  for (auto &x : th) x.join();
+}
+
+int host_threads() {
+  const unsigned h = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(16u, h));
+}
+
+uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
 void validate(const gsap_graph_i32 *g) {
   if (!g) throw Err(GSAP_ERR_INVALID, "graph is NULL");
   if (g->N <= 0 || g->E < 0) throw Err(GSAP_ERR_INVALID, "N must be > 0 and E >= 0");
@@ -185,34 +212,53 @@ void validate(const gsap_graph_i32 *g) {
     throw Err(GSAP_ERR_INVALID, "graph arrays must not be NULL");
   if (g->out_ptr[0] != 0 || g->in_ptr[0] != 0 || g->out_ptr[g->N] != g->E || g->in_ptr[g->N] != g->E)
     throw Err(GSAP_ERR_INVALID, "ptr arrays must start at 0 and end at E");
-  for (int64_t v = 0; v < g->N; ++v)
-    if (g->out_ptr[v + 1] < g->out_ptr[v] || g->in_ptr[v + 1] < g->in_ptr[v])
-      throw Err(GSAP_ERR_INVALID, "ptr arrays must be non-decreasing");
-  int64_t wsum = 0;
-  for (int64_t e = 0; e < g->E; ++e) {
-    if (g->out_adj[e] < 0 || g->out_adj[e] >= g->N || g->in_adj[e] < 0 || g->in_adj[e] >= g->N)
-      throw Err(GSAP_ERR_INVALID, "adjacency id out of range");
-    if (g->out_w[e] <= 0 || g->in_w[e] <= 0) throw Err(GSAP_ERR_INVALID, "weights must be > 0");
-    wsum += g->out_w[e];
+  const int nt = host_threads();
+  std::vector<int> bad(nt, 0);
+  parallel_chunks(g->N, nt, [&](int64_t lo, int64_t hi, int t) {
+    for (int64_t v = lo; v < hi; ++v)
+      if (g->out_ptr[v + 1] < g->out_ptr[v] || g->in_ptr[v + 1] < g->in_ptr[v]) bad[t] = 1;
+  });
+  for (int x : bad)
+    if (x) throw Err(GSAP_ERR_INVALID, "ptr arrays must be non-decreasing");
+  // per vertex range: ids in range, positive weights, total weight, and order-independent
+  // hash sums of the (source, target, weight) triples of both CSR views (the row and column
+  // segments of the block matrix are built from one and the other, so they must agree)
+  std::vector<int64_t> wsum(nt, 0);
+  std::vector<uint64_t> ho(nt, 0), hi_(nt, 0);
+  parallel_chunks(g->N, nt, [&](int64_t lo, int64_t hi, int t) {
+    int64_t ws = 0;
+    uint64_t a = 0, b = 0;
+    int err = 0;
+    for (int64_t v = lo; v < hi && !err; ++v) {
+      for (int32_t j = g->out_ptr[v]; j < g->out_ptr[v + 1]; ++j) {
+        const int32_t u = g->out_adj[j], w = g->out_w[j];
+        if (u < 0 || u >= g->N) { err = 1; break; }
+        if (w <= 0) { err = 2; break; }
+        ws += w;
+        a += mix64(((uint64_t)v << 32 | (uint32_t)u) ^ mix64((uint64_t)w));
+      }
+      for (int32_t j = g->in_ptr[v]; j < g->in_ptr[v + 1] && !err; ++j) {
+        const int32_t u = g->in_adj[j], w = g->in_w[j];
+        if (u < 0 || u >= g->N) { err = 1; break; }
+        if (w <= 0) { err = 2; break; }
+        b += mix64(((uint64_t)(uint32_t)u << 32 | (uint64_t)v) ^ mix64((uint64_t)w));
+      }
+    }
+    bad[t] = err;
+    wsum[t] = ws;
+    ho[t] = a;
+    hi_[t] = b;
+  });
+  for (int x : bad) {
+    if (x == 1) throw Err(GSAP_ERR_INVALID, "adjacency id out of range");
+    if (x == 2) throw Err(GSAP_ERR_INVALID, "weights must be > 0");
   }
-  if (2 * wsum >= ((int64_t)1 << 31) - 1)
+  int64_t W = 0;
+  uint64_t HO = 0, HI = 0;
+  for (int t = 0; t < nt; ++t) { W += wsum[t]; HO += ho[t]; HI += hi_[t]; }
+  if (2 * W >= ((int64_t)1 << 31) - 1)
     throw Err(GSAP_ERR_INVALID, "total edge weight must stay below 2^30");
-  // the in-CSR must hold the same weighted edges as the out-CSR (row and column segments
-  // of the block matrix are built from one and the other): order-independent hash sums
-  auto mix = [](uint64_t x) {
-    x += 0x9E3779B97F4A7C15ull;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-    return x ^ (x >> 31);
-  };
-  uint64_t ho = 0, hi = 0;
-  for (int64_t v = 0; v < g->N; ++v) {
-    for (int32_t j = g->out_ptr[v]; j < g->out_ptr[v + 1]; ++j)
-      ho += mix(((uint64_t)v << 32 | (uint32_t)g->out_adj[j]) ^ mix((uint64_t)g->out_w[j]));
-    for (int32_t j = g->in_ptr[v]; j < g->in_ptr[v + 1]; ++j)
-      hi += mix(((uint64_t)(uint32_t)g->in_adj[j] << 32 | (uint64_t)v) ^ mix((uint64_t)g->in_w[j]));
-  }
-  if (ho != hi) throw Err(GSAP_ERR_INVALID, "in-CSR and out-CSR hold different edges");
+  if (HO != HI) throw Err(GSAP_ERR_INVALID, "in-CSR and out-CSR hold different edges");
 }
 
 template <class T>
@@ -227,10 +273,12 @@ void upload_graph(gsap_ctx *c, const gsap_graph_i32 *g) {
   c->N = N; c->E = E;
   hipStream_t st = c->st;
   std::vector<int32_t> kout(N, 0), kin(N, 0), esrc(E), idst(E);
-  for (int64_t v = 0; v < N; ++v) {
-    for (int32_t j = g->out_ptr[v]; j < g->out_ptr[v + 1]; ++j) { kout[v] += g->out_w[j]; esrc[j] = (int32_t)v; }
-    for (int32_t j = g->in_ptr[v]; j < g->in_ptr[v + 1]; ++j) { kin[v] += g->in_w[j]; idst[j] = (int32_t)v; }
-  }
+  parallel_chunks(N, host_threads(), [&](int64_t lo, int64_t hi, int) {
+    for (int64_t v = lo; v < hi; ++v) {
+      for (int32_t j = g->out_ptr[v]; j < g->out_ptr[v + 1]; ++j) { kout[v] += g->out_w[j]; esrc[j] = (int32_t)v; }
+      for (int32_t j = g->in_ptr[v]; j < g->in_ptr[v + 1]; ++j) { kin[v] += g->in_w[j]; idst[j] = (int32_t)v; }
+    }
+  });
   // vertices of the largest nodal class get tables and slot arrays in the global pool
   // (used when they do not fit the launch's LDS); offsets in int32 words, even
   std::vector<int64_t> hoff(N, -1);
@@ -331,6 +379,7 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   c->partials.ensure(4 * pc);
   c->soff.ensure(4 * (2 * N + 1));
   c->su.ensure(4 * 2 * N); c->suoff.ensure(4 * (2 * N + 1)); c->vrel.ensure(4 * 2 * N);
+  c->sseg.ensure(8 * E);
   c->lists.ensure(4 * (size_t)SEG_NCL * 2 * N); c->ncl.ensure(4 * 8);
   c->cntc.ensure(4 * (size_t)std::max<int64_t>(seg_cntc_words(N, E), 1));
   c->OUT.ensure(8 * N); c->sof.ensure(4 * N); c->coin.ensure(N);
@@ -352,7 +401,7 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   r.k0 = c->k0.p; r.k1 = c->k1.p; r.tmp = c->tmp.as<uint32_t>();
   r.partials = c->partials.as<int32_t>(); r.partials_cap = pc;
   r.soff = c->soff.as<uint32_t>();
-  r.su = c->su.as<uint32_t>(); r.suoff = c->suoff.as<uint32_t>(); r.vrel = c->vrel.as<uint32_t>();
+  r.su = c->su.as<uint32_t>(); r.suoff = c->suoff.as<uint32_t>(); r.vrel = c->vrel.as<uint32_t>(); r.sseg = c->sseg.as<uint32_t>();
   r.lists = c->lists.as<uint32_t>(); r.lcap = 2 * N; r.ncl = c->ncl.as<uint32_t>();
   r.cntc = c->cntc.as<uint32_t>();
 }
@@ -592,6 +641,7 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
   upload_graph(c, gin);
   build_batch_lists(c, gin, o.num_batches);
   alloc_work(c, o.num_proposals, resolve_dense(o));
+  const double t_upload = now() - t0;
   c->trace.clear();
   c->prof.reset(o.profile != 0);
   c->prof.mask = o.profile_classes ? (uint32_t)o.profile_classes : 0xFFu;
@@ -764,6 +814,7 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
     rep->B = B; rep->levels = levels; rep->merge_phases = (int32_t)merge_ctr;
     rep->sweeps = sweeps; rep->batches = batches; rep->S = S;
     rep->t_total = now() - t0; rep->t_merge = t_merge; rep->t_sweep = t_sweep;
+    rep->t_upload = t_upload;
     rep->sweep_edges = c->E * (int64_t)sweeps;
     rep->n_trace = (int64_t)c->trace.size();
     for (int k = 0; k < 8; ++k) { rep->kernel_ms[k] = c->prof.ms[k]; rep->kernel_calls[k] = c->prof.calls[k]; }
@@ -893,7 +944,7 @@ int gsap_destroy(gsap_ctx *c) {
                  &c->snap[1], &c->snap[2], &c->rowptr, &c->colptr, &c->col, &c->val, &c->erow,
                  &c->row, &c->cval, &c->dout, &c->din, &c->ldout, &c->ldin, &c->dense, &c->k0,
                  &c->k1, &c->tmp, &c->partials, &c->soff, &c->su, &c->suoff,
-                 &c->lists, &c->ncl, &c->cntc, &c->vrel, &c->vlist,
+                 &c->lists, &c->ncl, &c->cntc, &c->vrel, &c->sseg, &c->vlist,
                  &c->OUT, &c->sof, &c->coin, &c->prop, &c->dS, &c->best, &c->dSbest, &c->map,
                  &c->terms, &c->tiles, &c->res, &c->ctr, &c->mk0, &c->mk1, &c->mv0,
                  &c->mv1, &c->errv, &c->evcls, &c->mixstat};

@@ -12,6 +12,23 @@ GSAP_D double wave_sum(double x) {
   return x;  // lane 0
 }
 
+// LDS copies of the canonical log2 tables: every lookup of a kernel that loads them at its
+// start is an LDS read instead of a vector-cache read (same values, so the same bits)
+__shared__ double s_log2_inv[128], s_log2_lg[128];
+GSAP_D void log2_tables_to_lds() {
+  for (int i = threadIdx.x; i < 128; i += blockDim.x) {
+    s_log2_inv[i] = d_log2_inv[i];
+    s_log2_lg[i] = d_log2_lg[i];
+  }
+  __syncthreads();
+}
+GSAP_D double clog2_s(double x) { return clog2_core(x, s_log2_inv, s_log2_lg, d_log2_poly); }
+// term() of common.h with the LDS tables
+GSAP_D double term_s(int64_t w, double la, double lb) {
+  const double dw = (double)w;
+  return dw * (clog2_s(dw) - (la + lb));
+}
+
 GSAP_D uint32_t wave_incl_scan_u32(uint32_t x) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -110,6 +127,7 @@ template <int G>
 __global__ __launch_bounds__(256) void k_block_terms(BMView m, double *OUT, int32_t *sof,
                                                      uint8_t *coin, RngKey key, PhaseRef pr,
                                                      int sample) {
+  log2_tables_to_lds();
   const int32_t x = (int32_t)(blockIdx.x * (256 / G) + threadIdx.x / G);
   if (x >= m.B) return;
   const int q = threadIdx.x & (G - 1);
@@ -119,7 +137,7 @@ __global__ __launch_bounds__(256) void k_block_terms(BMView m, double *OUT, int3
     const int32_t b0 = m.rowptr[x], n = m.rowptr[x + 1] - b0;
     const double lo = m.ldout[x];
     seg_for<G>(n, [&](int32_t k, int j) {
-      acc.p[j] += term(m.val[b0 + k], lo, m.ldin[m.col[b0 + k]]);
+      acc.p[j] += term_s(m.val[b0 + k], lo, m.ldin[m.col[b0 + k]]);
     });
     const double out = acc.reduce();
     if (q == 0) OUT[x] = out;
@@ -265,10 +283,10 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
   wave_lds_sync();
   const int64_t c_r = ht_weight(hi, r), c_s = ht_weight(hi, s);
   const bool r_in_a = ht_find(ho, r) >= 0, s_in_a = ht_find(ho, s) >= 0;
-  const double l_dr = clog2((double)(dout_r - kout));
-  const double l_ds = clog2((double)(dout_s + kout));
-  const double l_ir = clog2((double)(din_r - kin));
-  const double l_is = clog2((double)(din_s + kin));
+  const double l_dr = clog2_s((double)(dout_r - kout));
+  const double l_ds = clog2_s((double)(dout_s + kout));
+  const double l_ir = clog2_s((double)(din_r - kin));
+  const double l_is = clog2_s((double)(din_s + kin));
   // M[r][r], M[s][r], M[r][s], M[s][s] for the untouched column counts, one lane each
   int64_t corner = 0;
   if (lane < 4) {
@@ -323,10 +341,10 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
       int64_t Mr, Ms, R, S;
       if (!row_slot(j, t, Mr, Ms, R, S)) continue;
       const double ldt = m.ldin[t];
-      const double or_ = Mr > 0 ? term(Mr, ldo_r, ldt) : 0.0;
-      const double os_ = Ms > 0 ? term(Ms, ldo_s, ldt) : 0.0;
-      const double nr = R > 0 ? term(R, ldt, l_dr) : 0.0;
-      const double ns = S > 0 ? term(S, ldt, l_ds) : 0.0;
+      const double or_ = Mr > 0 ? term_s(Mr, ldo_r, ldt) : 0.0;
+      const double os_ = Ms > 0 ? term_s(Ms, ldo_s, ldt) : 0.0;
+      const double nr = R > 0 ? term_s(R, ldt, l_dr) : 0.0;
+      const double ns = S > 0 ? term_s(S, ldt, l_ds) : 0.0;
       p_row += ((or_ + os_) - nr) - ns;
       if (sums) {
         sum_r += Mr;
@@ -339,11 +357,11 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
       int64_t Mr, Ms, c;
       if (!col_slot(j, t, Mr, Ms, c)) continue;
       const double ldt = m.ldout[t];
-      const double ir = Mr > 0 ? term(Mr, ldt, ldi_r) : 0.0;
-      const double is = Ms > 0 ? term(Ms, ldt, ldi_s) : 0.0;
+      const double ir = Mr > 0 ? term_s(Mr, ldt, ldi_r) : 0.0;
+      const double is = Ms > 0 ? term_s(Ms, ldt, ldi_s) : 0.0;
       const int64_t wr = Mr - c, ws = Ms + c;
-      const double nr = wr > 0 ? term(wr, ldt, l_ir) : 0.0;
-      const double ns = ws > 0 ? term(ws, ldt, l_is) : 0.0;
+      const double nr = wr > 0 ? term_s(wr, ldt, l_ir) : 0.0;
+      const double ns = ws > 0 ? term_s(ws, ldt, l_is) : 0.0;
       p_col += ((ir + is) - nr) - ns;
       if (sums) {
         csum_r += Mr;
@@ -501,10 +519,10 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
   __syncthreads();
   const int64_t c_r = ht_weight(hi, r), c_s = ht_weight(hi, s);
   const bool r_in_a = ht_find(ho, r) >= 0, s_in_a = ht_find(ho, s) >= 0;
-  const double l_dr = clog2((double)(dout_r - kout));
-  const double l_ds = clog2((double)(dout_s + kout));
-  const double l_ir = clog2((double)(din_r - kin));
-  const double l_is = clog2((double)(din_s + kin));
+  const double l_dr = clog2_s((double)(dout_r - kout));
+  const double l_ds = clog2_s((double)(dout_s + kout));
+  const double l_ir = clog2_s((double)(din_r - kin));
+  const double l_is = clog2_s((double)(din_s + kin));
   int64_t sum_r = 0, sum_s = 0, csum_r = 0, csum_s = 0;
   // row slots: positions 0..no-1 (first occurrences), no (r), no+1 (s)
   for (int32_t j = tid; j < no + 2; j += 256) {
@@ -531,10 +549,10 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
       const int64_t R = Mr - av - (t == r ? c_r : 0) + (t == s ? c_r : 0);
       const int64_t S = Ms + av - (t == r ? c_s : 0) + (t == s ? c_s : 0);
       const double ldt = m.ldin[t];
-      const double or_ = Mr > 0 ? term(Mr, ldo_r, ldt) : 0.0;
-      const double os_ = Ms > 0 ? term(Ms, ldo_s, ldt) : 0.0;
-      const double nr = R > 0 ? term(R, ldt, l_dr) : 0.0;
-      const double ns = S > 0 ? term(S, ldt, l_ds) : 0.0;
+      const double or_ = Mr > 0 ? term_s(Mr, ldo_r, ldt) : 0.0;
+      const double os_ = Ms > 0 ? term_s(Ms, ldo_s, ldt) : 0.0;
+      const double nr = R > 0 ? term_s(R, ldt, l_dr) : 0.0;
+      const double ns = S > 0 ? term_s(S, ldt, l_ds) : 0.0;
       xv = ((or_ + os_) - nr) - ns;
       sum_r += Mr;
       sum_s += Ms;
@@ -552,11 +570,11 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
       bm_col_get2(m, r, s, t, Mr32, Ms32);
       const int64_t Mr = Mr32, Ms = Ms32;
       const double ldt = m.ldout[t];
-      const double ir = Mr > 0 ? term(Mr, ldt, ldi_r) : 0.0;
-      const double is = Ms > 0 ? term(Ms, ldt, ldi_s) : 0.0;
+      const double ir = Mr > 0 ? term_s(Mr, ldt, ldi_r) : 0.0;
+      const double is = Ms > 0 ? term_s(Ms, ldt, ldi_s) : 0.0;
       const int64_t wr = Mr - c, ws = Ms + c;
-      const double nr = wr > 0 ? term(wr, ldt, l_ir) : 0.0;
-      const double ns = ws > 0 ? term(ws, ldt, l_is) : 0.0;
+      const double nr = wr > 0 ? term_s(wr, ldt, l_ir) : 0.0;
+      const double ns = ws > 0 ? term_s(ws, ldt, l_is) : 0.0;
       xv = ((ir + is) - nr) - ns;
       csum_r += Mr;
       csum_s += Ms;
@@ -685,10 +703,10 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
   const bool s_in_a = ((__ballot(q < no && t_o == s) >> gbase) & gmask) != 0;
   const int64_t dout_r = m.dout[r], dout_s = m.dout[s], din_r = m.din[r], din_s = m.din[s];
   const double ldo_r = m.ldout[r], ldo_s = m.ldout[s], ldi_r = m.ldin[r], ldi_s = m.ldin[s];
-  const double l_dr = clog2((double)(dout_r - kout));
-  const double l_ds = clog2((double)(dout_s + kout));
-  const double l_ir = clog2((double)(din_r - kin));
-  const double l_is = clog2((double)(din_s + kin));
+  const double l_dr = clog2_s((double)(dout_r - kout));
+  const double l_ds = clog2_s((double)(dout_s + kout));
+  const double l_ir = clog2_s((double)(din_r - kin));
+  const double l_is = clog2_s((double)(din_s + kin));
   int64_t corner = 0;
   if (q < 4) {
     const int32_t cc = q < 2 ? r : s, rr = (q & 1) ? s : r;
@@ -742,10 +760,10 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
       int64_t Mr, Ms, R, S;
       if (!row_slot(j, t, Mr, Ms, R, S)) continue;
       const double ldt = m.ldin[t];
-      const double or_ = Mr > 0 ? term(Mr, ldo_r, ldt) : 0.0;
-      const double os_ = Ms > 0 ? term(Ms, ldo_s, ldt) : 0.0;
-      const double nr = R > 0 ? term(R, ldt, l_dr) : 0.0;
-      const double ns = S > 0 ? term(S, ldt, l_ds) : 0.0;
+      const double or_ = Mr > 0 ? term_s(Mr, ldo_r, ldt) : 0.0;
+      const double os_ = Ms > 0 ? term_s(Ms, ldo_s, ldt) : 0.0;
+      const double nr = R > 0 ? term_s(R, ldt, l_dr) : 0.0;
+      const double ns = S > 0 ? term_s(S, ldt, l_ds) : 0.0;
       prow.p[jj % CAcc<G>::S] += ((or_ + os_) - nr) - ns;
       if (sums) {
         sum_r += Mr;
@@ -758,11 +776,11 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
       int64_t Mr, Ms;
       col_get(Mr, Ms);
       const double ldt = m.ldout[t_i];
-      const double ir = Mr > 0 ? term(Mr, ldt, ldi_r) : 0.0;
-      const double is = Ms > 0 ? term(Ms, ldt, ldi_s) : 0.0;
+      const double ir = Mr > 0 ? term_s(Mr, ldt, ldi_r) : 0.0;
+      const double is = Ms > 0 ? term_s(Ms, ldt, ldi_s) : 0.0;
       const int64_t wr = Mr - c_sum, ws = Ms + c_sum;
-      const double nr = wr > 0 ? term(wr, ldt, l_ir) : 0.0;
-      const double ns = ws > 0 ? term(ws, ldt, l_is) : 0.0;
+      const double nr = wr > 0 ? term_s(wr, ldt, l_ir) : 0.0;
+      const double ns = ws > 0 ? term_s(ws, ldt, l_is) : 0.0;
       pcol.p[0] += ((ir + is) - nr) - ns;
       if (sums) {
         csum_r += Mr;
@@ -872,6 +890,7 @@ struct NodalLists {
 template <bool MIX>
 __global__ __launch_bounds__(256) void k_nodal_fused(NodalArgs a, NodalLists L) {
   extern __shared__ double lds_d[];
+  log2_tables_to_lds();
   int32_t b = (int32_t)blockIdx.x;
   if (b < L.nb[3]) {
     nodal_big<MIX>(a, L.list[3][b], lds_d, L.big_lds);
@@ -1013,10 +1032,11 @@ void launch_csum_array(const double *x, const int32_t *n_dev, int64_t n_host, in
 
 // data term of every block-matrix entry, CSR order (kernels.hpp:1066-1093)
 __global__ void k_mdl_terms(BMView m, const int32_t *erow, double *terms) {
+  log2_tables_to_lds();
   const int32_t nnz = m.rowptr[m.B];
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
        k += (int64_t)gridDim.x * blockDim.x)
-    terms[k] = term(m.val[k], m.ldout[erow[k]], m.ldin[m.col[k]]);
+    terms[k] = term_s(m.val[k], m.ldout[erow[k]], m.ldin[m.col[k]]);
 }
 
 void launch_mdl_terms(const BMView &m, const int32_t *erow, int64_t n_max, double *terms,
@@ -1033,6 +1053,7 @@ template <int G>
 __global__ __launch_bounds__(256) void k_merge(BMView m, const double *OUT, RngKey key,
                                                uint32_t phase, int32_t np, int32_t *prop,
                                                double *dSout) {
+  log2_tables_to_lds();
   const int64_t idx = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
   const int32_t B = m.B;
   if (idx >= (int64_t)B * np) return;
@@ -1058,7 +1079,7 @@ __global__ __launch_bounds__(256) void k_merge(BMView m, const double *OUT, RngK
     const double li = m.ldin[r];
     seg_for<G>(n, [&](int32_t k, int j) {
       const int32_t t = m.row[b0 + k];
-      if (t != r && t != s) acc.p[j] += term(m.cval[b0 + k], m.ldout[t], li);
+      if (t != r && t != s) acc.p[j] += term_s(m.cval[b0 + k], m.ldout[t], li);
     });
     inx_r = acc.reduce();
   }
@@ -1069,12 +1090,12 @@ __global__ __launch_bounds__(256) void k_merge(BMView m, const double *OUT, RngK
     const double li = m.ldin[s];
     seg_for<G>(n, [&](int32_t k, int j) {
       const int32_t t = m.row[b0 + k];
-      if (t != r && t != s) acc.p[j] += term(m.cval[b0 + k], m.ldout[t], li);
+      if (t != r && t != s) acc.p[j] += term_s(m.cval[b0 + k], m.ldout[t], li);
     });
     inx_s = acc.reduce();
   }
   // merged row: row r entries (M_rt + M_st), then row s entries absent from row r
-  const double ldo = clog2((double)((int64_t)m.dout[r] + m.dout[s]));
+  const double ldo = clog2_s((double)((int64_t)m.dout[r] + m.dout[s]));
   double new_out;
   {
     CAcc<G> na, nb;
@@ -1083,13 +1104,13 @@ __global__ __launch_bounds__(256) void k_merge(BMView m, const double *OUT, RngK
     const int32_t b0 = m.rowptr[r], n = m.rowptr[r + 1] - b0;
     seg_for<G>(n, [&](int32_t k, int j) {
       const int32_t t = m.col[b0 + k];
-      na.p[j] += term((int64_t)m.val[b0 + k] + bm_row_get(m, s, t), m.ldin[t], ldo);
+      na.p[j] += term_s((int64_t)m.val[b0 + k] + bm_row_get(m, s, t), m.ldin[t], ldo);
     });
     if (s != r) {
       const int32_t c0 = m.rowptr[s], c = m.rowptr[s + 1] - c0;
       seg_for<G>(c, [&](int32_t k, int j) {
         const int32_t t = m.col[c0 + k];
-        if (bm_row_get(m, r, t) == 0) nb.p[j] += term(m.val[c0 + k], m.ldin[t], ldo);
+        if (bm_row_get(m, r, t) == 0) nb.p[j] += term_s(m.val[c0 + k], m.ldin[t], ldo);
       });
     }
     const double a_ = na.reduce();
@@ -1097,7 +1118,7 @@ __global__ __launch_bounds__(256) void k_merge(BMView m, const double *OUT, RngK
     new_out = a_ + b_;
   }
   // merged column, t not in {r, s}
-  const double ldi = clog2((double)((int64_t)m.din[r] + m.din[s]));
+  const double ldi = clog2_s((double)((int64_t)m.din[r] + m.din[s]));
   double new_in;
   {
     CAcc<G> ia, ib;
@@ -1107,14 +1128,14 @@ __global__ __launch_bounds__(256) void k_merge(BMView m, const double *OUT, RngK
     seg_for<G>(n, [&](int32_t k, int j) {
       const int32_t t = m.row[b0 + k];
       if (t != r && t != s)
-        ia.p[j] += term((int64_t)m.cval[b0 + k] + bm_col_get(m, s, t), m.ldout[t], ldi);
+        ia.p[j] += term_s((int64_t)m.cval[b0 + k] + bm_col_get(m, s, t), m.ldout[t], ldi);
     });
     if (s != r) {
       const int32_t c0 = m.colptr[s], c = m.colptr[s + 1] - c0;
       seg_for<G>(c, [&](int32_t k, int j) {
         const int32_t t = m.row[c0 + k];
         if (t != r && t != s && bm_col_get(m, r, t) == 0)
-          ib.p[j] += term(m.cval[c0 + k], m.ldout[t], ldi);
+          ib.p[j] += term_s(m.cval[c0 + k], m.ldout[t], ldi);
       });
     }
     const double a_ = ia.reduce();

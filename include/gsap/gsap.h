@@ -110,6 +110,7 @@ typedef struct gsap_report {
                           /* 3 merge proposals+delta-S, 4 MDL, 5 accept+batch sum        */
   int64_t fp64_evals;     /* mixed form: nodal proposals evaluated in FP64 (rechecks plus
                              every accepted move, whose exact delta-S enters the batch sum) */
+  double t_upload;        /* graph validation, host preparation and upload */
 } gsap_report;
 
 /* One trace record; identical layout to the oracle's.
