@@ -39,12 +39,18 @@ __global__ void k_dense_clear(const int32_t *erow, const int32_t *col, const int
     dense[(int64_t)erow[k] * B + col[k]] = 0;
 }
 
+__global__ void k_lab16(const int32_t *lab, int64_t N, uint16_t *lab16) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < N;
+       v += (int64_t)gridDim.x * blockDim.x)
+    lab16[v] = (uint16_t)lab[v];
+}
+
 // ------------------------------------------------------------------ small path
 // B <= 128: the whole B x B histogram in LDS; workgroup c takes a contiguous chunk of the
 // edges (id order, one edge per thread per step, so that many label gathers are in flight)
 // and writes its plane of partial counts (or D itself when C == 1).
 __global__ __launch_bounds__(512) void k_edge_hist(const int32_t *esrc, const int32_t *oadj,
-                                                   const int32_t *ow, const int32_t *lab,
+                                                   const int32_t *ow, LabView lab,
                                                    int64_t E, int32_t B, int32_t C,
                                                    int32_t *partial) {
   extern __shared__ int32_t T[];
@@ -163,7 +169,7 @@ __global__ __launch_bounds__(1024) void k_small_bm(const int32_t *D, int32_t B, 
 // ---- vertices grouped by block (counting sort; order inside a block is irrelevant)
 constexpr int VT = 2048;  // vertices per workgroup
 
-__global__ __launch_bounds__(256) void k_label_hist(const int32_t *lab, int64_t N, int32_t B,
+__global__ __launch_bounds__(256) void k_label_hist(LabView lab, int64_t N, int32_t B,
                                                     uint32_t *H, int32_t W) {
   extern __shared__ uint32_t cnt[];
   for (int32_t b = threadIdx.x; b < B; b += blockDim.x) cnt[b] = 0;
@@ -175,7 +181,7 @@ __global__ __launch_bounds__(256) void k_label_hist(const int32_t *lab, int64_t 
   for (int32_t b = threadIdx.x; b < B; b += blockDim.x) H[(int64_t)b * W + blockIdx.x] = cnt[b];
 }
 
-__global__ __launch_bounds__(256) void k_label_scatter(const int32_t *lab, int64_t N, int32_t B,
+__global__ __launch_bounds__(256) void k_label_scatter(LabView lab, int64_t N, int32_t B,
                                                        const uint32_t *H, int32_t W,
                                                        int32_t *vperm) {
   extern __shared__ uint32_t cur[];
@@ -190,7 +196,7 @@ __global__ __launch_bounds__(256) void k_label_scatter(const int32_t *lab, int64
 // those rows' vertices; each wave takes a vertex and its lanes add its out-edges into the
 // LDS tile; the tile goes out as a plane of partial counts (no global atomics).
 __global__ __launch_bounds__(512) void k_tile_hist(const int32_t *optr, const int32_t *oadj,
-                                                   const int32_t *ow, const int32_t *lab,
+                                                   const int32_t *ow, LabView lab,
                                                    const int32_t *vperm, const uint32_t *H,
                                                    int32_t W, int64_t N, int32_t B, int32_t R,
                                                    int32_t C, int32_t *partial) {
@@ -278,8 +284,8 @@ __global__ __launch_bounds__(256) void k_transpose(const int32_t *D, int32_t B, 
 // nonzero keys in order over the segment's start (one pass); otherwise one workgroup's LDS
 // sort takes n <= 4096 and larger segments are counted per (segment, column range) in two
 // passes (counts, then writes at the final position).
-enum { CL_A16 = 0, CL_A32 = 1, CL_A64 = 2, CL_W128 = 3, CL_W256 = 4, CL_W512 = 5, CL_G = 6,
-       CL_C = 7 };
+enum { CL_A16 = 0, CL_A32 = 1, CL_A64 = 2, CL_W128 = 3, CL_W256 = 4, CL_W512 = 5, CL_G1K = 6,
+       CL_G2K = 7, CL_G4K = 8, CL_C = 9 };
 // SEG_RC counters of a column range plus the workgroup's reduction words stay under 64 KB
 constexpr int SEG_W = 512, SEG_G = 4096, SEG_RC = 16000;
 constexpr unsigned SEG_GRID = 1024;  // workgroups of the persistent segment kernels
@@ -293,7 +299,9 @@ GSAP_HD int seg_class(uint32_t n, bool one_range) {
   if (n <= 256) return CL_W256;
   if (n <= (uint32_t)SEG_W) return CL_W512;
   if (one_range) return CL_C;
-  return n <= (uint32_t)SEG_G ? CL_G : CL_C;
+  if (n <= 1024) return CL_G1K;
+  if (n <= 2048) return CL_G2K;
+  return n <= (uint32_t)SEG_G ? CL_G4K : CL_C;
 }
 
 __global__ void k_seg_zero(uint32_t *off, uint32_t *u, uint32_t *ncl, int32_t *dout,
@@ -314,7 +322,7 @@ __global__ void k_seg_zero(uint32_t *off, uint32_t *u, uint32_t *ncl, int32_t *d
 // degree sums) with one global atomic per block, so that blocks of many vertices do not
 // serialize on one address.
 template <bool PRIV>
-__global__ __launch_bounds__(256) void k_seg_reserve(const int32_t *lab, const int32_t *optr,
+__global__ __launch_bounds__(256) void k_seg_reserve(LabView lab, const int32_t *optr,
                                                      const int32_t *iptr, const int32_t *kout,
                                                      const int32_t *kin, int64_t N, int32_t B,
                                                      uint32_t *cnt, int32_t *dout,
@@ -375,7 +383,7 @@ __global__ __launch_bounds__(256) void k_seg_reserve(const int32_t *lab, const i
 // Edge-parallel: out-edge e goes to its source's slice of the source block's row segment,
 // keyed by the target's block; in-edge f to its target's slice of the target block's column
 // segment, keyed by the source's block; seg[pos] records the segment of every entry.
-__global__ __launch_bounds__(256) void k_seg_fill(const int32_t *lab, const int32_t *esrc,
+__global__ __launch_bounds__(256) void k_seg_fill(LabView lab, const int32_t *esrc,
                                                   const int32_t *optr, const int32_t *oadj,
                                                   const int32_t *ow, const int32_t *idst,
                                                   const int32_t *iptr, const int32_t *iadj,
@@ -702,18 +710,26 @@ __global__ __launch_bounds__(256) void k_seg_sort_w(const uint32_t *off, uint32_
   }
 }
 
-// one workgroup per segment of class G (n <= 4096): 16 entries per thread, 32 KB of LDS
+// one workgroup per segment of 513..4096 entries: 4, 8 or 16 entries per thread (networks
+// of 1024, 2048 or 4096), 32 KB of LDS
 __global__ __launch_bounds__(256) void k_seg_sort_g(const uint32_t *off, uint32_t *ekey,
                                                     uint32_t *ew, const uint32_t *lists,
                                                     int64_t lcap, const uint32_t *ncl,
                                                     uint32_t *u) {
   __shared__ uint32_t xk[SEG_G], xv[SEG_G];
   __shared__ uint32_t sc4[4];
-  const int64_t n = ncl[CL_G];
-  for (int64_t t = blockIdx.x; t < n; t += gridDim.x) {
-    const uint32_t x = lists[CL_G * lcap + t];
-    team_sort_regs<256, SEG_G / 256>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], xk,
-                                     xv, sc4);
+  const int64_t n4 = ncl[CL_G4K], n2 = ncl[CL_G2K], n1 = ncl[CL_G1K];
+  for (int64_t t = blockIdx.x; t < n4 + n2 + n1; t += gridDim.x) {
+    if (t < n4) {
+      const uint32_t x = lists[CL_G4K * lcap + t];
+      team_sort_regs<256, 16>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], xk, xv, sc4);
+    } else if (t < n4 + n2) {
+      const uint32_t x = lists[CL_G2K * lcap + (t - n4)];
+      team_sort_regs<256, 8>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], xk, xv, sc4);
+    } else {
+      const uint32_t x = lists[CL_G1K * lcap + (t - n4 - n2)];
+      team_sort_regs<256, 4>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], xk, xv, sc4);
+    }
   }
 }
 
@@ -936,7 +952,7 @@ int64_t seg_cntc_words(int64_t N, int64_t E) {
   return (2 * E / (SEG_G + 1) + 1) * nr;
 }
 
-static void seg_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm,
+static void seg_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
                      RebuildScratch &rs, bool dense, hipStream_t st) {
   const int64_t N = g.N, E = g.E, nseg = 2 * (int64_t)B;
   const int T = 256;
@@ -984,7 +1000,7 @@ static void seg_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm
 // the out-edges of its rows' vertices in LDS and writes a partial plane; the planes are
 // summed into D; CSR from D's rows, CSC from the rows of D's transpose.
 constexpr int TILE_B = 2048;
-static void tile_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm,
+static void tile_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
                       RebuildScratch &rs, hipStream_t st) {
   const int64_t N = g.N;
   const int64_t BB = (int64_t)B * B;
@@ -998,9 +1014,11 @@ static void tile_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &b
   k_label_hist<<<W, 256, (size_t)B * 4, st>>>(lab, N, B, vhist, W);
   scan_exclusive(vhist, vhist, (int64_t)B * W, nullptr, rs.tmp, st);
   k_label_scatter<<<W, 256, (size_t)B * 4, st>>>(lab, N, B, vhist, W, vperm);
-  // about 2048 workgroups, within a budget of 4M partial counters
+  // about 2048 workgroups, within a budget of partial counters that grows with E (a
+  // workgroup's chunk stays near E / 1024 edges at most sizes)
   int32_t C = 2048 / ntiles;
-  if ((int64_t)C * BB > ((int64_t)4 << 20)) C = (int32_t)(((int64_t)4 << 20) / BB);
+  const int64_t budget = std::min<int64_t>(std::max<int64_t>((int64_t)4 << 20, g.E / 2), rs.partials_cap);
+  if ((int64_t)C * BB > budget) C = (int32_t)(budget / BB);
   if ((int64_t)C * BB > rs.partials_cap) C = (int32_t)(rs.partials_cap / BB);
   if (C < 1) C = 1;
   k_tile_hist<<<dim3(ntiles, C), 512, (size_t)R * B * 4, st>>>(
@@ -1024,7 +1042,7 @@ static void tile_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &b
   k_logs<<<grid_for(B, 256), 256, 0, st>>>(bm.dout, bm.din, B, bm.ldout, bm.ldin);
 }
 
-static void small_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm,
+static void small_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
                        RebuildScratch &rs, hipStream_t st) {
   const int64_t BB = (int64_t)B * B;
   const int T = 256;
@@ -1046,7 +1064,7 @@ static void small_path(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &
                                  bm.cval, bm.dout, bm.din, bm.ldout, bm.ldin);
 }
 
-void rebuild_blockmodel(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm,
+void rebuild_blockmodel(const GraphDev &g, const int32_t *lab32, int32_t B, BMBuf &bm,
                         RebuildScratch &rs, int32_t dense_max_B, hipStream_t st) {
   const bool dense = B <= dense_max_B && (int64_t)B * B <= bm.dense_cap;
   const bool small = dense && B <= SMALL_B && (int64_t)B * B * 128 <= rs.partials_cap;
@@ -1059,6 +1077,12 @@ void rebuild_blockmodel(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf 
     bm.dense_B = 0;
   }
   bm.B = B;
+  // the 16-bit label copy read here and by the next nodal batch (same labels)
+  LabView lab{lab32, nullptr};
+  if (B <= 65535 && rs.lab16) {
+    k_lab16<<<grid_for(g.N, 256), 256, 0, st>>>(lab32, g.N, rs.lab16);
+    lab.l16 = rs.lab16;
+  }
   if (small) small_path(g, lab, B, bm, rs, st);
   else if (tile) tile_path(g, lab, B, bm, rs, st);
   else seg_path(g, lab, B, bm, rs, dense, st);

@@ -247,6 +247,16 @@ GSAP_D int32_t ht_weight(const HTab &h, int32_t t) {
   return s < 0 ? 0 : ld_relaxed(&h.w[s]);
 }
 
+// ------------------------------------------------------------------ labels
+// Block labels as read by the gather-heavy kernels: a 16-bit copy when every label fits
+// (half the cache footprint of the int32 array; the int32 array of a 2M-vertex graph no
+// longer fits one XCD's L2), else the int32 array.
+struct LabView {
+  const int32_t *l32;
+  const uint16_t *l16;
+  GSAP_D int32_t operator[](int64_t i) const { return l16 ? (int32_t)l16[i] : l32[i]; }
+};
+
 // ------------------------------------------------------------ block matrix
 // Device view of the block matrix M (CSR + CSC, degrees, cached logs).
 struct BMView {

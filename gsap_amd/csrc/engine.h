@@ -44,7 +44,7 @@ struct BMBuf {
   }
 };
 
-constexpr int SEG_NCL = 8;  // segment classes of the segmented rebuild
+constexpr int SEG_NCL = 10;  // segment classes of the segmented rebuild
 
 struct RebuildScratch {
   void *k0 = nullptr, *k1 = nullptr;  // 2E words each: segment keys and weights
@@ -55,6 +55,7 @@ struct RebuildScratch {
   uint32_t *su = nullptr;             // 2 Bmax: runs per segment
   uint32_t *vrel = nullptr;           // 2N: each vertex's slice start inside its row / column segment
   uint32_t *sseg = nullptr;           // 2E: segment of every segment entry
+  uint16_t *lab16 = nullptr;          // N: 16-bit labels when B <= 65535
   uint32_t *suoff = nullptr;          // 2 Bmax + 1: offsets of the runs
   uint32_t *lists = nullptr;          // SEG_NCL lists of lcap segment ids
   int64_t lcap = 0;
@@ -98,7 +99,7 @@ inline int64_t big_words(int64_t no, int64_t ni) {
          2 * (no + 2 + ni);
 }
 constexpr int64_t BIG_LDS_CAP = 40 * 1024;
-void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const int32_t *sof,
+void launch_nodal(const GraphDev &g, LabView lab, const BMView &m, const int32_t *sof,
                   const uint8_t *coin, RngKey key, PhaseRef phase, int64_t lo, int64_t BS,
                   const int32_t *const lists[NCLS], const int32_t counts[NCLS], uint32_t big_lds,
                   int32_t *prop, double *dS, float *err, uint8_t *near, hipStream_t st);

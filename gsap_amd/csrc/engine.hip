@@ -150,7 +150,7 @@ struct gsap_ctx {
   // block matrix
   DBuf rowptr, colptr, col, val, erow, row, cval, dout, din, ldout, ldin, dense;
   // rebuild scratch
-  DBuf k0, k1, tmp, partials, soff, su, suoff, lists, ncl, cntc, vrel, sseg;
+  DBuf k0, k1, tmp, partials, soff, su, suoff, lists, ncl, cntc, vrel, sseg, lab16;
   // sweep / merge
   DBuf OUT, sof, coin, prop, dS, best, dSbest, map, terms, tiles, res;
   // device batch counter (Philox phase of the sweep's first batch); merge-order sort
@@ -375,12 +375,14 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   // partial planes of the small-B histogram (up to 1024 planes of at most 128 x 128
   // counters) and of the row-tile histogram (up to 8M counters), then the transpose
   const int64_t sb = std::min<int64_t>(dB, 128);
-  const int64_t pc = std::max<int64_t>(1024 * sb * sb, 1);
+  const int64_t pc = std::max<int64_t>(
+      1024 * sb * sb, dB > 128 ? std::min<int64_t>(E / 2, (int64_t)32 << 20) : 0);
   c->partials.ensure(4 * pc);
   c->soff.ensure(4 * (2 * N + 1));
   c->su.ensure(4 * 2 * N); c->suoff.ensure(4 * (2 * N + 1)); c->vrel.ensure(4 * 2 * N);
   c->sseg.ensure(8 * E);
-  c->lists.ensure(4 * (size_t)SEG_NCL * 2 * N); c->ncl.ensure(4 * 8);
+  c->lab16.ensure(2 * N);
+  c->lists.ensure(4 * (size_t)SEG_NCL * 2 * N); c->ncl.ensure(4 * 16);
   c->cntc.ensure(4 * (size_t)std::max<int64_t>(seg_cntc_words(N, E), 1));
   c->OUT.ensure(8 * N); c->sof.ensure(4 * N); c->coin.ensure(N);
   c->prop.ensure(4 * (size_t)np * N); c->dS.ensure(8 * (size_t)np * N);
@@ -401,7 +403,7 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   r.k0 = c->k0.p; r.k1 = c->k1.p; r.tmp = c->tmp.as<uint32_t>();
   r.partials = c->partials.as<int32_t>(); r.partials_cap = pc;
   r.soff = c->soff.as<uint32_t>();
-  r.su = c->su.as<uint32_t>(); r.suoff = c->suoff.as<uint32_t>(); r.vrel = c->vrel.as<uint32_t>(); r.sseg = c->sseg.as<uint32_t>();
+  r.su = c->su.as<uint32_t>(); r.suoff = c->suoff.as<uint32_t>(); r.vrel = c->vrel.as<uint32_t>(); r.sseg = c->sseg.as<uint32_t>(); r.lab16 = c->lab16.as<uint16_t>();
   r.lists = c->lists.as<uint32_t>(); r.lcap = 2 * N; r.ncl = c->ncl.as<uint32_t>();
   r.cntc = c->cntc.as<uint32_t>();
 }
@@ -566,7 +568,9 @@ void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, PhaseRef phase, 
   const bool mixed = o.precision == GSAP_PRECISION_MIXED;
   float *err = mixed ? c->errv.as<float>() : nullptr;
   uint8_t *evcls = mixed ? c->evcls.as<uint8_t>() : nullptr;
-  launch_nodal(c->g, c->lab.as<int32_t>(), m, c->sof.as<int32_t>(), c->coin.as<uint8_t>(),
+  // the labels as the last rebuild saw them (it made the 16-bit copy when B <= 65535)
+  const LabView lv{c->lab.as<int32_t>(), c->bm.B <= 65535 ? c->rs.lab16 : nullptr};
+  launch_nodal(c->g, lv, m, c->sof.as<int32_t>(), c->coin.as<uint8_t>(),
                rng_key(o), phase, lo, BS, lists, counts, c->big_lds[b], c->prop.as<int32_t>(),
                c->dS.as<double>(), err, evcls, st);
   if (mixed) {
@@ -944,7 +948,7 @@ int gsap_destroy(gsap_ctx *c) {
                  &c->snap[1], &c->snap[2], &c->rowptr, &c->colptr, &c->col, &c->val, &c->erow,
                  &c->row, &c->cval, &c->dout, &c->din, &c->ldout, &c->ldin, &c->dense, &c->k0,
                  &c->k1, &c->tmp, &c->partials, &c->soff, &c->su, &c->suoff,
-                 &c->lists, &c->ncl, &c->cntc, &c->vrel, &c->sseg, &c->vlist,
+                 &c->lists, &c->ncl, &c->cntc, &c->vrel, &c->sseg, &c->lab16, &c->vlist,
                  &c->OUT, &c->sof, &c->coin, &c->prop, &c->dS, &c->best, &c->dSbest, &c->map,
                  &c->terms, &c->tiles, &c->res, &c->ctr, &c->mk0, &c->mk1, &c->mv0,
                  &c->mv1, &c->errv, &c->evcls, &c->mixstat};

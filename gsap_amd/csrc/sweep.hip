@@ -172,7 +172,8 @@ constexpr int LBITS = 8, LCAP = 1 << LBITS;
 
 struct NodalArgs {
   int64_t lo, BS;
-  const int32_t *optr, *oadj, *ow, *iptr, *iadj, *iw, *kout, *kin, *lab;
+  const int32_t *optr, *oadj, *ow, *iptr, *iadj, *iw, *kout, *kin;
+  LabView lab;
   BMView m;
   const int32_t *sof;
   const uint8_t *coin;
@@ -914,7 +915,7 @@ __global__ __launch_bounds__(256) void k_nodal_fused(NodalArgs a, NodalLists L) 
   nodal_small<16, MIX>(a, L.list[0], L.n[0], b);
 }
 
-void launch_nodal(const GraphDev &g, const int32_t *lab, const BMView &m, const int32_t *sof,
+void launch_nodal(const GraphDev &g, LabView lab, const BMView &m, const int32_t *sof,
                   const uint8_t *coin, RngKey key, PhaseRef phase, int64_t lo, int64_t BS,
                   const int32_t *const lists[NCLS], const int32_t counts[NCLS], uint32_t big_lds,
                   int32_t *prop, double *dS, float *err, uint8_t *near, hipStream_t st) {
