@@ -337,6 +337,7 @@ def main():
         if kn:
             traffic = kn["read_bytes_per_call"] + kn["write_bytes_per_call"]
     p, r = gsap_amd.pairwise(g.truth, lab)
+    qual = gsap_amd.quality(g.truth, lab)
     out = {
         "metric": "nodal-move edges/s (full partition of the 200k-vertex graph); full-partition "
                   "seconds and % HBM peak reported alongside",
@@ -358,6 +359,8 @@ def main():
         "profiled_partition_s": float(np.mean(pwalls)),
         "final_B": rep.B, "final_S": rep.S, "sweeps": rep.sweeps, "levels": rep.levels,
         "pairwise_precision": p, "pairwise_recall": r,
+        "quality": {k: qual[k] for k in ("accuracy", "nmi", "adjusted_rand_index", "B_truth",
+                                         "B_given")},
         "t_merge_s": float(np.mean([x.t_merge for x in reps])),
         "t_sweep_s": float(np.mean([x.t_sweep for x in reps])),
         "t_upload_s": float(np.mean([x.t_upload for x in reps])),

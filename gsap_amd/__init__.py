@@ -66,6 +66,13 @@ class CReport(C.Structure):
                 ("t_upload", C.c_double)]
 
 
+class CQuality(C.Structure):
+    _fields_ = [("n", C.c_int64), ("B_truth", C.c_int32), ("B_given", C.c_int32),
+                ("accuracy", C.c_double), ("precision", C.c_double), ("recall", C.c_double),
+                ("rand_index", C.c_double), ("adjusted_rand_index", C.c_double),
+                ("nmi", C.c_double), ("accuracy_computed", C.c_int32), ("reserved", C.c_int32)]
+
+
 class CTrace(C.Structure):
     _fields_ = [("kind", C.c_int32), ("B", C.c_int32), ("a", C.c_int32), ("b", C.c_int32),
                 ("h1", C.c_uint64), ("h2", C.c_uint64), ("x", C.c_double), ("y", C.c_double)]
@@ -82,7 +89,7 @@ EXPORTS = [
     "gsap_partition_i32", "gsap_get_trace", "gsap_blockmodel_i32", "gsap_merge_phase_i32",
     "gsap_nodal_batch_i32", "gsap_mdl_i32", "gsap_generate_dcsbm_i32", "gsap_read_tsv_i32",
     "gsap_pairwise_i32", "gsap_log2", "gsap_log2_tables", "gsap_partition_batch_i32",
-    "gsap_partition_chains_i32", "gsap_log2f_n",
+    "gsap_partition_chains_i32", "gsap_log2f_n", "gsap_quality_i32",
 ]
 
 _lib = None
@@ -125,6 +132,7 @@ def lib() -> C.CDLL:
                                     vp, vp]
     L.gsap_pairwise_i32.argtypes = [i64, _i32p, _i32p, C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]
+    L.gsap_quality_i32.argtypes = [i64, _i32p, _i32p, C.POINTER(CQuality)]
     L.gsap_log2.argtypes = [C.c_double]
     L.gsap_log2.restype = C.c_double
     L.gsap_log2_tables.argtypes = [_f64p, _f64p, _f64p]
@@ -398,6 +406,20 @@ def pairwise(truth: np.ndarray, given: np.ndarray):
     g = np.ascontiguousarray(given, np.int32)
     _check(lib().gsap_pairwise_i32(len(t), t, g, C.byref(p), C.byref(r)))
     return p.value, r.value
+
+
+def quality(truth: np.ndarray, given: np.ndarray) -> dict:
+    """Full quality report (GSAP/evaluate/evaluate.hpp:19-265): optimal-matching accuracy,
+    pairwise precision / recall, Rand and adjusted Rand index, NMI; accuracy is None when
+    the contingency table is too large to match (more than 64M cells)."""
+    q = CQuality()
+    t = np.ascontiguousarray(truth, np.int32)
+    g = np.ascontiguousarray(given, np.int32)
+    _check(lib().gsap_quality_i32(len(t), t, g, C.byref(q)))
+    return {"n": q.n, "B_truth": q.B_truth, "B_given": q.B_given,
+            "accuracy": q.accuracy if q.accuracy_computed else None,
+            "precision": q.precision, "recall": q.recall, "rand_index": q.rand_index,
+            "adjusted_rand_index": q.adjusted_rand_index, "nmi": q.nmi}
 
 
 def log2(x: float) -> float:

@@ -2,6 +2,7 @@
 // GraphChallenge TSV reader (GSAP/load.hpp:6-133), pairwise precision / recall
 // (GSAP/evaluate/evaluate.hpp:130-181) and the canonical log2 on the host.
 #include <algorithm>
+#include <limits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -370,6 +371,122 @@ int gsap_pairwise_i32(int64_t N, const int32_t *truth, const int32_t *given, dou
   for (int64_t x : b) sb += x * (x - 1) / 2;
   *recall = sa ? (double)same / (double)sa : 0.0;
   *precision = sb ? (double)same / (double)sb : 0.0;
+  return GSAP_OK;
+}
+
+
+// Full quality report of a partition against a truth labelling (the metrics of
+// bf::evaluate, GSAP/evaluate/evaluate.hpp:19-265, with int64 pair counts and FP64 sums).
+// Vertices whose truth is negative are left out (evaluate.hpp:43-48).  The accuracy is the
+// largest total of a one-to-one matching between truth and given blocks in the contingency
+// table (Kuhn-Munkres, evaluate.hpp:69-96), divided by the number of vertices; it is
+// computed when the smaller side times the larger side stays within 64M cells.
+int gsap_quality_i32(int64_t N, const int32_t *truth, const int32_t *given, gsap_quality *q) {
+  if (N <= 0 || !truth || !given || !q) return GSAP_ERR_INVALID;
+  int32_t Bt = 0, Bg = 0;
+  for (int64_t i = 0; i < N; ++i) {
+    if (given[i] < 0) return GSAP_ERR_INVALID;
+    Bt = std::max(Bt, truth[i] + 1);
+    Bg = std::max(Bg, given[i] + 1);
+  }
+  // contingency table as sorted (truth, given) pairs with counts
+  std::vector<uint64_t> key;
+  key.reserve(N);
+  for (int64_t i = 0; i < N; ++i)
+    if (truth[i] >= 0) key.push_back(((uint64_t)truth[i] << 32) | (uint32_t)given[i]);
+  std::sort(key.begin(), key.end());
+  const int64_t n = (int64_t)key.size();
+  struct Cell { int32_t t, g; int64_t c; };
+  std::vector<Cell> cells;
+  for (size_t i = 0; i < key.size();) {
+    size_t j = i;
+    while (j < key.size() && key[j] == key[i]) ++j;
+    cells.push_back(Cell{(int32_t)(key[i] >> 32), (int32_t)(uint32_t)key[i], (int64_t)(j - i)});
+    i = j;
+  }
+  std::vector<int64_t> rs(std::max(Bt, 1), 0), cs(std::max(Bg, 1), 0);
+  for (const Cell &c : cells) { rs[c.t] += c.c; cs[c.g] += c.c; }
+  std::memset(q, 0, sizeof(*q));
+  q->n = n;
+  int32_t nonempty_t = 0;
+  for (int64_t x : rs) nonempty_t += x > 0;
+  q->B_truth = nonempty_t;
+  q->B_given = Bg;
+  if (n == 0) return GSAP_ERR_INVALID;
+  // pair counts
+  auto ch2 = [](int64_t x) { return x * (x - 1) / 2; };
+  int64_t same = 0, sa = 0, sb = 0;
+  for (const Cell &c : cells) same += ch2(c.c);
+  for (int64_t x : rs) sa += ch2(x);
+  for (int64_t x : cs) sb += ch2(x);
+  const double pairs = (double)ch2(n);
+  q->recall = sa ? (double)same / (double)sa : 0.0;
+  q->precision = sb ? (double)same / (double)sb : 0.0;
+  // agreements: pairs together in both + pairs apart in both
+  const double apart_both = pairs - (double)sa - (double)sb + (double)same;
+  q->rand_index = pairs > 0 ? ((double)same + apart_both) / pairs : 1.0;
+  const double expct = pairs > 0 ? (double)sa * (double)sb / pairs : 0.0;
+  const double den = 0.5 * ((double)sa + (double)sb) - expct;
+  q->adjusted_rand_index = den != 0.0 ? ((double)same - expct) / den : 1.0;
+  // information-theoretic: entropies and mutual information in nats
+  double H1 = 0.0, H2 = 0.0, MI = 0.0;
+  const double dn = (double)n;
+  for (int64_t x : rs) if (x > 0) { const double p = (double)x / dn; H1 -= p * std::log(p); }
+  for (int64_t x : cs) if (x > 0) { const double p = (double)x / dn; H2 -= p * std::log(p); }
+  for (const Cell &c : cells) {
+    const double pij = (double)c.c / dn;
+    MI += pij * std::log(pij / (((double)rs[c.t] / dn) * ((double)cs[c.g] / dn)));
+  }
+  q->nmi = 2.0 * std::fabs(MI) / (std::fabs(H1) + std::fabs(H2));
+  // optimal one-to-one matching (Hungarian algorithm with row and column potentials on the
+  // rectangular table, rows = the smaller side; cost = -count, so the minimum-cost
+  // assignment maximises the matched count)
+  const bool tr = Bt > Bg;
+  const int64_t R = tr ? Bg : Bt, Cn = tr ? Bt : Bg;
+  if (R >= 1 && R * Cn <= ((int64_t)64 << 20)) {
+    std::vector<int64_t> cost((size_t)(R + 1) * (Cn + 1), 0);
+    for (const Cell &c : cells) {
+      const int64_t r = tr ? c.g : c.t, k = tr ? c.t : c.g;
+      cost[(size_t)(r + 1) * (Cn + 1) + (k + 1)] = -c.c;
+    }
+    const int64_t INF = std::numeric_limits<int64_t>::max() / 4;
+    std::vector<int64_t> u(R + 1, 0), v(Cn + 1, 0), minv(Cn + 1);
+    std::vector<int64_t> owner(Cn + 1, 0), prev(Cn + 1, 0);  // column -> row, path links
+    std::vector<char> seen(Cn + 1);
+    for (int64_t r = 1; r <= R; ++r) {
+      owner[0] = r;
+      int64_t col = 0;
+      std::fill(minv.begin(), minv.end(), INF);
+      std::fill(seen.begin(), seen.end(), 0);
+      do {
+        seen[col] = 1;
+        const int64_t row = owner[col];
+        int64_t delta = INF, next = 0;
+        const int64_t *crow = &cost[(size_t)row * (Cn + 1)];
+        for (int64_t k = 1; k <= Cn; ++k) {
+          if (seen[k]) continue;
+          const int64_t red = crow[k] - u[row] - v[k];
+          if (red < minv[k]) { minv[k] = red; prev[k] = col; }
+          if (minv[k] < delta) { delta = minv[k]; next = k; }
+        }
+        for (int64_t k = 0; k <= Cn; ++k) {
+          if (seen[k]) { u[owner[k]] += delta; v[k] -= delta; }
+          else minv[k] -= delta;
+        }
+        col = next;
+      } while (owner[col] != 0);
+      do {  // flip the augmenting path
+        const int64_t back = prev[col];
+        owner[col] = owner[back];
+        col = back;
+      } while (col != 0);
+    }
+    int64_t matched = 0;
+    for (int64_t k = 1; k <= Cn; ++k)
+      if (owner[k] != 0) matched -= cost[(size_t)owner[k] * (Cn + 1) + k];
+    q->accuracy = (double)matched / dn;
+    q->accuracy_computed = 1;
+  }
   return GSAP_OK;
 }
 

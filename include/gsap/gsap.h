@@ -17,6 +17,8 @@
  *   gsap_mdl_i32            <- compute_overall_entropy_cuda, GSAP/partition.hpp:665-704
  *   gsap_read_tsv_i32       <- readFile, GSAP/load.hpp:6-133 (whitespace tolerant)
  *   gsap_pairwise_i32       <- bf::evaluate pair counts, GSAP/evaluate/evaluate.hpp:130-181
+ *   gsap_quality_i32        <- bf::evaluate (accuracy, pair counts, NMI),
+ *                              GSAP/evaluate/evaluate.hpp:19-265, km.hpp:19-115
  *   gsap_generate_dcsbm_i32 -- no counterpart: the seeded DC-SBM generator that
  *                              produces the benchmark graphs.
  */
@@ -211,6 +213,27 @@ int gsap_read_tsv_i32(const char *path, int64_t *N, int64_t *E, int32_t *out_ptr
 /* Pairwise precision / recall against a truth labelling (int64 counts). */
 int gsap_pairwise_i32(int64_t N, const int32_t *truth, const int32_t *given, double *precision,
                       double *recall);
+
+/* Full quality report against a truth labelling (the metrics of bf::evaluate,
+ * GSAP/evaluate/evaluate.hpp:19-265, with int64 pair counts and FP64 sums; vertices
+ * whose truth is negative are left out).  accuracy is the best one-to-one matching of
+ * truth and given blocks (Kuhn-Munkres) over n; computed (accuracy_computed = 1) when the
+ * contingency table has at most 64M cells.  nmi = 2 |MI| / (|H_truth| + |H_given|). */
+typedef struct gsap_quality {
+  int64_t n;                 /* vertices with a truth label */
+  int32_t B_truth;           /* nonempty truth blocks */
+  int32_t B_given;           /* max given label + 1 */
+  double accuracy;
+  double precision;          /* pairwise */
+  double recall;             /* pairwise */
+  double rand_index;
+  double adjusted_rand_index;
+  double nmi;
+  int32_t accuracy_computed;
+  int32_t reserved;
+} gsap_quality;
+
+int gsap_quality_i32(int64_t N, const int32_t *truth, const int32_t *given, gsap_quality *q);
 
 /* Canonical log2 used by every FP64 delta-S / MDL term (host evaluation). */
 double gsap_log2(double x);
