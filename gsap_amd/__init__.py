@@ -89,7 +89,8 @@ EXPORTS = [
     "gsap_partition_i32", "gsap_get_trace", "gsap_blockmodel_i32", "gsap_merge_phase_i32",
     "gsap_nodal_batch_i32", "gsap_mdl_i32", "gsap_generate_dcsbm_i32", "gsap_read_tsv_i32",
     "gsap_pairwise_i32", "gsap_log2", "gsap_log2_tables", "gsap_partition_batch_i32",
-    "gsap_partition_chains_i32", "gsap_log2f_n", "gsap_quality_i32",
+    "gsap_partition_chains_i32", "gsap_log2f_n", "gsap_quality_i32", "gsap_save_csr_i32",
+    "gsap_load_csr_i32",
 ]
 
 _lib = None
@@ -133,6 +134,9 @@ def lib() -> C.CDLL:
     L.gsap_pairwise_i32.argtypes = [i64, _i32p, _i32p, C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]
     L.gsap_quality_i32.argtypes = [i64, _i32p, _i32p, C.POINTER(CQuality)]
+    L.gsap_save_csr_i32.argtypes = [C.c_char_p, C.POINTER(CGraph), vp]
+    L.gsap_load_csr_i32.argtypes = [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_int32)] + [vp] * 7
     L.gsap_log2.argtypes = [C.c_double]
     L.gsap_log2.restype = C.c_double
     L.gsap_log2_tables.argtypes = [_f64p, _f64p, _f64p]
@@ -396,6 +400,33 @@ def read_tsv(path: str, truth_path: Optional[str] = None) -> Graph:
         truth = np.full(n, -1, np.int32)
         truth[t[:, 0] - 1] = t[:, 1] - 1
         g.truth = truth
+    return g
+
+
+def save_csr(g: Graph, path: str) -> None:
+    """Binary CSR cache of the graph (and its truth labelling, when it has one)."""
+    cg = g.cstruct()
+    t = None
+    if g.truth is not None:
+        t = np.ascontiguousarray(g.truth, np.int32)
+    _check(lib().gsap_save_csr_i32(path.encode(), C.byref(cg),
+                                   t.ctypes.data if t is not None else None))
+
+
+def load_csr(path: str) -> Graph:
+    """Graph written by save_csr."""
+    N, E, ht = C.c_int64(), C.c_int64(), C.c_int32()
+    _check(lib().gsap_load_csr_i32(path.encode(), C.byref(N), C.byref(E), C.byref(ht),
+                                   *([None] * 7)))
+    n, e = N.value, E.value
+    arrs = [np.empty(n + 1, np.int32), np.empty(e, np.int32), np.empty(e, np.int32),
+            np.empty(n + 1, np.int32), np.empty(e, np.int32), np.empty(e, np.int32)]
+    truth = np.empty(n, np.int32) if ht.value else None
+    _check(lib().gsap_load_csr_i32(path.encode(), C.byref(N), C.byref(E), C.byref(ht),
+                                   *[a.ctypes.data for a in arrs],
+                                   truth.ctypes.data if truth is not None else None))
+    g = Graph(n, *arrs)
+    g.truth = truth
     return g
 
 

@@ -490,4 +490,79 @@ int gsap_quality_i32(int64_t N, const int32_t *truth, const int32_t *given, gsap
   return GSAP_OK;
 }
 
+
+// ---- binary CSR cache: "GSAPCSR1", flags (bit 0: truth present), N, E, then int32 arrays
+// out_ptr[N+1], out_adj[E], out_w[E], in_ptr[N+1], in_adj[E], in_w[E], [truth[N]], then a
+// 64-bit checksum of the arrays (little-endian host layout).
+namespace {
+const char kCsrMagic[8] = {'G', 'S', 'A', 'P', 'C', 'S', 'R', '1'};
+
+uint64_t csr_checksum(uint64_t h, const int32_t *p, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) {
+    h ^= (uint32_t)p[i];
+    h *= 0x100000001B3ull;
+    h ^= h >> 29;
+  }
+  return h;
+}
+}  // namespace
+
+int gsap_save_csr_i32(const char *path, const gsap_graph_i32 *g, const int32_t *truth) {
+  if (!path || !g || g->N <= 0 || g->E < 0 || !g->out_ptr || !g->in_ptr) return GSAP_ERR_INVALID;
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return GSAP_ERR_IO;
+  const uint32_t flags = truth ? 1u : 0u, pad = 0;
+  const int64_t N = g->N, E = g->E;
+  const int32_t *arr[7] = {g->out_ptr, g->out_adj, g->out_w, g->in_ptr, g->in_adj, g->in_w, truth};
+  const int64_t len[7] = {N + 1, E, E, N + 1, E, E, N};
+  bool ok = std::fwrite(kCsrMagic, 1, 8, f) == 8 && std::fwrite(&flags, 4, 1, f) == 1 &&
+            std::fwrite(&pad, 4, 1, f) == 1 && std::fwrite(&N, 8, 1, f) == 1 &&
+            std::fwrite(&E, 8, 1, f) == 1;
+  uint64_t h = 0xCBF29CE484222325ull;
+  for (int k = 0; k < 7 && ok; ++k) {
+    if (k == 6 && !truth) break;
+    if (len[k] > 0 && !arr[k]) { ok = false; break; }
+    ok = std::fwrite(arr[k], 4, (size_t)len[k], f) == (size_t)len[k];
+    h = csr_checksum(h, arr[k], len[k]);
+  }
+  ok = ok && std::fwrite(&h, 8, 1, f) == 1;
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? GSAP_OK : GSAP_ERR_IO;
+}
+
+int gsap_load_csr_i32(const char *path, int64_t *N_out, int64_t *E_out, int32_t *has_truth,
+                      int32_t *out_ptr, int32_t *out_adj, int32_t *out_w, int32_t *in_ptr,
+                      int32_t *in_adj, int32_t *in_w, int32_t *truth) {
+  if (!path || !N_out || !E_out || !has_truth) return GSAP_ERR_INVALID;
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return GSAP_ERR_IO;
+  char magic[8];
+  uint32_t flags = 0, pad = 0;
+  int64_t N = 0, E = 0;
+  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kCsrMagic, 8) == 0 &&
+            std::fread(&flags, 4, 1, f) == 1 && std::fread(&pad, 4, 1, f) == 1 &&
+            std::fread(&N, 8, 1, f) == 1 && std::fread(&E, 8, 1, f) == 1 && N > 0 && E >= 0 &&
+            N < ((int64_t)1 << 31) && E < ((int64_t)1 << 31);
+  if (!ok) { std::fclose(f); return GSAP_ERR_IO; }
+  *N_out = N;
+  *E_out = E;
+  *has_truth = (flags & 1u) ? 1 : 0;
+  if (!out_ptr) { std::fclose(f); return GSAP_OK; }  // sizes only
+  if (!out_adj || !out_w || !in_ptr || !in_adj || !in_w || (*has_truth && !truth)) {
+    std::fclose(f);
+    return GSAP_ERR_INVALID;
+  }
+  int32_t *arr[7] = {out_ptr, out_adj, out_w, in_ptr, in_adj, in_w, truth};
+  const int64_t len[7] = {N + 1, E, E, N + 1, E, E, N};
+  uint64_t h = 0xCBF29CE484222325ull, stored = 0;
+  for (int k = 0; k < 7 && ok; ++k) {
+    if (k == 6 && !*has_truth) break;
+    ok = std::fread(arr[k], 4, (size_t)len[k], f) == (size_t)len[k];
+    if (ok) h = csr_checksum(h, arr[k], len[k]);
+  }
+  ok = ok && std::fread(&stored, 8, 1, f) == 1 && stored == h;
+  std::fclose(f);
+  return ok ? GSAP_OK : GSAP_ERR_IO;
+}
+
 }  // extern "C"

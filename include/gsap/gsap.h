@@ -210,6 +210,16 @@ int gsap_generate_dcsbm_i32(const gsap_dcsbm_params *p, int64_t *E, int32_t *out
 int gsap_read_tsv_i32(const char *path, int64_t *N, int64_t *E, int32_t *out_ptr, int32_t *out_adj,
                       int32_t *out_w, int32_t *in_ptr, int32_t *in_adj, int32_t *in_w);
 
+/* Binary CSR cache: the graph's two CSR views (and optionally a truth labelling) with a
+ * checksum, so large graphs reload without parsing or regeneration.  Load is two-phase:
+ * first with out_ptr NULL (N, E and has_truth only), then with arrays of the right sizes
+ * (truth[N] needed when has_truth).  GSAP_ERR_IO for a missing, truncated, foreign or
+ * corrupted file.  No counterpart in the reference. */
+int gsap_save_csr_i32(const char *path, const gsap_graph_i32 *g, const int32_t *truth);
+int gsap_load_csr_i32(const char *path, int64_t *N, int64_t *E, int32_t *has_truth,
+                      int32_t *out_ptr, int32_t *out_adj, int32_t *out_w, int32_t *in_ptr,
+                      int32_t *in_adj, int32_t *in_w, int32_t *truth);
+
 /* Pairwise precision / recall against a truth labelling (int64 counts). */
 int gsap_pairwise_i32(int64_t N, const int32_t *truth, const int32_t *given, double *precision,
                       double *recall);

@@ -99,3 +99,38 @@ def test_fp32_log2_error_bound():
     # powers of two are exact
     p = 2.0 ** np.arange(0, 31)
     assert np.array_equal(gsap_amd.log2f(p.astype(np.float32)), np.arange(0, 31, dtype=np.float32))
+
+
+def test_csr_cache_round_trip(tmp_path):
+    """Binary CSR cache: a generated graph and a TSV graph with truth come back identical;
+    a flipped byte, a truncated file, a foreign file and a missing file are errors."""
+    g = gsap_amd.generate_dcsbm(3000, 9, 1.0, 2.0, 15.0, seed=11)
+    p = str(tmp_path / "g.csr")
+    gsap_amd.save_csr(g, p)
+    h = gsap_amd.load_csr(p)
+    assert h.N == g.N and h.E == g.E
+    for a in ("out_ptr", "out_adj", "out_w", "in_ptr", "in_adj", "in_w", "truth"):
+        assert np.array_equal(getattr(h, a), getattr(g, a)), a
+    t = tsv_graph("LH")
+    q = str(tmp_path / "t.csr")
+    gsap_amd.save_csr(t, q)
+    u = gsap_amd.load_csr(q)
+    assert np.array_equal(u.truth, t.truth) and np.array_equal(u.in_adj, t.in_adj)
+    # graph without truth
+    t.truth = None
+    gsap_amd.save_csr(t, q)
+    assert gsap_amd.load_csr(q).truth is None
+    raw = bytearray(open(p, "rb").read())
+    bad = str(tmp_path / "bad.csr")
+    raw2 = bytearray(raw); raw2[len(raw2) // 2] ^= 0x40
+    open(bad, "wb").write(raw2)
+    with pytest.raises(gsap_amd.GsapError):
+        gsap_amd.load_csr(bad)
+    open(bad, "wb").write(raw[:len(raw) - 100])
+    with pytest.raises(gsap_amd.GsapError):
+        gsap_amd.load_csr(bad)
+    open(bad, "wb").write(b"not a graph file at all, just text" * 4)
+    with pytest.raises(gsap_amd.GsapError):
+        gsap_amd.load_csr(bad)
+    with pytest.raises(gsap_amd.GsapError):
+        gsap_amd.load_csr(str(tmp_path / "missing.csr"))
