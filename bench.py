@@ -211,7 +211,7 @@ def bench_multi(a, prec):
                 f"seed 5; lowest S kept")
     t_gen = time.perf_counter() - t0
     ctx = gsap_amd.Context(0)
-    opts = gsap_amd.Options(seed=1, precision=prec, workers=a.workers)
+    opts = gsap_amd.Options(seed=1, precision=prec, workers=a.workers, graph=int(a.graph))
     walls, edges = [], []
     for k in range(a.warmup + a.steps):
         t1 = time.perf_counter()
@@ -237,7 +237,8 @@ def bench_multi(a, prec):
            "warmup": a.warmup, "ms_per_step": 1e3 * float(np.mean(walls)),
            "higher_is_better": a.config == "c4", "scaling": "weak", "vs_baseline": None,
            "dtype": "f64", "data": "synthetic (seeded DC-SBM generator)",
-           "config": {"workload": desc, "workers": a.workers or 8},
+           "config": {"workload": desc, "workers": a.workers or 8,
+                      "sweep_launch": "hip graph replay" if a.graph else "kernel by kernel"},
            "graph_generation_s": t_gen}
     out.update(extra)
     print(json.dumps(out))
@@ -253,6 +254,8 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the FP64-vs-mixed comparison run")
     ap.add_argument("--workers", type=int, default=0)
+    ap.add_argument("--graph", action="store_true",
+                    help="replay each level's captured sweep as a HIP graph")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -62,6 +62,40 @@ GSAP_D int32_t wave_pick(const int32_t *w1, const int32_t *id1, int32_t n1, cons
   return id1 ? id1[0] : id2[0];  // unreachable when target < total weight
 }
 
+// The same pick by a whole 256-thread workgroup: thread t sums a contiguous run of
+// ceil(n / 256) entries, a block scan of the run sums finds the run holding the target, and
+// that run's thread finishes the search.  A constant number of dependent memory rounds
+// instead of one per 64 entries.  sc: 5 words of LDS.  Every thread returns the id.
+GSAP_D int32_t block_pick(const int32_t *w1, const int32_t *id1, int32_t n1, const int32_t *w2,
+                          const int32_t *id2, int32_t n2, uint32_t target, uint32_t *sc) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int32_t n = n1 + n2;
+  const int32_t K = (n + 255) / 256;
+  const int32_t a0 = tid * K, a1 = a0 + K < n ? a0 + K : n;
+  uint32_t sum = 0;
+  for (int32_t k = a0; k < a1; ++k) sum += (uint32_t)(k < n1 ? w1[k] : w2[k - n1]);
+  uint32_t incl = wave_incl_scan_u32(sum);
+  if (lane == 63) sc[w] = incl;
+  if (tid == 0) sc[4] = (uint32_t)(n1 > 0 ? id1[0] : id2[0]);  // unreachable fallback
+  __syncthreads();
+  uint32_t pre = 0;
+  for (int i = 0; i < w; ++i) pre += sc[i];
+  const uint32_t ex = pre + incl - sum;
+  __syncthreads();
+  if (ex <= target && target - ex < sum) {  // the one run where the running weight crosses
+    uint32_t run = ex;
+    for (int32_t k = a0; k < a1; ++k) {
+      run += (uint32_t)(k < n1 ? w1[k] : w2[k - n1]);
+      if (run > target) {
+        sc[4] = (uint32_t)(k < n1 ? id1[k] : id2[k - n1]);
+        break;
+      }
+    }
+  }
+  __syncthreads();
+  return (int32_t)sc[4];
+}
+
 template <int G, class F>
 GSAP_D void seg_for(int32_t n, F &&f) {
   const int q = threadIdx.x & (G - 1);
@@ -468,6 +502,7 @@ template <bool MIX>
 GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds_bytes) {
   __shared__ int64_t red[4][4];
   __shared__ int64_t corner_s[4];
+  __shared__ uint32_t pick_sc[8];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int64_t i = v - a.lo;
   const BMView &m = a.m;
@@ -478,12 +513,12 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
   const int32_t i0 = a.iptr[v], ni = a.iptr[v + 1] - i0;
   const int64_t kout = a.kout[v], kin = a.kin[v];
   const int64_t tot = kout + kin;
-  int32_t s;  // every wave computes the same proposal
+  int32_t s;  // every thread computes the same proposal
   if (tot == 0) {
     s = (int32_t)mulhi(x.x0, B);
   } else {
-    const int32_t nbr = wave_pick(a.ow + o0, a.oadj + o0, no, a.iw + i0, a.iadj + i0, ni,
-                                  mulhi(x.x1, tot));
+    const int32_t nbr = block_pick(a.ow + o0, a.oadj + o0, no, a.iw + i0, a.iadj + i0, ni,
+                                   mulhi(x.x1, tot), pick_sc);
     const int32_t u = a.lab[nbr];
     s = a.coin[u] ? (int32_t)mulhi(x.x0, B) : a.sof[u];
   }
@@ -515,8 +550,26 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
     corner_s[tid] = bm_col_get(m, cc, rr);
   }
   __syncthreads();
-  for (int32_t j = tid; j < no; j += 256) ht_insert(ho, a.lab[a.oadj[o0 + j]], a.ow[o0 + j], j);
-  for (int32_t j = tid; j < ni; j += 256) ht_insert(hi, a.lab[a.iadj[i0 + j]], a.iw[i0 + j], j);
+  // four entries per thread in flight: ids and weights, then their blocks, then inserts
+  auto insert_all = [&](const HTab &h, const int32_t *adj, const int32_t *wt, int32_t n) {
+    for (int32_t j0 = tid; j0 < n; j0 += 4 * 256) {
+      int32_t t[4], w[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int32_t j = j0 + q * 256;
+        t[q] = j < n ? adj[j] : 0;
+        w[q] = j < n ? wt[j] : 0;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (j0 + q * 256 < n) t[q] = a.lab[t[q]];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (j0 + q * 256 < n) ht_insert(h, t[q], w[q], j0 + q * 256);
+    }
+  };
+  insert_all(ho, a.oadj + o0, a.ow + o0, no);
+  insert_all(hi, a.iadj + i0, a.iw + i0, ni);
   __syncthreads();
   const int64_t c_r = ht_weight(hi, r), c_s = ht_weight(hi, s);
   const bool r_in_a = ht_find(ho, r) >= 0, s_in_a = ht_find(ho, s) >= 0;
