@@ -52,7 +52,7 @@ class COptions(C.Structure):
                 ("trace", C.c_int32), ("dense_max_B", C.c_int32), ("profile", C.c_int32),
                 ("model", C.c_int32), ("workers", C.c_int32),
                 ("profile_classes", C.c_int32), ("graph", C.c_int32),
-                ("reserved", C.c_int32 * 3)]
+                ("tile_max_B", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 class CReport(C.Structure):
@@ -90,7 +90,7 @@ EXPORTS = [
     "gsap_nodal_batch_i32", "gsap_mdl_i32", "gsap_generate_dcsbm_i32", "gsap_read_tsv_i32",
     "gsap_pairwise_i32", "gsap_log2", "gsap_log2_tables", "gsap_partition_batch_i32",
     "gsap_partition_chains_i32", "gsap_log2f_n", "gsap_quality_i32", "gsap_save_csr_i32",
-    "gsap_load_csr_i32",
+    "gsap_load_csr_i32", "gsap_autotune_i32",
 ]
 
 _lib = None
@@ -134,6 +134,7 @@ def lib() -> C.CDLL:
     L.gsap_pairwise_i32.argtypes = [i64, _i32p, _i32p, C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]
     L.gsap_quality_i32.argtypes = [i64, _i32p, _i32p, C.POINTER(CQuality)]
+    L.gsap_autotune_i32.argtypes = [vp, C.POINTER(CGraph), C.POINTER(COptions)]
     L.gsap_save_csr_i32.argtypes = [C.c_char_p, C.POINTER(CGraph), vp]
     L.gsap_load_csr_i32.argtypes = [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                     C.POINTER(C.c_int32)] + [vp] * 7
@@ -211,13 +212,14 @@ class Options:
     model: int = 0  # 0: the reference's model term, 1: corrected (x ln x scaled by E)
     workers: int = 0  # batch / chain calls: concurrent sub-contexts (0 = 8)
     graph: int = 0  # 1: each level's sweep captured once and replayed as a HIP graph
+    tile_max_B: int = 0  # row-tile rebuild up to this B (0 = auto)
 
     def cstruct(self) -> COptions:
         o = COptions()
         for f in ("seed", "num_proposals", "reduction_rate", "max_nodal_itr", "num_batches",
                   "threshold1", "threshold2", "moving_avg_window", "max_levels", "precision",
                   "chain", "trace", "dense_max_B", "profile", "model", "workers",
-                  "profile_classes", "graph"):
+                  "profile_classes", "graph", "tile_max_B"):
             setattr(o, f, getattr(self, f))
         return o
 
@@ -289,6 +291,13 @@ class Context:
         _check(lib().gsap_partition_i32(self._h, C.byref(cg), C.byref(co), out, C.byref(rep)),
                self._h)
         return out, Report.from_c(rep)
+
+    def autotune(self, g: Graph, opts: Options = Options()) -> Options:
+        """Options with tile_max_B and dense_max_B measured on this graph and device."""
+        import dataclasses
+        cg, co = g.cstruct(), opts.cstruct()
+        _check(lib().gsap_autotune_i32(self._h, C.byref(cg), C.byref(co)), self._h)
+        return dataclasses.replace(opts, tile_max_B=co.tile_max_B, dense_max_B=co.dense_max_B)
 
     def partition_batch(self, graphs, opts: Options = Options()):
         """Batched form: independent graphs on concurrent sub-contexts."""

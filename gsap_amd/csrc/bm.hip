@@ -168,6 +168,7 @@ __global__ __launch_bounds__(1024) void k_small_bm(const int32_t *D, int32_t B, 
 // ------------------------------------------------------------------- tile path
 // ---- vertices grouped by block (counting sort; order inside a block is irrelevant)
 constexpr int VT = 2048;  // vertices per workgroup
+constexpr int TILE_MAX_B = 8192;  // largest B of the row-tile path
 
 __global__ __launch_bounds__(256) void k_label_hist(LabView lab, int64_t N, int32_t B,
                                                     uint32_t *H, int32_t W) {
@@ -996,10 +997,9 @@ static void seg_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
   k_logs<<<grid_for(B, 256), 256, 0, st>>>(bm.dout, bm.din, B, bm.ldout, bm.ldin);
 }
 
-// 128 < B <= TILE_B (dense): vertices grouped by block; workgroup (row tile, chunk) counts
+// 128 < B <= tile_max_B (dense): vertices grouped by block; workgroup (row tile, chunk) counts
 // the out-edges of its rows' vertices in LDS and writes a partial plane; the planes are
 // summed into D; CSR from D's rows, CSC from the rows of D's transpose.
-constexpr int TILE_B = 2048;
 static void tile_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
                       RebuildScratch &rs, hipStream_t st) {
   const int64_t N = g.N;
@@ -1065,10 +1065,17 @@ static void small_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
 }
 
 void rebuild_blockmodel(const GraphDev &g, const int32_t *lab32, int32_t B, BMBuf &bm,
-                        RebuildScratch &rs, int32_t dense_max_B, hipStream_t st) {
+                        RebuildScratch &rs, int32_t dense_max_B, int32_t tile_max_B,
+                        hipStream_t st) {
   const bool dense = B <= dense_max_B && (int64_t)B * B <= bm.dense_cap;
   const bool small = dense && B <= SMALL_B && (int64_t)B * B * 128 <= rs.partials_cap;
-  const bool tile = !small && dense && B <= TILE_B && (int64_t)B * B <= rs.partials_cap;
+  // the row-tile path: at most TILE_MAX_B blocks (a tile of 16384 / B rows, at least two,
+  // fills 64 KB of LDS); its vertex histogram (B x N / 2048 words) lives in the class lists
+  // and is scanned with the shared scan temporary
+  const int64_t hw = (int64_t)B * ((g.N + VT - 1) / VT);
+  const bool tile = !small && dense && B <= tile_max_B && B <= TILE_MAX_B &&
+                    (int64_t)B * B <= rs.partials_cap && hw <= SEG_NCL * rs.lcap &&
+                    (int64_t)scan_temp_words(hw) <= rs.tmp_words;
   // restore the all-zero dense buffer, unless the small or tile path overwrites the same
   // B x B region in full
   if (bm.dense_B > 0 && !((small || tile) && bm.dense_B == B)) {

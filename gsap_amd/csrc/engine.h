@@ -49,6 +49,7 @@ constexpr int SEG_NCL = 10;  // segment classes of the segmented rebuild
 struct RebuildScratch {
   void *k0 = nullptr, *k1 = nullptr;  // 2E words each: segment keys and weights
   uint32_t *tmp = nullptr;            // scan temporary
+  int64_t tmp_words = 0;
   int32_t *partials = nullptr;        // partial planes of the small-B histogram
   int64_t partials_cap = 0;           // words
   uint32_t *soff = nullptr;           // 2 Bmax + 1: segment sizes, then offsets
@@ -67,7 +68,8 @@ int64_t seg_cntc_words(int64_t N, int64_t E);
 
 // ---- bm.hip
 void rebuild_blockmodel(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf &bm,
-                        RebuildScratch &rs, int32_t dense_max_B, hipStream_t st);
+                        RebuildScratch &rs, int32_t dense_max_B, int32_t tile_max_B,
+                        hipStream_t st);
 
 // ---- sweep.hip
 struct RngKey {

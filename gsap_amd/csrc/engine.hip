@@ -166,7 +166,8 @@ struct gsap_ctx {
   BMBuf bm;
   RebuildScratch rs;
   int64_t N = 0, E = 0;
-  int32_t dense_max_B = 1024;
+  int32_t dense_max_B = 8192;
+  int32_t tile_max_B = 2048;
   // sub-contexts of batch / chain calls (one per worker thread)
   std::vector<gsap_ctx *> workers;
 };
@@ -401,6 +402,7 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   b.dense = c->dense.as<int32_t>(); b.dense_cap = (int64_t)(c->dense.cap / 4);
   RebuildScratch &r = c->rs;
   r.k0 = c->k0.p; r.k1 = c->k1.p; r.tmp = c->tmp.as<uint32_t>();
+  r.tmp_words = (int64_t)(c->tmp.cap / 4);
   r.partials = c->partials.as<int32_t>(); r.partials_cap = pc;
   r.soff = c->soff.as<uint32_t>();
   r.su = c->su.as<uint32_t>(); r.suoff = c->suoff.as<uint32_t>(); r.vrel = c->vrel.as<uint32_t>(); r.sseg = c->sseg.as<uint32_t>(); r.lab16 = c->lab16.as<uint16_t>();
@@ -409,6 +411,7 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
 }
 
 int32_t resolve_dense(const gsap_options &o) { return o.dense_max_B > 0 ? o.dense_max_B : 8192; }
+int32_t resolve_tile(const gsap_options &o) { return o.tile_max_B > 0 ? o.tile_max_B : 2048; }
 
 RngKey rng_key(const gsap_options &o) {
   return RngKey{(uint32_t)o.seed, (uint32_t)(o.seed >> 32), o.chain};
@@ -416,7 +419,8 @@ RngKey rng_key(const gsap_options &o) {
 
 void rebuild(gsap_ctx *c, int32_t B) {
   int p = c->prof.begin(1, c->st);
-  rebuild_blockmodel(c->g, c->lab.as<int32_t>(), B, c->bm, c->rs, c->dense_max_B, c->st);
+  rebuild_blockmodel(c->g, c->lab.as<int32_t>(), B, c->bm, c->rs, c->dense_max_B, c->tile_max_B,
+                     c->st);
   c->prof.end(p, c->st);
   HIPX(hipGetLastError());
 }
@@ -596,6 +600,8 @@ void check_options(const gsap_options &o) {
     throw Err(GSAP_ERR_INVALID, "unknown precision");
   if (o.model != GSAP_MODEL_REFERENCE && o.model != GSAP_MODEL_CORRECTED)
     throw Err(GSAP_ERR_INVALID, "unknown model");
+  if (o.dense_max_B < 0 || o.tile_max_B < 0)
+    throw Err(GSAP_ERR_INVALID, "dense_max_B and tile_max_B must be >= 0");
 }
 
 // A level's captured sweep: the executable graph, destroyed when the level ends or an error
@@ -645,6 +651,7 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
   upload_graph(c, gin);
   build_batch_lists(c, gin, o.num_batches);
   alloc_work(c, o.num_proposals, resolve_dense(o));
+  c->tile_max_B = resolve_tile(o);
   const double t_upload = now() - t0;
   c->trace.clear();
   c->prof.reset(o.profile != 0);
@@ -878,6 +885,7 @@ void setup_unit(gsap_ctx *c, const gsap_graph_i32 *g, const gsap_options *o, con
   upload_graph(c, g);
   build_batch_lists(c, g, oo.num_batches);
   alloc_work(c, oo.num_proposals, resolve_dense(oo));
+  c->tile_max_B = resolve_tile(oo);
   if (B <= 0 || B > c->N) throw Err(GSAP_ERR_INVALID, "B must be in [1, N]");
   for (int64_t v = 0; v < c->N; ++v)
     if (labels[v] < 0 || labels[v] >= B) throw Err(GSAP_ERR_INVALID, "label out of range");
@@ -1005,6 +1013,99 @@ int gsap_partition_chains_i32(gsap_ctx *c, const gsap_graph_i32 *g, const gsap_o
     if (S_chain)
       for (int32_t k = 0; k < n_chains; ++k) S_chain[k] = reps[k].S;
     if (best_chain) *best_chain = best;
+  });
+}
+
+int gsap_autotune_i32(gsap_ctx *c, const gsap_graph_i32 *g, gsap_options *o) {
+  return guarded(c, [&] {
+    if (!o) throw Err(GSAP_ERR_INVALID, "options is NULL");
+    gsap_options oo = *o;
+    check_options(oo);
+    set_device(c);
+    upload_graph(c, g);
+    build_batch_lists(c, g, oo.num_batches);
+    const int64_t N = c->N;
+    const int32_t dmax = (int32_t)std::min<int64_t>(16384, N);
+    alloc_work(c, oo.num_proposals, dmax);
+    hipStream_t st = c->st;
+    hipEvent_t e0, e1;
+    HIPX(hipEventCreate(&e0));
+    HIPX(hipEventCreate(&e1));
+    struct Ev {
+      hipEvent_t a, b;
+      ~Ev() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+    } ev{e0, e1};
+    std::vector<int32_t> lab(N);
+    // labels spread uniformly over B blocks (a pure function of the vertex id)
+    auto set_labels = [&](int32_t B) {
+      for (int64_t v = 0; v < N; ++v) lab[v] = (int32_t)(mix64((uint64_t)v * 0x9E3779B9ull + B) % (uint64_t)B);
+      HIPX(hipMemcpyAsync(c->lab.p, lab.data(), 4 * (size_t)N, hipMemcpyHostToDevice, st));
+    };
+    auto elapsed = [&]() {
+      HIPX(hipEventSynchronize(e1));
+      float ms = 0;
+      HIPX(hipEventElapsedTime(&ms, e0, e1));
+      return ms;
+    };
+    const uint32_t *ctr = c->ctr.as<uint32_t>();
+    HIPX(hipMemsetAsync(c->ctr.p, 0, 8, st));
+    auto time_rebuild = [&](int32_t B) {
+      rebuild(c, B);
+      float best = std::numeric_limits<float>::infinity();
+      for (int rep = 0; rep < 3; ++rep) {
+        HIPX(hipEventRecord(e0, st));
+        rebuild(c, B);
+        HIPX(hipEventRecord(e1, st));
+        best = std::min(best, elapsed());
+      }
+      return best;
+    };
+    // rebuild + one nodal batch (kernel work of a batch), labels reset before each run
+    auto time_batch = [&](int32_t B) {
+      float best = std::numeric_limits<float>::infinity();
+      for (int rep = 0; rep < 4; ++rep) {
+        set_labels(B);
+        rebuild(c, B);
+        const int32_t ar = avg_row(read_nnz(c), B);
+        HIPX(hipEventRecord(e0, st));
+        nodal_batch(c, oo, 0, PhaseRef{ctr, 0}, ar, c->res.as<double>() + 1);
+        rebuild(c, B);
+        HIPX(hipEventRecord(e1, st));
+        const float t = elapsed();
+        if (rep > 0) best = std::min(best, t);
+      }
+      return best;
+    };
+    // Row tile against segmented rebuild. A size where the tile path loses caps the
+    // threshold at the last size where it won; sizes the graph cannot reach (B > N / 2)
+    // leave the default in place.
+    int32_t tile = resolve_tile(oo);
+    for (int32_t B : {256, 512, 1024, 2048, 4096}) {
+      if (B > dmax || 2 * (int64_t)B > N) break;
+      set_labels(B);
+      c->dense_max_B = dmax;
+      c->tile_max_B = B;
+      const float t_tile = time_rebuild(B);
+      c->tile_max_B = 0;
+      const float t_seg = time_rebuild(B);
+      if (t_tile > t_seg) { tile = B / 2; break; }
+      tile = std::max(tile, B);
+    }
+    // dense against CSR block-matrix lookups, over a whole batch (rebuild included)
+    int32_t dense = resolve_dense(oo);
+    c->tile_max_B = tile;
+    for (int32_t B : {2048, 4096, 8192, 16384}) {
+      if (B > dmax || 2 * (int64_t)B > N) break;
+      c->dense_max_B = B;
+      const float t_dense = time_batch(B);
+      c->dense_max_B = B - 1;
+      const float t_csr = time_batch(B);
+      if (t_dense > t_csr) { dense = B - 1; break; }
+      dense = std::max(dense, B);
+    }
+    HIPX(hipStreamSynchronize(st));
+    o->tile_max_B = tile;
+    o->dense_max_B = dense;
   });
 }
 

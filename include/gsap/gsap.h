@@ -87,7 +87,9 @@ typedef struct gsap_options {
   int32_t graph;              /* 1: capture a level's sweep once as a HIP graph and replay it
                                  (same results; ignored with trace or profile); 0: launch
                                  the sweep kernel by kernel */
-  int32_t reserved[3];
+  int32_t tile_max_B;         /* row-tile rebuild for 128 < B <= this (dense B only;
+                                 0 = auto, 2048); above it the segmented rebuild */
+  int32_t reserved[2];
 } gsap_options;
 
 /* Per-call report.  Times are seconds of host wall clock around device work. */
@@ -154,6 +156,13 @@ int gsap_partition_batch_i32(gsap_ctx *ctx, const gsap_graph_i32 *graphs, int32_
 int gsap_partition_chains_i32(gsap_ctx *ctx, const gsap_graph_i32 *g, const gsap_options *o,
                               int32_t n_chains, int32_t *out, gsap_report *report,
                               double *S_chain, int32_t *best_chain);
+
+/* Autotuner: times, on this graph and device, the row-tile against the segmented
+ * rebuild at B = 256 .. 4096 and the nodal kernel with dense against CSR block-matrix
+ * lookups at B = 2048 .. 16384, and writes the crossovers into o->tile_max_B and
+ * o->dense_max_B (the other fields are kept).  These thresholds only choose code paths:
+ * results are identical for any values.  No counterpart in the reference. */
+int gsap_autotune_i32(gsap_ctx *ctx, const gsap_graph_i32 *g, gsap_options *o);
 
 /* Trace records of the last gsap_partition_i32 call with o->trace = 1. */
 int gsap_get_trace(const gsap_ctx *ctx, gsap_trace_rec *out, int64_t cap, int64_t *n);

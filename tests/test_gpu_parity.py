@@ -200,6 +200,32 @@ def test_c3_blockmodel_skewed(ctx, c3, B, big, dense):
         assert np.array_equal(got[k], ref[k]), k
 
 
+@pytest.mark.parametrize("B", [300, 2500, 6000])
+@pytest.mark.parametrize("tile", [1, 8192])
+def test_c3_blockmodel_tile_threshold(ctx, c3, B, tile):
+    """tile_max_B only picks the rebuild path (row tile up to 8192 blocks, else segmented):
+    both give the oracle's block matrix."""
+    g = c3
+    lab = labels_for(g, B, 13)
+    print(f"C3 blockmodel B={B} tile_max_B={tile}", flush=True)
+    got = ctx.blockmodel(g, lab, B, Options(dense_max_B=8192, tile_max_B=tile))
+    ref = O.blockmodel(g, lab, B)
+    for k in ("nnz", "rowptr", "col", "val", "colptr", "row", "cval", "dout", "din"):
+        assert np.array_equal(got[k], ref[k]), k
+
+
+def test_autotune_keeps_results(ctx):
+    """The autotuner measures the path thresholds on the caller's graph; the tuned options
+    give the oracle's partition."""
+    g = gsap_amd.generate_dcsbm(20000, 33, 1.0, 1.5, 20.0, seed=2)
+    print("autotune C2", flush=True)
+    o = ctx.autotune(g, Options(seed=1))
+    assert 128 <= o.tile_max_B <= 8192 and o.dense_max_B >= 2047
+    lab, rep = ctx.partition(g, o)
+    rlab, rrep, _ = O.partition(g, Options(seed=1))
+    assert np.array_equal(lab, rlab) and eq_bits(rep.S, rrep.S)
+
+
 @pytest.mark.parametrize("name", ["hubs", "LL1k", "HH5k"])
 def test_partition_graph_replay(ctx, name):
     """options.graph: every level's sweep captured once and replayed (at least three times
