@@ -15,6 +15,7 @@ The CPU baseline times the oracle on the host's cores (and on one thread) on a b
 sample of the same workload.
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -158,7 +159,7 @@ def cpu_baseline(g, labels, B, seconds, threads):
                        f"labelling (B={B}), {el:.1f} s")
 
 
-def compare_forms(ctx, g, lab, prec):
+def compare_forms(ctx, g, lab, prec, thr):
     """Plain FP64 form against the mixed form (FP32 delta-S with FP64 for accepted and
     near-threshold proposals): one untimed and one timed partition of the other form, no
     event profiling in either timed run."""
@@ -166,9 +167,9 @@ def compare_forms(ctx, g, lab, prec):
     walls, res = {}, {}
     for form in (prec, other):
         if form == other:
-            ctx.partition(g, gsap_amd.Options(seed=1, precision=form))
+            ctx.partition(g, gsap_amd.Options(seed=1, precision=form, **thr))
         t = time.perf_counter()
-        res[form] = ctx.partition(g, gsap_amd.Options(seed=1, precision=form))
+        res[form] = ctx.partition(g, gsap_amd.Options(seed=1, precision=form, **thr))
         walls[form] = time.perf_counter() - t
         print(f"form {'fp64' if form == 0 else 'mixed'}: {walls[form]:.3f} s", file=sys.stderr,
               flush=True)
@@ -212,6 +213,8 @@ def bench_multi(a, prec):
     t_gen = time.perf_counter() - t0
     ctx = gsap_amd.Context(0)
     opts = gsap_amd.Options(seed=1, precision=prec, workers=a.workers, graph=int(a.graph))
+    thr, t_tune = tune(a, ctx, max(graphs, key=lambda x: x.E), opts)
+    opts = dataclasses.replace(opts, **thr)
     walls, edges = [], []
     for k in range(a.warmup + a.steps):
         t1 = time.perf_counter()
@@ -238,10 +241,23 @@ def bench_multi(a, prec):
            "higher_is_better": a.config == "c4", "scaling": "weak", "vs_baseline": None,
            "dtype": "f64", "data": "synthetic (seeded DC-SBM generator)",
            "config": {"workload": desc, "workers": a.workers or 8,
-                      "sweep_launch": "hip graph replay" if a.graph else "kernel by kernel"},
-           "graph_generation_s": t_gen}
+                      "sweep_launch": "hip graph replay" if a.graph else "kernel by kernel",
+                      "thresholds": thr or "defaults"},
+           "graph_generation_s": t_gen, "autotune_s": t_tune}
     out.update(extra)
     print(json.dumps(out))
+
+
+def tune(a, ctx, g, opts):
+    """Thresholds measured on g (untimed) when --autotune is given, else none."""
+    if not a.autotune:
+        return {}, None
+    t0 = time.perf_counter()
+    o = ctx.autotune(g, opts)
+    t = time.perf_counter() - t0
+    thr = {"tile_max_B": o.tile_max_B, "dense_max_B": o.dense_max_B}
+    print(f"autotune: {thr} in {t:.2f} s", file=sys.stderr, flush=True)
+    return thr, t
 
 
 def main():
@@ -257,6 +273,9 @@ def main():
     ap.add_argument("--graph", action="store_true",
                     help="replay each level's captured sweep as a HIP graph")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--autotune", action="store_true",
+                    help="measure the row-tile and dense-lookup thresholds on the workload "
+                         "graph first (untimed) and run with them")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -293,12 +312,17 @@ def main():
     g = gsap_amd.generate_dcsbm(**cfg)
     t_gen = time.perf_counter() - t0
     ctx = gsap_amd.Context(0)
+    thr, t_tune = tune(a, ctx, g, gsap_amd.Options(seed=1, precision=prec))
+
+    def opt(**kw):
+        return gsap_amd.Options(seed=1, precision=prec, **thr, **kw)
+
     # untimed: warmups, the last one (or one extra run) with HIP events around every kernel
     # class, to pick the dominant class and report the per-class split
     prep = None
     for w in range(max(a.warmup, 1)):
         last = w == max(a.warmup, 1) - 1
-        _, r0 = ctx.partition(g, gsap_amd.Options(seed=1, precision=prec, profile=int(last)))
+        _, r0 = ctx.partition(g, opt(profile=int(last)))
         if last:
             prep = r0
         print(f"warmup {w} done", file=sys.stderr, flush=True)
@@ -308,7 +332,7 @@ def main():
     walls, reps = [], []
     for k in range(a.steps):
         t1 = time.perf_counter()
-        lab, rep = ctx.partition(g, gsap_amd.Options(seed=1, precision=prec))
+        lab, rep = ctx.partition(g, opt())
         walls.append(time.perf_counter() - t1)
         reps.append(rep)
         print(f"step {k}: {walls[-1]:.3f} s, B={rep.B}, sweeps={rep.sweeps}", file=sys.stderr,
@@ -324,8 +348,7 @@ def main():
     pwalls, preps = [], []
     for k in range(a.steps):
         t1 = time.perf_counter()
-        plab, prp = ctx.partition(g, gsap_amd.Options(seed=1, precision=prec, profile=1,
-                                                      profile_classes=1 << dom))
+        plab, prp = ctx.partition(g, opt(profile=1, profile_classes=1 << dom))
         pwalls.append(time.perf_counter() - t1)
         preps.append(prp)
         assert np.array_equal(plab, lab), "profiled partition differs"
@@ -357,7 +380,9 @@ def main():
         "data": "synthetic (seeded DC-SBM generator)",
         "config": {"workload": f"partition {a.config}: N={g.N}, E={g.E}, K={cfg['K']}, "
                                f"alpha={cfg['alpha']}, rho={cfg['rho']}, seed={cfg['seed']}",
-                   "precision": a.precision, "parallelism": "single GPU"},
+                   "precision": a.precision, "parallelism": "single GPU",
+                   "thresholds": thr or "defaults"},
+        "autotune_s": t_tune,
         "full_partition_s": float(np.mean(walls)),
         "profiled_partition_s": float(np.mean(pwalls)),
         "final_B": rep.B, "final_S": rep.S, "sweeps": rep.sweeps, "levels": rep.levels,
@@ -376,7 +401,7 @@ def main():
                      "algorithmic_bytes_per_launch": abytes},
     }
     if not a.no_compare:
-        out["precision_forms"] = compare_forms(ctx, g, lab, prec)
+        out["precision_forms"] = compare_forms(ctx, g, lab, prec, thr)
     if not a.no_cpu_baseline:
         nt = cpu_threads()
         out["cpu_baseline"] = cpu_baseline(g, lab, rep.B, a.cpu_seconds, nt)

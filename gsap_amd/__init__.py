@@ -18,11 +18,16 @@ import numpy as np
 
 __all__ = [
     "GsapError", "Graph", "Options", "Report", "Context", "lib", "generate_dcsbm", "read_tsv",
-    "pairwise", "log2", "log2_tables", "PRECISION_FP64", "PRECISION_MIXED",
+    "pairwise", "log2", "log2_tables", "PRECISION_FP64", "PRECISION_MIXED", "DELTA_REFERENCE",
+    "DELTA_EXACT", "ACCEPT_GREEDY", "ACCEPT_MH",
 ]
 
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+DELTA_REFERENCE = 0
+DELTA_EXACT = 1
+ACCEPT_GREEDY = 0
+ACCEPT_MH = 1
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgsap_amd.so")
 
@@ -52,7 +57,8 @@ class COptions(C.Structure):
                 ("trace", C.c_int32), ("dense_max_B", C.c_int32), ("profile", C.c_int32),
                 ("model", C.c_int32), ("workers", C.c_int32),
                 ("profile_classes", C.c_int32), ("graph", C.c_int32),
-                ("tile_max_B", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("tile_max_B", C.c_int32), ("delta_mode", C.c_int32), ("accept", C.c_int32),
+                ("beta", C.c_double)]
 
 
 class CReport(C.Structure):
@@ -213,13 +219,16 @@ class Options:
     workers: int = 0  # batch / chain calls: concurrent sub-contexts (0 = 8)
     graph: int = 0  # 1: each level's sweep captured once and replayed as a HIP graph
     tile_max_B: int = 0  # row-tile rebuild up to this B (0 = auto)
+    delta_mode: int = 0  # DELTA_REFERENCE or DELTA_EXACT
+    accept: int = 0  # ACCEPT_GREEDY or ACCEPT_MH
+    beta: float = 3.0  # inverse temperature of ACCEPT_MH
 
     def cstruct(self) -> COptions:
         o = COptions()
         for f in ("seed", "num_proposals", "reduction_rate", "max_nodal_itr", "num_batches",
                   "threshold1", "threshold2", "moving_avg_window", "max_levels", "precision",
                   "chain", "trace", "dense_max_B", "profile", "model", "workers",
-                  "profile_classes", "graph", "tile_max_B"):
+                  "profile_classes", "graph", "tile_max_B", "delta_mode", "accept", "beta"):
             setattr(o, f, getattr(self, f))
         return o
 

@@ -14,6 +14,7 @@ struct GraphDev {
   int32_t *iptr = nullptr, *iadj = nullptr, *iw = nullptr;
   int32_t *kout = nullptr, *kin = nullptr, *esrc = nullptr;
   int32_t *idst = nullptr;  // owner (target vertex) of every in-CSR entry
+  int32_t *sloop = nullptr; // per vertex: summed weight of its self loops
   // nodal hash tables for vertices whose adjacency does not fit the LDS table
   int64_t *hoff = nullptr;       // per vertex: offset into hpool (words), -1 when light
   uint8_t *hbits_o = nullptr;    // per vertex: log2 capacity of out table (0 = light)
@@ -104,18 +105,21 @@ constexpr int64_t BIG_LDS_CAP = 40 * 1024;
 void launch_nodal(const GraphDev &g, LabView lab, const BMView &m, const int32_t *sof,
                   const uint8_t *coin, RngKey key, PhaseRef phase, int64_t lo, int64_t BS,
                   const int32_t *const lists[NCLS], const int32_t counts[NCLS], uint32_t big_lds,
-                  int32_t *prop, double *dS, float *err, uint8_t *near, hipStream_t st);
+                  int32_t *prop, double *dS, float *err, uint8_t *near, bool exact, bool mh,
+                  double beta, hipStream_t st);
 void launch_mix_stats(const float *err, const uint8_t *near, int64_t n, uint32_t *max_bits,
                       unsigned long long *count, hipStream_t st);
+// labels of the batch: greedy (all = false) where dS < 0; Metropolis-Hastings (all = true)
+// prop holds every vertex's new label
 void launch_apply(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo, int64_t BS,
-                  hipStream_t st);
+                  bool all, hipStream_t st);
 // canonical array sum: tiles of 1024 (n read from n_dev when non-null, else n_host)
 void launch_csum_array(const double *x, const int32_t *n_dev, int64_t n_host, int64_t n_max,
                        double *tiles, double *out, hipStream_t st);
 void launch_mdl_terms(const BMView &m, const int32_t *erow, int64_t n_max, double *terms,
                       hipStream_t st);
 void launch_merge(const BMView &m, const double *OUT, RngKey key, uint32_t phase, int32_t np,
-                  int32_t *prop, double *dS, int32_t avg_row, hipStream_t st);
+                  int32_t *prop, double *dS, int32_t avg_row, bool exact, hipStream_t st);
 void launch_argmin(const double *dS, const int32_t *prop, int32_t B, int32_t np, int32_t *best,
                    double *dS_best, hipStream_t st);
 void launch_relabel(int32_t *lab, const int32_t *map, int64_t N, hipStream_t st);

@@ -144,7 +144,7 @@ struct gsap_ctx {
   std::string err;
   std::vector<gsap_trace_rec> trace;
   // graph
-  DBuf optr, oadj, ow, iptr, iadj, iw, kout, kin, esrc, idst, hoff, hbo, hbi, hpool;
+  DBuf optr, oadj, ow, iptr, iadj, iw, kout, kin, esrc, idst, sloop, hoff, hbo, hbi, hpool;
   // labels and golden-section snapshots
   DBuf lab, snap[3];
   // block matrix
@@ -273,10 +273,14 @@ void upload_graph(gsap_ctx *c, const gsap_graph_i32 *g) {
   const int64_t N = g->N, E = g->E;
   c->N = N; c->E = E;
   hipStream_t st = c->st;
-  std::vector<int32_t> kout(N, 0), kin(N, 0), esrc(E), idst(E);
+  std::vector<int32_t> kout(N, 0), kin(N, 0), esrc(E), idst(E), sloop(N, 0);
   parallel_chunks(N, host_threads(), [&](int64_t lo, int64_t hi, int) {
     for (int64_t v = lo; v < hi; ++v) {
-      for (int32_t j = g->out_ptr[v]; j < g->out_ptr[v + 1]; ++j) { kout[v] += g->out_w[j]; esrc[j] = (int32_t)v; }
+      for (int32_t j = g->out_ptr[v]; j < g->out_ptr[v + 1]; ++j) {
+        kout[v] += g->out_w[j];
+        esrc[j] = (int32_t)v;
+        if (g->out_adj[j] == v) sloop[v] += g->out_w[j];
+      }
       for (int32_t j = g->in_ptr[v]; j < g->in_ptr[v + 1]; ++j) { kin[v] += g->in_w[j]; idst[j] = (int32_t)v; }
     }
   });
@@ -304,6 +308,7 @@ void upload_graph(gsap_ctx *c, const gsap_graph_i32 *g) {
   h2d(c->kin, kin.data(), N, st);
   h2d(c->esrc, esrc.data(), E, st);
   h2d(c->idst, idst.data(), E, st);
+  h2d(c->sloop, sloop.data(), N, st);
   h2d(c->hoff, hoff.data(), N, st);
   h2d(c->hbo, hbo.data(), N, st);
   h2d(c->hbi, hbi.data(), N, st);
@@ -315,6 +320,7 @@ void upload_graph(gsap_ctx *c, const gsap_graph_i32 *g) {
   d.iptr = c->iptr.as<int32_t>(); d.iadj = c->iadj.as<int32_t>(); d.iw = c->iw.as<int32_t>();
   d.kout = c->kout.as<int32_t>(); d.kin = c->kin.as<int32_t>(); d.esrc = c->esrc.as<int32_t>();
   d.idst = c->idst.as<int32_t>();
+  d.sloop = c->sloop.as<int32_t>();
   d.hoff = c->hoff.as<int64_t>(); d.hbits_o = c->hbo.as<uint8_t>(); d.hbits_i = c->hbi.as<uint8_t>();
   d.hpool = c->hpool.as<int32_t>();
 }
@@ -489,7 +495,7 @@ int32_t merge_phase(gsap_ctx *c, const gsap_options &o, int32_t B_to_merge, uint
   c->prof.end(p, st);
   p = c->prof.begin(3, st);
   launch_merge(m, c->OUT.as<double>(), rng_key(o), phase, np, c->prop.as<int32_t>(),
-               c->dS.as<double>(), ar, st);
+               c->dS.as<double>(), ar, o.delta_mode == GSAP_DELTA_EXACT, st);
   c->prof.end(p, st);
   launch_argmin(c->dS.as<double>(), c->prop.as<int32_t>(), B, np, c->best.as<int32_t>(),
                 c->dSbest.as<double>(), st);
@@ -576,14 +582,16 @@ void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, PhaseRef phase, 
   const LabView lv{c->lab.as<int32_t>(), c->bm.B <= 65535 ? c->rs.lab16 : nullptr};
   launch_nodal(c->g, lv, m, c->sof.as<int32_t>(), c->coin.as<uint8_t>(),
                rng_key(o), phase, lo, BS, lists, counts, c->big_lds[b], c->prop.as<int32_t>(),
-               c->dS.as<double>(), err, evcls, st);
+               c->dS.as<double>(), err, evcls, o.delta_mode == GSAP_DELTA_EXACT,
+               o.accept == GSAP_ACCEPT_MH, o.beta, st);
   if (mixed) {
     uint32_t *ms = c->mixstat.as<uint32_t>();
     launch_mix_stats(err, evcls, BS, ms, reinterpret_cast<unsigned long long *>(ms + 2), st);
   }
   c->prof.end(p, st);
   p = c->prof.begin(5, st);
-  launch_apply(c->lab.as<int32_t>(), c->prop.as<int32_t>(), c->dS.as<double>(), lo, BS, st);
+  launch_apply(c->lab.as<int32_t>(), c->prop.as<int32_t>(), c->dS.as<double>(), lo, BS,
+               o.accept == GSAP_ACCEPT_MH, st);
   launch_csum_array(c->dS.as<double>(), nullptr, BS, BS, c->tiles.as<double>(), dE_dev, st);
   c->prof.end(p, st);
   HIPX(hipGetLastError());
@@ -602,6 +610,15 @@ void check_options(const gsap_options &o) {
     throw Err(GSAP_ERR_INVALID, "unknown model");
   if (o.dense_max_B < 0 || o.tile_max_B < 0)
     throw Err(GSAP_ERR_INVALID, "dense_max_B and tile_max_B must be >= 0");
+  if (o.delta_mode != GSAP_DELTA_REFERENCE && o.delta_mode != GSAP_DELTA_EXACT)
+    throw Err(GSAP_ERR_INVALID, "unknown delta_mode");
+  if (o.accept != GSAP_ACCEPT_GREEDY && o.accept != GSAP_ACCEPT_MH)
+    throw Err(GSAP_ERR_INVALID, "unknown accept");
+  if (o.accept == GSAP_ACCEPT_MH && !(o.beta >= 0.0 && o.beta <= 1e300))
+    throw Err(GSAP_ERR_INVALID, "beta must be finite and >= 0");
+  // the mixed form's FP32 pass proves rejections of dS >= 0, which MH may accept
+  if (o.accept == GSAP_ACCEPT_MH && o.precision == GSAP_PRECISION_MIXED)
+    throw Err(GSAP_ERR_INVALID, "Metropolis-Hastings acceptance needs precision FP64");
 }
 
 // A level's captured sweep: the executable graph, destroyed when the level ends or an error
@@ -676,7 +693,7 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
   double S = 0.0, t_merge = 0.0, t_sweep = 0.0;
   bool optimal = false;
   const int64_t BS = N / o.num_batches;
-  std::vector<int32_t> hprop, hlab;
+  std::vector<int32_t> hprop, hlab, hbefore, hafter;
   std::vector<double> hdS;
   auto rec = [&](int32_t kind, int32_t Bv, int32_t a, int32_t b, uint64_t h1, uint64_t h2,
                  double x, double y) {
@@ -723,18 +740,23 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
         batches += o.num_batches;
       }
       for (int b = 0; !use_graph && b < o.num_batches; ++b) {
+        const int32_t *lab_b = c->lab.as<int32_t>() + (int64_t)b * BS;
+        if (tr) {
+          hbefore.resize(BS);
+          HIPX(hipMemcpyAsync(hbefore.data(), lab_b, 4 * (size_t)BS, hipMemcpyDeviceToHost, st));
+        }
         nodal_batch(c, o, b, PhaseRef{ctr, (uint32_t)b}, ar, res + 1 + b);
         rebuild(c, B);
         if (tr) {
           double e = 0;
           hprop.resize(BS);
-          hdS.resize(BS);
+          hafter.resize(BS);
           HIPX(hipMemcpyAsync(hprop.data(), c->prop.p, 4 * (size_t)BS, hipMemcpyDeviceToHost, st));
-          HIPX(hipMemcpyAsync(hdS.data(), c->dS.p, 8 * (size_t)BS, hipMemcpyDeviceToHost, st));
+          HIPX(hipMemcpyAsync(hafter.data(), lab_b, 4 * (size_t)BS, hipMemcpyDeviceToHost, st));
           HIPX(hipMemcpyAsync(&e, res + 1 + b, 8, hipMemcpyDeviceToHost, st));
           HIPX(hipStreamSynchronize(st));
-          int64_t moved = 0;  // accepted moves of the batch
-          for (int64_t i = 0; i < BS; ++i) moved += hdS[i] < 0.0;
+          int64_t moved = 0;  // vertices of the batch whose label changed
+          for (int64_t i = 0; i < BS; ++i) moved += hafter[i] != hbefore[i];
           rec(2, B, itr, b, fnv(hprop.data(), 4 * BS), labels_hash(), e, (double)moved);
         }
         ++batch_ctr;
@@ -852,6 +874,7 @@ gsap_options defaults() {
   o.moving_avg_window = 3;
   o.max_levels = 0;
   o.precision = GSAP_PRECISION_FP64;
+  o.beta = 3.0;
   o.chain = 0;
   o.trace = 0;
   o.dense_max_B = 0;
@@ -952,7 +975,7 @@ int gsap_destroy(gsap_ctx *c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->st);
   DBuf *all[] = {&c->optr, &c->oadj, &c->ow, &c->iptr, &c->iadj, &c->iw, &c->kout, &c->kin,
-                 &c->esrc, &c->idst, &c->hoff, &c->hbo, &c->hbi, &c->hpool, &c->lab, &c->snap[0],
+                 &c->esrc, &c->idst, &c->sloop, &c->hoff, &c->hbo, &c->hbi, &c->hpool, &c->lab, &c->snap[0],
                  &c->snap[1], &c->snap[2], &c->rowptr, &c->colptr, &c->col, &c->val, &c->erow,
                  &c->row, &c->cval, &c->dout, &c->din, &c->ldout, &c->ldin, &c->dense, &c->k0,
                  &c->k1, &c->tmp, &c->partials, &c->soff, &c->su, &c->suoff,

@@ -220,7 +220,13 @@ struct NodalArgs {
   int32_t *hpool;
   float *err;      // mixed form: |dS32 - dS64| where FP64 was evaluated, else 0
   uint8_t *near;   // mixed form: 0 FP32 rejection, 1 FP64 evaluated, 2 |dS32| within bound
+  const int32_t *sloop;  // exact delta: self-loop weight per vertex
+  double beta;           // Metropolis-Hastings inverse temperature
 };
+
+// Variants of the nodal kernels (template bits): the exact delta-S and Metropolis-Hastings
+// acceptance.
+constexpr int VAR_EXACT = 1, VAR_MH = 2;
 
 GSAP_D uint32_t nodal_phase(const NodalArgs &a) {
   return (a.phase.dev ? *a.phase.dev : 0u) + a.phase.add;
@@ -252,6 +258,68 @@ GSAP_D bool mixed_rejects(double d32, float eb, double Y_r, double Y_s, double Y
   return d32 >= bound;
 }
 
+// Metropolis-Hastings decision: accept iff u < exp(-beta dS) H, compared in log2, with
+// u = (x + 1) 2^-32 from the vertex's third Philox word and lh = log2 H.
+GSAP_D bool mh_accepts(double d, double lh, double beta, uint32_t x) {
+  const double L = ((-beta) * d) * 1.4426950408889634 + lh;
+  return clog2_s((double)x + 1.0) - 32.0 < L;
+}
+
+// Per-vertex constants of the Hastings ratio.  The nodal sampler proposes s for v with
+// probability sum_t (k_t / k_v) (M_ts + M_st + 1) / (d_t + B) (neighbour block t, then a
+// uniform block with probability B / (d_t + B), else a block of t's row and column weighted
+// by M; d_t = d_out_t + d_in_t); the reverse proposal uses the state after the move.
+struct Hast {
+  int64_t v;
+  int32_t r, s, B;
+  int64_t Mrs, Msr, Mss;  // before the move
+  int64_t Rr, Rs, Sr;     // M'_rr, M'_rs, M'_sr after the move
+  int64_t kv, dr, ds;     // v's total degree; total degrees of r and s before the move
+};
+
+// True when the reverse term of an entry (neighbour u in block t) reads v's weights a_t, c_t.
+GSAP_D bool hast_needs(const Hast &h, int32_t u, int32_t t) {
+  return u != h.v && t != h.r && t != h.s;
+}
+
+// One adjacency entry's forward term w (M_ts + M_st + 1) / (d_t + B) and reverse term
+// w (M'_t'r + M'_rt' + 1) / (d'_t' + B), t' = s for a self loop (u = v), else t; at, ct
+// are v's out / in weights to block t (read only when hast_needs).
+GSAP_D void hast_terms(const BMView &m, const Hast &h, int32_t u, int32_t t, int32_t w,
+                       int64_t at, int64_t ct, double &fw, double &bk) {
+  const int64_t Bq = h.B;
+  int64_t nf, df, nb = 0, db = 1;
+  if (t == h.r) {
+    nf = h.Mrs + h.Msr + 1;
+    df = h.dr + Bq;
+  } else if (t == h.s) {
+    nf = 2 * h.Mss + 1;
+    df = h.ds + Bq;
+  } else {
+    int32_t Mrt, Mst, Mtr, Mts;
+    bm_row_get2(m, h.r, h.s, t, Mrt, Mst);
+    bm_col_get2(m, h.r, h.s, t, Mtr, Mts);
+    nf = (int64_t)Mts + Mst + 1;
+    df = (int64_t)m.dout[t] + m.din[t] + Bq;
+    nb = ((int64_t)Mtr - ct) + ((int64_t)Mrt - at) + 1;
+    db = df;
+  }
+  if (u == h.v || t == h.s) {
+    nb = h.Sr + h.Rs + 1;
+    db = h.ds + h.kv + Bq;
+  } else if (t == h.r) {
+    nb = 2 * h.Rr + 1;
+    db = h.dr - h.kv + Bq;
+  }
+  fw = ((double)w * (double)nf) / (double)df;
+  bk = ((double)w * (double)nb) / (double)db;
+}
+
+// log2 of the Hastings ratio from the canonical sums (0 for a vertex without edges)
+GSAP_D double hast_log2(double fo, double fi, double bo, double bi, int32_t n) {
+  return n == 0 ? 0.0 : clog2_s(bo + bi) - clog2_s(fo + fi);
+}
+
 // One wave per vertex v of the batch: proposal (kernels.hpp:877-939, 1000-1032) and delta-S
 // against the batch-start block matrix (kernels.hpp:494-643, 1153-1726, node-only entries
 // run-merged).  Only the entries v touches are evaluated term by term ("slots": v's
@@ -259,7 +327,7 @@ GSAP_D bool mixed_rejects(double d32, float eb, double Y_r, double Y_s, double Y
 // the untouched entries contribute (log2 d_old - log2 d_new) * (their summed count).
 // Labels are not written here (batch-Jacobi).  Hash tables of 2^lbits slots live in LDS.
 
-template <bool MIX>
+template <bool MIX, int VAR>
 GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbits) {
   const uint32_t lcap = 1u << lbits;
   int32_t *lko = lds, *lwo = lds + lcap, *lfo = lds + 2 * lcap;
@@ -318,6 +386,8 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
   wave_lds_sync();
   const int64_t c_r = ht_weight(hi, r), c_s = ht_weight(hi, s);
   const bool r_in_a = ht_find(ho, r) >= 0, s_in_a = ht_find(ho, s) >= 0;
+  constexpr bool EX = (VAR & VAR_EXACT) != 0, MH = (VAR & VAR_MH) != 0;
+  const int64_t sig = EX ? a.sloop[v] : 0;  // a self loop moves from (r, r) to (s, s)
   const double l_dr = clog2_s((double)(dout_r - kout));
   const double l_ds = clog2_s((double)(dout_s + kout));
   const double l_ir = clog2_s((double)(din_r - kin));
@@ -351,8 +421,8 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
     bm_row_get2(m, r, s, t, Mr32, Ms32);
     Mr = Mr32;
     Ms = Ms32;
-    R = Mr - av - (t == r ? c_r : 0) + (t == s ? c_r : 0);
-    S = Ms + av - (t == r ? c_s : 0) + (t == s ? c_s : 0);
+    R = Mr - av - (t == r ? c_r - sig : 0) + (t == s ? c_r - sig : 0);
+    S = Ms + av - (t == r ? c_s + sig : 0) + (t == s ? c_s + sig : 0);
     return true;
   };
   // columns r and s over the column slots (t not in {r, s})
@@ -376,10 +446,12 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
       int64_t Mr, Ms, R, S;
       if (!row_slot(j, t, Mr, Ms, R, S)) continue;
       const double ldt = m.ldin[t];
+      // exact form: the new rows see the new column degrees of r and s
+      const double ldn = !EX ? ldt : t == r ? l_ir : t == s ? l_is : ldt;
       const double or_ = Mr > 0 ? term_s(Mr, ldo_r, ldt) : 0.0;
       const double os_ = Ms > 0 ? term_s(Ms, ldo_s, ldt) : 0.0;
-      const double nr = R > 0 ? term_s(R, ldt, l_dr) : 0.0;
-      const double ns = S > 0 ? term_s(S, ldt, l_ds) : 0.0;
+      const double nr = R > 0 ? term_s(R, ldn, l_dr) : 0.0;
+      const double ns = S > 0 ? term_s(S, ldn, l_ds) : 0.0;
       p_row += ((or_ + os_) - nr) - ns;
       if (sums) {
         sum_r += Mr;
@@ -422,10 +494,11 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
       sum_r += Mr;
       sum_s += Ms;
       const float ldt = (float)m.ldin[t];
+      const float ldn = !EX ? ldt : t == r ? (float)l_ir : t == s ? (float)l_is : ldt;
       const float or_ = Mr > 0 ? term32(Mr, (float)ldo_r, ldt, eb) : 0.0f;
       const float os_ = Ms > 0 ? term32(Ms, (float)ldo_s, ldt, eb) : 0.0f;
-      const float nr = R > 0 ? term32(R, ldt, (float)l_dr, eb) : 0.0f;
-      const float ns = S > 0 ? term32(S, ldt, (float)l_ds, eb) : 0.0f;
+      const float nr = R > 0 ? term32(R, ldn, (float)l_dr, eb) : 0.0f;
+      const float ns = S > 0 ? term32(S, ldn, (float)l_ds, eb) : 0.0f;
       krow.add(((or_ + os_) - nr) - ns);
     }
     for (int32_t j = lane; j < ni; j += 64) {
@@ -479,8 +552,46 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
     }
     fp64_pass(drow, dcol, false);
   }
+  double lh = 0.0;
+  if constexpr (MH) {
+    const int64_t a_r = ht_weight(ho, r), a_s = ht_weight(ho, s);
+    const Hast h{v, r, s, B, M_rs, M_sr, M_ss, M_rr - a_r - c_r + sig, M_rs - a_s + c_r - sig,
+                 M_sr + a_r - c_s - sig, tot, dout_r + din_r, dout_s + din_s};
+    double fo = 0.0, fi = 0.0, bo = 0.0, bi = 0.0;
+    for (int32_t j = lane; j < no; j += 64) {
+      const int32_t u = a.oadj[o0 + j];
+      const int32_t t = j < 64 ? t_o0 : a.lab[u];
+      const bool nd = hast_needs(h, u, t);
+      double f, b;
+      hast_terms(m, h, u, t, j < 64 ? w_o0 : a.ow[o0 + j], nd ? ht_weight(ho, t) : 0,
+                 nd ? ht_weight(hi, t) : 0, f, b);
+      fo += f;
+      bo += b;
+    }
+    for (int32_t j = lane; j < ni; j += 64) {
+      const int32_t u = a.iadj[i0 + j];
+      const int32_t t = j < 64 ? t_i0 : a.lab[u];
+      const bool nd = hast_needs(h, u, t);
+      double f, b;
+      hast_terms(m, h, u, t, j < 64 ? w_i0 : a.iw[i0 + j], nd ? ht_weight(ho, t) : 0,
+                 nd ? ht_weight(hi, t) : 0, f, b);
+      fi += f;
+      bi += b;
+    }
+    fo = wave_sum(fo);
+    fi = wave_sum(fi);
+    bo = wave_sum(bo);
+    bi = wave_sum(bi);
+    if (lane == 0) lh = hast_log2(fo, fi, bo, bi, no + ni);
+  }
   if (lane == 0) {
     const double d = nodal_total(drow, dcol, Y_r, Y_s, Y2_r, Y2_s);
+    if constexpr (MH) {
+      const bool acc = mh_accepts(d, lh, a.beta, x.x2);
+      a.prop[i] = acc ? s : r;
+      a.dS[i] = acc ? d : 0.0;
+      return;
+    }
     a.dS[i] = d < 0.0 ? d : 0.0;  // accept iff dS < 0
     if constexpr (MIX) {
       // non-finite delta-S (an empty block's -inf log, the reference's quirk) only ever
@@ -498,11 +609,12 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
 // of the global pool); every row and column slot value is stored by position and wave 0
 // sums them in the canonical order, so the bits equal the one-wave form.  The mixed form
 // evaluates these few vertices in FP64 directly.
-template <bool MIX>
+template <bool MIX, int VAR>
 GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds_bytes) {
   __shared__ int64_t red[4][4];
   __shared__ int64_t corner_s[4];
   __shared__ uint32_t pick_sc[8];
+  __shared__ double dh_s[2];  // Metropolis-Hastings: delta-S and the forward sum
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int64_t i = v - a.lo;
   const BMView &m = a.m;
@@ -573,6 +685,8 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
   __syncthreads();
   const int64_t c_r = ht_weight(hi, r), c_s = ht_weight(hi, s);
   const bool r_in_a = ht_find(ho, r) >= 0, s_in_a = ht_find(ho, s) >= 0;
+  constexpr bool EX = (VAR & VAR_EXACT) != 0, MH = (VAR & VAR_MH) != 0;
+  const int64_t sig = EX ? a.sloop[v] : 0;  // a self loop moves from (r, r) to (s, s)
   const double l_dr = clog2_s((double)(dout_r - kout));
   const double l_ds = clog2_s((double)(dout_s + kout));
   const double l_ir = clog2_s((double)(din_r - kin));
@@ -600,13 +714,14 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
       int32_t Mr32, Ms32;
       bm_row_get2(m, r, s, t, Mr32, Ms32);
       const int64_t Mr = Mr32, Ms = Ms32;
-      const int64_t R = Mr - av - (t == r ? c_r : 0) + (t == s ? c_r : 0);
-      const int64_t S = Ms + av - (t == r ? c_s : 0) + (t == s ? c_s : 0);
+      const int64_t R = Mr - av - (t == r ? c_r - sig : 0) + (t == s ? c_r - sig : 0);
+      const int64_t S = Ms + av - (t == r ? c_s + sig : 0) + (t == s ? c_s + sig : 0);
       const double ldt = m.ldin[t];
+      const double ldn = !EX ? ldt : t == r ? l_ir : t == s ? l_is : ldt;
       const double or_ = Mr > 0 ? term_s(Mr, ldo_r, ldt) : 0.0;
       const double os_ = Ms > 0 ? term_s(Ms, ldo_s, ldt) : 0.0;
-      const double nr = R > 0 ? term_s(R, ldt, l_dr) : 0.0;
-      const double ns = S > 0 ? term_s(S, ldt, l_ds) : 0.0;
+      const double nr = R > 0 ? term_s(R, ldn, l_dr) : 0.0;
+      const double ns = S > 0 ? term_s(S, ldn, l_ds) : 0.0;
       xv = ((or_ + os_) - nr) - ns;
       sum_r += Mr;
       sum_s += Ms;
@@ -646,7 +761,8 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
     red[wv][3] = csum_s;
   }
   __syncthreads();
-  if (wv != 0) return;
+  if (!MH && wv != 0) return;
+  if (wv == 0) {
   // canonical sums: position k into partial k mod 64, ascending; unused slots hold +0.0
   double p_row = 0.0, p_col = 0.0;
   for (int32_t k = lane; k < no + 2; k += 64) p_row += xs[k];
@@ -666,10 +782,66 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
     const double Y2_r = W2_r > 0 ? (ldi_r - l_ir) * (double)W2_r : 0.0;
     const double Y2_s = W2_s > 0 ? (ldi_s - l_is) * (double)W2_s : 0.0;
     const double d = nodal_total(drow, dcol, Y_r, Y_s, Y2_r, Y2_s);
-    a.dS[i] = d < 0.0 ? d : 0.0;
-    if constexpr (MIX) {
-      a.err[i] = 0.0f;
-      a.near[i] = 1;
+    if constexpr (MH) {
+      dh_s[0] = d;
+    } else {
+      a.dS[i] = d < 0.0 ? d : 0.0;
+      if constexpr (MIX) {
+        a.err[i] = 0.0f;
+        a.near[i] = 1;
+      }
+    }
+  }
+  }
+  if constexpr (MH) {
+    // Hastings sums in two passes over the slot array: forward terms, then reverse terms,
+    // each summed canonically by wave 0 (out entries at 0.., in entries at no + 2 ..)
+    const int64_t M_rr = corner_s[0], M_sr = corner_s[1], M_rs = corner_s[2], M_ss = corner_s[3];
+    const int64_t a_r = ht_weight(ho, r), a_s = ht_weight(ho, s);
+    const Hast h{v, r, s, B, M_rs, M_sr, M_ss, M_rr - a_r - c_r + sig, M_rs - a_s + c_r - sig,
+                 M_sr + a_r - c_s - sig, tot, dout_r + din_r, dout_s + din_s};
+    auto fill = [&](bool back) {
+      for (int32_t j = tid; j < no + ni; j += 256) {
+        const bool out = j < no;
+        const int32_t k = out ? j : j - no;
+        const int32_t u = out ? a.oadj[o0 + k] : a.iadj[i0 + k];
+        const int32_t t = a.lab[u];
+        const bool nd = back && hast_needs(h, u, t);
+        double f, b;
+        hast_terms(m, h, u, t, out ? a.ow[o0 + k] : a.iw[i0 + k], nd ? ht_weight(ho, t) : 0,
+                   nd ? ht_weight(hi, t) : 0, f, b);
+        xs[out ? k : no + 2 + k] = back ? b : f;
+      }
+    };
+    auto sums = [&](double &so, double &si) {
+      double po = 0.0, pi = 0.0;
+      for (int32_t k = lane; k < no; k += 64) po += xs[k];
+      for (int32_t k = lane; k < ni; k += 64) pi += xs[no + 2 + k];
+      so = wave_sum(po);
+      si = wave_sum(pi);
+    };
+    __syncthreads();  // wave 0 has read the delta-S slots
+    fill(false);
+    __syncthreads();
+    if (wv == 0) {
+      double fo, fi;
+      sums(fo, fi);
+      if (lane == 0) dh_s[1] = fo + fi;
+    }
+    __syncthreads();
+    fill(true);
+    __syncthreads();
+    if (wv == 0) {
+      double bo, bi;
+      sums(bo, bi);
+      if (lane == 0) {
+        // same grouping as hast_log2: (bo + bi) and the forward (fo + fi)
+        const double lh = no + ni == 0 ? 0.0 : clog2_s(bo + bi) - clog2_s(dh_s[1]);
+        const double d = dh_s[0];
+        const bool acc = mh_accepts(d, lh, a.beta, x.x2);
+        a.prop[i] = acc ? s : r;
+        a.dS[i] = acc ? d : 0.0;
+      }
     }
   }
 }
@@ -686,7 +858,7 @@ GSAP_D int64_t group_allsum(int64_t x) {
 // per vertex: lane q holds out-entry q and in-entry q; the neighbour blocks are deduplicated
 // in registers (shuffles inside the group) instead of hash tables, and the block of the
 // sampled neighbour comes from the same gathered values.
-template <int G, bool MIX>
+template <int G, bool MIX, int VAR>
 GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int32_t bid) {
   const int32_t gid = (int32_t)(bid * (256 / G) + threadIdx.x / G);
   if (gid >= n) return;
@@ -703,9 +875,10 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
   const int32_t i0 = a.iptr[v], ni = a.iptr[v + 1] - i0;
   const int64_t kout = a.kout[v], kin = a.kin[v];
   const int64_t tot = kout + kin;
-  int32_t t_o = -1, w_o = 0, t_i = -1, w_i = 0;
-  if (q < no) { w_o = a.ow[o0 + q]; t_o = a.lab[a.oadj[o0 + q]]; }
-  if (q < ni) { w_i = a.iw[i0 + q]; t_i = a.lab[a.iadj[i0 + q]]; }
+  constexpr bool EX = (VAR & VAR_EXACT) != 0, MH = (VAR & VAR_MH) != 0;
+  int32_t t_o = -1, w_o = 0, t_i = -1, w_i = 0, u_o = -1, u_i = -1;
+  if (q < no) { w_o = a.ow[o0 + q]; u_o = a.oadj[o0 + q]; t_o = a.lab[u_o]; }
+  if (q < ni) { w_i = a.iw[i0 + q]; u_i = a.iadj[i0 + q]; t_i = a.lab[u_i]; }
   // proposal: first entry of (out list, in list) whose running weight exceeds the target
   int32_t s;
   if (tot == 0) {
@@ -740,8 +913,9 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
     }
     return;
   }
-  // per-lane block weight and first-occurrence flag
-  int64_t a_sum = 0, c_sum = 0;
+  // per-lane block weight and first-occurrence flag (Metropolis-Hastings also needs the
+  // in weight of the lane's out block and the out weight of its in block)
+  int64_t a_sum = 0, c_sum = 0, c_of_o = 0, a_of_i = 0;
   bool first_o = q < no, first_i = q < ni;
   const int pend = no > ni ? no : ni;
 #pragma unroll 1
@@ -750,7 +924,12 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
     const int32_t tpi = __shfl(t_i, p, G), wpi = __shfl(w_i, p, G);
     if (tpo == t_o) { a_sum += wpo; if (p < q) first_o = false; }
     if (tpi == t_i) { c_sum += wpi; if (p < q) first_i = false; }
+    if constexpr (MH) {
+      if (tpi == t_o) c_of_o += wpi;
+      if (tpo == t_i) a_of_i += wpo;
+    }
   }
+  const int64_t sig = EX ? a.sloop[v] : 0;  // a self loop moves from (r, r) to (s, s)
   const int64_t c_r = group_allsum<G>((q < ni && t_i == r) ? w_i : 0);
   const int64_t c_s = group_allsum<G>((q < ni && t_i == s) ? w_i : 0);
   const bool r_in_a = ((__ballot(q < no && t_o == r) >> gbase) & gmask) != 0;
@@ -787,8 +966,8 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
     bm_row_get2(m, r, s, t, Mr32, Ms32);
     Mr = Mr32;
     Ms = Ms32;
-    R = Mr - av - (t == r ? c_r : 0) + (t == s ? c_r : 0);
-    S = Ms + av - (t == r ? c_s : 0) + (t == s ? c_s : 0);
+    R = Mr - av - (t == r ? c_r - sig : 0) + (t == s ? c_r - sig : 0);
+    S = Ms + av - (t == r ? c_s + sig : 0) + (t == s ? c_s + sig : 0);
     return true;
   };
   // column slot of lane q (t not in {r, s})
@@ -814,10 +993,11 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
       int64_t Mr, Ms, R, S;
       if (!row_slot(j, t, Mr, Ms, R, S)) continue;
       const double ldt = m.ldin[t];
+      const double ldn = !EX ? ldt : t == r ? l_ir : t == s ? l_is : ldt;
       const double or_ = Mr > 0 ? term_s(Mr, ldo_r, ldt) : 0.0;
       const double os_ = Ms > 0 ? term_s(Ms, ldo_s, ldt) : 0.0;
-      const double nr = R > 0 ? term_s(R, ldt, l_dr) : 0.0;
-      const double ns = S > 0 ? term_s(S, ldt, l_ds) : 0.0;
+      const double nr = R > 0 ? term_s(R, ldn, l_dr) : 0.0;
+      const double ns = S > 0 ? term_s(S, ldn, l_ds) : 0.0;
       prow.p[jj % CAcc<G>::S] += ((or_ + os_) - nr) - ns;
       if (sums) {
         sum_r += Mr;
@@ -862,10 +1042,11 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
       sum_r += Mr;
       sum_s += Ms;
       const float ldt = (float)m.ldin[t];
+      const float ldn = !EX ? ldt : t == r ? (float)l_ir : t == s ? (float)l_is : ldt;
       const float or_ = Mr > 0 ? term32(Mr, (float)ldo_r, ldt, eb) : 0.0f;
       const float os_ = Ms > 0 ? term32(Ms, (float)ldo_s, ldt, eb) : 0.0f;
-      const float nr = R > 0 ? term32(R, ldt, (float)l_dr, eb) : 0.0f;
-      const float ns = S > 0 ? term32(S, ldt, (float)l_ds, eb) : 0.0f;
+      const float nr = R > 0 ? term32(R, ldn, (float)l_dr, eb) : 0.0f;
+      const float ns = S > 0 ? term32(S, ldn, (float)l_ds, eb) : 0.0f;
       krow.add(((or_ + os_) - nr) - ns);
     }
     if (col_use) {
@@ -918,8 +1099,40 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
     }
     fp64_pass(drow, dcol, false);
   }
+  double lh = 0.0;
+  if constexpr (MH) {
+    const int64_t a_r = group_allsum<G>((q < no && t_o == r) ? w_o : 0);
+    const int64_t a_s = group_allsum<G>((q < no && t_o == s) ? w_o : 0);
+    const Hast h{v, r, s, B, M_rs, M_sr, M_ss, M_rr - a_r - c_r + sig, M_rs - a_s + c_r - sig,
+                 M_sr + a_r - c_s - sig, tot, dout_r + din_r, dout_s + din_s};
+    CAcc<G> fo, fi, bo, bi;
+    fo.init();
+    fi.init();
+    bo.init();
+    bi.init();
+    if (q < no) {
+      double f, b;
+      hast_terms(m, h, u_o, t_o, w_o, a_sum, c_of_o, f, b);
+      fo.p[0] += f;
+      bo.p[0] += b;
+    }
+    if (q < ni) {
+      double f, b;
+      hast_terms(m, h, u_i, t_i, w_i, a_of_i, c_sum, f, b);
+      fi.p[0] += f;
+      bi.p[0] += b;
+    }
+    const double so = fo.reduce(), si = fi.reduce(), ro = bo.reduce(), ri = bi.reduce();
+    if (q == 0) lh = hast_log2(so, si, ro, ri, no + ni);
+  }
   if (q == 0) {
     const double d = nodal_total(drow, dcol, Y_r, Y_s, Y2_r, Y2_s);
+    if constexpr (MH) {
+      const bool acc = mh_accepts(d, lh, a.beta, x.x2);
+      a.prop[i] = acc ? s : r;
+      a.dS[i] = acc ? d : 0.0;
+      return;
+    }
     a.dS[i] = d < 0.0 ? d : 0.0;
     if constexpr (MIX) {
       // non-finite delta-S (an empty block's -inf log, the reference's quirk) only ever
@@ -941,37 +1154,38 @@ struct NodalLists {
 
 // One launch per batch: the largest vertices first (one workgroup each), then 4 vertices
 // per workgroup (a wave each, 6 KB of LDS tables per wave), then groups of 32 and 16 lanes.
-template <bool MIX>
+template <bool MIX, int VAR>
 __global__ __launch_bounds__(256) void k_nodal_fused(NodalArgs a, NodalLists L) {
   extern __shared__ double lds_d[];
   log2_tables_to_lds();
   int32_t b = (int32_t)blockIdx.x;
   if (b < L.nb[3]) {
-    nodal_big<MIX>(a, L.list[3][b], lds_d, L.big_lds);
+    nodal_big<MIX, VAR>(a, L.list[3][b], lds_d, L.big_lds);
     return;
   }
   b -= L.nb[3];
   if (b < L.nb[2]) {
     const int32_t k = b * 4 + (int32_t)(threadIdx.x >> 6);
     if (k < L.n[2])
-      nodal_wave<MIX>(a, L.list[2][k],
-                      reinterpret_cast<int32_t *>(lds_d) + (threadIdx.x >> 6) * 6 * LCAP,
-                      (uint32_t)LBITS);
+      nodal_wave<MIX, VAR>(a, L.list[2][k],
+                           reinterpret_cast<int32_t *>(lds_d) + (threadIdx.x >> 6) * 6 * LCAP,
+                           (uint32_t)LBITS);
     return;
   }
   b -= L.nb[2];
   if (b < L.nb[1]) {
-    nodal_small<32, MIX>(a, L.list[1], L.n[1], b);
+    nodal_small<32, MIX, VAR>(a, L.list[1], L.n[1], b);
     return;
   }
   b -= L.nb[1];
-  nodal_small<16, MIX>(a, L.list[0], L.n[0], b);
+  nodal_small<16, MIX, VAR>(a, L.list[0], L.n[0], b);
 }
 
 void launch_nodal(const GraphDev &g, LabView lab, const BMView &m, const int32_t *sof,
                   const uint8_t *coin, RngKey key, PhaseRef phase, int64_t lo, int64_t BS,
                   const int32_t *const lists[NCLS], const int32_t counts[NCLS], uint32_t big_lds,
-                  int32_t *prop, double *dS, float *err, uint8_t *near, hipStream_t st) {
+                  int32_t *prop, double *dS, float *err, uint8_t *near, bool exact, bool mh,
+                  double beta, hipStream_t st) {
   if (BS <= 0) return;
   NodalArgs a;
   a.lo = lo; a.BS = BS;
@@ -982,6 +1196,7 @@ void launch_nodal(const GraphDev &g, LabView lab, const BMView &m, const int32_t
   a.prop = prop; a.dS = dS;
   a.hoff = g.hoff; a.hbits_o = g.hbits_o; a.hbits_i = g.hbits_i; a.hpool = g.hpool;
   a.err = err; a.near = near;
+  a.sloop = g.sloop; a.beta = beta;
   NodalLists L;
   const int per_block[NCLS] = {16, 8, 4, 1};
   int64_t total = 0;
@@ -995,10 +1210,20 @@ void launch_nodal(const GraphDev &g, LabView lab, const BMView &m, const int32_t
   if (total == 0) return;
   size_t lds = (size_t)4 * 6 * 4 * LCAP;  // four waves' tables
   if (L.nb[3] > 0 && big_lds > lds) lds = big_lds;
-  if (err)
-    k_nodal_fused<true><<<(unsigned)total, 256, lds, st>>>(a, L);
+  const dim3 grid((unsigned)total);
+  // the mixed form supports the greedy rule only (the host rejects mixed with MH)
+  if (err && !exact)
+    k_nodal_fused<true, 0><<<grid, 256, lds, st>>>(a, L);
+  else if (err)
+    k_nodal_fused<true, VAR_EXACT><<<grid, 256, lds, st>>>(a, L);
+  else if (!exact && !mh)
+    k_nodal_fused<false, 0><<<grid, 256, lds, st>>>(a, L);
+  else if (!mh)
+    k_nodal_fused<false, VAR_EXACT><<<grid, 256, lds, st>>>(a, L);
+  else if (!exact)
+    k_nodal_fused<false, VAR_MH><<<grid, 256, lds, st>>>(a, L);
   else
-    k_nodal_fused<false><<<(unsigned)total, 256, lds, st>>>(a, L);
+    k_nodal_fused<false, VAR_EXACT | VAR_MH><<<grid, 256, lds, st>>>(a, L);
 }
 
 // Mixed-form statistics of one batch: max |dS32 - dS64| (float bits, order-preserving for
@@ -1038,15 +1263,15 @@ void launch_mix_stats(const float *err, const uint8_t *near, int64_t n, uint32_t
 }
 
 __global__ void k_apply(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo,
-                        int64_t BS) {
+                        int64_t BS, int all) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < BS && dS[i] < 0.0) lab[lo + i] = prop[i];
+  if (i < BS && (all || dS[i] < 0.0)) lab[lo + i] = prop[i];
 }
 
 void launch_apply(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo, int64_t BS,
-                  hipStream_t st) {
+                  bool all, hipStream_t st) {
   if (BS <= 0) return;
-  k_apply<<<grid_for(BS, 256), 256, 0, st>>>(lab, prop, dS, lo, BS);
+  k_apply<<<grid_for(BS, 256), 256, 0, st>>>(lab, prop, dS, lo, BS, all ? 1 : 0);
 }
 
 // ---------------------------------------------------------- canonical sums
@@ -1102,11 +1327,14 @@ void launch_mdl_terms(const BMView &m, const int32_t *erow, int64_t n_max, doubl
 
 // ------------------------------------------------------------- block merges
 // One group of G lanes per proposal (r, i): the proposal (kernels.hpp:726-856, every lane
-// computes it) and the merge delta-S (kernels.hpp:143-449, s-side column fixed).
+// computes it) and the merge delta-S (kernels.hpp:143-449, s-side column fixed).  Exact
+// form: the merged row and column hold t outside {r, s}, the four entries of rows and
+// columns r, s fold into one with the merged degrees, and a proposal of r itself gets the
+// largest double.
 template <int G>
 __global__ __launch_bounds__(256) void k_merge(BMView m, const double *OUT, RngKey key,
                                                uint32_t phase, int32_t np, int32_t *prop,
-                                               double *dSout) {
+                                               double *dSout, int exact) {
   log2_tables_to_lds();
   const int64_t idx = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
   const int32_t B = m.B;
@@ -1123,6 +1351,13 @@ __global__ __launch_bounds__(256) void k_merge(BMView m, const double *OUT, RngK
     const int64_t degu = (int64_t)m.dout[u] + m.din[u];
     if (coin_random(x.x3, degu, B)) s = (int32_t)mulhi(x.x0, B);
     else s = group_walk<G>(m, u, mulhi(x.x2, degu), r, 0);
+  }
+  if (exact && s == r) {
+    if (q == 0) {
+      prop[idx] = s;
+      dSout[idx] = 1.7976931348623157e308;
+    }
+    return;
   }
   // old column terms excluding t in {r, s}
   double inx_r, inx_s;
@@ -1158,12 +1393,14 @@ __global__ __launch_bounds__(256) void k_merge(BMView m, const double *OUT, RngK
     const int32_t b0 = m.rowptr[r], n = m.rowptr[r + 1] - b0;
     seg_for<G>(n, [&](int32_t k, int j) {
       const int32_t t = m.col[b0 + k];
+      if (exact && (t == r || t == s)) return;
       na.p[j] += term_s((int64_t)m.val[b0 + k] + bm_row_get(m, s, t), m.ldin[t], ldo);
     });
     if (s != r) {
       const int32_t c0 = m.rowptr[s], c = m.rowptr[s + 1] - c0;
       seg_for<G>(c, [&](int32_t k, int j) {
         const int32_t t = m.col[c0 + k];
+        if (exact && (t == r || t == s)) return;
         if (bm_row_get(m, r, t) == 0) nb.p[j] += term_s(m.val[c0 + k], m.ldin[t], ldo);
       });
     }
@@ -1173,6 +1410,12 @@ __global__ __launch_bounds__(256) void k_merge(BMView m, const double *OUT, RngK
   }
   // merged column, t not in {r, s}
   const double ldi = clog2_s((double)((int64_t)m.din[r] + m.din[s]));
+  if (exact) {
+    // the folded corner M_rr + M_rs + M_sr + M_ss
+    const int64_t X = (int64_t)bm_row_get(m, r, r) + bm_row_get(m, r, s) + bm_row_get(m, s, r) +
+                      bm_row_get(m, s, s);
+    new_out = new_out + (X > 0 ? term_s(X, ldi, ldo) : 0.0);
+  }
   double new_in;
   {
     CAcc<G> ia, ib;
@@ -1207,16 +1450,17 @@ __global__ __launch_bounds__(256) void k_merge(BMView m, const double *OUT, RngK
 }
 
 void launch_merge(const BMView &m, const double *OUT, RngKey key, uint32_t phase, int32_t np,
-                  int32_t *prop, double *dS, int32_t avg_row, hipStream_t st) {
+                  int32_t *prop, double *dS, int32_t avg_row, bool exact, hipStream_t st) {
   const int64_t n = (int64_t)m.B * np;
+  const int ex = exact ? 1 : 0;
   if (avg_row <= 12)
-    k_merge<8><<<grid_for(n, 32), 256, 0, st>>>(m, OUT, key, phase, np, prop, dS);
+    k_merge<8><<<grid_for(n, 32), 256, 0, st>>>(m, OUT, key, phase, np, prop, dS, ex);
   else if (avg_row <= 24)
-    k_merge<16><<<grid_for(n, 16), 256, 0, st>>>(m, OUT, key, phase, np, prop, dS);
+    k_merge<16><<<grid_for(n, 16), 256, 0, st>>>(m, OUT, key, phase, np, prop, dS, ex);
   else if (avg_row <= 48)
-    k_merge<32><<<grid_for(n, 8), 256, 0, st>>>(m, OUT, key, phase, np, prop, dS);
+    k_merge<32><<<grid_for(n, 8), 256, 0, st>>>(m, OUT, key, phase, np, prop, dS, ex);
   else
-    k_merge<64><<<grid_for(n, 4), 256, 0, st>>>(m, OUT, key, phase, np, prop, dS);
+    k_merge<64><<<grid_for(n, 4), 256, 0, st>>>(m, OUT, key, phase, np, prop, dS, ex);
 }
 
 // find_best_warp (kernels.hpp:452-491): 32 lanes padded with (+MAX, 0), shuffle-down

@@ -47,6 +47,20 @@ extern "C" {
 #define GSAP_MODEL_REFERENCE 0
 #define GSAP_MODEL_CORRECTED 1
 
+/* delta-S of nodal moves and block merges */
+#define GSAP_DELTA_REFERENCE 0 /* as the reference computes it (kernels.hpp:233-320,
+                                  1444-1726: new rows keep the old column degrees of r and s,
+                                  a merged row keeps columns r and s apart) */
+#define GSAP_DELTA_EXACT 1     /* the true change of the description length: every entry of
+                                  the touched rows and columns with its new degrees, merged
+                                  columns folded, self loops moved with the vertex */
+
+/* acceptance of nodal moves */
+#define GSAP_ACCEPT_GREEDY 0   /* accept iff delta-S < 0 (kernels.hpp:600-643) */
+#define GSAP_ACCEPT_MH 1       /* Metropolis-Hastings: accept with probability
+                                  min(1, exp(-beta dS) p(v: s->r) / p(v: r->s)), the proposal
+                                  probabilities of the nodal sampler before and after the move */
+
 typedef struct gsap_ctx gsap_ctx;
 
 /* A directed graph in two CSR views (out-adjacency and in-adjacency).  Ids are
@@ -89,7 +103,9 @@ typedef struct gsap_options {
                                  the sweep kernel by kernel */
   int32_t tile_max_B;         /* row-tile rebuild for 128 < B <= this (dense B only;
                                  0 = auto, 2048); above it the segmented rebuild */
-  int32_t reserved[2];
+  int32_t delta_mode;         /* GSAP_DELTA_* (nodal moves and merges) */
+  int32_t accept;             /* GSAP_ACCEPT_* (nodal moves; MH needs precision FP64) */
+  double beta;                /* inverse temperature of GSAP_ACCEPT_MH (3.0) */
 } gsap_options;
 
 /* Per-call report.  Times are seconds of host wall clock around device work. */

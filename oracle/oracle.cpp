@@ -313,8 +313,12 @@ int32_t merge_propose(const BM &m, const Rng &rng, uint32_t phase, int32_t r, in
   return walk_block(m, u, mulhi(x[2], degu), r, 0);
 }
 
-// delta-S of merging r into s (kernels.hpp:143-449 with Q9 fixed, Q10 kept)
-double merge_dS(const BM &m, const std::vector<double> &OUT, int32_t r, int32_t s) {
+// delta-S of merging r into s (kernels.hpp:143-449 with Q9 fixed, Q10 kept).  Exact form:
+// the merged row and column hold t outside {r, s}, and the four entries of rows and columns
+// r, s fold into one, M_rr + M_rs + M_sr + M_ss, with the merged row and column degrees; a
+// proposal of r itself changes nothing and gets the largest double (never preferred).
+double merge_dS(const BM &m, const std::vector<double> &OUT, int32_t r, int32_t s, bool exact) {
+  if (exact && s == r) return std::numeric_limits<double>::max();
   // old column terms excluding t in {r, s}
   auto col_old = [&](int32_t c) {
     CSum cs;
@@ -330,16 +334,22 @@ double merge_dS(const BM &m, const std::vector<double> &OUT, int32_t r, int32_t 
   CSum na, nb;
   for (int64_t k = m.rowptr[r]; k < m.rowptr[r + 1]; ++k) {
     int32_t t = m.col[k];
+    if (exact && (t == r || t == s)) continue;
     na.add(k - m.rowptr[r], term(m.val[k] + m.row_get(s, t), m.ldin[t], ldo));
   }
   if (s != r)
     for (int64_t k = m.rowptr[s]; k < m.rowptr[s + 1]; ++k) {
       int32_t t = m.col[k];
+      if (exact && (t == r || t == s)) continue;
       if (m.row_get(r, t) == 0) nb.add(k - m.rowptr[s], term(m.val[k], m.ldin[t], ldo));
     }
   double new_out = na.result() + nb.result();
   // merged column, t not in {r, s}
   double ldi = clog2((double)(m.din[r] + m.din[s]));
+  if (exact) {
+    const int64_t X = m.row_get(r, r) + m.row_get(r, s) + m.row_get(s, r) + m.row_get(s, s);
+    new_out = new_out + (X > 0 ? term(X, ldi, ldo) : 0.0);
+  }
   CSum ia, ib;
   for (int64_t k = m.colptr[r]; k < m.colptr[r + 1]; ++k) {
     int32_t t = m.row[k];
@@ -429,7 +439,7 @@ int32_t merge_phase(const G &g, const orc_options &o, const Rng &rng, int32_t *l
     for (int i = 0; i < np; ++i) {
       int64_t idx = (int64_t)r * np + i;
       prop[idx] = merge_propose(m, rng, phase, r, np, i);
-      dS[idx] = merge_dS(m, OUT, r, prop[idx]);
+      dS[idx] = merge_dS(m, OUT, r, prop[idx], o.delta_mode == 1);
     }
   std::vector<int32_t> bst(B);
   std::vector<double> bds(B);
@@ -465,8 +475,25 @@ struct Scratch {
 // ("slots": v's out-blocks, r and s for the rows; v's in-blocks outside {r, s} for the
 // columns) are evaluated term by term.  Slot order is the first-occurrence position in v's
 // out (in) adjacency, then r, then s.
+//
+// Exact form (delta_mode 1): the new rows r and s use the new column degrees of r and s, and
+// a self loop of weight sig moves with v, from (r, r) to (s, s):
+//   R_t = M_rt - a_t - [t=r](c_r - sig) + [t=s](c_r - sig)
+//   S_t = M_st + a_t - [t=r](c_s + sig) + [t=s](c_s + sig)
+// Every entry of rows r, s and columns r, s then enters with its new degrees, so dS is the
+// change of S.
+//
+// With *log2H set, also the log2 Hastings ratio of Metropolis-Hastings acceptance: the
+// nodal sampler proposes s for v with probability
+//   p(r -> s) = sum_t (k_t / k_v) (M_ts + M_st + 1) / (d_t + B)
+// (neighbour block t with probability k_t / k_v, then a uniform block with probability
+// B / (d_t + B), else a block of t's row and column weighted by M; d_t = d_out_t + d_in_t).
+// The sum runs over v's adjacency entries in order (out entries, then in entries, each in
+// canonical order); the reverse proposal p(s -> r) uses the state after the move (M', d',
+// and the self loop's neighbour block s).  log2H = log2 p(s -> r) - log2 p(r -> s); 0 for a
+// vertex without edges (both proposals uniform).
 double nodal_vertex(const G &g, const BM &m, const int32_t *lab, int64_t v, int32_t r, int32_t s,
-                    Scratch &sc) {
+                    Scratch &sc, bool exact, double *log2H) {
   if (r == s) return 0.0;
   ++sc.stamp;
   const int64_t st = sc.stamp;
@@ -485,6 +512,10 @@ double nodal_vertex(const G &g, const BM &m, const int32_t *lab, int64_t v, int3
   }
   const int64_t c_r = C(r), c_s = C(s);
   const int64_t kout = g.kout[v], kin = g.kin[v];
+  int64_t sig = 0;
+  if (exact)
+    for (int64_t j = o0; j < o1; ++j)
+      if (g.oadj[j] == v) sig += g.ow[j];
   const double l_dr = clog2((double)(m.dout[r] - kout)), l_ds = clog2((double)(m.dout[s] + kout));
   const double l_ir = clog2((double)(m.din[r] - kin)), l_is = clog2((double)(m.din[s] + kin));
 
@@ -497,10 +528,11 @@ double nodal_vertex(const G &g, const BM &m, const int32_t *lab, int64_t v, int3
     const int64_t Mr = m.row_get(r, t), Ms = m.row_get(s, t), a = A(t);
     const double or_ = Mr > 0 ? term(Mr, m.ldout[r], m.ldin[t]) : 0.0;
     const double os_ = Ms > 0 ? term(Ms, m.ldout[s], m.ldin[t]) : 0.0;
-    const int64_t R = Mr - a - (t == r ? c_r : 0) + (t == s ? c_r : 0);
-    const int64_t S = Ms + a - (t == r ? c_s : 0) + (t == s ? c_s : 0);
-    const double nr = R > 0 ? term(R, m.ldin[t], l_dr) : 0.0;
-    const double ns = S > 0 ? term(S, m.ldin[t], l_ds) : 0.0;
+    const int64_t R = Mr - a - (t == r ? c_r - sig : 0) + (t == s ? c_r - sig : 0);
+    const int64_t S = Ms + a - (t == r ? c_s + sig : 0) + (t == s ? c_s + sig : 0);
+    const double ln = !exact ? m.ldin[t] : t == r ? l_ir : t == s ? l_is : m.ldin[t];
+    const double nr = R > 0 ? term(R, ln, l_dr) : 0.0;
+    const double ns = S > 0 ? term(S, ln, l_ds) : 0.0;
     drow.add(k, ((or_ + os_) - nr) - ns);
     sum_r += Mr;
     sum_s += Ms;
@@ -544,7 +576,50 @@ double nodal_vertex(const G &g, const BM &m, const int32_t *lab, int64_t v, int3
   d = d + dcol.result();
   d = d - Y2_r;
   d = d - Y2_s;
+
+  if (log2H) {
+    const int64_t Bq = m.B, kv = kout + kin;
+    const int64_t Mrs = m.row_get(r, s), Msr = m.row_get(s, r), Mss = m.row_get(s, s);
+    // M'_rr, M'_rs, M'_sr after the move
+    const int64_t Rr = m.row_get(r, r) - A(r) - c_r + sig, Rs = Mrs - A(s) + c_r - sig;
+    const int64_t Sr = Msr + A(r) - c_s - sig;
+    const int64_t dr = m.dout[r] + m.din[r], ds = m.dout[s] + m.din[s];
+    auto hterms = [&](int32_t u, int64_t w, double &fw, double &bk) {
+      const int32_t t = lab[u];
+      int64_t nf, df, nb, db;
+      if (t == r) { nf = Mrs + Msr + 1; df = dr + Bq; }
+      else if (t == s) { nf = 2 * Mss + 1; df = ds + Bq; }
+      else { nf = m.col_get(s, t) + m.row_get(s, t) + 1; df = m.dout[t] + m.din[t] + Bq; }
+      if (u == v || t == s) { nb = Sr + Rs + 1; db = ds + kv + Bq; }      // t' = s
+      else if (t == r) { nb = 2 * Rr + 1; db = dr - kv + Bq; }             // t' = r
+      else { nb = (m.col_get(r, t) - C(t)) + (m.row_get(r, t) - A(t)) + 1; db = df; }
+      fw = ((double)w * (double)nf) / (double)df;
+      bk = ((double)w * (double)nb) / (double)db;
+    };
+    CSum fo, fi, bo, bi;
+    for (int64_t j = 0; j < no; ++j) {
+      double f, b;
+      hterms(g.oadj[o0 + j], g.ow[o0 + j], f, b);
+      fo.add(j, f);
+      bo.add(j, b);
+    }
+    for (int64_t j = 0; j < ni; ++j) {
+      double f, b;
+      hterms(g.iadj[i0 + j], g.iw[i0 + j], f, b);
+      fi.add(j, f);
+      bi.add(j, b);
+    }
+    const double pf = fo.result() + fi.result(), pb = bo.result() + bi.result();
+    *log2H = no + ni == 0 ? 0.0 : clog2(pb) - clog2(pf);
+  }
   return d;
+}
+
+// Metropolis-Hastings decision: accept iff u < exp(-beta dS) H, compared in log2 with
+// u = (x + 1) 2^-32 from the vertex's third Philox word.
+bool mh_accepts(double d, double log2H, double beta, uint32_t x) {
+  const double L = ((-beta) * d) * 1.4426950408889634 + log2H;
+  return clog2((double)x + 1.0) - 32.0 < L;
 }
 
 void nodal_batch(const G &g, const orc_options &o, const Rng &rng, int32_t *lab, const BM &m,
@@ -591,13 +666,22 @@ void nodal_batch(const G &g, const orc_options &o, const Rng &rng, int32_t *lab,
       int32_t u = lab[nbr];
       s = coin[u] ? (int32_t)mulhi(x[0], B) : sof[u];
     }
-    ps[i] = s;
-    double d = nodal_vertex(g, m, lab, v, lab[v], s, tsc);
-    ds[i] = d < 0.0 ? d : 0.0;
+    const bool mh = o.accept == 1;
+    double lh = 0.0;
+    double d = nodal_vertex(g, m, lab, v, lab[v], s, tsc, o.delta_mode == 1, mh ? &lh : nullptr);
+    if (!mh) {
+      ps[i] = s;
+      ds[i] = d < 0.0 ? d : 0.0;
+    } else {
+      const bool acc = s != lab[v] && mh_accepts(d, lh, o.beta, x[2]);
+      ps[i] = acc ? s : lab[v];
+      ds[i] = acc ? d : 0.0;
+    }
   }
   });
+  // greedy: accepted moves are those with dS < 0; MH: prop holds every vertex's new label
   for (int64_t i = 0; i < BS; ++i)
-    if (ds[i] < 0.0) lab[lo + i] = ps[i];
+    if (o.accept == 1 || ds[i] < 0.0) lab[lo + i] = ps[i];
   if (prop) std::copy(ps.begin(), ps.end(), prop);
   if (dS) std::copy(ds.begin(), ds.end(), dS);
   *dE = csum_array(ds.data(), BS);
@@ -721,6 +805,24 @@ void orc_nodal_batch_mt(const orc_graph *gg, const orc_options *o, int32_t *labe
   nodal_batch(g, *o, rng, labels, m, b, q, prop, dS, dE, sc, threads);
 }
 
+double orc_nodal_delta(const orc_graph *gg, const int32_t *labels, int32_t B, int64_t v,
+                       int32_t s, int32_t exact, double *log2H) {
+  G g(gg);
+  BM m;
+  build_bm(g.N, g.optr, g.oadj, g.ow, labels, B, m);
+  Scratch sc;
+  sc.ensure(B);
+  if (log2H) *log2H = 0.0;
+  return nodal_vertex(g, m, labels, v, labels[v], s, sc, exact != 0, log2H);
+}
+
+double orc_merge_delta(const orc_graph *gg, const int32_t *labels, int32_t B, int32_t r,
+                       int32_t s, int32_t exact) {
+  BM m;
+  build_bm(gg->N, gg->out_ptr, gg->out_adj, gg->out_w, labels, B, m);
+  return merge_dS(m, row_terms(m), r, s, exact != 0);
+}
+
 double orc_mdl(const orc_graph *gg, const int32_t *labels, int32_t B, int32_t model) {
   BM m;
   build_bm(gg->N, gg->out_ptr, gg->out_adj, gg->out_w, labels, B, m);
@@ -779,10 +881,11 @@ int orc_partition(const orc_graph *gg, const orc_options *o, int32_t *labels_out
     for (int itr = 0; itr < o->max_nodal_itr; ++itr) {
       for (int b = 0; b < o->num_batches; ++b) {
         double e = 0.0;
+        std::vector<int32_t> before(lab.begin() + (int64_t)b * BS, lab.begin() + (int64_t)(b + 1) * BS);
         nodal_batch(g, *o, rng, lab.data(), m, b, batch_ctr, bp.data(), bd.data(), &e, sc);
         build_bm(N, g.optr, g.oadj, g.ow, lab.data(), B, m);
-        int64_t moved = 0;  // accepted moves of the batch
-        for (int64_t i = 0; i < BS; ++i) moved += bd[i] < 0.0;
+        int64_t moved = 0;  // vertices of the batch whose label changed
+        for (int64_t i = 0; i < BS; ++i) moved += lab[(int64_t)b * BS + i] != before[i];
         rec(2, B, itr, b, fnv(bp.data(), 4 * BS), fnv(lab.data(), 4 * N), e, (double)moved);
         ++batch_ctr; ++batches;
         dE[itr] = dE[itr] + e;

@@ -68,6 +68,9 @@ typedef struct orc_options {
   int32_t max_levels;        /* safety cap on outer iterations */
   uint32_t chain;            /* Philox stream id (counter word 2) */
   int32_t model;             /* 0: model term as partition.hpp:696-699, 1: x ln x scaled by E */
+  int32_t delta_mode;        /* 0: delta-S as kernels.hpp computes it, 1: exact change of S */
+  int32_t accept;            /* 0: greedy dS < 0 (kernels.hpp:600-643), 1: Metropolis-Hastings */
+  double beta;               /* inverse temperature of accept = 1 */
 } orc_options;
 
 typedef struct orc_trace_rec {
@@ -96,7 +99,8 @@ int32_t orc_merge_phase(const orc_graph *g, const orc_options *o, int32_t *label
                         int32_t *best, double *dS_best);
 
 /* Single nodal batch b (0..num_batches-1) at batch counter q.  Writes
- * prop[BS], dS[BS] (0 for rejected moves), updates labels in place and
+ * prop[BS] (greedy: the proposals; Metropolis-Hastings: the label each vertex
+ * ends with), dS[BS] (0 for rejected moves), updates labels in place and
  * returns the canonical sum of accepted delta-S in *dE. */
 void orc_nodal_batch(const orc_graph *g, const orc_options *o, int32_t *labels, int32_t B,
                      int32_t b, uint32_t q, int32_t *prop, double *dS, double *dE);
@@ -105,6 +109,15 @@ void orc_nodal_batch(const orc_graph *g, const orc_options *o, int32_t *labels, 
 void orc_nodal_batch_mt(const orc_graph *g, const orc_options *o, int32_t *labels, int32_t B,
                         int32_t b, uint32_t q, int32_t *prop, double *dS, double *dE,
                         int32_t threads);
+
+/* delta-S of moving vertex v from its block to s against the labelling's block matrix
+ * (exact = 0: as kernels.hpp computes it; 1: the change of S).  With log2H, also the log2
+ * Hastings ratio log2 p(s -> r) - log2 p(r -> s) of the nodal sampler. */
+double orc_nodal_delta(const orc_graph *g, const int32_t *labels, int32_t B, int64_t v,
+                       int32_t s, int32_t exact, double *log2H);
+/* delta-S of merging block r into block s (exact as above). */
+double orc_merge_delta(const orc_graph *g, const int32_t *labels, int32_t B, int32_t r,
+                       int32_t s, int32_t exact);
 
 /* Total description length of a labelling. */
 double orc_mdl(const orc_graph *g, const int32_t *labels, int32_t B, int32_t model);

@@ -29,7 +29,8 @@ class OOptions(C.Structure):
                 ("reduction_rate", C.c_float), ("max_nodal_itr", C.c_int32),
                 ("num_batches", C.c_int32), ("threshold1", C.c_double),
                 ("threshold2", C.c_double), ("moving_avg_window", C.c_int32),
-                ("max_levels", C.c_int32), ("chain", C.c_uint32), ("model", C.c_int32)]
+                ("max_levels", C.c_int32), ("chain", C.c_uint32), ("model", C.c_int32),
+                ("delta_mode", C.c_int32), ("accept", C.c_int32), ("beta", C.c_double)]
 
 
 class OTrace(C.Structure):
@@ -73,6 +74,12 @@ def lib():
         L.orc_nodal_batch_mt.argtypes = [C.POINTER(OGraph), C.POINTER(OOptions), _i32p, C.c_int32,
                                          C.c_int32, C.c_uint32, _i32p, _f64p,
                                          C.POINTER(C.c_double), C.c_int32]
+        L.orc_nodal_delta.restype = C.c_double
+        L.orc_nodal_delta.argtypes = [C.POINTER(OGraph), _i32p, C.c_int32, C.c_int64, C.c_int32,
+                                      C.c_int32, C.POINTER(C.c_double)]
+        L.orc_merge_delta.restype = C.c_double
+        L.orc_merge_delta.argtypes = [C.POINTER(OGraph), _i32p, C.c_int32, C.c_int32, C.c_int32,
+                                      C.c_int32]
         L.orc_mdl.restype = C.c_double
         L.orc_mdl.argtypes = [C.POINTER(OGraph), _i32p, C.c_int32, C.c_int32]
         L.orc_partition.restype = C.c_int
@@ -101,7 +108,7 @@ def ograph(g):
 def ooptions(o):
     return OOptions(o.seed, o.num_proposals, o.reduction_rate, o.max_nodal_itr, o.num_batches,
                     o.threshold1, o.threshold2, o.moving_avg_window, o.max_levels, o.chain,
-                    o.model)
+                    o.model, o.delta_mode, o.accept, o.beta)
 
 
 def philox(ctr, key):
@@ -176,6 +183,21 @@ def nodal_batch_mt(g, opts, labels, B, b, q, threads):
     lib().orc_nodal_batch_mt(C.byref(og), C.byref(oo), lab, B, b, q, prop, dS, C.byref(dE),
                              threads)
     return dict(labels=lab, prop=prop[:BS], dS=dS[:BS], dE=dE.value)
+
+
+def nodal_delta(g, labels, B, v, s, exact=0):
+    """(delta-S, log2 Hastings ratio) of moving v to block s."""
+    og = ograph(g)
+    lh = C.c_double()
+    d = lib().orc_nodal_delta(C.byref(og), np.ascontiguousarray(labels, np.int32), B, v, s,
+                              exact, C.byref(lh))
+    return d, lh.value
+
+
+def merge_delta(g, labels, B, r, s, exact=0):
+    og = ograph(g)
+    return lib().orc_merge_delta(C.byref(og), np.ascontiguousarray(labels, np.int32), B, r, s,
+                                 exact)
 
 
 def mdl(g, labels, B, model=0):

@@ -8,7 +8,7 @@ import pytest
 
 import gsap_amd
 import oracle_lib as O
-from conftest import random_graph, tsv_graph
+from conftest import hub_graph, random_graph, tsv_graph
 from gsap_amd import Options
 
 pytestmark = pytest.mark.gpu
@@ -19,23 +19,6 @@ def ctx():
     c = gsap_amd.Context(0)
     yield c
     c.close()
-
-
-def hub_graph(seed=5, N=600, hub_deg=300):
-    """Random graph plus two hubs with many in/out neighbours (large LDS tables; beyond 1024
-    neighbours, the global hash-table pool)."""
-    rng = np.random.default_rng(seed)
-    src, dst = [], []
-    seen = set()
-    def add(a, b):
-        if a != b and (a, b) not in seen:
-            seen.add((a, b)); src.append(a); dst.append(b)
-    for _ in range(3000):
-        add(int(rng.integers(N)), int(rng.integers(N)))
-    for h in (3, 250):
-        for t in rng.choice(N, hub_deg, replace=False):
-            add(h, int(t)); add(int(t), h)
-    return gsap_amd.Graph.from_edges(N, src, dst)
 
 
 CASES = {
