@@ -169,6 +169,8 @@ __global__ __launch_bounds__(1024) void k_small_bm(const int32_t *D, int32_t B, 
 // ---- vertices grouped by block (counting sort; order inside a block is irrelevant)
 constexpr int VT = 2048;  // vertices per workgroup
 constexpr int TILE_MAX_B = 8192;  // largest B of the row-tile path
+constexpr int INC_MAX_B = 4096;   // largest B of the incremental dense update
+constexpr int INC_MIN_B = 512;    // its global-atomic form runs above this B
 
 __global__ __launch_bounds__(256) void k_label_hist(LabView lab, int64_t N, int32_t B,
                                                     uint32_t *H, int32_t W) {
@@ -1000,6 +1002,23 @@ static void seg_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
 // 128 < B <= tile_max_B (dense): vertices grouped by block; workgroup (row tile, chunk) counts
 // the out-edges of its rows' vertices in LDS and writes a partial plane; the planes are
 // summed into D; CSR from D's rows, CSC from the rows of D's transpose.
+// CSR, CSC, degrees and their logs from the dense matrix D = bm.dense (B x B); the
+// transpose goes through the partial-plane buffer (B x B words)
+static void dense_to_csr(int32_t B, BMBuf &bm, RebuildScratch &rs, hipStream_t st) {
+  const int32_t *D = bm.dense;
+  uint32_t *cnt = rs.su;
+  k_dense_rows<<<B, 64, 0, st>>>(D, B, cnt, bm.dout);
+  scan_exclusive(cnt, (uint32_t *)bm.rowptr, B, (uint32_t *)(bm.rowptr + B), rs.tmp, st);
+  k_dense_write_rows<<<B, 64, 0, st>>>(D, B, bm.rowptr, bm.col, bm.val, bm.erow);
+  int32_t *DT = rs.partials;
+  const unsigned tb = (unsigned)((B + 31) / 32);
+  k_transpose<<<dim3(tb, tb), 256, 0, st>>>(D, B, DT);
+  k_dense_rows<<<B, 64, 0, st>>>(DT, B, cnt, bm.din);
+  scan_exclusive(cnt, (uint32_t *)bm.colptr, B, (uint32_t *)(bm.colptr + B), rs.tmp, st);
+  k_dense_write_rows<<<B, 64, 0, st>>>(DT, B, bm.colptr, bm.row, bm.cval, nullptr);
+  k_logs<<<grid_for(B, 256), 256, 0, st>>>(bm.dout, bm.din, B, bm.ldout, bm.ldin);
+}
+
 static void tile_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
                       RebuildScratch &rs, hipStream_t st) {
   const int64_t N = g.N;
@@ -1030,16 +1049,7 @@ static void tile_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
     k_hist_reduce<8><<<grid_for(BB * 8, T), T, 0, st>>>(rs.partials, C, BB, D);
   else
     k_hist_reduce<64><<<grid_for(BB * 64, T), T, 0, st>>>(rs.partials, C, BB, D);
-  k_dense_rows<<<B, 64, 0, st>>>(D, B, cnt, bm.dout);
-  scan_exclusive(cnt, (uint32_t *)bm.rowptr, B, (uint32_t *)(bm.rowptr + B), rs.tmp, st);
-  k_dense_write_rows<<<B, 64, 0, st>>>(D, B, bm.rowptr, bm.col, bm.val, bm.erow);
-  int32_t *DT = rs.partials;
-  const unsigned tb = (unsigned)((B + 31) / 32);
-  k_transpose<<<dim3(tb, tb), 256, 0, st>>>(D, B, DT);
-  k_dense_rows<<<B, 64, 0, st>>>(DT, B, cnt, bm.din);
-  scan_exclusive(cnt, (uint32_t *)bm.colptr, B, (uint32_t *)(bm.colptr + B), rs.tmp, st);
-  k_dense_write_rows<<<B, 64, 0, st>>>(DT, B, bm.colptr, bm.row, bm.cval, nullptr);
-  k_logs<<<grid_for(B, 256), 256, 0, st>>>(bm.dout, bm.din, B, bm.ldout, bm.ldin);
+  dense_to_csr(B, bm, rs, st);
 }
 
 static void small_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
@@ -1062,6 +1072,110 @@ static void small_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
     k_hist_reduce<64><<<grid_for(BB * 64, T), T, 0, st>>>(rs.partials, C, BB, D);
   k_small_bm<<<1, 1024, 0, st>>>(D, B, bm.rowptr, bm.col, bm.val, bm.erow, bm.colptr, bm.row,
                                  bm.cval, bm.dout, bm.din, bm.ldout, bm.ldin);
+}
+
+// ---- incremental update of the dense matrix after a nodal batch
+// One group of 16 lanes per vertex of the batch [lo, lo + BS); olab holds the batch's labels
+// before the batch was applied.  Every edge with a moved endpoint leaves its old cell and
+// enters its new one, counted once: the out-edges of a moved vertex (target labels old and
+// new), and the in-edges of a moved vertex whose source did not move.
+__global__ __launch_bounds__(256) void k_dense_moves(const int32_t *optr, const int32_t *oadj,
+                                                     const int32_t *ow, const int32_t *iptr,
+                                                     const int32_t *iadj, const int32_t *iw,
+                                                     const int32_t *lab, const int32_t *olab,
+                                                     int64_t lo, int64_t BS, int32_t B,
+                                                     int32_t *D) {
+  const int64_t i = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+  const int q = threadIdx.x & 15;
+  if (i >= BS) return;
+  const int64_t x = lo + i;
+  const int32_t xo = olab[i], xn = lab[x];
+  if (xo == xn) return;
+  for (int32_t j = optr[x] + q; j < optr[x + 1]; j += 16) {
+    const int32_t y = oadj[j], w = ow[j];
+    const int64_t k = (int64_t)y - lo;
+    const int32_t yn = lab[y];
+    const int32_t yo = (k >= 0 && k < BS) ? olab[k] : yn;
+    atomicSub(&D[(int64_t)xo * B + yo], w);
+    atomicAdd(&D[(int64_t)xn * B + yn], w);
+  }
+  for (int32_t j = iptr[x] + q; j < iptr[x + 1]; j += 16) {
+    const int32_t y = iadj[j], w = iw[j];  // edge y -> x
+    const int64_t k = (int64_t)y - lo;
+    const int32_t yl = lab[y];
+    if (k >= 0 && k < BS && olab[k] != yl) continue;  // y moved: one of y's out-edges
+    atomicSub(&D[(int64_t)yl * B + xo], w);
+    atomicAdd(&D[(int64_t)yl * B + xn], w);
+  }
+}
+
+// B <= SMALL_B: the same edge moves summed in an LDS copy of the B x B matrix per workgroup
+// (a grid-stride loop over the batch, 32 vertices per pass), then added to D with one global
+// atomic per changed cell: a few blocks would otherwise funnel every move into the same
+// cells.
+__global__ __launch_bounds__(512) void k_dense_moves_lds(const int32_t *optr, const int32_t *oadj,
+                                                         const int32_t *ow, const int32_t *iptr,
+                                                         const int32_t *iadj, const int32_t *iw,
+                                                         const int32_t *lab, const int32_t *olab,
+                                                         int64_t lo, int64_t BS, int32_t B,
+                                                         int32_t *D) {
+  extern __shared__ int32_t dl[];
+  const int32_t BB = B * B;
+  for (int32_t k = threadIdx.x; k < BB; k += 512) dl[k] = 0;
+  __syncthreads();
+  const int q = threadIdx.x & 15;
+  for (int64_t i = (int64_t)blockIdx.x * 32 + (threadIdx.x >> 4); i < BS;
+       i += (int64_t)gridDim.x * 32) {
+    const int64_t x = lo + i;
+    const int32_t xo = olab[i], xn = lab[x];
+    if (xo == xn) continue;
+    for (int32_t j = optr[x] + q; j < optr[x + 1]; j += 16) {
+      const int32_t y = oadj[j], w = ow[j];
+      const int64_t k = (int64_t)y - lo;
+      const int32_t yn = lab[y];
+      const int32_t yo = (k >= 0 && k < BS) ? olab[k] : yn;
+      atomicSub(&dl[xo * B + yo], w);
+      atomicAdd(&dl[xn * B + yn], w);
+    }
+    for (int32_t j = iptr[x] + q; j < iptr[x + 1]; j += 16) {
+      const int32_t y = iadj[j], w = iw[j];
+      const int64_t k = (int64_t)y - lo;
+      const int32_t yl = lab[y];
+      if (k >= 0 && k < BS && olab[k] != yl) continue;
+      atomicSub(&dl[yl * B + xo], w);
+      atomicAdd(&dl[yl * B + xn], w);
+    }
+  }
+  __syncthreads();
+  for (int32_t k = threadIdx.x; k < BB; k += 512) {
+    const int32_t d = dl[k];
+    if (d != 0) atomicAdd(&D[k], d);
+  }
+}
+
+bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t *olab, int64_t lo,
+                         int64_t BS, int32_t B, BMBuf &bm, RebuildScratch &rs, hipStream_t st) {
+  // the dense buffer must hold the block matrix of the labels before the batch; between
+  // SMALL_B and INC_MIN_B neither form of the update beats the full rebuild (LDS cannot hold
+  // the matrix, global atomics contend on few cells)
+  if (!(bm.dense_valid && bm.dense_B == B && bm.B == B && B <= INC_MAX_B &&
+        (B <= SMALL_B || B > INC_MIN_B) && (int64_t)B * B <= rs.partials_cap))
+    return false;
+  if (BS > 0 && B <= SMALL_B) {
+    const unsigned grid = (unsigned)std::min<int64_t>(256, (BS + 31) / 32);
+    k_dense_moves_lds<<<grid, 512, (size_t)B * B * 4, st>>>(
+        g.optr, g.oadj, g.ow, g.iptr, g.iadj, g.iw, lab32, olab, lo, BS, B, bm.dense);
+  } else if (BS > 0) {
+    k_dense_moves<<<(unsigned)((BS + 15) / 16), 256, 0, st>>>(
+        g.optr, g.oadj, g.ow, g.iptr, g.iadj, g.iw, lab32, olab, lo, BS, B, bm.dense);
+  }
+  if (B <= 65535 && rs.lab16) k_lab16<<<grid_for(g.N, 256), 256, 0, st>>>(lab32, g.N, rs.lab16);
+  if (B <= SMALL_B)
+    k_small_bm<<<1, 1024, 0, st>>>(bm.dense, B, bm.rowptr, bm.col, bm.val, bm.erow, bm.colptr,
+                                   bm.row, bm.cval, bm.dout, bm.din, bm.ldout, bm.ldin);
+  else
+    dense_to_csr(B, bm, rs, st);
+  return true;
 }
 
 void rebuild_blockmodel(const GraphDev &g, const int32_t *lab32, int32_t B, BMBuf &bm,

@@ -72,6 +72,13 @@ void rebuild_blockmodel(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf 
                         RebuildScratch &rs, int32_t dense_max_B, int32_t tile_max_B,
                         hipStream_t st);
 
+// After a nodal batch over [lo, lo + BS) (olab: the batch's labels before it), the block
+// matrix by moving the moved vertices' edges between cells of the dense matrix, when that
+// holds the matrix of the labels before the batch (dense B, previous rebuild at this B).
+// Returns false (nothing launched) otherwise.
+bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t *olab, int64_t lo,
+                         int64_t BS, int32_t B, BMBuf &bm, RebuildScratch &rs, hipStream_t st);
+
 // ---- sweep.hip
 struct RngKey {
   uint32_t k0, k1, chain;
@@ -112,7 +119,7 @@ void launch_mix_stats(const float *err, const uint8_t *near, int64_t n, uint32_t
 // labels of the batch: greedy (all = false) where dS < 0; Metropolis-Hastings (all = true)
 // prop holds every vertex's new label
 void launch_apply(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo, int64_t BS,
-                  bool all, hipStream_t st);
+                  bool all, int32_t *olab, hipStream_t st);
 // canonical array sum: tiles of 1024 (n read from n_dev when non-null, else n_host)
 void launch_csum_array(const double *x, const int32_t *n_dev, int64_t n_host, int64_t n_max,
                        double *tiles, double *out, hipStream_t st);

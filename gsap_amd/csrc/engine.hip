@@ -157,6 +157,8 @@ struct gsap_ctx {
   DBuf ctr, mk0, mk1, mv0, mv1;
   // mixed form: per-vertex |dS32 - dS64| and evaluation class; batch statistics
   DBuf errv, evcls, mixstat;
+  // the current batch's labels before it was applied (incremental dense rebuild)
+  DBuf olab;
   // per batch, the batch's vertices split by neighbour-count class (nodal_class)
   DBuf vlist;
   std::vector<int64_t> list_off;  // [batch * 4 + class] -> offset into vlist; last = total
@@ -399,6 +401,7 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   c->ctr.ensure(8);
   c->mk0.ensure(8 * N); c->mk1.ensure(8 * N); c->mv0.ensure(4 * N); c->mv1.ensure(4 * N);
   c->errv.ensure(4 * N); c->evcls.ensure(N); c->mixstat.ensure(32);
+  c->olab.ensure(4 * N);
   BMBuf &b = c->bm;
   b.rowptr = c->rowptr.as<int32_t>(); b.colptr = c->colptr.as<int32_t>();
   b.col = c->col.as<int32_t>(); b.val = c->val.as<int32_t>(); b.erow = c->erow.as<int32_t>();
@@ -427,6 +430,19 @@ void rebuild(gsap_ctx *c, int32_t B) {
   int p = c->prof.begin(1, c->st);
   rebuild_blockmodel(c->g, c->lab.as<int32_t>(), B, c->bm, c->rs, c->dense_max_B, c->tile_max_B,
                      c->st);
+  c->prof.end(p, c->st);
+  HIPX(hipGetLastError());
+}
+
+// The rebuild after nodal batch b: the moved vertices' edges moved between cells of the
+// dense matrix when it holds the matrix of the labels before the batch, else a full rebuild.
+void rebuild_after_batch(gsap_ctx *c, int32_t B, const gsap_options &o, int32_t b) {
+  int p = c->prof.begin(1, c->st);
+  const int64_t BS = c->N / o.num_batches, lo = (int64_t)b * BS;
+  if (!rebuild_after_moves(c->g, c->lab.as<int32_t>(), c->olab.as<int32_t>(), lo, BS, B, c->bm,
+                           c->rs, c->st))
+    rebuild_blockmodel(c->g, c->lab.as<int32_t>(), B, c->bm, c->rs, c->dense_max_B,
+                       c->tile_max_B, c->st);
   c->prof.end(p, c->st);
   HIPX(hipGetLastError());
 }
@@ -591,7 +607,7 @@ void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, PhaseRef phase, 
   c->prof.end(p, st);
   p = c->prof.begin(5, st);
   launch_apply(c->lab.as<int32_t>(), c->prop.as<int32_t>(), c->dS.as<double>(), lo, BS,
-               o.accept == GSAP_ACCEPT_MH, st);
+               o.accept == GSAP_ACCEPT_MH, c->olab.as<int32_t>(), st);
   launch_csum_array(c->dS.as<double>(), nullptr, BS, BS, c->tiles.as<double>(), dE_dev, st);
   c->prof.end(p, st);
   HIPX(hipGetLastError());
@@ -730,7 +746,7 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
         if (!gg.exec) gg.capture([&] {
           for (int b = 0; b < o.num_batches; ++b) {
             nodal_batch(c, o, b, PhaseRef{ctr, (uint32_t)b}, ar_level, res + 1 + b);
-            rebuild(c, B);
+            rebuild_after_batch(c, B, o, b);
           }
           launch_mdl(c, res);
           launch_add_u32(ctr, (uint32_t)o.num_batches, st);
@@ -746,7 +762,7 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
           HIPX(hipMemcpyAsync(hbefore.data(), lab_b, 4 * (size_t)BS, hipMemcpyDeviceToHost, st));
         }
         nodal_batch(c, o, b, PhaseRef{ctr, (uint32_t)b}, ar, res + 1 + b);
-        rebuild(c, B);
+        rebuild_after_batch(c, B, o, b);
         if (tr) {
           double e = 0;
           hprop.resize(BS);
@@ -982,7 +998,7 @@ int gsap_destroy(gsap_ctx *c) {
                  &c->lists, &c->ncl, &c->cntc, &c->vrel, &c->sseg, &c->lab16, &c->vlist,
                  &c->OUT, &c->sof, &c->coin, &c->prop, &c->dS, &c->best, &c->dSbest, &c->map,
                  &c->terms, &c->tiles, &c->res, &c->ctr, &c->mk0, &c->mk1, &c->mv0,
-                 &c->mv1, &c->errv, &c->evcls, &c->mixstat};
+                 &c->mv1, &c->errv, &c->evcls, &c->mixstat, &c->olab};
   for (DBuf *b : all) b->release();
   try { c->prof.destroy(); } catch (...) {}
   (void)hipStreamDestroy(c->st);

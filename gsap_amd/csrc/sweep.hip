@@ -1262,16 +1262,19 @@ void launch_mix_stats(const float *err, const uint8_t *near, int64_t n, uint32_t
   k_mix_stats<<<grid, 256, 0, st>>>(err, near, n, max_bits, count);
 }
 
+// labels of the batch (olab, when given, keeps the previous ones)
 __global__ void k_apply(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo,
-                        int64_t BS, int all) {
+                        int64_t BS, int all, int32_t *olab) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < BS && (all || dS[i] < 0.0)) lab[lo + i] = prop[i];
+  if (i >= BS) return;
+  if (olab) olab[i] = lab[lo + i];
+  if (all || dS[i] < 0.0) lab[lo + i] = prop[i];
 }
 
 void launch_apply(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo, int64_t BS,
-                  bool all, hipStream_t st) {
+                  bool all, int32_t *olab, hipStream_t st) {
   if (BS <= 0) return;
-  k_apply<<<grid_for(BS, 256), 256, 0, st>>>(lab, prop, dS, lo, BS, all ? 1 : 0);
+  k_apply<<<grid_for(BS, 256), 256, 0, st>>>(lab, prop, dS, lo, BS, all ? 1 : 0, olab);
 }
 
 // ---------------------------------------------------------- canonical sums
