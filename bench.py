@@ -39,7 +39,8 @@ KCLASS = ["nodal", "rebuild", "block_terms", "merge", "mdl", "accept_sum"]
 KERNELS_OF = {
     "nodal": ["k_nodal"],
     "rebuild": ["k_label_", "k_tile_hist", "k_edge_hist", "k_hist_reduce", "k_dense_",
-                "k_small_bm", "k_transpose", "k_logs", "k_seg_", "k_scan_", "k_fill_i32"],
+                "k_small_bm", "k_transpose", "k_logs", "k_seg_", "k_scan_", "k_fill_i32",
+                "k_lab16"],
     "block_terms": ["k_block_terms"],
     "merge": ["k_merge", "k_argmin", "k_order_keys", "k_radix_"],
 }
@@ -358,8 +359,10 @@ def main():
     achieved = abytes / (avg_ms * 1e-3) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
-        kn = json.load(open(TRAFFIC_FILE)).get("kernels", {}).get(
-            "k_nodal_fused<true>" if prec == gsap_amd.PRECISION_MIXED else "k_nodal_fused<false>")
+        # the default-variant instance of the nodal kernel (reference delta, greedy accept)
+        want = ("k_nodal_fused<true, 0>" if prec == gsap_amd.PRECISION_MIXED
+                else "k_nodal_fused<false, 0>")
+        kn = json.load(open(TRAFFIC_FILE)).get("kernels", {}).get(want)
         if kn:
             traffic = kn["read_bytes_per_call"] + kn["write_bytes_per_call"]
     p, r = gsap_amd.pairwise(g.truth, lab)
