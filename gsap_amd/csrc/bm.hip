@@ -1169,7 +1169,7 @@ bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t 
     k_dense_moves<<<(unsigned)((BS + 15) / 16), 256, 0, st>>>(
         g.optr, g.oadj, g.ow, g.iptr, g.iadj, g.iw, lab32, olab, lo, BS, B, bm.dense);
   }
-  if (B <= 65535 && rs.lab16) k_lab16<<<grid_for(g.N, 256), 256, 0, st>>>(lab32, g.N, rs.lab16);
+  // (the 16-bit label copy was kept current by the batch's apply)
   if (B <= SMALL_B)
     k_small_bm<<<1, 1024, 0, st>>>(bm.dense, B, bm.rowptr, bm.col, bm.val, bm.erow, bm.colptr,
                                    bm.row, bm.cval, bm.dout, bm.din, bm.ldout, bm.ldin);

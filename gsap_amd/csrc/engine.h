@@ -116,15 +116,19 @@ void launch_nodal(const GraphDev &g, LabView lab, const BMView &m, const int32_t
                   double beta, hipStream_t st);
 void launch_mix_stats(const float *err, const uint8_t *near, int64_t n, uint32_t *max_bits,
                       unsigned long long *count, hipStream_t st);
-// labels of the batch: greedy (all = false) where dS < 0; Metropolis-Hastings (all = true)
-// prop holds every vertex's new label
-void launch_apply(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo, int64_t BS,
-                  bool all, int32_t *olab, hipStream_t st);
+// Labels of a nodal batch (greedy, all = false: where dS < 0; Metropolis-Hastings, all = true:
+// prop holds every vertex's new label; olab keeps the old ones, lab16 when given follows)
+// and the canonical sum of the batch's dS into *out.
+void launch_apply_sum(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo, int64_t BS,
+                      bool all, int32_t *olab, uint16_t *lab16, double *tiles, double *out,
+                      hipStream_t st);
 // canonical array sum: tiles of 1024 (n read from n_dev when non-null, else n_host)
 void launch_csum_array(const double *x, const int32_t *n_dev, int64_t n_host, int64_t n_max,
                        double *tiles, double *out, hipStream_t st);
-void launch_mdl_terms(const BMView &m, const int32_t *erow, int64_t n_max, double *terms,
-                      hipStream_t st);
+// description-length data term summed in canonical order into *out; ctr (when given)
+// advanced by add afterwards
+void launch_mdl_sum(const BMView &m, const int32_t *erow, int64_t n_max, double *tiles,
+                    double *out, uint32_t *ctr, uint32_t add, hipStream_t st);
 void launch_merge(const BMView &m, const double *OUT, RngKey key, uint32_t phase, int32_t np,
                   int32_t *prop, double *dS, int32_t avg_row, bool exact, hipStream_t st);
 void launch_argmin(const double *dS, const int32_t *prop, int32_t B, int32_t np, int32_t *best,

@@ -152,7 +152,7 @@ struct gsap_ctx {
   // rebuild scratch
   DBuf k0, k1, tmp, partials, soff, su, suoff, lists, ncl, cntc, vrel, sseg, lab16;
   // sweep / merge
-  DBuf OUT, sof, coin, prop, dS, best, dSbest, map, terms, tiles, res;
+  DBuf OUT, sof, coin, prop, dS, best, dSbest, map, tiles, res;
   // device batch counter (Philox phase of the sweep's first batch); merge-order sort
   DBuf ctr, mk0, mk1, mv0, mv1;
   // mixed form: per-vertex |dS32 - dS64| and evaluation class; batch statistics
@@ -396,7 +396,7 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   c->OUT.ensure(8 * N); c->sof.ensure(4 * N); c->coin.ensure(N);
   c->prop.ensure(4 * (size_t)np * N); c->dS.ensure(8 * (size_t)np * N);
   c->best.ensure(4 * N); c->dSbest.ensure(8 * N); c->map.ensure(4 * N);
-  c->terms.ensure(8 * E); c->tiles.ensure(8 * (std::max(E, N) / 1024 + 2));
+  c->tiles.ensure(8 * (std::max(E, N) / 1024 + 2));
   c->res.ensure(8 * 64);
   c->ctr.ensure(8);
   c->mk0.ensure(8 * N); c->mk1.ensure(8 * N); c->mv0.ensure(4 * N); c->mv1.ensure(4 * N);
@@ -471,12 +471,10 @@ double model_S(double data_S, int64_t N, int64_t E, int32_t B, int32_t model_kin
   return model - data_S;
 }
 
-void launch_mdl(gsap_ctx *c, double *out) {
+void launch_mdl(gsap_ctx *c, double *out, uint32_t *ctr = nullptr, uint32_t add = 0) {
   int p = c->prof.begin(4, c->st);
   BMView m = c->bm.view();
-  launch_mdl_terms(m, c->bm.erow, c->E, c->terms.as<double>(), c->st);
-  launch_csum_array(c->terms.as<double>(), c->bm.rowptr + c->bm.B, 0, c->E,
-                    c->tiles.as<double>(), out, c->st);
+  launch_mdl_sum(m, c->bm.erow, c->E, c->tiles.as<double>(), out, ctr, add, c->st);
   c->prof.end(p, c->st);
 }
 
@@ -606,9 +604,11 @@ void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, PhaseRef phase, 
   }
   c->prof.end(p, st);
   p = c->prof.begin(5, st);
-  launch_apply(c->lab.as<int32_t>(), c->prop.as<int32_t>(), c->dS.as<double>(), lo, BS,
-               o.accept == GSAP_ACCEPT_MH, c->olab.as<int32_t>(), st);
-  launch_csum_array(c->dS.as<double>(), nullptr, BS, BS, c->tiles.as<double>(), dE_dev, st);
+  // the 16-bit label copy follows the batch's changes (the rebuild after the batch then
+  // finds it current)
+  launch_apply_sum(c->lab.as<int32_t>(), c->prop.as<int32_t>(), c->dS.as<double>(), lo, BS,
+                   o.accept == GSAP_ACCEPT_MH, c->olab.as<int32_t>(),
+                   c->bm.B <= 65535 ? c->rs.lab16 : nullptr, c->tiles.as<double>(), dE_dev, st);
   c->prof.end(p, st);
   HIPX(hipGetLastError());
 }
@@ -748,8 +748,7 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
             nodal_batch(c, o, b, PhaseRef{ctr, (uint32_t)b}, ar_level, res + 1 + b);
             rebuild_after_batch(c, B, o, b);
           }
-          launch_mdl(c, res);
-          launch_add_u32(ctr, (uint32_t)o.num_batches, st);
+          launch_mdl(c, res, ctr, (uint32_t)o.num_batches);
         });
         HIPX(hipGraphLaunch(gg.exec, st));
         batch_ctr += (uint32_t)o.num_batches;
@@ -779,8 +778,7 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
         ++batches;
       }
       if (!use_graph) {
-        launch_mdl(c, res);
-        launch_add_u32(ctr, (uint32_t)o.num_batches, st);
+        launch_mdl(c, res, ctr, (uint32_t)o.num_batches);
       }
       double h[64];
       int32_t nnz = 0;
@@ -997,7 +995,7 @@ int gsap_destroy(gsap_ctx *c) {
                  &c->k1, &c->tmp, &c->partials, &c->soff, &c->su, &c->suoff,
                  &c->lists, &c->ncl, &c->cntc, &c->vrel, &c->sseg, &c->lab16, &c->vlist,
                  &c->OUT, &c->sof, &c->coin, &c->prop, &c->dS, &c->best, &c->dSbest, &c->map,
-                 &c->terms, &c->tiles, &c->res, &c->ctr, &c->mk0, &c->mk1, &c->mv0,
+                 &c->tiles, &c->res, &c->ctr, &c->mk0, &c->mk1, &c->mv0,
                  &c->mv1, &c->errv, &c->evcls, &c->mixstat, &c->olab};
   for (DBuf *b : all) b->release();
   try { c->prof.destroy(); } catch (...) {}

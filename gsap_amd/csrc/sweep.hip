@@ -1262,20 +1262,6 @@ void launch_mix_stats(const float *err, const uint8_t *near, int64_t n, uint32_t
   k_mix_stats<<<grid, 256, 0, st>>>(err, near, n, max_bits, count);
 }
 
-// labels of the batch (olab, when given, keeps the previous ones)
-__global__ void k_apply(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo,
-                        int64_t BS, int all, int32_t *olab) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= BS) return;
-  if (olab) olab[i] = lab[lo + i];
-  if (all || dS[i] < 0.0) lab[lo + i] = prop[i];
-}
-
-void launch_apply(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo, int64_t BS,
-                  bool all, int32_t *olab, hipStream_t st) {
-  if (BS <= 0) return;
-  k_apply<<<grid_for(BS, 256), 256, 0, st>>>(lab, prop, dS, lo, BS, all ? 1 : 0, olab);
-}
 
 // ---------------------------------------------------------- canonical sums
 __global__ __launch_bounds__(64) void k_tile_sums(const double *x, const int32_t *n_dev,
@@ -1294,39 +1280,103 @@ __global__ __launch_bounds__(64) void k_tile_sums(const double *x, const int32_t
   if (lane == 0) tiles[blockIdx.x] = p;
 }
 
+// the sum of the tile sums (ctr, when given, advanced by add afterwards: the sweep's batch
+// counter, stream-ordered after the sweep's last read of it)
 __global__ __launch_bounds__(64) void k_final_sum(const double *tiles, const int32_t *n_dev,
-                                                  int64_t n_host, double *out) {
+                                                  int64_t n_host, double *out, uint32_t *ctr,
+                                                  uint32_t add) {
   const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
   const int64_t nt = (n + 1023) / 1024;
   const int lane = threadIdx.x;
   double p = 0.0;
   for (int64_t k = lane; k < nt; k += 64) p += tiles[k];
   p = wave_sum(p);
-  if (lane == 0) *out = p;
+  if (lane == 0) {
+    *out = p;
+    if (ctr) *ctr += add;
+  }
 }
 
 void launch_csum_array(const double *x, const int32_t *n_dev, int64_t n_host, int64_t n_max,
                        double *tiles, double *out, hipStream_t st) {
   const int64_t nt = (n_max + 1023) / 1024;
   if (nt > 0) k_tile_sums<<<(unsigned)nt, 64, 0, st>>>(x, n_dev, n_host, tiles);
-  k_final_sum<<<1, 64, 0, st>>>(tiles, n_dev, n_host, out);
+  k_final_sum<<<1, 64, 0, st>>>(tiles, n_dev, n_host, out, nullptr, 0);
 }
 
-// data term of every block-matrix entry, CSR order (kernels.hpp:1066-1093)
-__global__ void k_mdl_terms(BMView m, const int32_t *erow, double *terms) {
-  log2_tables_to_lds();
-  const int32_t nnz = m.rowptr[m.B];
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
-       k += (int64_t)gridDim.x * blockDim.x)
-    terms[k] = term_s(m.val[k], m.ldout[erow[k]], m.ldin[m.col[k]]);
+// Labels of a nodal batch and the tile sums of its delta-S, one 1024-entry tile per
+// 256-thread workgroup (four entries per thread, then wave 0 sums the tile from LDS in the
+// canonical order): greedy (all = 0) takes prop where dS < 0, Metropolis-Hastings (all = 1)
+// takes prop; olab keeps the previous labels, lab16 (when given) follows the changed ones.
+__global__ __launch_bounds__(256) void k_apply_tiles(int32_t *lab, const int32_t *prop,
+                                                     const double *dS, int64_t lo, int64_t BS,
+                                                     int all, int32_t *olab, uint16_t *lab16,
+                                                     double *tiles) {
+  __shared__ double xs[1024];
+  const int64_t t0 = (int64_t)blockIdx.x * 1024;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int k = (int)threadIdx.x + 256 * j;
+    const int64_t i = t0 + k;
+    double d = 0.0;
+    if (i < BS) {
+      d = dS[i];
+      const int32_t old = lab[lo + i];
+      if (olab) olab[i] = old;
+      if (all || d < 0.0) {
+        const int32_t nl = prop[i];
+        lab[lo + i] = nl;
+        if (lab16) lab16[lo + i] = (uint16_t)nl;
+      }
+    }
+    xs[k] = d;
+  }
+  __syncthreads();
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
+  double p = 0.0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j)
+    if (t0 + lane + 64 * j < BS) p += xs[lane + 64 * j];
+  p = wave_sum(p);
+  if (lane == 0) tiles[blockIdx.x] = p;
 }
 
-void launch_mdl_terms(const BMView &m, const int32_t *erow, int64_t n_max, double *terms,
+void launch_apply_sum(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo, int64_t BS,
+                      bool all, int32_t *olab, uint16_t *lab16, double *tiles, double *out,
                       hipStream_t st) {
-  int64_t g = grid_for(n_max, 256);
-  if (g > 16384) g = 16384;
-  k_mdl_terms<<<(unsigned)g, 256, 0, st>>>(m, erow, terms);
+  const int64_t nt = (BS + 1023) / 1024;
+  if (nt > 0)
+    k_apply_tiles<<<(unsigned)nt, 256, 0, st>>>(lab, prop, dS, lo, BS, all ? 1 : 0, olab, lab16,
+                                                tiles);
+  k_final_sum<<<1, 64, 0, st>>>(tiles, nullptr, BS, out, nullptr, 0);
 }
+
+// data term of every block-matrix entry in CSR order (kernels.hpp:1066-1093), summed per
+// 1024-entry tile (one wave each)
+__global__ __launch_bounds__(64) void k_mdl_tiles(BMView m, const int32_t *erow, double *tiles) {
+  const int64_t n = m.rowptr[m.B];
+  const int64_t t0 = (int64_t)blockIdx.x * 1024;
+  if (t0 >= n) return;
+  log2_tables_to_lds();
+  const int lane = threadIdx.x;
+  double p = 0.0;
+#pragma unroll 4
+  for (int j = 0; j < 16; ++j) {
+    const int64_t k = t0 + lane + 64 * j;
+    if (k < n) p += term_s(m.val[k], m.ldout[erow[k]], m.ldin[m.col[k]]);
+  }
+  p = wave_sum(p);
+  if (lane == 0) tiles[blockIdx.x] = p;
+}
+
+void launch_mdl_sum(const BMView &m, const int32_t *erow, int64_t n_max, double *tiles,
+                    double *out, uint32_t *ctr, uint32_t add, hipStream_t st) {
+  const int64_t nt = (n_max + 1023) / 1024;
+  if (nt > 0) k_mdl_tiles<<<(unsigned)nt, 64, 0, st>>>(m, erow, tiles);
+  k_final_sum<<<1, 64, 0, st>>>(tiles, m.rowptr + m.B, 0, out, ctr, add);
+}
+
 
 // ------------------------------------------------------------- block merges
 // One group of G lanes per proposal (r, i): the proposal (kernels.hpp:726-856, every lane
