@@ -1385,13 +1385,9 @@ void launch_mdl_sum(const BMView &m, const int32_t *erow, int64_t n_max, double 
 // columns r, s fold into one with the merged degrees, and a proposal of r itself gets the
 // largest double.
 template <int G>
-__global__ __launch_bounds__(256) void k_merge(BMView m, const double *OUT, RngKey key,
-                                               uint32_t phase, int32_t np, int32_t *prop,
-                                               double *dSout, int exact) {
-  log2_tables_to_lds();
-  const int64_t idx = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
+GSAP_D void merge_one(const BMView &m, const double *OUT, RngKey key, uint32_t phase,
+                      int32_t np, int64_t idx, int32_t *prop, double *dSout, int exact) {
   const int32_t B = m.B;
-  if (idx >= (int64_t)B * np) return;
   const int q = threadIdx.x & (G - 1);
   const int32_t r = (int32_t)(idx / np);
   const Philox x = philox((uint32_t)idx, phase, key.chain, TAG_MERGE, key.k0, key.k1);
@@ -1502,10 +1498,183 @@ __global__ __launch_bounds__(256) void k_merge(BMView m, const double *OUT, RngK
   }
 }
 
+template <int G>
+__global__ __launch_bounds__(256) void k_merge(BMView m, const double *OUT, RngKey key,
+                                               uint32_t phase, int32_t np, int32_t *prop,
+                                               double *dSout, int exact) {
+  log2_tables_to_lds();
+  const int64_t idx = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
+  if (idx >= (int64_t)m.B * np) return;
+  merge_one<G>(m, OUT, key, phase, np, idx, prop, dSout, exact);
+}
+
+// position of t in the sorted keys[0, n) (LDS), or -1
+GSAP_D int32_t lds_find(const int32_t *keys, int32_t n, int32_t t) {
+  int32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (keys[mid] < t) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < n && keys[lo] == t) ? lo : -1;
+}
+
+// Block merges with row r and column r staged in LDS: one 256-thread workgroup per block r,
+// one wave per proposal (r, i), i = wave, wave + 4, ...  Row s is streamed once: each of its
+// entries is found in row r by binary search in LDS (absent: a row-s-only term of the merged
+// row; present: its value M_st lands at that position of the wave's match array, read by the
+// row-r terms); column s likewise against column r.  The random lookups of merge_one become
+// LDS work.  Same positions, terms and canonical sums as merge_one, hence the same bits; a
+// block whose row and column exceed CAP entries uses merge_one.
+template <int CAP>
+__global__ __launch_bounds__(256) void k_merge_blk(BMView m, const double *OUT, RngKey key,
+                                                   uint32_t phase, int32_t np, int32_t *prop,
+                                                   double *dSout, int exact) {
+  __shared__ int32_t sk[CAP], sv[CAP];  // row r (columns, values), then column r (rows, values)
+  __shared__ int32_t mt[4][CAP];        // per wave: M_st at row r's positions, M_ts at column r's
+  log2_tables_to_lds();
+  const int32_t r = (int32_t)blockIdx.x;
+  const int32_t B = m.B;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int32_t r0 = m.rowptr[r], nr = m.rowptr[r + 1] - r0;
+  const int32_t c0 = m.colptr[r], nc = m.colptr[r + 1] - c0;
+  const int32_t n = nr + nc;
+  if (n > CAP) {
+    for (int32_t i = wv; i < np; i += 4)
+      merge_one<64>(m, OUT, key, phase, np, (int64_t)r * np + i, prop, dSout, exact);
+    return;
+  }
+  for (int32_t k = threadIdx.x; k < nr; k += 256) {
+    sk[k] = m.col[r0 + k];
+    sv[k] = m.val[r0 + k];
+  }
+  for (int32_t k = threadIdx.x; k < nc; k += 256) {
+    sk[nr + k] = m.row[c0 + k];
+    sv[nr + k] = m.cval[c0 + k];
+  }
+  __syncthreads();
+  const int32_t *rk = sk, *ck = sk + nr;
+  int32_t *mw = mt[wv];
+  const int64_t deg = (int64_t)m.dout[r] + m.din[r];
+  const double li_r = m.ldin[r];
+  for (int32_t i = wv; i < np; i += 4) {
+    const int64_t idx = (int64_t)r * np + i;
+    const Philox x = philox((uint32_t)idx, phase, key.chain, TAG_MERGE, key.k0, key.k1);
+    int32_t s;
+    if (deg == 0) {
+      s = (int32_t)mulhi(x.x0, B);
+    } else {
+      // u: first entry of row r then column r (staged) whose running weight exceeds target
+      const uint64_t target = mulhi(x.x1, deg);
+      int32_t u = 0;
+      uint64_t base = 0;
+      for (int32_t c = 0; c < n; c += 64) {
+        const int32_t k = c + lane;
+        uint64_t incl = k < n ? (uint64_t)sv[k] : 0;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const uint64_t y = __shfl_up(incl, off, 64);
+          if (lane >= off) incl += y;
+        }
+        const uint64_t bal = __ballot(k < n && base + incl > target);
+        if (bal) {
+          u = sk[c + __ffsll((long long)bal) - 1];
+          break;
+        }
+        base += __shfl(incl, 63, 64);
+      }
+      const int64_t degu = (int64_t)m.dout[u] + m.din[u];
+      if (coin_random(x.x3, degu, B)) s = (int32_t)mulhi(x.x0, B);
+      else s = group_walk<64>(m, u, mulhi(x.x2, degu), r, 0);
+    }
+    if (exact && s == r) {
+      if (lane == 0) {
+        prop[idx] = s;
+        dSout[idx] = 1.7976931348623157e308;
+      }
+      continue;
+    }
+    wave_lds_sync();  // the previous proposal's reads of the match array are done
+    for (int32_t k = lane; k < n; k += 64) mw[k] = s == r ? sv[k] : 0;
+    wave_lds_sync();
+    const double ldo = clog2_s((double)((int64_t)m.dout[r] + m.dout[s]));
+    const double ldi = clog2_s((double)((int64_t)m.din[r] + m.din[s]));
+    const double li_s = m.ldin[s];
+    // row s: the terms of entries absent from row r, and M_st at row r's positions
+    double p_nb = 0.0;
+    if (s != r) {
+      const int32_t s0 = m.rowptr[s], ns = m.rowptr[s + 1] - s0;
+      for (int32_t k = lane; k < ns; k += 64) {
+        const int32_t t = m.col[s0 + k], v = m.val[s0 + k];
+        const int32_t pos = lds_find(rk, nr, t);
+        if (pos >= 0) mw[pos] = v;
+        else if (!(exact && (t == r || t == s))) p_nb += term_s(v, m.ldin[t], ldo);
+      }
+    }
+    // column s: its old terms (t not in {r, s}), the terms of entries absent from column r,
+    // and M_ts at column r's positions
+    double p_inxs = 0.0, p_ib = 0.0;
+    {
+      const int32_t cs0 = m.colptr[s], ncs = m.colptr[s + 1] - cs0;
+      for (int32_t k = lane; k < ncs; k += 64) {
+        const int32_t t = m.row[cs0 + k], v = m.cval[cs0 + k];
+        const bool ex = t == r || t == s;
+        if (!ex) p_inxs += term_s(v, m.ldout[t], li_s);
+        if (s != r) {
+          const int32_t pos = lds_find(ck, nc, t);
+          if (pos >= 0) mw[nr + pos] = v;
+          else if (!ex) p_ib += term_s(v, m.ldout[t], ldi);
+        }
+      }
+    }
+    wave_lds_sync();
+    // merged row over row r's entries, old column r and merged column over column r's
+    double p_na = 0.0, p_inxr = 0.0, p_ia = 0.0;
+    for (int32_t k = lane; k < nr; k += 64) {
+      const int32_t t = rk[k];
+      if (exact && (t == r || t == s)) continue;
+      p_na += term_s((int64_t)sv[k] + mw[k], m.ldin[t], ldo);
+    }
+    for (int32_t k = lane; k < nc; k += 64) {
+      const int32_t t = ck[k];
+      if (t == r || t == s) continue;
+      const int32_t v = sv[nr + k];
+      p_inxr += term_s(v, m.ldout[t], li_r);
+      p_ia += term_s((int64_t)v + mw[nr + k], m.ldout[t], ldi);
+    }
+    const double inx_r = wave_sum(p_inxr), inx_s = wave_sum(p_inxs);
+    const double a_ = wave_sum(p_na), b_ = wave_sum(p_nb);
+    const double ia_ = wave_sum(p_ia), ib_ = wave_sum(p_ib);
+    if (lane == 0) {
+      double new_out = a_ + b_;
+      if (exact) {
+        const int64_t X = (int64_t)bm_row_get(m, r, r) + bm_row_get(m, r, s) +
+                          bm_row_get(m, s, r) + bm_row_get(m, s, s);
+        new_out = new_out + (X > 0 ? term_s(X, ldi, ldo) : 0.0);
+      }
+      const double new_in = ia_ + ib_;
+      double d = OUT[r] + OUT[s];
+      d = d + (inx_r + inx_s);
+      d = d - new_out;
+      d = d - new_in;
+      prop[idx] = s;
+      dSout[idx] = d;
+    }
+  }
+}
+
 void launch_merge(const BMView &m, const double *OUT, RngKey key, uint32_t phase, int32_t np,
                   int32_t *prop, double *dS, int32_t avg_row, bool exact, hipStream_t st) {
   const int64_t n = (int64_t)m.B * np;
   const int ex = exact ? 1 : 0;
+  if (avg_row > 24) {
+    // rows worth staging: a workgroup per block, row and column in LDS
+    if (avg_row <= 96)
+      k_merge_blk<512><<<(unsigned)m.B, 256, 0, st>>>(m, OUT, key, phase, np, prop, dS, ex);
+    else
+      k_merge_blk<2048><<<(unsigned)m.B, 256, 0, st>>>(m, OUT, key, phase, np, prop, dS, ex);
+    return;
+  }
   if (avg_row <= 12)
     k_merge<8><<<grid_for(n, 32), 256, 0, st>>>(m, OUT, key, phase, np, prop, dS, ex);
   else if (avg_row <= 24)

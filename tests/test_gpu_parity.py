@@ -183,6 +183,25 @@ def test_c3_blockmodel_skewed(ctx, c3, B, big, dense):
         assert np.array_equal(got[k], ref[k]), k
 
 
+@pytest.mark.parametrize("B,big,dense,exact", [(3000, 0.3, 0, 0), (3000, 0.0, 0, 1),
+                                               (20000, 0.0, 0, 0), (900, 0.05, 1, 0)])
+def test_c3_merge_phase(ctx, c3, B, big, dense, exact):
+    """Merge phases with long rows (blocks staged in LDS, 512 or 2048 entries) and one block
+    holding a fraction `big` of the vertices (its row and column exceed the staging arrays)."""
+    g = c3
+    rng = np.random.default_rng(21)
+    lab = rng.integers(1, B, g.N).astype(np.int32)
+    lab[rng.random(g.N) < big] = 0
+    o = Options(seed=31, dense_max_B=dense, delta_mode=exact)
+    print(f"C3 merge B={B} big={big} dense={dense} exact={exact}", flush=True)
+    got = ctx.merge_phase(g, o, lab, B, int(B * 0.35), 7)
+    ref = O.merge_phase(g, o, lab, B, int(B * 0.35), 7)
+    assert np.array_equal(got["prop"], ref["prop"])
+    assert eq_bits(got["dS"], ref["dS"])
+    assert np.array_equal(got["best"], ref["best"])
+    assert got["B"] == ref["B"] and np.array_equal(got["labels"], ref["labels"])
+
+
 @pytest.mark.parametrize("B", [300, 2500, 6000])
 @pytest.mark.parametrize("tile", [1, 8192])
 def test_c3_blockmodel_tile_threshold(ctx, c3, B, tile):
