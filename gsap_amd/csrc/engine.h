@@ -129,8 +129,12 @@ void launch_csum_array(const double *x, const int32_t *n_dev, int64_t n_host, in
 // advanced by add afterwards
 void launch_mdl_sum(const BMView &m, const int32_t *erow, int64_t n_max, double *tiles,
                     double *out, uint32_t *ctr, uint32_t add, hipStream_t st);
+// Merge proposals and delta-S of every block.  big_list (B words) and big_count (one word)
+// serve the workgroup-per-block form at large B (it synchronizes the stream once); without
+// them every proposal gets a group of lanes.
 void launch_merge(const BMView &m, const double *OUT, RngKey key, uint32_t phase, int32_t np,
-                  int32_t *prop, double *dS, int32_t avg_row, bool exact, hipStream_t st);
+                  int32_t *prop, double *dS, int32_t avg_row, bool exact, int32_t *big_list,
+                  uint32_t *big_count, hipStream_t st);
 void launch_argmin(const double *dS, const int32_t *prop, int32_t B, int32_t np, int32_t *best,
                    double *dS_best, hipStream_t st);
 void launch_relabel(int32_t *lab, const int32_t *map, int64_t N, hipStream_t st);

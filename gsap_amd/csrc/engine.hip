@@ -509,7 +509,8 @@ int32_t merge_phase(gsap_ctx *c, const gsap_options &o, int32_t B_to_merge, uint
   c->prof.end(p, st);
   p = c->prof.begin(3, st);
   launch_merge(m, c->OUT.as<double>(), rng_key(o), phase, np, c->prop.as<int32_t>(),
-               c->dS.as<double>(), ar, o.delta_mode == GSAP_DELTA_EXACT, st);
+               c->dS.as<double>(), ar, o.delta_mode == GSAP_DELTA_EXACT, c->sof.as<int32_t>(),
+               c->ctr.as<uint32_t>() + 1, st);
   c->prof.end(p, st);
   launch_argmin(c->dS.as<double>(), c->prop.as<int32_t>(), B, np, c->best.as<int32_t>(),
                 c->dSbest.as<double>(), st);

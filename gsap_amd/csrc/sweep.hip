@@ -1508,6 +1508,26 @@ __global__ __launch_bounds__(256) void k_merge(BMView m, const double *OUT, RngK
   merge_one<G>(m, OUT, key, phase, np, idx, prop, dSout, exact);
 }
 
+// blocks whose row and column hold more than cap entries, in any order
+__global__ void k_big_blocks(BMView m, int32_t cap, int32_t *list, uint32_t *count) {
+  const int32_t r = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (r >= m.B) return;
+  if ((m.rowptr[r + 1] - m.rowptr[r]) + (m.colptr[r + 1] - m.colptr[r]) > cap)
+    list[atomicAdd(count, 1u)] = r;
+}
+
+// one wave per proposal of the listed blocks
+__global__ __launch_bounds__(256) void k_merge_list(BMView m, const double *OUT, RngKey key,
+                                                    uint32_t phase, int32_t np, int32_t *prop,
+                                                    double *dSout, int exact,
+                                                    const int32_t *list, int32_t nlist) {
+  log2_tables_to_lds();
+  const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= (int64_t)nlist * np) return;
+  const int32_t r = list[j / np];
+  merge_one<64>(m, OUT, key, phase, np, (int64_t)r * np + j % np, prop, dSout, exact);
+}
+
 // position of t in the sorted keys[0, n) (LDS), or -1
 GSAP_D int32_t lds_find(const int32_t *keys, int32_t n, int32_t t) {
   int32_t lo = 0, hi = n;
@@ -1524,8 +1544,8 @@ GSAP_D int32_t lds_find(const int32_t *keys, int32_t n, int32_t t) {
 // entries is found in row r by binary search in LDS (absent: a row-s-only term of the merged
 // row; present: its value M_st lands at that position of the wave's match array, read by the
 // row-r terms); column s likewise against column r.  The random lookups of merge_one become
-// LDS work.  Same positions, terms and canonical sums as merge_one, hence the same bits; a
-// block whose row and column exceed CAP entries uses merge_one.
+// LDS work.  Same positions, terms and canonical sums as merge_one, hence the same bits; the
+// proposals of a block whose row and column exceed CAP entries are left to k_merge_list.
 template <int CAP>
 __global__ __launch_bounds__(256) void k_merge_blk(BMView m, const double *OUT, RngKey key,
                                                    uint32_t phase, int32_t np, int32_t *prop,
@@ -1539,11 +1559,7 @@ __global__ __launch_bounds__(256) void k_merge_blk(BMView m, const double *OUT, 
   const int32_t r0 = m.rowptr[r], nr = m.rowptr[r + 1] - r0;
   const int32_t c0 = m.colptr[r], nc = m.colptr[r + 1] - c0;
   const int32_t n = nr + nc;
-  if (n > CAP) {
-    for (int32_t i = wv; i < np; i += 4)
-      merge_one<64>(m, OUT, key, phase, np, (int64_t)r * np + i, prop, dSout, exact);
-    return;
-  }
+  if (n > CAP) return;  // listed by k_big_blocks, one wave per proposal
   for (int32_t k = threadIdx.x; k < nr; k += 256) {
     sk[k] = m.col[r0 + k];
     sv[k] = m.val[r0 + k];
@@ -1664,15 +1680,28 @@ __global__ __launch_bounds__(256) void k_merge_blk(BMView m, const double *OUT, 
 }
 
 void launch_merge(const BMView &m, const double *OUT, RngKey key, uint32_t phase, int32_t np,
-                  int32_t *prop, double *dS, int32_t avg_row, bool exact, hipStream_t st) {
+                  int32_t *prop, double *dS, int32_t avg_row, bool exact, int32_t *big_list,
+                  uint32_t *big_count, hipStream_t st) {
   const int64_t n = (int64_t)m.B * np;
   const int ex = exact ? 1 : 0;
-  if (avg_row > 24) {
-    // rows worth staging: a workgroup per block, row and column in LDS
-    if (avg_row <= 96)
+  if (avg_row > 24 && m.B >= 4096 && big_list) {
+    // long rows and enough blocks to fill the device: a workgroup per block with its row and
+    // column in LDS; the few blocks beyond the staging arrays get a wave per proposal (their
+    // count comes back to the host to size that launch)
+    const int32_t cap = avg_row <= 96 ? 512 : 2048;
+    if (cap == 512)
       k_merge_blk<512><<<(unsigned)m.B, 256, 0, st>>>(m, OUT, key, phase, np, prop, dS, ex);
     else
       k_merge_blk<2048><<<(unsigned)m.B, 256, 0, st>>>(m, OUT, key, phase, np, prop, dS, ex);
+    uint32_t nbig = 0;
+    if (hipMemsetAsync(big_count, 0, 4, st) != hipSuccess) return;
+    k_big_blocks<<<grid_for(m.B, 256), 256, 0, st>>>(m, cap, big_list, big_count);
+    if (hipMemcpyAsync(&nbig, big_count, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return;  // the caller's error check reports it
+    if (nbig > 0)
+      k_merge_list<<<grid_for((int64_t)nbig * np, 4), 256, 0, st>>>(
+          m, OUT, key, phase, np, prop, dS, ex, big_list, (int32_t)nbig);
     return;
   }
   if (avg_row <= 12)
