@@ -137,7 +137,7 @@ def cpu_threads():
 
 def cpu_baseline(g, labels, B, seconds, threads):
     """The oracle's nodal batches (each with its block-matrix rebuild) from the GPU's final
-    labelling, for about `seconds` or 64 batches, on `threads` host threads."""
+    labelling, for about `seconds` (at most 1024 batches), on `threads` host threads."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     opts = gsap_amd.Options(seed=1)
@@ -151,7 +151,7 @@ def cpu_baseline(g, labels, B, seconds, threads):
             O.nodal_batch_mt(g, opts, lab, B, nbat % opts.num_batches, 100000 + nbat, threads)
         nbat += 1
         el = time.perf_counter() - t0
-        if el >= seconds or nbat >= 64:
+        if el >= seconds or nbat >= 1024:
             break
     sweeps = nbat / opts.num_batches
     return dict(value=g.E * sweeps / el, unit="edges/s", cores=threads, kind="port",
@@ -277,7 +277,8 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="measure the row-tile and dense-lookup thresholds on the workload "
                          "graph first (untimed) and run with them")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="bounded CPU-oracle sample per thread count (two samples)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 512 MiB device copy (PMC byte-counter calibration)")
