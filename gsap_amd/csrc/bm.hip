@@ -169,7 +169,7 @@ __global__ __launch_bounds__(1024) void k_small_bm(const int32_t *D, int32_t B, 
 // ---- vertices grouped by block (counting sort; order inside a block is irrelevant)
 constexpr int VT = 2048;  // vertices per workgroup
 constexpr int TILE_MAX_B = 8192;  // largest B of the row-tile path
-constexpr int INC_MAX_B = 4096;   // largest B of the incremental dense update
+constexpr int INC_MAX_B = 8192;   // largest B of the incremental dense update
 constexpr int INC_MIN_B = 512;    // its global-atomic form runs above this B
 
 __global__ __launch_bounds__(256) void k_label_hist(LabView lab, int64_t N, int32_t B,
@@ -225,17 +225,29 @@ __global__ __launch_bounds__(512) void k_tile_hist(const int32_t *optr, const in
   for (int32_t i = threadIdx.x; i < nt; i += blockDim.x) out[i] = T[i];
 }
 
-// one wave per row: nonzero count and row sum
-__global__ __launch_bounds__(64) void k_dense_rows(const int32_t *D, int32_t B, uint32_t *rcnt,
-                                                   int32_t *dout) {
-  const int32_t r = blockIdx.x;
-  const int lane = threadIdx.x;
+// One wave per row (four rows per workgroup): nonzero count and row sum.  Each step reads
+// 256 entries with four independent coalesced loads per lane.
+__global__ __launch_bounds__(256) void k_dense_rows(const int32_t *__restrict__ D, int32_t B,
+                                                    uint32_t *__restrict__ rcnt,
+                                                    int32_t *__restrict__ dout) {
+  const int32_t r = (int32_t)(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (r >= B) return;
+  const int lane = threadIdx.x & 63;
+  const int32_t *row = D + (int64_t)r * B;
   uint32_t c = 0;
   int32_t s = 0;
-  for (int32_t t = lane; t < B; t += 64) {
-    int32_t x = D[(int64_t)r * B + t];
-    c += x != 0;
-    s += x;
+  for (int32_t base = 0; base < B; base += 256) {
+    int32_t x[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int32_t t = base + 64 * k + lane;
+      x[k] = t < B ? row[t] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      c += x[k] != 0;
+      s += x[k];
+    }
   }
   for (int off = 32; off >= 1; off >>= 1) {
     c += __shfl_down(c, off, 64);
@@ -244,23 +256,37 @@ __global__ __launch_bounds__(64) void k_dense_rows(const int32_t *D, int32_t B, 
   if (lane == 0) { rcnt[r] = c; dout[r] = s; }
 }
 
-__global__ __launch_bounds__(64) void k_dense_write_rows(const int32_t *D, int32_t B,
-                                                         const int32_t *rowptr, int32_t *col,
-                                                         int32_t *val, int32_t *erow) {
-  const int32_t r = blockIdx.x;
-  const int lane = threadIdx.x;
+// One wave per row: the row's nonzero entries in column order at rowptr[r] (and erow when
+// given); four 64-entry sub-chunks per step, their loads issued before the writes.
+__global__ __launch_bounds__(256) void k_dense_write_rows(const int32_t *__restrict__ D, int32_t B,
+                                                          const int32_t *__restrict__ rowptr,
+                                                          int32_t *__restrict__ col,
+                                                          int32_t *__restrict__ val,
+                                                          int32_t *__restrict__ erow) {
+  const int32_t r = (int32_t)(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (r >= B) return;
+  const int lane = threadIdx.x & 63;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int32_t *row = D + (int64_t)r * B;
   int32_t run = rowptr[r];
-  for (int32_t base = 0; base < B; base += 64) {
-    int32_t t = base + lane;
-    int32_t x = t < B ? D[(int64_t)r * B + t] : 0;
-    uint64_t m = __ballot(x != 0);
-    if (x != 0) {
-      int32_t pos = run + (int32_t)__popcll(m & lt);
-      col[pos] = t; val[pos] = x;
-      if (erow) erow[pos] = r;
+  for (int32_t base = 0; base < B; base += 256) {
+    int32_t x[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int32_t t = base + 64 * k + lane;
+      x[k] = t < B ? row[t] : 0;
     }
-    run += (int32_t)__popcll(m);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint64_t m = __ballot(x[k] != 0);
+      if (x[k] != 0) {
+        const int32_t pos = run + (int32_t)__popcll(m & lt);
+        col[pos] = base + 64 * k + lane;
+        val[pos] = x[k];
+        if (erow) erow[pos] = r;
+      }
+      run += (int32_t)__popcll(m);
+    }
   }
 }
 
@@ -1007,15 +1033,16 @@ static void seg_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
 static void dense_to_csr(int32_t B, BMBuf &bm, RebuildScratch &rs, hipStream_t st) {
   const int32_t *D = bm.dense;
   uint32_t *cnt = rs.su;
-  k_dense_rows<<<B, 64, 0, st>>>(D, B, cnt, bm.dout);
+  const unsigned rb = (unsigned)((B + 3) / 4);
+  k_dense_rows<<<rb, 256, 0, st>>>(D, B, cnt, bm.dout);
   scan_exclusive(cnt, (uint32_t *)bm.rowptr, B, (uint32_t *)(bm.rowptr + B), rs.tmp, st);
-  k_dense_write_rows<<<B, 64, 0, st>>>(D, B, bm.rowptr, bm.col, bm.val, bm.erow);
+  k_dense_write_rows<<<rb, 256, 0, st>>>(D, B, bm.rowptr, bm.col, bm.val, bm.erow);
   int32_t *DT = rs.partials;
   const unsigned tb = (unsigned)((B + 31) / 32);
   k_transpose<<<dim3(tb, tb), 256, 0, st>>>(D, B, DT);
-  k_dense_rows<<<B, 64, 0, st>>>(DT, B, cnt, bm.din);
+  k_dense_rows<<<rb, 256, 0, st>>>(DT, B, cnt, bm.din);
   scan_exclusive(cnt, (uint32_t *)bm.colptr, B, (uint32_t *)(bm.colptr + B), rs.tmp, st);
-  k_dense_write_rows<<<B, 64, 0, st>>>(DT, B, bm.colptr, bm.row, bm.cval, nullptr);
+  k_dense_write_rows<<<rb, 256, 0, st>>>(DT, B, bm.colptr, bm.row, bm.cval, nullptr);
   k_logs<<<grid_for(B, 256), 256, 0, st>>>(bm.dout, bm.din, B, bm.ldout, bm.ldin);
 }
 
