@@ -111,33 +111,44 @@ GSAP_D void seg_for(int32_t n, F &&f) {
 // Group-parallel form of bm_walk: entries of row x then column x in chunks of G lanes, an
 // inclusive prefix sum of the weights inside the group, and the first lane whose running
 // weight exceeds target with an id other than skip.  Same result as the sequential walk.
+// Four chunks' loads are issued together, then the chunks are scanned in order.
 template <int G>
 GSAP_D int32_t group_walk(const BMView &m, int32_t x, uint64_t target, int32_t skip,
                           int32_t fallback) {
+  constexpr int U = 4;
   const int q = threadIdx.x & (G - 1);
   const int gbase = (threadIdx.x & 63) & ~(G - 1);
   const int32_t r0 = m.rowptr[x], nr = m.rowptr[x + 1] - r0;
   const int32_t c0 = m.colptr[x], n = nr + (m.colptr[x + 1] - c0);
   uint64_t base = 0;
-  for (int32_t c = 0; c < n; c += G) {
-    const int32_t k = c + q;
-    uint64_t w = 0;
-    int32_t id = skip;
-    if (k < nr) { w = (uint64_t)m.val[r0 + k]; id = m.col[r0 + k]; }
-    else if (k < n) { w = (uint64_t)m.cval[c0 + k - nr]; id = m.row[c0 + k - nr]; }
-    uint64_t incl = w;
+  for (int32_t c = 0; c < n; c += U * G) {
+    uint32_t w[U];
+    int32_t id[U];
 #pragma unroll
-    for (int off = 1; off < G; off <<= 1) {
-      uint64_t y = __shfl_up(incl, off, G);
-      if (q >= off) incl += y;
+    for (int u = 0; u < U; ++u) {
+      const int32_t k = c + u * G + q;
+      w[u] = 0;
+      id[u] = skip;
+      if (k < nr) { w[u] = (uint32_t)m.val[r0 + k]; id[u] = m.col[r0 + k]; }
+      else if (k < n) { w[u] = (uint32_t)m.cval[c0 + k - nr]; id[u] = m.row[c0 + k - nr]; }
     }
-    const bool pred = k < n && base + incl > target && id != skip;
-    const uint64_t bal = (__ballot(pred) >> gbase) & (G == 64 ? ~0ull : ((1ull << G) - 1ull));
-    if (bal) {
-      const int first = __ffsll((long long)bal) - 1;
-      return __shfl(id, first, G);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int32_t k = c + u * G + q;
+      uint64_t incl = w[u];
+#pragma unroll
+      for (int off = 1; off < G; off <<= 1) {
+        uint64_t y = __shfl_up(incl, off, G);
+        if (q >= off) incl += y;
+      }
+      const bool pred = k < n && base + incl > target && id[u] != skip;
+      const uint64_t bal = (__ballot(pred) >> gbase) & (G == 64 ? ~0ull : ((1ull << G) - 1ull));
+      if (bal) {
+        const int first = __ffsll((long long)bal) - 1;
+        return __shfl(id[u], first, G);
+      }
+      base += __shfl(incl, G - 1, G);
     }
-    base += __shfl(incl, G - 1, G);
   }
   return fallback;
 }
