@@ -43,21 +43,28 @@ GSAP_D uint32_t wave_incl_scan_u32(uint32_t x) {
 // exceeds target (kernels.hpp:916-936 as integers).  Wave-wide; every lane returns it.
 GSAP_D int32_t wave_pick(const int32_t *w1, const int32_t *id1, int32_t n1, const int32_t *w2,
                          const int32_t *id2, int32_t n2, uint32_t target) {
+  constexpr int U = 4;  // chunks whose weights and ids are loaded together
   const int lane = threadIdx.x & 63;
   uint32_t base = 0;
   const int32_t n = n1 + n2;
-  for (int32_t c = 0; c < n; c += 64) {
-    const int32_t k = c + lane;
-    uint32_t w = 0;
-    if (k < n1) w = (uint32_t)w1[k];
-    else if (k < n) w = (uint32_t)w2[k - n1];
-    const uint32_t incl = wave_incl_scan_u32(w);
-    const uint64_t mask = __ballot(base + incl > target);
-    if (mask) {
-      const int32_t kk = c + (int32_t)__ffsll((long long)mask) - 1;
-      return kk < n1 ? id1[kk] : id2[kk - n1];
+  for (int32_t c = 0; c < n; c += U * 64) {
+    uint32_t w[U];
+    int32_t id[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int32_t k = c + 64 * u + lane;
+      w[u] = 0;
+      id[u] = 0;
+      if (k < n1) { w[u] = (uint32_t)w1[k]; id[u] = id1[k]; }
+      else if (k < n) { w[u] = (uint32_t)w2[k - n1]; id[u] = id2[k - n1]; }
     }
-    base += __shfl(incl, 63, 64);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t incl = wave_incl_scan_u32(w[u]);
+      const uint64_t mask = __ballot(base + incl > target);
+      if (mask) return __shfl(id[u], __ffsll((long long)mask) - 1, 64);
+      base += __shfl(incl, 63, 64);
+    }
   }
   return id1 ? id1[0] : id2[0];  // unreachable when target < total weight
 }
