@@ -1205,6 +1205,17 @@ bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t 
   return true;
 }
 
+// every 16th entry of the CSR columns and CSC rows (the coarse lookup index)
+__global__ void k_build_index(const int32_t *rowptr, int32_t B, const int32_t *col,
+                              const int32_t *row, int32_t *gcol, int32_t *grow) {
+  const int64_t nnz = rowptr[B];
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; 16 * j < nnz;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    gcol[j] = col[16 * j];
+    grow[j] = row[16 * j];
+  }
+}
+
 void rebuild_blockmodel(const GraphDev &g, const int32_t *lab32, int32_t B, BMBuf &bm,
                         RebuildScratch &rs, int32_t dense_max_B, int32_t tile_max_B,
                         hipStream_t st) {
@@ -1236,6 +1247,11 @@ void rebuild_blockmodel(const GraphDev &g, const int32_t *lab32, int32_t B, BMBu
   else seg_path(g, lab, B, bm, rs, dense, st);
   bm.dense_valid = dense;
   bm.dense_B = dense ? B : 0;
+  // lookups without the dense copy search long rows through the coarse index
+  bm.index_valid = !dense && bm.gcol && bm.grow;
+  if (bm.index_valid)
+    k_build_index<<<grid_for(g.E / 16 + 1, 256, SEG_GRID), 256, 0, st>>>(
+        bm.rowptr, B, bm.col, bm.row, bm.gcol, bm.grow);
 }
 
 }  // namespace gsap

@@ -266,30 +266,53 @@ struct BMView {
   const int32_t *dout, *din;
   const double *ldout, *ldin;
   const int32_t *dense;                  // B*B row-major, or nullptr
+  // without the dense copy: every 16th entry of col / row (gcol[j] = col[16 j]), a coarse
+  // index that turns a long row's binary search into a few lines (nullptr: plain search)
+  const int32_t *gcol, *grow;
 };
+
+// Position of t in the sorted keys[lo, hi), or -1.  Long ranges first search the coarse
+// index (idx[j] = keys[16 j]) for the 16-entry block that can hold t, then that block.
+GSAP_D int32_t idx_find(const int32_t *keys, const int32_t *idx, int32_t lo, int32_t hi,
+                        int32_t t) {
+  int32_t a = lo, b = hi;
+  if (idx && hi - lo > 32) {
+    const int32_t jl = (lo + 15) >> 4, jh = ((hi - 1) >> 4) + 1;
+    int32_t l = jl, h = jh;  // first indexed entry greater than t
+    while (l < h) {
+      const int32_t mid = (l + h) >> 1;
+      if (idx[mid] <= t) l = mid + 1;
+      else h = mid;
+    }
+    if (l == jl) {
+      b = jl << 4;
+    } else {
+      a = (l - 1) << 4;
+      b = a + 16 < hi ? a + 16 : hi;
+    }
+  }
+  while (a < b) {
+    const int32_t mid = (a + b) >> 1;
+    if (keys[mid] < t) a = mid + 1;
+    else b = mid;
+  }
+  return (a < hi && keys[a] == t) ? a : -1;
+}
 
 // M[r][t] (0 when absent)
 GSAP_D int32_t bm_row_get(const BMView &m, int32_t r, int32_t t) {
   if (m.dense) return m.dense[(int64_t)r * m.B + t];
-  int32_t lo = m.rowptr[r], hi = m.rowptr[r + 1];
-  while (lo < hi) {
-    int32_t mid = (lo + hi) >> 1;
-    if (m.col[mid] < t) lo = mid + 1; else hi = mid;
-  }
-  return (lo < m.rowptr[r + 1] && m.col[lo] == t) ? m.val[lo] : 0;
+  const int32_t k = idx_find(m.col, m.gcol, m.rowptr[r], m.rowptr[r + 1], t);
+  return k >= 0 ? m.val[k] : 0;
 }
 // M[t][s] via column s (0 when absent)
 GSAP_D int32_t bm_col_get(const BMView &m, int32_t s, int32_t t) {
   if (m.dense) return m.dense[(int64_t)t * m.B + s];
-  int32_t lo = m.colptr[s], hi = m.colptr[s + 1];
-  while (lo < hi) {
-    int32_t mid = (lo + hi) >> 1;
-    if (m.row[mid] < t) lo = mid + 1; else hi = mid;
-  }
-  return (lo < m.colptr[s + 1] && m.row[lo] == t) ? m.cval[lo] : 0;
+  const int32_t k = idx_find(m.row, m.grow, m.colptr[s], m.colptr[s + 1], t);
+  return k >= 0 ? m.cval[k] : 0;
 }
 
-// M[r][t] and M[s][t] together (independent probes interleaved)
+// M[r][t] and M[s][t] together
 GSAP_D void bm_row_get2(const BMView &m, int32_t r, int32_t s, int32_t t, int32_t &Mr,
                         int32_t &Ms) {
   if (m.dense) {
@@ -297,20 +320,10 @@ GSAP_D void bm_row_get2(const BMView &m, int32_t r, int32_t s, int32_t t, int32_
     Ms = m.dense[(int64_t)s * m.B + t];
     return;
   }
-  int32_t lo1 = m.rowptr[r], hi1 = m.rowptr[r + 1], e1 = hi1;
-  int32_t lo2 = m.rowptr[s], hi2 = m.rowptr[s + 1], e2 = hi2;
-  while (lo1 < hi1 || lo2 < hi2) {
-    if (lo1 < hi1) {
-      int32_t mid = (lo1 + hi1) >> 1;
-      if (m.col[mid] < t) lo1 = mid + 1; else hi1 = mid;
-    }
-    if (lo2 < hi2) {
-      int32_t mid = (lo2 + hi2) >> 1;
-      if (m.col[mid] < t) lo2 = mid + 1; else hi2 = mid;
-    }
-  }
-  Mr = (lo1 < e1 && m.col[lo1] == t) ? m.val[lo1] : 0;
-  Ms = (lo2 < e2 && m.col[lo2] == t) ? m.val[lo2] : 0;
+  const int32_t k1 = idx_find(m.col, m.gcol, m.rowptr[r], m.rowptr[r + 1], t);
+  const int32_t k2 = idx_find(m.col, m.gcol, m.rowptr[s], m.rowptr[s + 1], t);
+  Mr = k1 >= 0 ? m.val[k1] : 0;
+  Ms = k2 >= 0 ? m.val[k2] : 0;
 }
 
 // M[t][r] and M[t][s] together via columns r and s
@@ -321,20 +334,10 @@ GSAP_D void bm_col_get2(const BMView &m, int32_t r, int32_t s, int32_t t, int32_
     Ms = m.dense[(int64_t)t * m.B + s];
     return;
   }
-  int32_t lo1 = m.colptr[r], hi1 = m.colptr[r + 1], e1 = hi1;
-  int32_t lo2 = m.colptr[s], hi2 = m.colptr[s + 1], e2 = hi2;
-  while (lo1 < hi1 || lo2 < hi2) {
-    if (lo1 < hi1) {
-      int32_t mid = (lo1 + hi1) >> 1;
-      if (m.row[mid] < t) lo1 = mid + 1; else hi1 = mid;
-    }
-    if (lo2 < hi2) {
-      int32_t mid = (lo2 + hi2) >> 1;
-      if (m.row[mid] < t) lo2 = mid + 1; else hi2 = mid;
-    }
-  }
-  Mr = (lo1 < e1 && m.row[lo1] == t) ? m.cval[lo1] : 0;
-  Ms = (lo2 < e2 && m.row[lo2] == t) ? m.cval[lo2] : 0;
+  const int32_t k1 = idx_find(m.row, m.grow, m.colptr[r], m.colptr[r + 1], t);
+  const int32_t k2 = idx_find(m.row, m.grow, m.colptr[s], m.colptr[s + 1], t);
+  Mr = k1 >= 0 ? m.cval[k1] : 0;
+  Ms = k2 >= 0 ? m.cval[k2] : 0;
 }
 
 // Weighted pick over row x then column x: first entry whose running weight

@@ -32,6 +32,8 @@ struct BMBuf {
   double *ldout = nullptr, *ldin = nullptr;       // Bmax
   int32_t *dense = nullptr;                       // dense_cap words
   int64_t dense_cap = 0;
+  int32_t *gcol = nullptr, *grow = nullptr;       // E/16 + 1: coarse index of col / row
+  bool index_valid = false;
   bool dense_valid = false;
   // B of the entries the dense buffer holds (the current CSR's); 0 = all zero
   int32_t dense_B = 0;
@@ -41,6 +43,8 @@ struct BMBuf {
     v.colptr = colptr; v.row = row; v.cval = cval;
     v.dout = dout; v.din = din; v.ldout = ldout; v.ldin = ldin;
     v.dense = dense_valid ? dense : nullptr;
+    v.gcol = index_valid ? gcol : nullptr;
+    v.grow = index_valid ? grow : nullptr;
     return v;
   }
 };

@@ -148,7 +148,7 @@ struct gsap_ctx {
   // labels and golden-section snapshots
   DBuf lab, snap[3];
   // block matrix
-  DBuf rowptr, colptr, col, val, erow, row, cval, dout, din, ldout, ldin, dense;
+  DBuf rowptr, colptr, col, val, erow, row, cval, dout, din, ldout, ldin, dense, gcol, grow;
   // rebuild scratch
   DBuf k0, k1, tmp, partials, soff, su, suoff, lists, ncl, cntc, vrel, sseg, lab16;
   // sweep / merge
@@ -370,6 +370,7 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   c->rowptr.ensure(4 * (N + 1)); c->colptr.ensure(4 * (N + 1));
   c->col.ensure(4 * E); c->val.ensure(4 * E); c->erow.ensure(4 * E);
   c->row.ensure(4 * E); c->cval.ensure(4 * E);
+  c->gcol.ensure(4 * (E / 16 + 2)); c->grow.ensure(4 * (E / 16 + 2));
   c->dout.ensure(4 * N); c->din.ensure(4 * N);
   c->ldout.ensure(8 * N); c->ldin.ensure(8 * N);
   const int64_t dB = std::min<int64_t>(dense_max_B, N);
@@ -411,6 +412,7 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   b.dout = c->dout.as<int32_t>(); b.din = c->din.as<int32_t>();
   b.ldout = c->ldout.as<double>(); b.ldin = c->ldin.as<double>();
   b.dense = c->dense.as<int32_t>(); b.dense_cap = (int64_t)(c->dense.cap / 4);
+  b.gcol = c->gcol.as<int32_t>(); b.grow = c->grow.as<int32_t>();
   RebuildScratch &r = c->rs;
   r.k0 = c->k0.p; r.k1 = c->k1.p; r.tmp = c->tmp.as<uint32_t>();
   r.tmp_words = (int64_t)(c->tmp.cap / 4);
@@ -994,7 +996,8 @@ int gsap_destroy(gsap_ctx *c) {
   DBuf *all[] = {&c->optr, &c->oadj, &c->ow, &c->iptr, &c->iadj, &c->iw, &c->kout, &c->kin,
                  &c->esrc, &c->idst, &c->sloop, &c->hoff, &c->hbo, &c->hbi, &c->hpool, &c->lab, &c->snap[0],
                  &c->snap[1], &c->snap[2], &c->rowptr, &c->colptr, &c->col, &c->val, &c->erow,
-                 &c->row, &c->cval, &c->dout, &c->din, &c->ldout, &c->ldin, &c->dense, &c->k0,
+                 &c->row, &c->cval, &c->dout, &c->din, &c->ldout, &c->ldin, &c->dense,
+                 &c->gcol, &c->grow, &c->k0,
                  &c->k1, &c->tmp, &c->partials, &c->soff, &c->su, &c->suoff,
                  &c->lists, &c->ncl, &c->cntc, &c->vrel, &c->sseg, &c->lab16, &c->vlist,
                  &c->OUT, &c->sof, &c->coin, &c->prop, &c->dS, &c->best, &c->dSbest, &c->map,

@@ -147,18 +147,20 @@ def c3():
     return gsap_amd.generate_dcsbm(200000, 71, 1.0, 1.5, 22.5, seed=3)
 
 
-@pytest.mark.parametrize("B", [71, 5000, 200000])
-def test_c3_blockmodel_and_batch(ctx, c3, B):
+@pytest.mark.parametrize("B,dense", [(71, 0), (5000, 0), (200000, 0), (20000, 0), (3000, 1)])
+def test_c3_blockmodel_and_batch(ctx, c3, B, dense):
+    """dense = 1 (and B > 8192) look the block matrix up in CSR/CSC rows of hundreds of
+    entries through the coarse index."""
     g = c3
     lab = labels_for(g, B, 9) if B < g.N else np.arange(g.N, dtype=np.int32)
-    print(f"C3 blockmodel B={B}", flush=True)
-    got = ctx.blockmodel(g, lab, B)
+    print(f"C3 blockmodel B={B} dense={dense}", flush=True)
+    got = ctx.blockmodel(g, lab, B, Options(dense_max_B=dense))
     ref = O.blockmodel(g, lab, B)
     for k in ("nnz", "rowptr", "col", "val", "colptr", "row", "cval", "dout", "din"):
         assert np.array_equal(got[k], ref[k]), k
     # size-independent invariants
     assert got["val"].sum() == g.E and got["dout"].sum() == g.E and got["din"].sum() == g.E
-    o = Options(seed=5)
+    o = Options(seed=5, dense_max_B=dense)
     print(f"C3 nodal B={B}", flush=True)
     gb = ctx.nodal_batch(g, o, lab, B, 1, 40)
     rb = O.nodal_batch(g, o, lab, B, 1, 40)
