@@ -741,24 +741,19 @@ __global__ __launch_bounds__(256) void k_seg_sort_w(const uint32_t *off, uint32_
 
 // one workgroup per segment of 513..4096 entries: 4, 8 or 16 entries per thread (networks
 // of 1024, 2048 or 4096), 32 KB of LDS
+// One size class per instance (its own register and LDS budget: the 1K and 2K classes run
+// at a higher occupancy than the 4K class would allow them).
+template <int R, int CL>
 __global__ __launch_bounds__(256) void k_seg_sort_g(const uint32_t *off, uint32_t *ekey,
                                                     uint32_t *ew, const uint32_t *lists,
                                                     int64_t lcap, const uint32_t *ncl,
                                                     uint32_t *u) {
-  __shared__ uint32_t xk[SEG_G], xv[SEG_G];
+  __shared__ uint32_t xk[256 * R], xv[256 * R];
   __shared__ uint32_t sc4[4];
-  const int64_t n4 = ncl[CL_G4K], n2 = ncl[CL_G2K], n1 = ncl[CL_G1K];
-  for (int64_t t = blockIdx.x; t < n4 + n2 + n1; t += gridDim.x) {
-    if (t < n4) {
-      const uint32_t x = lists[CL_G4K * lcap + t];
-      team_sort_regs<256, 16>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], xk, xv, sc4);
-    } else if (t < n4 + n2) {
-      const uint32_t x = lists[CL_G2K * lcap + (t - n4)];
-      team_sort_regs<256, 8>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], xk, xv, sc4);
-    } else {
-      const uint32_t x = lists[CL_G1K * lcap + (t - n4 - n2)];
-      team_sort_regs<256, 4>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], xk, xv, sc4);
-    }
+  const int64_t n = ncl[CL];
+  for (int64_t t = blockIdx.x; t < n; t += gridDim.x) {
+    const uint32_t x = lists[CL * lcap + t];
+    team_sort_regs<256, R>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], xk, xv, sc4);
   }
 }
 
@@ -1007,11 +1002,17 @@ static void seg_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
     k_seg_c1<<<SEG_GRID, 512, (size_t)RC * 4, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl,
                                                     B, rs.su);
   } else {
-    k_seg_sort_g<<<SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl, rs.su);
+    k_seg_sort_g<16, CL_G4K><<<SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
+                                                       rs.ncl, rs.su);
+    k_seg_sort_g<8, CL_G2K><<<SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
+                                                      rs.ncl, rs.su);
+    k_seg_sort_g<4, CL_G1K><<<2 * SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
+                                                          rs.ncl, rs.su);
     k_seg_c_count<<<SEG_GRID / 2, 512, (size_t)RC * 4, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
                                                              rs.ncl, B, nr, rs.cntc, rs.su);
   }
-  k_seg_sort_w<<<SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl, rs.su);
+  k_seg_sort_w<<<SEG_GRID * 3 / 2, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl,
+                                               rs.su);
   k_seg_sort_reg<<<SEG_GRID * 2, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl,
                                                rs.su);
   scan_exclusive(rs.su, rs.suoff, nseg, rs.suoff + nseg, rs.tmp, st);
