@@ -299,6 +299,53 @@ GSAP_D int32_t idx_find(const int32_t *keys, const int32_t *idx, int32_t lo, int
   return (a < hi && keys[a] == t) ? a : -1;
 }
 
+// Both positions of t: in keys[lo1, hi1) and keys[lo2, hi2) (two rows of one array),
+// the two searches advancing in the same loop so their loads overlap.
+GSAP_D void idx_find2(const int32_t *keys, const int32_t *idx, int32_t lo1, int32_t hi1,
+                      int32_t lo2, int32_t hi2, int32_t t, int32_t &k1, int32_t &k2) {
+  int32_t a1 = lo1, b1 = hi1, a2 = lo2, b2 = hi2;
+  if (idx) {
+    const bool u1 = hi1 - lo1 > 32, u2 = hi2 - lo2 > 32;
+    const int32_t j1 = (lo1 + 15) >> 4, j2 = (lo2 + 15) >> 4;
+    int32_t l1 = j1, h1 = u1 ? ((hi1 - 1) >> 4) + 1 : j1;
+    int32_t l2 = j2, h2 = u2 ? ((hi2 - 1) >> 4) + 1 : j2;
+    while (l1 < h1 || l2 < h2) {
+      if (l1 < h1) {
+        const int32_t mid = (l1 + h1) >> 1;
+        if (idx[mid] <= t) l1 = mid + 1;
+        else h1 = mid;
+      }
+      if (l2 < h2) {
+        const int32_t mid = (l2 + h2) >> 1;
+        if (idx[mid] <= t) l2 = mid + 1;
+        else h2 = mid;
+      }
+    }
+    if (u1) {
+      if (l1 == j1) b1 = j1 << 4;
+      else { a1 = (l1 - 1) << 4; b1 = a1 + 16 < hi1 ? a1 + 16 : hi1; }
+    }
+    if (u2) {
+      if (l2 == j2) b2 = j2 << 4;
+      else { a2 = (l2 - 1) << 4; b2 = a2 + 16 < hi2 ? a2 + 16 : hi2; }
+    }
+  }
+  while (a1 < b1 || a2 < b2) {
+    if (a1 < b1) {
+      const int32_t mid = (a1 + b1) >> 1;
+      if (keys[mid] < t) a1 = mid + 1;
+      else b1 = mid;
+    }
+    if (a2 < b2) {
+      const int32_t mid = (a2 + b2) >> 1;
+      if (keys[mid] < t) a2 = mid + 1;
+      else b2 = mid;
+    }
+  }
+  k1 = (a1 < hi1 && keys[a1] == t) ? a1 : -1;
+  k2 = (a2 < hi2 && keys[a2] == t) ? a2 : -1;
+}
+
 // M[r][t] (0 when absent)
 GSAP_D int32_t bm_row_get(const BMView &m, int32_t r, int32_t t) {
   if (m.dense) return m.dense[(int64_t)r * m.B + t];
@@ -320,8 +367,9 @@ GSAP_D void bm_row_get2(const BMView &m, int32_t r, int32_t s, int32_t t, int32_
     Ms = m.dense[(int64_t)s * m.B + t];
     return;
   }
-  const int32_t k1 = idx_find(m.col, m.gcol, m.rowptr[r], m.rowptr[r + 1], t);
-  const int32_t k2 = idx_find(m.col, m.gcol, m.rowptr[s], m.rowptr[s + 1], t);
+  int32_t k1, k2;
+  idx_find2(m.col, m.gcol, m.rowptr[r], m.rowptr[r + 1], m.rowptr[s], m.rowptr[s + 1], t, k1,
+            k2);
   Mr = k1 >= 0 ? m.val[k1] : 0;
   Ms = k2 >= 0 ? m.val[k2] : 0;
 }
@@ -334,8 +382,9 @@ GSAP_D void bm_col_get2(const BMView &m, int32_t r, int32_t s, int32_t t, int32_
     Ms = m.dense[(int64_t)t * m.B + s];
     return;
   }
-  const int32_t k1 = idx_find(m.row, m.grow, m.colptr[r], m.colptr[r + 1], t);
-  const int32_t k2 = idx_find(m.row, m.grow, m.colptr[s], m.colptr[s + 1], t);
+  int32_t k1, k2;
+  idx_find2(m.row, m.grow, m.colptr[r], m.colptr[r + 1], m.colptr[s], m.colptr[s + 1], t, k1,
+            k2);
   Mr = k1 >= 0 ? m.cval[k1] : 0;
   Ms = k2 >= 0 ? m.cval[k2] : 0;
 }
