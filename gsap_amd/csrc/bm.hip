@@ -793,10 +793,12 @@ GSAP_D uint32_t block_sum512(uint32_t x, uint32_t *red8) {
 }
 
 // Nonzero counters cnt[0..cn) in ascending order to keys c0 + i / weights at out + pos
-// (and the matching dense row when given); returns the number written.
+// (and the matching dense row, and the coarse index entry of every 16th position, when
+// given); returns the number written.
 GSAP_D uint32_t seg_emit(const uint32_t *cnt, uint32_t cn, uint32_t c0, uint32_t pos,
                          uint32_t *red8, uint32_t *okey, uint32_t *ow_, int32_t *col,
-                         int32_t *val, int32_t *erow, int32_t x_row, int32_t *dense_row) {
+                         int32_t *val, int32_t *erow, int32_t x_row, int32_t *dense_row,
+                         int32_t *gidx = nullptr) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const uint32_t start = pos;
@@ -823,6 +825,7 @@ GSAP_D uint32_t seg_emit(const uint32_t *cnt, uint32_t cn, uint32_t c0, uint32_t
         col[p] = (int32_t)(c0 + i);
         val[p] = (int32_t)c;
         if (erow) erow[p] = x_row;
+        if (gidx && (p & 15u) == 0) gidx[p >> 4] = (int32_t)(c0 + i);
       }
       if (dense_row) dense_row[c0 + i] = (int32_t)c;
     }
@@ -885,7 +888,8 @@ __global__ __launch_bounds__(512) void k_seg_c_write(const uint32_t *off, const 
                                                      const uint32_t *cntc, const uint32_t *uoff,
                                                      int32_t *col, int32_t *val, int32_t *erow,
                                                      int32_t *row, int32_t *cval,
-                                                     int32_t *dense) {
+                                                     int32_t *dense, int32_t *gcol,
+                                                     int32_t *grow) {
   extern __shared__ uint32_t cnt[];
   __shared__ uint32_t red8[8];
   const uint32_t RC = B < SEG_RC ? (uint32_t)B : (uint32_t)SEG_RC;
@@ -905,9 +909,9 @@ __global__ __launch_bounds__(512) void k_seg_c_write(const uint32_t *off, const 
     seg_range_count(cnt, ekey, ew, off[x], off[x + 1], c0, cn);
     if (isrow)
       seg_emit(cnt, cn, c0, pos, red8, nullptr, nullptr, col, val, erow, (int32_t)x,
-               dense ? dense + (int64_t)x * B : nullptr);
+               dense ? dense + (int64_t)x * B : nullptr, gcol);
     else
-      seg_emit(cnt, cn, c0, pos, red8, nullptr, nullptr, row, cval, nullptr, 0, nullptr);
+      seg_emit(cnt, cn, c0, pos, red8, nullptr, nullptr, row, cval, nullptr, 0, nullptr, grow);
   }
 }
 
@@ -921,7 +925,8 @@ __global__ __launch_bounds__(256) void k_seg_compact(const uint32_t *off, const 
                                                      int32_t *rowptr, int32_t *col,
                                                      int32_t *val, int32_t *erow,
                                                      int32_t *colptr, int32_t *row,
-                                                     int32_t *cval, int32_t *dense) {
+                                                     int32_t *cval, int32_t *dense,
+                                                     int32_t *gcol, int32_t *grow) {
   const uint32_t nnz = uoff[B];
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -956,9 +961,11 @@ __global__ __launch_bounds__(256) void k_seg_compact(const uint32_t *off, const 
         val[q] = (int32_t)w[j];
         erow[q] = (int32_t)x[j];
         if (dense) dense[(int64_t)x[j] * B + key[j]] = (int32_t)w[j];
+        if (gcol && (q & 15u) == 0) gcol[q >> 4] = (int32_t)key[j];
       } else {
         row[q] = (int32_t)key[j];
         cval[q] = (int32_t)w[j];
+        if (grow && (q & 15u) == 0) grow[q >> 4] = (int32_t)key[j];
       }
     }
   }
@@ -985,6 +992,8 @@ static void seg_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
   const int32_t nr = (int32_t)((B + SEG_RC - 1) / SEG_RC);
   const uint32_t RC = one ? (uint32_t)B : (uint32_t)SEG_RC;
   int32_t *D = dense ? bm.dense : nullptr;
+  // without the dense copy, the coarse lookup index is written with the entries
+  int32_t *gcol = dense ? nullptr : bm.gcol, *grow = dense ? nullptr : bm.grow;
   k_seg_zero<<<grid_for(nseg + 1, T), T, 0, st>>>(rs.soff, rs.su, rs.ncl, bm.dout, bm.din, B);
   if (B <= SEG_PRIV_B)
     k_seg_reserve<true><<<(unsigned)((N + SEG_VT - 1) / SEG_VT), 256, (size_t)16 * B, st>>>(
@@ -1018,11 +1027,11 @@ static void seg_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
   scan_exclusive(rs.su, rs.suoff, nseg, rs.suoff + nseg, rs.tmp, st);
   k_seg_compact<<<grid_for((2 * E + 3) / 4, T), T, 0, st>>>(
       rs.soff, ekey, ew, rs.sseg, rs.suoff, B, one ? 1 : 0, 2 * E, bm.rowptr, bm.col, bm.val,
-      bm.erow, bm.colptr, bm.row, bm.cval, D);
+      bm.erow, bm.colptr, bm.row, bm.cval, D, gcol, grow);
   if (!one)
     k_seg_c_write<<<SEG_GRID / 2, 512, (size_t)RC * 4, st>>>(
         rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl, B, nr, rs.cntc, rs.suoff, bm.col, bm.val,
-        bm.erow, bm.row, bm.cval, D);
+        bm.erow, bm.row, bm.cval, D, gcol, grow);
   k_logs<<<grid_for(B, 256), 256, 0, st>>>(bm.dout, bm.din, B, bm.ldout, bm.ldin);
 }
 
@@ -1206,17 +1215,6 @@ bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t 
   return true;
 }
 
-// every 16th entry of the CSR columns and CSC rows (the coarse lookup index)
-__global__ void k_build_index(const int32_t *rowptr, int32_t B, const int32_t *col,
-                              const int32_t *row, int32_t *gcol, int32_t *grow) {
-  const int64_t nnz = rowptr[B];
-  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; 16 * j < nnz;
-       j += (int64_t)gridDim.x * blockDim.x) {
-    gcol[j] = col[16 * j];
-    grow[j] = row[16 * j];
-  }
-}
-
 void rebuild_blockmodel(const GraphDev &g, const int32_t *lab32, int32_t B, BMBuf &bm,
                         RebuildScratch &rs, int32_t dense_max_B, int32_t tile_max_B,
                         hipStream_t st) {
@@ -1248,11 +1246,9 @@ void rebuild_blockmodel(const GraphDev &g, const int32_t *lab32, int32_t B, BMBu
   else seg_path(g, lab, B, bm, rs, dense, st);
   bm.dense_valid = dense;
   bm.dense_B = dense ? B : 0;
-  // lookups without the dense copy search long rows through the coarse index
+  // lookups without the dense copy search long rows through the coarse index (the
+  // segmented path, the only one without it, writes the index with the entries)
   bm.index_valid = !dense && bm.gcol && bm.grow;
-  if (bm.index_valid)
-    k_build_index<<<grid_for(g.E / 16 + 1, 256, SEG_GRID), 256, 0, st>>>(
-        bm.rowptr, B, bm.col, bm.row, bm.gcol, bm.grow);
 }
 
 }  // namespace gsap
