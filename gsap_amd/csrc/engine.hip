@@ -385,9 +385,11 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   // partial planes of the small-B histogram (up to 1024 planes of at most 128 x 128
   // counters) and of the row-tile histogram (up to 8M counters), then the transpose
   const int64_t sb = std::min<int64_t>(dB, 128);
-  // (and the transpose of the dense matrix: dB x dB words)
+  // (and the transpose of the dense matrix, which the row-tile and incremental paths
+  // build up to 8192 blocks: at most 8192 x 8192 words)
+  const int64_t tB = std::min<int64_t>(dB, 8192);
   const int64_t pc = std::max<int64_t>(
-      std::max<int64_t>(1024 * sb * sb, dB * dB),
+      std::max<int64_t>(1024 * sb * sb, tB * tB),
       dB > 128 ? std::min<int64_t>(E / 2, (int64_t)32 << 20) : 0);
   c->partials.ensure(4 * pc);
   c->soff.ensure(4 * (2 * N + 1));
