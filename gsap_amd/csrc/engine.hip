@@ -65,6 +65,7 @@ struct DBuf {
     if (bytes == 0) return;
     if (hipMalloc(&p, bytes) != hipSuccess) {
       p = nullptr;
+      (void)hipGetLastError();  // the context stays usable: no stale error for the next call
       throw Err(GSAP_ERR_MEMORY, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
     }
     cap = bytes;
@@ -1114,7 +1115,7 @@ int gsap_autotune_i32(gsap_ctx *c, const gsap_graph_i32 *g, gsap_options *o) {
         const int32_t ar = avg_row(read_nnz(c), B);
         HIPX(hipEventRecord(e0, st));
         nodal_batch(c, oo, 0, PhaseRef{ctr, 0}, ar, c->res.as<double>() + 1);
-        rebuild(c, B);
+        rebuild_after_batch(c, B, oo, 0);
         HIPX(hipEventRecord(e1, st));
         const float t = elapsed();
         if (rep > 0) best = std::min(best, t);
