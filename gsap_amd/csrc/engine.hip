@@ -700,6 +700,17 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
   const bool tr = o.trace != 0;
   const int64_t N = c->N;
   hipStream_t st = c->st;
+  if (c->E == 0) {
+    // no edges: the model term divides by E (partition.hpp:696-699); one block, S = 0
+    std::fill(out, out + N, 0);
+    if (rep) {
+      std::memset(rep, 0, sizeof(*rep));
+      rep->B = 1;
+      rep->t_total = now() - t0;
+      rep->t_upload = t_upload;
+    }
+    return;
+  }
   int32_t *lab = c->lab.as<int32_t>();
   double *res = c->res.as<double>();
   launch_iota(lab, N, st);

@@ -835,6 +835,15 @@ int orc_partition(const orc_graph *gg, const orc_options *o, int32_t *labels_out
   if (o->num_proposals < 1 || o->num_proposals > 32 || o->num_batches < 1 ||
       o->max_nodal_itr < 1 || o->moving_avg_window < 2)
     return -1;
+  if (gg->E == 0) {
+    // no edges: the model term divides by E (partition.hpp:696-699); one block, S = 0
+    std::fill(labels_out, labels_out + gg->N, 0);
+    if (rep) {
+      *rep = orc_report{};
+      rep->B = 1;
+    }
+    return 0;
+  }
   G g(gg);
   Rng rng{(uint32_t)o->seed, (uint32_t)(o->seed >> 32), o->chain};
   const int64_t N = g.N;

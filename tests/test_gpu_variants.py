@@ -139,3 +139,46 @@ def test_variant_options_checked(ctx):
         ctx.partition(g, Options(accept=ACCEPT_MH, beta=-1.0))
     with pytest.raises(gsap_amd.GsapError, match="delta_mode"):
         ctx.partition(g, Options(delta_mode=7))
+
+
+def _edge_cases():
+    rng = np.random.default_rng(5)
+    star_src = [0] * 40 + list(range(1, 41))
+    star_dst = list(range(1, 41)) + [0] * 40
+    clique = [(a, b) for a in range(12) for b in range(12) if a != b]
+    iso_src = rng.integers(0, 50, 120)
+    iso_dst = (iso_src + 1 + rng.integers(0, 48, 120)) % 50
+    pairs = sorted(set(zip(iso_src.tolist(), iso_dst.tolist())))
+    return {
+        "one_vertex": gsap_amd.Graph.from_edges(1, [], []),
+        "no_edges": gsap_amd.Graph.from_edges(7, [], []),
+        "isolated": gsap_amd.Graph.from_edges(90, [p[0] for p in pairs], [p[1] for p in pairs]),
+        "star": gsap_amd.Graph.from_edges(41, star_src, star_dst),
+        "clique": gsap_amd.Graph.from_edges(12, [a for a, _ in clique], [b for _, b in clique]),
+        "self_loops_only": gsap_amd.Graph.from_edges(9, list(range(9)), list(range(9)),
+                                                     np.arange(1, 10, dtype=np.int32)),
+        "odd_N": with_self_loops(random_graph(203, 900, 8, wmax=5), step=7, seed=4),
+    }
+
+
+EDGE = _edge_cases()
+
+
+@pytest.mark.parametrize("name", list(EDGE))
+@pytest.mark.parametrize("var", ["reference", "exact_mh"])
+def test_partition_edge_cases(ctx, name, var):
+    """Degenerate inputs: a single vertex, no edges, isolated vertices, N not a multiple of
+    the batch count (its tail never moves), stars, cliques, self loops only; the whole
+    trace equals the oracle's."""
+    g = EDGE[name]
+    o = Options(seed=7, trace=1, **({} if var == "reference" else VARIANTS[var]))
+    print(f"edge case {name} {var}", flush=True)
+    lab, rep = ctx.partition(g, o)
+    tr = ctx.trace()
+    rlab, rrep, rtr = O.partition(g, o, trace_cap=len(tr) + 10)
+    assert len(tr) == rrep.n_trace
+    for k, (a, b) in enumerate(zip(tr, rtr)):
+        assert a[:6] == b[:6], (k, a, b)
+        assert eq_bits(a[6], b[6]) and eq_bits(a[7], b[7]), (k, a, b)
+    assert np.array_equal(lab, rlab) and rep.B == rrep.B and eq_bits(rep.S, rrep.S)
+    assert lab.min() >= 0 and lab.max() < max(rep.B, 1) or g.N == 1

@@ -181,3 +181,11 @@ def test_mh_partition_runs_and_is_deterministic():
     assert 1 <= rep.B < g.N and rep.S == O.mdl(g, lab, rep.B)
     # some accepted moves raise S (MH), unlike the greedy rule
     assert any(t[0] == 2 and t[6] > 0 for t in tr) or any(t[0] == 2 and t[7] > 0 for t in tr)
+
+
+def test_partition_without_edges():
+    """No edges: one block, S = 0 (the model term would divide by E)."""
+    for N in (1, 7):
+        lab, rep, tr = O.partition(Graph.from_edges(N, [], []), Options(seed=1), trace_cap=10)
+        assert rep.B == 1 and rep.S == 0.0 and rep.n_trace == 0
+        assert np.array_equal(lab, np.zeros(N, np.int32))
