@@ -143,6 +143,8 @@ void launch_argmin(const double *dS, const int32_t *prop, int32_t B, int32_t np,
                    double *dS_best, hipStream_t st);
 void launch_relabel(int32_t *lab, const int32_t *map, int64_t N, hipStream_t st);
 void launch_iota(int32_t *x, int64_t n, hipStream_t st);
+// owner[j] = v for every entry j of vertex v's row of the CSR ptr[N + 1]
+void launch_owners(const int32_t *ptr, int64_t N, int32_t *owner, hipStream_t st);
 void launch_add_u32(uint32_t *x, uint32_t v, hipStream_t st);
 // keys[r] = order key of the double x[r], vals[r] = r
 void launch_order_keys(const double *x, int32_t n, uint64_t *keys, uint32_t *vals,

@@ -276,15 +276,14 @@ void upload_graph(gsap_ctx *c, const gsap_graph_i32 *g) {
   const int64_t N = g->N, E = g->E;
   c->N = N; c->E = E;
   hipStream_t st = c->st;
-  std::vector<int32_t> kout(N, 0), kin(N, 0), esrc(E), idst(E), sloop(N, 0);
+  std::vector<int32_t> kout(N, 0), kin(N, 0), sloop(N, 0);
   parallel_chunks(N, host_threads(), [&](int64_t lo, int64_t hi, int) {
     for (int64_t v = lo; v < hi; ++v) {
       for (int32_t j = g->out_ptr[v]; j < g->out_ptr[v + 1]; ++j) {
         kout[v] += g->out_w[j];
-        esrc[j] = (int32_t)v;
         if (g->out_adj[j] == v) sloop[v] += g->out_w[j];
       }
-      for (int32_t j = g->in_ptr[v]; j < g->in_ptr[v + 1]; ++j) { kin[v] += g->in_w[j]; idst[j] = (int32_t)v; }
+      for (int32_t j = g->in_ptr[v]; j < g->in_ptr[v + 1]; ++j) kin[v] += g->in_w[j];
     }
   });
   // vertices of the largest nodal class get tables and slot arrays in the global pool
@@ -309,8 +308,11 @@ void upload_graph(gsap_ctx *c, const gsap_graph_i32 *g) {
   h2d(c->iw, g->in_w, E, st);
   h2d(c->kout, kout.data(), N, st);
   h2d(c->kin, kin.data(), N, st);
-  h2d(c->esrc, esrc.data(), E, st);
-  h2d(c->idst, idst.data(), E, st);
+  // the owner of every CSR entry, filled on the device from the pointer arrays
+  c->esrc.ensure(4 * (size_t)std::max<int64_t>(E, 1));
+  c->idst.ensure(4 * (size_t)std::max<int64_t>(E, 1));
+  launch_owners(c->optr.as<int32_t>(), N, c->esrc.as<int32_t>(), st);
+  launch_owners(c->iptr.as<int32_t>(), N, c->idst.as<int32_t>(), st);
   h2d(c->sloop, sloop.data(), N, st);
   h2d(c->hoff, hoff.data(), N, st);
   h2d(c->hbo, hbo.data(), N, st);

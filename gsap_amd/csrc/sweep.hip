@@ -1799,6 +1799,16 @@ void launch_order_keys(const double *x, int32_t n, uint64_t *keys, uint32_t *val
   k_order_keys<<<grid_for(n, 256), 256, 0, st>>>(x, n, keys, vals);
 }
 
+// owner[j] = v for every entry j of vertex v's CSR row
+__global__ void k_owners(const int32_t *ptr, int64_t N, int32_t *owner) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < N;
+       v += (int64_t)gridDim.x * blockDim.x)
+    for (int32_t j = ptr[v]; j < ptr[v + 1]; ++j) owner[j] = (int32_t)v;
+}
+void launch_owners(const int32_t *ptr, int64_t N, int32_t *owner, hipStream_t st) {
+  if (N > 0) k_owners<<<grid_for(N, 256), 256, 0, st>>>(ptr, N, owner);
+}
+
 void launch_iota(int32_t *x, int64_t n, hipStream_t st) {
   k_iota<<<grid_for(n, 256), 256, 0, st>>>(x, n);
 }
