@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -335,25 +336,47 @@ void upload_graph(gsap_ctx *c, const gsap_graph_i32 *g) {
 // slots; beyond that the kernel uses the global pool).
 void build_batch_lists(gsap_ctx *c, const gsap_graph_i32 *g, int32_t nb) {
   const int64_t N = g->N, BS = N / nb;
-  std::vector<int32_t> lst;
-  lst.reserve((size_t)(BS * nb));
-  c->list_off.assign((size_t)nb * NCLS + 1, 0);
+  // per batch (one host thread each): its vertices by class, classes in order
+  std::vector<std::vector<int32_t>> per(nb);
+  std::vector<std::array<int64_t, NCLS>> cnt(nb);
   c->big_lds.assign((size_t)nb, 0);
-  for (int32_t b = 0; b < nb; ++b)
-    for (int k = 0; k < NCLS; ++k) {
-      c->list_off[(size_t)b * NCLS + k] = (int64_t)lst.size();
-      for (int64_t v = (int64_t)b * BS; v < (int64_t)(b + 1) * BS; ++v) {
+  parallel_chunks(nb, std::min(nb, host_threads()), [&](int64_t lo, int64_t hi, int) {
+    for (int64_t b = lo; b < hi; ++b) {
+      std::vector<int32_t> cls((size_t)BS);
+      std::array<int64_t, NCLS> n{};
+      uint32_t lds = 0;
+      for (int64_t i = 0; i < BS; ++i) {
+        const int64_t v = b * BS + i;
         const int64_t no = g->out_ptr[v + 1] - g->out_ptr[v];
         const int64_t ni = g->in_ptr[v + 1] - g->in_ptr[v];
-        if (nodal_class(std::max(no, ni)) != k) continue;
-        lst.push_back((int32_t)v);
+        const int k = nodal_class(std::max(no, ni));
+        cls[(size_t)i] = k;
+        ++n[k];
         if (k == NCLS - 1) {
           // LDS of the launch: enough for its largest vertex, within the cap
           const int64_t need = 4 * big_words(no, ni);
-          if (need <= BIG_LDS_CAP) c->big_lds[b] = std::max<uint32_t>(c->big_lds[b], (uint32_t)need);
+          if (need <= BIG_LDS_CAP) lds = std::max<uint32_t>(lds, (uint32_t)need);
         }
       }
+      std::array<int64_t, NCLS> pos{};
+      for (int k = 1; k < NCLS; ++k) pos[k] = pos[k - 1] + n[k - 1];
+      per[b].resize((size_t)BS);
+      for (int64_t i = 0; i < BS; ++i) per[b][(size_t)pos[cls[(size_t)i]]++] = (int32_t)(b * BS + i);
+      cnt[b] = n;
+      c->big_lds[b] = lds;
     }
+  });
+  std::vector<int32_t> lst;
+  lst.reserve((size_t)(BS * nb));
+  c->list_off.assign((size_t)nb * NCLS + 1, 0);
+  for (int32_t b = 0; b < nb; ++b) {
+    int64_t o = (int64_t)lst.size();
+    for (int k = 0; k < NCLS; ++k) {
+      c->list_off[(size_t)b * NCLS + k] = o;
+      o += cnt[b][k];
+    }
+    lst.insert(lst.end(), per[b].begin(), per[b].end());
+  }
   c->list_off[(size_t)nb * NCLS] = (int64_t)lst.size();
   c->vlist.ensure(4 * std::max<size_t>(lst.size(), 1));
   if (!lst.empty())
