@@ -122,13 +122,20 @@ void launch_mix_stats(const float *err, const uint8_t *near, int64_t n, uint32_t
                       unsigned long long *count, hipStream_t st);
 // Labels of a nodal batch (greedy, all = false: where dS < 0; Metropolis-Hastings, all = true:
 // prop holds every vertex's new label; olab keeps the old ones, lab16 when given follows)
-// and the canonical sum of the batch's dS into *out.
+// and the canonical sum of the batch's dS into *out (out = nullptr: tile sums only, the
+// total left to launch_sweep_sums).
 void launch_apply_sum(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo, int64_t BS,
                       bool all, int32_t *olab, uint16_t *lab16, double *tiles, double *out,
                       hipStream_t st);
 // canonical array sum: tiles of 1024 (n read from n_dev when non-null, else n_host)
 void launch_csum_array(const double *x, const int32_t *n_dev, int64_t n_host, int64_t n_max,
                        double *tiles, double *out, hipStream_t st);
+// The end of a sweep: the description-length data term (tiles at tiles[0..)) into out[0] and
+// the deferred delta-S sums of the nb batches (tiles at tiles + boff + b * bstride) into
+// out[1 + b], one launch; ctr advanced by add afterwards.
+void launch_sweep_sums(const BMView &m, const int32_t *erow, int64_t n_max, double *tiles,
+                       int64_t boff, int64_t bstride, int32_t nb, int64_t BS, double *out,
+                       uint32_t *ctr, uint32_t add, hipStream_t st);
 // description-length data term summed in canonical order into *out; ctr (when given)
 // advanced by add afterwards
 void launch_mdl_sum(const BMView &m, const int32_t *erow, int64_t n_max, double *tiles,

@@ -427,7 +427,8 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   c->OUT.ensure(8 * N); c->sof.ensure(4 * N); c->coin.ensure(N);
   c->prop.ensure(4 * (size_t)np * N); c->dS.ensure(8 * (size_t)np * N);
   c->best.ensure(4 * N); c->dSbest.ensure(8 * N); c->map.ensure(4 * N);
-  c->tiles.ensure(8 * (std::max(E, N) / 1024 + 2));
+  // tile sums: the MDL's (E / 1024 + 1), then one region of BS / 1024 + 1 per batch
+  c->tiles.ensure(8 * (E / 1024 + N / 1024 + 2 * 64 + 8));
   c->res.ensure(8 * 64);
   c->ctr.ensure(8);
   c->mk0.ensure(8 * N); c->mk1.ensure(8 * N); c->mv0.ensure(4 * N); c->mv1.ensure(4 * N);
@@ -503,10 +504,25 @@ double model_S(double data_S, int64_t N, int64_t E, int32_t B, int32_t model_kin
   return model - data_S;
 }
 
+// tile-sum region of nodal batch b (after the MDL's)
+int64_t batch_tiles_off(const gsap_ctx *c) { return std::max<int64_t>(c->E, 1) / 1024 + 2; }
+int64_t batch_tiles_stride(const gsap_ctx *c, int32_t nb) { return c->N / nb / 1024 + 1; }
+
 void launch_mdl(gsap_ctx *c, double *out, uint32_t *ctr = nullptr, uint32_t add = 0) {
   int p = c->prof.begin(4, c->st);
   BMView m = c->bm.view();
   launch_mdl_sum(m, c->bm.erow, c->E, c->tiles.as<double>(), out, ctr, add, c->st);
+  c->prof.end(p, c->st);
+}
+
+// the sweep's closing sums: the description length's data term into res[0], each batch's
+// deferred delta-S total into res[1 + b]; the batch counter advanced for the next sweep
+void sweep_sums(gsap_ctx *c, const gsap_options &o, double *res, uint32_t *ctr) {
+  int p = c->prof.begin(4, c->st);
+  BMView m = c->bm.view();
+  launch_sweep_sums(m, c->bm.erow, c->E, c->tiles.as<double>(), batch_tiles_off(c),
+                    batch_tiles_stride(c, o.num_batches), o.num_batches, c->N / o.num_batches,
+                    res, ctr, (uint32_t)o.num_batches, c->st);
   c->prof.end(p, c->st);
 }
 
@@ -603,7 +619,8 @@ int32_t merge_phase(gsap_ctx *c, const gsap_options &o, int32_t B_to_merge, uint
   return B - merged;
 }
 
-// One nodal batch on the current block matrix; the accepted delta-S sum lands in *dE_dev.
+// One nodal batch on the current block matrix; the accepted delta-S sum lands in *dE_dev
+// (nullptr: only the batch's tile sums, totalled by the sweep's launch_sweep_sums).
 void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, PhaseRef phase, int32_t ar,
                  double *dE_dev) {
   const int64_t BS = c->N / o.num_batches, lo = (int64_t)b * BS;
@@ -641,7 +658,10 @@ void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, PhaseRef phase, 
   // finds it current)
   launch_apply_sum(c->lab.as<int32_t>(), c->prop.as<int32_t>(), c->dS.as<double>(), lo, BS,
                    o.accept == GSAP_ACCEPT_MH, c->olab.as<int32_t>(),
-                   c->bm.B <= 65535 ? c->rs.lab16 : nullptr, c->tiles.as<double>(), dE_dev, st);
+                   c->bm.B <= 65535 ? c->rs.lab16 : nullptr,
+                   c->tiles.as<double>() + batch_tiles_off(c) +
+                       (int64_t)b * batch_tiles_stride(c, o.num_batches),
+                   dE_dev, st);
   c->prof.end(p, st);
   HIPX(hipGetLastError());
 }
@@ -789,10 +809,10 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
       if (use_graph) {
         if (!gg.exec) gg.capture([&] {
           for (int b = 0; b < o.num_batches; ++b) {
-            nodal_batch(c, o, b, PhaseRef{ctr, (uint32_t)b}, ar_level, res + 1 + b);
+            nodal_batch(c, o, b, PhaseRef{ctr, (uint32_t)b}, ar_level, nullptr);
             rebuild_after_batch(c, B, o, b);
           }
-          launch_mdl(c, res, ctr, (uint32_t)o.num_batches);
+          sweep_sums(c, o, res, ctr);
         });
         HIPX(hipGraphLaunch(gg.exec, st));
         batch_ctr += (uint32_t)o.num_batches;
@@ -804,7 +824,8 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
           hbefore.resize(BS);
           HIPX(hipMemcpyAsync(hbefore.data(), lab_b, 4 * (size_t)BS, hipMemcpyDeviceToHost, st));
         }
-        nodal_batch(c, o, b, PhaseRef{ctr, (uint32_t)b}, ar, res + 1 + b);
+        // traced: every batch's sum at once (read below); else totalled at the sweep's end
+        nodal_batch(c, o, b, PhaseRef{ctr, (uint32_t)b}, ar, tr ? res + 1 + b : nullptr);
         rebuild_after_batch(c, B, o, b);
         if (tr) {
           double e = 0;
@@ -822,7 +843,8 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
         ++batches;
       }
       if (!use_graph) {
-        launch_mdl(c, res, ctr, (uint32_t)o.num_batches);
+        if (tr) launch_mdl(c, res, ctr, (uint32_t)o.num_batches);
+        else sweep_sums(c, o, res, ctr);
       }
       double h[64];
       int32_t nnz = 0;

@@ -1367,8 +1367,9 @@ void launch_apply_sum(int32_t *lab, const int32_t *prop, const double *dS, int64
   if (nt > 0)
     k_apply_tiles<<<(unsigned)nt, 256, 0, st>>>(lab, prop, dS, lo, BS, all ? 1 : 0, olab, lab16,
                                                 tiles);
-  k_final_sum<<<1, 64, 0, st>>>(tiles, nullptr, BS, out, nullptr, 0);
+  if (out) k_final_sum<<<1, 64, 0, st>>>(tiles, nullptr, BS, out, nullptr, 0);
 }
+
 
 // data term of every block-matrix entry in CSR order (kernels.hpp:1066-1093), summed per
 // 1024-entry tile (one wave each)
@@ -1393,6 +1394,35 @@ void launch_mdl_sum(const BMView &m, const int32_t *erow, int64_t n_max, double 
   const int64_t nt = (n_max + 1023) / 1024;
   if (nt > 0) k_mdl_tiles<<<(unsigned)nt, 64, 0, st>>>(m, erow, tiles);
   k_final_sum<<<1, 64, 0, st>>>(tiles, m.rowptr + m.B, 0, out, ctr, add);
+}
+
+// The sums that end a sweep, one wave each: workgroup 0 the description-length data term
+// (its tiles at tiles[0..), n from nnz_dev; it also advances ctr by add), workgroup 1 + b
+// batch b's delta-S (its tiles at tiles + boff + b * bstride, n = BS).  out[0], out[1 + b].
+__global__ __launch_bounds__(64) void k_sweep_final(const double *tiles, const int32_t *nnz_dev,
+                                                    int64_t boff, int64_t bstride, int64_t BS,
+                                                    double *out, uint32_t *ctr, uint32_t add) {
+  const int x = blockIdx.x;
+  const int64_t n = x == 0 ? (int64_t)*nnz_dev : BS;
+  const double *t = x == 0 ? tiles : tiles + boff + (int64_t)(x - 1) * bstride;
+  const int64_t nt = (n + 1023) / 1024;
+  const int lane = threadIdx.x;
+  double p = 0.0;
+  for (int64_t k = lane; k < nt; k += 64) p += t[k];
+  p = wave_sum(p);
+  if (lane == 0) {
+    out[x] = p;
+    if (x == 0 && ctr) *ctr += add;
+  }
+}
+
+void launch_sweep_sums(const BMView &m, const int32_t *erow, int64_t n_max, double *tiles,
+                       int64_t boff, int64_t bstride, int32_t nb, int64_t BS, double *out,
+                       uint32_t *ctr, uint32_t add, hipStream_t st) {
+  const int64_t nt = (n_max + 1023) / 1024;
+  if (nt > 0) k_mdl_tiles<<<(unsigned)nt, 64, 0, st>>>(m, erow, tiles);
+  k_sweep_final<<<(unsigned)(nb + 1), 64, 0, st>>>(tiles, m.rowptr + m.B, boff, bstride, BS, out,
+                                                   ctr, add);
 }
 
 
