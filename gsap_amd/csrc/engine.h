@@ -136,6 +136,10 @@ void launch_csum_array(const double *x, const int32_t *n_dev, int64_t n_host, in
 void launch_sweep_sums(const BMView &m, const int32_t *erow, int64_t n_max, double *tiles,
                        int64_t boff, int64_t bstride, int32_t nb, int64_t BS, double *out,
                        uint32_t *ctr, uint32_t add, hipStream_t st);
+// res[0..n) and *nnz copied into host memory mapped for the device (hv, hnnz), then seq
+// stored at *hseq with system scope once they are visible to the host
+void launch_publish(const double *res, int32_t n, const int32_t *nnz, double *hv, int32_t *hnnz,
+                    uint32_t *hseq, uint32_t seq, hipStream_t st);
 // description-length data term summed in canonical order into *out; ctr (when given)
 // advanced by add afterwards
 void launch_mdl_sum(const BMView &m, const int32_t *erow, int64_t n_max, double *tiles,
@@ -148,7 +152,11 @@ void launch_merge(const BMView &m, const double *OUT, RngKey key, uint32_t phase
                   uint32_t *big_count, hipStream_t st);
 void launch_argmin(const double *dS, const int32_t *prop, int32_t B, int32_t np, int32_t *best,
                    double *dS_best, hipStream_t st);
-void launch_relabel(int32_t *lab, const int32_t *map, int64_t N, hipStream_t st);
+// After a merge phase: par[B] is the merge forest (par[x] == x for the surviving blocks).
+// Every block's pointer is replaced by its root, the roots are numbered in increasing id
+// order (rank, B words; tmp: scan_temp_words(B)), and lab[v] becomes its block's root rank.
+void launch_merge_relabel(int32_t *par, int32_t B, uint32_t *rank, uint32_t *tmp, int32_t *lab,
+                          int64_t N, hipStream_t st);
 void launch_iota(int32_t *x, int64_t n, hipStream_t st);
 // owner[j] = v for every entry j of vertex v's row of the CSR ptr[N + 1]
 void launch_owners(const int32_t *ptr, int64_t N, int32_t *owner, hipStream_t st);

@@ -8,7 +8,9 @@
 //               -> golden-section bracket update over B (:938, :80-322)
 // One host thread, one HIP stream, one host<->device sync per sweep (the convergence test
 // needs S) and one per merge phase (the greedy merge order is a sequential union-find on
-// the host, like the reference's host loop).
+// the host, like the reference's host loop).  The host learns that a sweep or a merge phase
+// has finished by polling a sequence number that a one-thread kernel stores into coherent
+// host memory after the results (no interrupt-driven wake-up per sweep).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -78,6 +80,37 @@ struct DBuf {
   }
   template <class T>
   T *as() const { return (T *)p; }
+};
+
+// Pinned host allocation that remembers its capacity.
+struct HPin {
+  void *p = nullptr;
+  size_t cap = 0;
+  void ensure(size_t bytes) {
+    if (bytes <= cap) return;
+    release();
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      throw Err(GSAP_ERR_MEMORY, "hipHostMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    cap = bytes;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <class T>
+  T *as() const { return (T *)p; }
+};
+
+// Host memory the device writes with k_publish: results, the block matrix's entry count and
+// the sequence number of the last publication.
+struct HostPub {
+  double v[64];
+  int32_t nnz;
+  uint32_t seq;
 };
 
 }  // namespace
@@ -174,6 +207,12 @@ struct gsap_ctx {
   int32_t tile_max_B = 2048;
   // sub-contexts of batch / chain calls (one per worker thread)
   std::vector<gsap_ctx *> workers;
+  // coherent host memory mapped for the device (host and device addresses) and the last
+  // sequence number published into it
+  HostPub *pub = nullptr, *pub_dev = nullptr;
+  uint32_t pub_seq = 0;
+  // pinned staging of the merge phase's arrays (delta-S, best partner, order, merge forest)
+  HPin stage;
 };
 
 namespace {
@@ -480,12 +519,46 @@ void rebuild_after_batch(gsap_ctx *c, int32_t B, const gsap_options &o, int32_t 
   HIPX(hipGetLastError());
 }
 
-int32_t read_nnz(gsap_ctx *c) {
-  int32_t nnz = 0;
-  HIPX(hipMemcpyAsync(&nnz, c->bm.rowptr + c->bm.B, 4, hipMemcpyDeviceToHost, c->st));
-  HIPX(hipStreamSynchronize(c->st));
-  return nnz;
+// Waits for all work queued on the stream and reads res[0..n) (device memory) and the block
+// matrix's entry count: a one-thread kernel writes them into coherent host memory followed
+// by a sequence number, which the host polls (no interrupt-driven wake-up between a sweep
+// and the next).  The stream is queried now and then, so a failed launch ends the wait with
+// its error.
+uint32_t publish(gsap_ctx *c, const double *res, int32_t n) {
+  const uint32_t seq = ++c->pub_seq;
+  launch_publish(res, n, c->bm.rowptr + c->bm.B, c->pub_dev->v, &c->pub_dev->nnz,
+                 &c->pub_dev->seq, seq, c->st);
+  HIPX(hipGetLastError());
+  return seq;
 }
+
+// Waits for publication seq; work queued after it may still run.
+const HostPub &wait_published(gsap_ctx *c, uint32_t seq) {
+  const uint32_t *f = &c->pub->seq;
+  double next_query = now() + 1e-3;
+  for (uint32_t it = 1; __atomic_load_n(f, __ATOMIC_ACQUIRE) != seq; ++it) {
+    // the runtime is asked about the stream at most once per millisecond of waiting, so
+    // that it does not slow the work queued behind the publication
+    if ((it & 4095u) != 0 || now() < next_query) continue;
+    next_query = now() + 1e-3;
+    const hipError_t e = hipStreamQuery(c->st);
+    if (e == hipErrorNotReady) continue;
+    HIPX(e);
+    // the stream has drained, so the stores are complete; allow for their visibility
+    for (int k = 0; k < (1 << 20) && __atomic_load_n(f, __ATOMIC_ACQUIRE) != seq; ++k) {}
+    if (__atomic_load_n(f, __ATOMIC_ACQUIRE) != seq)
+      throw Err(GSAP_ERR_DEVICE, "published results did not reach host memory");
+  }
+  // per-kernel-class event timing reads its events after a stream synchronisation
+  if (c->prof.on) HIPX(hipStreamSynchronize(c->st));
+  return *c->pub;
+}
+
+const HostPub &publish_wait(gsap_ctx *c, const double *res, int32_t n) {
+  return wait_published(c, publish(c, res, n));
+}
+
+int32_t read_nnz(gsap_ctx *c) { return publish_wait(c, nullptr, 0).nnz; }
 
 int32_t avg_row(int64_t nnz, int32_t B) { return (int32_t)(B > 0 ? (nnz + B - 1) / B : 0); }
 
@@ -535,7 +608,7 @@ double mdl_now(gsap_ctx *c, int32_t model_kind) {
   return model_S(h, c->N, c->E, c->bm.B, model_kind);
 }
 
-int32_t find_root(std::vector<int32_t> &par, int32_t x) {
+int32_t find_root(int32_t *par, int32_t x) {
   int32_t r = x;
   while (par[r] != r) r = par[r];
   while (par[x] != r) { int32_t n = par[x]; par[x] = r; x = n; }
@@ -567,70 +640,75 @@ int32_t merge_phase(gsap_ctx *c, const gsap_options &o, int32_t B_to_merge, uint
   // a stable LSD radix sort of the order-preserving 64-bit keys on the device for large B,
   // std::stable_sort of the same keys on the host otherwise (identical order)
   const bool dev_sort = B >= 4096;
-  std::vector<int32_t> best(B), order(B);
-  std::vector<double> dsb;
-  HIPX(hipMemcpyAsync(best.data(), c->best.p, 4 * (size_t)B, hipMemcpyDeviceToHost, st));
+  // pinned staging sized for the graph (B <= N): delta-S, best partner, order, merge forest
+  c->stage.ensure((size_t)c->N * 20);
+  double *dsb = c->stage.as<double>();
+  int32_t *best = (int32_t *)(dsb + B), *order = best + B, *map = order + B;
+  HIPX(hipMemcpyAsync(best, c->best.p, 4 * (size_t)B, hipMemcpyDeviceToHost, st));
   if (dev_sort) {
     uint64_t *k0 = c->mk0.as<uint64_t>(), *k1 = c->mk1.as<uint64_t>();
     uint32_t *v0 = c->mv0.as<uint32_t>(), *v1 = c->mv1.as<uint32_t>();
     launch_order_keys(c->dSbest.as<double>(), B, k0, v0, st);
     radix_sort_pairs<uint64_t>(k0, v0, k1, v1, B, 64, c->tmp.as<uint32_t>(), st);
     HIPX(hipGetLastError());
-    HIPX(hipMemcpyAsync(order.data(), v0, 4 * (size_t)B, hipMemcpyDeviceToHost, st));
+    HIPX(hipMemcpyAsync(order, v0, 4 * (size_t)B, hipMemcpyDeviceToHost, st));
   }
-  if (!dev_sort || dSb_out) {
-    dsb.resize(B);
-    HIPX(hipMemcpyAsync(dsb.data(), c->dSbest.p, 8 * (size_t)B, hipMemcpyDeviceToHost, st));
-  }
+  if (!dev_sort || dSb_out)
+    HIPX(hipMemcpyAsync(dsb, c->dSbest.p, 8 * (size_t)B, hipMemcpyDeviceToHost, st));
   if (prop_out) {
     prop_out->resize((size_t)B * np);
     dS_out->resize((size_t)B * np);
     HIPX(hipMemcpyAsync(prop_out->data(), c->prop.p, 4 * (size_t)B * np, hipMemcpyDeviceToHost, st));
     HIPX(hipMemcpyAsync(dS_out->data(), c->dS.p, 8 * (size_t)B * np, hipMemcpyDeviceToHost, st));
   }
-  HIPX(hipStreamSynchronize(st));
+  publish_wait(c, nullptr, 0);
   c->prof.flush();
-  if (best_out) *best_out = best;
-  if (dSb_out) *dSb_out = dsb;
+  if (best_out) best_out->assign(best, best + B);
+  if (dSb_out) dSb_out->assign(dsb, dsb + B);
   if (!dev_sort) {
     std::vector<uint64_t> key(B);
     for (int32_t r = 0; r < B; ++r) key[r] = order_key(dsb[r]);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+    std::iota(order, order + B, 0);
+    std::stable_sort(order, order + B, [&](int32_t a, int32_t b) { return key[a] < key[b]; });
   }
-  std::vector<int32_t> par(B);
-  std::iota(par.begin(), par.end(), 0);
+  // The merge loop links a block that is still a root (each block comes once in order) to
+  // the root of another tree, so par stays a forest.
+  int32_t *par = map;
+  std::iota(par, par + B, 0);
   int32_t merged = 0, cidx = 0;
   while (merged < B_to_merge && cidx < B) {
     int32_t from = order[cidx++];
     int32_t to = find_root(par, best[from]);
     if (to != from) { par[from] = to; ++merged; }
   }
-  std::vector<int32_t> rank(B, -1), map(B);
-  int32_t nb = 0;
-  for (int32_t x = 0; x < B; ++x)
-    if (find_root(par, x) == x) rank[x] = nb++;
-  for (int32_t x = 0; x < B; ++x) map[x] = rank[find_root(par, x)];
-  HIPX(hipMemcpyAsync(c->map.p, map.data(), 4 * (size_t)B, hipMemcpyHostToDevice, st));
-  launch_relabel(c->lab.as<int32_t>(), c->map.as<int32_t>(), c->N, st);
+  // the device resolves every block's root, numbers the surviving roots in increasing id
+  // order and relabels; the staging stays untouched until the next wait on this stream
+  HIPX(hipMemcpyAsync(c->map.p, par, 4 * (size_t)B, hipMemcpyHostToDevice, st));
+  launch_merge_relabel(c->map.as<int32_t>(), B, c->mv0.as<uint32_t>(), c->tmp.as<uint32_t>(),
+                       c->lab.as<int32_t>(), c->N, st);
   HIPX(hipGetLastError());
-  HIPX(hipStreamSynchronize(st));  // map goes out of scope
   return B - merged;
 }
 
 // One nodal batch on the current block matrix; the accepted delta-S sum lands in *dE_dev
 // (nullptr: only the batch's tile sums, totalled by the sweep's launch_sweep_sums).
+// The shared neighbour samples of a nodal batch (its block terms); they only read the block
+// matrix and the batch counter.
+void batch_terms(gsap_ctx *c, const gsap_options &o, PhaseRef phase, int32_t ar) {
+  int p = c->prof.begin(2, c->st);
+  launch_block_terms(c->bm.view(), nullptr, c->sof.as<int32_t>(), c->coin.as<uint8_t>(),
+                     rng_key(o), phase, true, ar, c->st);
+  c->prof.end(p, c->st);
+}
+
+// terms_queued: the batch's block terms are already on the stream
 void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, PhaseRef phase, int32_t ar,
-                 double *dE_dev) {
+                 double *dE_dev, bool terms_queued = false) {
   const int64_t BS = c->N / o.num_batches, lo = (int64_t)b * BS;
   hipStream_t st = c->st;
   BMView m = c->bm.view();
-  int p = c->prof.begin(2, st);
-  launch_block_terms(m, nullptr, c->sof.as<int32_t>(), c->coin.as<uint8_t>(), rng_key(o), phase,
-                     true, ar, st);
-  c->prof.end(p, st);
-  p = c->prof.begin(0, st);
+  if (!terms_queued) batch_terms(c, o, phase, ar);
+  int p = c->prof.begin(0, st);
   const int32_t *lists[NCLS];
   int32_t counts[NCLS];
   for (int k = 0; k < NCLS; ++k) {
@@ -804,6 +882,7 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
     const bool use_graph = o.graph && !tr && !c->prof.on;
     GraphGuard gg(c);
     const int32_t ar_level = ar;
+    bool terms_queued = false;  // the sweep's first block terms went ahead of the last wait
     for (int itr = 0; itr < o.max_nodal_itr; ++itr) {
       const double ts = now();
       if (use_graph) {
@@ -825,7 +904,8 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
           HIPX(hipMemcpyAsync(hbefore.data(), lab_b, 4 * (size_t)BS, hipMemcpyDeviceToHost, st));
         }
         // traced: every batch's sum at once (read below); else totalled at the sweep's end
-        nodal_batch(c, o, b, PhaseRef{ctr, (uint32_t)b}, ar, tr ? res + 1 + b : nullptr);
+        nodal_batch(c, o, b, PhaseRef{ctr, (uint32_t)b}, ar, tr ? res + 1 + b : nullptr,
+                    b == 0 && terms_queued);
         rebuild_after_batch(c, B, o, b);
         if (tr) {
           double e = 0;
@@ -847,10 +927,17 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
         else sweep_sums(c, o, res, ctr);
       }
       double h[64];
-      int32_t nnz = 0;
-      HIPX(hipMemcpyAsync(h, res, 8 * (size_t)(1 + o.num_batches), hipMemcpyDeviceToHost, st));
-      HIPX(hipMemcpyAsync(&nnz, c->bm.rowptr + B, 4, hipMemcpyDeviceToHost, st));
-      HIPX(hipStreamSynchronize(st));
+      const uint32_t seq = publish(c, res, 1 + o.num_batches);
+      // The next sweep's first block terms are queued behind the publication, so the device
+      // works while the host takes the convergence decision; they read only the block matrix
+      // and the advanced counter (no state changes; unused when the level ends here).  Their
+      // lane-group width follows the previous sweep's average row, which does not change
+      // what they compute.
+      terms_queued = !use_graph && !tr && itr + 1 < o.max_nodal_itr;
+      if (terms_queued) batch_terms(c, o, PhaseRef{ctr, 0}, ar);
+      const HostPub &hp = wait_published(c, seq);
+      std::memcpy(h, hp.v, 8 * (size_t)(1 + o.num_batches));
+      const int32_t nnz = hp.nnz;
       c->prof.flush();
       ar = avg_row(nnz, B);
       for (int b = 0; b < o.num_batches; ++b) dE[itr] = dE[itr] + h[1 + b];
@@ -1040,10 +1127,16 @@ int gsap_create(gsap_ctx **ctx, int device) {
   if (device < 0 || device >= n) return GSAP_ERR_INVALID;
   gsap_ctx *c = new gsap_ctx();
   c->device = device;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess) {
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess ||
+      hipHostMalloc((void **)&c->pub, sizeof(HostPub), hipHostMallocMapped | hipHostMallocCoherent) !=
+          hipSuccess ||
+      hipHostGetDevicePointer((void **)&c->pub_dev, c->pub, 0) != hipSuccess) {
+    if (c->pub) (void)hipHostFree(c->pub);
+    if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
     return GSAP_ERR_DEVICE;
   }
+  std::memset(c->pub, 0, sizeof(HostPub));
   *ctx = c;
   return GSAP_OK;
 }
@@ -1066,6 +1159,8 @@ int gsap_destroy(gsap_ctx *c) {
                  &c->mv1, &c->errv, &c->evcls, &c->mixstat, &c->olab};
   for (DBuf *b : all) b->release();
   try { c->prof.destroy(); } catch (...) {}
+  c->stage.release();
+  (void)hipHostFree(c->pub);
   (void)hipStreamDestroy(c->st);
   delete c;
   return GSAP_OK;
@@ -1259,7 +1354,8 @@ int gsap_merge_phase_i32(gsap_ctx *c, const gsap_graph_i32 *g, const gsap_option
     std::vector<int32_t> hp, hb;
     std::vector<double> hd, hdb;
     *B_new = merge_phase(c, oo, B_to_merge, m, &hp, &hd, &hb, &hdb);
-    HIPX(hipMemcpy(labels, c->lab.p, 4 * (size_t)c->N, hipMemcpyDeviceToHost));
+    HIPX(hipMemcpyAsync(labels, c->lab.p, 4 * (size_t)c->N, hipMemcpyDeviceToHost, c->st));
+    HIPX(hipStreamSynchronize(c->st));
     if (prop) std::copy(hp.begin(), hp.end(), prop);
     if (dS) std::copy(hd.begin(), hd.end(), dS);
     if (best) std::copy(hb.begin(), hb.end(), best);

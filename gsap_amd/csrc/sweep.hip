@@ -1425,6 +1425,23 @@ void launch_sweep_sums(const BMView &m, const int32_t *erow, int64_t n_max, doub
                                                    ctr, add);
 }
 
+// One thread: n doubles and the entry count into host memory, a system-scope fence, then
+// the sequence number, which the host polls.
+__global__ __launch_bounds__(64) void k_publish(const double *res, int32_t n, const int32_t *nnz,
+                                                double *hv, int32_t *hnnz, uint32_t *hseq,
+                                                uint32_t seq) {
+  if (threadIdx.x != 0) return;
+  for (int32_t i = 0; i < n; ++i) hv[i] = res[i];
+  *hnnz = *nnz;
+  __threadfence_system();
+  __hip_atomic_store(hseq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+void launch_publish(const double *res, int32_t n, const int32_t *nnz, double *hv, int32_t *hnnz,
+                    uint32_t *hseq, uint32_t seq, hipStream_t st) {
+  k_publish<<<1, 64, 0, st>>>(res, n, nnz, hv, hnnz, hseq, seq);
+}
+
 
 // ------------------------------------------------------------- block merges
 // One group of G lanes per proposal (r, i): the proposal (kernels.hpp:726-856, every lane
@@ -1801,13 +1818,39 @@ void launch_argmin(const double *dS, const int32_t *prop, int32_t B, int32_t np,
   k_argmin<<<grid_for(B, 128), 128, 0, st>>>(dS, prop, B, np, best, dS_best);
 }
 
-__global__ void k_relabel(int32_t *lab, const int32_t *map, int64_t N) {
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v < N) lab[v] = map[lab[v]];
+// One round of pointer jumping on the merge forest par[B] (par[x] == x: a root): every
+// block follows up to 64 links and keeps the ancestor reached.  Only thread x writes par[x],
+// and a concurrent update only moves a pointer closer to its root, so after round k every
+// pointer is the root or an ancestor at least 64^k links up.  The first round also flags
+// the roots.
+__global__ void k_jump(int32_t *par, int32_t B, uint32_t *is_root) {
+  const int32_t x = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (x >= B) return;
+  int32_t r = par[x];
+  if (is_root) is_root[x] = r == x ? 1u : 0u;
+  for (int k = 0; k < 64; ++k) {
+    const int32_t p = par[r];
+    if (p == r) break;
+    r = p;
+  }
+  par[x] = r;
 }
 
-void launch_relabel(int32_t *lab, const int32_t *map, int64_t N, hipStream_t st) {
-  k_relabel<<<grid_for(N, 256), 256, 0, st>>>(lab, map, N);
+// lab[v] = rank of the root of lab[v]'s block (rank: exclusive scan of the root flags)
+__global__ void k_relabel_roots(int32_t *lab, const int32_t *par, const uint32_t *rank,
+                                int64_t N) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < N) lab[v] = (int32_t)rank[par[lab[v]]];
+}
+
+void launch_merge_relabel(int32_t *par, int32_t B, uint32_t *rank, uint32_t *tmp, int32_t *lab,
+                          int64_t N, hipStream_t st) {
+  int rounds = 1;
+  for (int64_t reach = 64; reach < B; reach *= 64) ++rounds;
+  for (int k = 0; k < rounds; ++k)
+    k_jump<<<grid_for(B, 256), 256, 0, st>>>(par, B, k == 0 ? rank : nullptr);
+  scan_exclusive(rank, rank, B, nullptr, tmp, st);
+  k_relabel_roots<<<grid_for(N, 256), 256, 0, st>>>(lab, par, rank, N);
 }
 
 __global__ void k_iota(int32_t *x, int64_t n) {
