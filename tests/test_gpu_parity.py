@@ -28,6 +28,8 @@ CASES = {
     "bighubs": lambda: hub_graph(6, N=4000, hub_deg=1500),
     "LL1k": lambda: tsv_graph("LL"),
     "HH5k": lambda: tsv_graph("HH", "5k"),
+    # degrees spread over every nodal class (max(out, in) up to 16, 32, 64, 128 and above)
+    "dcsbm3k": lambda: gsap_amd.generate_dcsbm(3000, 8, 1.0, 2.0, 22.5, seed=9),
 }
 _graphs = {}
 

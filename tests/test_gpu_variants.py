@@ -27,6 +27,9 @@ CASES = {
     "loops_bighubs": lambda: with_self_loops(hub_graph(6, N=4000, hub_deg=1500), seed=3),
     "LL1k": lambda: tsv_graph("LL"),
     "HH5k": lambda: tsv_graph("HH", "5k"),
+    # degrees spread over every nodal class (max(out, in) up to 16, 32, 64, 128 and above)
+    "dcsbm3k": lambda: with_self_loops(gsap_amd.generate_dcsbm(3000, 8, 1.0, 2.0, 22.5, seed=9),
+                                       seed=4),
 }
 _graphs = {}
 
@@ -96,7 +99,7 @@ def test_merge_phase_exact(ctx, name, Bfrac, dense):
     assert np.array_equal(got["labels"], ref["labels"])
 
 
-@pytest.mark.parametrize("name", ["loops_hubs", "LL1k"])
+@pytest.mark.parametrize("name", ["loops_hubs", "LL1k", "dcsbm3k"])
 @pytest.mark.parametrize("var", ["exact", "mh", "exact_mh"])
 def test_partition_trace_variant(ctx, name, var):
     g = graph(name)
@@ -113,7 +116,7 @@ def test_partition_trace_variant(ctx, name, var):
     assert rep.B == rrep.B and eq_bits(rep.S, rrep.S)
 
 
-@pytest.mark.parametrize("name", ["loops_rand", "loops_hubs", "HH5k"])
+@pytest.mark.parametrize("name", ["loops_rand", "loops_hubs", "HH5k", "dcsbm3k"])
 @pytest.mark.parametrize("Bfrac", [0.15, 0.02])
 def test_nodal_batch_mixed_exact(ctx, name, Bfrac):
     """Mixed form with the exact delta: FP32 rejections, FP64 for the rest; same bits as the
