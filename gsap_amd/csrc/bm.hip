@@ -1199,7 +1199,7 @@ bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t 
         (B <= SMALL_B || B > INC_MIN_B) && (int64_t)B * B <= rs.partials_cap))
     return false;
   if (BS > 0 && B <= SMALL_B) {
-    const unsigned grid = (unsigned)std::min<int64_t>(256, (BS + 31) / 32);
+    const unsigned grid = (unsigned)std::min<int64_t>(512, (BS + 31) / 32);
     k_dense_moves_lds<<<grid, 512, (size_t)B * B * 4, st>>>(
         g.optr, g.oadj, g.ow, g.iptr, g.iadj, g.iw, lab32, olab, lo, BS, B, bm.dense);
   } else if (BS > 0) {
