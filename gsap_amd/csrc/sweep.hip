@@ -360,13 +360,42 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
   const int32_t i0 = a.iptr[v], ni = a.iptr[v + 1] - i0;
   const int64_t kout = a.kout[v], kin = a.kin[v];
   const int64_t tot = kout + kin;
+  // first chunk of both adjacency lists and its neighbour blocks, and the degrees of r:
+  // gathered before the proposal, which they do not depend on, so that their loads overlap
+  // its chain (when the proposal is r itself they go unused)
+  const int32_t u_o0 = lane < no ? a.oadj[o0 + lane] : -1;
+  const int32_t u_i0 = lane < ni ? a.iadj[i0 + lane] : -1;
+  const int32_t w_o0 = lane < no ? a.ow[o0 + lane] : 0;
+  const int32_t w_i0 = lane < ni ? a.iw[i0 + lane] : 0;
+  const int32_t t_o0 = u_o0 >= 0 ? a.lab[u_o0] : -1;
+  const int32_t t_i0 = u_i0 >= 0 ? a.lab[u_i0] : -1;
+  const int64_t dout_r = m.dout[r], din_r = m.din[r];
+  const double ldo_r = m.ldout[r], ldi_r = m.ldin[r];
   int32_t s;
   if (tot == 0) {
     s = (int32_t)mulhi(x.x0, B);
   } else {
-    const int32_t nbr = wave_pick(a.ow + o0, a.oadj + o0, no, a.iw + i0, a.iadj + i0, ni,
-                                  mulhi(x.x1, tot));
-    const int32_t u = a.lab[nbr];
+    const uint32_t target = mulhi(x.x1, tot);
+    int32_t u;
+    if (no <= 64 && ni <= 64) {
+      // both lists inside the first chunk: the same pick as wave_pick (first entry of the
+      // out list, then the in list, whose running weight exceeds target), and its block,
+      // from the registers
+      const uint32_t io = wave_incl_scan_u32((uint32_t)w_o0);
+      const uint32_t ii = wave_incl_scan_u32((uint32_t)w_i0);
+      const uint32_t tot_o = __shfl(io, 63, 64);
+      const uint64_t mo = __ballot(lane < no && io > target);
+      if (mo) {
+        u = __shfl(t_o0, __ffsll((long long)mo) - 1, 64);
+      } else {
+        const uint64_t mi = __ballot(lane < ni && tot_o + ii > target);
+        u = __shfl(t_i0, __ffsll((long long)mi) - 1, 64);
+      }
+    } else {
+      const int32_t nbr =
+          wave_pick(a.ow + o0, a.oadj + o0, no, a.iw + i0, a.iadj + i0, ni, target);
+      u = a.lab[nbr];
+    }
     s = a.coin[u] ? (int32_t)mulhi(x.x0, B) : a.sof[u];
   }
   if (lane == 0) a.prop[i] = s;
@@ -388,14 +417,9 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
   const HTab hi{lki, lwi, lfi, lbits};
   ht_clear(ho, lane, 64);
   ht_clear(hi, lane, 64);
-  // first chunk of both adjacency lists: gather the neighbour blocks once, keep them
-  const int32_t t_o0 = lane < no ? a.lab[a.oadj[o0 + lane]] : -1;
-  const int32_t t_i0 = lane < ni ? a.lab[a.iadj[i0 + lane]] : -1;
-  const int32_t w_o0 = lane < no ? a.ow[o0 + lane] : 0;
-  const int32_t w_i0 = lane < ni ? a.iw[i0 + lane] : 0;
-  // block degrees of r and s, needed after the tables
-  const int64_t dout_r = m.dout[r], dout_s = m.dout[s], din_r = m.din[r], din_s = m.din[s];
-  const double ldo_r = m.ldout[r], ldo_s = m.ldout[s], ldi_r = m.ldin[r], ldi_s = m.ldin[s];
+  // block degrees of s, needed after the tables
+  const int64_t dout_s = m.dout[s], din_s = m.din[s];
+  const double ldo_s = m.ldout[s], ldi_s = m.ldin[s];
   wave_lds_sync();
   if (lane < no) ht_insert(ho, t_o0, w_o0, lane);
   if (lane < ni) ht_insert(hi, t_i0, w_i0, lane);
