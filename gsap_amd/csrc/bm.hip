@@ -411,7 +411,8 @@ __global__ __launch_bounds__(256) void k_seg_reserve(LabView lab, const int32_t 
 
 // Edge-parallel: out-edge e goes to its source's slice of the source block's row segment,
 // keyed by the target's block; in-edge f to its target's slice of the target block's column
-// segment, keyed by the source's block; seg[pos] records the segment of every entry.
+// segment, keyed by the source's block; seg[pos] records the segment of every entry.  With
+// wb > 0 an entry is one word, key << wb | weight (ew untouched).
 __global__ __launch_bounds__(256) void k_seg_fill(LabView lab, const int32_t *esrc,
                                                   const int32_t *optr, const int32_t *oadj,
                                                   const int32_t *ow, const int32_t *idst,
@@ -419,23 +420,33 @@ __global__ __launch_bounds__(256) void k_seg_fill(LabView lab, const int32_t *es
                                                   const int32_t *iw, int64_t N, int64_t E,
                                                   int32_t B, const uint32_t *off,
                                                   const uint32_t *vrel, uint32_t *ekey,
-                                                  uint32_t *ew, uint32_t *seg) {
+                                                  uint32_t *ew, uint32_t *seg, int wb) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < 2 * E;
        e += (int64_t)gridDim.x * blockDim.x) {
     if (e < E) {
       const int32_t v = esrc[e];
       const uint32_t x = (uint32_t)lab[v];
       const uint32_t pos = off[x] + vrel[v] + (uint32_t)(e - optr[v]);
-      ekey[pos] = (uint32_t)lab[oadj[e]];
-      ew[pos] = (uint32_t)ow[e];
+      const uint32_t k = (uint32_t)lab[oadj[e]], w = (uint32_t)ow[e];
+      if (wb) {
+        ekey[pos] = k << wb | w;
+      } else {
+        ekey[pos] = k;
+        ew[pos] = w;
+      }
       seg[pos] = x;
     } else {
       const int64_t f = e - E;
       const int32_t v = idst[f];
       const uint32_t x = (uint32_t)(B + lab[v]);
       const uint32_t pos = off[x] + vrel[N + v] + (uint32_t)(f - iptr[v]);
-      ekey[pos] = (uint32_t)lab[iadj[f]];
-      ew[pos] = (uint32_t)iw[f];
+      const uint32_t k = (uint32_t)lab[iadj[f]], w = (uint32_t)iw[f];
+      if (wb) {
+        ekey[pos] = k << wb | w;
+      } else {
+        ekey[pos] = k;
+        ew[pos] = w;
+      }
       seg[pos] = x;
     }
   }
@@ -488,23 +499,34 @@ __global__ __launch_bounds__(256) void k_seg_classify(const uint32_t *off, int64
 // Sort a segment of n <= G entries held one per lane of a G-lane group (bitonic network on
 // shuffles), then reduce runs of equal keys: run totals by a segmented suffix sum, heads
 // compacted by ballot.  Writes the runs in place at the segment start and their count.
-template <int G>
-GSAP_D void seg_sort_group(uint32_t *ekey, uint32_t *ew, uint32_t s0, int n, uint32_t *uout) {
+// PK: packed entries (key << wb | weight) sorted as single words, split after the network.
+template <int G, bool PK>
+GSAP_D void seg_sort_group(uint32_t *ekey, uint32_t *ew, uint32_t s0, int n, uint32_t *uout,
+                           int wb) {
   const int q = threadIdx.x & (G - 1);
   const int gbase = (threadIdx.x & 63) & ~(G - 1);
   const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
   uint32_t key = q < n ? ekey[s0 + q] : 0xFFFFFFFFu;
-  uint32_t val = q < n ? ew[s0 + q] : 0u;
+  uint32_t val = 0u;
+  if constexpr (!PK) val = q < n ? ew[s0 + q] : 0u;
 #pragma unroll
   for (int k = 2; k <= G; k <<= 1) {
 #pragma unroll
     for (int j = k >> 1; j > 0; j >>= 1) {
-      const uint32_t pk = __shfl_xor(key, j, G), pv = __shfl_xor(val, j, G);
+      const uint32_t pk = __shfl_xor(key, j, G);
       const uint32_t mmin = ((q & k) == 0) == ((q & j) == 0) ? ~0u : 0u;
       const uint32_t nk = (min(pk, key) & mmin) | (max(pk, key) & ~mmin);
-      val = nk != key ? pv : val;
+      if constexpr (!PK) {
+        const uint32_t pv = __shfl_xor(val, j, G);
+        val = nk != key ? pv : val;
+      }
       key = nk;
     }
+  }
+  if constexpr (PK) {
+    // the padding words sort after every entry: lanes q >= n hold padding only
+    val = q < n ? key & ((1u << wb) - 1u) : 0u;
+    key >>= wb;
   }
   uint32_t acc = val;
 #pragma unroll
@@ -525,10 +547,11 @@ GSAP_D void seg_sort_group(uint32_t *ekey, uint32_t *ew, uint32_t s0, int n, uin
 }
 
 // Wave tasks: one segment of <= 64 entries, two of <= 32 or four of <= 16.
+template <bool PK>
 __global__ __launch_bounds__(256) void k_seg_sort_reg(const uint32_t *off, uint32_t *ekey,
                                                       uint32_t *ew, const uint32_t *lists,
                                                       int64_t lcap, const uint32_t *ncl,
-                                                      uint32_t *u) {
+                                                      uint32_t *u, int wb) {
   const int64_t n64 = ncl[CL_A64], n32 = ncl[CL_A32], n16 = ncl[CL_A16];
   const int64_t t32 = (n32 + 1) / 2, t16 = (n16 + 3) / 4;
   const int64_t tasks = n64 + t32 + t16;
@@ -537,18 +560,18 @@ __global__ __launch_bounds__(256) void k_seg_sort_reg(const uint32_t *off, uint3
   for (int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < tasks; t += nw) {
     if (t < n64) {
       const uint32_t x = lists[CL_A64 * lcap + t];
-      seg_sort_group<64>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x]);
+      seg_sort_group<64, PK>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], wb);
     } else if (t < n64 + t32) {
       const int64_t k = (t - n64) * 2 + (lane >> 5);
       if (k < n32) {
         const uint32_t x = lists[CL_A32 * lcap + k];
-        seg_sort_group<32>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x]);
+        seg_sort_group<32, PK>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], wb);
       }
     } else {
       const int64_t k = (t - n64 - t32) * 4 + (lane >> 4);
       if (k < n16) {
         const uint32_t x = lists[CL_A16 * lcap + k];
-        seg_sort_group<16>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x]);
+        seg_sort_group<16, PK>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], wb);
       }
     }
   }
@@ -589,10 +612,12 @@ GSAP_D uint32_t team_incl_scan(uint32_t x, uint32_t *sc4, uint32_t &total) {
 // one thread, R <= j < 64R exchange with lane (lane ^ j/R) by shuffles, larger j (T = 256)
 // go through LDS (xk, xv: T*R words each).  Then runs of equal keys are reduced: run heads
 // record the exclusive prefix of the weights in xv[run], run tails write key and total to
-// the segment's start.  Padding keys are 0xFFFFFFFF (above every block id).
-template <int T, int R>
+// the segment's start.  Padding keys are 0xFFFFFFFF (above every block id, and above every
+// packed entry).  PK: packed entries (key << wb | weight) sorted as single words, split after
+// the network.
+template <int T, int R, bool PK>
 GSAP_D void team_sort_regs(uint32_t *ekey, uint32_t *ew, uint32_t s0, int n, uint32_t *uout,
-                           uint32_t *xk, uint32_t *xv, uint32_t *sc4) {
+                           uint32_t *xk, uint32_t *xv, uint32_t *sc4, int wb) {
   const int tid = T == 64 ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
   constexpr int P = T * R;
   uint32_t key[R], val[R];
@@ -600,7 +625,8 @@ GSAP_D void team_sort_regs(uint32_t *ekey, uint32_t *ew, uint32_t s0, int n, uin
   for (int i = 0; i < R; ++i) {
     const int g = tid * R + i;
     key[i] = g < n ? ekey[s0 + g] : 0xFFFFFFFFu;
-    val[i] = g < n ? ew[s0 + g] : 0u;
+    val[i] = 0u;
+    if constexpr (!PK) val[i] = g < n ? ew[s0 + g] : 0u;
   }
 #pragma unroll
   for (int k = 2; k <= P; k <<= 1) {
@@ -613,12 +639,15 @@ GSAP_D void team_sort_regs(uint32_t *ekey, uint32_t *ew, uint32_t s0, int n, uin
           const int l = i ^ j;
           if (l > i) {
             const bool asc = ((tid * R + i) & k) == 0;
-            const uint32_t a = key[i], b = key[l], va = val[i], vb = val[l];
+            const uint32_t a = key[i], b = key[l];
             const bool sw = (a > b) == asc;
             key[i] = sw ? b : a;
             key[l] = sw ? a : b;
-            val[i] = sw ? vb : va;
-            val[l] = sw ? va : vb;
+            if constexpr (!PK) {
+              const uint32_t va = val[i], vb = val[l];
+              val[i] = sw ? vb : va;
+              val[l] = sw ? va : vb;
+            }
           }
         }
       } else if (j < 64 * R) {
@@ -629,34 +658,42 @@ GSAP_D void team_sort_regs(uint32_t *ekey, uint32_t *ew, uint32_t s0, int n, uin
 #pragma unroll
         for (int i = 0; i < R; ++i) {
           pk[i] = __shfl_xor(key[i], lm, 64);
-          pv[i] = __shfl_xor(val[i], lm, 64);
+          if constexpr (!PK) pv[i] = __shfl_xor(val[i], lm, 64);
         }
         const int g0 = tid * R;
         const uint32_t mmin = ((g0 & j) == 0) == ((g0 & k) == 0) ? ~0u : 0u;
 #pragma unroll
         for (int i = 0; i < R; ++i) {
           const uint32_t nk = (min(pk[i], key[i]) & mmin) | (max(pk[i], key[i]) & ~mmin);
-          val[i] = nk != key[i] ? pv[i] : val[i];
+          if constexpr (!PK) val[i] = nk != key[i] ? pv[i] : val[i];
           key[i] = nk;
         }
       } else {
 #pragma unroll
         for (int i = 0; i < R; ++i) {
           xk[tid * R + i] = key[i];
-          xv[tid * R + i] = val[i];
+          if constexpr (!PK) xv[tid * R + i] = val[i];
         }
         __syncthreads();
         const int g0 = tid * R;
         const uint32_t mmin = ((g0 & j) == 0) == ((g0 & k) == 0) ? ~0u : 0u;
 #pragma unroll
         for (int i = 0; i < R; ++i) {
-          const uint32_t pk = xk[(g0 + i) ^ j], pv = xv[(g0 + i) ^ j];
+          const uint32_t pk = xk[(g0 + i) ^ j];
           const uint32_t nk = (min(pk, key[i]) & mmin) | (max(pk, key[i]) & ~mmin);
-          val[i] = nk != key[i] ? pv : val[i];
+          if constexpr (!PK) val[i] = nk != key[i] ? xv[(g0 + i) ^ j] : val[i];
           key[i] = nk;
         }
         __syncthreads();
       }
+    }
+  }
+  if constexpr (PK) {
+    // the padding words sort after every entry: positions g >= n hold padding only
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      val[i] = tid * R + i < n ? key[i] & ((1u << wb) - 1u) : 0u;
+      key[i] >>= wb;
     }
   }
   // neighbours of the thread's first and last element
@@ -714,10 +751,11 @@ GSAP_D void team_sort_regs(uint32_t *ekey, uint32_t *ew, uint32_t s0, int n, uin
 
 // one wave per segment of 65..512 entries: 2, 4 or 8 entries per lane (networks of 128,
 // 256 or 512), 2 KB of LDS per wave
+template <bool PK>
 __global__ __launch_bounds__(256) void k_seg_sort_w(const uint32_t *off, uint32_t *ekey,
                                                     uint32_t *ew, const uint32_t *lists,
                                                     int64_t lcap, const uint32_t *ncl,
-                                                    uint32_t *u) {
+                                                    uint32_t *u, int wb) {
   __shared__ uint32_t lds[4][SEG_W];
   const int w = threadIdx.x >> 6;
   const int64_t n512 = ncl[CL_W512], n256 = ncl[CL_W256], n128 = ncl[CL_W128];
@@ -725,16 +763,16 @@ __global__ __launch_bounds__(256) void k_seg_sort_w(const uint32_t *off, uint32_
   for (int64_t t = (int64_t)blockIdx.x * 4 + w; t < tasks; t += (int64_t)gridDim.x * 4) {
     if (t < n512) {
       const uint32_t x = lists[CL_W512 * lcap + t];
-      team_sort_regs<64, 8>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], nullptr,
-                            lds[w], nullptr);
+      team_sort_regs<64, 8, PK>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], nullptr,
+                                lds[w], nullptr, wb);
     } else if (t < n512 + n256) {
       const uint32_t x = lists[CL_W256 * lcap + (t - n512)];
-      team_sort_regs<64, 4>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], nullptr,
-                            lds[w], nullptr);
+      team_sort_regs<64, 4, PK>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], nullptr,
+                                lds[w], nullptr, wb);
     } else {
       const uint32_t x = lists[CL_W128 * lcap + (t - n512 - n256)];
-      team_sort_regs<64, 2>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], nullptr,
-                            lds[w], nullptr);
+      team_sort_regs<64, 2, PK>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], nullptr,
+                                lds[w], nullptr, wb);
     }
   }
 }
@@ -743,25 +781,27 @@ __global__ __launch_bounds__(256) void k_seg_sort_w(const uint32_t *off, uint32_
 // of 1024, 2048 or 4096), 32 KB of LDS
 // One size class per instance (its own register and LDS budget: the 1K and 2K classes run
 // at a higher occupancy than the 4K class would allow them).
-template <int R, int CL>
+template <int R, int CL, bool PK>
 __global__ __launch_bounds__(256) void k_seg_sort_g(const uint32_t *off, uint32_t *ekey,
                                                     uint32_t *ew, const uint32_t *lists,
                                                     int64_t lcap, const uint32_t *ncl,
-                                                    uint32_t *u) {
+                                                    uint32_t *u, int wb) {
   __shared__ uint32_t xk[256 * R], xv[256 * R];
   __shared__ uint32_t sc4[4];
   const int64_t n = ncl[CL];
   for (int64_t t = blockIdx.x; t < n; t += gridDim.x) {
     const uint32_t x = lists[CL * lcap + t];
-    team_sort_regs<256, R>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], xk, xv, sc4);
+    team_sort_regs<256, R, PK>(ekey, ew, off[x], (int)(off[x + 1] - off[x]), &u[x], xk, xv, sc4,
+                               wb);
   }
 }
 
 // Class C: weights of the segment's keys in [c0, c0 + cn) counted in LDS by a workgroup of
 // 512 threads (four entries in flight per thread).  Ends with a barrier: every read of the
-// segment is complete.
+// segment is complete.  PK: packed entries (key << wb | weight).
+template <bool PK>
 GSAP_D void seg_range_count(uint32_t *cnt, const uint32_t *ekey, const uint32_t *ew,
-                            uint32_t s0, uint32_t s1, uint32_t c0, uint32_t cn) {
+                            uint32_t s0, uint32_t s1, uint32_t c0, uint32_t cn, int wb) {
   for (uint32_t i = threadIdx.x; i < cn; i += 512) cnt[i] = 0;
   __syncthreads();
   for (uint64_t e0 = (uint64_t)s0 + threadIdx.x; e0 < s1; e0 += 4 * 512) {
@@ -769,8 +809,14 @@ GSAP_D void seg_range_count(uint32_t *cnt, const uint32_t *ekey, const uint32_t 
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const uint64_t e = e0 + (uint64_t)j * 512;
-      k[j] = e < s1 ? ekey[e] - c0 : 0xFFFFFFFFu;
-      w[j] = e < s1 ? ew[e] : 0u;
+      if constexpr (PK) {
+        const uint32_t p = e < s1 ? ekey[e] : 0u;
+        k[j] = e < s1 ? (p >> wb) - c0 : 0xFFFFFFFFu;
+        w[j] = p & ((1u << wb) - 1u);
+      } else {
+        k[j] = e < s1 ? ekey[e] - c0 : 0xFFFFFFFFu;
+        w[j] = e < s1 ? ew[e] : 0u;
+      }
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j)
@@ -836,17 +882,18 @@ GSAP_D uint32_t seg_emit(const uint32_t *cnt, uint32_t cn, uint32_t c0, uint32_t
 
 // B <= SEG_RC: one workgroup per class-C segment, all B keys in one LDS range; the runs
 // overwrite the segment's start (its reads are complete after the count).
+template <bool PK>
 __global__ __launch_bounds__(512) void k_seg_c1(const uint32_t *off, uint32_t *ekey,
                                                 uint32_t *ew, const uint32_t *lists,
                                                 int64_t lcap, const uint32_t *ncl, int32_t B,
-                                                uint32_t *u) {
+                                                uint32_t *u, int wb) {
   extern __shared__ uint32_t cnt[];
   __shared__ uint32_t red8[8];
   const int64_t n = ncl[CL_C];
   for (int64_t t = blockIdx.x; t < n; t += gridDim.x) {
     const uint32_t x = lists[CL_C * lcap + t];
     const uint32_t s0 = off[x];
-    seg_range_count(cnt, ekey, ew, s0, off[x + 1], 0, (uint32_t)B);
+    seg_range_count<PK>(cnt, ekey, ew, s0, off[x + 1], 0, (uint32_t)B, wb);
     const uint32_t m = seg_emit(cnt, (uint32_t)B, 0, s0, red8, ekey, ew, nullptr, nullptr,
                                 nullptr, 0, nullptr);
     if (threadIdx.x == 0) u[x] = m;
@@ -854,11 +901,12 @@ __global__ __launch_bounds__(512) void k_seg_c1(const uint32_t *off, uint32_t *e
 }
 
 // B > SEG_RC, pass 1: nonzero count of every (class-C segment, column range)
+template <bool PK>
 __global__ __launch_bounds__(512) void k_seg_c_count(const uint32_t *off, const uint32_t *ekey,
                                                      const uint32_t *ew, const uint32_t *lists,
                                                      int64_t lcap, const uint32_t *ncl,
                                                      int32_t B, int32_t nr, uint32_t *cntc,
-                                                     uint32_t *u) {
+                                                     uint32_t *u, int wb) {
   extern __shared__ uint32_t cnt[];
   __shared__ uint32_t red8[8];
   const uint32_t RC = B < SEG_RC ? (uint32_t)B : (uint32_t)SEG_RC;
@@ -868,7 +916,7 @@ __global__ __launch_bounds__(512) void k_seg_c_count(const uint32_t *off, const 
     const uint32_t q = (uint32_t)(it - k * nr);
     const uint32_t x = lists[CL_C * lcap + k];
     const uint32_t c0 = q * RC, cn = min(RC, (uint32_t)B - c0);
-    seg_range_count(cnt, ekey, ew, off[x], off[x + 1], c0, cn);
+    seg_range_count<PK>(cnt, ekey, ew, off[x], off[x + 1], c0, cn, wb);
     uint32_t nz = 0;
     for (uint32_t i = threadIdx.x; i < cn; i += 512) nz += cnt[i] != 0;
     nz = block_sum512(nz, red8);
@@ -881,6 +929,7 @@ __global__ __launch_bounds__(512) void k_seg_c_count(const uint32_t *off, const 
 
 // B > SEG_RC, pass 2: recount each (segment, range) and write its nonzeros at the
 // segment's final position plus the earlier ranges' counts
+template <bool PK>
 __global__ __launch_bounds__(512) void k_seg_c_write(const uint32_t *off, const uint32_t *ekey,
                                                      const uint32_t *ew, const uint32_t *lists,
                                                      int64_t lcap, const uint32_t *ncl,
@@ -889,7 +938,7 @@ __global__ __launch_bounds__(512) void k_seg_c_write(const uint32_t *off, const 
                                                      int32_t *col, int32_t *val, int32_t *erow,
                                                      int32_t *row, int32_t *cval,
                                                      int32_t *dense, int32_t *gcol,
-                                                     int32_t *grow) {
+                                                     int32_t *grow, int wb) {
   extern __shared__ uint32_t cnt[];
   __shared__ uint32_t red8[8];
   const uint32_t RC = B < SEG_RC ? (uint32_t)B : (uint32_t)SEG_RC;
@@ -906,7 +955,7 @@ __global__ __launch_bounds__(512) void k_seg_c_write(const uint32_t *off, const 
     pre = block_sum512(pre, red8);
     const uint32_t pos = (isrow ? uoff[x] : uoff[x] - nnz) + pre;
     const uint32_t c0 = q * RC, cn = min(RC, (uint32_t)B - c0);
-    seg_range_count(cnt, ekey, ew, off[x], off[x + 1], c0, cn);
+    seg_range_count<PK>(cnt, ekey, ew, off[x], off[x + 1], c0, cn, wb);
     if (isrow)
       seg_emit(cnt, cn, c0, pos, red8, nullptr, nullptr, col, val, erow, (int32_t)x,
                dense ? dense + (int64_t)x * B : nullptr, gcol);
@@ -983,8 +1032,18 @@ int64_t seg_cntc_words(int64_t N, int64_t E) {
   return (2 * E / (SEG_G + 1) + 1) * nr;
 }
 
-static void seg_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
-                     RebuildScratch &rs, bool dense, hipStream_t st) {
+// Bits of a packed segment entry's weight (key << wb | weight) when every entry and the
+// padding key fit: wmax < 2^wb and B << wb < 2^32 (then 0xFFFFFFFF lies above every packed
+// entry and, shifted, above every block id); 0 otherwise (keys and weights in two words).
+static int seg_pack_bits(int32_t B, int32_t wmax) {
+  int b = 1;
+  while (b < 31 && ((uint32_t)wmax >> b) != 0) ++b;
+  return ((uint64_t)B << b) < (1ull << 32) ? b : 0;
+}
+
+template <bool PK>
+static void seg_path_t(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
+                       RebuildScratch &rs, bool dense, int wb, hipStream_t st) {
   const int64_t N = g.N, E = g.E, nseg = 2 * (int64_t)B;
   const int T = 256;
   uint32_t *ekey = (uint32_t *)rs.k0, *ew = (uint32_t *)rs.k1;
@@ -1004,35 +1063,42 @@ static void seg_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
   scan_exclusive(rs.soff, rs.soff, nseg, rs.soff + nseg, rs.tmp, st);
   k_seg_fill<<<grid_for(2 * E, T), T, 0, st>>>(lab, g.esrc, g.optr, g.oadj, g.ow, g.idst, g.iptr,
                                                g.iadj, g.iw, N, E, B, rs.soff, rs.vrel, ekey, ew,
-                                               rs.sseg);
+                                               rs.sseg, wb);
   k_seg_classify<<<grid_for(nseg, T, SEG_GRID), T, 0, st>>>(rs.soff, nseg, one ? 1 : 0, rs.lists,
                                                             rs.lcap, rs.ncl);
   if (one) {
-    k_seg_c1<<<SEG_GRID, 512, (size_t)RC * 4, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl,
-                                                    B, rs.su);
+    k_seg_c1<PK><<<SEG_GRID, 512, (size_t)RC * 4, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
+                                                        rs.ncl, B, rs.su, wb);
   } else {
-    k_seg_sort_g<16, CL_G4K><<<SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
-                                                       rs.ncl, rs.su);
-    k_seg_sort_g<8, CL_G2K><<<SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
-                                                      rs.ncl, rs.su);
-    k_seg_sort_g<4, CL_G1K><<<2 * SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
-                                                          rs.ncl, rs.su);
-    k_seg_c_count<<<SEG_GRID / 2, 512, (size_t)RC * 4, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
-                                                             rs.ncl, B, nr, rs.cntc, rs.su);
+    k_seg_sort_g<16, CL_G4K, PK><<<SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
+                                                           rs.ncl, rs.su, wb);
+    k_seg_sort_g<8, CL_G2K, PK><<<SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
+                                                          rs.ncl, rs.su, wb);
+    k_seg_sort_g<4, CL_G1K, PK><<<2 * SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists,
+                                                              rs.lcap, rs.ncl, rs.su, wb);
+    k_seg_c_count<PK><<<SEG_GRID / 2, 512, (size_t)RC * 4, st>>>(
+        rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl, B, nr, rs.cntc, rs.su, wb);
   }
-  k_seg_sort_w<<<SEG_GRID * 3 / 2, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl,
-                                               rs.su);
-  k_seg_sort_reg<<<SEG_GRID * 2, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl,
-                                               rs.su);
+  k_seg_sort_w<PK><<<SEG_GRID * 3 / 2, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
+                                                   rs.ncl, rs.su, wb);
+  k_seg_sort_reg<PK><<<SEG_GRID * 2, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl,
+                                                   rs.su, wb);
   scan_exclusive(rs.su, rs.suoff, nseg, rs.suoff + nseg, rs.tmp, st);
   k_seg_compact<<<grid_for((2 * E + 3) / 4, T), T, 0, st>>>(
       rs.soff, ekey, ew, rs.sseg, rs.suoff, B, one ? 1 : 0, 2 * E, bm.rowptr, bm.col, bm.val,
       bm.erow, bm.colptr, bm.row, bm.cval, D, gcol, grow);
   if (!one)
-    k_seg_c_write<<<SEG_GRID / 2, 512, (size_t)RC * 4, st>>>(
+    k_seg_c_write<PK><<<SEG_GRID / 2, 512, (size_t)RC * 4, st>>>(
         rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl, B, nr, rs.cntc, rs.suoff, bm.col, bm.val,
-        bm.erow, bm.row, bm.cval, D, gcol, grow);
+        bm.erow, bm.row, bm.cval, D, gcol, grow, wb);
   k_logs<<<grid_for(B, 256), 256, 0, st>>>(bm.dout, bm.din, B, bm.ldout, bm.ldin);
+}
+
+static void seg_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm, RebuildScratch &rs,
+                     bool dense, hipStream_t st) {
+  const int wb = seg_pack_bits(B, g.wmax);
+  if (wb) seg_path_t<true>(g, lab, B, bm, rs, dense, wb, st);
+  else seg_path_t<false>(g, lab, B, bm, rs, dense, 0, st);
 }
 
 // 128 < B <= tile_max_B (dense): vertices grouped by block; workgroup (row tile, chunk) counts

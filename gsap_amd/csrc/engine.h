@@ -15,6 +15,7 @@ struct GraphDev {
   int32_t *kout = nullptr, *kin = nullptr, *esrc = nullptr;
   int32_t *idst = nullptr;  // owner (target vertex) of every in-CSR entry
   int32_t *sloop = nullptr; // per vertex: summed weight of its self loops
+  int32_t wmax = 1;          // largest edge weight
   // nodal hash tables for vertices whose adjacency does not fit the LDS table
   int64_t *hoff = nullptr;       // per vertex: offset into hpool (words), -1 when light
   uint8_t *hbits_o = nullptr;    // per vertex: log2 capacity of out table (0 = light)

@@ -317,14 +317,19 @@ void upload_graph(gsap_ctx *c, const gsap_graph_i32 *g) {
   c->N = N; c->E = E;
   hipStream_t st = c->st;
   std::vector<int32_t> kout(N, 0), kin(N, 0), sloop(N, 0);
-  parallel_chunks(N, host_threads(), [&](int64_t lo, int64_t hi, int) {
+  const int nt = host_threads();
+  std::vector<int32_t> wmx(nt, 1);
+  parallel_chunks(N, nt, [&](int64_t lo, int64_t hi, int t) {
+    int32_t m = 1;
     for (int64_t v = lo; v < hi; ++v) {
       for (int32_t j = g->out_ptr[v]; j < g->out_ptr[v + 1]; ++j) {
         kout[v] += g->out_w[j];
+        m = std::max(m, g->out_w[j]);
         if (g->out_adj[j] == v) sloop[v] += g->out_w[j];
       }
       for (int32_t j = g->in_ptr[v]; j < g->in_ptr[v + 1]; ++j) kin[v] += g->in_w[j];
     }
+    wmx[t] = m;
   });
   // vertices of the largest nodal class get tables and slot arrays in the global pool
   // (used when they do not fit the launch's LDS); offsets in int32 words, even
@@ -366,6 +371,7 @@ void upload_graph(gsap_ctx *c, const gsap_graph_i32 *g) {
   d.kout = c->kout.as<int32_t>(); d.kin = c->kin.as<int32_t>(); d.esrc = c->esrc.as<int32_t>();
   d.idst = c->idst.as<int32_t>();
   d.sloop = c->sloop.as<int32_t>();
+  d.wmax = *std::max_element(wmx.begin(), wmx.end());
   d.hoff = c->hoff.as<int64_t>(); d.hbits_o = c->hbo.as<uint8_t>(); d.hbits_i = c->hbi.as<uint8_t>();
   d.hpool = c->hpool.as<int32_t>();
 }

@@ -170,6 +170,37 @@ def test_c3_blockmodel_and_batch(ctx, c3, B, dense):
     assert eq_bits(gb["dE"], rb["dE"])
 
 
+def heavy_graph():
+    """65536 vertices, 60000 edges of weight 1..4 and one of weight 300000: 19 weight bits,
+    so the segmented rebuild packs (block, weight) into one word only while B < 8192."""
+    g = random_graph(65536, 60000, 12, wmax=4)
+    src = np.repeat(np.arange(g.N), np.diff(g.out_ptr))
+    w = g.out_w.copy()
+    w[7] = 300000
+    return gsap_amd.Graph.from_edges(g.N, src, g.out_adj, w)
+
+
+@pytest.fixture(scope="module")
+def heavy():
+    return heavy_graph()
+
+
+@pytest.mark.parametrize("B,big", [(65536, 0.0), (20000, 0.3), (9000, 0.3), (3000, 0.05),
+                                   (9000, 0.0)])
+def test_blockmodel_unpacked_and_packed(ctx, heavy, B, big):
+    """Segment entries as separate key and weight words (B >= 8192 here: sorts, one-range
+    and two-range counting) and packed into one word (B < 8192), on a skewed labelling."""
+    g = heavy
+    rng = np.random.default_rng(13)
+    lab = rng.integers(1, B, g.N).astype(np.int32) if B < g.N else np.arange(g.N, dtype=np.int32)
+    lab[rng.random(g.N) < big] = 0
+    print(f"heavy blockmodel B={B} big={big}", flush=True)
+    got = ctx.blockmodel(g, lab, B, Options(dense_max_B=1, tile_max_B=1))
+    ref = O.blockmodel(g, lab, B)
+    for k in ("nnz", "rowptr", "col", "val", "colptr", "row", "cval", "dout", "din"):
+        assert np.array_equal(got[k], ref[k]), k
+
+
 @pytest.mark.parametrize("B,big,dense", [(20000, 0.3, 0), (20000, 0.3, 30000), (3000, 0.05, 0),
                                          (300, 0.0, 0), (300, 0.0, 1), (129, 0.5, 0)])
 def test_c3_blockmodel_skewed(ctx, c3, B, big, dense):
