@@ -170,7 +170,7 @@ __global__ __launch_bounds__(1024) void k_small_bm(const int32_t *D, int32_t B, 
 constexpr int VT = 2048;  // vertices per workgroup
 constexpr int TILE_MAX_B = 8192;  // largest B of the row-tile path
 constexpr int INC_MAX_B = 8192;   // largest B of the incremental dense update
-constexpr int INC_MIN_B = 512;    // its global-atomic form runs above this B
+constexpr int INC_TILE_B = 512;   // its LDS row-tile form runs up to this B, global atomics above
 
 __global__ __launch_bounds__(256) void k_label_hist(LabView lab, int64_t N, int32_t B,
                                                     uint32_t *H, int32_t W) {
@@ -1256,22 +1256,79 @@ __global__ __launch_bounds__(512) void k_dense_moves_lds(const int32_t *optr, co
   }
 }
 
+// SMALL_B < B: workgroup (row tile, vertex chunk) sums the edge moves of its chunk's moved
+// vertices that land in its MT_WORDS / B rows in an LDS copy of those rows, then adds them to
+// D with one global atomic per changed cell.  Every chunk is read by every tile (the moved
+// vertices are a small part of a batch), in exchange for no contention on hot cells.
+constexpr int MT_WORDS = 16384;
+__global__ __launch_bounds__(512) void k_dense_moves_tile(const int32_t *optr, const int32_t *oadj,
+                                                          const int32_t *ow, const int32_t *iptr,
+                                                          const int32_t *iadj, const int32_t *iw,
+                                                          const int32_t *lab, const int32_t *olab,
+                                                          int64_t lo, int64_t BS, int32_t B,
+                                                          int32_t nchunk, int32_t *D) {
+  __shared__ int32_t tl[MT_WORDS];
+  const int32_t RT = MT_WORDS / B;
+  const int32_t tile = (int32_t)blockIdx.x / nchunk, chunk = (int32_t)blockIdx.x % nchunk;
+  const int32_t r0 = tile * RT, r1 = min(B, r0 + RT);
+  const int32_t cells = (r1 - r0) * B;
+  for (int32_t k = threadIdx.x; k < cells; k += 512) tl[k] = 0;
+  __syncthreads();
+  const int64_t c0 = BS * chunk / nchunk, c1 = BS * (chunk + 1) / nchunk;
+  const int q = threadIdx.x & 15;
+  for (int64_t i = c0 + (threadIdx.x >> 4); i < c1; i += 32) {
+    const int64_t x = lo + i;
+    const int32_t xo = olab[i], xn = lab[x];
+    if (xo == xn) continue;
+    const bool ho = xo >= r0 && xo < r1, hn = xn >= r0 && xn < r1;
+    if (ho || hn) {
+      for (int32_t j = optr[x] + q; j < optr[x + 1]; j += 16) {
+        const int32_t y = oadj[j], w = ow[j];
+        const int64_t k = (int64_t)y - lo;
+        const int32_t yn = lab[y];
+        const int32_t yo = (k >= 0 && k < BS) ? olab[k] : yn;
+        if (ho) atomicSub(&tl[(xo - r0) * B + yo], w);
+        if (hn) atomicAdd(&tl[(xn - r0) * B + yn], w);
+      }
+    }
+    for (int32_t j = iptr[x] + q; j < iptr[x + 1]; j += 16) {
+      const int32_t y = iadj[j], w = iw[j];
+      const int64_t k = (int64_t)y - lo;
+      const int32_t yl = lab[y];
+      if (yl < r0 || yl >= r1 || (k >= 0 && k < BS && olab[k] != yl)) continue;
+      atomicSub(&tl[(yl - r0) * B + xo], w);
+      atomicAdd(&tl[(yl - r0) * B + xn], w);
+    }
+  }
+  __syncthreads();
+  for (int32_t k = threadIdx.x; k < cells; k += 512) {
+    const int32_t d = tl[k];
+    if (d != 0) atomicAdd(&D[(int64_t)r0 * B + k], d);
+  }
+}
+
 bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t *olab, int64_t lo,
                          int64_t BS, int32_t B, BMBuf &bm, RebuildScratch &rs, hipStream_t st) {
-  // the dense buffer must hold the block matrix of the labels before the batch; between
-  // SMALL_B and INC_MIN_B neither form of the update beats the full rebuild (LDS cannot hold
-  // the matrix, global atomics contend on few cells)
+  // the dense buffer must hold the block matrix of the labels before the batch; the moves
+  // are summed in an LDS copy of the matrix (B <= SMALL_B) or of a tile of its rows
+  // (B <= INC_TILE_B, where global atomics would contend on few cells), or added with global
+  // atomics (spread over many cells at larger B)
   if (!(bm.dense_valid && bm.dense_B == B && bm.B == B && B <= INC_MAX_B &&
-        (B <= SMALL_B || B > INC_MIN_B) && (int64_t)B * B <= rs.partials_cap))
+        (int64_t)B * B <= rs.partials_cap))
     return false;
-  if (BS > 0 && B <= SMALL_B) {
-    const unsigned grid = (unsigned)std::min<int64_t>(512, (BS + 31) / 32);
+  const unsigned grid = (unsigned)std::min<int64_t>(512, (BS + 31) / 32);
+  if (BS > 0 && B <= SMALL_B)
     k_dense_moves_lds<<<grid, 512, (size_t)B * B * 4, st>>>(
         g.optr, g.oadj, g.ow, g.iptr, g.iadj, g.iw, lab32, olab, lo, BS, B, bm.dense);
-  } else if (BS > 0) {
+  else if (BS > 0 && B <= INC_TILE_B) {
+    const int32_t tiles = (B + MT_WORDS / B - 1) / (MT_WORDS / B);
+    const int32_t nchunk = (int32_t)std::max<int64_t>(1, std::min<int64_t>(512 / tiles, BS / 64));
+    k_dense_moves_tile<<<(unsigned)(tiles * nchunk), 512, 0, st>>>(
+        g.optr, g.oadj, g.ow, g.iptr, g.iadj, g.iw, lab32, olab, lo, BS, B, nchunk, bm.dense);
+  }
+  else if (BS > 0)
     k_dense_moves<<<(unsigned)((BS + 15) / 16), 256, 0, st>>>(
         g.optr, g.oadj, g.ow, g.iptr, g.iadj, g.iw, lab32, olab, lo, BS, B, bm.dense);
-  }
   // (the 16-bit label copy was kept current by the batch's apply)
   if (B <= SMALL_B)
     k_small_bm<<<1, 1024, 0, st>>>(bm.dense, B, bm.rowptr, bm.col, bm.val, bm.erow, bm.colptr,
