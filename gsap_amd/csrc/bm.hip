@@ -1268,37 +1268,48 @@ __global__ __launch_bounds__(512) void k_dense_moves_tile(const int32_t *optr, c
                                                           int64_t lo, int64_t BS, int32_t B,
                                                           int32_t nchunk, int32_t *D) {
   __shared__ int32_t tl[MT_WORDS];
+  __shared__ int32_t mv[512];
+  __shared__ int32_t nmv;
   const int32_t RT = MT_WORDS / B;
   const int32_t tile = (int32_t)blockIdx.x / nchunk, chunk = (int32_t)blockIdx.x % nchunk;
   const int32_t r0 = tile * RT, r1 = min(B, r0 + RT);
   const int32_t cells = (r1 - r0) * B;
   for (int32_t k = threadIdx.x; k < cells; k += 512) tl[k] = 0;
-  __syncthreads();
   const int64_t c0 = BS * chunk / nchunk, c1 = BS * (chunk + 1) / nchunk;
   const int q = threadIdx.x & 15;
-  for (int64_t i = c0 + (threadIdx.x >> 4); i < c1; i += 32) {
-    const int64_t x = lo + i;
-    const int32_t xo = olab[i], xn = lab[x];
-    if (xo == xn) continue;
-    const bool ho = xo >= r0 && xo < r1, hn = xn >= r0 && xn < r1;
-    if (ho || hn) {
-      for (int32_t j = optr[x] + q; j < optr[x + 1]; j += 16) {
-        const int32_t y = oadj[j], w = ow[j];
+  // rounds of 512 vertices: the moved ones listed in LDS, then a group of 16 lanes each
+  for (int64_t b0 = c0; b0 < c1; b0 += 512) {
+    if (threadIdx.x == 0) nmv = 0;
+    __syncthreads();
+    const int64_t iv = b0 + threadIdx.x;
+    if (iv < c1 && olab[iv] != lab[lo + iv]) mv[atomicAdd(&nmv, 1)] = (int32_t)(iv - b0);
+    __syncthreads();
+    const int32_t nm = nmv;
+    for (int32_t m = threadIdx.x >> 4; m < nm; m += 32) {
+      const int64_t i = b0 + mv[m];
+      const int64_t x = lo + i;
+      const int32_t xo = olab[i], xn = lab[x];
+      const bool ho = xo >= r0 && xo < r1, hn = xn >= r0 && xn < r1;
+      if (ho || hn) {
+        for (int32_t j = optr[x] + q; j < optr[x + 1]; j += 16) {
+          const int32_t y = oadj[j], w = ow[j];
+          const int64_t k = (int64_t)y - lo;
+          const int32_t yn = lab[y];
+          const int32_t yo = (k >= 0 && k < BS) ? olab[k] : yn;
+          if (ho) atomicSub(&tl[(xo - r0) * B + yo], w);
+          if (hn) atomicAdd(&tl[(xn - r0) * B + yn], w);
+        }
+      }
+      for (int32_t j = iptr[x] + q; j < iptr[x + 1]; j += 16) {
+        const int32_t y = iadj[j], w = iw[j];
         const int64_t k = (int64_t)y - lo;
-        const int32_t yn = lab[y];
-        const int32_t yo = (k >= 0 && k < BS) ? olab[k] : yn;
-        if (ho) atomicSub(&tl[(xo - r0) * B + yo], w);
-        if (hn) atomicAdd(&tl[(xn - r0) * B + yn], w);
+        const int32_t yl = lab[y];
+        if (yl < r0 || yl >= r1 || (k >= 0 && k < BS && olab[k] != yl)) continue;
+        atomicSub(&tl[(yl - r0) * B + xo], w);
+        atomicAdd(&tl[(yl - r0) * B + xn], w);
       }
     }
-    for (int32_t j = iptr[x] + q; j < iptr[x + 1]; j += 16) {
-      const int32_t y = iadj[j], w = iw[j];
-      const int64_t k = (int64_t)y - lo;
-      const int32_t yl = lab[y];
-      if (yl < r0 || yl >= r1 || (k >= 0 && k < BS && olab[k] != yl)) continue;
-      atomicSub(&tl[(yl - r0) * B + xo], w);
-      atomicAdd(&tl[(yl - r0) * B + xn], w);
-    }
+    __syncthreads();  // the list is read before the next round rewrites it
   }
   __syncthreads();
   for (int32_t k = threadIdx.x; k < cells; k += 512) {
