@@ -412,7 +412,8 @@ __global__ __launch_bounds__(256) void k_seg_reserve(LabView lab, const int32_t 
 // Edge-parallel: out-edge e goes to its source's slice of the source block's row segment,
 // keyed by the target's block; in-edge f to its target's slice of the target block's column
 // segment, keyed by the source's block; seg[pos] records the segment of every entry.  With
-// wb > 0 an entry is one word, key << wb | weight (ew untouched).
+// wb > 0 an entry is one word, key << wb | weight (ew untouched).  ow = iw = nullptr: unit
+// weights.
 __global__ __launch_bounds__(256) void k_seg_fill(LabView lab, const int32_t *esrc,
                                                   const int32_t *optr, const int32_t *oadj,
                                                   const int32_t *ow, const int32_t *idst,
@@ -427,7 +428,7 @@ __global__ __launch_bounds__(256) void k_seg_fill(LabView lab, const int32_t *es
       const int32_t v = esrc[e];
       const uint32_t x = (uint32_t)lab[v];
       const uint32_t pos = off[x] + vrel[v] + (uint32_t)(e - optr[v]);
-      const uint32_t k = (uint32_t)lab[oadj[e]], w = (uint32_t)ow[e];
+      const uint32_t k = (uint32_t)lab[oadj[e]], w = ow ? (uint32_t)ow[e] : 1u;
       if (wb) {
         ekey[pos] = k << wb | w;
       } else {
@@ -440,7 +441,7 @@ __global__ __launch_bounds__(256) void k_seg_fill(LabView lab, const int32_t *es
       const int32_t v = idst[f];
       const uint32_t x = (uint32_t)(B + lab[v]);
       const uint32_t pos = off[x] + vrel[N + v] + (uint32_t)(f - iptr[v]);
-      const uint32_t k = (uint32_t)lab[iadj[f]], w = (uint32_t)iw[f];
+      const uint32_t k = (uint32_t)lab[iadj[f]], w = iw ? (uint32_t)iw[f] : 1u;
       if (wb) {
         ekey[pos] = k << wb | w;
       } else {
@@ -1061,9 +1062,11 @@ static void seg_path_t(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
     k_seg_reserve<false><<<grid_for(N, T), T, 0, st>>>(lab, g.optr, g.iptr, g.kout, g.kin, N, B,
                                                        rs.soff, bm.dout, bm.din, rs.vrel);
   scan_exclusive(rs.soff, rs.soff, nseg, rs.soff + nseg, rs.tmp, st);
-  k_seg_fill<<<grid_for(2 * E, T), T, 0, st>>>(lab, g.esrc, g.optr, g.oadj, g.ow, g.idst, g.iptr,
-                                               g.iadj, g.iw, N, E, B, rs.soff, rs.vrel, ekey, ew,
-                                               rs.sseg, wb);
+  // unit weights are not read
+  const bool unit = g.wmax == 1;
+  k_seg_fill<<<grid_for(2 * E, T), T, 0, st>>>(lab, g.esrc, g.optr, g.oadj, unit ? nullptr : g.ow,
+                                               g.idst, g.iptr, g.iadj, unit ? nullptr : g.iw, N, E,
+                                               B, rs.soff, rs.vrel, ekey, ew, rs.sseg, wb);
   k_seg_classify<<<grid_for(nseg, T, SEG_GRID), T, 0, st>>>(rs.soff, nseg, one ? 1 : 0, rs.lists,
                                                             rs.lcap, rs.ncl);
   if (one) {
