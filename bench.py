@@ -362,7 +362,7 @@ def main():
     if os.path.exists(TRAFFIC_FILE):
         # the default-variant instance of the nodal kernel (reference delta, greedy accept)
         want = ("k_nodal_fused<true, 0>" if prec == gsap_amd.PRECISION_MIXED
-                else "k_nodal_fused<false, 0>")
+                else "k_nodal_fused5<false, 0>")
         kn = json.load(open(TRAFFIC_FILE)).get("kernels", {}).get(want)
         if kn:
             traffic = kn["read_bytes_per_call"] + kn["write_bytes_per_call"]
@@ -399,7 +399,7 @@ def main():
         "graph_generation_s": t_gen,
         "kernel_ms_per_step": {KCLASS[i]: float(kms[i]) for i in range(len(KCLASS))},
         "kernel_calls_per_step": {KCLASS[i]: int(kcalls[i]) for i in range(len(KCLASS))},
-        "roofline": {"bound": "hbm", "kernel": "k_nodal_fused", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": "k_nodal_fused5" if prec != gsap_amd.PRECISION_MIXED else "k_nodal_fused", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": abytes},

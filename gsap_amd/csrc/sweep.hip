@@ -358,8 +358,10 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
   const Philox x = philox((uint32_t)v, nodal_phase(a), a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
   const int32_t o0 = a.optr[v], no = a.optr[v + 1] - o0;
   const int32_t i0 = a.iptr[v], ni = a.iptr[v + 1] - i0;
-  const int64_t kout = a.kout[v], kin = a.kin[v];
-  const int64_t tot = kout + kin;
+  // weights, degrees and block-matrix entries stay below 2^30 (the host rejects larger total
+  // edge weights), so their sums and differences here fit int32 (fewer registers)
+  const int32_t kout = a.kout[v], kin = a.kin[v];
+  const int32_t tot = kout + kin;
   // first chunk of both adjacency lists and its neighbour blocks, and the degrees of r:
   // gathered before the proposal, which they do not depend on, so that their loads overlap
   // its chain (when the proposal is r itself they go unused)
@@ -369,7 +371,7 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
   const int32_t w_i0 = lane < ni ? a.iw[i0 + lane] : 0;
   const int32_t t_o0 = u_o0 >= 0 ? a.lab[u_o0] : -1;
   const int32_t t_i0 = u_i0 >= 0 ? a.lab[u_i0] : -1;
-  const int64_t dout_r = m.dout[r], din_r = m.din[r];
+  const int32_t dout_r = m.dout[r], din_r = m.din[r];
   const double ldo_r = m.ldout[r], ldi_r = m.ldin[r];
   int32_t s;
   if (tot == 0) {
@@ -418,7 +420,7 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
   ht_clear(ho, lane, 64);
   ht_clear(hi, lane, 64);
   // block degrees of s, needed after the tables
-  const int64_t dout_s = m.dout[s], din_s = m.din[s];
+  const int32_t dout_s = m.dout[s], din_s = m.din[s];
   const double ldo_s = m.ldout[s], ldi_s = m.ldin[s];
   wave_lds_sync();
   if (lane < no) ht_insert(ho, t_o0, w_o0, lane);
@@ -426,25 +428,25 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
   for (int32_t j = lane + 64; j < no; j += 64) ht_insert(ho, a.lab[a.oadj[o0 + j]], a.ow[o0 + j], j);
   for (int32_t j = lane + 64; j < ni; j += 64) ht_insert(hi, a.lab[a.iadj[i0 + j]], a.iw[i0 + j], j);
   wave_lds_sync();
-  const int64_t c_r = ht_weight(hi, r), c_s = ht_weight(hi, s);
+  const int32_t c_r = ht_weight(hi, r), c_s = ht_weight(hi, s);
   const bool r_in_a = ht_find(ho, r) >= 0, s_in_a = ht_find(ho, s) >= 0;
   constexpr bool EX = (VAR & VAR_EXACT) != 0, MH = (VAR & VAR_MH) != 0;
-  const int64_t sig = EX ? a.sloop[v] : 0;  // a self loop moves from (r, r) to (s, s)
+  const int32_t sig = EX ? a.sloop[v] : 0;  // a self loop moves from (r, r) to (s, s)
   const double l_dr = clog2_s((double)(dout_r - kout));
   const double l_ds = clog2_s((double)(dout_s + kout));
   const double l_ir = clog2_s((double)(din_r - kin));
   const double l_is = clog2_s((double)(din_s + kin));
   // M[r][r], M[s][r], M[r][s], M[s][s] for the untouched column counts, one lane each
-  int64_t corner = 0;
+  int32_t corner = 0;
   if (lane < 4) {
     const int32_t cc = lane < 2 ? r : s, rr = (lane & 1) ? s : r;
     corner = bm_col_get(m, cc, rr);
   }
 
   // rows r and s over the row slots; gathers shared by the FP32 and FP64 passes
-  auto row_slot = [&](int32_t j, int32_t &t, int64_t &Mr, int64_t &Ms, int64_t &R,
-                      int64_t &S) -> bool {
-    int64_t av = 0;
+  auto row_slot = [&](int32_t j, int32_t &t, int32_t &Mr, int32_t &Ms, int32_t &R,
+                      int32_t &S) -> bool {
+    int32_t av = 0;
     bool use;
     if (j < no) {
       t = j < 64 ? t_o0 : a.lab[a.oadj[o0 + j]];
@@ -468,7 +470,7 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
     return true;
   };
   // columns r and s over the column slots (t not in {r, s})
-  auto col_slot = [&](int32_t j, int32_t &t, int64_t &Mr, int64_t &Ms, int64_t &c) -> bool {
+  auto col_slot = [&](int32_t j, int32_t &t, int32_t &Mr, int32_t &Ms, int32_t &c) -> bool {
     t = j < 64 ? t_i0 : a.lab[a.iadj[i0 + j]];
     const int32_t slot = ht_find(hi, t);
     if (ld_relaxed(&hi.first[slot]) != j || t == r || t == s) return false;
@@ -479,13 +481,13 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
     Ms = Ms32;
     return true;
   };
-  int64_t sum_r = 0, sum_s = 0, csum_r = 0, csum_s = 0;
+  int32_t sum_r = 0, sum_s = 0, csum_r = 0, csum_s = 0;
   // FP64 terms (and, when sums is set, the integer sums of the touched entries)
   auto fp64_pass = [&](double &drow, double &dcol, bool sums) {
     double p_row = 0.0;
     for (int32_t j = lane; j < no + 2; j += 64) {
       int32_t t;
-      int64_t Mr, Ms, R, S;
+      int32_t Mr, Ms, R, S;
       if (!row_slot(j, t, Mr, Ms, R, S)) continue;
       const double ldt = m.ldin[t];
       // exact form: the new rows see the new column degrees of r and s
@@ -503,12 +505,12 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
     double p_col = 0.0;
     for (int32_t j = lane; j < ni; j += 64) {
       int32_t t;
-      int64_t Mr, Ms, c;
+      int32_t Mr, Ms, c;
       if (!col_slot(j, t, Mr, Ms, c)) continue;
       const double ldt = m.ldout[t];
       const double ir = Mr > 0 ? term_s(Mr, ldt, ldi_r) : 0.0;
       const double is = Ms > 0 ? term_s(Ms, ldt, ldi_s) : 0.0;
-      const int64_t wr = Mr - c, ws = Ms + c;
+      const int32_t wr = Mr - c, ws = Ms + c;
       const double nr = wr > 0 ? term_s(wr, ldt, l_ir) : 0.0;
       const double ns = ws > 0 ? term_s(ws, ldt, l_is) : 0.0;
       p_col += ((ir + is) - nr) - ns;
@@ -531,7 +533,7 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
   } else {
     for (int32_t j = lane; j < no + 2; j += 64) {
       int32_t t;
-      int64_t Mr, Ms, R, S;
+      int32_t Mr, Ms, R, S;
       if (!row_slot(j, t, Mr, Ms, R, S)) continue;
       sum_r += Mr;
       sum_s += Ms;
@@ -545,14 +547,14 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
     }
     for (int32_t j = lane; j < ni; j += 64) {
       int32_t t;
-      int64_t Mr, Ms, c;
+      int32_t Mr, Ms, c;
       if (!col_slot(j, t, Mr, Ms, c)) continue;
       csum_r += Mr;
       csum_s += Ms;
       const float ldt = (float)m.ldout[t];
       const float ir = Mr > 0 ? term32(Mr, ldt, (float)ldi_r, eb) : 0.0f;
       const float is = Ms > 0 ? term32(Ms, ldt, (float)ldi_s, eb) : 0.0f;
-      const int64_t wr = Mr - c, ws = Ms + c;
+      const int32_t wr = Mr - c, ws = Ms + c;
       const float nr = wr > 0 ? term32(wr, ldt, (float)l_ir, eb) : 0.0f;
       const float ns = ws > 0 ? term32(ws, ldt, (float)l_is, eb) : 0.0f;
       kcol.add(((ir + is) - nr) - ns);
@@ -562,13 +564,13 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
   sum_s = wave_sum_i64(sum_s);
   csum_r = wave_sum_i64(csum_r);
   csum_s = wave_sum_i64(csum_s);
-  const int64_t M_rr = __shfl(corner, 0, 64), M_sr = __shfl(corner, 1, 64);
-  const int64_t M_rs = __shfl(corner, 2, 64), M_ss = __shfl(corner, 3, 64);
-  const int64_t W_r = dout_r - sum_r, W_s = dout_s - sum_s;
+  const int32_t M_rr = __shfl(corner, 0, 64), M_sr = __shfl(corner, 1, 64);
+  const int32_t M_rs = __shfl(corner, 2, 64), M_ss = __shfl(corner, 3, 64);
+  const int32_t W_r = dout_r - sum_r, W_s = dout_s - sum_s;
   const double Y_r = W_r > 0 ? (ldo_r - l_dr) * (double)W_r : 0.0;
   const double Y_s = W_s > 0 ? (ldo_s - l_ds) * (double)W_s : 0.0;
-  const int64_t W2_r = din_r - M_rr - M_sr - csum_r;
-  const int64_t W2_s = din_s - M_rs - M_ss - csum_s;
+  const int32_t W2_r = din_r - M_rr - M_sr - csum_r;
+  const int32_t W2_s = din_s - M_rs - M_ss - csum_s;
   const double Y2_r = W2_r > 0 ? (ldi_r - l_ir) * (double)W2_r : 0.0;
   const double Y2_s = W2_s > 0 ? (ldi_s - l_is) * (double)W2_s : 0.0;
   double d32 = 0.0, bound = 0.0;
@@ -596,7 +598,7 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
   }
   double lh = 0.0;
   if constexpr (MH) {
-    const int64_t a_r = ht_weight(ho, r), a_s = ht_weight(ho, s);
+    const int32_t a_r = ht_weight(ho, r), a_s = ht_weight(ho, s);
     const Hast h{v, r, s, B, M_rs, M_sr, M_ss, M_rr - a_r - c_r + sig, M_rs - a_s + c_r - sig,
                  M_sr + a_r - c_s - sig, tot, dout_r + din_r, dout_s + din_s};
     double fo = 0.0, fi = 0.0, bo = 0.0, bi = 0.0;
@@ -1197,8 +1199,7 @@ struct NodalLists {
 // One launch per batch: the largest vertices first (one workgroup each), then 4 vertices
 // per workgroup (a wave each, 6 KB of LDS tables per wave), then groups of 32 and 16 lanes.
 template <bool MIX, int VAR>
-__global__ __launch_bounds__(256) void k_nodal_fused(NodalArgs a, NodalLists L) {
-  extern __shared__ double lds_d[];
+GSAP_D void nodal_fused(const NodalArgs &a, const NodalLists &L, double *lds_d) {
   log2_tables_to_lds();
   int32_t b = (int32_t)blockIdx.x;
   if (b < L.nb[3]) {
@@ -1221,6 +1222,21 @@ __global__ __launch_bounds__(256) void k_nodal_fused(NodalArgs a, NodalLists L) 
   }
   b -= L.nb[1];
   nodal_small<16, MIX, VAR>(a, L.list[0], L.n[0], b);
+}
+
+template <bool MIX, int VAR>
+__global__ __launch_bounds__(256) void k_nodal_fused(NodalArgs a, NodalLists L) {
+  extern __shared__ double lds_d[];
+  nodal_fused<MIX, VAR>(a, L, lds_d);
+}
+
+// The FP64 greedy forms fit 96 registers without spills: five waves per SIMD instead of four
+// (the mixed and Metropolis-Hastings forms would spill there).
+template <bool MIX, int VAR>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_nodal_fused5(
+    NodalArgs a, NodalLists L) {
+  extern __shared__ double lds_d[];
+  nodal_fused<MIX, VAR>(a, L, lds_d);
 }
 
 void launch_nodal(const GraphDev &g, LabView lab, const BMView &m, const int32_t *sof,
@@ -1259,9 +1275,9 @@ void launch_nodal(const GraphDev &g, LabView lab, const BMView &m, const int32_t
   else if (err)
     k_nodal_fused<true, VAR_EXACT><<<grid, 256, lds, st>>>(a, L);
   else if (!exact && !mh)
-    k_nodal_fused<false, 0><<<grid, 256, lds, st>>>(a, L);
+    k_nodal_fused5<false, 0><<<grid, 256, lds, st>>>(a, L);
   else if (!mh)
-    k_nodal_fused<false, VAR_EXACT><<<grid, 256, lds, st>>>(a, L);
+    k_nodal_fused5<false, VAR_EXACT><<<grid, 256, lds, st>>>(a, L);
   else if (!exact)
     k_nodal_fused<false, VAR_MH><<<grid, 256, lds, st>>>(a, L);
   else
