@@ -891,8 +891,8 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
 }
 
 // Group sums over the G lanes of a group (every lane gets the result).
-template <int G>
-GSAP_D int64_t group_allsum(int64_t x) {
+template <int G, class T>
+GSAP_D T group_allsum(T x) {
 #pragma unroll
   for (int off = G / 2; off >= 1; off >>= 1) x += __shfl_xor(x, off, G);
   return x;
@@ -917,8 +917,8 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
   const Philox x = philox((uint32_t)v, nodal_phase(a), a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
   const int32_t o0 = a.optr[v], no = a.optr[v + 1] - o0;
   const int32_t i0 = a.iptr[v], ni = a.iptr[v + 1] - i0;
-  const int64_t kout = a.kout[v], kin = a.kin[v];
-  const int64_t tot = kout + kin;
+  const int32_t kout = a.kout[v], kin = a.kin[v];
+  const int32_t tot = kout + kin;
   constexpr bool EX = (VAR & VAR_EXACT) != 0, MH = (VAR & VAR_MH) != 0;
   int32_t t_o = -1, w_o = 0, t_i = -1, w_i = 0, u_o = -1, u_i = -1;
   if (q < no) { w_o = a.ow[o0 + q]; u_o = a.oadj[o0 + q]; t_o = a.lab[u_o]; }
@@ -959,7 +959,7 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
   }
   // per-lane block weight and first-occurrence flag (Metropolis-Hastings also needs the
   // in weight of the lane's out block and the out weight of its in block)
-  int64_t a_sum = 0, c_sum = 0, c_of_o = 0, a_of_i = 0;
+  int32_t a_sum = 0, c_sum = 0, c_of_o = 0, a_of_i = 0;
   bool first_o = q < no, first_i = q < ni;
   const int pend = no > ni ? no : ni;
 #pragma unroll 1
@@ -973,26 +973,26 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
       if (tpo == t_i) a_of_i += wpo;
     }
   }
-  const int64_t sig = EX ? a.sloop[v] : 0;  // a self loop moves from (r, r) to (s, s)
-  const int64_t c_r = group_allsum<G>((q < ni && t_i == r) ? w_i : 0);
-  const int64_t c_s = group_allsum<G>((q < ni && t_i == s) ? w_i : 0);
+  const int32_t sig = EX ? a.sloop[v] : 0;  // a self loop moves from (r, r) to (s, s)
+  const int32_t c_r = group_allsum<G>((q < ni && t_i == r) ? w_i : 0);
+  const int32_t c_s = group_allsum<G>((q < ni && t_i == s) ? w_i : 0);
   const bool r_in_a = ((__ballot(q < no && t_o == r) >> gbase) & gmask) != 0;
   const bool s_in_a = ((__ballot(q < no && t_o == s) >> gbase) & gmask) != 0;
-  const int64_t dout_r = m.dout[r], dout_s = m.dout[s], din_r = m.din[r], din_s = m.din[s];
+  const int32_t dout_r = m.dout[r], dout_s = m.dout[s], din_r = m.din[r], din_s = m.din[s];
   const double ldo_r = m.ldout[r], ldo_s = m.ldout[s], ldi_r = m.ldin[r], ldi_s = m.ldin[s];
   const double l_dr = clog2_s((double)(dout_r - kout));
   const double l_ds = clog2_s((double)(dout_s + kout));
   const double l_ir = clog2_s((double)(din_r - kin));
   const double l_is = clog2_s((double)(din_s + kin));
-  int64_t corner = 0;
+  int32_t corner = 0;
   if (q < 4) {
     const int32_t cc = q < 2 ? r : s, rr = (q & 1) ? s : r;
     corner = bm_col_get(m, cc, rr);
   }
   // rows r and s over the row slots: positions 0..no-1 (first occurrences), no (r), no+1 (s)
-  auto row_slot = [&](int32_t j, int32_t &t, int64_t &Mr, int64_t &Ms, int64_t &R,
-                      int64_t &S) -> bool {
-    int64_t av = 0;
+  auto row_slot = [&](int32_t j, int32_t &t, int32_t &Mr, int32_t &Ms, int32_t &R,
+                      int32_t &S) -> bool {
+    int32_t av = 0;
     bool use;
     if (j < no) {
       t = t_o;
@@ -1016,13 +1016,13 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
   };
   // column slot of lane q (t not in {r, s})
   const bool col_use = first_i && t_i != r && t_i != s;
-  auto col_get = [&](int64_t &Mr, int64_t &Ms) {
+  auto col_get = [&](int32_t &Mr, int32_t &Ms) {
     int32_t Mr32, Ms32;
     bm_col_get2(m, r, s, t_i, Mr32, Ms32);
     Mr = Mr32;
     Ms = Ms32;
   };
-  int64_t sum_r = 0, sum_s = 0, csum_r = 0, csum_s = 0;
+  int32_t sum_r = 0, sum_s = 0, csum_r = 0, csum_s = 0;
   // positions j = q + jj G (jj < JJ cover 0 .. no + 1 <= G + 1); position j belongs to
   // canonical partial j mod 64, which is this lane's p[jj mod (64 / G)]
   constexpr int JJ = G == 64 ? 2 : 64 / G;
@@ -1034,7 +1034,7 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
       const int32_t j = q + jj * G;
       if (j >= no + 2) break;
       int32_t t;
-      int64_t Mr, Ms, R, S;
+      int32_t Mr, Ms, R, S;
       if (!row_slot(j, t, Mr, Ms, R, S)) continue;
       const double ldt = m.ldin[t];
       const double ldn = !EX ? ldt : t == r ? l_ir : t == s ? l_is : ldt;
@@ -1051,12 +1051,12 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
     CAcc<G> pcol;
     pcol.init();
     if (col_use) {
-      int64_t Mr, Ms;
+      int32_t Mr, Ms;
       col_get(Mr, Ms);
       const double ldt = m.ldout[t_i];
       const double ir = Mr > 0 ? term_s(Mr, ldt, ldi_r) : 0.0;
       const double is = Ms > 0 ? term_s(Ms, ldt, ldi_s) : 0.0;
-      const int64_t wr = Mr - c_sum, ws = Ms + c_sum;
+      const int32_t wr = Mr - c_sum, ws = Ms + c_sum;
       const double nr = wr > 0 ? term_s(wr, ldt, l_ir) : 0.0;
       const double ns = ws > 0 ? term_s(ws, ldt, l_is) : 0.0;
       pcol.p[0] += ((ir + is) - nr) - ns;
@@ -1081,7 +1081,7 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
       const int32_t j = q + jj * G;
       if (j >= no + 2) break;
       int32_t t;
-      int64_t Mr, Ms, R, S;
+      int32_t Mr, Ms, R, S;
       if (!row_slot(j, t, Mr, Ms, R, S)) continue;
       sum_r += Mr;
       sum_s += Ms;
@@ -1094,14 +1094,14 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
       krow.add(((or_ + os_) - nr) - ns);
     }
     if (col_use) {
-      int64_t Mr, Ms;
+      int32_t Mr, Ms;
       col_get(Mr, Ms);
       csum_r += Mr;
       csum_s += Ms;
       const float ldt = (float)m.ldout[t_i];
       const float ir = Mr > 0 ? term32(Mr, ldt, (float)ldi_r, eb) : 0.0f;
       const float is = Ms > 0 ? term32(Ms, ldt, (float)ldi_s, eb) : 0.0f;
-      const int64_t wr = Mr - c_sum, ws = Ms + c_sum;
+      const int32_t wr = Mr - c_sum, ws = Ms + c_sum;
       const float nr = wr > 0 ? term32(wr, ldt, (float)l_ir, eb) : 0.0f;
       const float ns = ws > 0 ? term32(ws, ldt, (float)l_is, eb) : 0.0f;
       kcol.add(((ir + is) - nr) - ns);
@@ -1111,13 +1111,13 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
   sum_s = group_allsum<G>(sum_s);
   csum_r = group_allsum<G>(csum_r);
   csum_s = group_allsum<G>(csum_s);
-  const int64_t M_rr = __shfl(corner, 0, G), M_sr = __shfl(corner, 1, G);
-  const int64_t M_rs = __shfl(corner, 2, G), M_ss = __shfl(corner, 3, G);
-  const int64_t W_r = dout_r - sum_r, W_s = dout_s - sum_s;
+  const int32_t M_rr = __shfl(corner, 0, G), M_sr = __shfl(corner, 1, G);
+  const int32_t M_rs = __shfl(corner, 2, G), M_ss = __shfl(corner, 3, G);
+  const int32_t W_r = dout_r - sum_r, W_s = dout_s - sum_s;
   const double Y_r = W_r > 0 ? (ldo_r - l_dr) * (double)W_r : 0.0;
   const double Y_s = W_s > 0 ? (ldo_s - l_ds) * (double)W_s : 0.0;
-  const int64_t W2_r = din_r - M_rr - M_sr - csum_r;
-  const int64_t W2_s = din_s - M_rs - M_ss - csum_s;
+  const int32_t W2_r = din_r - M_rr - M_sr - csum_r;
+  const int32_t W2_s = din_s - M_rs - M_ss - csum_s;
   const double Y2_r = W2_r > 0 ? (ldi_r - l_ir) * (double)W2_r : 0.0;
   const double Y2_s = W2_s > 0 ? (ldi_s - l_is) * (double)W2_s : 0.0;
   double d32 = 0.0, bound = 0.0;
@@ -1145,8 +1145,8 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
   }
   double lh = 0.0;
   if constexpr (MH) {
-    const int64_t a_r = group_allsum<G>((q < no && t_o == r) ? w_o : 0);
-    const int64_t a_s = group_allsum<G>((q < no && t_o == s) ? w_o : 0);
+    const int32_t a_r = group_allsum<G>((q < no && t_o == r) ? w_o : 0);
+    const int32_t a_s = group_allsum<G>((q < no && t_o == s) ? w_o : 0);
     const Hast h{v, r, s, B, M_rs, M_sr, M_ss, M_rr - a_r - c_r + sig, M_rs - a_s + c_r - sig,
                  M_sr + a_r - c_s - sig, tot, dout_r + din_r, dout_s + din_s};
     CAcc<G> fo, fi, bo, bi;
