@@ -179,7 +179,7 @@ template <int G>
 __global__ __launch_bounds__(256) void k_block_terms(BMView m, double *OUT, int32_t *sof,
                                                      uint8_t *coin, RngKey key, PhaseRef pr,
                                                      int sample) {
-  log2_tables_to_lds();
+  if (OUT) log2_tables_to_lds();  // only the row terms take logarithms
   const int32_t x = (int32_t)(blockIdx.x * (256 / G) + threadIdx.x / G);
   if (x >= m.B) return;
   const int q = threadIdx.x & (G - 1);
