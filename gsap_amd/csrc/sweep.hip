@@ -1793,11 +1793,11 @@ void launch_merge(const BMView &m, const double *OUT, RngKey key, uint32_t phase
     // long rows and enough blocks to fill the device: a workgroup per block with its row and
     // column in LDS; the few blocks beyond the staging arrays get a wave per proposal (their
     // count comes back to the host to size that launch)
-    const int32_t cap = avg_row <= 96 ? 512 : 2048;
+    const int32_t cap = avg_row <= 96 ? 512 : 1024;
     if (cap == 512)
       k_merge_blk<512><<<(unsigned)m.B, 256, 0, st>>>(m, OUT, key, phase, np, prop, dS, ex);
     else
-      k_merge_blk<2048><<<(unsigned)m.B, 256, 0, st>>>(m, OUT, key, phase, np, prop, dS, ex);
+      k_merge_blk<1024><<<(unsigned)m.B, 256, 0, st>>>(m, OUT, key, phase, np, prop, dS, ex);
     uint32_t nbig = 0;
     if (hipMemsetAsync(big_count, 0, 4, st) != hipSuccess) return;
     k_big_blocks<<<grid_for(m.B, 256), 256, 0, st>>>(m, cap, big_list, big_count);
