@@ -90,6 +90,20 @@ __global__ void k_logs(const int32_t *dout, const int32_t *din, int32_t B, doubl
   }
 }
 
+// Unit weights: block degrees are the row and column segment sizes (off: the scanned
+// segment offsets, 2B + 1), then their logs.
+__global__ void k_logs_seg(const uint32_t *off, int32_t B, int32_t *dout, int32_t *din,
+                           double *ldout, double *ldin) {
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < B;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t o = (int32_t)(off[x + 1] - off[x]), i = (int32_t)(off[B + x + 1] - off[B + x]);
+    dout[x] = o;
+    din[x] = i;
+    ldout[x] = clog2((double)o);
+    ldin[x] = clog2((double)i);
+  }
+}
+
 // Small block matrices (B <= 128): CSR, CSC, degrees and their logs from D in one
 // workgroup of 1024 threads (16 waves).  Row r (column c) is one wave's ballot compaction
 // of D's row (column) in ascending order.
@@ -363,8 +377,10 @@ __global__ __launch_bounds__(256) void k_seg_reserve(LabView lab, const int32_t 
       const int32_t no = optr[v + 1] - optr[v], ni = iptr[v + 1] - iptr[v];
       vrel[v] = no ? atomicAdd(&cnt[r], (uint32_t)no) : 0u;
       vrel[N + v] = ni ? atomicAdd(&cnt[B + r], (uint32_t)ni) : 0u;
-      if (kout[v]) atomicAdd(&dout[r], kout[v]);
-      if (kin[v]) atomicAdd(&din[r], kin[v]);
+      if (kout) {  // (unit weights: the degrees are the segment sizes, see k_logs_seg)
+        if (kout[v]) atomicAdd(&dout[r], kout[v]);
+        if (kin[v]) atomicAdd(&din[r], kin[v]);
+      }
     }
   } else {
     extern __shared__ uint32_t sh[];
@@ -1054,16 +1070,18 @@ static void seg_path_t(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
   int32_t *D = dense ? bm.dense : nullptr;
   // without the dense copy, the coarse lookup index is written with the entries
   int32_t *gcol = dense ? nullptr : bm.gcol, *grow = dense ? nullptr : bm.grow;
+  // unit weights are not read (degrees = segment sizes, entry weights = 1)
+  const bool unit = g.wmax == 1;
   k_seg_zero<<<grid_for(nseg + 1, T), T, 0, st>>>(rs.soff, rs.su, rs.ncl, bm.dout, bm.din, B);
   if (B <= SEG_PRIV_B)
     k_seg_reserve<true><<<(unsigned)((N + SEG_VT - 1) / SEG_VT), 256, (size_t)16 * B, st>>>(
         lab, g.optr, g.iptr, g.kout, g.kin, N, B, rs.soff, bm.dout, bm.din, rs.vrel);
   else
-    k_seg_reserve<false><<<grid_for(N, T), T, 0, st>>>(lab, g.optr, g.iptr, g.kout, g.kin, N, B,
-                                                       rs.soff, bm.dout, bm.din, rs.vrel);
+    k_seg_reserve<false><<<grid_for(N, T), T, 0, st>>>(lab, g.optr, g.iptr,
+                                                       unit ? nullptr : g.kout,
+                                                       unit ? nullptr : g.kin, N, B, rs.soff,
+                                                       bm.dout, bm.din, rs.vrel);
   scan_exclusive(rs.soff, rs.soff, nseg, rs.soff + nseg, rs.tmp, st);
-  // unit weights are not read
-  const bool unit = g.wmax == 1;
   k_seg_fill<<<grid_for(2 * E, T), T, 0, st>>>(lab, g.esrc, g.optr, g.oadj, unit ? nullptr : g.ow,
                                                g.idst, g.iptr, g.iadj, unit ? nullptr : g.iw, N, E,
                                                B, rs.soff, rs.vrel, ekey, ew, rs.sseg, wb);
@@ -1094,7 +1112,10 @@ static void seg_path_t(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
     k_seg_c_write<PK><<<SEG_GRID / 2, 512, (size_t)RC * 4, st>>>(
         rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl, B, nr, rs.cntc, rs.suoff, bm.col, bm.val,
         bm.erow, bm.row, bm.cval, D, gcol, grow, wb);
-  k_logs<<<grid_for(B, 256), 256, 0, st>>>(bm.dout, bm.din, B, bm.ldout, bm.ldin);
+  if (unit)
+    k_logs_seg<<<grid_for(B, 256), 256, 0, st>>>(rs.soff, B, bm.dout, bm.din, bm.ldout, bm.ldin);
+  else
+    k_logs<<<grid_for(B, 256), 256, 0, st>>>(bm.dout, bm.din, B, bm.ldout, bm.ldin);
 }
 
 static void seg_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm, RebuildScratch &rs,
