@@ -39,10 +39,10 @@ KCLASS = ["nodal", "rebuild", "block_terms", "merge", "mdl", "accept_sum"]
 KERNELS_OF = {
     "nodal": ["k_nodal"],
     "rebuild": ["k_label_", "k_tile_hist", "k_edge_hist", "k_hist_reduce", "k_dense_",
-                "k_small_bm", "k_transpose", "k_logs", "k_seg_", "k_scan_", "k_fill_i32",
-                "k_lab16"],
+                "k_small_bm", "k_transpose", "k_logs", "k_seg_", "k_scan_", "k_lab16"],
     "block_terms": ["k_block_terms"],
-    "merge": ["k_merge", "k_argmin", "k_order_keys", "k_radix_"],
+    "merge": ["k_merge", "k_argmin", "k_order_keys", "k_radix_", "k_big_blocks", "k_jump",
+              "k_relabel_roots"],
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c3_fp64_traffic.json")
 

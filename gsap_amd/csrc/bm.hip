@@ -24,12 +24,6 @@
 
 namespace gsap {
 
-__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = v;
-}
-
 // dense[r * B + c] = 0 for every entry (r, c) of the previous CSR (nnz on the device)
 __global__ void k_dense_clear(const int32_t *erow, const int32_t *col, const int32_t *nnz_dev,
                               int32_t B, int32_t *dense) {

@@ -128,9 +128,6 @@ void launch_mix_stats(const float *err, const uint8_t *near, int64_t n, uint32_t
 void launch_apply_sum(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo, int64_t BS,
                       bool all, int32_t *olab, uint16_t *lab16, double *tiles, double *out,
                       hipStream_t st);
-// canonical array sum: tiles of 1024 (n read from n_dev when non-null, else n_host)
-void launch_csum_array(const double *x, const int32_t *n_dev, int64_t n_host, int64_t n_max,
-                       double *tiles, double *out, hipStream_t st);
 // The end of a sweep: the description-length data term (tiles at tiles[0..)) into out[0] and
 // the deferred delta-S sums of the nb batches (tiles at tiles + boff + b * bstride) into
 // out[1 + b], one launch; ctr advanced by add afterwards.
@@ -161,7 +158,6 @@ void launch_merge_relabel(int32_t *par, int32_t B, uint32_t *rank, uint32_t *tmp
 void launch_iota(int32_t *x, int64_t n, hipStream_t st);
 // owner[j] = v for every entry j of vertex v's row of the CSR ptr[N + 1]
 void launch_owners(const int32_t *ptr, int64_t N, int32_t *owner, hipStream_t st);
-void launch_add_u32(uint32_t *x, uint32_t v, hipStream_t st);
 // keys[r] = order key of the double x[r], vals[r] = r
 void launch_order_keys(const double *x, int32_t n, uint64_t *keys, uint32_t *vals,
                        hipStream_t st);

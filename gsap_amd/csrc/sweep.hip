@@ -1322,22 +1322,6 @@ void launch_mix_stats(const float *err, const uint8_t *near, int64_t n, uint32_t
 
 
 // ---------------------------------------------------------- canonical sums
-__global__ __launch_bounds__(64) void k_tile_sums(const double *x, const int32_t *n_dev,
-                                                  int64_t n_host, double *tiles) {
-  const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
-  const int64_t t0 = (int64_t)blockIdx.x * 1024;
-  if (t0 >= n) return;
-  const int lane = threadIdx.x;
-  double p = 0.0;
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const int64_t i = t0 + lane + 64 * j;
-    if (i < n) p += x[i];
-  }
-  p = wave_sum(p);
-  if (lane == 0) tiles[blockIdx.x] = p;
-}
-
 // the sum of the tile sums (ctr, when given, advanced by add afterwards: the sweep's batch
 // counter, stream-ordered after the sweep's last read of it)
 __global__ __launch_bounds__(64) void k_final_sum(const double *tiles, const int32_t *n_dev,
@@ -1353,13 +1337,6 @@ __global__ __launch_bounds__(64) void k_final_sum(const double *tiles, const int
     *out = p;
     if (ctr) *ctr += add;
   }
-}
-
-void launch_csum_array(const double *x, const int32_t *n_dev, int64_t n_host, int64_t n_max,
-                       double *tiles, double *out, hipStream_t st) {
-  const int64_t nt = (n_max + 1023) / 1024;
-  if (nt > 0) k_tile_sums<<<(unsigned)nt, 64, 0, st>>>(x, n_dev, n_host, tiles);
-  k_final_sum<<<1, 64, 0, st>>>(tiles, n_dev, n_host, out, nullptr, 0);
 }
 
 // Labels of a nodal batch and the tile sums of its delta-S, one 1024-entry tile per
@@ -1898,8 +1875,6 @@ __global__ void k_iota(int32_t *x, int64_t n) {
   if (i < n) x[i] = (int32_t)i;
 }
 
-__global__ void k_add_u32(uint32_t *x, uint32_t v) { *x += v; }
-void launch_add_u32(uint32_t *x, uint32_t v, hipStream_t st) { k_add_u32<<<1, 1, 0, st>>>(x, v); }
 
 __global__ void k_order_keys(const double *x, int32_t n, uint64_t *keys, uint32_t *vals) {
   const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
