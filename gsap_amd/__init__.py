@@ -19,7 +19,7 @@ import numpy as np
 __all__ = [
     "GsapError", "Graph", "Options", "Report", "Context", "lib", "generate_dcsbm", "read_tsv",
     "pairwise", "log2", "log2_tables", "PRECISION_FP64", "PRECISION_MIXED", "DELTA_REFERENCE",
-    "DELTA_EXACT", "ACCEPT_GREEDY", "ACCEPT_MH",
+    "DELTA_EXACT", "ACCEPT_GREEDY", "ACCEPT_MH", "GoldenSection", "converged",
 ]
 
 PRECISION_FP64 = 0
@@ -84,6 +84,11 @@ class CTrace(C.Structure):
                 ("h1", C.c_uint64), ("h2", C.c_uint64), ("x", C.c_double), ("y", C.c_double)]
 
 
+class CGoldenSlot(C.Structure):
+    _fields_ = [("B", C.c_int32), ("B_to_merge", C.c_int32), ("S", C.c_double),
+                ("snap", C.c_int32), ("reserved", C.c_int32)]
+
+
 class CDcsbm(C.Structure):
     _fields_ = [("N", C.c_int64), ("K", C.c_int32), ("alpha", C.c_double), ("rho", C.c_double),
                 ("mean_deg", C.c_double), ("deg_min", C.c_int32), ("deg_max", C.c_int32),
@@ -96,7 +101,8 @@ EXPORTS = [
     "gsap_nodal_batch_i32", "gsap_mdl_i32", "gsap_generate_dcsbm_i32", "gsap_read_tsv_i32",
     "gsap_pairwise_i32", "gsap_log2", "gsap_log2_tables", "gsap_partition_batch_i32",
     "gsap_partition_chains_i32", "gsap_log2f_n", "gsap_quality_i32", "gsap_save_csr_i32",
-    "gsap_load_csr_i32", "gsap_autotune_i32",
+    "gsap_load_csr_i32", "gsap_autotune_i32", "gsap_golden_init", "gsap_golden_step",
+    "gsap_converged",
 ]
 
 _lib = None
@@ -150,6 +156,14 @@ def lib() -> C.CDLL:
     L.gsap_log2_tables.restype = None
     L.gsap_log2f_n.argtypes = [_f32p, _f32p, C.c_int64]
     L.gsap_log2f_n.restype = None
+    L.gsap_golden_init.argtypes = [C.POINTER(CGoldenSlot)]
+    L.gsap_golden_init.restype = None
+    L.gsap_golden_step.argtypes = [C.POINTER(CGoldenSlot), C.POINTER(CGoldenSlot), C.c_float,
+                                   C.POINTER(i32), C.POINTER(i32), C.POINTER(CGoldenSlot),
+                                   C.POINTER(i32)]
+    L.gsap_golden_step.restype = None
+    L.gsap_converged.argtypes = [_f64p, i32, i32, C.c_double, i32, C.c_double, C.c_double]
+    L.gsap_converged.restype = i32
     _lib = L
     return L
 
@@ -386,6 +400,38 @@ class Context:
 
 
 # ---------------------------------------------------------------- host helpers
+class GoldenSection:
+    """The golden-section search over the block count as the library runs it
+    (prepare_for_partition_next, GSAP/partition.hpp:80-322; OldData, GSAP/graph.hpp:57-70).
+    slots() gives (B, B_to_merge, S, snapshot) of large, med and small."""
+
+    def __init__(self, rate: float = 0.35):
+        self.rate = rate
+        self._s = (CGoldenSlot * 3)()
+        lib().gsap_golden_init(self._s)
+
+    def slots(self):
+        return [(x.B, x.B_to_merge, x.S, x.snap) for x in self._s]
+
+    def step(self, B: int, B_to_merge: int, S: float):
+        """Records a level's result; returns (store, restore, (B, B_to_merge, S), optimal):
+        the snapshot that takes the level's labels, the one the next level starts from,
+        that level's state, and whether the search ends."""
+        P = CGoldenSlot(B, B_to_merge, S, -1, 0)
+        nxt = CGoldenSlot()
+        st, rs, opt = C.c_int32(), C.c_int32(), C.c_int32()
+        lib().gsap_golden_step(self._s, C.byref(P), self.rate, C.byref(st), C.byref(rs),
+                               C.byref(nxt), C.byref(opt))
+        return st.value, rs.value, (nxt.B, nxt.B_to_merge, nxt.S), bool(opt.value)
+
+
+def converged(dE, itr: int, S: float, bracketed: bool, W: int = 3, threshold1: float = 5e-4,
+              threshold2: float = 1e-4) -> bool:
+    """The library's convergence test after sweep itr (GSAP/partition.hpp:918-934)."""
+    d = np.ascontiguousarray(dE, dtype=np.float64)
+    return bool(lib().gsap_converged(d, itr, W, S, int(bracketed), threshold1, threshold2))
+
+
 def generate_dcsbm(N: int, K: int, alpha: float, rho: float, mean_deg: float, seed: int,
                    deg_min: int = 5, deg_max: int = 100, exponent: float = 2.5) -> Graph:
     """Seeded degree-corrected SBM (the benchmark input generator)."""

@@ -807,11 +807,6 @@ struct GraphGuard {
   }
 };
 
-struct Slot {
-  int32_t B = 0, B_to_merge = 0;
-  double S = std::numeric_limits<double>::infinity();
-  int idx = 0;  // snapshot buffer
-};
 
 void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &o, int32_t *out,
                     gsap_report *rep) {
@@ -849,8 +844,8 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
   int32_t B = (int32_t)N;
   int32_t B_to_merge = (int32_t)((float)B * o.reduction_rate);
   rebuild(c, B);
-  Slot large, med, small;
-  large.idx = 0; med.idx = 1; small.idx = 2;
+  gsap_golden_slot slots[3];  // large, med, small; slot k owns label snapshot k
+  gsap_golden_init(slots);
   uint32_t merge_ctr = 0, batch_ctr = 0;
   int32_t levels = 0, sweeps = 0;
   int64_t batches = 0;
@@ -958,53 +953,24 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
         rec(3, B, itr, 0, fnv(col.data(), 4 * (int64_t)nnz), fnv(val.data(), 4 * (int64_t)nnz), S,
             dE[itr]);
       }
-      const int W = o.moving_avg_window;
-      if (itr >= W - 1) {
-        bool isf = std::isfinite(large.S) && std::isfinite(med.S) && std::isfinite(small.S);
-        double mean = 0.0;
-        for (int i = itr - W + 1; i < itr; ++i) mean += dE[i];
-        mean /= (double)(W - 1);
-        if (!isf) { if (-mean < o.threshold1 * S) break; }
-        else { if (-mean < o.threshold2 * S) break; }
-      }
+      const int32_t bracketed = std::isfinite(slots[0].S) && std::isfinite(slots[1].S) &&
+                                std::isfinite(slots[2].S);
+      if (gsap_converged(dE.data(), itr, o.moving_avg_window, S, bracketed, o.threshold1,
+                         o.threshold2))
+        break;
     }
     // golden section (partition.hpp:80-322); snapshots hold labels only, the block
     // matrix of a restored snapshot is rebuilt from them
-    int32_t PB = B, PBm = B_to_merge;
-    double PS = S;
-    auto save = [&](Slot &s) {
-      s.B = PB; s.B_to_merge = PBm; s.S = PS;
-      HIPX(hipMemcpyAsync(c->snap[s.idx].p, lab, 4 * (size_t)N, hipMemcpyDeviceToDevice, st));
-    };
-    if (PS <= med.S) {
-      if (med.B > PB) std::swap(large, med);
-      else std::swap(small, med);
-      save(med);
-    } else {
-      save((med.B > PB) ? small : large);
-    }
-    const Slot *from;
-    if (std::isinf(small.S)) {
-      from = &med;
-      B_to_merge = (int32_t)((float)med.B * o.reduction_rate);
-      if (B_to_merge == 0) optimal = true;
-    } else if (large.B - small.B == 2) {
-      optimal = true;
-      from = &med;
-      B_to_merge = med.B_to_merge;
-    } else if ((large.B - med.B) >= (med.B - small.B)) {
-      int32_t next_B = med.B + (int32_t)std::round((double)(large.B - med.B) * 0.618);
-      from = &large;
-      B_to_merge = large.B - next_B;
-    } else {
-      int32_t next_B = small.B + (int32_t)std::round((double)(med.B - small.B) * 0.618);
-      from = &med;
-      B_to_merge = med.B - next_B;
-    }
-    if (B_to_merge <= 0 && !optimal) optimal = true;
-    B = from->B;
-    S = from->S;
-    HIPX(hipMemcpyAsync(lab, c->snap[from->idx].p, 4 * (size_t)N, hipMemcpyDeviceToDevice, st));
+    const gsap_golden_slot P{B, B_to_merge, S, -1, 0};
+    gsap_golden_slot next;
+    int32_t store = 0, restore = 0, opt = 0;
+    gsap_golden_step(slots, &P, o.reduction_rate, &store, &restore, &next, &opt);
+    HIPX(hipMemcpyAsync(c->snap[store].p, lab, 4 * (size_t)N, hipMemcpyDeviceToDevice, st));
+    optimal = opt != 0;
+    B = next.B;
+    B_to_merge = next.B_to_merge;
+    S = next.S;
+    HIPX(hipMemcpyAsync(lab, c->snap[restore].p, 4 * (size_t)N, hipMemcpyDeviceToDevice, st));
     rebuild(c, B);
     ++levels;
     if (tr) rec(4, B, B_to_merge, optimal ? 1 : 0, labels_hash(), 0, S, 0.0);
@@ -1024,6 +990,7 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
     rep->sweep_edges = c->E * (int64_t)sweeps;
     rep->n_trace = (int64_t)c->trace.size();
     for (int k = 0; k < 8; ++k) { rep->kernel_ms[k] = c->prof.ms[k]; rep->kernel_calls[k] = c->prof.calls[k]; }
+    if (c->prof.on) rep->t_rebuild = c->prof.ms[1] / 1e3;  // class 1: block-matrix rebuilds
     float mx;
     std::memcpy(&mx, &mix[0], 4);
     uint64_t cnt[2];

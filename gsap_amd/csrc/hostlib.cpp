@@ -169,6 +169,65 @@ extern "C" {
 
 double gsap_log2(double x) { return clog2_host(x); }
 
+void gsap_golden_init(gsap_golden_slot slots[3]) {
+  // OldData() (graph.hpp:62-69): B = 0 and S = inf in every slot; slot k owns snapshot k
+  for (int32_t k = 0; k < 3; ++k)
+    slots[k] = gsap_golden_slot{0, 0, std::numeric_limits<double>::infinity(), k, 0};
+}
+
+int32_t gsap_converged(const double *dE, int32_t itr, int32_t W, double S, int32_t bracketed,
+                       double threshold1, double threshold2) {
+  // partition.hpp:918-934: from itr = W-1 on, the mean of the W-1 sweeps before itr (the
+  // current one is outside the window) against threshold1 * S while the bracket is open,
+  // threshold2 * S once all three slots hold a finite S
+  if (itr < W - 1) return 0;
+  double mean = 0.0;
+  for (int32_t i = itr - W + 1; i < itr; ++i) mean += dE[i];
+  mean /= (double)(W - 1);
+  return (-mean < (bracketed ? threshold2 : threshold1) * S) ? 1 : 0;
+}
+
+void gsap_golden_step(gsap_golden_slot slots[3], const gsap_golden_slot *P, float rate,
+                      int32_t *store, int32_t *restore, gsap_golden_slot *next,
+                      int32_t *optimal) {
+  gsap_golden_slot &large = slots[0], &med = slots[1], &small = slots[2];
+  // partition.hpp:89-161: a result no worse than med takes med's place (med moving to large
+  // when it had more blocks, else to small, the two snapshot buffers exchanged); a worse one
+  // replaces small or large
+  gsap_golden_slot *dst;
+  if (P->S <= med.S) {
+    std::swap(med.B > P->B ? large : small, med);
+    dst = &med;
+  } else {
+    dst = med.B > P->B ? &small : &large;
+  }
+  dst->B = P->B;
+  dst->B_to_merge = P->B_to_merge;
+  dst->S = P->S;
+  *store = dst->snap;
+  // partition.hpp:163-320: the slot restored and the next number of merges
+  const gsap_golden_slot *from = &med;
+  int32_t Bm, opt = 0;
+  if (std::isinf(small.S)) {
+    Bm = (int32_t)((float)med.B * rate);  // (int)P.B * rate: a float product, truncated
+    opt = Bm == 0;
+  } else if (large.B - small.B == 2) {
+    opt = 1;
+    Bm = med.B_to_merge;
+  } else if (large.B - med.B >= med.B - small.B) {
+    from = &large;
+    Bm = (large.B - med.B) - (int32_t)std::round((double)(large.B - med.B) * 0.618);
+  } else {
+    Bm = (med.B - small.B) - (int32_t)std::round((double)(med.B - small.B) * 0.618);
+  }
+  // not in the reference: a bracket step without merges (two slots of equal B, possible
+  // when a merge phase runs out of candidates) ends the search instead of repeating a level
+  if (Bm <= 0) opt = 1;
+  *restore = from->snap;
+  *next = gsap_golden_slot{from->B, Bm, from->S, from->snap, 0};
+  *optimal = opt;
+}
+
 void gsap_log2_tables(double *inv_c, double *log2_c, double *poly) {
   std::memcpy(inv_c, h_log2_inv, sizeof h_log2_inv);
   std::memcpy(log2_c, h_log2_lg, sizeof h_log2_lg);

@@ -15,6 +15,9 @@
  *                              GSAP/partition.hpp:324-454 and 26-78
  *   gsap_nodal_batch_i32    <- propose_nodal_move, GSAP/partition.hpp:456-662
  *   gsap_mdl_i32            <- compute_overall_entropy_cuda, GSAP/partition.hpp:665-704
+ *   gsap_converged          <- the convergence test of GSAP/partition.hpp:918-934
+ *   gsap_golden_step        <- prepare_for_partition_next, GSAP/partition.hpp:80-322
+ *   gsap_golden_init        <- OldData(), GSAP/graph.hpp:57-70
  *   gsap_read_tsv_i32       <- readFile, GSAP/load.hpp:6-133 (whitespace tolerant)
  *   gsap_pairwise_i32       <- bf::evaluate pair counts, GSAP/evaluate/evaluate.hpp:130-181
  *   gsap_quality_i32        <- bf::evaluate (accuracy, pair counts, NMI),
@@ -119,7 +122,8 @@ typedef struct gsap_report {
   double t_total;
   double t_merge;
   double t_sweep;         /* sum over sweeps of sweep wall time */
-  double t_rebuild;       /* block-matrix rebuild time inside sweeps (if profiled) */
+  double t_rebuild;       /* device time of every block-matrix rebuild (kernel class 1) when
+                             options.profile = 1, else 0 */
   int64_t sweep_edges;    /* E * sweeps */
   double max_abs_err_fp32;/* mixed form: max |dS32 - dS64| over proposals evaluated in FP64 */
   int64_t rechecks;       /* mixed form: decisions whose FP32 delta-S lay within its error
@@ -210,6 +214,36 @@ int gsap_mdl_i32(gsap_ctx *ctx, const gsap_graph_i32 *g, const int32_t *labels, 
                  double *S, const gsap_options *o);
 
 /* ---- host-side helpers (no device needed) ---- */
+
+/* Golden-section search over the block count (host only).  A slot is the reference's
+ * Partition (GSAP/graph.hpp:48-54) with S in FP64 and the id (0..2) of the label snapshot
+ * it owns; the three slots are OldData's large, med, small (graph.hpp:57-70). */
+typedef struct gsap_golden_slot {
+  int32_t B;
+  int32_t B_to_merge;
+  double S;
+  int32_t snap;
+  int32_t reserved;
+} gsap_golden_slot;
+
+/* slots[k] = {B 0, S inf, snapshot k}: OldData() of GSAP/graph.hpp:62-69. */
+void gsap_golden_init(gsap_golden_slot slots[3]);
+
+/* Convergence test after sweep itr (GSAP/partition.hpp:918-934): returns 1 when the level
+ * stops.  dE[0..itr] are the sweeps' accepted delta-S sums, W the moving-average window,
+ * bracketed = 1 when all three slots hold a finite S (threshold2 then, else threshold1). */
+int32_t gsap_converged(const double *dE, int32_t itr, int32_t W, double S, int32_t bracketed,
+                       double threshold1, double threshold2);
+
+/* Bracket update after a level (prepare_for_partition_next, GSAP/partition.hpp:80-322).
+ * slots = {large, med, small}, updated in place; P = the level's result (B, B_to_merge, S).
+ * *store receives the snapshot that must take the level's labels, *restore the snapshot the
+ * next level starts from, *next its B, the next B_to_merge and its S, *optimal 1 when the
+ * search ends.  One addition to the reference: a step of B_to_merge <= 0 in the bracketed
+ * phase also ends the search (the reference would repeat the level without merging). */
+void gsap_golden_step(gsap_golden_slot slots[3], const gsap_golden_slot *P, float rate,
+                      int32_t *store, int32_t *restore, gsap_golden_slot *next,
+                      int32_t *optimal);
 
 /* Seeded degree-corrected SBM.  Two-phase: call with out arrays NULL to learn
  * E (= llround(N*mean_deg)), then again with arrays of the right sizes.

@@ -714,11 +714,6 @@ uint64_t fnv(const void *p, int64_t n, uint64_t h = 1469598103934665603ull) {
   return h;
 }
 
-struct Slot {
-  int32_t B = 0, B_to_merge = 0;
-  double S = std::numeric_limits<double>::infinity();
-  std::vector<int32_t> lab;
-};
 
 }  // namespace
 
@@ -860,7 +855,11 @@ int orc_partition(const orc_graph *gg, const orc_options *o, int32_t *labels_out
   // partition.hpp:871-875
   int32_t B = (int32_t)N;
   int32_t B_to_merge = (int32_t)((float)B * o->reduction_rate);
-  Slot large, med, small;
+  // golden-section slots large, med, small (graph.hpp:57-70: B = 0, S = inf), each holding
+  // one of three label snapshots
+  orc_golden_slot slots[3];
+  orc_golden_init(slots);
+  std::vector<int32_t> snap[3];
   BM m;
   build_bm(N, g.optr, g.oadj, g.ow, lab.data(), B, m);
   Scratch sc;
@@ -903,51 +902,24 @@ int orc_partition(const orc_graph *gg, const orc_options *o, int32_t *labels_out
       ++sweeps;
       std::vector<int32_t> v32(m.val.begin(), m.val.end());
       rec(3, B, itr, 0, fnv(m.col.data(), 4 * m.nnz), fnv(v32.data(), 4 * m.nnz), S, dE[itr]);
-      int W = o->moving_avg_window;
-      if (itr >= W - 1) {
-        bool isf = std::isfinite(large.S) && std::isfinite(med.S) && std::isfinite(small.S);
-        double mean = 0.0;
-        for (int i = itr - W + 1; i < itr; ++i) mean += dE[i];
-        mean /= (double)(W - 1);
-        if (!isf) { if (-mean < o->threshold1 * S) break; }
-        else { if (-mean < o->threshold2 * S) break; }
-      }
+      const int bracketed = std::isfinite(slots[0].S) && std::isfinite(slots[1].S) &&
+                            std::isfinite(slots[2].S);
+      if (orc_converged(dE.data(), itr, o->moving_avg_window, S, bracketed, o->threshold1,
+                        o->threshold2))
+        break;
     }
-    // golden section   (partition.hpp:80-322)
-    Slot P;
-    P.B = B; P.B_to_merge = B_to_merge; P.S = S;
-    bool restored = true;
-    if (P.S <= med.S) {
-      if (med.B > P.B) std::swap(large, med);
-      else std::swap(small, med);
-      med.B = P.B; med.B_to_merge = P.B_to_merge; med.S = P.S; med.lab = lab;
-    } else {
-      Slot &dst = (med.B > P.B) ? small : large;
-      dst.B = P.B; dst.B_to_merge = P.B_to_merge; dst.S = P.S; dst.lab = lab;
-    }
-    const Slot *from = nullptr;
-    if (std::isinf(small.S)) {
-      from = &med;
-      B_to_merge = (int32_t)((float)med.B * o->reduction_rate);
-      if (B_to_merge == 0) optimal = true;
-    } else if (large.B - small.B == 2) {
-      optimal = true;
-      from = &med;
-      B_to_merge = med.B_to_merge;
-    } else if ((large.B - med.B) >= (med.B - small.B)) {
-      int32_t next_B = med.B + (int32_t)std::round((double)(large.B - med.B) * 0.618);
-      from = &large;
-      B_to_merge = large.B - next_B;
-    } else {
-      int32_t next_B = small.B + (int32_t)std::round((double)(med.B - small.B) * 0.618);
-      from = &med;
-      B_to_merge = med.B - next_B;
-    }
-    if (B_to_merge <= 0 && !optimal) optimal = true;
-    B = from->B;
-    lab = from->lab;
-    S = from->S;
-    (void)restored;
+    // golden section   (partition.hpp:80-322); the labels of the level just finished go
+    // to snapshot `store`, the next level starts from snapshot `restore`
+    const orc_golden_slot P{B, B_to_merge, S, -1, 0};
+    orc_golden_slot next;
+    int32_t store = 0, restore = 0, opt = 0;
+    orc_golden_step(slots, &P, o->reduction_rate, &store, &restore, &next, &opt);
+    snap[store] = lab;
+    optimal = opt != 0;
+    B = next.B;
+    B_to_merge = next.B_to_merge;
+    S = next.S;
+    lab = snap[restore];
     build_bm(N, g.optr, g.oadj, g.ow, lab.data(), B, m);
     ++levels;
     rec(4, B, B_to_merge, optimal ? 1 : 0, fnv(lab.data(), 4 * N), 0, S, 0.0);
@@ -959,6 +931,69 @@ int orc_partition(const orc_graph *gg, const orc_options *o, int32_t *labels_out
     rep->batches = batches; rep->S = S; rep->n_trace = ntr;
   }
   return 0;
+}
+
+void orc_golden_init(orc_golden_slot slots[3]) {
+  // OldData() (graph.hpp:62-69): B = 0 and S = inf in every slot; slot k owns snapshot k
+  for (int32_t k = 0; k < 3; ++k)
+    slots[k] = orc_golden_slot{0, 0, std::numeric_limits<double>::infinity(), k, 0};
+}
+
+int32_t orc_converged(const double *dE, int32_t itr, int32_t W, double S, int32_t bracketed,
+                      double threshold1, double threshold2) {
+  // partition.hpp:918-934: from itr = W-1 on, the mean of the W-1 sweeps before itr (the
+  // current sweep is not in the window) against threshold1 * S until all three slots hold a
+  // finite S, threshold2 * S after
+  if (itr < W - 1) return 0;
+  double mean = 0.0;
+  for (int32_t i = itr - W + 1; i < itr; ++i) mean += dE[i];
+  mean /= (double)(W - 1);
+  return (-mean < (bracketed ? threshold2 : threshold1) * S) ? 1 : 0;
+}
+
+void orc_golden_step(orc_golden_slot slots[3], const orc_golden_slot *P, float rate,
+                     int32_t *store, int32_t *restore, orc_golden_slot *next, int32_t *optimal) {
+  // partition.hpp:89-161: the level's result P replaces med (med moving to large or small,
+  // their snapshot buffers exchanged) when it is no worse, else it becomes small or large
+  orc_golden_slot &large = slots[0], &med = slots[1], &small = slots[2];
+  orc_golden_slot *dst;
+  if (P->S <= med.S) {
+    if (med.B > P->B) std::swap(large, med);
+    else std::swap(small, med);
+    dst = &med;
+  } else {
+    dst = (med.B > P->B) ? &small : &large;
+  }
+  dst->B = P->B;
+  dst->B_to_merge = P->B_to_merge;
+  dst->S = P->S;
+  *store = dst->snap;
+  // partition.hpp:163-320: restore a slot and choose the next B_to_merge
+  const orc_golden_slot *from;
+  int32_t Bm, opt = 0;
+  if (std::isinf(small.S)) {
+    from = &med;
+    Bm = (int32_t)((float)med.B * rate);  // (int)P.B * rate: a float product, truncated
+    if (Bm == 0) opt = 1;
+  } else if (large.B - small.B == 2) {
+    opt = 1;
+    from = &med;
+    Bm = med.B_to_merge;
+  } else if ((large.B - med.B) >= (med.B - small.B)) {
+    const int32_t next_B = med.B + (int32_t)std::round((double)(large.B - med.B) * 0.618);
+    from = &large;
+    Bm = large.B - next_B;
+  } else {
+    const int32_t next_B = small.B + (int32_t)std::round((double)(med.B - small.B) * 0.618);
+    from = &med;
+    Bm = med.B - next_B;
+  }
+  // not in the reference: a bracket step of no merges (two slots with the same B, possible
+  // when a merge phase runs out of candidates) ends the search instead of repeating the level
+  if (Bm <= 0 && !opt) opt = 1;
+  *restore = from->snap;
+  *next = orc_golden_slot{from->B, Bm, from->S, from->snap, 0};
+  *optimal = opt;
 }
 
 int32_t orc_carry_out_merges(const int32_t *best, const double *dS_best, int32_t B,

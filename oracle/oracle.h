@@ -126,6 +126,30 @@ double orc_mdl(const orc_graph *g, const int32_t *labels, int32_t B, int32_t mod
 int orc_partition(const orc_graph *g, const orc_options *o, int32_t *labels_out,
                   orc_report *rep, orc_trace_rec *trace, int64_t trace_cap);
 
+/* One slot of the golden-section bracket: Partition (graph.hpp:48-54) with S in FP64 and
+ * the id (0..2) of the label snapshot it owns. */
+typedef struct orc_golden_slot {
+  int32_t B;
+  int32_t B_to_merge;
+  double S;
+  int32_t snap;
+  int32_t pad;
+} orc_golden_slot;
+
+/* Slots large, med, small as OldData() leaves them (B = 0, S = inf); slot k owns snapshot k. */
+void orc_golden_init(orc_golden_slot slots[3]);
+/* Convergence test after sweep itr (partition.hpp:918-934): 1 when the level stops.
+ * dE[0..itr] are the sweeps' accepted delta-S sums; bracketed = all three slots finite. */
+int32_t orc_converged(const double *dE, int32_t itr, int32_t W, double S, int32_t bracketed,
+                      double threshold1, double threshold2);
+/* Golden-section bracket update after a level (prepare_for_partition_next,
+ * partition.hpp:80-322).  slots = {large, med, small}, updated in place; P = the level's
+ * result (B, B_to_merge, S).  *store = snapshot that receives the level's labels;
+ * *restore = snapshot the next level starts from; *next = its B, next B_to_merge and S;
+ * *optimal = 1 when the search ends. */
+void orc_golden_step(orc_golden_slot slots[3], const orc_golden_slot *P, float rate,
+                     int32_t *store, int32_t *restore, orc_golden_slot *next, int32_t *optimal);
+
 /* Greedy merge order + relabel (partition.hpp:26-78) as union-find; map_out[B] gets the
  * new label of every old block.  Returns the number of merges carried out. */
 int32_t orc_carry_out_merges(const int32_t *best, const double *dS_best, int32_t B,

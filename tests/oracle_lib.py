@@ -44,6 +44,11 @@ class OReport(C.Structure):
                 ("n_trace", C.c_int64)]
 
 
+class OGoldenSlot(C.Structure):
+    _fields_ = [("B", C.c_int32), ("B_to_merge", C.c_int32), ("S", C.c_double),
+                ("snap", C.c_int32), ("pad", C.c_int32)]
+
+
 _lib = None
 
 
@@ -96,6 +101,15 @@ def lib():
         L.orc_philox4x32_10.restype = None
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                         C.POINTER(C.c_uint32)]
+        L.orc_golden_init.restype = None
+        L.orc_golden_init.argtypes = [C.POINTER(OGoldenSlot)]
+        L.orc_golden_step.restype = None
+        L.orc_golden_step.argtypes = [C.POINTER(OGoldenSlot), C.POINTER(OGoldenSlot), C.c_float,
+                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                      C.POINTER(OGoldenSlot), C.POINTER(C.c_int32)]
+        L.orc_converged.restype = C.c_int32
+        L.orc_converged.argtypes = [_f64p, C.c_int32, C.c_int32, C.c_double, C.c_int32,
+                                    C.c_double, C.c_double]
         _lib = L
     return _lib
 
@@ -238,3 +252,29 @@ def pairwise(truth, given):
     lib().orc_pairwise(len(truth), np.ascontiguousarray(truth, np.int32),
                        np.ascontiguousarray(given, np.int32), C.byref(p), C.byref(r))
     return p.value, r.value
+
+
+class GoldenSection:
+    """The oracle's golden-section bracket (orc_golden_step); same interface as
+    gsap_amd.GoldenSection."""
+
+    def __init__(self, rate=0.35):
+        self.rate = rate
+        self._s = (OGoldenSlot * 3)()
+        lib().orc_golden_init(self._s)
+
+    def slots(self):
+        return [(x.B, x.B_to_merge, x.S, x.snap) for x in self._s]
+
+    def step(self, B, B_to_merge, S):
+        P = OGoldenSlot(B, B_to_merge, S, -1, 0)
+        nxt = OGoldenSlot()
+        st, rs, opt = C.c_int32(), C.c_int32(), C.c_int32()
+        lib().orc_golden_step(self._s, C.byref(P), self.rate, C.byref(st), C.byref(rs),
+                              C.byref(nxt), C.byref(opt))
+        return st.value, rs.value, (nxt.B, nxt.B_to_merge, nxt.S), bool(opt.value)
+
+
+def converged(dE, itr, S, bracketed, W=3, threshold1=5e-4, threshold2=1e-4):
+    d = np.ascontiguousarray(dE, dtype=np.float64)
+    return bool(lib().orc_converged(d, itr, W, S, int(bracketed), threshold1, threshold2))
