@@ -432,6 +432,9 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
   const bool r_in_a = ht_find(ho, r) >= 0, s_in_a = ht_find(ho, s) >= 0;
   constexpr bool EX = (VAR & VAR_EXACT) != 0, MH = (VAR & VAR_MH) != 0;
   const int32_t sig = EX ? a.sloop[v] : 0;  // a self loop moves from (r, r) to (s, s)
+  // the Hastings ratio takes the move's real block matrix, self loop included, whatever
+  // the delta-S form
+  const int32_t sl = MH ? (EX ? sig : (int32_t)a.sloop[v]) : 0;
   const double l_dr = clog2_s((double)(dout_r - kout));
   const double l_ds = clog2_s((double)(dout_s + kout));
   const double l_ir = clog2_s((double)(din_r - kin));
@@ -599,8 +602,8 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
   double lh = 0.0;
   if constexpr (MH) {
     const int32_t a_r = ht_weight(ho, r), a_s = ht_weight(ho, s);
-    const Hast h{v, r, s, B, M_rs, M_sr, M_ss, M_rr - a_r - c_r + sig, M_rs - a_s + c_r - sig,
-                 M_sr + a_r - c_s - sig, tot, dout_r + din_r, dout_s + din_s};
+    const Hast h{v, r, s, B, M_rs, M_sr, M_ss, M_rr - a_r - c_r + sl, M_rs - a_s + c_r - sl,
+                 M_sr + a_r - c_s - sl, tot, dout_r + din_r, dout_s + din_s};
     double fo = 0.0, fi = 0.0, bo = 0.0, bi = 0.0;
     for (int32_t j = lane; j < no; j += 64) {
       const int32_t u = a.oadj[o0 + j];
@@ -731,6 +734,9 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
   const bool r_in_a = ht_find(ho, r) >= 0, s_in_a = ht_find(ho, s) >= 0;
   constexpr bool EX = (VAR & VAR_EXACT) != 0, MH = (VAR & VAR_MH) != 0;
   const int64_t sig = EX ? a.sloop[v] : 0;  // a self loop moves from (r, r) to (s, s)
+  // the Hastings ratio takes the move's real block matrix, self loop included, whatever
+  // the delta-S form
+  const int64_t sl = MH ? (EX ? sig : (int64_t)a.sloop[v]) : 0;
   const double l_dr = clog2_s((double)(dout_r - kout));
   const double l_ds = clog2_s((double)(dout_s + kout));
   const double l_ir = clog2_s((double)(din_r - kin));
@@ -842,8 +848,8 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
     // each summed canonically by wave 0 (out entries at 0.., in entries at no + 2 ..)
     const int64_t M_rr = corner_s[0], M_sr = corner_s[1], M_rs = corner_s[2], M_ss = corner_s[3];
     const int64_t a_r = ht_weight(ho, r), a_s = ht_weight(ho, s);
-    const Hast h{v, r, s, B, M_rs, M_sr, M_ss, M_rr - a_r - c_r + sig, M_rs - a_s + c_r - sig,
-                 M_sr + a_r - c_s - sig, tot, dout_r + din_r, dout_s + din_s};
+    const Hast h{v, r, s, B, M_rs, M_sr, M_ss, M_rr - a_r - c_r + sl, M_rs - a_s + c_r - sl,
+                 M_sr + a_r - c_s - sl, tot, dout_r + din_r, dout_s + din_s};
     auto fill = [&](bool back) {
       for (int32_t j = tid; j < no + ni; j += 256) {
         const bool out = j < no;
@@ -974,6 +980,9 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
     }
   }
   const int32_t sig = EX ? a.sloop[v] : 0;  // a self loop moves from (r, r) to (s, s)
+  // the Hastings ratio takes the move's real block matrix, self loop included, whatever
+  // the delta-S form
+  const int32_t sl = MH ? (EX ? sig : (int32_t)a.sloop[v]) : 0;
   const int32_t c_r = group_allsum<G>((q < ni && t_i == r) ? w_i : 0);
   const int32_t c_s = group_allsum<G>((q < ni && t_i == s) ? w_i : 0);
   const bool r_in_a = ((__ballot(q < no && t_o == r) >> gbase) & gmask) != 0;
@@ -1147,8 +1156,8 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
   if constexpr (MH) {
     const int32_t a_r = group_allsum<G>((q < no && t_o == r) ? w_o : 0);
     const int32_t a_s = group_allsum<G>((q < no && t_o == s) ? w_o : 0);
-    const Hast h{v, r, s, B, M_rs, M_sr, M_ss, M_rr - a_r - c_r + sig, M_rs - a_s + c_r - sig,
-                 M_sr + a_r - c_s - sig, tot, dout_r + din_r, dout_s + din_s};
+    const Hast h{v, r, s, B, M_rs, M_sr, M_ss, M_rr - a_r - c_r + sl, M_rs - a_s + c_r - sl,
+                 M_sr + a_r - c_s - sl, tot, dout_r + din_r, dout_s + din_s};
     CAcc<G> fo, fi, bo, bi;
     fo.init();
     fi.init();

@@ -512,10 +512,12 @@ double nodal_vertex(const G &g, const BM &m, const int32_t *lab, int64_t v, int3
   }
   const int64_t c_r = C(r), c_s = C(s);
   const int64_t kout = g.kout[v], kin = g.kin[v];
-  int64_t sig = 0;
-  if (exact)
-    for (int64_t j = o0; j < o1; ++j)
-      if (g.oadj[j] == v) sig += g.ow[j];
+  int64_t sloop = 0;  // weight of v's self loop
+  for (int64_t j = o0; j < o1; ++j)
+    if (g.oadj[j] == v) sloop += g.ow[j];
+  // the exact delta moves the self loop from (r, r) to (s, s); the reference's form leaves
+  // it out of R_t and S_t
+  const int64_t sig = exact ? sloop : 0;
   const double l_dr = clog2((double)(m.dout[r] - kout)), l_ds = clog2((double)(m.dout[s] + kout));
   const double l_ir = clog2((double)(m.din[r] - kin)), l_is = clog2((double)(m.din[s] + kin));
 
@@ -580,9 +582,10 @@ double nodal_vertex(const G &g, const BM &m, const int32_t *lab, int64_t v, int3
   if (log2H) {
     const int64_t Bq = m.B, kv = kout + kin;
     const int64_t Mrs = m.row_get(r, s), Msr = m.row_get(s, r), Mss = m.row_get(s, s);
-    // M'_rr, M'_rs, M'_sr after the move
-    const int64_t Rr = m.row_get(r, r) - A(r) - c_r + sig, Rs = Mrs - A(s) + c_r - sig;
-    const int64_t Sr = Msr + A(r) - c_s - sig;
+    // M'_rr, M'_rs, M'_sr of the block matrix after the move (the self loop goes from
+    // (r, r) to (s, s) whatever the delta-S form)
+    const int64_t Rr = m.row_get(r, r) - A(r) - c_r + sloop, Rs = Mrs - A(s) + c_r - sloop;
+    const int64_t Sr = Msr + A(r) - c_s - sloop;
     const int64_t dr = m.dout[r] + m.din[r], ds = m.dout[s] + m.din[s];
     auto hterms = [&](int32_t u, int64_t w, double &fw, double &bk) {
       const int32_t t = lab[u];

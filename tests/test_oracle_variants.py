@@ -102,7 +102,7 @@ def proposal_prob(g, lab, B, v, target):
 def test_hastings_ratio_vs_dense(seed, N, E, B, wmax, exact):
     g = graph_with_loops(seed, N, E, wmax)
     lab = np.random.default_rng(seed + 3).integers(0, B, N).astype(np.int32)
-    checked = 0
+    checked = loops = 0  # loops: checked moves of a vertex with a self loop
     for v in range(N):
         r = int(lab[v])
         for s in range(B):
@@ -112,10 +112,10 @@ def test_hastings_ratio_vs_dense(seed, N, E, B, wmax, exact):
             l2 = lab.copy(); l2[v] = s
             fwd = proposal_prob(g, lab, B, v, s)
             rev = proposal_prob(g, l2, B, v, r)
-            if exact or v % 3:  # the reference form ignores self loops in the reverse state
-                assert abs(lh - (math.log2(rev) - math.log2(fwd))) < 1e-10, (v, s)
-                checked += 1
-    assert checked >= 50
+            assert abs(lh - (math.log2(rev) - math.log2(fwd))) < 1e-10, (v, s)
+            checked += 1
+            loops += int(np.any(g.out_adj[g.out_ptr[v]:g.out_ptr[v + 1]] == v))
+    assert checked >= 50 and loops >= 10
 
 
 def test_nodal_proposal_frequencies_match_closed_form():
