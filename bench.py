@@ -27,12 +27,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import gsap_amd  # noqa: E402
+from gsap_amd import workloads  # noqa: E402
 
-CONFIGS = {
-    "c1": dict(N=1000, K=11, alpha=0.0, rho=4.0, mean_deg=20.0, seed=1),
-    "c2": dict(N=20000, K=33, alpha=1.0, rho=1.5, mean_deg=20.0, seed=2),
-    "c3": dict(N=200000, K=71, alpha=1.0, rho=1.5, mean_deg=22.5, seed=3),
-}
+CONFIGS = {k: v for k, v in workloads.CONFIGS.items() if k != "c5"}
 HBM_PEAK_GBS = 8000.0
 KCLASS = ["nodal", "rebuild", "block_terms", "merge", "mdl", "accept_sum"]
 # device kernels of each class (the library times a class with HIP events around all of them)
@@ -184,30 +181,17 @@ def compare_forms(ctx, g, lab, prec, thr):
             "nodal_proposals": int(r32.batches * (g.N // 4))}
 
 
-def c4_graphs(n=256):
-    """256 independent DC-SBM graphs, N log-uniform in [500, 50000], K = round(N^(1/3)),
-    Dirichlet(1) sizes, intra/inter ratio 4 (even g) or 1.5 (odd g), seed 4 + g."""
-    rng = np.random.default_rng(4)
-    Ns = np.exp(rng.uniform(np.log(500), np.log(50000), n)).astype(int)
-    out = []
-    for g, N in enumerate(Ns):
-        K = max(2, int(round(N ** (1 / 3))))
-        out.append(gsap_amd.generate_dcsbm(int(N), K, 1.0, 4.0 if g % 2 == 0 else 1.5, 20.0,
-                                           seed=4 + g))
-    return out
-
-
 def bench_multi(a, prec):
     """C4 (batched graphs) and C5 (parallel chains): aggregate nodal-move edges/s over the
     wall time of the whole call."""
     t0 = time.perf_counter()
     if a.config == "c4":
-        graphs = c4_graphs()
+        graphs = workloads.c4_graphs()
         desc = (f"batched: {len(graphs)} DC-SBM graphs, N log-uniform [500, 50000] "
                 f"(sum N={sum(g.N for g in graphs)}, sum E={sum(g.E for g in graphs)}), "
                 f"K=round(N^(1/3)), rho 4/1.5, seeds 4+g")
     else:
-        g5 = gsap_amd.generate_dcsbm(2000000, 150, 0.5, 2.0, 22.5, seed=5)
+        g5 = workloads.graph("c5")
         graphs = [g5]
         desc = (f"chains: {a.chains} chains on N={g5.N}, E={g5.E}, K=150, alpha=0.5, rho=2, "
                 f"seed 5; lowest S kept")
@@ -311,7 +295,7 @@ def main():
         return
     cfg = CONFIGS[a.config]
     t0 = time.perf_counter()
-    g = gsap_amd.generate_dcsbm(**cfg)
+    g = workloads.graph(a.config)
     t_gen = time.perf_counter() - t0
     ctx = gsap_amd.Context(0)
     thr, t_tune = tune(a, ctx, g, gsap_amd.Options(seed=1, precision=prec))

@@ -829,6 +829,11 @@ double orc_mdl(const orc_graph *gg, const int32_t *labels, int32_t B, int32_t mo
 
 int orc_partition(const orc_graph *gg, const orc_options *o, int32_t *labels_out, orc_report *rep,
                   orc_trace_rec *trace, int64_t trace_cap) {
+  return orc_partition_mt(gg, o, labels_out, rep, trace, trace_cap, 1);
+}
+
+int orc_partition_mt(const orc_graph *gg, const orc_options *o, int32_t *labels_out,
+                     orc_report *rep, orc_trace_rec *trace, int64_t trace_cap, int32_t threads) {
   if (!gg || !o || !labels_out || gg->N <= 0) return -1;
   if (o->num_proposals < 1 || o->num_proposals > 32 || o->num_batches < 1 ||
       o->max_nodal_itr < 1 || o->moving_avg_window < 2)
@@ -864,7 +869,7 @@ int orc_partition(const orc_graph *gg, const orc_options *o, int32_t *labels_out
   orc_golden_init(slots);
   std::vector<int32_t> snap[3];
   BM m;
-  build_bm(N, g.optr, g.oadj, g.ow, lab.data(), B, m);
+  build_bm(N, g.optr, g.oadj, g.ow, lab.data(), B, m, threads);
   Scratch sc;
   std::vector<int32_t> prop, best;
   std::vector<double> dS, dS_best;
@@ -884,7 +889,7 @@ int orc_partition(const orc_graph *gg, const orc_options *o, int32_t *labels_out
         fnv(lab.data(), 4 * N), 0.0, 0.0);
     ++merge_ctr;
     B = nB;
-    build_bm(N, g.optr, g.oadj, g.ow, lab.data(), B, m);
+    build_bm(N, g.optr, g.oadj, g.ow, lab.data(), B, m, threads);
     // nodal sweeps   (partition.hpp:906-935)
     std::vector<double> dE(o->max_nodal_itr, 0.0);
     std::vector<int32_t> bp(BS);
@@ -893,8 +898,8 @@ int orc_partition(const orc_graph *gg, const orc_options *o, int32_t *labels_out
       for (int b = 0; b < o->num_batches; ++b) {
         double e = 0.0;
         std::vector<int32_t> before(lab.begin() + (int64_t)b * BS, lab.begin() + (int64_t)(b + 1) * BS);
-        nodal_batch(g, *o, rng, lab.data(), m, b, batch_ctr, bp.data(), bd.data(), &e, sc);
-        build_bm(N, g.optr, g.oadj, g.ow, lab.data(), B, m);
+        nodal_batch(g, *o, rng, lab.data(), m, b, batch_ctr, bp.data(), bd.data(), &e, sc, threads);
+        build_bm(N, g.optr, g.oadj, g.ow, lab.data(), B, m, threads);
         int64_t moved = 0;  // vertices of the batch whose label changed
         for (int64_t i = 0; i < BS; ++i) moved += lab[(int64_t)b * BS + i] != before[i];
         rec(2, B, itr, b, fnv(bp.data(), 4 * BS), fnv(lab.data(), 4 * N), e, (double)moved);
@@ -923,7 +928,7 @@ int orc_partition(const orc_graph *gg, const orc_options *o, int32_t *labels_out
     B_to_merge = next.B_to_merge;
     S = next.S;
     lab = snap[restore];
-    build_bm(N, g.optr, g.oadj, g.ow, lab.data(), B, m);
+    build_bm(N, g.optr, g.oadj, g.ow, lab.data(), B, m, threads);
     ++levels;
     rec(4, B, B_to_merge, optimal ? 1 : 0, fnv(lab.data(), 4 * N), 0, S, 0.0);
     if (o->max_levels > 0 && levels >= o->max_levels) break;

@@ -150,6 +150,11 @@ int32_t orc_converged(const double *dE, int32_t itr, int32_t W, double S, int32_
 void orc_golden_step(orc_golden_slot slots[3], const orc_golden_slot *P, float rate,
                      int32_t *store, int32_t *restore, orc_golden_slot *next, int32_t *optimal);
 
+/* The same partition with the nodal batches and block-matrix builds spread over `threads`
+ * host threads (identical results). */
+int orc_partition_mt(const orc_graph *g, const orc_options *o, int32_t *labels_out,
+                     orc_report *rep, orc_trace_rec *trace, int64_t trace_cap, int32_t threads);
+
 /* Greedy merge order + relabel (partition.hpp:26-78) as union-find; map_out[B] gets the
  * new label of every old block.  Returns the number of merges carried out. */
 int32_t orc_carry_out_merges(const int32_t *best, const double *dS_best, int32_t B,

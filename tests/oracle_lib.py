@@ -90,6 +90,10 @@ def lib():
         L.orc_partition.restype = C.c_int
         L.orc_partition.argtypes = [C.POINTER(OGraph), C.POINTER(OOptions), _i32p,
                                     C.POINTER(OReport), C.POINTER(OTrace), C.c_int64]
+        L.orc_partition_mt.restype = C.c_int
+        L.orc_partition_mt.argtypes = [C.POINTER(OGraph), C.POINTER(OOptions), _i32p,
+                                       C.POINTER(OReport), C.POINTER(OTrace), C.c_int64,
+                                       C.c_int32]
         L.orc_carry_out_merges.restype = C.c_int32
         L.orc_carry_out_merges.argtypes = [_i32p, _f64p, C.c_int32, C.c_int32, _i32p]
         L.orc_argmin_tree.restype = None
@@ -219,13 +223,15 @@ def mdl(g, labels, B, model=0):
     return lib().orc_mdl(C.byref(og), np.ascontiguousarray(labels, np.int32), B, model)
 
 
-def partition(g, opts, trace_cap=0):
+def partition(g, opts, trace_cap=0, threads=1):
+    """The oracle's whole partition; threads > 1 spreads its nodal batches and block-matrix
+    builds over host threads (identical results)."""
     out = np.zeros(g.N, np.int32)
     rep = OReport()
     og, oo = ograph(g), ooptions(opts)
     buf = (OTrace * max(trace_cap, 1))()
-    rc = lib().orc_partition(C.byref(og), C.byref(oo), out, C.byref(rep),
-                             buf if trace_cap else None, trace_cap)
+    rc = lib().orc_partition_mt(C.byref(og), C.byref(oo), out, C.byref(rep),
+                                buf if trace_cap else None, trace_cap, threads)
     assert rc == 0
     n = min(rep.n_trace, trace_cap)
     tr = [(t.kind, t.B, t.a, t.b, t.h1, t.h2, t.x, t.y) for t in buf[:n]]

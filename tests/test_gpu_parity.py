@@ -263,14 +263,16 @@ def test_autotune_keeps_results(ctx):
     assert np.array_equal(lab, rlab) and eq_bits(rep.S, rrep.S)
 
 
-@pytest.mark.parametrize("name", ["hubs", "LL1k", "HH5k"])
-def test_partition_graph_replay(ctx, name):
+@pytest.mark.parametrize("name,dense", [("hubs", 0), ("LL1k", 0), ("HH5k", 0), ("HH5k", 64)])
+def test_partition_graph_replay(ctx, name, dense):
     """options.graph: every level's sweep captured once and replayed (at least three times
     per level, with the level's eager merge, rebuild and golden-section work in between);
-    labels and S must equal the oracle's, and a second call must reproduce the first."""
+    labels and S must equal the oracle's, and a second call must reproduce the first.  With
+    dense_max_B = 64 the levels above 64 blocks capture the segmented rebuild and the CSR
+    lookups instead of the dense block matrix."""
     g = graph(name)
-    o = Options(seed=4, graph=1)
-    print(f"partition graph replay {name}", flush=True)
+    o = Options(seed=4, graph=1, dense_max_B=dense)
+    print(f"partition graph replay {name} dense_max_B={dense}", flush=True)
     lab, rep = ctx.partition(g, o)
     rlab, rrep, _ = O.partition(g, Options(seed=4))
     assert np.array_equal(lab, rlab) and eq_bits(rep.S, rrep.S)
