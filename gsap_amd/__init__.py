@@ -102,7 +102,7 @@ EXPORTS = [
     "gsap_pairwise_i32", "gsap_log2", "gsap_log2_tables", "gsap_partition_batch_i32",
     "gsap_partition_chains_i32", "gsap_log2f_n", "gsap_quality_i32", "gsap_save_csr_i32",
     "gsap_load_csr_i32", "gsap_autotune_i32", "gsap_golden_init", "gsap_golden_step",
-    "gsap_converged",
+    "gsap_converged", "gsap_mdl_noalloc_i32",
 ]
 
 _lib = None
@@ -156,6 +156,7 @@ def lib() -> C.CDLL:
     L.gsap_log2_tables.restype = None
     L.gsap_log2f_n.argtypes = [_f32p, _f32p, C.c_int64]
     L.gsap_log2f_n.restype = None
+    L.gsap_mdl_noalloc_i32.argtypes = [vp, vp, vp, vp, vp, i64, i64, i32, i32, vp, vp]
     L.gsap_golden_init.argtypes = [C.POINTER(CGoldenSlot)]
     L.gsap_golden_init.restype = None
     L.gsap_golden_step.argtypes = [C.POINTER(CGoldenSlot), C.POINTER(CGoldenSlot), C.c_float,
@@ -389,6 +390,13 @@ class Context:
         _check(lib().gsap_nodal_batch_i32(self._h, C.byref(cg), C.byref(co), lab, B, b, q, prop,
                                           dS, C.byref(dE)), self._h)
         return dict(labels=lab, prop=prop[:BS], dS=dS[:BS], dE=dE.value)
+
+    def mdl_noalloc(self, d_out_ptr: int, d_out_adj: int, d_out_w: int, d_labels: int, N: int,
+                    E: int, B: int, d_S: int, stream: int = 0, model: int = 0) -> None:
+        """gsap_mdl_noalloc_i32 on device buffers given by address (no allocation; the
+        result is written to d_S on `stream`, not waited for)."""
+        _check(lib().gsap_mdl_noalloc_i32(self._h, d_out_ptr, d_out_adj, d_out_w, d_labels, N,
+                                          E, B, model, d_S, stream or None), self._h)
 
     def mdl(self, g: Graph, labels: np.ndarray, B: int, opts: Optional[Options] = None) -> float:
         S = C.c_double()

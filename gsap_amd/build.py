@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgsap_amd.so")
-SOURCES = ["prims.hip", "bm.hip", "sweep.hip", "engine.hip", "hostlib.cpp"]
+SOURCES = ["prims.hip", "bm.hip", "sweep.hip", "engine.hip", "noalloc.hip", "hostlib.cpp"]
 ARCH = os.environ.get("GSAP_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

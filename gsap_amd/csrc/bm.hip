@@ -1403,4 +1403,22 @@ void rebuild_blockmodel(const GraphDev &g, const int32_t *lab32, int32_t B, BMBu
   bm.index_valid = !dense && bm.gcol && bm.grow;
 }
 
+// D = sum of the C partial planes of BB counters, then the CSR, CSC, degrees and their logs
+// of the B <= 128 block matrix D (the small path's last two steps, on caller buffers)
+void launch_small_from_planes(const int32_t *partials, int32_t C, int32_t B, int32_t *D,
+                              int32_t *rowptr, int32_t *col, int32_t *val, int32_t *erow,
+                              int32_t *colptr, int32_t *row, int32_t *cval, int32_t *dout,
+                              int32_t *din, double *ldout, double *ldin, hipStream_t st) {
+  const int64_t BB = (int64_t)B * B;
+  const int T = 256;
+  if (C <= 4)
+    k_hist_reduce<1><<<grid_for(BB, T), T, 0, st>>>(partials, C, BB, D);
+  else if (C <= 32)
+    k_hist_reduce<8><<<grid_for(BB * 8, T), T, 0, st>>>(partials, C, BB, D);
+  else
+    k_hist_reduce<64><<<grid_for(BB * 64, T), T, 0, st>>>(partials, C, BB, D);
+  k_small_bm<<<1, 1024, 0, st>>>(D, B, rowptr, col, val, erow, colptr, row, cval, dout, din,
+                                 ldout, ldin);
+}
+
 }  // namespace gsap

@@ -1,6 +1,8 @@
 // Device state of one partition run and the kernel entry points the host
 // orchestrator (engine.hip) drives.
 #pragma once
+#include <string>
+
 #include "common.h"
 #include "prims.h"
 
@@ -83,6 +85,23 @@ void rebuild_blockmodel(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf 
 // Returns false (nothing launched) otherwise.
 bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t *olab, int64_t lo,
                          int64_t BS, int32_t B, BMBuf &bm, RebuildScratch &rs, hipStream_t st);
+
+// D = sum of C partial planes of B*B counters, then the block matrix of D (B <= 128) as CSR,
+// CSC, degrees and log2 degrees (the small rebuild path's reduction and compaction)
+void launch_small_from_planes(const int32_t *partials, int32_t C, int32_t B, int32_t *D,
+                              int32_t *rowptr, int32_t *col, int32_t *val, int32_t *erow,
+                              int32_t *colptr, int32_t *row, int32_t *cval, int32_t *dout,
+                              int32_t *din, double *ldout, double *ldin, hipStream_t st);
+
+// ---- noalloc.hip
+// Resolves this device's address of the no-allocation entry point's scratch array (called
+// once per device with the device current, by gsap_create); returns false on failure.
+bool noalloc_init_device(int device);
+// Description length of a labelling (B <= 128) on device buffers with every intermediate in
+// the scratch array; *S (device) receives it.  Returns a GSAP_* status, msg on error.
+int noalloc_mdl(int device, const int32_t *optr, const int32_t *oadj, const int32_t *ow,
+                const int32_t *lab, int64_t N, int64_t E, int32_t B, int32_t model_kind,
+                double *S, hipStream_t st, std::string &msg);
 
 // ---- sweep.hip
 struct RngKey {

@@ -1110,6 +1110,13 @@ int gsap_create(gsap_ctx **ctx, int device) {
     return GSAP_ERR_DEVICE;
   }
   std::memset(c->pub, 0, sizeof(HostPub));
+  // this device's address of the no-allocation entry point's scratch array
+  if (!noalloc_init_device(device)) {
+    (void)hipHostFree(c->pub);
+    (void)hipStreamDestroy(c->st);
+    delete c;
+    return GSAP_ERR_DEVICE;
+  }
   *ctx = c;
   return GSAP_OK;
 }
@@ -1365,6 +1372,18 @@ int gsap_mdl_i32(gsap_ctx *c, const gsap_graph_i32 *g, const int32_t *labels, in
     if (!labels || !S) throw Err(GSAP_ERR_INVALID, "NULL argument");
     setup_unit(c, g, o, labels, B);
     *S = mdl_now(c, o ? o->model : GSAP_MODEL_REFERENCE);
+  });
+}
+
+int gsap_mdl_noalloc_i32(gsap_ctx *c, const int32_t *d_out_ptr, const int32_t *d_out_adj,
+                         const int32_t *d_out_w, const int32_t *d_labels, int64_t N, int64_t E,
+                         int32_t B, int32_t model, double *d_S, void *stream) {
+  return guarded(c, [&] {
+    set_device(c);
+    std::string msg;
+    const int rc = noalloc_mdl(c->device, d_out_ptr, d_out_adj, d_out_w, d_labels, N, E, B,
+                               model, d_S, stream ? (hipStream_t)stream : c->st, msg);
+    if (rc != GSAP_OK) throw Err(rc, msg);
   });
 }
 

@@ -213,6 +213,20 @@ int gsap_nodal_batch_i32(gsap_ctx *ctx, const gsap_graph_i32 *g, const gsap_opti
 int gsap_mdl_i32(gsap_ctx *ctx, const gsap_graph_i32 *g, const int32_t *labels, int32_t B,
                  double *S, const gsap_options *o);
 
+/* Description length of a labelling without allocating device memory.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Every input and the result are the
+ * caller's DEVICE buffers: d_out_ptr[N+1], d_out_adj[E], d_out_w[E] (the out-CSR view),
+ * d_labels[N]; d_S receives the FP64 description length, equal bit for bit to what
+ * gsap_mdl_i32 returns for the same graph and labels (NaN when an id, weight or row pointer
+ * is out of range).  Limit: 1 <= B <= 128 (GSAP_ERR_INVALID otherwise).  Work is queued on
+ * `stream` (a hipStream_t; NULL = the context's stream) and not waited for.  Intermediates
+ * live in one fixed device array of about 4.5 MiB per GPU, held until the process ends and
+ * shared by every call on that GPU: calls that overlap on one GPU race. */
+int gsap_mdl_noalloc_i32(gsap_ctx *ctx, const int32_t *d_out_ptr, const int32_t *d_out_adj,
+                         const int32_t *d_out_w, const int32_t *d_labels, int64_t N, int64_t E,
+                         int32_t B, int32_t model, double *d_S, void *stream);
+
 /* ---- host-side helpers (no device needed) ---- */
 
 /* Golden-section search over the block count (host only).  A slot is the reference's
