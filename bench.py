@@ -240,7 +240,7 @@ def tune(a, ctx, g, opts):
     t0 = time.perf_counter()
     o = ctx.autotune(g, opts)
     t = time.perf_counter() - t0
-    thr = {"tile_max_B": o.tile_max_B, "dense_max_B": o.dense_max_B}
+    thr = {f: getattr(o, f) for f in gsap_amd.TUNED}
     print(f"autotune: {thr} in {t:.2f} s", file=sys.stderr, flush=True)
     return thr, t
 

@@ -58,7 +58,9 @@ class COptions(C.Structure):
                 ("model", C.c_int32), ("workers", C.c_int32),
                 ("profile_classes", C.c_int32), ("graph", C.c_int32),
                 ("tile_max_B", C.c_int32), ("delta_mode", C.c_int32), ("accept", C.c_int32),
-                ("beta", C.c_double)]
+                ("beta", C.c_double), ("nodal_max16", C.c_int32), ("nodal_max32", C.c_int32),
+                ("nodal_maxwave", C.c_int32), ("seg_reg_max", C.c_int32),
+                ("seg_count_min", C.c_int32)]
 
 
 class CReport(C.Structure):
@@ -104,6 +106,10 @@ EXPORTS = [
     "gsap_load_csr_i32", "gsap_autotune_i32", "gsap_golden_init", "gsap_golden_step",
     "gsap_converged", "gsap_mdl_noalloc_i32",
 ]
+
+# the option fields the autotuner writes
+TUNED = ("tile_max_B", "dense_max_B", "nodal_max16", "nodal_max32", "nodal_maxwave",
+         "seg_reg_max", "seg_count_min")
 
 _lib = None
 
@@ -237,13 +243,22 @@ class Options:
     delta_mode: int = 0  # DELTA_REFERENCE or DELTA_EXACT
     accept: int = 0  # ACCEPT_GREEDY or ACCEPT_MH
     beta: float = 3.0  # inverse temperature of ACCEPT_MH
+    # work-split choices (identical results; 0 = default): nodal neighbour-count classes and
+    # segment-sort classes of the segmented rebuild
+    nodal_max16: int = 0
+    nodal_max32: int = 0
+    nodal_maxwave: int = 0
+    seg_reg_max: int = 0
+    seg_count_min: int = 0
 
     def cstruct(self) -> COptions:
         o = COptions()
         for f in ("seed", "num_proposals", "reduction_rate", "max_nodal_itr", "num_batches",
                   "threshold1", "threshold2", "moving_avg_window", "max_levels", "precision",
                   "chain", "trace", "dense_max_B", "profile", "model", "workers",
-                  "profile_classes", "graph", "tile_max_B", "delta_mode", "accept", "beta"):
+                  "profile_classes", "graph", "tile_max_B", "delta_mode", "accept", "beta",
+                  "nodal_max16", "nodal_max32", "nodal_maxwave", "seg_reg_max",
+                  "seg_count_min"):
             setattr(o, f, getattr(self, f))
         return o
 
@@ -317,11 +332,12 @@ class Context:
         return out, Report.from_c(rep)
 
     def autotune(self, g: Graph, opts: Options = Options()) -> Options:
-        """Options with tile_max_B and dense_max_B measured on this graph and device."""
+        """Options with the path thresholds and work-split classes measured on this graph
+        and device (identical results, other fields kept)."""
         import dataclasses
         cg, co = g.cstruct(), opts.cstruct()
         _check(lib().gsap_autotune_i32(self._h, C.byref(cg), C.byref(co)), self._h)
-        return dataclasses.replace(opts, tile_max_B=co.tile_max_B, dense_max_B=co.dense_max_B)
+        return dataclasses.replace(opts, **{f: getattr(co, f) for f in TUNED})
 
     def partition_batch(self, graphs, opts: Options = Options()):
         """Batched form: independent graphs on concurrent sub-contexts."""

@@ -328,14 +328,15 @@ constexpr int SEG_W = 512, SEG_G = 4096, SEG_RC = 16000;
 constexpr unsigned SEG_GRID = 1024;  // workgroups of the persistent segment kernels
 constexpr int SEG_PRIV_B = 4096;     // up to this B, reservations are counted in LDS first
 constexpr int SEG_VT = 2048;         // vertices per workgroup of the LDS-counted reservation
-GSAP_HD int seg_class(uint32_t n, bool one_range) {
-  if (n <= 16) return CL_A16;
-  if (n <= 32) return CL_A32;
-  if (n <= 64) return CL_A64;
+// reg_max: largest segment sorted in register groups (16, 32 or 64); count_min: with one
+// column range, segments longer than this (512..4096) are counted in LDS, shorter ones sorted
+GSAP_HD int seg_class(uint32_t n, bool one_range, uint32_t reg_max = 64,
+                      uint32_t count_min = SEG_W) {
+  if (n <= reg_max) return n <= 16 ? CL_A16 : n <= 32 ? CL_A32 : CL_A64;
   if (n <= 128) return CL_W128;
   if (n <= 256) return CL_W256;
   if (n <= (uint32_t)SEG_W) return CL_W512;
-  if (one_range) return CL_C;
+  if (one_range && n > count_min) return CL_C;
   if (n <= 1024) return CL_G1K;
   if (n <= 2048) return CL_G2K;
   return n <= (uint32_t)SEG_G ? CL_G4K : CL_C;
@@ -466,7 +467,8 @@ __global__ __launch_bounds__(256) void k_seg_fill(LabView lab, const int32_t *es
 // Nonempty segments into per-class lists: per-wave ballots, per-workgroup LDS counts, one
 // global atomic per class and workgroup round (1024 segments).
 __global__ __launch_bounds__(256) void k_seg_classify(const uint32_t *off, int64_t nseg,
-                                                      int one_range, uint32_t *lists,
+                                                      int one_range, uint32_t reg_max,
+                                                      uint32_t count_min, uint32_t *lists,
                                                       int64_t lcap, uint32_t *ncl) {
   __shared__ uint32_t wcnt[SEG_NCL], wbase[SEG_NCL];
   const int lane = threadIdx.x & 63;
@@ -482,7 +484,7 @@ __global__ __launch_bounds__(256) void k_seg_classify(const uint32_t *off, int64
       c[j] = -1;
       if (x < nseg) {
         const uint32_t n = off[x + 1] - off[x];
-        if (n) c[j] = seg_class(n, one_range != 0);
+        if (n) c[j] = seg_class(n, one_range != 0, reg_max, count_min);
       }
 #pragma unroll
       for (int k = 0; k < SEG_NCL; ++k) {
@@ -1079,11 +1081,21 @@ static void seg_path_t(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
   k_seg_fill<<<grid_for(2 * E, T), T, 0, st>>>(lab, g.esrc, g.optr, g.oadj, unit ? nullptr : g.ow,
                                                g.idst, g.iptr, g.iadj, unit ? nullptr : g.iw, N, E,
                                                B, rs.soff, rs.vrel, ekey, ew, rs.sseg, wb);
-  k_seg_classify<<<grid_for(nseg, T, SEG_GRID), T, 0, st>>>(rs.soff, nseg, one ? 1 : 0, rs.lists,
-                                                            rs.lcap, rs.ncl);
+  const uint32_t reg_max = (uint32_t)rs.seg_reg_max, count_min = (uint32_t)rs.seg_count_min;
+  k_seg_classify<<<grid_for(nseg, T, SEG_GRID), T, 0, st>>>(rs.soff, nseg, one ? 1 : 0, reg_max,
+                                                            count_min, rs.lists, rs.lcap, rs.ncl);
   if (one) {
     k_seg_c1<PK><<<SEG_GRID, 512, (size_t)RC * 4, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
                                                         rs.ncl, B, rs.su, wb);
+    // segments of 513..count_min entries sorted instead of counted
+    if (count_min > (uint32_t)SEG_W) {
+      k_seg_sort_g<16, CL_G4K, PK><<<SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists,
+                                                             rs.lcap, rs.ncl, rs.su, wb);
+      k_seg_sort_g<8, CL_G2K, PK><<<SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
+                                                            rs.ncl, rs.su, wb);
+      k_seg_sort_g<4, CL_G1K, PK><<<2 * SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists,
+                                                                rs.lcap, rs.ncl, rs.su, wb);
+    }
   } else {
     k_seg_sort_g<16, CL_G4K, PK><<<SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
                                                            rs.ncl, rs.su, wb);

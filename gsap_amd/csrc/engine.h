@@ -70,6 +70,9 @@ struct RebuildScratch {
   int64_t lcap = 0;
   uint32_t *ncl = nullptr;            // SEG_NCL list lengths
   uint32_t *cntc = nullptr;           // runs per (large segment, column range)
+  // segment classes: at most seg_reg_max entries sorted in register groups (16, 32 or 64);
+  // with B in one column range, segments longer than seg_count_min (512..4096) counted in LDS
+  int32_t seg_reg_max = 64, seg_count_min = 512;
 };
 // words of cntc for a graph of N vertices and E edges
 int64_t seg_cntc_words(int64_t N, int64_t E);
@@ -120,7 +123,16 @@ void launch_block_terms(const BMView &m, double *OUT, int32_t *sof, uint8_t *coi
 // LDS hash tables; larger d a 256-thread workgroup with tables sized for the vertex (LDS
 // up to big_lds bytes, else its slice of the global pool).
 constexpr int NCLS = 4;
-inline int nodal_class(int64_t d) { return d <= 16 ? 0 : d <= 32 ? 1 : d <= 128 ? 2 : 3; }
+// Largest neighbour count of each form (at most 16, 32 and 128; -1: the form unused)
+struct NodalBounds {
+  int32_t max16 = 16, max32 = 32, maxwave = 128;
+  bool operator==(const NodalBounds &o) const {
+    return max16 == o.max16 && max32 == o.max32 && maxwave == o.maxwave;
+  }
+};
+inline int nodal_class(int64_t d, const NodalBounds &nb = NodalBounds()) {
+  return d <= nb.max16 ? 0 : d <= nb.max32 ? 1 : d <= nb.maxwave ? 2 : 3;
+}
 // Space (int32 words) a class-3 vertex needs: three int32 arrays per hash table of 2^bits
 // slots (bits = smallest with 2^bits >= 2 n) and one double per row and column slot
 inline int64_t big_table_bits(int64_t n) {

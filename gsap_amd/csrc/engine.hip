@@ -172,8 +172,16 @@ struct Prof {
   }
 };
 
+// A level's labelling at the end of its sweeps and the batches the level ran (autotuner)
+struct LevelSnap {
+  int32_t B = 0;
+  int64_t batches = 0;
+  std::vector<int32_t> lab;
+};
+
 struct gsap_ctx {
   Prof prof;
+  std::vector<LevelSnap> *level_rec = nullptr;  // when set, partition_impl appends snapshots
   int device = 0;
   hipStream_t st = nullptr;
   std::string err;
@@ -199,6 +207,7 @@ struct gsap_ctx {
   std::vector<int64_t> list_off;  // [batch * 4 + class] -> offset into vlist; last = total
   std::vector<uint32_t> big_lds;  // per batch: LDS bytes of the largest class's launch
   int32_t lists_nb = 0;
+  NodalBounds lists_bounds;
   GraphDev g;
   BMBuf bm;
   RebuildScratch rs;
@@ -336,11 +345,13 @@ void upload_graph(gsap_ctx *c, const gsap_graph_i32 *g) {
   std::vector<int64_t> hoff(N, -1);
   std::vector<uint8_t> hbo(N, 0), hbi(N, 0);
   int64_t pool = 0;
+  // (table sizes for every vertex: options may send any vertex to the workgroup form; those
+  // above the one-wave form's limit of 128 neighbours may not fit LDS and get pool space)
   for (int64_t v = 0; v < N; ++v) {
     int64_t no = g->out_ptr[v + 1] - g->out_ptr[v], ni = g->in_ptr[v + 1] - g->in_ptr[v];
+    hbo[v] = (uint8_t)big_table_bits(no);
+    hbi[v] = (uint8_t)big_table_bits(ni);
     if (nodal_class(std::max(no, ni)) == NCLS - 1) {
-      hbo[v] = (uint8_t)big_table_bits(no);
-      hbi[v] = (uint8_t)big_table_bits(ni);
       hoff[v] = pool;
       pool += (big_words(no, ni) + 1) & ~(int64_t)1;
     }
@@ -379,7 +390,8 @@ void upload_graph(gsap_ctx *c, const gsap_graph_i32 *g) {
 // Vertex lists of every nodal batch by class of max(out, in) neighbour count: <= 16, <= 32,
 // <= 128, larger.  The last class gets LDS tables sized for its largest vertex (at most 2048
 // slots; beyond that the kernel uses the global pool).
-void build_batch_lists(gsap_ctx *c, const gsap_graph_i32 *g, int32_t nb) {
+void build_batch_lists(gsap_ctx *c, const gsap_graph_i32 *g, int32_t nb,
+                       const NodalBounds &bounds = NodalBounds()) {
   const int64_t N = g->N, BS = N / nb;
   // per batch (one host thread each): its vertices by class, classes in order
   std::vector<std::vector<int32_t>> per(nb);
@@ -394,7 +406,7 @@ void build_batch_lists(gsap_ctx *c, const gsap_graph_i32 *g, int32_t nb) {
         const int64_t v = b * BS + i;
         const int64_t no = g->out_ptr[v + 1] - g->out_ptr[v];
         const int64_t ni = g->in_ptr[v + 1] - g->in_ptr[v];
-        const int k = nodal_class(std::max(no, ni));
+        const int k = nodal_class(std::max(no, ni), bounds);
         cls[(size_t)i] = k;
         ++n[k];
         if (k == NCLS - 1) {
@@ -428,6 +440,7 @@ void build_batch_lists(gsap_ctx *c, const gsap_graph_i32 *g, int32_t nb) {
     HIPX(hipMemcpyAsync(c->vlist.p, lst.data(), 4 * lst.size(), hipMemcpyHostToDevice, c->st));
   HIPX(hipStreamSynchronize(c->st));
   c->lists_nb = nb;
+  c->lists_bounds = bounds;
 }
 
 void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
@@ -497,7 +510,20 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   r.cntc = c->cntc.as<uint32_t>();
 }
 
+constexpr int32_t SMALL_B_TUNE = 128;  // the small rebuild path's limit (never tuned)
 int32_t resolve_dense(const gsap_options &o) { return o.dense_max_B > 0 ? o.dense_max_B : 8192; }
+NodalBounds resolve_nodal(const gsap_options &o) {
+  NodalBounds b;
+  if (o.nodal_max16) b.max16 = o.nodal_max16;
+  if (o.nodal_max32) b.max32 = o.nodal_max32;
+  if (o.nodal_maxwave) b.maxwave = o.nodal_maxwave;
+  return b;
+}
+// the segment-class choices of the options into the rebuild scratch
+void apply_seg_classes(gsap_ctx *c, const gsap_options &o) {
+  c->rs.seg_reg_max = o.seg_reg_max ? o.seg_reg_max : 64;
+  c->rs.seg_count_min = o.seg_count_min ? o.seg_count_min : 512;
+}
 int32_t resolve_tile(const gsap_options &o) { return o.tile_max_B > 0 ? o.tile_max_B : 2048; }
 
 RngKey rng_key(const gsap_options &o) {
@@ -763,6 +789,17 @@ void check_options(const gsap_options &o) {
     throw Err(GSAP_ERR_INVALID, "unknown model");
   if (o.dense_max_B < 0 || o.tile_max_B < 0)
     throw Err(GSAP_ERR_INVALID, "dense_max_B and tile_max_B must be >= 0");
+  {
+    const NodalBounds b = resolve_nodal(o);
+    if (b.max16 < -1 || b.max16 > 16 || b.max32 < -1 || b.max32 > 32 || b.maxwave < -1 ||
+        b.maxwave > 128)
+      throw Err(GSAP_ERR_INVALID, "nodal_max16, nodal_max32, nodal_maxwave must be -1, 0 or at "
+                                  "most 16, 32 and 128");
+  }
+  if (o.seg_reg_max != 0 && o.seg_reg_max != 16 && o.seg_reg_max != 32 && o.seg_reg_max != 64)
+    throw Err(GSAP_ERR_INVALID, "seg_reg_max must be 0, 16, 32 or 64");
+  if (o.seg_count_min != 0 && (o.seg_count_min < 512 || o.seg_count_min > 4096))
+    throw Err(GSAP_ERR_INVALID, "seg_count_min must be 0 or in 512..4096");
   if (o.delta_mode != GSAP_DELTA_REFERENCE && o.delta_mode != GSAP_DELTA_EXACT)
     throw Err(GSAP_ERR_INVALID, "unknown delta_mode");
   if (o.accept != GSAP_ACCEPT_GREEDY && o.accept != GSAP_ACCEPT_MH)
@@ -814,8 +851,9 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
   const double t0 = now();
   set_device(c);
   upload_graph(c, gin);
-  build_batch_lists(c, gin, o.num_batches);
+  build_batch_lists(c, gin, o.num_batches, resolve_nodal(o));
   alloc_work(c, o.num_proposals, resolve_dense(o));
+  apply_seg_classes(c, o);
   c->tile_max_B = resolve_tile(o);
   const double t_upload = now() - t0;
   c->trace.clear();
@@ -876,6 +914,7 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
     int32_t ar = avg_row(read_nnz(c), B);
     t_merge += now() - tm;
     std::vector<double> dE(o.max_nodal_itr, 0.0);
+    const int64_t batches_before = batches;
     // one sweep: num_batches x (nodal batch + rebuild), the description length, and the
     // device batch counter advanced; with options.graph (and no tracing or event timing:
     // HIP cannot time events recorded inside a graph) it is captured once per level (B and
@@ -958,6 +997,15 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
       if (gsap_converged(dE.data(), itr, o.moving_avg_window, S, bracketed, o.threshold1,
                          o.threshold2))
         break;
+    }
+    if (c->level_rec) {  // the autotuner's snapshot of the level's labelling
+      LevelSnap ls;
+      ls.B = B;
+      ls.batches = batches - batches_before;
+      ls.lab.resize((size_t)N);
+      HIPX(hipMemcpyAsync(ls.lab.data(), lab, 4 * (size_t)N, hipMemcpyDeviceToHost, st));
+      HIPX(hipStreamSynchronize(st));
+      c->level_rec->push_back(std::move(ls));
     }
     // golden section (partition.hpp:80-322); snapshots hold labels only, the block
     // matrix of a restored snapshot is rebuilt from them
@@ -1046,8 +1094,9 @@ void setup_unit(gsap_ctx *c, const gsap_graph_i32 *g, const gsap_options *o, con
   check_options(oo);
   set_device(c);
   upload_graph(c, g);
-  build_batch_lists(c, g, oo.num_batches);
+  build_batch_lists(c, g, oo.num_batches, resolve_nodal(oo));
   alloc_work(c, oo.num_proposals, resolve_dense(oo));
+  apply_seg_classes(c, oo);
   c->tile_max_B = resolve_tile(oo);
   if (B <= 0 || B > c->N) throw Err(GSAP_ERR_INVALID, "B must be in [1, N]");
   for (int64_t v = 0; v < c->N; ++v)
@@ -1201,8 +1250,46 @@ int gsap_autotune_i32(gsap_ctx *c, const gsap_graph_i32 *g, gsap_options *o) {
     gsap_options oo = *o;
     check_options(oo);
     set_device(c);
-    upload_graph(c, g);
-    build_batch_lists(c, g, oo.num_batches);
+    // 1. one partition with the caller's options, keeping every level's labelling
+    std::vector<LevelSnap> levels;
+    {
+      gsap_options po = oo;
+      po.trace = 0;
+      po.profile = 0;
+      po.graph = 0;
+      std::vector<int32_t> out((size_t)std::max<int64_t>(g->N, 1));
+      c->level_rec = &levels;
+      struct Unset {
+        gsap_ctx *c;
+        ~Unset() { c->level_rec = nullptr; }
+      } unset{c};
+      partition_impl(c, g, po, out.data(), nullptr);
+    }
+    if (levels.empty() || c->E == 0) return;
+    // 2. up to 8 snapshots spread over the distinct block counts, each weighted by the
+    //    batches run at the levels nearest to it in log B
+    std::vector<const LevelSnap *> distinct;
+    {
+      std::vector<const LevelSnap *> byB;
+      for (const LevelSnap &l : levels) byB.push_back(&l);
+      std::stable_sort(byB.begin(), byB.end(),
+                       [](const LevelSnap *x, const LevelSnap *y) { return x->B > y->B; });
+      for (const LevelSnap *l : byB)
+        if (distinct.empty() || distinct.back()->B != l->B) distinct.push_back(l);
+    }
+    const size_t K = std::min<size_t>(8, distinct.size());
+    std::vector<const LevelSnap *> snaps;
+    for (size_t k = 0; k < K; ++k)
+      snaps.push_back(distinct[K == 1 ? 0 : k * (distinct.size() - 1) / (K - 1)]);
+    std::vector<double> wt(snaps.size(), 0.0);
+    for (const LevelSnap &l : levels) {
+      size_t best = 0;
+      for (size_t k = 1; k < snaps.size(); ++k)
+        if (std::fabs(std::log((double)snaps[k]->B / l.B)) <
+            std::fabs(std::log((double)snaps[best]->B / l.B)))
+          best = k;
+      wt[best] += (double)std::max<int64_t>(l.batches, 1);
+    }
     const int64_t N = c->N;
     const int32_t dmax = (int32_t)std::min<int64_t>(16384, N);
     alloc_work(c, oo.num_proposals, dmax);
@@ -1214,77 +1301,154 @@ int gsap_autotune_i32(gsap_ctx *c, const gsap_graph_i32 *g, gsap_options *o) {
       hipEvent_t a, b;
       ~Ev() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
     } ev{e0, e1};
-    std::vector<int32_t> lab(N);
-    // labels spread uniformly over B blocks (a pure function of the vertex id)
-    auto set_labels = [&](int32_t B) {
-      for (int64_t v = 0; v < N; ++v) lab[v] = (int32_t)(mix64((uint64_t)v * 0x9E3779B9ull + B) % (uint64_t)B);
-      HIPX(hipMemcpyAsync(c->lab.p, lab.data(), 4 * (size_t)N, hipMemcpyHostToDevice, st));
-    };
     auto elapsed = [&]() {
       HIPX(hipEventSynchronize(e1));
       float ms = 0;
       HIPX(hipEventElapsedTime(&ms, e0, e1));
-      return ms;
+      return (double)ms;
+    };
+    auto load = [&](const LevelSnap *sn) {
+      HIPX(hipMemcpyAsync(c->lab.p, sn->lab.data(), 4 * (size_t)N, hipMemcpyHostToDevice, st));
     };
     const uint32_t *ctr = c->ctr.as<uint32_t>();
     HIPX(hipMemsetAsync(c->ctr.p, 0, 8, st));
-    auto time_rebuild = [&](int32_t B) {
-      rebuild(c, B);
-      float best = std::numeric_limits<float>::infinity();
+    // device time of a rebuild of the snapshot's labelling (best of 3)
+    auto t_rebuild = [&](const LevelSnap *sn) {
+      load(sn);
+      rebuild(c, sn->B);
+      double best = 1e30;
       for (int rep = 0; rep < 3; ++rep) {
         HIPX(hipEventRecord(e0, st));
-        rebuild(c, B);
+        rebuild(c, sn->B);
         HIPX(hipEventRecord(e1, st));
         best = std::min(best, elapsed());
       }
       return best;
     };
-    // rebuild + one nodal batch (kernel work of a batch), labels reset before each run
-    auto time_batch = [&](int32_t B) {
-      float best = std::numeric_limits<float>::infinity();
+    // device time of a nodal batch from the snapshot (nodal kernels and the labels' apply;
+    // with rebuild = true also the rebuild that follows it), best of 3 after a warm-up
+    auto t_batch = [&](const LevelSnap *sn, bool with_rebuild) {
+      double best = 1e30;
       for (int rep = 0; rep < 4; ++rep) {
-        set_labels(B);
-        rebuild(c, B);
-        const int32_t ar = avg_row(read_nnz(c), B);
+        load(sn);
+        rebuild(c, sn->B);
+        const int32_t ar = avg_row(read_nnz(c), sn->B);
         HIPX(hipEventRecord(e0, st));
         nodal_batch(c, oo, 0, PhaseRef{ctr, 0}, ar, c->res.as<double>() + 1);
-        rebuild_after_batch(c, B, oo, 0);
+        if (with_rebuild) rebuild_after_batch(c, sn->B, oo, 0);
         HIPX(hipEventRecord(e1, st));
-        const float t = elapsed();
+        const double t = elapsed();
         if (rep > 0) best = std::min(best, t);
       }
       return best;
     };
-    // Row tile against segmented rebuild. A size where the tile path loses caps the
-    // threshold at the last size where it won; sizes the graph cannot reach (B > N / 2)
-    // leave the default in place.
-    int32_t tile = resolve_tile(oo);
-    for (int32_t B : {256, 512, 1024, 2048, 4096}) {
-      if (B > dmax || 2 * (int64_t)B > N) break;
-      set_labels(B);
-      c->dense_max_B = dmax;
-      c->tile_max_B = B;
-      const float t_tile = time_rebuild(B);
-      c->tile_max_B = 0;
-      const float t_seg = time_rebuild(B);
-      if (t_tile > t_seg) { tile = B / 2; break; }
-      tile = std::max(tile, B);
+    // a candidate replaces the default only when it saves more than 1 % of the weighted time
+    auto pick = [](const std::vector<double> &cost, size_t def) {
+      size_t b = def;
+      for (size_t k = 0; k < cost.size(); ++k)
+        if (cost[k] < cost[b]) b = k;
+      return cost[b] < 0.99 * cost[def] ? b : def;
+    };
+    int32_t tile = resolve_tile(oo), dense = resolve_dense(oo);
+    // 3. row tile against segmented rebuild at the snapshots in (128, 8192] (dense both ways)
+    {
+      std::vector<int32_t> cand{tile, SMALL_B_TUNE};
+      std::vector<double> tt, ts, ws;
+      std::vector<int32_t> Bs;
+      for (size_t k = 0; k < snaps.size(); ++k) {
+        const int32_t B = snaps[k]->B;
+        if (B <= SMALL_B_TUNE || B > 8192 || B > dmax) continue;
+        c->dense_max_B = dmax;
+        c->tile_max_B = B;
+        tt.push_back(t_rebuild(snaps[k]));
+        c->tile_max_B = 0;
+        ts.push_back(t_rebuild(snaps[k]));
+        ws.push_back(wt[k]);
+        Bs.push_back(B);
+        cand.push_back(B);
+      }
+      if (!Bs.empty()) {
+        std::vector<double> cost;
+        for (int32_t T : cand) {
+          double x = 0;
+          for (size_t j = 0; j < Bs.size(); ++j) x += ws[j] * (Bs[j] <= T ? tt[j] : ts[j]);
+          cost.push_back(x);
+        }
+        tile = std::max<int32_t>(SMALL_B_TUNE, cand[pick(cost, 0)]);
+      }
     }
-    // dense against CSR block-matrix lookups, over a whole batch (rebuild included)
-    int32_t dense = resolve_dense(oo);
     c->tile_max_B = tile;
-    for (int32_t B : {2048, 4096, 8192, 16384}) {
-      if (B > dmax || 2 * (int64_t)B > N) break;
-      c->dense_max_B = B;
-      const float t_dense = time_batch(B);
-      c->dense_max_B = B - 1;
-      const float t_csr = time_batch(B);
-      if (t_dense > t_csr) { dense = B - 1; break; }
-      dense = std::max(dense, B);
+    // 4. dense against CSR lookups over a whole batch at the snapshots in [2048, 16384]
+    {
+      std::vector<int32_t> cand{dense};
+      std::vector<double> td, tc, ws;
+      std::vector<int32_t> Bs;
+      for (size_t k = 0; k < snaps.size(); ++k) {
+        const int32_t B = snaps[k]->B;
+        if (B < 2048 || B > dmax) continue;
+        c->dense_max_B = B;
+        td.push_back(t_batch(snaps[k], true));
+        c->dense_max_B = B - 1;
+        tc.push_back(t_batch(snaps[k], true));
+        ws.push_back(wt[k]);
+        Bs.push_back(B);
+        cand.push_back(B);
+        cand.push_back(B - 1);
+      }
+      if (!Bs.empty()) {
+        std::vector<double> cost;
+        for (int32_t D : cand) {
+          double x = 0;
+          for (size_t j = 0; j < Bs.size(); ++j) x += ws[j] * (Bs[j] <= D ? td[j] : tc[j]);
+          cost.push_back(x);
+        }
+        dense = std::max<int32_t>(2047, cand[pick(cost, 0)]);
+      }
+    }
+    c->dense_max_B = dense;
+    // 5. segment-sort classes at the snapshots the segmented rebuild serves
+    const int32_t seg_cand[][2] = {{64, 512}, {32, 512}, {16, 512}, {64, 4096}, {32, 4096}};
+    {
+      std::vector<double> cost(5, 0.0);
+      bool any = false;
+      for (size_t k = 0; k < snaps.size(); ++k) {
+        const int32_t B = snaps[k]->B;
+        const bool dn = B <= dense;
+        if (B <= SMALL_B_TUNE || (dn && B <= tile)) continue;
+        any = true;
+        for (int j = 0; j < 5; ++j) {
+          c->rs.seg_reg_max = seg_cand[j][0];
+          c->rs.seg_count_min = seg_cand[j][1];
+          cost[j] += wt[k] * t_rebuild(snaps[k]);
+        }
+      }
+      const size_t j = any ? pick(cost, 0) : 0;
+      oo.seg_reg_max = seg_cand[j][0] == 64 ? 0 : seg_cand[j][0];
+      oo.seg_count_min = seg_cand[j][1] == 512 ? 0 : seg_cand[j][1];
+      apply_seg_classes(c, oo);
+    }
+    // 6. nodal neighbour-count classes over every snapshot (the batch's nodal kernels)
+    {
+      const NodalBounds nc[] = {{16, 32, 128}, {16, 32, 64}, {16, 32, 96}, {-1, 32, 128},
+                                {16, -1, 128}, {8, 32, 128}};
+      std::vector<double> cost(6, 0.0);
+      for (int j = 0; j < 6; ++j) {
+        build_batch_lists(c, g, oo.num_batches, nc[j]);
+        for (size_t k = 0; k < snaps.size(); ++k) cost[j] += wt[k] * t_batch(snaps[k], false);
+      }
+      const NodalBounds b = nc[pick(cost, 0)];
+      oo.nodal_max16 = b.max16 == 16 ? 0 : b.max16;
+      oo.nodal_max32 = b.max32 == 32 ? 0 : b.max32;
+      oo.nodal_maxwave = b.maxwave == 128 ? 0 : b.maxwave;
     }
     HIPX(hipStreamSynchronize(st));
     o->tile_max_B = tile;
     o->dense_max_B = dense;
+    o->seg_reg_max = oo.seg_reg_max;
+    o->seg_count_min = oo.seg_count_min;
+    o->nodal_max16 = oo.nodal_max16;
+    o->nodal_max32 = oo.nodal_max32;
+    o->nodal_maxwave = oo.nodal_maxwave;
   });
 }
 

@@ -109,6 +109,19 @@ typedef struct gsap_options {
   int32_t delta_mode;         /* GSAP_DELTA_* (nodal moves and merges) */
   int32_t accept;             /* GSAP_ACCEPT_* (nodal moves; MH needs precision FP64) */
   double beta;                /* inverse temperature of GSAP_ACCEPT_MH (3.0) */
+  /* Work-split choices (results are identical for any valid values; 0 = default).  Nodal
+   * moves by a vertex's neighbour count d = max(out, in): d <= nodal_max16 -> 16-lane
+   * groups (default 16, at most 16), d <= nodal_max32 -> 32-lane groups (32, at most 32),
+   * d <= nodal_maxwave -> one wave (128, at most 128), larger d -> a workgroup; -1 leaves
+   * a form unused.  Segmented rebuild: segments of at most seg_reg_max entries are sorted
+   * in register groups (64; 16, 32 or 64), longer ones by one wave up to 512 entries; when
+   * B <= 16000, segments longer than seg_count_min (512, up to 4096) are counted in LDS
+   * instead of sorted. */
+  int32_t nodal_max16;
+  int32_t nodal_max32;
+  int32_t nodal_maxwave;
+  int32_t seg_reg_max;
+  int32_t seg_count_min;
 } gsap_options;
 
 /* Per-call report.  Times are seconds of host wall clock around device work. */
@@ -177,11 +190,14 @@ int gsap_partition_chains_i32(gsap_ctx *ctx, const gsap_graph_i32 *g, const gsap
                               int32_t n_chains, int32_t *out, gsap_report *report,
                               double *S_chain, int32_t *best_chain);
 
-/* Autotuner: times, on this graph and device, the row-tile against the segmented
- * rebuild at B = 256 .. 4096 and the nodal kernel with dense against CSR block-matrix
- * lookups at B = 2048 .. 16384, and writes the crossovers into o->tile_max_B and
- * o->dense_max_B (the other fields are kept).  These thresholds only choose code paths:
- * results are identical for any values.  No counterpart in the reference. */
+/* Autotuner: runs one partition of this graph to take label snapshots at its levels, then
+ * times on those labellings, on this device: the row-tile against the segmented rebuild
+ * (tile_max_B), dense against CSR block-matrix lookups over a whole batch (dense_max_B),
+ * the segment-sort classes (seg_reg_max, seg_count_min) and the nodal neighbour-count
+ * classes (nodal_max16, nodal_max32, nodal_maxwave), each weighted by the batches the
+ * partition ran at that level, and writes the fastest choices into *o (the other fields
+ * are kept).  These fields only choose code paths: results are identical for any values.
+ * No counterpart in the reference (its bins are fixed, bb_segsort/bb_bin.h:23). */
 int gsap_autotune_i32(gsap_ctx *ctx, const gsap_graph_i32 *g, gsap_options *o);
 
 /* Trace records of the last gsap_partition_i32 call with o->trace = 1. */

@@ -185,6 +185,51 @@ def heavy():
     return heavy_graph()
 
 
+@pytest.mark.parametrize("reg,cnt", [(16, 4096), (32, 1024), (64, 4096)])
+@pytest.mark.parametrize("B,big", [(9000, 0.3), (3000, 0.05), (20000, 0.3)])
+def test_blockmodel_segment_classes(ctx, heavy, B, big, reg, cnt):
+    """The segment-sort classes only move segments between sorting teams: register groups
+    up to seg_reg_max entries, and (B <= 16000) workgroup sorts instead of LDS counting up to
+    seg_count_min entries; every choice gives the oracle's block matrix."""
+    g = heavy
+    rng = np.random.default_rng(17)
+    lab = rng.integers(1, B, g.N).astype(np.int32)
+    lab[rng.random(g.N) < big] = 0
+    print(f"heavy blockmodel B={B} big={big} seg_reg_max={reg} seg_count_min={cnt}", flush=True)
+    got = ctx.blockmodel(g, lab, B, Options(dense_max_B=1, tile_max_B=1, seg_reg_max=reg,
+                                            seg_count_min=cnt))
+    ref = O.blockmodel(g, lab, B)
+    for k in ("nnz", "rowptr", "col", "val", "colptr", "row", "cval", "dout", "din"):
+        assert np.array_equal(got[k], ref[k]), k
+
+
+NODAL_CLASSES = [(-1, 32, 128), (16, -1, 128), (8, 32, 64), (-1, -1, -1), (16, 32, 96)]
+
+
+@pytest.mark.parametrize("bounds", NODAL_CLASSES)
+@pytest.mark.parametrize("name", ["dcsbm3k", "bighubs"])
+def test_partition_nodal_classes(ctx, name, bounds):
+    """The nodal neighbour-count classes only choose the team of a vertex (16 or 32 lanes,
+    a wave, a workgroup; -1 leaves a form unused); the partition is the oracle's."""
+    g = graph(name)
+    m16, m32, mw = bounds
+    o = Options(seed=6, nodal_max16=m16, nodal_max32=m32, nodal_maxwave=mw, seg_reg_max=32,
+                seg_count_min=2048)
+    print(f"partition {name} nodal classes {bounds}", flush=True)
+    lab, rep = ctx.partition(g, o)
+    rlab, rrep, _ = O.partition(g, Options(seed=6))
+    assert np.array_equal(lab, rlab) and eq_bits(rep.S, rrep.S)
+
+
+def test_rejects_bad_work_split(ctx):
+    g = graph("rand")
+    for kw in (dict(nodal_max16=17), dict(nodal_max32=33), dict(nodal_maxwave=129),
+               dict(nodal_max16=-2), dict(seg_reg_max=48), dict(seg_count_min=100),
+               dict(seg_count_min=5000)):
+        with pytest.raises(gsap_amd.GsapError):
+            ctx.partition(g, Options(seed=1, **kw))
+
+
 @pytest.mark.parametrize("B,big", [(65536, 0.0), (20000, 0.3), (9000, 0.3), (3000, 0.05),
                                    (9000, 0.0)])
 def test_blockmodel_unpacked_and_packed(ctx, heavy, B, big):
@@ -252,12 +297,15 @@ def test_c3_blockmodel_tile_threshold(ctx, c3, B, tile):
 
 
 def test_autotune_keeps_results(ctx):
-    """The autotuner measures the path thresholds on the caller's graph; the tuned options
-    give the oracle's partition."""
+    """The autotuner measures the path thresholds and the work-split classes on snapshots of
+    the caller's partition; the tuned options give the oracle's partition."""
     g = gsap_amd.generate_dcsbm(20000, 33, 1.0, 1.5, 20.0, seed=2)
     print("autotune C2", flush=True)
     o = ctx.autotune(g, Options(seed=1))
     assert 128 <= o.tile_max_B <= 8192 and o.dense_max_B >= 2047
+    assert o.seg_reg_max in (0, 16, 32) and o.seg_count_min in (0, 4096)
+    assert o.nodal_max16 in (0, -1, 8) and o.nodal_max32 in (0, -1)
+    assert o.nodal_maxwave in (0, 64, 96)
     lab, rep = ctx.partition(g, o)
     rlab, rrep, _ = O.partition(g, Options(seed=1))
     assert np.array_equal(lab, rlab) and eq_bits(rep.S, rrep.S)
