@@ -39,11 +39,6 @@ GSAP_D uint32_t wave_incl_scan_u32(uint32_t x) {
   return x;
 }
 
-// Edge weights: graphs whose weights are all 1 pass no weight arrays to the nodal kernels
-// (null), which then read no weights.
-GSAP_D int32_t wload(const int32_t *w, int64_t k) { return w ? w[k] : 1; }
-GSAP_D const int32_t *wptr(const int32_t *w, int64_t off) { return w ? w + off : nullptr; }
-
 // Weighted pick over v's out list then in list: the first entry whose running weight
 // exceeds target (kernels.hpp:916-936 as integers).  Wave-wide; every lane returns it.
 GSAP_D int32_t wave_pick(const int32_t *w1, const int32_t *id1, int32_t n1, const int32_t *w2,
@@ -60,8 +55,8 @@ GSAP_D int32_t wave_pick(const int32_t *w1, const int32_t *id1, int32_t n1, cons
       const int32_t k = c + 64 * u + lane;
       w[u] = 0;
       id[u] = 0;
-      if (k < n1) { w[u] = (uint32_t)wload(w1, k); id[u] = id1[k]; }
-      else if (k < n) { w[u] = (uint32_t)wload(w2, k - n1); id[u] = id2[k - n1]; }
+      if (k < n1) { w[u] = (uint32_t)w1[k]; id[u] = id1[k]; }
+      else if (k < n) { w[u] = (uint32_t)w2[k - n1]; id[u] = id2[k - n1]; }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -85,7 +80,7 @@ GSAP_D int32_t block_pick(const int32_t *w1, const int32_t *id1, int32_t n1, con
   const int32_t K = (n + 255) / 256;
   const int32_t a0 = tid * K, a1 = a0 + K < n ? a0 + K : n;
   uint32_t sum = 0;
-  for (int32_t k = a0; k < a1; ++k) sum += (uint32_t)(k < n1 ? wload(w1, k) : wload(w2, k - n1));
+  for (int32_t k = a0; k < a1; ++k) sum += (uint32_t)(k < n1 ? w1[k] : w2[k - n1]);
   uint32_t incl = wave_incl_scan_u32(sum);
   if (lane == 63) sc[w] = incl;
   if (tid == 0) sc[4] = (uint32_t)(n1 > 0 ? id1[0] : id2[0]);  // unreachable fallback
@@ -97,7 +92,7 @@ GSAP_D int32_t block_pick(const int32_t *w1, const int32_t *id1, int32_t n1, con
   if (ex <= target && target - ex < sum) {  // the one run where the running weight crosses
     uint32_t run = ex;
     for (int32_t k = a0; k < a1; ++k) {
-      run += (uint32_t)(k < n1 ? wload(w1, k) : wload(w2, k - n1));
+      run += (uint32_t)(k < n1 ? w1[k] : w2[k - n1]);
       if (run > target) {
         sc[4] = (uint32_t)(k < n1 ? id1[k] : id2[k - n1]);
         break;
@@ -372,8 +367,8 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
   // its chain (when the proposal is r itself they go unused)
   const int32_t u_o0 = lane < no ? a.oadj[o0 + lane] : -1;
   const int32_t u_i0 = lane < ni ? a.iadj[i0 + lane] : -1;
-  const int32_t w_o0 = lane < no ? wload(a.ow, o0 + lane) : 0;
-  const int32_t w_i0 = lane < ni ? wload(a.iw, i0 + lane) : 0;
+  const int32_t w_o0 = lane < no ? a.ow[o0 + lane] : 0;
+  const int32_t w_i0 = lane < ni ? a.iw[i0 + lane] : 0;
   const int32_t t_o0 = u_o0 >= 0 ? a.lab[u_o0] : -1;
   const int32_t t_i0 = u_i0 >= 0 ? a.lab[u_i0] : -1;
   const int32_t dout_r = m.dout[r], din_r = m.din[r];
@@ -400,7 +395,7 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
       }
     } else {
       const int32_t nbr =
-          wave_pick(wptr(a.ow, o0), a.oadj + o0, no, wptr(a.iw, i0), a.iadj + i0, ni, target);
+          wave_pick(a.ow + o0, a.oadj + o0, no, a.iw + i0, a.iadj + i0, ni, target);
       u = a.lab[nbr];
     }
     s = a.coin[u] ? (int32_t)mulhi(x.x0, B) : a.sof[u];
@@ -430,8 +425,8 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
   wave_lds_sync();
   if (lane < no) ht_insert(ho, t_o0, w_o0, lane);
   if (lane < ni) ht_insert(hi, t_i0, w_i0, lane);
-  for (int32_t j = lane + 64; j < no; j += 64) ht_insert(ho, a.lab[a.oadj[o0 + j]], wload(a.ow, o0 + j), j);
-  for (int32_t j = lane + 64; j < ni; j += 64) ht_insert(hi, a.lab[a.iadj[i0 + j]], wload(a.iw, i0 + j), j);
+  for (int32_t j = lane + 64; j < no; j += 64) ht_insert(ho, a.lab[a.oadj[o0 + j]], a.ow[o0 + j], j);
+  for (int32_t j = lane + 64; j < ni; j += 64) ht_insert(hi, a.lab[a.iadj[i0 + j]], a.iw[i0 + j], j);
   wave_lds_sync();
   const int32_t c_r = ht_weight(hi, r), c_s = ht_weight(hi, s);
   const bool r_in_a = ht_find(ho, r) >= 0, s_in_a = ht_find(ho, s) >= 0;
@@ -615,7 +610,7 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
       const int32_t t = j < 64 ? t_o0 : a.lab[u];
       const bool nd = hast_needs(h, u, t);
       double f, b;
-      hast_terms(m, h, u, t, j < 64 ? w_o0 : wload(a.ow, o0 + j), nd ? ht_weight(ho, t) : 0,
+      hast_terms(m, h, u, t, j < 64 ? w_o0 : a.ow[o0 + j], nd ? ht_weight(ho, t) : 0,
                  nd ? ht_weight(hi, t) : 0, f, b);
       fo += f;
       bo += b;
@@ -625,7 +620,7 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
       const int32_t t = j < 64 ? t_i0 : a.lab[u];
       const bool nd = hast_needs(h, u, t);
       double f, b;
-      hast_terms(m, h, u, t, j < 64 ? w_i0 : wload(a.iw, i0 + j), nd ? ht_weight(ho, t) : 0,
+      hast_terms(m, h, u, t, j < 64 ? w_i0 : a.iw[i0 + j], nd ? ht_weight(ho, t) : 0,
                  nd ? ht_weight(hi, t) : 0, f, b);
       fi += f;
       bi += b;
@@ -681,7 +676,7 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
   if (tot == 0) {
     s = (int32_t)mulhi(x.x0, B);
   } else {
-    const int32_t nbr = block_pick(wptr(a.ow, o0), a.oadj + o0, no, wptr(a.iw, i0), a.iadj + i0, ni,
+    const int32_t nbr = block_pick(a.ow + o0, a.oadj + o0, no, a.iw + i0, a.iadj + i0, ni,
                                    mulhi(x.x1, tot), pick_sc);
     const int32_t u = a.lab[nbr];
     s = a.coin[u] ? (int32_t)mulhi(x.x0, B) : a.sof[u];
@@ -722,7 +717,7 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
       for (int q = 0; q < 4; ++q) {
         const int32_t j = j0 + q * 256;
         t[q] = j < n ? adj[j] : 0;
-        w[q] = j < n ? wload(wt, j) : 0;
+        w[q] = j < n ? wt[j] : 0;
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q)
@@ -732,8 +727,8 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
         if (j0 + q * 256 < n) ht_insert(h, t[q], w[q], j0 + q * 256);
     }
   };
-  insert_all(ho, a.oadj + o0, wptr(a.ow, o0), no);
-  insert_all(hi, a.iadj + i0, wptr(a.iw, i0), ni);
+  insert_all(ho, a.oadj + o0, a.ow + o0, no);
+  insert_all(hi, a.iadj + i0, a.iw + i0, ni);
   __syncthreads();
   const int64_t c_r = ht_weight(hi, r), c_s = ht_weight(hi, s);
   const bool r_in_a = ht_find(ho, r) >= 0, s_in_a = ht_find(ho, s) >= 0;
@@ -863,7 +858,7 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
         const int32_t t = a.lab[u];
         const bool nd = back && hast_needs(h, u, t);
         double f, b;
-        hast_terms(m, h, u, t, out ? wload(a.ow, o0 + k) : wload(a.iw, i0 + k), nd ? ht_weight(ho, t) : 0,
+        hast_terms(m, h, u, t, out ? a.ow[o0 + k] : a.iw[i0 + k], nd ? ht_weight(ho, t) : 0,
                    nd ? ht_weight(hi, t) : 0, f, b);
         xs[out ? k : no + 2 + k] = back ? b : f;
       }
@@ -932,8 +927,8 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
   const int32_t tot = kout + kin;
   constexpr bool EX = (VAR & VAR_EXACT) != 0, MH = (VAR & VAR_MH) != 0;
   int32_t t_o = -1, w_o = 0, t_i = -1, w_i = 0, u_o = -1, u_i = -1;
-  if (q < no) { w_o = wload(a.ow, o0 + q); u_o = a.oadj[o0 + q]; t_o = a.lab[u_o]; }
-  if (q < ni) { w_i = wload(a.iw, i0 + q); u_i = a.iadj[i0 + q]; t_i = a.lab[u_i]; }
+  if (q < no) { w_o = a.ow[o0 + q]; u_o = a.oadj[o0 + q]; t_o = a.lab[u_o]; }
+  if (q < ni) { w_i = a.iw[i0 + q]; u_i = a.iadj[i0 + q]; t_i = a.lab[u_i]; }
   // proposal: first entry of (out list, in list) whose running weight exceeds the target
   int32_t s;
   if (tot == 0) {
@@ -1261,9 +1256,8 @@ void launch_nodal(const GraphDev &g, LabView lab, const BMView &m, const int32_t
   if (BS <= 0) return;
   NodalArgs a;
   a.lo = lo; a.BS = BS;
-  // unit-weight graphs: no weight loads in the nodal kernels
-  a.optr = g.optr; a.oadj = g.oadj; a.ow = g.wmax == 1 ? nullptr : g.ow;
-  a.iptr = g.iptr; a.iadj = g.iadj; a.iw = g.wmax == 1 ? nullptr : g.iw;
+  a.optr = g.optr; a.oadj = g.oadj; a.ow = g.ow;
+  a.iptr = g.iptr; a.iadj = g.iadj; a.iw = g.iw;
   a.kout = g.kout; a.kin = g.kin; a.lab = lab;
   a.m = m; a.sof = sof; a.coin = coin; a.key = key; a.phase = phase;
   a.prop = prop; a.dS = dS;
