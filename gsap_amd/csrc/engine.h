@@ -145,17 +145,11 @@ inline int64_t big_words(int64_t no, int64_t ni) {
          2 * (no + 2 + ni);
 }
 constexpr int64_t BIG_LDS_CAP = 40 * 1024;
-// A second stream of the context and the two events that fork work onto it and join it back
-// (st = nullptr: everything on the main stream)
-struct SideStream {
-  hipStream_t st = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
-};
 void launch_nodal(const GraphDev &g, LabView lab, const BMView &m, const int32_t *sof,
                   const uint8_t *coin, RngKey key, PhaseRef phase, int64_t lo, int64_t BS,
                   const int32_t *const lists[NCLS], const int32_t counts[NCLS], uint32_t big_lds,
                   int32_t *prop, double *dS, float *err, uint8_t *near, bool exact, bool mh,
-                  double beta, hipStream_t st, const SideStream &side);
+                  double beta, hipStream_t st);
 void launch_mix_stats(const float *err, const uint8_t *near, int64_t n, uint32_t *max_bits,
                       unsigned long long *count, hipStream_t st);
 // Labels of a nodal batch (greedy, all = false: where dS < 0; Metropolis-Hastings, all = true:

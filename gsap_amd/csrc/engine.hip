@@ -184,7 +184,6 @@ struct gsap_ctx {
   std::vector<LevelSnap> *level_rec = nullptr;  // when set, partition_impl appends snapshots
   int device = 0;
   hipStream_t st = nullptr;
-  SideStream side;  // the nodal workgroup form runs here beside the main stream's kernels
   std::string err;
   std::vector<gsap_trace_rec> trace;
   // graph
@@ -758,7 +757,7 @@ void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, PhaseRef phase, 
   launch_nodal(c->g, lv, m, c->sof.as<int32_t>(), c->coin.as<uint8_t>(),
                rng_key(o), phase, lo, BS, lists, counts, c->big_lds[b], c->prop.as<int32_t>(),
                c->dS.as<double>(), err, evcls, o.delta_mode == GSAP_DELTA_EXACT,
-               o.accept == GSAP_ACCEPT_MH, o.beta, st, c->side);
+               o.accept == GSAP_ACCEPT_MH, o.beta, st);
   if (mixed) {
     uint32_t *ms = c->mixstat.as<uint32_t>();
     launch_mix_stats(err, evcls, BS, ms, reinterpret_cast<unsigned long long *>(ms + 2), st);
@@ -1160,14 +1159,8 @@ int gsap_create(gsap_ctx **ctx, int device) {
     return GSAP_ERR_DEVICE;
   }
   std::memset(c->pub, 0, sizeof(HostPub));
-  // this device's address of the no-allocation entry point's scratch array; the side stream
-  // (default priority) and its fork / join events
-  if (!noalloc_init_device(device) || hipStreamCreate(&c->side.st) != hipSuccess ||
-      hipEventCreateWithFlags(&c->side.fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->side.join, hipEventDisableTiming) != hipSuccess) {
-    if (c->side.join) (void)hipEventDestroy(c->side.join);
-    if (c->side.fork) (void)hipEventDestroy(c->side.fork);
-    if (c->side.st) (void)hipStreamDestroy(c->side.st);
+  // this device's address of the no-allocation entry point's scratch array
+  if (!noalloc_init_device(device)) {
     (void)hipHostFree(c->pub);
     (void)hipStreamDestroy(c->st);
     delete c;
@@ -1196,10 +1189,6 @@ int gsap_destroy(gsap_ctx *c) {
   for (DBuf *b : all) b->release();
   try { c->prof.destroy(); } catch (...) {}
   c->stage.release();
-  (void)hipStreamSynchronize(c->side.st);
-  (void)hipEventDestroy(c->side.join);
-  (void)hipEventDestroy(c->side.fork);
-  (void)hipStreamDestroy(c->side.st);
   (void)hipHostFree(c->pub);
   (void)hipStreamDestroy(c->st);
   delete c;

@@ -710,12 +710,7 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
   }
   __syncthreads();
   // four entries per thread in flight: ids and weights, then their blocks, then inserts
-  // Each entry's block is also kept in the first word of its slot in xs, which the same
-  // thread reads back in the slot loops below (entry j and slot j belong to thread
-  // j mod 256 in both), so the slots need no second gather of ids and labels.
-  int32_t *xt = reinterpret_cast<int32_t *>(xs);
-  auto insert_all = [&](const HTab &h, const int32_t *adj, const int32_t *wt, int32_t n,
-                        int32_t xoff) {
+  auto insert_all = [&](const HTab &h, const int32_t *adj, const int32_t *wt, int32_t n) {
     for (int32_t j0 = tid; j0 < n; j0 += 4 * 256) {
       int32_t t[4], w[4];
 #pragma unroll
@@ -729,14 +724,11 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
         if (j0 + q * 256 < n) t[q] = a.lab[t[q]];
 #pragma unroll
       for (int q = 0; q < 4; ++q)
-        if (j0 + q * 256 < n) {
-          ht_insert(h, t[q], w[q], j0 + q * 256);
-          xt[2 * (xoff + j0 + q * 256)] = t[q];
-        }
+        if (j0 + q * 256 < n) ht_insert(h, t[q], w[q], j0 + q * 256);
     }
   };
-  insert_all(ho, a.oadj + o0, a.ow + o0, no, 0);
-  insert_all(hi, a.iadj + i0, a.iw + i0, ni, no + 2);
+  insert_all(ho, a.oadj + o0, a.ow + o0, no);
+  insert_all(hi, a.iadj + i0, a.iw + i0, ni);
   __syncthreads();
   const int64_t c_r = ht_weight(hi, r), c_s = ht_weight(hi, s);
   const bool r_in_a = ht_find(ho, r) >= 0, s_in_a = ht_find(ho, s) >= 0;
@@ -756,7 +748,7 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
     int64_t av = 0;
     bool use;
     if (j < no) {
-      t = xt[2 * j];
+      t = a.lab[a.oadj[o0 + j]];
       const int32_t slot = ht_find(ho, t);
       use = ld_relaxed(&ho.first[slot]) == j;
       av = ld_relaxed(&ho.w[slot]);
@@ -788,7 +780,7 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
   }
   // column slots (t not in {r, s})
   for (int32_t j = tid; j < ni; j += 256) {
-    const int32_t t = xt[2 * (no + 2 + j)];
+    const int32_t t = a.lab[a.iadj[i0 + j]];
     const int32_t slot = ht_find(hi, t);
     double xv = 0.0;
     if (ld_relaxed(&hi.first[slot]) == j && t != r && t != s) {
@@ -1213,14 +1205,17 @@ struct NodalLists {
   uint32_t big_lds;
 };
 
-// One launch per batch for the vertices of at most 128 neighbours: 4 vertices per
-// workgroup (a wave each, 6 KB of LDS tables per wave), then groups of 32 and 16 lanes.  The
-// larger vertices (one workgroup each) have a kernel of their own, with its own register
-// budget, that runs beside this one.
+// One launch per batch: the largest vertices first (one workgroup each), then 4 vertices
+// per workgroup (a wave each, 6 KB of LDS tables per wave), then groups of 32 and 16 lanes.
 template <bool MIX, int VAR>
 GSAP_D void nodal_fused(const NodalArgs &a, const NodalLists &L, double *lds_d) {
   log2_tables_to_lds();
   int32_t b = (int32_t)blockIdx.x;
+  if (b < L.nb[3]) {
+    nodal_big<MIX, VAR>(a, L.list[3][b], lds_d, L.big_lds);
+    return;
+  }
+  b -= L.nb[3];
   if (b < L.nb[2]) {
     const int32_t k = b * 4 + (int32_t)(threadIdx.x >> 6);
     if (k < L.n[2])
@@ -1244,14 +1239,6 @@ __global__ __launch_bounds__(256) void k_nodal_fused(NodalArgs a, NodalLists L) 
   nodal_fused<MIX, VAR>(a, L, lds_d);
 }
 
-// vertices of more than 128 neighbours (class 3), one workgroup each
-template <bool MIX, int VAR>
-__global__ __launch_bounds__(256) void k_nodal_big(NodalArgs a, NodalLists L) {
-  extern __shared__ double lds_d[];
-  log2_tables_to_lds();
-  nodal_big<MIX, VAR>(a, L.list[3][blockIdx.x], lds_d, L.big_lds);
-}
-
 // The FP64 greedy forms fit 96 registers without spills: five waves per SIMD instead of four
 // (the mixed and Metropolis-Hastings forms would spill there).
 template <bool MIX, int VAR>
@@ -1265,7 +1252,7 @@ void launch_nodal(const GraphDev &g, LabView lab, const BMView &m, const int32_t
                   const uint8_t *coin, RngKey key, PhaseRef phase, int64_t lo, int64_t BS,
                   const int32_t *const lists[NCLS], const int32_t counts[NCLS], uint32_t big_lds,
                   int32_t *prop, double *dS, float *err, uint8_t *near, bool exact, bool mh,
-                  double beta, hipStream_t st, const SideStream &side) {
+                  double beta, hipStream_t st) {
   if (BS <= 0) return;
   NodalArgs a;
   a.lo = lo; a.BS = BS;
@@ -1279,51 +1266,31 @@ void launch_nodal(const GraphDev &g, LabView lab, const BMView &m, const int32_t
   a.sloop = g.sloop; a.beta = beta;
   NodalLists L;
   const int per_block[NCLS] = {16, 8, 4, 1};
+  int64_t total = 0;
   for (int k = 0; k < NCLS; ++k) {
     L.list[k] = lists[k];
     L.n[k] = counts[k];
     L.nb[k] = (counts[k] + per_block[k] - 1) / per_block[k];
+    total += L.nb[k];
   }
   L.big_lds = big_lds;
-  const int64_t small = (int64_t)L.nb[0] + L.nb[1] + L.nb[2];
-  // the workgroup form's kernel on the side stream (when there is one), beside the rest
-  const bool fork = L.nb[3] > 0 && small > 0 && side.st;
-  hipStream_t bst = fork ? side.st : st;
-  if (fork) {
-    if (hipEventRecord(side.fork, st) != hipSuccess ||
-        hipStreamWaitEvent(side.st, side.fork, 0) != hipSuccess)
-      return;  // the caller's error check reports it
-  }
-  if (L.nb[3] > 0) {
-    const dim3 gb((unsigned)L.nb[3]);
-    if (err && !exact) k_nodal_big<true, 0><<<gb, 256, big_lds, bst>>>(a, L);
-    else if (err) k_nodal_big<true, VAR_EXACT><<<gb, 256, big_lds, bst>>>(a, L);
-    else if (!exact && !mh) k_nodal_big<false, 0><<<gb, 256, big_lds, bst>>>(a, L);
-    else if (!mh) k_nodal_big<false, VAR_EXACT><<<gb, 256, big_lds, bst>>>(a, L);
-    else if (!exact) k_nodal_big<false, VAR_MH><<<gb, 256, big_lds, bst>>>(a, L);
-    else k_nodal_big<false, VAR_EXACT | VAR_MH><<<gb, 256, big_lds, bst>>>(a, L);
-  }
-  if (small > 0) {
-    const size_t lds = (size_t)4 * 6 * 4 * LCAP;  // four waves' tables
-    const dim3 grid((unsigned)small);
-    // the mixed form supports the greedy rule only (the host rejects mixed with MH)
-    if (err && !exact)
-      k_nodal_fused<true, 0><<<grid, 256, lds, st>>>(a, L);
-    else if (err)
-      k_nodal_fused<true, VAR_EXACT><<<grid, 256, lds, st>>>(a, L);
-    else if (!exact && !mh)
-      k_nodal_fused5<false, 0><<<grid, 256, lds, st>>>(a, L);
-    else if (!mh)
-      k_nodal_fused5<false, VAR_EXACT><<<grid, 256, lds, st>>>(a, L);
-    else if (!exact)
-      k_nodal_fused<false, VAR_MH><<<grid, 256, lds, st>>>(a, L);
-    else
-      k_nodal_fused<false, VAR_EXACT | VAR_MH><<<grid, 256, lds, st>>>(a, L);
-  }
-  if (fork) {
-    if (hipEventRecord(side.join, side.st) != hipSuccess) return;
-    (void)hipStreamWaitEvent(st, side.join, 0);
-  }
+  if (total == 0) return;
+  size_t lds = (size_t)4 * 6 * 4 * LCAP;  // four waves' tables
+  if (L.nb[3] > 0 && big_lds > lds) lds = big_lds;
+  const dim3 grid((unsigned)total);
+  // the mixed form supports the greedy rule only (the host rejects mixed with MH)
+  if (err && !exact)
+    k_nodal_fused<true, 0><<<grid, 256, lds, st>>>(a, L);
+  else if (err)
+    k_nodal_fused<true, VAR_EXACT><<<grid, 256, lds, st>>>(a, L);
+  else if (!exact && !mh)
+    k_nodal_fused5<false, 0><<<grid, 256, lds, st>>>(a, L);
+  else if (!mh)
+    k_nodal_fused5<false, VAR_EXACT><<<grid, 256, lds, st>>>(a, L);
+  else if (!exact)
+    k_nodal_fused<false, VAR_MH><<<grid, 256, lds, st>>>(a, L);
+  else
+    k_nodal_fused<false, VAR_EXACT | VAR_MH><<<grid, 256, lds, st>>>(a, L);
 }
 
 // Mixed-form statistics of one batch: max |dS32 - dS64| (float bits, order-preserving for
