@@ -1242,10 +1242,11 @@ __global__ __launch_bounds__(256) void k_dense_moves(const int32_t *optr, const 
   }
 }
 
-// B <= SMALL_B: the same edge moves summed in an LDS copy of the B x B matrix per workgroup
-// (a grid-stride loop over the batch, 32 vertices per pass), then added to D with one global
-// atomic per changed cell: a few blocks would otherwise funnel every move into the same
-// cells.
+// B <= SMALL_B: the same edge moves summed in an LDS copy of the B x B matrix per workgroup,
+// then added to D with one global atomic per changed cell: a few blocks would otherwise
+// funnel every move into the same cells.  Each wave takes 64 vertices at a time (a lane
+// each), finds the moved ones by ballot, and walks every moved vertex's edges with all 64
+// lanes in turn (a vertex of thousands of neighbours is not left to a small team).
 __global__ __launch_bounds__(512) void k_dense_moves_lds(const int32_t *optr, const int32_t *oadj,
                                                          const int32_t *ow, const int32_t *iptr,
                                                          const int32_t *iadj, const int32_t *iw,
@@ -1256,27 +1257,38 @@ __global__ __launch_bounds__(512) void k_dense_moves_lds(const int32_t *optr, co
   const int32_t BB = B * B;
   for (int32_t k = threadIdx.x; k < BB; k += 512) dl[k] = 0;
   __syncthreads();
-  const int q = threadIdx.x & 15;
-  for (int64_t i = (int64_t)blockIdx.x * 32 + (threadIdx.x >> 4); i < BS;
-       i += (int64_t)gridDim.x * 32) {
-    const int64_t x = lo + i;
-    const int32_t xo = olab[i], xn = lab[x];
-    if (xo == xn) continue;
-    for (int32_t j = optr[x] + q; j < optr[x + 1]; j += 16) {
-      const int32_t y = oadj[j], w = ow[j];
-      const int64_t k = (int64_t)y - lo;
-      const int32_t yn = lab[y];
-      const int32_t yo = (k >= 0 && k < BS) ? olab[k] : yn;
-      atomicSub(&dl[xo * B + yo], w);
-      atomicAdd(&dl[xn * B + yn], w);
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = (int64_t)gridDim.x * 8;
+  for (int64_t b0 = ((int64_t)blockIdx.x * 8 + (threadIdx.x >> 6)) * 64; b0 < BS; b0 += nw * 64) {
+    const int64_t iv = b0 + lane;
+    int32_t xo = 0, xn = 0;
+    if (iv < BS) {
+      xo = olab[iv];
+      xn = lab[lo + iv];
     }
-    for (int32_t j = iptr[x] + q; j < iptr[x + 1]; j += 16) {
-      const int32_t y = iadj[j], w = iw[j];
-      const int64_t k = (int64_t)y - lo;
-      const int32_t yl = lab[y];
-      if (k >= 0 && k < BS && olab[k] != yl) continue;
-      atomicSub(&dl[yl * B + xo], w);
-      atomicAdd(&dl[yl * B + xn], w);
+    uint64_t moved = __ballot(iv < BS && xo != xn);
+    while (moved) {
+      const int src = __ffsll((long long)moved) - 1;
+      moved &= moved - 1;
+      const int64_t x = lo + b0 + src;
+      const int32_t vo = __shfl(xo, src, 64), vn = __shfl(xn, src, 64);
+      const int32_t o0 = optr[x], o1 = optr[x + 1], i0 = iptr[x], i1 = iptr[x + 1];
+      for (int32_t j = o0 + lane; j < o1; j += 64) {
+        const int32_t y = oadj[j], w = ow[j];
+        const int64_t k = (int64_t)y - lo;
+        const int32_t yn = lab[y];
+        const int32_t yo = (k >= 0 && k < BS) ? olab[k] : yn;
+        atomicSub(&dl[vo * B + yo], w);
+        atomicAdd(&dl[vn * B + yn], w);
+      }
+      for (int32_t j = i0 + lane; j < i1; j += 64) {
+        const int32_t y = iadj[j], w = iw[j];  // edge y -> x
+        const int64_t k = (int64_t)y - lo;
+        const int32_t yl = lab[y];
+        if (k >= 0 && k < BS && olab[k] != yl) continue;  // y moved: one of y's out-edges
+        atomicSub(&dl[yl * B + vo], w);
+        atomicAdd(&dl[yl * B + vn], w);
+      }
     }
   }
   __syncthreads();
@@ -1359,7 +1371,8 @@ bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t 
     return false;
   const unsigned grid = (unsigned)std::min<int64_t>(512, (BS + 31) / 32);
   if (BS > 0 && B <= SMALL_B)
-    k_dense_moves_lds<<<grid, 512, (size_t)B * B * 4, st>>>(
+    k_dense_moves_lds<<<(unsigned)std::min<int64_t>(512, (BS + 255) / 256), 512,
+                        (size_t)B * B * 4, st>>>(
         g.optr, g.oadj, g.ow, g.iptr, g.iadj, g.iw, lab32, olab, lo, BS, B, bm.dense);
   else if (BS > 0 && B <= INC_TILE_B) {
     const int32_t tiles = (B + MT_WORDS / B - 1) / (MT_WORDS / B);
