@@ -1144,8 +1144,11 @@ static void dense_to_csr(int32_t B, BMBuf &bm, RebuildScratch &rs, hipStream_t s
   scan_exclusive(cnt, (uint32_t *)bm.rowptr, B, (uint32_t *)(bm.rowptr + B), rs.tmp, st);
   k_dense_write_rows<<<rb, 256, 0, st>>>(D, B, bm.rowptr, bm.col, bm.val, bm.erow);
   int32_t *DT = rs.partials;
-  const unsigned tb = (unsigned)((B + 31) / 32);
-  k_transpose<<<dim3(tb, tb), 256, 0, st>>>(D, B, DT);
+  if (bm.dt_B != B) {  // else the incremental update kept the transpose current
+    const unsigned tb = (unsigned)((B + 31) / 32);
+    k_transpose<<<dim3(tb, tb), 256, 0, st>>>(D, B, DT);
+    bm.dt_B = B;
+  }
   k_dense_rows<<<rb, 256, 0, st>>>(DT, B, cnt, bm.din);
   scan_exclusive(cnt, (uint32_t *)bm.colptr, B, (uint32_t *)(bm.colptr + B), rs.tmp, st);
   k_dense_write_rows<<<rb, 256, 0, st>>>(DT, B, bm.colptr, bm.row, bm.cval, nullptr);
@@ -1212,12 +1215,13 @@ static void small_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
 // before the batch was applied.  Every edge with a moved endpoint leaves its old cell and
 // enters its new one, counted once: the out-edges of a moved vertex (target labels old and
 // new), and the in-edges of a moved vertex whose source did not move.
+// (DT, when given, is D's transpose and receives the mirrored changes)
 __global__ __launch_bounds__(256) void k_dense_moves(const int32_t *optr, const int32_t *oadj,
                                                      const int32_t *ow, const int32_t *iptr,
                                                      const int32_t *iadj, const int32_t *iw,
                                                      const int32_t *lab, const int32_t *olab,
                                                      int64_t lo, int64_t BS, int32_t B,
-                                                     int32_t *D) {
+                                                     int32_t *D, int32_t *DT) {
   const int64_t i = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
   const int q = threadIdx.x & 15;
   if (i >= BS) return;
@@ -1231,6 +1235,10 @@ __global__ __launch_bounds__(256) void k_dense_moves(const int32_t *optr, const 
     const int32_t yo = (k >= 0 && k < BS) ? olab[k] : yn;
     atomicSub(&D[(int64_t)xo * B + yo], w);
     atomicAdd(&D[(int64_t)xn * B + yn], w);
+    if (DT) {
+      atomicSub(&DT[(int64_t)yo * B + xo], w);
+      atomicAdd(&DT[(int64_t)yn * B + xn], w);
+    }
   }
   for (int32_t j = iptr[x] + q; j < iptr[x + 1]; j += 16) {
     const int32_t y = iadj[j], w = iw[j];  // edge y -> x
@@ -1239,6 +1247,10 @@ __global__ __launch_bounds__(256) void k_dense_moves(const int32_t *optr, const 
     if (k >= 0 && k < BS && olab[k] != yl) continue;  // y moved: one of y's out-edges
     atomicSub(&D[(int64_t)yl * B + xo], w);
     atomicAdd(&D[(int64_t)yl * B + xn], w);
+    if (DT) {
+      atomicSub(&DT[(int64_t)xo * B + yl], w);
+      atomicAdd(&DT[(int64_t)xn * B + yl], w);
+    }
   }
 }
 
@@ -1380,9 +1392,11 @@ bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t 
     k_dense_moves_tile<<<(unsigned)(tiles * nchunk), 512, 0, st>>>(
         g.optr, g.oadj, g.ow, g.iptr, g.iadj, g.iw, lab32, olab, lo, BS, B, nchunk, bm.dense);
   }
-  else if (BS > 0)
+  else if (BS > 0)  // the transpose follows when it is current for this B
     k_dense_moves<<<(unsigned)((BS + 15) / 16), 256, 0, st>>>(
-        g.optr, g.oadj, g.ow, g.iptr, g.iadj, g.iw, lab32, olab, lo, BS, B, bm.dense);
+        g.optr, g.oadj, g.ow, g.iptr, g.iadj, g.iw, lab32, olab, lo, BS, B, bm.dense,
+        bm.dt_B == B ? rs.partials : nullptr);
+  if (B <= INC_TILE_B) bm.dt_B = 0;  // the LDS forms change D alone
   // (the 16-bit label copy was kept current by the batch's apply)
   if (B <= SMALL_B)
     k_small_bm<<<1, 1024, 0, st>>>(bm.dense, B, bm.rowptr, bm.col, bm.val, bm.erow, bm.colptr,
@@ -1418,6 +1432,9 @@ void rebuild_blockmodel(const GraphDev &g, const int32_t *lab32, int32_t B, BMBu
     k_lab16<<<grid_for(g.N, 256), 256, 0, st>>>(lab32, g.N, rs.lab16);
     lab.l16 = rs.lab16;
   }
+  // the full paths rewrite D (and the small and tile paths the partial-plane buffer that
+  // holds its transpose); dense_to_csr makes the transpose current again where it runs
+  bm.dt_B = 0;
   if (small) small_path(g, lab, B, bm, rs, st);
   else if (tile) tile_path(g, lab, B, bm, rs, st);
   else seg_path(g, lab, B, bm, rs, dense, st);

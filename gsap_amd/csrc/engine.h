@@ -38,6 +38,9 @@ struct BMBuf {
   int32_t *gcol = nullptr, *grow = nullptr;       // E/16 + 1: coarse index of col / row
   bool index_valid = false;
   bool dense_valid = false;
+  // the transpose of the dense matrix in the rebuild scratch's partial-plane buffer is
+  // current for B = dt_B (0: not current); the incremental update keeps it so
+  int32_t dt_B = 0;
   // B of the entries the dense buffer holds (the current CSR's); 0 = all zero
   int32_t dense_B = 0;
   BMView view() const {
