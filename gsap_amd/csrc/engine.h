@@ -119,8 +119,10 @@ struct PhaseRef {
   const uint32_t *dev;
   uint32_t add;
 };
+// zero: NCLS words cleared by the launch (the nodal proposal pass's counters), or null
 void launch_block_terms(const BMView &m, double *OUT, int32_t *sof, uint8_t *coin, RngKey key,
-                        PhaseRef phase, bool sample, int32_t avg_row, hipStream_t st);
+                        PhaseRef phase, bool sample, int32_t avg_row, hipStream_t st,
+                        int32_t *zero = nullptr);
 // Nodal kernels by the larger of a vertex's out- and in-neighbour counts d: d <= 16 and 32
 // use register deduplication in groups of 16 and 32 lanes; d <= 128 a wave with 256-slot
 // LDS hash tables; larger d a 256-thread workgroup with tables sized for the vertex (LDS
@@ -148,11 +150,18 @@ inline int64_t big_words(int64_t no, int64_t ni) {
          2 * (no + 2 + ni);
 }
 constexpr int64_t BIG_LDS_CAP = 40 * 1024;
+// Unit-weight graphs: the proposal pass lists the vertices that propose another block by
+// class (dlist: NCLS x BS words) and counts them in dcnt (NCLS words, cleared by the batch's
+// block terms before it).  dlist = null: no proposal pass.
+struct NodalPre {
+  int32_t *dlist = nullptr, *dcnt = nullptr;
+  NodalBounds bounds;
+};
 void launch_nodal(const GraphDev &g, LabView lab, const BMView &m, const int32_t *sof,
                   const uint8_t *coin, RngKey key, PhaseRef phase, int64_t lo, int64_t BS,
                   const int32_t *const lists[NCLS], const int32_t counts[NCLS], uint32_t big_lds,
                   int32_t *prop, double *dS, float *err, uint8_t *near, bool exact, bool mh,
-                  double beta, hipStream_t st);
+                  double beta, const NodalPre &pre, hipStream_t st);
 void launch_mix_stats(const float *err, const uint8_t *near, int64_t n, uint32_t *max_bits,
                       unsigned long long *count, hipStream_t st);
 // Labels of a nodal batch (greedy, all = false: where dS < 0; Metropolis-Hastings, all = true:

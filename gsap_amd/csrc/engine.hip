@@ -202,6 +202,8 @@ struct gsap_ctx {
   DBuf errv, evcls, mixstat;
   // the current batch's labels before it was applied (incremental dense rebuild)
   DBuf olab;
+  // the nodal proposal pass's class lists and their counters (NodalPre)
+  DBuf dlist, dcnt;
   // per batch, the batch's vertices split by neighbour-count class (nodal_class)
   DBuf vlist;
   std::vector<int64_t> list_off;  // [batch * 4 + class] -> offset into vlist; last = total
@@ -492,6 +494,8 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   c->mk0.ensure(8 * N); c->mk1.ensure(8 * N); c->mv0.ensure(4 * N); c->mv1.ensure(4 * N);
   c->errv.ensure(4 * N); c->evcls.ensure(N); c->mixstat.ensure(32);
   c->olab.ensure(4 * N);
+  c->dlist.ensure(4 * (size_t)NCLS * N);
+  c->dcnt.ensure(4 * NCLS);
   BMBuf &b = c->bm;
   b.rowptr = c->rowptr.as<int32_t>(); b.colptr = c->colptr.as<int32_t>();
   b.col = c->col.as<int32_t>(); b.val = c->val.as<int32_t>(); b.erow = c->erow.as<int32_t>();
@@ -722,18 +726,27 @@ int32_t merge_phase(gsap_ctx *c, const gsap_options &o, int32_t B_to_merge, uint
   return B - merged;
 }
 
-// One nodal batch on the current block matrix; the accepted delta-S sum lands in *dE_dev
-// (nullptr: only the batch's tile sums, totalled by the sweep's launch_sweep_sums).
 // The shared neighbour samples of a nodal batch (its block terms); they only read the block
-// matrix and the batch counter.
+// matrix and the batch counter, and clear the proposal pass's class counters.
 void batch_terms(gsap_ctx *c, const gsap_options &o, PhaseRef phase, int32_t ar) {
   int p = c->prof.begin(2, c->st);
   launch_block_terms(c->bm.view(), nullptr, c->sof.as<int32_t>(), c->coin.as<uint8_t>(),
-                     rng_key(o), phase, true, ar, c->st);
+                     rng_key(o), phase, true, ar, c->st, c->dcnt.as<int32_t>());
   c->prof.end(p, c->st);
 }
 
-// terms_queued: the batch's block terms are already on the stream
+// The proposal pass's lists and counters (the counters cleared by the batch's block terms)
+NodalPre nodal_pre(gsap_ctx *c) {
+  NodalPre p;
+  p.dlist = c->dlist.as<int32_t>();
+  p.dcnt = c->dcnt.as<int32_t>();
+  p.bounds = c->lists_bounds;
+  return p;
+}
+
+// One nodal batch on the current block matrix; the accepted delta-S sum lands in *dE_dev
+// (nullptr: only the batch's tile sums, totalled by the sweep's launch_sweep_sums).
+// terms_queued: the batch's block terms are already on the stream.
 void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, PhaseRef phase, int32_t ar,
                  double *dE_dev, bool terms_queued = false) {
   const int64_t BS = c->N / o.num_batches, lo = (int64_t)b * BS;
@@ -757,7 +770,7 @@ void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, PhaseRef phase, 
   launch_nodal(c->g, lv, m, c->sof.as<int32_t>(), c->coin.as<uint8_t>(),
                rng_key(o), phase, lo, BS, lists, counts, c->big_lds[b], c->prop.as<int32_t>(),
                c->dS.as<double>(), err, evcls, o.delta_mode == GSAP_DELTA_EXACT,
-               o.accept == GSAP_ACCEPT_MH, o.beta, st);
+               o.accept == GSAP_ACCEPT_MH, o.beta, nodal_pre(c), st);
   if (mixed) {
     uint32_t *ms = c->mixstat.as<uint32_t>();
     launch_mix_stats(err, evcls, BS, ms, reinterpret_cast<unsigned long long *>(ms + 2), st);
@@ -1177,7 +1190,8 @@ int gsap_destroy(gsap_ctx *c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->st);
   DBuf *all[] = {&c->optr, &c->oadj, &c->ow, &c->iptr, &c->iadj, &c->iw, &c->kout, &c->kin,
-                 &c->esrc, &c->idst, &c->sloop, &c->hoff, &c->hbo, &c->hbi, &c->hpool, &c->lab, &c->snap[0],
+                 &c->esrc, &c->idst, &c->sloop, &c->hoff, &c->hbo, &c->hbi, &c->hpool, &c->lab,
+                 &c->dlist, &c->dcnt, &c->snap[0],
                  &c->snap[1], &c->snap[2], &c->rowptr, &c->colptr, &c->col, &c->val, &c->erow,
                  &c->row, &c->cval, &c->dout, &c->din, &c->ldout, &c->ldin, &c->dense,
                  &c->gcol, &c->grow, &c->k0,

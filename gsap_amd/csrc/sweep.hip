@@ -176,9 +176,11 @@ static unsigned grid_for(int64_t n, int threads) {
 // by the merge delta-S, and the nodal per-block neighbour sample and coin (kernels.hpp:942-997,
 // 1024-1029; one draw per block per batch, shared by its vertices).
 template <int G>
+// (zero, when given: NCLS words cleared for the nodal proposal pass that follows)
 __global__ __launch_bounds__(256) void k_block_terms(BMView m, double *OUT, int32_t *sof,
                                                      uint8_t *coin, RngKey key, PhaseRef pr,
-                                                     int sample) {
+                                                     int sample, int32_t *zero) {
+  if (zero && blockIdx.x == 0 && threadIdx.x < NCLS) zero[threadIdx.x] = 0;
   if (OUT) log2_tables_to_lds();  // only the row terms take logarithms
   const int32_t x = (int32_t)(blockIdx.x * (256 / G) + threadIdx.x / G);
   if (x >= m.B) return;
@@ -207,16 +209,17 @@ __global__ __launch_bounds__(256) void k_block_terms(BMView m, double *OUT, int3
 }
 
 void launch_block_terms(const BMView &m, double *OUT, int32_t *sof, uint8_t *coin, RngKey key,
-                        PhaseRef phase, bool sample, int32_t avg_row, hipStream_t st) {
+                        PhaseRef phase, bool sample, int32_t avg_row, hipStream_t st,
+                        int32_t *zero) {
   const int s = sample ? 1 : 0;
   if (avg_row <= 12)
-    k_block_terms<8><<<grid_for(m.B, 32), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s);
+    k_block_terms<8><<<grid_for(m.B, 32), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s, zero);
   else if (avg_row <= 24)
-    k_block_terms<16><<<grid_for(m.B, 16), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s);
+    k_block_terms<16><<<grid_for(m.B, 16), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s, zero);
   else if (avg_row <= 48)
-    k_block_terms<32><<<grid_for(m.B, 8), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s);
+    k_block_terms<32><<<grid_for(m.B, 8), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s, zero);
   else
-    k_block_terms<64><<<grid_for(m.B, 4), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s);
+    k_block_terms<64><<<grid_for(m.B, 4), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s, zero);
 }
 
 // ------------------------------------------------------------- nodal moves
@@ -240,6 +243,12 @@ struct NodalArgs {
   uint8_t *near;   // mixed form: 0 FP32 rejection, 1 FP64 evaluated, 2 |dS32| within bound
   const int32_t *sloop;  // exact delta: self-loop weight per vertex
   double beta;           // Metropolis-Hastings inverse temperature
+  int32_t pre;           // 1: prop already holds every proposal (k_nodal_propose)
+  // with pre: the vertices whose proposal is another block, listed by class by the proposal
+  // kernel (class c at dlist + c * BS, dcnt[c] of them), and the class bounds
+  int32_t *dlist;
+  int32_t *dcnt;
+  int32_t cb16, cb32, cbwave;
 };
 
 // Variants of the nodal kernels (template bits): the exact delta-S and Metropolis-Hastings
@@ -355,7 +364,10 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
   const BMView &m = a.m;
   const int32_t B = m.B;
   const int32_t r = a.lab[v];
-  const Philox x = philox((uint32_t)v, nodal_phase(a), a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
+  constexpr bool MHV = (VAR & VAR_MH) != 0;
+  Philox x{0u, 0u, 0u, 0u};
+  if (!a.pre || MHV)
+    x = philox((uint32_t)v, nodal_phase(a), a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
   const int32_t o0 = a.optr[v], no = a.optr[v + 1] - o0;
   const int32_t i0 = a.iptr[v], ni = a.iptr[v + 1] - i0;
   // weights, degrees and block-matrix entries stay below 2^30 (the host rejects larger total
@@ -374,7 +386,9 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
   const int32_t dout_r = m.dout[r], din_r = m.din[r];
   const double ldo_r = m.ldout[r], ldi_r = m.ldin[r];
   int32_t s;
-  if (tot == 0) {
+  if (a.pre) {
+    s = a.prop[i];
+  } else if (tot == 0) {
     s = (int32_t)mulhi(x.x0, B);
   } else {
     const uint32_t target = mulhi(x.x1, tot);
@@ -400,7 +414,7 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
     }
     s = a.coin[u] ? (int32_t)mulhi(x.x0, B) : a.sof[u];
   }
-  if (lane == 0) a.prop[i] = s;
+  if (lane == 0 && !a.pre) a.prop[i] = s;
   if (s == r) {
     if (lane == 0) {
       a.dS[i] = 0.0;
@@ -667,13 +681,18 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
   const BMView &m = a.m;
   const int32_t B = m.B;
   const int32_t r = a.lab[v];
-  const Philox x = philox((uint32_t)v, nodal_phase(a), a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
+  constexpr bool MHV = (VAR & VAR_MH) != 0;
+  Philox x{0u, 0u, 0u, 0u};
+  if (!a.pre || MHV)
+    x = philox((uint32_t)v, nodal_phase(a), a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
   const int32_t o0 = a.optr[v], no = a.optr[v + 1] - o0;
   const int32_t i0 = a.iptr[v], ni = a.iptr[v + 1] - i0;
   const int64_t kout = a.kout[v], kin = a.kin[v];
   const int64_t tot = kout + kin;
   int32_t s;  // every thread computes the same proposal
-  if (tot == 0) {
+  if (a.pre) {
+    s = a.prop[i];
+  } else if (tot == 0) {
     s = (int32_t)mulhi(x.x0, B);
   } else {
     const int32_t nbr = block_pick(a.ow + o0, a.oadj + o0, no, a.iw + i0, a.iadj + i0, ni,
@@ -681,7 +700,7 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
     const int32_t u = a.lab[nbr];
     s = a.coin[u] ? (int32_t)mulhi(x.x0, B) : a.sof[u];
   }
-  if (tid == 0) a.prop[i] = s;
+  if (tid == 0 && !a.pre) a.prop[i] = s;
   if (s == r) {
     if (tid == 0) {
       a.dS[i] = 0.0;
@@ -920,18 +939,22 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
   const BMView &m = a.m;
   const int32_t B = m.B;
   const int32_t r = a.lab[v];
-  const Philox x = philox((uint32_t)v, nodal_phase(a), a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
+  constexpr bool EX = (VAR & VAR_EXACT) != 0, MH = (VAR & VAR_MH) != 0;
+  Philox x{0u, 0u, 0u, 0u};
+  if (!a.pre || MH)
+    x = philox((uint32_t)v, nodal_phase(a), a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
   const int32_t o0 = a.optr[v], no = a.optr[v + 1] - o0;
   const int32_t i0 = a.iptr[v], ni = a.iptr[v + 1] - i0;
   const int32_t kout = a.kout[v], kin = a.kin[v];
   const int32_t tot = kout + kin;
-  constexpr bool EX = (VAR & VAR_EXACT) != 0, MH = (VAR & VAR_MH) != 0;
   int32_t t_o = -1, w_o = 0, t_i = -1, w_i = 0, u_o = -1, u_i = -1;
   if (q < no) { w_o = a.ow[o0 + q]; u_o = a.oadj[o0 + q]; t_o = a.lab[u_o]; }
   if (q < ni) { w_i = a.iw[i0 + q]; u_i = a.iadj[i0 + q]; t_i = a.lab[u_i]; }
   // proposal: first entry of (out list, in list) whose running weight exceeds the target
   int32_t s;
-  if (tot == 0) {
+  if (a.pre) {
+    s = a.prop[i];
+  } else if (tot == 0) {
     s = (int32_t)mulhi(x.x0, B);
   } else {
     const uint32_t target = mulhi(x.x1, tot);
@@ -952,7 +975,7 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
     }
     s = a.coin[u] ? (int32_t)mulhi(x.x0, B) : a.sof[u];
   }
-  if (q == 0) a.prop[i] = s;
+  if (q == 0 && !a.pre) a.prop[i] = s;
   if (s == r) {
     if (q == 0) {
       a.dS[i] = 0.0;
@@ -1197,28 +1220,88 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
   }
 }
 
+// Proposals of a batch on a unit-weight graph, one lane per vertex (kernels.hpp:877-939,
+// 1000-1032): with unit weights the first adjacency entry whose running weight exceeds the
+// target is entry `target` itself, so one load finds the neighbour.  Writes prop for every
+// vertex and the outcome of those that propose their own block; the fused kernel's teams
+// then start from prop (NodalArgs::pre).
+__global__ __launch_bounds__(256) void k_nodal_propose(NodalArgs a) {
+  __shared__ uint32_t wc[NCLS], wb[NCLS];
+  if (threadIdx.x < NCLS) wc[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int cls = -1;
+  uint32_t lp = 0;
+  int64_t v = 0;
+  if (i < a.BS) {
+    v = a.lo + i;
+    const int32_t B = a.m.B;
+    const int32_t r = a.lab[v];
+    const int32_t o0 = a.optr[v], no = a.optr[v + 1] - o0;
+    const int32_t i0 = a.iptr[v], ni = a.iptr[v + 1] - i0;
+    const Philox x =
+        philox((uint32_t)v, nodal_phase(a), a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
+    const uint32_t tot = (uint32_t)(no + ni);
+    int32_t s;
+    if (tot == 0) {
+      s = (int32_t)mulhi(x.x0, B);
+    } else {
+      const uint32_t target = mulhi(x.x1, tot);
+      const int32_t nbr = target < (uint32_t)no ? a.oadj[o0 + target] : a.iadj[i0 + target - no];
+      const int32_t u = a.lab[nbr];
+      s = a.coin[u] ? (int32_t)mulhi(x.x0, B) : a.sof[u];
+    }
+    a.prop[i] = s;
+    if (s == r) {
+      a.dS[i] = 0.0;
+      if (a.err) {
+        a.err[i] = 0.0f;
+        a.near[i] = 0;
+      }
+    } else {
+      // the neighbour-count class of the fused kernel's teams
+      const int32_t d = no > ni ? no : ni;
+      cls = d <= a.cb16 ? 0 : d <= a.cb32 ? 1 : d <= a.cbwave ? 2 : 3;
+      lp = atomicAdd(&wc[cls], 1u);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < NCLS)
+    wb[threadIdx.x] = wc[threadIdx.x] ? atomicAdd((uint32_t *)&a.dcnt[threadIdx.x],
+                                                  wc[threadIdx.x]) : 0u;
+  __syncthreads();
+  if (cls >= 0) a.dlist[(int64_t)cls * a.BS + wb[cls] + lp] = (int32_t)v;
+}
+
 // Vertex lists of one batch by class, and the workgroups of each class in the fused launch
 struct NodalLists {
   const int32_t *list[NCLS];
   int32_t n[NCLS];
   int32_t nb[NCLS];
   uint32_t big_lds;
+  const int32_t *dn;  // device counts of the lists (proposal kernel's), or null: n
 };
 
 // One launch per batch: the largest vertices first (one workgroup each), then 4 vertices
 // per workgroup (a wave each, 6 KB of LDS tables per wave), then groups of 32 and 16 lanes.
 template <bool MIX, int VAR>
 GSAP_D void nodal_fused(const NodalArgs &a, const NodalLists &L, double *lds_d) {
-  log2_tables_to_lds();
+  // listed vertices of each class (the grid is sized for all of a batch's vertices)
+  auto cnt = [&](int k) { return L.dn ? L.dn[k] : L.n[k]; };
   int32_t b = (int32_t)blockIdx.x;
   if (b < L.nb[3]) {
+    if (b >= cnt(3)) return;
+    log2_tables_to_lds();
     nodal_big<MIX, VAR>(a, L.list[3][b], lds_d, L.big_lds);
     return;
   }
   b -= L.nb[3];
   if (b < L.nb[2]) {
+    const int32_t n2 = cnt(2);
+    if (b * 4 >= n2) return;
+    log2_tables_to_lds();
     const int32_t k = b * 4 + (int32_t)(threadIdx.x >> 6);
-    if (k < L.n[2])
+    if (k < n2)
       nodal_wave<MIX, VAR>(a, L.list[2][k],
                            reinterpret_cast<int32_t *>(lds_d) + (threadIdx.x >> 6) * 6 * LCAP,
                            (uint32_t)LBITS);
@@ -1226,11 +1309,17 @@ GSAP_D void nodal_fused(const NodalArgs &a, const NodalLists &L, double *lds_d) 
   }
   b -= L.nb[2];
   if (b < L.nb[1]) {
-    nodal_small<32, MIX, VAR>(a, L.list[1], L.n[1], b);
+    const int32_t n1 = cnt(1);
+    if (b * 8 >= n1) return;
+    log2_tables_to_lds();
+    nodal_small<32, MIX, VAR>(a, L.list[1], n1, b);
     return;
   }
   b -= L.nb[1];
-  nodal_small<16, MIX, VAR>(a, L.list[0], L.n[0], b);
+  const int32_t n0 = cnt(0);
+  if (b * 16 >= n0) return;
+  log2_tables_to_lds();
+  nodal_small<16, MIX, VAR>(a, L.list[0], n0, b);
 }
 
 template <bool MIX, int VAR>
@@ -1252,7 +1341,7 @@ void launch_nodal(const GraphDev &g, LabView lab, const BMView &m, const int32_t
                   const uint8_t *coin, RngKey key, PhaseRef phase, int64_t lo, int64_t BS,
                   const int32_t *const lists[NCLS], const int32_t counts[NCLS], uint32_t big_lds,
                   int32_t *prop, double *dS, float *err, uint8_t *near, bool exact, bool mh,
-                  double beta, hipStream_t st) {
+                  double beta, const NodalPre &pre, hipStream_t st) {
   if (BS <= 0) return;
   NodalArgs a;
   a.lo = lo; a.BS = BS;
@@ -1264,16 +1353,27 @@ void launch_nodal(const GraphDev &g, LabView lab, const BMView &m, const int32_t
   a.hoff = g.hoff; a.hbits_o = g.hbits_o; a.hbits_i = g.hbits_i; a.hpool = g.hpool;
   a.err = err; a.near = near;
   a.sloop = g.sloop; a.beta = beta;
+  // unit-weight graphs: every proposal first, one lane per vertex, and the vertices that
+  // propose another block listed by class for the fused kernel
+  a.pre = (g.wmax == 1 && pre.dlist) ? 1 : 0;
+  a.dlist = pre.dlist;
+  a.dcnt = pre.dcnt;
+  a.cb16 = pre.bounds.max16;
+  a.cb32 = pre.bounds.max32;
+  a.cbwave = pre.bounds.maxwave;
+  if (a.pre)
+    k_nodal_propose<<<(unsigned)((BS + 255) / 256), 256, 0, st>>>(a);
   NodalLists L;
   const int per_block[NCLS] = {16, 8, 4, 1};
   int64_t total = 0;
   for (int k = 0; k < NCLS; ++k) {
-    L.list[k] = lists[k];
+    L.list[k] = a.pre ? pre.dlist + (int64_t)k * BS : lists[k];
     L.n[k] = counts[k];
     L.nb[k] = (counts[k] + per_block[k] - 1) / per_block[k];
     total += L.nb[k];
   }
   L.big_lds = big_lds;
+  L.dn = a.pre ? pre.dcnt : nullptr;
   if (total == 0) return;
   size_t lds = (size_t)4 * 6 * 4 * LCAP;  // four waves' tables
   if (L.nb[3] > 0 && big_lds > lds) lds = big_lds;
