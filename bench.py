@@ -31,9 +31,11 @@ from gsap_amd import workloads  # noqa: E402
 
 CONFIGS = {k: v for k, v in workloads.CONFIGS.items() if k != "c5"}
 HBM_PEAK_GBS = 8000.0
-KCLASS = ["nodal", "rebuild", "block_terms", "merge", "mdl", "accept_sum"]
-# device kernels of each class (the library times a class with HIP events around all of them)
+KCLASS = ["nodal", "rebuild", "block_terms", "merge", "mdl", "accept_sum", "nodal_propose"]
+# device kernels of each class (the library times a class with HIP events around all of them;
+# the first matching class names a kernel)
 KERNELS_OF = {
+    "nodal_propose": ["k_nodal_propose"],
     "nodal": ["k_nodal"],
     "rebuild": ["k_label_", "k_tile_hist", "k_edge_hist", "k_hist_reduce", "k_dense_",
                 "k_small_bm", "k_transpose", "k_logs", "k_seg_", "k_scan_", "k_lab16"],
@@ -116,14 +118,13 @@ def summarize_profile(stats_csv, fetch_csv, write_csv, cal_fetch_csv, cal_write_
               open(out_json, "w"), indent=1)
 
 
-def algorithmic_bytes(kclass, g, nb, B, nnz):
-    """Algorithmic HBM bytes of one launch (int32 ids and weights)."""
-    E, N = g.E, g.N
-    if kclass == "nodal":       # one batch: its vertices' out+in adjacency + per-vertex words
-        return (16 * E + 24 * N) / nb
-    if kclass == "rebuild":     # out-adjacency + labels read, CSR+CSC+degrees written
-        return 8 * E + 8 * N + 16 * nnz + 16 * B
-    return None
+def nodal_algorithmic_bytes(evals, adj, launches):
+    """Algorithmic HBM bytes of one launch of the nodal moves kernel: per vertex it
+    evaluates (a proposal of another block) its list entry, row pointers, label, proposal
+    and delta-S (4 + 16 + 4 + 4 + 8 = 36 bytes), per out/in adjacency entry of those
+    vertices the neighbour id, the weight and the neighbour's block as a 16-bit label
+    (4 + 4 + 2 = 10 bytes); evals and adj are totals over `launches` launches."""
+    return (36 * evals + 10 * adj) / max(launches, 1)
 
 
 def cpu_threads():
@@ -330,7 +331,6 @@ def main():
     # batch): HIP events on the work stream around each of its launches, in as many profiled
     # partitions of the same workload
     dom = 0
-    dname = KCLASS[dom]
     pwalls, preps = [], []
     for k in range(a.steps):
         t1 = time.perf_counter()
@@ -339,8 +339,11 @@ def main():
         preps.append(prp)
         assert np.array_equal(plab, lab), "profiled partition differs"
     avg_ms = sum(r.kernel_ms[dom] for r in preps) / max(sum(r.kernel_calls[dom] for r in preps), 1)
-    final = ctx.blockmodel(g, lab, rep.B)
-    abytes = algorithmic_bytes(dname, g, 4, rep.B, final["nnz"])
+    n_launch = sum(r.kernel_calls[dom] for r in preps)
+    evals = sum(r.nodal_evals for r in preps)
+    adj = sum(r.nodal_eval_adj for r in preps)
+    assert evals >= 0 and adj >= 0, "evaluation counts missing (unit-weight graph expected)"
+    abytes = nodal_algorithmic_bytes(evals, adj, n_launch)
     achieved = abytes / (avg_ms * 1e-3) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
@@ -386,7 +389,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_nodal_fused5" if prec != gsap_amd.PRECISION_MIXED else "k_nodal_fused", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "avg_launch_ms": avg_ms,
-                     "algorithmic_bytes_per_launch": abytes},
+                     "algorithmic_bytes_per_launch": abytes,
+                     "evaluated_vertices_per_launch": evals / max(n_launch, 1),
+                     "evaluated_adjacency_per_launch": adj / max(n_launch, 1)},
     }
     if not a.no_compare:
         out["precision_forms"] = compare_forms(ctx, g, lab, prec, thr)

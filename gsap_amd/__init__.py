@@ -71,7 +71,8 @@ class CReport(C.Structure):
                 ("max_abs_err_fp32", C.c_double), ("rechecks", C.c_int64),
                 ("n_trace", C.c_int64), ("kernel_ms", C.c_double * 8),
                 ("kernel_calls", C.c_int64 * 8), ("fp64_evals", C.c_int64),
-                ("t_upload", C.c_double)]
+                ("t_upload", C.c_double), ("nodal_evals", C.c_int64),
+                ("nodal_eval_adj", C.c_int64)]
 
 
 class CQuality(C.Structure):
@@ -282,13 +283,15 @@ class Report:
     kernel_calls: list = field(default_factory=list)
     fp64_evals: int = 0
     t_upload: float = 0.0
+    nodal_evals: int = -1
+    nodal_eval_adj: int = -1
 
     @staticmethod
     def from_c(r: CReport) -> "Report":
         return Report(r.B, r.levels, r.merge_phases, r.sweeps, r.batches, r.S, r.t_total,
                       r.t_merge, r.t_sweep, r.sweep_edges, r.max_abs_err_fp32, r.rechecks,
                       r.n_trace, list(r.kernel_ms), list(r.kernel_calls), r.fp64_evals,
-                      r.t_upload)
+                      r.t_upload, r.nodal_evals, r.nodal_eval_adj)
 
 
 def _check(rc: int, ctx=None):

@@ -153,15 +153,19 @@ constexpr int64_t BIG_LDS_CAP = 40 * 1024;
 // Unit-weight graphs: the proposal pass lists the vertices that propose another block by
 // class (dlist: NCLS x BS words) and counts them in dcnt (NCLS words, cleared by the batch's
 // block terms before it).  dlist = null: no proposal pass.
+// evstat, or null: two running totals (vertices evaluated, their adjacency entries).
 struct NodalPre {
   int32_t *dlist = nullptr, *dcnt = nullptr;
+  unsigned long long *evstat = nullptr;
   NodalBounds bounds;
 };
+// stages of launch_nodal: the proposal pass (unit-weight graphs) and the moves kernel
+constexpr int NODAL_PROPOSE = 1, NODAL_MOVES = 2;
 void launch_nodal(const GraphDev &g, LabView lab, const BMView &m, const int32_t *sof,
                   const uint8_t *coin, RngKey key, PhaseRef phase, int64_t lo, int64_t BS,
                   const int32_t *const lists[NCLS], const int32_t counts[NCLS], uint32_t big_lds,
                   int32_t *prop, double *dS, float *err, uint8_t *near, bool exact, bool mh,
-                  double beta, const NodalPre &pre, hipStream_t st);
+                  double beta, const NodalPre &pre, int stages, hipStream_t st);
 void launch_mix_stats(const float *err, const uint8_t *near, int64_t n, uint32_t *max_bits,
                       unsigned long long *count, hipStream_t st);
 // Labels of a nodal batch (greedy, all = false: where dS < 0; Metropolis-Hastings, all = true:

@@ -492,7 +492,7 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   c->res.ensure(8 * 64);
   c->ctr.ensure(8);
   c->mk0.ensure(8 * N); c->mk1.ensure(8 * N); c->mv0.ensure(4 * N); c->mv1.ensure(4 * N);
-  c->errv.ensure(4 * N); c->evcls.ensure(N); c->mixstat.ensure(32);
+  c->errv.ensure(4 * N); c->evcls.ensure(N); c->mixstat.ensure(48);
   c->olab.ensure(4 * N);
   c->dlist.ensure(4 * (size_t)NCLS * N);
   c->dcnt.ensure(4 * NCLS);
@@ -741,6 +741,8 @@ NodalPre nodal_pre(gsap_ctx *c) {
   p.dlist = c->dlist.as<int32_t>();
   p.dcnt = c->dcnt.as<int32_t>();
   p.bounds = c->lists_bounds;
+  // counted in profiled runs only
+  if (c->prof.on) p.evstat = reinterpret_cast<unsigned long long *>(c->mixstat.as<uint32_t>() + 8);
   return p;
 }
 
@@ -753,7 +755,6 @@ void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, PhaseRef phase, 
   hipStream_t st = c->st;
   BMView m = c->bm.view();
   if (!terms_queued) batch_terms(c, o, phase, ar);
-  int p = c->prof.begin(0, st);
   const int32_t *lists[NCLS];
   int32_t counts[NCLS];
   for (int k = 0; k < NCLS; ++k) {
@@ -767,10 +768,17 @@ void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, PhaseRef phase, 
   uint8_t *evcls = mixed ? c->evcls.as<uint8_t>() : nullptr;
   // the labels as the last rebuild saw them (it made the 16-bit copy when B <= 65535)
   const LabView lv{c->lab.as<int32_t>(), c->bm.B <= 65535 ? c->rs.lab16 : nullptr};
-  launch_nodal(c->g, lv, m, c->sof.as<int32_t>(), c->coin.as<uint8_t>(),
-               rng_key(o), phase, lo, BS, lists, counts, c->big_lds[b], c->prop.as<int32_t>(),
-               c->dS.as<double>(), err, evcls, o.delta_mode == GSAP_DELTA_EXACT,
-               o.accept == GSAP_ACCEPT_MH, o.beta, nodal_pre(c), st);
+  const NodalPre pre = nodal_pre(c);
+  int p = -1;
+  for (int stage : {NODAL_PROPOSE, NODAL_MOVES}) {
+    if (stage == NODAL_PROPOSE && c->g.wmax != 1) continue;  // unit-weight graphs only
+    p = c->prof.begin(stage == NODAL_PROPOSE ? 6 : 0, st);
+    launch_nodal(c->g, lv, m, c->sof.as<int32_t>(), c->coin.as<uint8_t>(),
+                 rng_key(o), phase, lo, BS, lists, counts, c->big_lds[b], c->prop.as<int32_t>(),
+                 c->dS.as<double>(), err, evcls, o.delta_mode == GSAP_DELTA_EXACT,
+                 o.accept == GSAP_ACCEPT_MH, o.beta, pre, stage, st);
+    if (stage == NODAL_PROPOSE) c->prof.end(p, st);
+  }
   if (mixed) {
     uint32_t *ms = c->mixstat.as<uint32_t>();
     launch_mix_stats(err, evcls, BS, ms, reinterpret_cast<unsigned long long *>(ms + 2), st);
@@ -889,7 +897,7 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
   int32_t *lab = c->lab.as<int32_t>();
   double *res = c->res.as<double>();
   launch_iota(lab, N, st);
-  HIPX(hipMemsetAsync(c->mixstat.p, 0, 32, st));
+  HIPX(hipMemsetAsync(c->mixstat.p, 0, 48, st));
   uint32_t *ctr = c->ctr.as<uint32_t>();
   HIPX(hipMemsetAsync(ctr, 0, 4, st));
   int32_t B = (int32_t)N;
@@ -1038,8 +1046,8 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
     if (o.max_levels > 0 && levels >= o.max_levels) break;
   }
   HIPX(hipMemcpyAsync(out, lab, 4 * (size_t)N, hipMemcpyDeviceToHost, st));
-  uint32_t mix[8] = {0};
-  HIPX(hipMemcpyAsync(mix, c->mixstat.p, 32, hipMemcpyDeviceToHost, st));
+  uint32_t mix[12] = {0};
+  HIPX(hipMemcpyAsync(mix, c->mixstat.p, 48, hipMemcpyDeviceToHost, st));
   HIPX(hipStreamSynchronize(st));
   c->prof.flush();
   if (rep) {
@@ -1059,6 +1067,11 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
     rep->max_abs_err_fp32 = (double)mx;
     rep->rechecks = (int64_t)cnt[0];
     rep->fp64_evals = (int64_t)cnt[1];
+    uint64_t ev[2];
+    std::memcpy(ev, &mix[8], 16);
+    const bool counted = c->prof.on && c->g.wmax == 1;
+    rep->nodal_evals = counted ? (int64_t)ev[0] : -1;
+    rep->nodal_eval_adj = counted ? (int64_t)ev[1] : -1;
   }
 }
 

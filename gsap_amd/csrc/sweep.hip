@@ -249,6 +249,9 @@ struct NodalArgs {
   int32_t *dlist;
   int32_t *dcnt;
   int32_t cb16, cb32, cbwave;
+  // with pre, or null: running totals of the vertices listed for evaluation and of their
+  // out+in adjacency entries
+  unsigned long long *evstat;
 };
 
 // Variants of the nodal kernels (template bits): the exact delta-S and Metropolis-Hastings
@@ -1265,6 +1268,19 @@ __global__ __launch_bounds__(256) void k_nodal_propose(NodalArgs a) {
       lp = atomicAdd(&wc[cls], 1u);
     }
   }
+  if (a.evstat) {
+    uint32_t nv = cls >= 0, ne = 0;
+    if (cls >= 0) ne = (uint32_t)((a.optr[v + 1] - a.optr[v]) + (a.iptr[v + 1] - a.iptr[v]));
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      nv += __shfl_xor(nv, off, 64);
+      ne += __shfl_xor(ne, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && nv) {
+      atomicAdd(&a.evstat[0], (unsigned long long)nv);
+      atomicAdd(&a.evstat[1], (unsigned long long)ne);
+    }
+  }
   __syncthreads();
   if (threadIdx.x < NCLS)
     wb[threadIdx.x] = wc[threadIdx.x] ? atomicAdd((uint32_t *)&a.dcnt[threadIdx.x],
@@ -1341,7 +1357,7 @@ void launch_nodal(const GraphDev &g, LabView lab, const BMView &m, const int32_t
                   const uint8_t *coin, RngKey key, PhaseRef phase, int64_t lo, int64_t BS,
                   const int32_t *const lists[NCLS], const int32_t counts[NCLS], uint32_t big_lds,
                   int32_t *prop, double *dS, float *err, uint8_t *near, bool exact, bool mh,
-                  double beta, const NodalPre &pre, hipStream_t st) {
+                  double beta, const NodalPre &pre, int stages, hipStream_t st) {
   if (BS <= 0) return;
   NodalArgs a;
   a.lo = lo; a.BS = BS;
@@ -1361,8 +1377,10 @@ void launch_nodal(const GraphDev &g, LabView lab, const BMView &m, const int32_t
   a.cb16 = pre.bounds.max16;
   a.cb32 = pre.bounds.max32;
   a.cbwave = pre.bounds.maxwave;
-  if (a.pre)
+  a.evstat = pre.evstat;
+  if (a.pre && (stages & NODAL_PROPOSE))
     k_nodal_propose<<<(unsigned)((BS + 255) / 256), 256, 0, st>>>(a);
+  if (!(stages & NODAL_MOVES)) return;
   NodalLists L;
   const int per_block[NCLS] = {16, 8, 4, 1};
   int64_t total = 0;

@@ -144,10 +144,14 @@ typedef struct gsap_report {
   int64_t n_trace;        /* trace records produced (may exceed capacity) */
   double kernel_ms[8];    /* device time per kernel class when options.profile = 1:     */
   int64_t kernel_calls[8];/* 0 nodal moves, 1 block-matrix rebuild, 2 per-block terms,   */
-                          /* 3 merge proposals+delta-S, 4 MDL, 5 accept+batch sum        */
+                          /* 3 merge proposals+delta-S, 4 MDL, 5 accept+batch sum,       */
+                          /* 6 nodal proposal pass (unit-weight graphs)                  */
   int64_t fp64_evals;     /* mixed form: nodal proposals evaluated in FP64 (rechecks plus
                              every accepted move, whose exact delta-S enters the batch sum) */
   double t_upload;        /* graph validation, host preparation and upload */
+  int64_t nodal_evals;    /* options.profile = 1 on a unit-weight graph: nodal proposals of
+                             another block (the moves kernel evaluates these), else -1 */
+  int64_t nodal_eval_adj; /* their out+in adjacency entries, summed over batches, else -1 */
 } gsap_report;
 
 /* One trace record; identical layout to the oracle's.
