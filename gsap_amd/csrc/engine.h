@@ -151,11 +151,14 @@ inline int64_t big_words(int64_t no, int64_t ni) {
 }
 constexpr int64_t BIG_LDS_CAP = 40 * 1024;
 // Unit-weight graphs: the proposal pass lists the vertices that propose another block by
-// class (dlist: NCLS x BS words) and counts them in dcnt (NCLS words, cleared by the batch's
+// class (dlist: NCLS x BS records) and counts them in dcnt (NCLS words, cleared by the batch's
 // block terms before it).  dlist = null: no proposal pass.
-// evstat, or null: two running totals (vertices evaluated, their adjacency entries).
+// evstat, or null: running totals (vertices evaluated, their adjacency entries) in EV_SLOTS
+// pairs, which the host sums.
+constexpr int EV_SLOTS = 32;
 struct NodalPre {
-  int32_t *dlist = nullptr, *dcnt = nullptr;
+  int4 *dlist = nullptr;
+  int32_t *dcnt = nullptr;
   unsigned long long *evstat = nullptr;
   NodalBounds bounds;
 };

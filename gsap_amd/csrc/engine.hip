@@ -199,7 +199,7 @@ struct gsap_ctx {
   // device batch counter (Philox phase of the sweep's first batch); merge-order sort
   DBuf ctr, mk0, mk1, mv0, mv1;
   // mixed form: per-vertex |dS32 - dS64| and evaluation class; batch statistics
-  DBuf errv, evcls, mixstat;
+  DBuf errv, evcls, mixstat, evstat;
   // the current batch's labels before it was applied (incremental dense rebuild)
   DBuf olab;
   // the nodal proposal pass's class lists and their counters (NodalPre)
@@ -492,9 +492,10 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   c->res.ensure(8 * 64);
   c->ctr.ensure(8);
   c->mk0.ensure(8 * N); c->mk1.ensure(8 * N); c->mv0.ensure(4 * N); c->mv1.ensure(4 * N);
-  c->errv.ensure(4 * N); c->evcls.ensure(N); c->mixstat.ensure(48);
+  c->errv.ensure(4 * N); c->evcls.ensure(N); c->mixstat.ensure(32);
+  c->evstat.ensure(16 * EV_SLOTS);
   c->olab.ensure(4 * N);
-  c->dlist.ensure(4 * (size_t)NCLS * N);
+  c->dlist.ensure(16 * (size_t)NCLS * N);
   c->dcnt.ensure(4 * NCLS);
   BMBuf &b = c->bm;
   b.rowptr = c->rowptr.as<int32_t>(); b.colptr = c->colptr.as<int32_t>();
@@ -738,11 +739,11 @@ void batch_terms(gsap_ctx *c, const gsap_options &o, PhaseRef phase, int32_t ar)
 // The proposal pass's lists and counters (the counters cleared by the batch's block terms)
 NodalPre nodal_pre(gsap_ctx *c) {
   NodalPre p;
-  p.dlist = c->dlist.as<int32_t>();
+  p.dlist = c->dlist.as<int4>();
   p.dcnt = c->dcnt.as<int32_t>();
   p.bounds = c->lists_bounds;
   // counted in profiled runs only
-  if (c->prof.on) p.evstat = reinterpret_cast<unsigned long long *>(c->mixstat.as<uint32_t>() + 8);
+  if (c->prof.on) p.evstat = c->evstat.as<unsigned long long>();
   return p;
 }
 
@@ -897,7 +898,8 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
   int32_t *lab = c->lab.as<int32_t>();
   double *res = c->res.as<double>();
   launch_iota(lab, N, st);
-  HIPX(hipMemsetAsync(c->mixstat.p, 0, 48, st));
+  HIPX(hipMemsetAsync(c->mixstat.p, 0, 32, st));
+  HIPX(hipMemsetAsync(c->evstat.p, 0, 16 * EV_SLOTS, st));
   uint32_t *ctr = c->ctr.as<uint32_t>();
   HIPX(hipMemsetAsync(ctr, 0, 4, st));
   int32_t B = (int32_t)N;
@@ -1046,8 +1048,10 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
     if (o.max_levels > 0 && levels >= o.max_levels) break;
   }
   HIPX(hipMemcpyAsync(out, lab, 4 * (size_t)N, hipMemcpyDeviceToHost, st));
-  uint32_t mix[12] = {0};
-  HIPX(hipMemcpyAsync(mix, c->mixstat.p, 48, hipMemcpyDeviceToHost, st));
+  uint32_t mix[8] = {0};
+  HIPX(hipMemcpyAsync(mix, c->mixstat.p, 32, hipMemcpyDeviceToHost, st));
+  uint64_t ev[2 * EV_SLOTS] = {0};
+  HIPX(hipMemcpyAsync(ev, c->evstat.p, sizeof(ev), hipMemcpyDeviceToHost, st));
   HIPX(hipStreamSynchronize(st));
   c->prof.flush();
   if (rep) {
@@ -1067,11 +1071,14 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
     rep->max_abs_err_fp32 = (double)mx;
     rep->rechecks = (int64_t)cnt[0];
     rep->fp64_evals = (int64_t)cnt[1];
-    uint64_t ev[2];
-    std::memcpy(ev, &mix[8], 16);
+    int64_t evals = 0, adj = 0;
+    for (int k = 0; k < EV_SLOTS; ++k) {
+      evals += (int64_t)ev[2 * k];
+      adj += (int64_t)ev[2 * k + 1];
+    }
     const bool counted = c->prof.on && c->g.wmax == 1;
-    rep->nodal_evals = counted ? (int64_t)ev[0] : -1;
-    rep->nodal_eval_adj = counted ? (int64_t)ev[1] : -1;
+    rep->nodal_evals = counted ? evals : -1;
+    rep->nodal_eval_adj = counted ? adj : -1;
   }
 }
 
@@ -1212,7 +1219,7 @@ int gsap_destroy(gsap_ctx *c) {
                  &c->lists, &c->ncl, &c->cntc, &c->vrel, &c->sseg, &c->lab16, &c->vlist,
                  &c->OUT, &c->sof, &c->coin, &c->prop, &c->dS, &c->best, &c->dSbest, &c->map,
                  &c->tiles, &c->res, &c->ctr, &c->mk0, &c->mk1, &c->mv0,
-                 &c->mv1, &c->errv, &c->evcls, &c->mixstat, &c->olab};
+                 &c->mv1, &c->errv, &c->evcls, &c->mixstat, &c->evstat, &c->olab};
   for (DBuf *b : all) b->release();
   try { c->prof.destroy(); } catch (...) {}
   c->stage.release();

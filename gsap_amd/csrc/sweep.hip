@@ -245,12 +245,14 @@ struct NodalArgs {
   double beta;           // Metropolis-Hastings inverse temperature
   int32_t pre;           // 1: prop already holds every proposal (k_nodal_propose)
   // with pre: the vertices whose proposal is another block, listed by class by the proposal
-  // kernel (class c at dlist + c * BS, dcnt[c] of them), and the class bounds
-  int32_t *dlist;
+  // kernel (class c at dlist + c * BS, dcnt[c] of them) as records {v, optr[v], iptr[v],
+  // out-degree | in-degree << 16} (degrees packed below the largest class, 0 there), and
+  // the class bounds
+  int4 *dlist;
   int32_t *dcnt;
   int32_t cb16, cb32, cbwave;
   // with pre, or null: running totals of the vertices listed for evaluation and of their
-  // out+in adjacency entries
+  // out+in adjacency entries, in EV_SLOTS pairs (summed by the host)
   unsigned long long *evstat;
 };
 
@@ -260,6 +262,20 @@ constexpr int VAR_EXACT = 1, VAR_MH = 2;
 
 GSAP_D uint32_t nodal_phase(const NodalArgs &a) {
   return (a.phase.dev ? *a.phase.dev : 0u) + a.phase.add;
+}
+
+// A vertex's adjacency ranges: from its proposal record (unit weights, classes below the
+// largest) or from the row pointers
+struct VHead {
+  int64_t v;
+  int32_t o0, no, i0, ni;
+};
+GSAP_D VHead vhead_rec(const int4 R) {
+  return VHead{(int64_t)R.x, R.y, R.w & 0xFFFF, R.z, (int32_t)((uint32_t)R.w >> 16)};
+}
+GSAP_D VHead vhead_load(const NodalArgs &a, int64_t v) {
+  const int32_t o0 = a.optr[v], i0 = a.iptr[v];
+  return VHead{v, o0, a.optr[v + 1] - o0, i0, a.iptr[v + 1] - i0};
 }
 
 constexpr float U32 = 5.9604645e-08f;  // 2^-24
@@ -357,8 +373,11 @@ GSAP_D double hast_log2(double fo, double fi, double bo, double bi, int32_t n) {
 // the untouched entries contribute (log2 d_old - log2 d_new) * (their summed count).
 // Labels are not written here (batch-Jacobi).  Hash tables of 2^lbits slots live in LDS.
 
+// unit: the head comes from a proposal record (unit weights: weights 1, degrees = counts)
 template <bool MIX, int VAR>
-GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbits) {
+GSAP_D void nodal_wave(const NodalArgs &a, const VHead hd, bool unit, int32_t *lds,
+                       uint32_t lbits) {
+  const int64_t v = hd.v;
   const uint32_t lcap = 1u << lbits;
   int32_t *lko = lds, *lwo = lds + lcap, *lfo = lds + 2 * lcap;
   int32_t *lki = lds + 3 * lcap, *lwi = lds + 4 * lcap, *lfi = lds + 5 * lcap;
@@ -371,19 +390,19 @@ GSAP_D void nodal_wave(const NodalArgs &a, int64_t v, int32_t *lds, uint32_t lbi
   Philox x{0u, 0u, 0u, 0u};
   if (!a.pre || MHV)
     x = philox((uint32_t)v, nodal_phase(a), a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
-  const int32_t o0 = a.optr[v], no = a.optr[v + 1] - o0;
-  const int32_t i0 = a.iptr[v], ni = a.iptr[v + 1] - i0;
+  const int32_t o0 = hd.o0, no = hd.no;
+  const int32_t i0 = hd.i0, ni = hd.ni;
   // weights, degrees and block-matrix entries stay below 2^30 (the host rejects larger total
   // edge weights), so their sums and differences here fit int32 (fewer registers)
-  const int32_t kout = a.kout[v], kin = a.kin[v];
+  const int32_t kout = unit ? no : a.kout[v], kin = unit ? ni : a.kin[v];
   const int32_t tot = kout + kin;
   // first chunk of both adjacency lists and its neighbour blocks, and the degrees of r:
   // gathered before the proposal, which they do not depend on, so that their loads overlap
   // its chain (when the proposal is r itself they go unused)
   const int32_t u_o0 = lane < no ? a.oadj[o0 + lane] : -1;
   const int32_t u_i0 = lane < ni ? a.iadj[i0 + lane] : -1;
-  const int32_t w_o0 = lane < no ? a.ow[o0 + lane] : 0;
-  const int32_t w_i0 = lane < ni ? a.iw[i0 + lane] : 0;
+  const int32_t w_o0 = lane < no ? (unit ? 1 : a.ow[o0 + lane]) : 0;
+  const int32_t w_i0 = lane < ni ? (unit ? 1 : a.iw[i0 + lane]) : 0;
   const int32_t t_o0 = u_o0 >= 0 ? a.lab[u_o0] : -1;
   const int32_t t_i0 = u_i0 >= 0 ? a.lab[u_i0] : -1;
   const int32_t dout_r = m.dout[r], din_r = m.din[r];
@@ -930,14 +949,18 @@ GSAP_D T group_allsum(T x) {
 // per vertex: lane q holds out-entry q and in-entry q; the neighbour blocks are deduplicated
 // in registers (shuffles inside the group) instead of hash tables, and the block of the
 // sampled neighbour comes from the same gathered values.
+// (rec: the proposal kernel's records, unit weights; else list)
 template <int G, bool MIX, int VAR>
-GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int32_t bid) {
+GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, const int4 *rec, int32_t n,
+                        int32_t bid) {
   const int32_t gid = (int32_t)(bid * (256 / G) + threadIdx.x / G);
   if (gid >= n) return;
   const int q = threadIdx.x & (G - 1);
   const int gbase = (threadIdx.x & 63) & ~(G - 1);
   const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
-  const int64_t v = list[gid];
+  const bool unit = rec != nullptr;
+  const VHead hd = unit ? vhead_rec(rec[gid]) : vhead_load(a, list[gid]);
+  const int64_t v = hd.v;
   const int64_t i = v - a.lo;
   const BMView &m = a.m;
   const int32_t B = m.B;
@@ -946,13 +969,13 @@ GSAP_D void nodal_small(const NodalArgs &a, const int32_t *list, int32_t n, int3
   Philox x{0u, 0u, 0u, 0u};
   if (!a.pre || MH)
     x = philox((uint32_t)v, nodal_phase(a), a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
-  const int32_t o0 = a.optr[v], no = a.optr[v + 1] - o0;
-  const int32_t i0 = a.iptr[v], ni = a.iptr[v + 1] - i0;
-  const int32_t kout = a.kout[v], kin = a.kin[v];
+  const int32_t o0 = hd.o0, no = hd.no;
+  const int32_t i0 = hd.i0, ni = hd.ni;
+  const int32_t kout = unit ? no : a.kout[v], kin = unit ? ni : a.kin[v];
   const int32_t tot = kout + kin;
   int32_t t_o = -1, w_o = 0, t_i = -1, w_i = 0, u_o = -1, u_i = -1;
-  if (q < no) { w_o = a.ow[o0 + q]; u_o = a.oadj[o0 + q]; t_o = a.lab[u_o]; }
-  if (q < ni) { w_i = a.iw[i0 + q]; u_i = a.iadj[i0 + q]; t_i = a.lab[u_i]; }
+  if (q < no) { w_o = unit ? 1 : a.ow[o0 + q]; u_o = a.oadj[o0 + q]; t_o = a.lab[u_o]; }
+  if (q < ni) { w_i = unit ? 1 : a.iw[i0 + q]; u_i = a.iadj[i0 + q]; t_i = a.lab[u_i]; }
   // proposal: first entry of (out list, in list) whose running weight exceeds the target
   int32_t s;
   if (a.pre) {
@@ -1236,12 +1259,14 @@ __global__ __launch_bounds__(256) void k_nodal_propose(NodalArgs a) {
   int cls = -1;
   uint32_t lp = 0;
   int64_t v = 0;
+  int32_t rec_o0 = 0, rec_i0 = 0, rec_no = 0, rec_ni = 0;
   if (i < a.BS) {
     v = a.lo + i;
     const int32_t B = a.m.B;
     const int32_t r = a.lab[v];
     const int32_t o0 = a.optr[v], no = a.optr[v + 1] - o0;
     const int32_t i0 = a.iptr[v], ni = a.iptr[v + 1] - i0;
+    rec_o0 = o0; rec_i0 = i0; rec_no = no; rec_ni = ni;
     const Philox x =
         philox((uint32_t)v, nodal_phase(a), a.key.chain, TAG_NODE, a.key.k0, a.key.k1);
     const uint32_t tot = (uint32_t)(no + ni);
@@ -1269,29 +1294,34 @@ __global__ __launch_bounds__(256) void k_nodal_propose(NodalArgs a) {
     }
   }
   if (a.evstat) {
-    uint32_t nv = cls >= 0, ne = 0;
-    if (cls >= 0) ne = (uint32_t)((a.optr[v + 1] - a.optr[v]) + (a.iptr[v + 1] - a.iptr[v]));
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      nv += __shfl_xor(nv, off, 64);
-      ne += __shfl_xor(ne, off, 64);
+    // workgroup totals in LDS, then one pair of global atomics per workgroup, spread over
+    // EV_SLOTS slot pairs
+    __shared__ uint32_t ev[2];
+    if (threadIdx.x < 2) ev[threadIdx.x] = 0;
+    __syncthreads();
+    if (cls >= 0) {
+      atomicAdd(&ev[0], 1u);
+      atomicAdd(&ev[1], (uint32_t)((a.optr[v + 1] - a.optr[v]) + (a.iptr[v + 1] - a.iptr[v])));
     }
-    if ((threadIdx.x & 63) == 0 && nv) {
-      atomicAdd(&a.evstat[0], (unsigned long long)nv);
-      atomicAdd(&a.evstat[1], (unsigned long long)ne);
-    }
+    __syncthreads();
+    if (threadIdx.x < 2 && ev[threadIdx.x])
+      atomicAdd(&a.evstat[2 * (blockIdx.x % EV_SLOTS) + threadIdx.x],
+                (unsigned long long)ev[threadIdx.x]);
   }
   __syncthreads();
   if (threadIdx.x < NCLS)
     wb[threadIdx.x] = wc[threadIdx.x] ? atomicAdd((uint32_t *)&a.dcnt[threadIdx.x],
                                                   wc[threadIdx.x]) : 0u;
   __syncthreads();
-  if (cls >= 0) a.dlist[(int64_t)cls * a.BS + wb[cls] + lp] = (int32_t)v;
+  if (cls >= 0)
+    a.dlist[(int64_t)cls * a.BS + wb[cls] + lp] =
+        make_int4((int32_t)v, rec_o0, rec_i0, cls < NCLS - 1 ? (rec_no | rec_ni << 16) : 0);
 }
 
 // Vertex lists of one batch by class, and the workgroups of each class in the fused launch
 struct NodalLists {
-  const int32_t *list[NCLS];
+  const int32_t *list[NCLS];  // host-built lists (vertex ids), or
+  const int4 *rec[NCLS];      // the proposal kernel's records (NodalArgs::dlist), or null
   int32_t n[NCLS];
   int32_t nb[NCLS];
   uint32_t big_lds;
@@ -1308,7 +1338,8 @@ GSAP_D void nodal_fused(const NodalArgs &a, const NodalLists &L, double *lds_d) 
   if (b < L.nb[3]) {
     if (b >= cnt(3)) return;
     log2_tables_to_lds();
-    nodal_big<MIX, VAR>(a, L.list[3][b], lds_d, L.big_lds);
+    nodal_big<MIX, VAR>(a, L.rec[3] ? (int64_t)L.rec[3][b].x : (int64_t)L.list[3][b], lds_d,
+                        L.big_lds);
     return;
   }
   b -= L.nb[3];
@@ -1318,7 +1349,8 @@ GSAP_D void nodal_fused(const NodalArgs &a, const NodalLists &L, double *lds_d) 
     log2_tables_to_lds();
     const int32_t k = b * 4 + (int32_t)(threadIdx.x >> 6);
     if (k < n2)
-      nodal_wave<MIX, VAR>(a, L.list[2][k],
+      nodal_wave<MIX, VAR>(a, L.rec[2] ? vhead_rec(L.rec[2][k]) : vhead_load(a, L.list[2][k]),
+                           L.rec[2] != nullptr,
                            reinterpret_cast<int32_t *>(lds_d) + (threadIdx.x >> 6) * 6 * LCAP,
                            (uint32_t)LBITS);
     return;
@@ -1328,14 +1360,14 @@ GSAP_D void nodal_fused(const NodalArgs &a, const NodalLists &L, double *lds_d) 
     const int32_t n1 = cnt(1);
     if (b * 8 >= n1) return;
     log2_tables_to_lds();
-    nodal_small<32, MIX, VAR>(a, L.list[1], n1, b);
+    nodal_small<32, MIX, VAR>(a, L.list[1], L.rec[1], n1, b);
     return;
   }
   b -= L.nb[1];
   const int32_t n0 = cnt(0);
   if (b * 16 >= n0) return;
   log2_tables_to_lds();
-  nodal_small<16, MIX, VAR>(a, L.list[0], n0, b);
+  nodal_small<16, MIX, VAR>(a, L.list[0], L.rec[0], n0, b);
 }
 
 template <bool MIX, int VAR>
@@ -1385,7 +1417,8 @@ void launch_nodal(const GraphDev &g, LabView lab, const BMView &m, const int32_t
   const int per_block[NCLS] = {16, 8, 4, 1};
   int64_t total = 0;
   for (int k = 0; k < NCLS; ++k) {
-    L.list[k] = a.pre ? pre.dlist + (int64_t)k * BS : lists[k];
+    L.list[k] = lists[k];
+    L.rec[k] = a.pre ? pre.dlist + (int64_t)k * BS : nullptr;
     L.n[k] = counts[k];
     L.nb[k] = (counts[k] + per_block[k] - 1) / per_block[k];
     total += L.nb[k];
