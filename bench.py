@@ -234,6 +234,76 @@ def bench_multi(a, prec):
     print(json.dumps(out))
 
 
+def mixed_report():
+    """The mixed form against the plain FP64 form on every configuration: identical labels
+    and S, max |dS32 - dS64| over the proposals evaluated in FP64, the share of proposals
+    whose FP32 delta-S lay within its error bound (rechecks) and the wall times; for the
+    200k-vertex graph also per level (cumulative counters of partitions cut after k levels,
+    differenced)."""
+    F, M = gsap_amd.PRECISION_FP64, gsap_amd.PRECISION_MIXED
+    out = {"configs": {}}
+    ctx = gsap_amd.Context(0)
+
+    def one(name, graphs, chains=None):
+        res = {}
+        for prec in (F, M):
+            o = gsap_amd.Options(seed=1, precision=prec)
+            if name == "c4":
+                ctx.partition_batch(graphs, o)  # warm-up
+                t = time.perf_counter()
+                labs, reps = ctx.partition_batch(graphs, o)
+            else:
+                ctx.partition(graphs[0], o)
+                t = time.perf_counter()
+                lab, rep = ctx.partition(graphs[0], o)
+                labs, reps = [lab], [rep]
+            res[prec] = (time.perf_counter() - t, labs, reps)
+        (tf, lf, rf), (tm, lm, rm) = res[F], res[M]
+        props = sum(r.batches * (g.N // 4) for r, g in zip(rm, graphs))
+        d = {"fp64_s": tf, "mixed_s": tm, "mixed_over_fp64": tm / tf,
+             "identical": all(np.array_equal(a, b) for a, b in zip(lf, lm)) and
+             all(x.S == y.S for x, y in zip(rf, rm)),
+             "max_abs_err_fp32": max(r.max_abs_err_fp32 for r in rm),
+             "S": [r.S for r in rf][:4],
+             "rechecks": sum(r.rechecks for r in rm), "fp64_evals": sum(r.fp64_evals for r in rm),
+             "nodal_proposals": props}
+        d["recheck_share"] = d["rechecks"] / max(props, 1)
+        print(f"{name}: {json.dumps(d)}", file=sys.stderr, flush=True)
+        return d
+
+    for name in ("c1", "c2", "c3"):
+        out["configs"][name] = one(name, [workloads.graph(name)])
+    out["configs"]["c4"] = one("c4", workloads.c4_graphs())
+    out["configs"]["c5"] = one("c5", [workloads.graph("c5")])
+    # per level of the 200k-vertex graph
+    g = workloads.graph("c3")
+    prev = (0, 0, 0)
+    levels = []
+    k = 1
+    while True:
+        _, rep = ctx.partition(g, gsap_amd.Options(seed=1, precision=M, max_levels=k))
+        cur = (rep.batches, rep.rechecks, rep.fp64_evals)
+        nb = cur[0] - prev[0]
+        if nb > 0:
+            props = nb * (g.N // 4)
+            levels.append({"level": k - 1, "batches": nb,
+                           "recheck_share": (cur[1] - prev[1]) / props,
+                           "fp64_share": (cur[2] - prev[2]) / props})
+        prev = cur
+        if rep.levels < k:
+            break
+        k += 1
+    # the block count of each level from the FP64 trace
+    _, _ = ctx.partition(g, gsap_amd.Options(seed=1, trace=1))
+    gold = [t for t in ctx.trace() if t[0] == 1]
+    for lv in levels:
+        if lv["level"] < len(gold):
+            lv["B"] = gold[lv["level"]][2]
+    out["c3_levels"] = levels
+    ctx.close()
+    print(json.dumps(out))
+
+
 def tune(a, ctx, g, opts):
     """Thresholds measured on g (untimed) when --autotune is given, else none."""
     if not a.autotune:
@@ -272,7 +342,12 @@ def main():
                              "OUT_JSON"),
                     help="turn rocprofv3 CSVs into the profiles/ summary")
     ap.add_argument("--profile-command", default="")
+    ap.add_argument("--mixed-report", action="store_true",
+                    help="mixed against FP64 form on C1-C5: identity, FP32 error, rechecks")
     a = ap.parse_args()
+    if a.mixed_report:
+        mixed_report()
+        return
     if a.summarize_profile:
         summarize_profile(*a.summarize_profile, command=a.profile_command)
         return
