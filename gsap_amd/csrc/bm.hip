@@ -422,9 +422,8 @@ __global__ __launch_bounds__(256) void k_seg_reserve(LabView lab, const int32_t 
 
 // Edge-parallel: out-edge e goes to its source's slice of the source block's row segment,
 // keyed by the target's block; in-edge f to its target's slice of the target block's column
-// segment, keyed by the source's block; seg[pos] records the segment of every entry.  With
-// wb > 0 an entry is one word, key << wb | weight (ew untouched).  ow = iw = nullptr: unit
-// weights.
+// segment, keyed by the source's block.  With wb > 0 an entry is one word, key << wb | weight
+// (ew untouched).  ow = iw = nullptr: unit weights.
 __global__ __launch_bounds__(256) void k_seg_fill(LabView lab, const int32_t *esrc,
                                                   const int32_t *optr, const int32_t *oadj,
                                                   const int32_t *ow, const int32_t *idst,
@@ -432,7 +431,7 @@ __global__ __launch_bounds__(256) void k_seg_fill(LabView lab, const int32_t *es
                                                   const int32_t *iw, int64_t N, int64_t E,
                                                   int32_t B, const uint32_t *off,
                                                   const uint32_t *vrel, uint32_t *ekey,
-                                                  uint32_t *ew, uint32_t *seg, int wb) {
+                                                  uint32_t *ew, int wb) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < 2 * E;
        e += (int64_t)gridDim.x * blockDim.x) {
     if (e < E) {
@@ -446,7 +445,6 @@ __global__ __launch_bounds__(256) void k_seg_fill(LabView lab, const int32_t *es
         ekey[pos] = k;
         ew[pos] = w;
       }
-      seg[pos] = x;
     } else {
       const int64_t f = e - E;
       const int32_t v = idst[f];
@@ -459,7 +457,6 @@ __global__ __launch_bounds__(256) void k_seg_fill(LabView lab, const int32_t *es
         ekey[pos] = k;
         ew[pos] = w;
       }
-      seg[pos] = x;
     }
   }
 }
@@ -977,11 +974,23 @@ __global__ __launch_bounds__(512) void k_seg_c_write(const uint32_t *off, const 
   }
 }
 
+// The segment holding position 64 c of the segment array, for every chunk c (one thread per
+// segment writes the chunks that start inside it)
+__global__ __launch_bounds__(256) void k_seg_chunks(const uint32_t *off, int64_t nseg,
+                                                    uint32_t *first) {
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nseg;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t s0 = off[x], s1 = off[x + 1];
+    for (uint32_t c = (s0 + 63) >> 6; (c << 6) < s1; ++c) first[c] = (uint32_t)x;
+  }
+}
+
 // Row/column pointers from the scanned run counts, then entry-parallel: the runs of every
 // segment (except two-pass class C, written at their place already) copied from the
-// segment's start to its final position, four entries per thread.
+// segment's start to its final position, four entries per thread.  An entry's segment is
+// found from its chunk's first segment by a short forward walk over the segment offsets.
 __global__ __launch_bounds__(256) void k_seg_compact(const uint32_t *off, const uint32_t *ekey,
-                                                     const uint32_t *ew, const uint32_t *seg,
+                                                     const uint32_t *ew, const uint32_t *first,
                                                      const uint32_t *uoff, int32_t B,
                                                      int one_range, int64_t n_ent,
                                                      int32_t *rowptr, int32_t *col,
@@ -998,26 +1007,33 @@ __global__ __launch_bounds__(256) void k_seg_compact(const uint32_t *off, const 
     else { rowptr[B] = (int32_t)nnz; colptr[B] = (int32_t)nnz; }
   }
   for (int64_t e0 = tid; e0 < n_ent; e0 += 4 * stride) {
-    uint32_t x[4], key[4], w[4], k[4], cnt[4], p[4], n[4];
+    uint32_t x[4], key[4], w[4], s0[4];
     bool ok[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int64_t e = e0 + j * stride;
       ok[j] = e < n_ent;
       if (!ok[j]) continue;
-      x[j] = seg[e];
+      x[j] = first[e >> 6];
       key[j] = ekey[e];
       w[j] = ew[e];
-      const uint32_t s0 = off[x[j]];
-      n[j] = off[x[j] + 1] - s0;
-      k[j] = (uint32_t)e - s0;
-      cnt[j] = uoff[x[j] + 1] - uoff[x[j]];
-      p[j] = uoff[x[j]] - (x[j] < (uint32_t)B ? 0u : nnz);
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      if (!ok[j] || k[j] >= cnt[j] || (!one_range && seg_class(n[j], false) == CL_C)) continue;
-      const uint32_t q = p[j] + k[j];
+      if (!ok[j]) continue;
+      const uint32_t e = (uint32_t)(e0 + j * stride);
+      while (off[x[j] + 1] <= e) ++x[j];
+      s0[j] = off[x[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (!ok[j]) continue;
+      const uint32_t e = (uint32_t)(e0 + j * stride);
+      const uint32_t n = off[x[j] + 1] - s0[j];
+      const uint32_t k = e - s0[j];
+      const uint32_t cnt = uoff[x[j] + 1] - uoff[x[j]];
+      if (k >= cnt || (!one_range && seg_class(n, false) == CL_C)) continue;
+      const uint32_t q = uoff[x[j]] - (x[j] < (uint32_t)B ? 0u : nnz) + k;
       if (x[j] < (uint32_t)B) {
         col[q] = (int32_t)key[j];
         val[q] = (int32_t)w[j];
@@ -1080,7 +1096,7 @@ static void seg_path_t(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
   scan_exclusive(rs.soff, rs.soff, nseg, rs.soff + nseg, rs.tmp, st);
   k_seg_fill<<<grid_for(2 * E, T), T, 0, st>>>(lab, g.esrc, g.optr, g.oadj, unit ? nullptr : g.ow,
                                                g.idst, g.iptr, g.iadj, unit ? nullptr : g.iw, N, E,
-                                               B, rs.soff, rs.vrel, ekey, ew, rs.sseg, wb);
+                                               B, rs.soff, rs.vrel, ekey, ew, wb);
   const uint32_t reg_max = (uint32_t)rs.seg_reg_max, count_min = (uint32_t)rs.seg_count_min;
   k_seg_classify<<<grid_for(nseg, T, SEG_GRID), T, 0, st>>>(rs.soff, nseg, one ? 1 : 0, reg_max,
                                                             count_min, rs.lists, rs.lcap, rs.ncl);
@@ -1111,9 +1127,10 @@ static void seg_path_t(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
   k_seg_sort_reg<PK><<<SEG_GRID * 2, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl,
                                                    rs.su, wb);
   scan_exclusive(rs.su, rs.suoff, nseg, rs.suoff + nseg, rs.tmp, st);
+  k_seg_chunks<<<grid_for(nseg, T), T, 0, st>>>(rs.soff, nseg, rs.chunk_seg);
   k_seg_compact<<<grid_for((2 * E + 3) / 4, T), T, 0, st>>>(
-      rs.soff, ekey, ew, rs.sseg, rs.suoff, B, one ? 1 : 0, 2 * E, bm.rowptr, bm.col, bm.val,
-      bm.erow, bm.colptr, bm.row, bm.cval, D, gcol, grow);
+      rs.soff, ekey, ew, rs.chunk_seg, rs.suoff, B, one ? 1 : 0, 2 * E, bm.rowptr, bm.col,
+      bm.val, bm.erow, bm.colptr, bm.row, bm.cval, D, gcol, grow);
   if (!one)
     k_seg_c_write<PK><<<SEG_GRID / 2, 512, (size_t)RC * 4, st>>>(
         rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl, B, nr, rs.cntc, rs.suoff, bm.col, bm.val,

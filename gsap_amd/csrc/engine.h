@@ -66,7 +66,7 @@ struct RebuildScratch {
   uint32_t *soff = nullptr;           // 2 Bmax + 1: segment sizes, then offsets
   uint32_t *su = nullptr;             // 2 Bmax: runs per segment
   uint32_t *vrel = nullptr;           // 2N: each vertex's slice start inside its row / column segment
-  uint32_t *sseg = nullptr;           // 2E: segment of every segment entry
+  uint32_t *chunk_seg = nullptr;      // 2E / 64 + 1: segment holding each 64-entry chunk's start
   uint16_t *lab16 = nullptr;          // N: 16-bit labels when B <= 65535
   uint32_t *suoff = nullptr;          // 2 Bmax + 1: offsets of the runs
   uint32_t *lists = nullptr;          // SEG_NCL lists of lcap segment ids

@@ -193,7 +193,7 @@ struct gsap_ctx {
   // block matrix
   DBuf rowptr, colptr, col, val, erow, row, cval, dout, din, ldout, ldin, dense, gcol, grow;
   // rebuild scratch
-  DBuf k0, k1, tmp, partials, soff, su, suoff, lists, ncl, cntc, vrel, sseg, lab16;
+  DBuf k0, k1, tmp, partials, soff, su, suoff, lists, ncl, cntc, vrel, chunk_seg, lab16;
   // sweep / merge
   DBuf OUT, sof, coin, prop, dS, best, dSbest, map, tiles, res;
   // device batch counter (Philox phase of the sweep's first batch); merge-order sort
@@ -479,8 +479,8 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
       dB > 128 ? std::min<int64_t>(E / 2, (int64_t)32 << 20) : 0);
   c->partials.ensure(4 * pc);
   c->soff.ensure(4 * (2 * N + 1));
+  c->chunk_seg.ensure(4 * (2 * E / 64 + 2));
   c->su.ensure(4 * 2 * N); c->suoff.ensure(4 * (2 * N + 1)); c->vrel.ensure(4 * 2 * N);
-  c->sseg.ensure(8 * E);
   c->lab16.ensure(2 * N);
   c->lists.ensure(4 * (size_t)SEG_NCL * 2 * N); c->ncl.ensure(4 * 16);
   c->cntc.ensure(4 * (size_t)std::max<int64_t>(seg_cntc_words(N, E), 1));
@@ -510,7 +510,7 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   r.tmp_words = (int64_t)(c->tmp.cap / 4);
   r.partials = c->partials.as<int32_t>(); r.partials_cap = pc;
   r.soff = c->soff.as<uint32_t>();
-  r.su = c->su.as<uint32_t>(); r.suoff = c->suoff.as<uint32_t>(); r.vrel = c->vrel.as<uint32_t>(); r.sseg = c->sseg.as<uint32_t>(); r.lab16 = c->lab16.as<uint16_t>();
+  r.su = c->su.as<uint32_t>(); r.suoff = c->suoff.as<uint32_t>(); r.vrel = c->vrel.as<uint32_t>(); r.chunk_seg = c->chunk_seg.as<uint32_t>(); r.lab16 = c->lab16.as<uint16_t>();
   r.lists = c->lists.as<uint32_t>(); r.lcap = 2 * N; r.ncl = c->ncl.as<uint32_t>();
   r.cntc = c->cntc.as<uint32_t>();
 }
@@ -1216,7 +1216,7 @@ int gsap_destroy(gsap_ctx *c) {
                  &c->row, &c->cval, &c->dout, &c->din, &c->ldout, &c->ldin, &c->dense,
                  &c->gcol, &c->grow, &c->k0,
                  &c->k1, &c->tmp, &c->partials, &c->soff, &c->su, &c->suoff,
-                 &c->lists, &c->ncl, &c->cntc, &c->vrel, &c->sseg, &c->lab16, &c->vlist,
+                 &c->lists, &c->ncl, &c->cntc, &c->vrel, &c->chunk_seg, &c->lab16, &c->vlist,
                  &c->OUT, &c->sof, &c->coin, &c->prop, &c->dS, &c->best, &c->dSbest, &c->map,
                  &c->tiles, &c->res, &c->ctr, &c->mk0, &c->mk1, &c->mv0,
                  &c->mv1, &c->errv, &c->evcls, &c->mixstat, &c->evstat, &c->olab};
