@@ -1926,7 +1926,7 @@ void launch_merge(const BMView &m, const double *OUT, RngKey key, uint32_t phase
                   uint32_t *big_count, hipStream_t st) {
   const int64_t n = (int64_t)m.B * np;
   const int ex = exact ? 1 : 0;
-  if (avg_row > 24 && m.B >= 4096 && big_list) {
+  if (avg_row > 12 && m.B >= 4096 && big_list) {
     // long rows and enough blocks to fill the device: a workgroup per block with its row and
     // column in LDS; the few blocks beyond the staging arrays get a wave per proposal (their
     // count comes back to the host to size that launch)
