@@ -60,7 +60,7 @@ class COptions(C.Structure):
                 ("tile_max_B", C.c_int32), ("delta_mode", C.c_int32), ("accept", C.c_int32),
                 ("beta", C.c_double), ("nodal_max16", C.c_int32), ("nodal_max32", C.c_int32),
                 ("nodal_maxwave", C.c_int32), ("seg_reg_max", C.c_int32),
-                ("seg_count_min", C.c_int32)]
+                ("seg_count_min", C.c_int32), ("sparse_update", C.c_int32)]
 
 
 class CReport(C.Structure):
@@ -101,7 +101,7 @@ class CDcsbm(C.Structure):
 EXPORTS = [
     "gsap_options_default", "gsap_create", "gsap_destroy", "gsap_last_error",
     "gsap_partition_i32", "gsap_get_trace", "gsap_blockmodel_i32", "gsap_merge_phase_i32",
-    "gsap_nodal_batch_i32", "gsap_mdl_i32", "gsap_generate_dcsbm_i32", "gsap_read_tsv_i32",
+    "gsap_nodal_batch_i32", "gsap_nodal_batch_bm_i32", "gsap_mdl_i32", "gsap_generate_dcsbm_i32", "gsap_read_tsv_i32",
     "gsap_pairwise_i32", "gsap_log2", "gsap_log2_tables", "gsap_partition_batch_i32",
     "gsap_partition_chains_i32", "gsap_log2f_n", "gsap_quality_i32", "gsap_save_csr_i32",
     "gsap_load_csr_i32", "gsap_autotune_i32", "gsap_golden_init", "gsap_golden_step",
@@ -140,6 +140,9 @@ def lib() -> C.CDLL:
     L.gsap_blockmodel_i32.argtypes = [vp, C.POINTER(CGraph), _i32p, i32, _i64p, _i32p, _i64p,
                                       _i64p, _i32p, _i64p, _i64p, _i64p, C.POINTER(i64),
                                       C.POINTER(COptions)]
+    L.gsap_nodal_batch_bm_i32.argtypes = [vp, C.POINTER(CGraph), C.POINTER(COptions), _i32p,
+                                          i32, i32, u32, i32, _i64p, _i32p, _i64p, _i64p,
+                                          _i32p, _i64p, _i64p, _i64p, C.POINTER(i64)]
     L.gsap_merge_phase_i32.argtypes = [vp, C.POINTER(CGraph), C.POINTER(COptions), _i32p, i32,
                                        i32, u32, _i32p, _f64p, _i32p, _f64p, C.POINTER(i32)]
     L.gsap_nodal_batch_i32.argtypes = [vp, C.POINTER(CGraph), C.POINTER(COptions), _i32p, i32,
@@ -251,6 +254,9 @@ class Options:
     nodal_maxwave: int = 0
     seg_reg_max: int = 0
     seg_count_min: int = 0
+    # after a batch without the dense block matrix: 0 rebuilds from every edge, 1 merges the
+    # moves into the current CSR/CSC (identical results)
+    sparse_update: int = 0
 
     def cstruct(self) -> COptions:
         o = COptions()
@@ -259,7 +265,7 @@ class Options:
                   "chain", "trace", "dense_max_B", "profile", "model", "workers",
                   "profile_classes", "graph", "tile_max_B", "delta_mode", "accept", "beta",
                   "nodal_max16", "nodal_max32", "nodal_maxwave", "seg_reg_max",
-                  "seg_count_min"):
+                  "seg_count_min", "sparse_update"):
             setattr(o, f, getattr(self, f))
         return o
 
@@ -409,6 +415,26 @@ class Context:
         _check(lib().gsap_nodal_batch_i32(self._h, C.byref(cg), C.byref(co), lab, B, b, q, prop,
                                           dS, C.byref(dE)), self._h)
         return dict(labels=lab, prop=prop[:BS], dS=dS[:BS], dE=dE.value)
+
+    def nodal_batch_bm(self, g: Graph, opts: Options, labels: np.ndarray, B: int, b: int,
+                       q: int, keep: bool = False):
+        """Nodal batch b, then the block-matrix update the partition loop runs after it; the
+        labels and block matrix afterwards.  keep: continue from the labels and block matrix
+        the previous call left on this context (labels is then only the output's shape)."""
+        E = g.E
+        lab = np.ascontiguousarray(labels, dtype=np.int32).copy()
+        rowptr = np.zeros(B + 1, np.int64); colptr = np.zeros(B + 1, np.int64)
+        col = np.zeros(max(E, 1), np.int32); row = np.zeros(max(E, 1), np.int32)
+        val = np.zeros(max(E, 1), np.int64); cval = np.zeros(max(E, 1), np.int64)
+        dout = np.zeros(B, np.int64); din = np.zeros(B, np.int64)
+        nnz = C.c_int64()
+        cg, co = g.cstruct(), opts.cstruct()
+        _check(lib().gsap_nodal_batch_bm_i32(self._h, C.byref(cg), C.byref(co), lab, B, b, q,
+                                             1 if keep else 0, rowptr, col, val, colptr, row,
+                                             cval, dout, din, C.byref(nnz)), self._h)
+        k = nnz.value
+        return lab, dict(rowptr=rowptr, col=col[:k], val=val[:k], colptr=colptr, row=row[:k],
+                         cval=cval[:k], dout=dout, din=din, nnz=k)
 
     def mdl_noalloc(self, d_out_ptr: int, d_out_adj: int, d_out_w: int, d_labels: int, N: int,
                     E: int, B: int, d_S: int, stream: int = 0, model: int = 0) -> None:

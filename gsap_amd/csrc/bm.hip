@@ -1423,6 +1423,676 @@ bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t 
   return true;
 }
 
+// ---- sparse update of the block matrix after a nodal batch (B without the dense copy)
+// The batch's edge moves form a signed delta matrix dM.  For a moved vertex x (block xo
+// before the batch, xn after): every out-edge x -> y of weight w gives row xo key yo: -w,
+// row xn key yn: +w, column yo key xo: -w and column yn key xn: +w (yo, yn: y's labels
+// before and after); every in-edge y -> x from a vertex y that did not move gives row y' key
+// xo: -w, row y' key xn: +w, column xo key y': -w and column xn key y': +w (y': y's label).
+// An edge between two moved vertices is counted once, as an out-edge.  Segment x < B of dM
+// is row x and segment B + x column x, as in the segmented rebuild, whose sorts reduce each
+// segment to runs (key, summed weight; the sums wrap in 32 bits, so negative totals come out
+// in two's complement).  Each segment's runs D are then merged with the current row or
+// column O into the alternate buffers (an entry whose sum is zero dropped) by a team sized
+// to the segment: lane groups with a bitonic merge for up to 64 entries, a wave with the
+// keys in LDS up to MW_CAP, a wave over 64-rank chunks above.  The work follows the moved
+// vertices' edges plus one pass over the current entries, instead of sorting every edge
+// again.
+
+// nmv: the moved count followed by the merge classes' counts (8 words)
+__global__ void k_delta_zero(uint32_t *cnt, uint32_t *cur, uint32_t *su, uint32_t *ncl,
+                             uint32_t *nmv, int64_t nseg) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= nseg;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    cnt[i] = 0;
+    if (i < nseg) { cur[i] = 0; su[i] = 0; }
+  }
+  if (blockIdx.x == 0 && threadIdx.x < SEG_NCL) ncl[threadIdx.x] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < 8) nmv[threadIdx.x] = 0;
+}
+
+// the batch's moved vertices (index in the batch), listed in any order; *nmv their count
+__global__ __launch_bounds__(256) void k_moved_list(const int32_t *lab, const int32_t *olab,
+                                                    int64_t lo, int64_t BS, uint32_t *list,
+                                                    uint32_t *nmv) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < BS; i0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = i0 + threadIdx.x;
+    const bool mv = i < BS && olab[i] != lab[lo + i];
+    const uint64_t m = __ballot(mv);
+    uint32_t base = 0;
+    if (lane == 0 && m) base = atomicAdd(nmv, (uint32_t)__popcll(m));
+    base = __shfl(base, 0, 64);
+    if (mv) list[base + (uint32_t)__popcll(m & lt)] = (uint32_t)i;
+  }
+}
+
+// A group of 16 lanes per listed vertex walks its out-edges, then its in-edges.
+// FILL = false: every delta counted into cnt[segment]; FILL = true: written at off[segment]
+// plus a slot reserved in cur (a group reserves its chunk's entries of the vertex's own rows
+// and columns with one atomic each).
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_delta_gen(const int32_t *optr, const int32_t *oadj,
+                                                   const int32_t *ow, const int32_t *iptr,
+                                                   const int32_t *iadj, const int32_t *iw,
+                                                   const int32_t *lab, const int32_t *olab,
+                                                   int64_t lo, int64_t BS, const uint32_t *list,
+                                                   const uint32_t *nmv, int32_t B,
+                                                   uint32_t *cnt, const uint32_t *off,
+                                                   uint32_t *cur, uint32_t *ekey, uint32_t *ew) {
+  constexpr int G = 16;
+  const int q = threadIdx.x & (G - 1);
+  const int gbase = (threadIdx.x & 63) & ~(G - 1);
+  const uint32_t lt = (1u << q) - 1u;
+  const uint32_t Bu = (uint32_t)B;
+  const uint32_t n = *nmv;
+  const int64_t ng = (int64_t)gridDim.x * (blockDim.x / G);
+  for (int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; k < n; k += ng) {
+    const uint32_t i = list[k];
+    const int64_t x = lo + i;
+    const uint32_t vo = (uint32_t)olab[i], vn = (uint32_t)lab[x];
+    const int32_t o0 = optr[x], o1 = optr[x + 1], i0 = iptr[x], i1 = iptr[x + 1];
+    for (int32_t c = o0; c < o1; c += G) {
+      const int32_t j = c + q;
+      const bool on = j < o1;
+      const uint32_t nc = (uint32_t)min(G, o1 - c);
+      uint32_t ko = 0, kn = 0, w = 0;
+      if (on) {
+        const int32_t y = oadj[j];
+        w = ow ? (uint32_t)ow[j] : 1u;
+        const int64_t ky = (int64_t)y - lo;
+        kn = (uint32_t)lab[y];
+        ko = (ky >= 0 && ky < BS) ? (uint32_t)olab[ky] : kn;
+      }
+      if (!FILL) {
+        if (q == 0) {
+          atomicAdd(&cnt[vo], nc);
+          atomicAdd(&cnt[vn], nc);
+        }
+        if (on) {
+          if (ko == kn) {
+            atomicAdd(&cnt[Bu + ko], 2u);
+          } else {
+            atomicAdd(&cnt[Bu + ko], 1u);
+            atomicAdd(&cnt[Bu + kn], 1u);
+          }
+        }
+      } else {
+        uint32_t ro = 0, rn = 0;
+        if (q == 0) {
+          ro = off[vo] + atomicAdd(&cur[vo], nc);
+          rn = off[vn] + atomicAdd(&cur[vn], nc);
+        }
+        ro = __shfl(ro, 0, G) + (uint32_t)q;
+        rn = __shfl(rn, 0, G) + (uint32_t)q;
+        if (on) {
+          ekey[ro] = ko; ew[ro] = 0u - w;  // row vo, key yo: -w
+          ekey[rn] = kn; ew[rn] = w;       // row vn, key yn: +w
+          if (ko == kn) {
+            const uint32_t p = off[Bu + ko] + atomicAdd(&cur[Bu + ko], 2u);
+            ekey[p] = vo; ew[p] = 0u - w;
+            ekey[p + 1] = vn; ew[p + 1] = w;
+          } else {
+            const uint32_t p = off[Bu + ko] + atomicAdd(&cur[Bu + ko], 1u);
+            ekey[p] = vo; ew[p] = 0u - w;
+            const uint32_t r = off[Bu + kn] + atomicAdd(&cur[Bu + kn], 1u);
+            ekey[r] = vn; ew[r] = w;
+          }
+        }
+      }
+    }
+    for (int32_t c = i0; c < i1; c += G) {
+      const int32_t j = c + q;
+      bool on = j < i1;
+      uint32_t yl = 0, w = 0;
+      if (on) {
+        const int32_t y = iadj[j];  // edge y -> x
+        w = iw ? (uint32_t)iw[j] : 1u;
+        const int64_t ky = (int64_t)y - lo;
+        yl = (uint32_t)lab[y];
+        if (ky >= 0 && ky < BS && (uint32_t)olab[ky] != yl) on = false;  // y moved: its out-edge
+      }
+      const uint32_t m = (uint32_t)((__ballot(on) >> gbase) & 0xFFFFull);
+      const uint32_t nc = (uint32_t)__popc(m);
+      if (!FILL) {
+        if (q == 0 && nc) {
+          atomicAdd(&cnt[Bu + vo], nc);
+          atomicAdd(&cnt[Bu + vn], nc);
+        }
+        if (on) atomicAdd(&cnt[yl], 2u);
+      } else {
+        uint32_t ro = 0, rn = 0;
+        if (q == 0 && nc) {
+          ro = off[Bu + vo] + atomicAdd(&cur[Bu + vo], nc);
+          rn = off[Bu + vn] + atomicAdd(&cur[Bu + vn], nc);
+        }
+        const uint32_t r = (uint32_t)__popc(m & lt);
+        ro = __shfl(ro, 0, G) + r;
+        rn = __shfl(rn, 0, G) + r;
+        if (on) {
+          ekey[ro] = yl; ew[ro] = 0u - w;  // column vo, key y': -w
+          ekey[rn] = yl; ew[rn] = w;       // column vn, key y': +w
+          const uint32_t p = off[yl] + atomicAdd(&cur[yl], 2u);
+          ekey[p] = vo; ew[p] = 0u - w;    // row y', key vo: -w
+          ekey[p + 1] = vn; ew[p + 1] = w; // row y', key vn: +w
+        }
+      }
+    }
+  }
+}
+
+// Class C with B > SEG_RC (segments of more than SEG_G entries): the runs of each (segment,
+// column range), counted by k_seg_c_count, written at the segment's start in rk / rw (the
+// segment's own entries are re-read for every range, so they cannot be overwritten).
+template <bool PK>
+__global__ __launch_bounds__(512) void k_seg_c_runs(const uint32_t *off, const uint32_t *ekey,
+                                                    const uint32_t *ew, const uint32_t *lists,
+                                                    int64_t lcap, const uint32_t *ncl,
+                                                    int32_t B, int32_t nr,
+                                                    const uint32_t *cntc, uint32_t *rk,
+                                                    uint32_t *rw, int wb) {
+  extern __shared__ uint32_t cnt[];
+  __shared__ uint32_t red8[8];
+  const uint32_t RC = B < SEG_RC ? (uint32_t)B : (uint32_t)SEG_RC;
+  const int64_t items = (int64_t)ncl[CL_C] * nr;
+  for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const int64_t k = it / nr;
+    const uint32_t q = (uint32_t)(it - k * nr);
+    if (cntc[it] == 0) continue;  // uniform across the workgroup
+    const uint32_t x = lists[CL_C * lcap + k];
+    uint32_t pre = 0;
+    for (uint32_t qq = threadIdx.x; qq < q; qq += 512) pre += cntc[k * nr + qq];
+    pre = block_sum512(pre, red8);
+    const uint32_t c0 = q * RC, cn = min(RC, (uint32_t)B - c0);
+    seg_range_count<PK>(cnt, ekey, ew, off[x], off[x + 1], c0, cn, wb);
+    seg_emit(cnt, cn, c0, off[x] + pre, red8, rk, rw, nullptr, nullptr, nullptr, 0, nullptr);
+  }
+}
+
+// The current matrix (read) and the alternate buffers (written) of a sparse update
+struct BMOld {
+  const int32_t *rowptr, *col, *val, *colptr, *row, *cval;
+};
+struct BMNew {
+  int32_t *rowptr, *col, *val, *erow, *colptr, *row, *cval, *gcol, *grow;
+  int64_t cap;  // entries each of col / val / erow / row / cval hold
+};
+// One segment of a sparse update: its current entries O (keys ascending) and delta runs D
+// (keys ascending, signed weights)
+struct MSeg {
+  const int32_t *ok, *ov;
+  const uint32_t *dk, *dv;
+  int32_t m, d, y;
+  bool isrow;
+};
+GSAP_D MSeg mseg(const BMOld &o, const uint32_t *off, const uint32_t *su, const uint32_t *ekey,
+                 const uint32_t *ew, const uint32_t *rk, const uint32_t *rw, int one_range,
+                 int32_t B, uint32_t x) {
+  MSeg s;
+  s.isrow = x < (uint32_t)B;
+  s.y = (int32_t)(s.isrow ? x : x - B);
+  const int32_t o0 = s.isrow ? o.rowptr[s.y] : o.colptr[s.y];
+  s.m = (s.isrow ? o.rowptr[s.y + 1] : o.colptr[s.y + 1]) - o0;
+  s.ok = (s.isrow ? o.col : o.row) + o0;
+  s.ov = (s.isrow ? o.val : o.cval) + o0;
+  const uint32_t s0 = off[x], nd = off[x + 1] - s0;
+  s.d = nd ? (int32_t)su[x] : 0;
+  const bool apart = !one_range && nd > (uint32_t)SEG_G;  // runs of the column-range counts
+  s.dk = (apart ? rk : ekey) + s0;
+  s.dv = (apart ? rw : ew) + s0;
+  return s;
+}
+// output q of segment s (row or column entry, with its coarse-index word)
+GSAP_D void mput(const BMNew &n, const MSeg &s, uint32_t q, int32_t key, int32_t v) {
+  if ((int64_t)q >= n.cap) return;  // (a consistent matrix has at most E entries)
+  if (s.isrow) {
+    n.col[q] = key;
+    n.val[q] = v;
+    n.erow[q] = s.y;
+    if (n.gcol && (q & 15u) == 0) n.gcol[q >> 4] = key;
+  } else {
+    n.row[q] = key;
+    n.cval[q] = v;
+    if (n.grow && (q & 15u) == 0) n.grow[q >> 4] = key;
+  }
+}
+// first output position of segment x (rows, then columns from 0)
+GSAP_D uint32_t mout0(const uint32_t *uoff, int32_t B, uint32_t x) {
+  return x < (uint32_t)B ? uoff[x] : uoff[x] - uoff[B];
+}
+
+// merge classes by n = |O| + |D|: <= 16, 32, 64 (lane groups), <= MW_CAP (a wave with the
+// keys in LDS), longer (a wave over 64-rank chunks)
+enum { MC_16 = 0, MC_32 = 1, MC_64 = 2, MC_W = 3, MC_H = 4, MC_N = 5 };
+constexpr int MW_CAP = 1024;
+
+// Segments into class lists (empty ones get length 0 at once)
+__global__ __launch_bounds__(256) void k_merge_classify(BMOld o, const uint32_t *off,
+                                                        const uint32_t *su, int32_t B,
+                                                        uint32_t *lists, int64_t lcap,
+                                                        uint32_t *ncls, uint32_t *len) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int64_t nseg = 2 * (int64_t)B;
+  for (int64_t x0 = (int64_t)blockIdx.x * blockDim.x; x0 < nseg; x0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t x = x0 + threadIdx.x;
+    int c = -1;
+    if (x < nseg) {
+      const bool isrow = x < B;
+      const int32_t y = (int32_t)(isrow ? x : x - B);
+      const int32_t m = isrow ? o.rowptr[y + 1] - o.rowptr[y] : o.colptr[y + 1] - o.colptr[y];
+      const int32_t d = off[x + 1] != off[x] ? (int32_t)su[x] : 0;
+      const int32_t n = m + d;
+      if (n == 0) len[x] = 0;
+      else c = n <= 16 ? MC_16 : n <= 32 ? MC_32 : n <= 64 ? MC_64 : n <= MW_CAP ? MC_W : MC_H;
+    }
+#pragma unroll
+    for (int k = 0; k < MC_N; ++k) {
+      const uint64_t msk = __ballot(c == k);
+      if (!msk) continue;
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(&ncls[k], (uint32_t)__popcll(msk));
+      base = __shfl(base, 0, 64);
+      if (c == k) lists[k * lcap + base + (uint32_t)__popcll(msk & lt)] = (uint32_t)x;
+    }
+  }
+}
+
+// n <= G: a group of G lanes holds O ascending, padding, D descending (a bitonic sequence of
+// (key << 1 | from-D) words) and merges it; an O entry and the D run of its key end up
+// adjacent and are summed, zero values dropped, the rest compacted by ballot.
+template <int G, bool WRITE>
+GSAP_D void merge_group(const MSeg &s, uint32_t x, int32_t B, uint32_t *len, int32_t *dout,
+                        int32_t *din, const BMNew &nw) {
+  const int q = threadIdx.x & (G - 1);
+  const int gbase = (threadIdx.x & 63) & ~(G - 1);
+  const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+  uint32_t ck = 0xFFFFFFFFu;
+  int32_t v = 0;
+  if (q < s.m) {
+    ck = (uint32_t)s.ok[q] << 1;
+    v = s.ov[q];
+  } else if (q >= G - s.d) {
+    const int j = G - 1 - q;
+    ck = (s.dk[j] << 1) | 1u;
+    v = (int32_t)s.dv[j];
+  }
+  const int32_t dpart = (ck & 1u) && ck != 0xFFFFFFFFu ? v : 0;
+#pragma unroll
+  for (int j = G / 2; j > 0; j >>= 1) {
+    const uint32_t pk = __shfl_xor(ck, j, G);
+    const int32_t pv = __shfl_xor(v, j, G);
+    const bool lower = (q & j) == 0;
+    const bool take = lower ? pk < ck : pk > ck;
+    if (take) { ck = pk; v = pv; }
+  }
+  const uint32_t nk = __shfl_down(ck, 1, G);
+  const int32_t nv = __shfl_down(v, 1, G);
+  const uint32_t pk = __shfl_up(ck, 1, G);
+  const bool real = ck != 0xFFFFFFFFu;
+  const bool head = real && !(ck & 1u) && q + 1 < G && (nk >> 1) == (ck >> 1) && nk != 0xFFFFFFFFu;
+  const bool tail = real && (ck & 1u) && q > 0 && (pk >> 1) == (ck >> 1) && !(pk & 1u);
+  const int32_t outv = head ? v + nv : v;
+  const bool valid = real && !tail && outv != 0;
+  const uint64_t mk = (__ballot(valid) >> gbase) & gmask;
+  if (WRITE) {
+    if (valid) {
+      const uint32_t p = (uint32_t)__popcll(mk & ((1ull << q) - 1ull));
+      mput(nw, s, mout0(len, B, x) + p, (int32_t)(ck >> 1), outv);
+    }
+  } else {
+    int32_t ds = dpart;
+#pragma unroll
+    for (int o_ = G / 2; o_ >= 1; o_ >>= 1) ds += __shfl_xor(ds, o_, G);
+    if (q == 0) {
+      len[x] = (uint32_t)__popcll(mk);
+      if (ds != 0) {
+        if (s.isrow) dout[s.y] += ds;
+        else din[s.y] += ds;
+      }
+    }
+  }
+}
+
+// 64 < n <= MW_CAP: one wave, O and D keys staged in LDS (sk: O then D); each element's rank
+// in the merged sequence comes from a binary search in the other list (ties: O first, so an
+// O entry and its D run take consecutive ranks); sc[rank] = 1 for an output, scanned in place.
+template <bool WRITE>
+GSAP_D void merge_wave(const MSeg &s, uint32_t x, int32_t B, uint32_t *len, int32_t *dout,
+                       int32_t *din, const BMNew &nw, int32_t *sk, uint32_t *sc) {
+  const int lane = threadIdx.x & 63;
+  const int32_t n = s.m + s.d;
+  for (int32_t e = lane; e < n; e += 64) {
+    sk[e] = e < s.m ? s.ok[e] : (int32_t)s.dk[e - s.m];
+    sc[e] = 0;
+  }
+  wave_lds_sync();
+  int32_t ds = 0;
+  for (int32_t e = lane; e < n; e += 64) {
+    const bool fromO = e < s.m;
+    const int32_t key = sk[e];
+    int32_t lo_ = 0, hi_ = fromO ? s.d : s.m;
+    const int32_t *other = fromO ? sk + s.m : sk;
+    while (lo_ < hi_) {
+      const int32_t mid = (lo_ + hi_) >> 1;
+      if (fromO ? other[mid] < key : other[mid] <= key) lo_ = mid + 1;
+      else hi_ = mid;
+    }
+    bool valid;
+    int32_t rank;
+    if (fromO) {
+      const bool pair = lo_ < s.d && other[lo_] == key;
+      const int32_t v = s.ov[e] + (pair ? (int32_t)s.dv[lo_] : 0);
+      valid = v != 0;
+      rank = e + lo_;
+    } else {
+      const int32_t j = e - s.m;
+      const bool pair = lo_ > 0 && other[lo_ - 1] == key;
+      const int32_t w = (int32_t)s.dv[j];
+      ds += w;
+      valid = !pair && w != 0;
+      rank = j + lo_;
+    }
+    if (valid) sc[rank] = 1;
+  }
+  wave_lds_sync();
+  // exclusive scan of sc over the ranks
+  uint32_t run = 0;
+  for (int32_t b0 = 0; b0 < n; b0 += 64) {
+    const int32_t r = b0 + lane;
+    uint32_t f = r < n ? sc[r] : 0u, incl = f;
+#pragma unroll
+    for (int o_ = 1; o_ < 64; o_ <<= 1) {
+      const uint32_t yv = __shfl_up(incl, o_, 64);
+      if (lane >= o_) incl += yv;
+    }
+    if (r < n) sc[r] = run + incl - f;
+    run += __shfl(incl, 63, 64);
+  }
+  wave_lds_sync();
+  if (WRITE) {
+    const uint32_t base = mout0(len, B, x);
+    for (int32_t e = lane; e < n; e += 64) {
+      const bool fromO = e < s.m;
+      const int32_t key = sk[e];
+      int32_t lo_ = 0, hi_ = fromO ? s.d : s.m;
+      const int32_t *other = fromO ? sk + s.m : sk;
+      while (lo_ < hi_) {
+        const int32_t mid = (lo_ + hi_) >> 1;
+        if (fromO ? other[mid] < key : other[mid] <= key) lo_ = mid + 1;
+        else hi_ = mid;
+      }
+      if (fromO) {
+        const bool pair = lo_ < s.d && other[lo_] == key;
+        const int32_t v = s.ov[e] + (pair ? (int32_t)s.dv[lo_] : 0);
+        if (v != 0) mput(nw, s, base + sc[e + lo_], key, v);
+      } else {
+        const int32_t j = e - s.m;
+        const bool pair = lo_ > 0 && other[lo_ - 1] == key;
+        const int32_t w = (int32_t)s.dv[j];
+        if (!pair && w != 0) mput(nw, s, base + sc[j + lo_], key, w);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int o_ = 32; o_ >= 1; o_ >>= 1) ds += __shfl_xor(ds, o_, 64);
+    if (lane == 0) {
+      len[x] = run;
+      if (ds != 0) {
+        if (s.isrow) dout[s.y] += ds;
+        else din[s.y] += ds;
+      }
+    }
+  }
+  wave_lds_sync();  // the LDS is reused by the wave's next segment
+}
+
+// Classes 16 / 32 / 64 / wave: wave tasks (four, two or one segments of a lane-group class,
+// or one wave-class segment).  WRITE = false: len[x] = the segment's new length, and its runs'
+// sum added to the block degree; WRITE = true (len scanned): the outputs, and the new row and
+// column pointers.
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_merge_small(BMOld o, const uint32_t *off,
+                                                     const uint32_t *su, const uint32_t *ekey,
+                                                     const uint32_t *ew, const uint32_t *rk,
+                                                     const uint32_t *rw, int one_range,
+                                                     int32_t B, const uint32_t *lists,
+                                                     int64_t lcap, const uint32_t *ncls,
+                                                     uint32_t *len, int32_t *dout, int32_t *din,
+                                                     BMNew nw) {
+  __shared__ int32_t sk[4][MW_CAP];
+  __shared__ uint32_t sc[4][MW_CAP];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (WRITE) {
+    const int64_t nseg = 2 * (int64_t)B;
+    const uint32_t nnz = len[B];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= nseg;
+         i += (int64_t)gridDim.x * blockDim.x) {
+      if (i < B) nw.rowptr[i] = (int32_t)len[i];
+      else if (i < nseg) nw.colptr[i - B] = (int32_t)(len[i] - nnz);
+      else { nw.rowptr[B] = (int32_t)nnz; nw.colptr[B] = (int32_t)(len[nseg] - nnz); }
+    }
+  }
+  const int64_t n16 = ncls[MC_16], n32 = ncls[MC_32], n64 = ncls[MC_64], nW = ncls[MC_W];
+  const int64_t t16 = (n16 + 3) / 4, t32 = (n32 + 1) / 2;
+  const int64_t tasks = t16 + t32 + n64 + nW;
+  uint32_t *lenw = len;
+  for (int64_t t = (int64_t)blockIdx.x * 4 + w; t < tasks; t += (int64_t)gridDim.x * 4) {
+    if (t < t16) {
+      const int64_t k = t * 4 + (lane >> 4);
+      if (k < n16) {
+        const uint32_t x = lists[MC_16 * lcap + k];
+        const MSeg s = mseg(o, off, su, ekey, ew, rk, rw, one_range, B, x);
+        merge_group<16, WRITE>(s, x, B, lenw, dout, din, nw);
+      }
+    } else if (t < t16 + t32) {
+      const int64_t k = (t - t16) * 2 + (lane >> 5);
+      if (k < n32) {
+        const uint32_t x = lists[MC_32 * lcap + k];
+        const MSeg s = mseg(o, off, su, ekey, ew, rk, rw, one_range, B, x);
+        merge_group<32, WRITE>(s, x, B, lenw, dout, din, nw);
+      }
+    } else if (t < t16 + t32 + n64) {
+      const uint32_t x = lists[MC_64 * lcap + (t - t16 - t32)];
+      const MSeg s = mseg(o, off, su, ekey, ew, rk, rw, one_range, B, x);
+      merge_group<64, WRITE>(s, x, B, lenw, dout, din, nw);
+    } else {
+      const uint32_t x = lists[MC_W * lcap + (t - t16 - t32 - n64)];
+      const MSeg s = mseg(o, off, su, ekey, ew, rk, rw, one_range, B, x);
+      merge_wave<WRITE>(s, x, B, lenw, dout, din, nw, sk[w], sc[w]);
+    }
+  }
+}
+
+// Class H (n > MW_CAP): one wave per listed segment merges O and D in key order, 64 outputs
+// of the merged sequence at a time.  A chunk's split (a entries of O, b of D) is the merge
+// path found by one ballot: lane t tests a = amin + 1 + t (O's a-th entry not above D's
+// (L - a + 1)-th; ties put O first), and an equal (O, D) pair is kept in one chunk.  Each
+// lane finds its rank in the chunk by a binary search over the other part (shuffles); an O
+// entry adds its equal D run's weight, a D run without an equal O entry is new; entries of
+// value 0 are dropped.
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_merge_long(BMOld o, const uint32_t *off,
+                                                    const uint32_t *su, const uint32_t *ekey,
+                                                    const uint32_t *ew, const uint32_t *rk,
+                                                    const uint32_t *rw, int one_range,
+                                                    int32_t B, const uint32_t *lists,
+                                                    int64_t lcap, const uint32_t *ncls,
+                                                    uint32_t *len, int32_t *dout, int32_t *din,
+                                                    BMNew nw) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nl = ncls[MC_H];
+  for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < nl; t += (int64_t)gridDim.x * 4) {
+    const uint32_t x = lists[MC_H * lcap + t];
+    const MSeg s = mseg(o, off, su, ekey, ew, rk, rw, one_range, B, x);
+    const int32_t m = s.m, d = s.d;
+    const uint32_t out0 = WRITE ? mout0(len, B, x) : 0u;
+    int32_t io = 0, jd = 0;
+    uint32_t pos = 0;
+    int32_t dsum = 0;
+    while (io < m || jd < d) {
+      const int32_t remO = m - io, remD = d - jd;
+      int32_t L = min(64, remO + remD);
+      const int32_t amin = max(0, L - remD), amax = min(L, remO);
+      // (a = amin always qualifies; lanes test amin + 1 .. amin + 64, which covers amax)
+      const int32_t at = amin + 1 + lane;
+      bool P = false;
+      if (at <= amax) {
+        if (L - at >= remD) P = true;
+        else P = s.ok[io + at - 1] <= (int32_t)s.dk[jd + L - at];
+      }
+      int32_t a = amin + __popcll(__ballot(P));
+      int32_t b = L - a;
+      if (a > 0 && b < remD && s.ok[io + a - 1] == (int32_t)s.dk[jd + b]) {
+        if (L < 64) { ++b; ++L; }
+        else { --a; --L; }
+      }
+      const bool isO = lane < a, isD = lane >= a && lane < L;
+      int32_t key = 0, v = 0;
+      if (isO) {
+        key = s.ok[io + lane];
+        v = s.ov[io + lane];
+      } else if (isD) {
+        key = (int32_t)s.dk[jd + lane - a];
+        v = (int32_t)s.dv[jd + lane - a];
+      }
+      // O entry: D runs of the chunk below its key; D run: O entries not above its key
+      int32_t lo_ = 0, hi_ = isO ? b : (isD ? a : 0);
+      const int base = isO ? a : 0;
+#pragma unroll
+      for (int st_ = 0; st_ < 7; ++st_) {
+        const int32_t mid = (lo_ + hi_) >> 1;
+        const int32_t kk = __shfl(key, (base + mid) & 63, 64);
+        if (lo_ < hi_) {
+          if (isO ? (kk < key) : (kk <= key)) lo_ = mid + 1;
+          else hi_ = mid;
+        }
+      }
+      const int32_t cnt = lo_;
+      const int psrc = isO ? a + cnt : cnt - 1;
+      const int32_t pk = __shfl(key, psrc & 63, 64), pv = __shfl(v, psrc & 63, 64);
+      const bool matched = isO ? (cnt < b && pk == key) : (isD && cnt > 0 && pk == key);
+      int32_t outv = v;
+      bool valid;
+      if (isO) {
+        outv = v + (matched ? pv : 0);
+        valid = outv != 0;
+      } else {
+        valid = isD && !matched && v != 0;
+      }
+      const int32_t rank = isO ? lane + cnt : lane - a + cnt;
+      uint64_t mk = valid ? (1ull << rank) : 0ull;
+#pragma unroll
+      for (int q = 1; q < 64; q <<= 1) mk |= __shfl_xor(mk, q, 64);
+      if (WRITE && valid)
+        mput(nw, s, out0 + pos + (uint32_t)__popcll(mk & ((1ull << rank) - 1ull)), key, outv);
+      if (!WRITE && isD) dsum += v;
+      pos += (uint32_t)__popcll(mk);
+      io += a;
+      jd += b;
+    }
+    if (!WRITE) {
+#pragma unroll
+      for (int q = 32; q >= 1; q >>= 1) dsum += __shfl_xor(dsum, q, 64);
+      if (lane == 0) {
+        len[x] = pos;
+        if (dsum != 0) {
+          if (s.isrow) dout[s.y] += dsum;
+          else din[s.y] += dsum;
+        }
+      }
+    }
+  }
+}
+
+bool rebuild_after_moves_sparse(const GraphDev &g, const int32_t *lab32, const int32_t *olab,
+                                int64_t lo, int64_t BS, int32_t B, int64_t bound, BMBuf &bm,
+                                RebuildScratch &rs, hipStream_t st) {
+  if (!(rs.sparse_on && !bm.dense_valid && bm.dense_B == 0 && bm.B == B && B > SMALL_B &&
+        bm.a_rowptr && rs.rk && rs.mwork && bound <= rs.kcap && bound <= rs.rcap &&
+        BS <= rs.mwork_list))
+    return false;
+  const int64_t nseg = 2 * (int64_t)B;
+  const int T = 256;
+  uint32_t *ekey = (uint32_t *)rs.k0, *ew = (uint32_t *)rs.k1;
+  const bool one = B <= SEG_RC;
+  const int32_t nr = (int32_t)((B + SEG_RC - 1) / SEG_RC);
+  const uint32_t RC = one ? (uint32_t)B : (uint32_t)SEG_RC;
+  const int32_t *ow = g.wmax == 1 ? nullptr : g.ow, *iw = g.wmax == 1 ? nullptr : g.iw;
+  // the work area: the moved list, its count and the merge classes' counts
+  uint32_t *list = rs.mwork, *nmv = list + rs.mwork_list, *mcls = nmv + 1;
+  k_delta_zero<<<grid_for(nseg + 1, T, SEG_GRID), T, 0, st>>>(rs.soff, rs.vrel, rs.su, rs.ncl,
+                                                             nmv, nseg);
+  if (BS > 0) {
+    k_moved_list<<<grid_for(BS, T, 1024), T, 0, st>>>(lab32, olab, lo, BS, list, nmv);
+    const unsigned gg = (unsigned)std::min<int64_t>(4096, (BS + 15) / 16);
+    k_delta_gen<false><<<gg, 256, 0, st>>>(g.optr, g.oadj, ow, g.iptr, g.iadj, iw, lab32, olab,
+                                           lo, BS, list, nmv, B, rs.soff, nullptr, nullptr,
+                                           nullptr, nullptr);
+    scan_exclusive(rs.soff, rs.soff, nseg, rs.soff + nseg, rs.tmp, st);
+    k_delta_gen<true><<<gg, 256, 0, st>>>(g.optr, g.oadj, ow, g.iptr, g.iadj, iw, lab32, olab,
+                                          lo, BS, list, nmv, B, nullptr, rs.soff, rs.vrel, ekey,
+                                          ew);
+  } else {
+    scan_exclusive(rs.soff, rs.soff, nseg, rs.soff + nseg, rs.tmp, st);
+  }
+  const uint32_t reg_max = (uint32_t)rs.seg_reg_max, count_min = (uint32_t)rs.seg_count_min;
+  k_seg_classify<<<grid_for(nseg, T, SEG_GRID), T, 0, st>>>(rs.soff, nseg, one ? 1 : 0, reg_max,
+                                                            count_min, rs.lists, rs.lcap, rs.ncl);
+  if (one) {
+    k_seg_c1<false><<<SEG_GRID, 512, (size_t)RC * 4, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
+                                                           rs.ncl, B, rs.su, 0);
+    if (count_min > (uint32_t)SEG_W) {
+      k_seg_sort_g<16, CL_G4K, false><<<SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists,
+                                                                rs.lcap, rs.ncl, rs.su, 0);
+      k_seg_sort_g<8, CL_G2K, false><<<SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists,
+                                                               rs.lcap, rs.ncl, rs.su, 0);
+      k_seg_sort_g<4, CL_G1K, false><<<2 * SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists,
+                                                                   rs.lcap, rs.ncl, rs.su, 0);
+    }
+  } else {
+    k_seg_sort_g<16, CL_G4K, false><<<SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists,
+                                                              rs.lcap, rs.ncl, rs.su, 0);
+    k_seg_sort_g<8, CL_G2K, false><<<SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists,
+                                                             rs.lcap, rs.ncl, rs.su, 0);
+    k_seg_sort_g<4, CL_G1K, false><<<2 * SEG_GRID, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists,
+                                                                 rs.lcap, rs.ncl, rs.su, 0);
+    k_seg_c_count<false><<<SEG_GRID / 2, 512, (size_t)RC * 4, st>>>(
+        rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl, B, nr, rs.cntc, rs.su, 0);
+    k_seg_c_runs<false><<<SEG_GRID / 2, 512, (size_t)RC * 4, st>>>(
+        rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl, B, nr, rs.cntc, rs.rk, rs.rw, 0);
+  }
+  k_seg_sort_w<false><<<SEG_GRID * 3 / 2, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
+                                                      rs.ncl, rs.su, 0);
+  k_seg_sort_reg<false><<<SEG_GRID * 2, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
+                                                      rs.ncl, rs.su, 0);
+  // merge each segment's current entries with its delta runs: classes, new lengths (and
+  // block degrees), offsets, outputs and pointers
+  const BMOld o{bm.rowptr, bm.col, bm.val, bm.colptr, bm.row, bm.cval};
+  const BMNew n{bm.a_rowptr, bm.a_col, bm.a_val, bm.a_erow, bm.a_colptr,
+                bm.a_row, bm.a_cval, bm.a_gcol, bm.a_grow, g.E};
+  k_merge_classify<<<grid_for(nseg, T, SEG_GRID), T, 0, st>>>(o, rs.soff, rs.su, B, rs.lists,
+                                                              rs.lcap, mcls, rs.suoff);
+  const int oc = one ? 1 : 0;
+  k_merge_small<false><<<SEG_GRID * 2, 256, 0, st>>>(o, rs.soff, rs.su, ekey, ew, rs.rk, rs.rw,
+                                                     oc, B, rs.lists, rs.lcap, mcls, rs.suoff,
+                                                     bm.dout, bm.din, n);
+  k_merge_long<false><<<SEG_GRID / 4, 256, 0, st>>>(o, rs.soff, rs.su, ekey, ew, rs.rk, rs.rw,
+                                                    oc, B, rs.lists, rs.lcap, mcls, rs.suoff,
+                                                    bm.dout, bm.din, n);
+  scan_exclusive(rs.suoff, rs.suoff, nseg, rs.suoff + nseg, rs.tmp, st);
+  k_merge_small<true><<<SEG_GRID * 2, 256, 0, st>>>(o, rs.soff, rs.su, ekey, ew, rs.rk, rs.rw,
+                                                    oc, B, rs.lists, rs.lcap, mcls, rs.suoff,
+                                                    bm.dout, bm.din, n);
+  k_merge_long<true><<<SEG_GRID / 4, 256, 0, st>>>(o, rs.soff, rs.su, ekey, ew, rs.rk, rs.rw,
+                                                   oc, B, rs.lists, rs.lcap, mcls, rs.suoff,
+                                                   bm.dout, bm.din, n);
+  k_logs<<<grid_for(B, 256), 256, 0, st>>>(bm.dout, bm.din, B, bm.ldout, bm.ldin);
+  bm.swap_sets();
+  return true;
+}
+
 void rebuild_blockmodel(const GraphDev &g, const int32_t *lab32, int32_t B, BMBuf &bm,
                         RebuildScratch &rs, int32_t dense_max_B, int32_t tile_max_B,
                         hipStream_t st) {

@@ -2,6 +2,7 @@
 // orchestrator (engine.hip) drives.
 #pragma once
 #include <string>
+#include <utility>
 
 #include "common.h"
 #include "prims.h"
@@ -38,11 +39,25 @@ struct BMBuf {
   int32_t *gcol = nullptr, *grow = nullptr;       // E/16 + 1: coarse index of col / row
   bool index_valid = false;
   bool dense_valid = false;
+
   // the transpose of the dense matrix in the rebuild scratch's partial-plane buffer is
   // current for B = dt_B (0: not current); the incremental update keeps it so
   int32_t dt_B = 0;
   // B of the entries the dense buffer holds (the current CSR's); 0 = all zero
   int32_t dense_B = 0;
+  // The alternate CSR/CSC buffers (same sizes): the sparse update after a nodal batch merges
+  // the current matrix and the batch's delta into them, then the two sets change roles;
+  // flipped records that the alternate set holds the context's primary buffers.
+  int32_t *a_rowptr = nullptr, *a_colptr = nullptr, *a_col = nullptr, *a_val = nullptr,
+          *a_erow = nullptr, *a_row = nullptr, *a_cval = nullptr, *a_gcol = nullptr,
+          *a_grow = nullptr;
+  bool flipped = false;
+  void swap_sets() {
+    std::swap(rowptr, a_rowptr); std::swap(colptr, a_colptr); std::swap(col, a_col);
+    std::swap(val, a_val); std::swap(erow, a_erow); std::swap(row, a_row);
+    std::swap(cval, a_cval); std::swap(gcol, a_gcol); std::swap(grow, a_grow);
+    flipped = !flipped;
+  }
   BMView view() const {
     BMView v;
     v.B = B; v.rowptr = rowptr; v.col = col; v.val = val;
@@ -76,6 +91,14 @@ struct RebuildScratch {
   // segment classes: at most seg_reg_max entries sorted in register groups (16, 32 or 64);
   // with B in one column range, segments longer than seg_count_min (512..4096) counted in LDS
   int32_t seg_reg_max = 64, seg_count_min = 512;
+  // sparse update after a nodal batch: k0 / k1 hold kcap words each; rk / rw (rcap words
+  // each) receive the runs of segments longer than 4096 entries when B > 16000
+  int64_t kcap = 0, rcap = 0;
+  uint32_t *rk = nullptr, *rw = nullptr;
+  bool sparse_on = true;
+  // its work area: the moved list (mwork_list words), then 8 counter words
+  uint32_t *mwork = nullptr;
+  int64_t mwork_list = 0;
 };
 // words of cntc for a graph of N vertices and E edges
 int64_t seg_cntc_words(int64_t N, int64_t E);
@@ -91,6 +114,16 @@ void rebuild_blockmodel(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf 
 // Returns false (nothing launched) otherwise.
 bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t *olab, int64_t lo,
                          int64_t BS, int32_t B, BMBuf &bm, RebuildScratch &rs, hipStream_t st);
+
+// After a nodal batch over [lo, lo + BS) at a B without the dense copy: the batch's edge
+// moves as a signed delta matrix, sorted per row and column by the segmented rebuild's
+// sorts and merged with the current CSR and CSC into the alternate buffers (then swapped
+// in); bound = an upper bound of the delta entries (4 x the batch's out- plus in-adjacency).
+// Returns false (nothing launched) when the current matrix is not the batch-start one at
+// this B or the delta does not fit the scratch.
+bool rebuild_after_moves_sparse(const GraphDev &g, const int32_t *lab32, const int32_t *olab,
+                                int64_t lo, int64_t BS, int32_t B, int64_t bound, BMBuf &bm,
+                                RebuildScratch &rs, hipStream_t st);
 
 // D = sum of C partial planes of B*B counters, then the block matrix of D (B <= 128) as CSR,
 // CSC, degrees and log2 degrees (the small rebuild path's reduction and compaction)

@@ -192,6 +192,9 @@ struct gsap_ctx {
   DBuf lab, snap[3];
   // block matrix
   DBuf rowptr, colptr, col, val, erow, row, cval, dout, din, ldout, ldin, dense, gcol, grow;
+  // the alternate CSR/CSC set of the sparse update after a nodal batch, and its long-segment
+  // runs
+  DBuf rowptr2, colptr2, col2, val2, erow2, row2, cval2, gcol2, grow2, rk, rw, mwork;
   // rebuild scratch
   DBuf k0, k1, tmp, partials, soff, su, suoff, lists, ncl, cntc, vrel, chunk_seg, lab16;
   // sweep / merge
@@ -208,6 +211,11 @@ struct gsap_ctx {
   DBuf vlist;
   std::vector<int64_t> list_off;  // [batch * 4 + class] -> offset into vlist; last = total
   std::vector<uint32_t> big_lds;  // per batch: LDS bytes of the largest class's launch
+  // per batch: 4 x (out- plus in-adjacency of its vertices), the most delta entries its
+  // sparse block-matrix update can produce
+  std::vector<int64_t> delta_bound;
+  // set while a sweep is captured as a HIP graph
+  bool capturing = false;
   int32_t lists_nb = 0;
   NodalBounds lists_bounds;
   GraphDev g;
@@ -437,6 +445,12 @@ void build_batch_lists(gsap_ctx *c, const gsap_graph_i32 *g, int32_t nb,
     lst.insert(lst.end(), per[b].begin(), per[b].end());
   }
   c->list_off[(size_t)nb * NCLS] = (int64_t)lst.size();
+  c->delta_bound.assign((size_t)nb, 0);
+  for (int32_t b = 0; b < nb; ++b) {
+    const int64_t lo = b * BS, hi = lo + BS;
+    c->delta_bound[b] = 4 * ((int64_t)(g->out_ptr[hi] - g->out_ptr[lo]) +
+                             (int64_t)(g->in_ptr[hi] - g->in_ptr[lo]));
+  }
   c->vlist.ensure(4 * std::max<size_t>(lst.size(), 1));
   if (!lst.empty())
     HIPX(hipMemcpyAsync(c->vlist.p, lst.data(), 4 * lst.size(), hipMemcpyHostToDevice, c->st));
@@ -448,6 +462,15 @@ void build_batch_lists(gsap_ctx *c, const gsap_graph_i32 *g, int32_t nb,
 void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   const int64_t N = c->N, E = std::max<int64_t>(c->E, 1);
   c->dense_max_B = dense_max_B;
+  if (c->bm.flipped) {
+    // the last sparse update left the current matrix in the alternate buffers: they become
+    // the primary ones again (the dense buffer's entries are described by them)
+    std::swap(c->rowptr, c->rowptr2); std::swap(c->colptr, c->colptr2);
+    std::swap(c->col, c->col2); std::swap(c->val, c->val2); std::swap(c->erow, c->erow2);
+    std::swap(c->row, c->row2); std::swap(c->cval, c->cval2);
+    std::swap(c->gcol, c->gcol2); std::swap(c->grow, c->grow2);
+    c->bm.flipped = false;
+  }
   c->lab.ensure(4 * N);
   for (auto &s : c->snap) s.ensure(4 * N);
   // the dense buffer's zero-except-current-entries state survives only if neither it nor
@@ -457,6 +480,16 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   c->col.ensure(4 * E); c->val.ensure(4 * E); c->erow.ensure(4 * E);
   c->row.ensure(4 * E); c->cval.ensure(4 * E);
   c->gcol.ensure(4 * (E / 16 + 2)); c->grow.ensure(4 * (E / 16 + 2));
+  c->rowptr2.ensure(4 * (N + 1)); c->colptr2.ensure(4 * (N + 1));
+  c->col2.ensure(4 * E); c->val2.ensure(4 * E); c->erow2.ensure(4 * E);
+  c->row2.ensure(4 * E); c->cval2.ensure(4 * E);
+  c->gcol2.ensure(4 * (E / 16 + 2)); c->grow2.ensure(4 * (E / 16 + 2));
+  // delta entries of the sparse update: at most the largest batch's bound (4 x its out- plus
+  // in-adjacency, about 2E with 4 batches); the segment arrays k0 / k1 hold 2E words already
+  int64_t dmax = 2 * E;
+  for (int64_t x : c->delta_bound) dmax = std::max(dmax, x);
+  c->rk.ensure(4 * dmax); c->rw.ensure(4 * dmax);
+  c->mwork.ensure(4 * (size_t)(N + 8));
   c->dout.ensure(4 * N); c->din.ensure(4 * N);
   c->ldout.ensure(8 * N); c->ldin.ensure(8 * N);
   const int64_t dB = std::min<int64_t>(dense_max_B, N);
@@ -466,7 +499,7 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
     HIPX(hipMemsetAsync(c->dense.p, 0, c->dense.cap, c->st));
     c->bm.dense_B = 0;
   }
-  c->k0.ensure(8 * E); c->k1.ensure(8 * E);
+  c->k0.ensure(4 * dmax); c->k1.ensure(4 * dmax);
   c->tmp.ensure(4 * std::max(scan_temp_words(2 * N + 1), radix_temp_words(N)));
   // partial planes of the small-B histogram (up to 1024 planes of at most 128 x 128
   // counters) and of the row-tile histogram (up to 8M counters), then the transpose
@@ -505,6 +538,11 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   b.ldout = c->ldout.as<double>(); b.ldin = c->ldin.as<double>();
   b.dense = c->dense.as<int32_t>(); b.dense_cap = (int64_t)(c->dense.cap / 4);
   b.gcol = c->gcol.as<int32_t>(); b.grow = c->grow.as<int32_t>();
+  b.a_rowptr = c->rowptr2.as<int32_t>(); b.a_colptr = c->colptr2.as<int32_t>();
+  b.a_col = c->col2.as<int32_t>(); b.a_val = c->val2.as<int32_t>();
+  b.a_erow = c->erow2.as<int32_t>(); b.a_row = c->row2.as<int32_t>();
+  b.a_cval = c->cval2.as<int32_t>(); b.a_gcol = c->gcol2.as<int32_t>();
+  b.a_grow = c->grow2.as<int32_t>();
   RebuildScratch &r = c->rs;
   r.k0 = c->k0.p; r.k1 = c->k1.p; r.tmp = c->tmp.as<uint32_t>();
   r.tmp_words = (int64_t)(c->tmp.cap / 4);
@@ -513,6 +551,10 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   r.su = c->su.as<uint32_t>(); r.suoff = c->suoff.as<uint32_t>(); r.vrel = c->vrel.as<uint32_t>(); r.chunk_seg = c->chunk_seg.as<uint32_t>(); r.lab16 = c->lab16.as<uint16_t>();
   r.lists = c->lists.as<uint32_t>(); r.lcap = 2 * N; r.ncl = c->ncl.as<uint32_t>();
   r.cntc = c->cntc.as<uint32_t>();
+  r.kcap = (int64_t)(c->k0.cap / 4);
+  r.rk = c->rk.as<uint32_t>(); r.rw = c->rw.as<uint32_t>(); r.rcap = (int64_t)(c->rk.cap / 4);
+  r.mwork = c->mwork.as<uint32_t>();
+  r.mwork_list = N;
 }
 
 constexpr int32_t SMALL_B_TUNE = 128;  // the small rebuild path's limit (never tuned)
@@ -545,13 +587,18 @@ void rebuild(gsap_ctx *c, int32_t B) {
 
 // The rebuild after nodal batch b: the moved vertices' edges moved between cells of the
 // dense matrix when it holds the matrix of the labels before the batch, else a full rebuild.
+// Without the dense matrix and with options.sparse_update = 1, the sparse update merges the
+// batch's delta into the current CSR/CSC (a captured sweep of an odd number of batches would
+// end on the other buffer set, so it rebuilds instead).
 void rebuild_after_batch(gsap_ctx *c, int32_t B, const gsap_options &o, int32_t b) {
   int p = c->prof.begin(1, c->st);
   const int64_t BS = c->N / o.num_batches, lo = (int64_t)b * BS;
-  if (!rebuild_after_moves(c->g, c->lab.as<int32_t>(), c->olab.as<int32_t>(), lo, BS, B, c->bm,
-                           c->rs, c->st))
-    rebuild_blockmodel(c->g, c->lab.as<int32_t>(), B, c->bm, c->rs, c->dense_max_B,
-                       c->tile_max_B, c->st);
+  const int32_t *lab = c->lab.as<int32_t>(), *olab = c->olab.as<int32_t>();
+  c->rs.sparse_on = o.sparse_update == 1 && !(c->capturing && (o.num_batches & 1));
+  const int64_t bound = b < (int32_t)c->delta_bound.size() ? c->delta_bound[b] : INT64_MAX;
+  if (!rebuild_after_moves(c->g, lab, olab, lo, BS, B, c->bm, c->rs, c->st) &&
+      !rebuild_after_moves_sparse(c->g, lab, olab, lo, BS, B, bound, c->bm, c->rs, c->st))
+    rebuild_blockmodel(c->g, lab, B, c->bm, c->rs, c->dense_max_B, c->tile_max_B, c->st);
   c->prof.end(p, c->st);
   HIPX(hipGetLastError());
 }
@@ -822,6 +869,9 @@ void check_options(const gsap_options &o) {
     throw Err(GSAP_ERR_INVALID, "seg_reg_max must be 0, 16, 32 or 64");
   if (o.seg_count_min != 0 && (o.seg_count_min < 512 || o.seg_count_min > 4096))
     throw Err(GSAP_ERR_INVALID, "seg_count_min must be 0 or in 512..4096");
+  if (o.sparse_update != 0 && o.sparse_update != 1)
+    throw Err(GSAP_ERR_INVALID, "sparse_update must be 0 or 1");
+
   if (o.delta_mode != GSAP_DELTA_REFERENCE && o.delta_mode != GSAP_DELTA_EXACT)
     throw Err(GSAP_ERR_INVALID, "unknown delta_mode");
   if (o.accept != GSAP_ACCEPT_GREEDY && o.accept != GSAP_ACCEPT_MH)
@@ -843,9 +893,12 @@ struct GraphGuard {
   void capture(F &&body) {
     HIPX(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
     hipGraph_t g = nullptr;
+    c->capturing = true;
     try {
       body();
+      c->capturing = false;
     } catch (...) {
+      c->capturing = false;
       (void)hipStreamEndCapture(c->st, &g);
       if (g) (void)hipGraphDestroy(g);
       throw;
@@ -1214,7 +1267,9 @@ int gsap_destroy(gsap_ctx *c) {
                  &c->dlist, &c->dcnt, &c->snap[0],
                  &c->snap[1], &c->snap[2], &c->rowptr, &c->colptr, &c->col, &c->val, &c->erow,
                  &c->row, &c->cval, &c->dout, &c->din, &c->ldout, &c->ldin, &c->dense,
-                 &c->gcol, &c->grow, &c->k0,
+                 &c->gcol, &c->grow, &c->rowptr2, &c->colptr2, &c->col2, &c->val2,
+                 &c->erow2, &c->row2, &c->cval2, &c->gcol2, &c->grow2, &c->rk, &c->rw, &c->mwork,
+                 &c->k0,
                  &c->k1, &c->tmp, &c->partials, &c->soff, &c->su, &c->suoff,
                  &c->lists, &c->ncl, &c->cntc, &c->vrel, &c->chunk_seg, &c->lab16, &c->vlist,
                  &c->OUT, &c->sof, &c->coin, &c->prop, &c->dS, &c->best, &c->dSbest, &c->map,
@@ -1494,14 +1549,12 @@ int gsap_get_trace(const gsap_ctx *c, gsap_trace_rec *out, int64_t cap, int64_t 
   return GSAP_OK;
 }
 
-int gsap_blockmodel_i32(gsap_ctx *c, const gsap_graph_i32 *g, const int32_t *labels, int32_t B,
-                        int64_t *rowptr, int32_t *col, int64_t *val, int64_t *colptr, int32_t *row,
-                        int64_t *cval, int64_t *dout, int64_t *din, int64_t *nnz_out,
-                        const gsap_options *o) {
-  return guarded(c, [&] {
-    if (!labels || !nnz_out) throw Err(GSAP_ERR_INVALID, "NULL argument");
-    setup_unit(c, g, o, labels, B);
+// the current block matrix (B blocks) into caller arrays
+static void copy_blockmodel(gsap_ctx *c, int32_t B, int64_t *rowptr, int32_t *col, int64_t *val,
+                            int64_t *colptr, int32_t *row, int64_t *cval, int64_t *dout,
+                            int64_t *din, int64_t *nnz_out) {
     const int32_t nnz = read_nnz(c);
+    if (nnz < 0 || nnz > c->E) throw Err(GSAP_ERR_DEVICE, "block matrix entry count out of range");
     *nnz_out = nnz;
     std::vector<int32_t> rp(B + 1), cp(B + 1), v(nnz), cv(nnz), dO(B), dI(B);
     hipStream_t st = c->st;
@@ -1519,6 +1572,43 @@ int gsap_blockmodel_i32(gsap_ctx *c, const gsap_graph_i32 *g, const int32_t *lab
     for (int32_t x = 0; x <= B; ++x) { rowptr[x] = rp[x]; colptr[x] = cp[x]; }
     for (int32_t k = 0; k < nnz; ++k) { val[k] = v[k]; cval[k] = cv[k]; }
     for (int32_t x = 0; x < B; ++x) { dout[x] = dO[x]; din[x] = dI[x]; }
+}
+
+int gsap_blockmodel_i32(gsap_ctx *c, const gsap_graph_i32 *g, const int32_t *labels, int32_t B,
+                        int64_t *rowptr, int32_t *col, int64_t *val, int64_t *colptr, int32_t *row,
+                        int64_t *cval, int64_t *dout, int64_t *din, int64_t *nnz_out,
+                        const gsap_options *o) {
+  return guarded(c, [&] {
+    if (!labels || !nnz_out) throw Err(GSAP_ERR_INVALID, "NULL argument");
+    setup_unit(c, g, o, labels, B);
+    copy_blockmodel(c, B, rowptr, col, val, colptr, row, cval, dout, din, nnz_out);
+  });
+}
+
+int gsap_nodal_batch_bm_i32(gsap_ctx *c, const gsap_graph_i32 *g, const gsap_options *o,
+                            int32_t *labels, int32_t B, int32_t b, uint32_t q, int32_t keep,
+                            int64_t *rowptr, int32_t *col, int64_t *val, int64_t *colptr,
+                            int32_t *row, int64_t *cval, int64_t *dout, int64_t *din,
+                            int64_t *nnz_out) {
+  return guarded(c, [&] {
+    if (!labels || !nnz_out) throw Err(GSAP_ERR_INVALID, "NULL argument");
+    gsap_options oo = o ? *o : defaults();
+    if (b < 0 || b >= oo.num_batches) throw Err(GSAP_ERR_INVALID, "batch index out of range");
+    if (!keep) {
+      setup_unit(c, g, &oo, labels, B);
+    } else {
+      // the context's labels and block matrix as the previous call left them
+      if (c->N != g->N || c->E != g->E || c->bm.B != B || c->lists_nb != oo.num_batches)
+        throw Err(GSAP_ERR_INVALID, "keep = 1 needs the previous call's graph, B and batches");
+      check_options(oo);
+      set_device(c);
+    }
+    int32_t ar = avg_row(read_nnz(c), B);
+    HIPX(hipMemcpyAsync(c->ctr.p, &q, 4, hipMemcpyHostToDevice, c->st));
+    nodal_batch(c, oo, b, PhaseRef{c->ctr.as<uint32_t>(), 0}, ar, c->res.as<double>());
+    rebuild_after_batch(c, B, oo, b);
+    HIPX(hipMemcpyAsync(labels, c->lab.p, 4 * (size_t)c->N, hipMemcpyDeviceToHost, c->st));
+    copy_blockmodel(c, B, rowptr, col, val, colptr, row, cval, dout, din, nnz_out);
   });
 }
 

@@ -14,6 +14,8 @@
  *   gsap_merge_phase_i32    <- propose_block_merge + carry_out_best_merges,
  *                              GSAP/partition.hpp:324-454 and 26-78
  *   gsap_nodal_batch_i32    <- propose_nodal_move, GSAP/partition.hpp:456-662
+ *   gsap_nodal_batch_bm_i32 <- propose_nodal_move + calculate_new_csr,
+ *                              GSAP/partition.hpp:914-915
  *   gsap_mdl_i32            <- compute_overall_entropy_cuda, GSAP/partition.hpp:665-704
  *   gsap_converged          <- the convergence test of GSAP/partition.hpp:918-934
  *   gsap_golden_step        <- prepare_for_partition_next, GSAP/partition.hpp:80-322
@@ -122,6 +124,10 @@ typedef struct gsap_options {
   int32_t nodal_maxwave;
   int32_t seg_reg_max;
   int32_t seg_count_min;
+  /* After a nodal batch at a B without the dense block matrix: 0 (default) rebuilds the
+   * block matrix from every edge; 1 merges the batch's moves into the current CSR/CSC as a
+   * sorted signed delta (work follows the moved vertices' edges).  Identical results. */
+  int32_t sparse_update;
 } gsap_options;
 
 /* Per-call report.  Times are seconds of host wall clock around device work. */
@@ -228,6 +234,19 @@ int gsap_merge_phase_i32(gsap_ctx *ctx, const gsap_graph_i32 *g, const gsap_opti
 int gsap_nodal_batch_i32(gsap_ctx *ctx, const gsap_graph_i32 *g, const gsap_options *o,
                          int32_t *labels, int32_t B, int32_t b, uint32_t q, int32_t *prop,
                          double *dS, double *dE);
+
+/* One nodal batch (as gsap_nodal_batch_i32) followed by the block-matrix update that the
+ * partition loop runs after every batch (calculate_new_csr, GSAP/partition.hpp:914-915):
+ * the dense-matrix update, the sparse merge of the batch's moves or a full rebuild, as the
+ * options and B select.  keep = 0: starts from labels[N] (block matrix rebuilt); keep = 1:
+ * continues from the labels and block matrix the previous call on this context left (same
+ * graph, B and num_batches).  labels[N] receives the labels after the batch; the block
+ * matrix afterwards is written like gsap_blockmodel_i32. */
+int gsap_nodal_batch_bm_i32(gsap_ctx *ctx, const gsap_graph_i32 *g, const gsap_options *o,
+                            int32_t *labels, int32_t B, int32_t b, uint32_t q, int32_t keep,
+                            int64_t *rowptr, int32_t *col, int64_t *val, int64_t *colptr,
+                            int32_t *row, int64_t *cval, int64_t *dout, int64_t *din,
+                            int64_t *nnz);
 
 /* Description length of a labelling (GSAP/partition.hpp:665-704), FP64. */
 int gsap_mdl_i32(gsap_ctx *ctx, const gsap_graph_i32 *g, const int32_t *labels, int32_t B,

@@ -282,6 +282,83 @@ def test_c3_merge_phase(ctx, c3, B, big, dense, exact):
     assert got["B"] == ref["B"] and np.array_equal(got["labels"], ref["labels"])
 
 
+BM_KEYS = ("nnz", "rowptr", "col", "val", "colptr", "row", "cval", "dout", "din")
+
+
+@pytest.mark.parametrize("name,B,big,dense", [("c3", 20000, 0.3, 0), ("c3", 3000, 0.05, 1),
+                                              ("c3", 200000, 0.0, 0), ("c3", 9000, 0.0, 0),
+                                              ("heavy", 9000, 0.3, 1), ("heavy", 20000, 0.0, 1),
+                                              ("hubs", 150, 0.0, 1), ("tiny_w", 16, 0.0, 1),
+                                              ("bighubs", 1000, 0.1, 1)])
+def test_block_matrix_after_batch(ctx, c3, heavy, name, B, big, dense):
+    """The block matrix the partition loop holds after each nodal batch of a sweep (without
+    the dense copy: the batch's moves merged into the current CSR/CSC as a sorted signed delta
+    per row and column; B > 16000 with one block of 30 % of the vertices puts delta segments
+    of more than 4096 entries on the column-range counts) equals the oracle's block matrix
+    of the batch's labels, and the full rebuild (sparse_update = 0) gives the same."""
+    g = {"c3": c3, "heavy": heavy}.get(name) or graph(name)
+    rng = np.random.default_rng(29)
+    lab = rng.integers(1, B, g.N).astype(np.int32) if B < g.N else np.arange(g.N, dtype=np.int32)
+    if big:
+        lab[rng.random(g.N) < big] = 0
+    o = Options(seed=41, dense_max_B=dense, sparse_update=1)
+    for b in range(o.num_batches):
+        print(f"batch and block matrix {name} B={B} big={big} dense={dense} batch={b}", flush=True)
+        got_lab, got = ctx.nodal_batch_bm(g, o, lab, B, b, 60 + b)
+        ref_lab = O.nodal_batch(g, o, lab, B, b, 60 + b)["labels"]
+        assert np.array_equal(got_lab, ref_lab)
+        ref = O.blockmodel(g, got_lab, B)
+        for k in BM_KEYS:
+            assert np.array_equal(got[k], ref[k]), (b, k)
+        full_lab, full = ctx.nodal_batch_bm(g, Options(seed=41, dense_max_B=dense), lab, B, b,
+                                            60 + b)
+        assert np.array_equal(full_lab, got_lab)
+        for k in BM_KEYS:
+            assert np.array_equal(full[k], got[k]), (b, k)
+        lab = got_lab
+
+
+def test_block_matrix_consecutive_batches(ctx, c3):
+    """Two sweeps of batches after the first merge phase of the 200k-vertex graph (B =
+    130000), each continuing from the block matrix the previous batch's sparse update left
+    (rows of hundreds of entries next to delta rows of a few: chunks where 64 current entries
+    precede the next delta run): every step equals the full rebuild of the same labels, and
+    the last one the oracle's block matrix."""
+    g = c3
+    o = Options(seed=1, sparse_update=1)
+    m = ctx.merge_phase(g, o, np.arange(g.N, dtype=np.int32), g.N, int(g.N * 0.35), 0)
+    lab, B = m["labels"], m["B"]
+    with gsap_amd.Context(0) as full_ctx:
+        for q in range(8):
+            print(f"consecutive batch {q} B={B}", flush=True)
+            lab, got = ctx.nodal_batch_bm(g, o, lab, B, q % 4, q, keep=q > 0)
+            full = full_ctx.blockmodel(g, lab, B)
+            for k in BM_KEYS:
+                assert np.array_equal(got[k], full[k]), (q, k)
+    ref = O.blockmodel(g, lab, B)
+    for k in BM_KEYS:
+        assert np.array_equal(got[k], ref[k]), k
+
+
+@pytest.mark.parametrize("name,dense", [("hubs", 1), ("LL1k", 64), ("HH5k", 1),
+                                        ("tiny_w", 1)])
+def test_partition_trace_sparse_update(ctx, name, dense):
+    """Whole partitions whose levels above the dense range update the block matrix after
+    every batch by merging the moves: the trace (labels, proposals, block-matrix hashes per
+    sweep) equals the oracle's."""
+    g = graph(name)
+    o = Options(seed=6, trace=1, dense_max_B=dense, sparse_update=1)
+    print(f"partition sparse update {name} dense_max_B={dense}", flush=True)
+    lab, rep = ctx.partition(g, o)
+    tr = ctx.trace()
+    rlab, rrep, rtr = O.partition(g, o, trace_cap=len(tr) + 10)
+    assert len(tr) == rrep.n_trace
+    for k, (a, b) in enumerate(zip(tr, rtr)):
+        assert a[:6] == b[:6], (k, a, b)
+        assert eq_bits(a[6], b[6]) and eq_bits(a[7], b[7]), (k, a, b)
+    assert np.array_equal(lab, rlab) and eq_bits(rep.S, rrep.S)
+
+
 @pytest.mark.parametrize("B", [300, 2500, 6000])
 @pytest.mark.parametrize("tile", [1, 8192])
 def test_c3_blockmodel_tile_threshold(ctx, c3, B, tile):
@@ -311,15 +388,17 @@ def test_autotune_keeps_results(ctx):
     assert np.array_equal(lab, rlab) and eq_bits(rep.S, rrep.S)
 
 
-@pytest.mark.parametrize("name,dense", [("hubs", 0), ("LL1k", 0), ("HH5k", 0), ("HH5k", 64)])
-def test_partition_graph_replay(ctx, name, dense):
+@pytest.mark.parametrize("name,dense,sparse", [("hubs", 0, 0), ("LL1k", 0, 0), ("HH5k", 0, 0),
+                                               ("HH5k", 64, 0), ("HH5k", 64, 1)])
+def test_partition_graph_replay(ctx, name, dense, sparse):
     """options.graph: every level's sweep captured once and replayed (at least three times
     per level, with the level's eager merge, rebuild and golden-section work in between);
     labels and S must equal the oracle's, and a second call must reproduce the first.  With
     dense_max_B = 64 the levels above 64 blocks capture the segmented rebuild and the CSR
-    lookups instead of the dense block matrix."""
+    lookups instead of the dense block matrix (with sparse = 1, the sparse update after each
+    batch, which alternates between two buffer sets)."""
     g = graph(name)
-    o = Options(seed=4, graph=1, dense_max_B=dense)
+    o = Options(seed=4, graph=1, dense_max_B=dense, sparse_update=sparse)
     print(f"partition graph replay {name} dense_max_B={dense}", flush=True)
     lab, rep = ctx.partition(g, o)
     rlab, rrep, _ = O.partition(g, Options(seed=4))
