@@ -233,65 +233,89 @@ __global__ __launch_bounds__(512) void k_tile_hist(const int32_t *optr, const in
   for (int32_t i = threadIdx.x; i < nt; i += blockDim.x) out[i] = T[i];
 }
 
-// One wave per row (four rows per workgroup): nonzero count and row sum.  Each step reads
-// 256 entries with four independent coalesced loads per lane.
-__global__ __launch_bounds__(256) void k_dense_rows(const int32_t *__restrict__ D, int32_t B,
-                                                    uint32_t *__restrict__ rcnt,
-                                                    int32_t *__restrict__ dout) {
-  const int32_t r = (int32_t)(blockIdx.x * 4 + (threadIdx.x >> 6));
-  if (r >= B) return;
+// Rows of D (blocks 0..B-1) and of its transpose DT (blocks B..2B-1) in one launch: the
+// nonzero count of row x into cnt[x] and its sum into deg (d_out for D, d_in for DT)
+__global__ __launch_bounds__(256) void k_dense_rows2(const int32_t *__restrict__ D,
+                                                     const int32_t *__restrict__ DT, int32_t B,
+                                                     uint32_t *__restrict__ cnt,
+                                                     int32_t *__restrict__ dout,
+                                                     int32_t *__restrict__ din) {
+  const int32_t x = (int32_t)(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (x >= 2 * B) return;
   const int lane = threadIdx.x & 63;
-  const int32_t *row = D + (int64_t)r * B;
+  const bool tr = x >= B;
+  const int32_t r = tr ? x - B : x;
+  const int32_t *row = (tr ? DT : D) + (int64_t)r * B;
   uint32_t c = 0;
   int32_t s = 0;
   for (int32_t base = 0; base < B; base += 256) {
-    int32_t x[4];
+    int32_t v[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int32_t t = base + 64 * k + lane;
-      x[k] = t < B ? row[t] : 0;
+      v[k] = t < B ? row[t] : 0;
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      c += x[k] != 0;
-      s += x[k];
+      c += v[k] != 0;
+      s += v[k];
     }
   }
   for (int off = 32; off >= 1; off >>= 1) {
     c += __shfl_down(c, off, 64);
     s += __shfl_down(s, off, 64);
   }
-  if (lane == 0) { rcnt[r] = c; dout[r] = s; }
+  if (lane == 0) {
+    cnt[x] = c;
+    (tr ? din : dout)[r] = s;
+  }
 }
 
-// One wave per row: the row's nonzero entries in column order at rowptr[r] (and erow when
-// given); four 64-entry sub-chunks per step, their loads issued before the writes.
-__global__ __launch_bounds__(256) void k_dense_write_rows(const int32_t *__restrict__ D, int32_t B,
-                                                          const int32_t *__restrict__ rowptr,
-                                                          int32_t *__restrict__ col,
-                                                          int32_t *__restrict__ val,
-                                                          int32_t *__restrict__ erow) {
-  const int32_t r = (int32_t)(blockIdx.x * 4 + (threadIdx.x >> 6));
-  if (r >= B) return;
+// The nonzero entries of D's rows (CSR, with erow) and of DT's rows (CSC) at the scanned
+// offsets off[0..2B] (columns from off[B]), the row and column pointers, and the log2
+// block degrees
+__global__ __launch_bounds__(256) void k_dense_write_rows2(
+    const int32_t *__restrict__ D, const int32_t *__restrict__ DT, int32_t B,
+    const uint32_t *__restrict__ off, int32_t *__restrict__ rowptr, int32_t *__restrict__ col,
+    int32_t *__restrict__ val, int32_t *__restrict__ erow, int32_t *__restrict__ colptr,
+    int32_t *__restrict__ rowc, int32_t *__restrict__ cval, const int32_t *__restrict__ dout,
+    const int32_t *__restrict__ din, double *__restrict__ ldout, double *__restrict__ ldin) {
+  const int32_t x = (int32_t)(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (x >= 2 * B) return;
   const int lane = threadIdx.x & 63;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int32_t *row = D + (int64_t)r * B;
-  int32_t run = rowptr[r];
+  const bool tr = x >= B;
+  const int32_t r = tr ? x - B : x;
+  const uint32_t nnz = off[B];
+  int32_t run = (int32_t)(tr ? off[x] - nnz : off[x]);
+  if (lane == 0) {
+    if (tr) {
+      colptr[r] = run;
+      if (r == B - 1) colptr[B] = (int32_t)(off[2 * B] - nnz);
+      ldin[r] = clog2((double)din[r]);
+    } else {
+      rowptr[r] = run;
+      if (r == B - 1) rowptr[B] = (int32_t)nnz;
+      ldout[r] = clog2((double)dout[r]);
+    }
+  }
+  const int32_t *row = (tr ? DT : D) + (int64_t)r * B;
+  int32_t *ok = tr ? rowc : col, *ov = tr ? cval : val;
   for (int32_t base = 0; base < B; base += 256) {
-    int32_t x[4];
+    int32_t v[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int32_t t = base + 64 * k + lane;
-      x[k] = t < B ? row[t] : 0;
+      v[k] = t < B ? row[t] : 0;
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const uint64_t m = __ballot(x[k] != 0);
-      if (x[k] != 0) {
+      const uint64_t m = __ballot(v[k] != 0);
+      if (v[k] != 0) {
         const int32_t pos = run + (int32_t)__popcll(m & lt);
-        col[pos] = base + 64 * k + lane;
-        val[pos] = x[k];
-        if (erow) erow[pos] = r;
+        ok[pos] = base + 64 * k + lane;
+        ov[pos] = v[k];
+        if (!tr) erow[pos] = r;
       }
       run += (int32_t)__popcll(m);
     }
@@ -1155,21 +1179,20 @@ static void seg_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm, Rebui
 // transpose goes through the partial-plane buffer (B x B words)
 static void dense_to_csr(int32_t B, BMBuf &bm, RebuildScratch &rs, hipStream_t st) {
   const int32_t *D = bm.dense;
-  uint32_t *cnt = rs.su;
-  const unsigned rb = (unsigned)((B + 3) / 4);
-  k_dense_rows<<<rb, 256, 0, st>>>(D, B, cnt, bm.dout);
-  scan_exclusive(cnt, (uint32_t *)bm.rowptr, B, (uint32_t *)(bm.rowptr + B), rs.tmp, st);
-  k_dense_write_rows<<<rb, 256, 0, st>>>(D, B, bm.rowptr, bm.col, bm.val, bm.erow);
   int32_t *DT = rs.partials;
   if (bm.dt_B != B) {  // else the incremental update kept the transpose current
     const unsigned tb = (unsigned)((B + 31) / 32);
     k_transpose<<<dim3(tb, tb), 256, 0, st>>>(D, B, DT);
     bm.dt_B = B;
   }
-  k_dense_rows<<<rb, 256, 0, st>>>(DT, B, cnt, bm.din);
-  scan_exclusive(cnt, (uint32_t *)bm.colptr, B, (uint32_t *)(bm.colptr + B), rs.tmp, st);
-  k_dense_write_rows<<<rb, 256, 0, st>>>(DT, B, bm.colptr, bm.row, bm.cval, nullptr);
-  k_logs<<<grid_for(B, 256), 256, 0, st>>>(bm.dout, bm.din, B, bm.ldout, bm.ldin);
+  // rows of D then rows of DT: counts, one scan over both, entries and pointers
+  uint32_t *cnt = rs.su, *off = rs.suoff;
+  const unsigned rb = (unsigned)((2 * B + 3) / 4);
+  k_dense_rows2<<<rb, 256, 0, st>>>(D, DT, B, cnt, bm.dout, bm.din);
+  scan_exclusive(cnt, off, 2 * (int64_t)B, off + 2 * (int64_t)B, rs.tmp, st);
+  k_dense_write_rows2<<<rb, 256, 0, st>>>(D, DT, B, off, bm.rowptr, bm.col, bm.val, bm.erow,
+                                          bm.colptr, bm.row, bm.cval, bm.dout, bm.din,
+                                          bm.ldout, bm.ldin);
 }
 
 static void tile_path(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,

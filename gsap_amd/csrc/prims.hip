@@ -116,6 +116,44 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_single(const uint32_t *in, uint
   if (threadIdx.x == 0 && total) *total = carry;
 }
 
+// single workgroup of 1024 threads, tiles of 8192 elements in order (n up to a few tiles)
+constexpr int SCAN_T1K = 1024, SCAN_TILE1K = SCAN_T1K * SCAN_I;
+__global__ __launch_bounds__(SCAN_T1K) void k_scan_single1k(const uint32_t *in, uint32_t *out,
+                                                            int64_t n, uint32_t *total) {
+  __shared__ uint32_t lds16[16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t carry = 0;
+  for (int64_t t0 = 0; t0 < n; t0 += SCAN_TILE1K) {
+    const int64_t base = t0 + (int64_t)threadIdx.x * SCAN_I;
+    uint32_t v[SCAN_I];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_I; ++i) {
+      v[i] = (base + i < n) ? in[base + i] : 0u;
+      s += v[i];
+    }
+    const uint32_t incl = wave_incl_scan(s);
+    if (lane == 63) lds16[w] = incl;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const uint32_t c = lds16[i];
+      if (i < w) pre += c;
+      tot += c;
+    }
+    __syncthreads();
+    uint32_t ex = carry + pre + incl - s;
+#pragma unroll
+    for (int i = 0; i < SCAN_I; ++i) {
+      if (base + i < n) out[base + i] = ex;
+      ex += v[i];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
 size_t scan_temp_words(int64_t n) { return (size_t)((n + SCAN_TILE - 1) / SCAN_TILE) + 1; }
 
 void scan_exclusive(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *total, uint32_t *tmp,
@@ -127,6 +165,10 @@ void scan_exclusive(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *tota
   }
   if (nt <= 2) {  // up to 4096 elements: one launch
     k_scan_single<<<1, SCAN_T, 0, st>>>(in, out, n, total);
+    return;
+  }
+  if (n <= 2 * SCAN_TILE1K) {  // up to 16384 elements: one launch of 1024 threads
+    k_scan_single1k<<<1, SCAN_T1K, 0, st>>>(in, out, n, total);
     return;
   }
   k_scan_reduce<<<(unsigned)nt, SCAN_T, 0, st>>>(in, n, tmp);
