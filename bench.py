@@ -55,7 +55,8 @@ def kernel_class(name):
 
 
 def summarize_profile(stats_csv, fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, bench_log,
-                      out_md, out_json, command):
+                      out_md, out_json, command,
+                      title="full partition of the 200k-vertex graph (FP64 form)"):
     """rocprofv3 CSVs -> a short markdown summary + per-class HBM bytes per launch."""
     import csv
     from collections import defaultdict
@@ -80,7 +81,7 @@ def summarize_profile(stats_csv, fetch_csv, write_csv, cal_fetch_csv, cal_write_
     f_fac = copy_bytes / (cf[copy] * 1024)
     w_fac = copy_bytes / (cw[copy] * 1024)
     tot = sum(float(r["TotalDurationNs"]) for r in stats)
-    lines = [f"# Kernel profile: full partition of the 200k-vertex graph (FP64 form)", "",
+    lines = [f"# Kernel profile: {title}", "",
              f"Command: `{command}`", "",
              f"PMC calibration on {ncopy} plain 512 MiB device copies (bench.py --calibrate-copy): "
              f"bytes read = {f_fac:.3f} x FETCH_SIZE KiB x 1024, "
@@ -110,7 +111,8 @@ def summarize_profile(stats_csv, fetch_csv, write_csv, cal_fetch_csv, cal_write_
     bench = json.loads([l for l in open(bench_log) if l.startswith("{")][-1])
     json.dump({"command": command, "fetch_factor": f_fac, "write_factor": w_fac,
                "partitions_profiled": n_part,
-               "class_launches": {k: n_part * v for k, v in bench["kernel_calls_per_step"].items()},
+               "class_launches": {k: n_part * v
+                                  for k, v in bench.get("kernel_calls_per_step", {}).items()},
                "class_bytes_total": {k: v[0] + v[1] for k, v in cls_bytes.items()},
                "class_read_total": {k: v[0] for k, v in cls_bytes.items()},
                "class_write_total": {k: v[1] for k, v in cls_bytes.items()},
@@ -202,10 +204,11 @@ def bench_multi(a, prec):
     thr, t_tune = tune(a, ctx, max(graphs, key=lambda x: x.E), opts)
     opts = dataclasses.replace(opts, **thr)
     walls, edges = [], []
+    labs = None
     for k in range(a.warmup + a.steps):
         t1 = time.perf_counter()
         if a.config == "c4":
-            _, reps = ctx.partition_batch(graphs, opts)
+            labs, reps = ctx.partition_batch(graphs, opts)
             e = sum(r.sweep_edges for r in reps)
             extra = {}
         else:
@@ -218,6 +221,27 @@ def bench_multi(a, prec):
         if k >= a.warmup:
             walls.append(w)
             edges.append(e)
+    if a.config == "c4":
+        # the nodal-move kernel over every graph of the batch: HIP events on each worker's
+        # stream around each of its launches (kernels of other graphs run beside it), in one
+        # profiled batch call of the same graphs
+        plabs, preps = ctx.partition_batch(graphs, dataclasses.replace(opts, profile=1,
+                                                                        profile_classes=1))
+        assert all(np.array_equal(x, y) for x, y in zip(plabs, labs)), "profiled batch differs"
+        n_launch = sum(r.kernel_calls[0] for r in preps)
+        nodal_ms = sum(r.kernel_ms[0] for r in preps)
+        evals = sum(r.nodal_evals for r in preps)
+        adj = sum(r.nodal_eval_adj for r in preps)
+        abytes = nodal_algorithmic_bytes(evals, adj, n_launch)
+        avg_ms = nodal_ms / max(n_launch, 1)
+        achieved = abytes / (avg_ms * 1e-3) / 1e9
+        extra["roofline"] = {"bound": "hbm", "kernel": "k_nodal_fused5 (all graphs of the batch)",
+                             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                             "avg_launch_ms": avg_ms, "launches": n_launch,
+                             "algorithmic_bytes_per_launch": abytes,
+                             "evaluated_vertices_per_launch": evals / max(n_launch, 1)}
+        extra["nodal_kernel_ms_total"] = nodal_ms
     ctx.close()
     out = {"metric": "aggregate nodal-move edges/s over the whole call" if a.config == "c4"
                      else "partition seconds of the parallel-chain run",
@@ -342,6 +366,7 @@ def main():
                              "OUT_JSON"),
                     help="turn rocprofv3 CSVs into the profiles/ summary")
     ap.add_argument("--profile-command", default="")
+    ap.add_argument("--profile-title", default="full partition of the 200k-vertex graph (FP64 form)")
     ap.add_argument("--mixed-report", action="store_true",
                     help="mixed against FP64 form on C1-C5: identity, FP32 error, rechecks")
     a = ap.parse_args()
@@ -349,7 +374,7 @@ def main():
         mixed_report()
         return
     if a.summarize_profile:
-        summarize_profile(*a.summarize_profile, command=a.profile_command)
+        summarize_profile(*a.summarize_profile, command=a.profile_command, title=a.profile_title)
         return
     if a.calibrate_copy:
         import torch
