@@ -175,11 +175,12 @@ static unsigned grid_for(int64_t n, int threads) {
 // OUT[x] = sum_t M_xt (log2 M_xt - (log2 d_out_x + log2 d_in_t))  (kernels.hpp:143-182), used
 // by the merge delta-S, and the nodal per-block neighbour sample and coin (kernels.hpp:942-997,
 // 1024-1029; one draw per block per batch, shared by its vertices).
+// (zero, when given: NCLS words cleared for the nodal proposal pass that follows; blocks
+// whose row and column hold more than big entries are sampled by k_block_walk_big)
 template <int G>
-// (zero, when given: NCLS words cleared for the nodal proposal pass that follows)
 __global__ __launch_bounds__(256) void k_block_terms(BMView m, double *OUT, int32_t *sof,
                                                      uint8_t *coin, RngKey key, PhaseRef pr,
-                                                     int sample, int32_t *zero) {
+                                                     int sample, int32_t *zero, int32_t big) {
   if (zero && blockIdx.x == 0 && threadIdx.x < NCLS) zero[threadIdx.x] = 0;
   if (OUT) log2_tables_to_lds();  // only the row terms take logarithms
   const int32_t x = (int32_t)(blockIdx.x * (256 / G) + threadIdx.x / G);
@@ -197,6 +198,8 @@ __global__ __launch_bounds__(256) void k_block_terms(BMView m, double *OUT, int3
     if (q == 0) OUT[x] = out;
   }
   if (sample) {
+    const int32_t n = (m.rowptr[x + 1] - m.rowptr[x]) + (m.colptr[x + 1] - m.colptr[x]);
+    if (n > big) return;
     const uint32_t phase = (pr.dev ? *pr.dev : 0u) + pr.add;
     const Philox y = philox((uint32_t)x, phase, key.chain, TAG_BLOCK, key.k0, key.k1);
     const int64_t deg = (int64_t)m.dout[x] + m.din[x];
@@ -208,18 +211,130 @@ __global__ __launch_bounds__(256) void k_block_terms(BMView m, double *OUT, int3
   }
 }
 
+// The neighbour sample of the blocks whose row and column hold more than big entries (the
+// few blocks that greedy merges grow large), one 1024-thread workgroup per block: rounds of
+// sixteen 1024-entry chunks, a chunk per wave summed with all its loads issued together,
+// then a prefix over the chunk sums finds the chunk where the running weight crosses the
+// target and its wave walks it.  Same entry as the sequential walk (integer sums).
+constexpr int BT_BIG = 8192;     // entries of row + column above which a block walks here
+constexpr int BT_BIG_MIN_B = 2048;  // smallest B with the separate walk
+__global__ __launch_bounds__(1024) void k_block_walk_big(BMView m, int32_t *sof, uint8_t *coin,
+                                                         RngKey key, PhaseRef pr, int32_t big) {
+  __shared__ int32_t list[1024];
+  __shared__ int32_t nlist;
+  __shared__ unsigned long long wsum[16];
+  __shared__ int32_t found;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t phase = (pr.dev ? *pr.dev : 0u) + pr.add;
+  for (int64_t base = (int64_t)blockIdx.x * 1024; base < m.B; base += (int64_t)gridDim.x * 1024) {
+    if (threadIdx.x == 0) nlist = 0;
+    __syncthreads();
+    const int64_t xt = base + threadIdx.x;
+    if (xt < m.B) {
+      const int32_t x = (int32_t)xt;
+      const int32_t n = (m.rowptr[x + 1] - m.rowptr[x]) + (m.colptr[x + 1] - m.colptr[x]);
+      if (n > big) list[atomicAdd(&nlist, 1)] = x;
+    }
+    __syncthreads();
+    const int32_t nl = nlist;
+    for (int32_t li = 0; li < nl; ++li) {
+      const int32_t x = list[li];
+      const Philox y = philox((uint32_t)x, phase, key.chain, TAG_BLOCK, key.k0, key.k1);
+      const int64_t deg = (int64_t)m.dout[x] + m.din[x];
+      const uint64_t target = mulhi(y.x0, deg);
+      const int32_t r0 = m.rowptr[x], nr = m.rowptr[x + 1] - r0;
+      const int32_t c0 = m.colptr[x], n = nr + (m.colptr[x + 1] - c0);
+      if (threadIdx.x == 0) found = 0;
+      __syncthreads();
+      uint64_t rbase = 0;  // running weight before this round
+      for (int32_t c0r = 0; c0r < n && deg > 0; c0r += 16 * 1024) {
+        // this wave's chunk: entries c0r + w * 1024 + u * 64 + lane
+        const int32_t k0 = c0r + w * 1024;
+        uint32_t wt[16];
+        int32_t id[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int32_t k = k0 + u * 64 + lane;
+          wt[u] = 0;
+          id[u] = 0;
+          if (k < nr) { wt[u] = (uint32_t)m.val[r0 + k]; id[u] = m.col[r0 + k]; }
+          else if (k < n) { wt[u] = (uint32_t)m.cval[c0 + k - nr]; id[u] = m.row[c0 + k - nr]; }
+        }
+        uint64_t s = 0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) s += wt[u];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (lane == 0) wsum[w] = s;
+        __syncthreads();
+        uint64_t pre = rbase, tot = 0;
+        for (int i = 0; i < 16; ++i) {
+          const uint64_t c = wsum[i];
+          if (i < w) pre += c;
+          tot += c;
+        }
+        // the chunk where the running weight first exceeds the target
+        if (pre <= target && target < pre + s) {
+          uint64_t b = pre;
+#pragma unroll
+          for (int u = 0; u < 16; ++u) {
+            uint64_t incl = wt[u];
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+              const uint64_t yv = __shfl_up(incl, off, 64);
+              if (lane >= off) incl += yv;
+            }
+            const uint64_t bal = __ballot(b + incl > target);
+            if (bal) {
+              const int32_t pick = __shfl(id[u], __ffsll((long long)bal) - 1, 64);
+              if (lane == 0) {
+                sof[x] = pick;
+                found = 1;
+              }
+              break;
+            }
+            b += __shfl(incl, 63, 64);
+          }
+        }
+        __syncthreads();
+        const int32_t done = found;
+        rbase += tot;
+        __syncthreads();  // wsum is rewritten by the next round
+        if (done) break;
+      }
+      if (threadIdx.x == 0) {
+        if (deg == 0) sof[x] = 0;
+        coin[x] = coin_random(y.x1, deg, m.B) ? 1 : 0;
+      }
+      __syncthreads();
+    }
+    __syncthreads();  // the list is rewritten by the next range of blocks
+  }
+}
+
 void launch_block_terms(const BMView &m, double *OUT, int32_t *sof, uint8_t *coin, RngKey key,
                         PhaseRef phase, bool sample, int32_t avg_row, hipStream_t st,
                         int32_t *zero) {
   const int s = sample ? 1 : 0;
+  // the samples of very long rows and columns walk in a kernel of their own (B >= 2048)
+  const bool sep = sample && m.B >= BT_BIG_MIN_B;
+  const int32_t big = sep ? BT_BIG : INT32_MAX;
   if (avg_row <= 12)
-    k_block_terms<8><<<grid_for(m.B, 32), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s, zero);
+    k_block_terms<8><<<grid_for(m.B, 32), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s, zero,
+                                                        big);
   else if (avg_row <= 24)
-    k_block_terms<16><<<grid_for(m.B, 16), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s, zero);
+    k_block_terms<16><<<grid_for(m.B, 16), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s, zero,
+                                                         big);
   else if (avg_row <= 48)
-    k_block_terms<32><<<grid_for(m.B, 8), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s, zero);
+    k_block_terms<32><<<grid_for(m.B, 8), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s, zero,
+                                                        big);
   else
-    k_block_terms<64><<<grid_for(m.B, 4), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s, zero);
+    k_block_terms<64><<<grid_for(m.B, 4), 256, 0, st>>>(m, OUT, sof, coin, key, phase, s, zero,
+                                                        big);
+  if (sep) {
+    const unsigned gb = (unsigned)std::min<int64_t>(1024, ((int64_t)m.B + 1023) / 1024);
+    k_block_walk_big<<<gb, 1024, 0, st>>>(m, sof, coin, key, phase, big);
+  }
 }
 
 // ------------------------------------------------------------- nodal moves
