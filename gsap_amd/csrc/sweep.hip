@@ -115,6 +115,48 @@ GSAP_D void seg_for(int32_t n, F &&f) {
   }
 }
 
+// seg_for with the loads batched: P entries per lane at a time (in the lane's entry order),
+// first a(k) for all P (the entry's own loads; k clamped into [0, n) so every load is in
+// bounds), then b(a(k)) for all P (loads that depend on the entry), then c(a, b, j) for the
+// entries below n in order.  Per slot j the entries arrive in the order seg_for gives them,
+// so canonical sums come out the same; the batching only lets the loads overlap.
+// an entry's looked-up count and log2 degree
+struct GL {
+  int32_t g;
+  double l;
+};
+
+template <int G, int P, class A, class Bf, class C>
+GSAP_D void seg_for3(int32_t n, A &&a, Bf &&b, C &&c) {
+  constexpr int S = 64 / G;                   // entries per lane in a 64-entry chunk
+  constexpr int CH = P >= S ? P / S : 1;      // chunks per step
+  constexpr int NB = P >= S ? 1 : S / P;      // batches per step
+  const int q = threadIdx.x & (G - 1);
+  using TA = decltype(a(0));
+  using TB = decltype(b(a(0)));
+  for (int32_t base = 0; base < n; base += 64 * CH) {
+#pragma unroll
+    for (int bb = 0; bb < NB; ++bb) {
+      TA x[P];
+      TB y[P];
+#pragma unroll
+      for (int e = 0; e < P; ++e) {
+        const int el = bb * P + e;
+        const int32_t k = base + 64 * (el / S) + q + (el % S) * G;
+        x[e] = a(k < n ? k : n - 1);
+      }
+#pragma unroll
+      for (int e = 0; e < P; ++e) y[e] = b(x[e]);
+#pragma unroll
+      for (int e = 0; e < P; ++e) {
+        const int el = bb * P + e;
+        const int32_t k = base + 64 * (el / S) + q + (el % S) * G;
+        if (k < n) c(x[e], y[e], el % S);
+      }
+    }
+  }
+}
+
 // Group-parallel form of bm_walk: entries of row x then column x in chunks of G lanes, an
 // inclusive prefix sum of the weights inside the group, and the first lane whose running
 // weight exceeds target with an id other than skip.  Same result as the sequential walk.
@@ -1765,86 +1807,83 @@ GSAP_D void merge_one(const BMView &m, const double *OUT, RngKey key, uint32_t p
     }
     return;
   }
-  // old column terms excluding t in {r, s}
-  double inx_r, inx_s;
+  // Column r: its old terms and the merged column's terms over its entries (t not in
+  // {r, s}); column s: its old terms and the merged column's terms of entries absent from
+  // column r.  Then the merged row: row r's entries (M_rt + M_st), and row s's entries
+  // absent from row r.  Four entries per lane (two in groups of 8) are loaded before any is
+  // summed.
+  constexpr int P = G >= 16 ? 4 : 2;
+  const double ldo = clog2_s((double)((int64_t)m.dout[r] + m.dout[s]));
+  const double ldi = clog2_s((double)((int64_t)m.din[r] + m.din[s]));
+  double inx_r, inx_s, ia_, ib_;
   {
-    CAcc<G> acc;
-    acc.init();
+    CAcc<G> ox, na;
+    ox.init();
+    na.init();
     const int32_t b0 = m.colptr[r], n = m.colptr[r + 1] - b0;
     const double li = m.ldin[r];
-    seg_for<G>(n, [&](int32_t k, int j) {
-      const int32_t t = m.row[b0 + k];
-      if (t != r && t != s) acc.p[j] += term_s(m.cval[b0 + k], m.ldout[t], li);
-    });
-    inx_r = acc.reduce();
+    seg_for3<G, P>(
+        n, [&](int32_t k) { return make_int2(m.row[b0 + k], m.cval[b0 + k]); },
+        [&](int2 e) { return GL{bm_col_get(m, s, e.x), m.ldout[e.x]}; },
+        [&](int2 e, GL g, int j) {
+          if (e.x == r || e.x == s) return;
+          ox.p[j] += term_s(e.y, g.l, li);
+          na.p[j] += term_s((int64_t)e.y + g.g, g.l, ldi);
+        });
+    inx_r = ox.reduce();
+    ia_ = na.reduce();
   }
   {
-    CAcc<G> acc;
-    acc.init();
+    CAcc<G> ox, nb;
+    ox.init();
+    nb.init();
     const int32_t b0 = m.colptr[s], n = m.colptr[s + 1] - b0;
     const double li = m.ldin[s];
-    seg_for<G>(n, [&](int32_t k, int j) {
-      const int32_t t = m.row[b0 + k];
-      if (t != r && t != s) acc.p[j] += term_s(m.cval[b0 + k], m.ldout[t], li);
-    });
-    inx_s = acc.reduce();
+    seg_for3<G, P>(
+        n, [&](int32_t k) { return make_int2(m.row[b0 + k], m.cval[b0 + k]); },
+        [&](int2 e) { return GL{s != r ? bm_col_get(m, r, e.x) : 1, m.ldout[e.x]}; },
+        [&](int2 e, GL g, int j) {
+          if (e.x == r || e.x == s) return;
+          ox.p[j] += term_s(e.y, g.l, li);
+          if (g.g == 0) nb.p[j] += term_s(e.y, g.l, ldi);
+        });
+    inx_s = ox.reduce();
+    ib_ = nb.reduce();
   }
-  // merged row: row r entries (M_rt + M_st), then row s entries absent from row r
-  const double ldo = clog2_s((double)((int64_t)m.dout[r] + m.dout[s]));
   double new_out;
   {
     CAcc<G> na, nb;
     na.init();
     nb.init();
     const int32_t b0 = m.rowptr[r], n = m.rowptr[r + 1] - b0;
-    seg_for<G>(n, [&](int32_t k, int j) {
-      const int32_t t = m.col[b0 + k];
-      if (exact && (t == r || t == s)) return;
-      na.p[j] += term_s((int64_t)m.val[b0 + k] + bm_row_get(m, s, t), m.ldin[t], ldo);
-    });
+    seg_for3<G, P>(
+        n, [&](int32_t k) { return make_int2(m.col[b0 + k], m.val[b0 + k]); },
+        [&](int2 e) { return GL{bm_row_get(m, s, e.x), m.ldin[e.x]}; },
+        [&](int2 e, GL g, int j) {
+          if (exact && (e.x == r || e.x == s)) return;
+          na.p[j] += term_s((int64_t)e.y + g.g, g.l, ldo);
+        });
     if (s != r) {
       const int32_t c0 = m.rowptr[s], c = m.rowptr[s + 1] - c0;
-      seg_for<G>(c, [&](int32_t k, int j) {
-        const int32_t t = m.col[c0 + k];
-        if (exact && (t == r || t == s)) return;
-        if (bm_row_get(m, r, t) == 0) nb.p[j] += term_s(m.val[c0 + k], m.ldin[t], ldo);
-      });
+      seg_for3<G, P>(
+          c, [&](int32_t k) { return make_int2(m.col[c0 + k], m.val[c0 + k]); },
+          [&](int2 e) { return GL{bm_row_get(m, r, e.x), m.ldin[e.x]}; },
+          [&](int2 e, GL g, int j) {
+            if (exact && (e.x == r || e.x == s)) return;
+            if (g.g == 0) nb.p[j] += term_s(e.y, g.l, ldo);
+          });
     }
     const double a_ = na.reduce();
     const double b_ = nb.reduce();
     new_out = a_ + b_;
   }
-  // merged column, t not in {r, s}
-  const double ldi = clog2_s((double)((int64_t)m.din[r] + m.din[s]));
   if (exact) {
     // the folded corner M_rr + M_rs + M_sr + M_ss
     const int64_t X = (int64_t)bm_row_get(m, r, r) + bm_row_get(m, r, s) + bm_row_get(m, s, r) +
                       bm_row_get(m, s, s);
     new_out = new_out + (X > 0 ? term_s(X, ldi, ldo) : 0.0);
   }
-  double new_in;
-  {
-    CAcc<G> ia, ib;
-    ia.init();
-    ib.init();
-    const int32_t b0 = m.colptr[r], n = m.colptr[r + 1] - b0;
-    seg_for<G>(n, [&](int32_t k, int j) {
-      const int32_t t = m.row[b0 + k];
-      if (t != r && t != s)
-        ia.p[j] += term_s((int64_t)m.cval[b0 + k] + bm_col_get(m, s, t), m.ldout[t], ldi);
-    });
-    if (s != r) {
-      const int32_t c0 = m.colptr[s], c = m.colptr[s + 1] - c0;
-      seg_for<G>(c, [&](int32_t k, int j) {
-        const int32_t t = m.row[c0 + k];
-        if (t != r && t != s && bm_col_get(m, r, t) == 0)
-          ib.p[j] += term_s(m.cval[c0 + k], m.ldout[t], ldi);
-      });
-    }
-    const double a_ = ia.reduce();
-    const double b_ = ib.reduce();
-    new_in = a_ + b_;
-  }
+  const double new_in = ia_ + ib_;
   if (q == 0) {
     double d = OUT[r] + OUT[s];
     d = d + (inx_r + inx_s);
@@ -1903,8 +1942,9 @@ GSAP_D int32_t lds_find(const int32_t *keys, int32_t n, int32_t t) {
 // row-r terms); column s likewise against column r.  The random lookups of merge_one become
 // LDS work.  Same positions, terms and canonical sums as merge_one, hence the same bits; the
 // proposals of a block whose row and column exceed CAP entries are left to k_merge_list.
+// (k_merge_blk_g: the same with groups of G lanes, for short rows.)
 template <int CAP>
-__global__ __launch_bounds__(256) void k_merge_blk(BMView m, const double *OUT, RngKey key,
+__global__ __launch_bounds__(256) void k_merge_blk_w(BMView m, const double *OUT, RngKey key,
                                                    uint32_t phase, int32_t np, int32_t *prop,
                                                    double *dSout, int exact) {
   __shared__ int32_t sk[CAP], sv[CAP];  // row r (columns, values), then column r (rows, values)
@@ -2036,6 +2076,171 @@ __global__ __launch_bounds__(256) void k_merge_blk(BMView m, const double *OUT, 
   }
 }
 
+// k_merge_blk_w with one group of G lanes per proposal (r, i), i = group, group + NT / G,
+// ...: groups of 16 lanes keep several proposals of a block with short rows in flight in one
+// wave.  Same positions, terms and canonical sums, hence the same bits.
+template <int G, int CAP, int NT, int P, int W>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) void k_merge_blk_g(BMView m, const double *OUT, RngKey key,
+                                                   uint32_t phase, int32_t np, int32_t *prop,
+                                                   double *dSout, int exact) {
+  constexpr int NG = NT / G;
+  __shared__ int32_t sk[CAP], sv[CAP];  // row r (columns, values), then column r (rows, values)
+  __shared__ int32_t mt[NG][CAP];       // per group: M_st at row r's positions, M_ts at column r's
+  log2_tables_to_lds();
+  const int32_t r = (int32_t)blockIdx.x;
+  const int32_t B = m.B;
+  const int grp = threadIdx.x / G, q = threadIdx.x & (G - 1);
+  const int gbase = (threadIdx.x & 63) & ~(G - 1);
+  const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+  const int32_t r0 = m.rowptr[r], nr = m.rowptr[r + 1] - r0;
+  const int32_t c0 = m.colptr[r], nc = m.colptr[r + 1] - c0;
+  const int32_t n = nr + nc;
+  if (n > CAP) return;  // listed by k_big_blocks, one wave per proposal
+  for (int32_t k = threadIdx.x; k < nr; k += NT) {
+    sk[k] = m.col[r0 + k];
+    sv[k] = m.val[r0 + k];
+  }
+  for (int32_t k = threadIdx.x; k < nc; k += NT) {
+    sk[nr + k] = m.row[c0 + k];
+    sv[nr + k] = m.cval[c0 + k];
+  }
+  __syncthreads();
+  const int32_t *rk = sk, *ck = sk + nr;
+  int32_t *mw = mt[grp];
+  const int64_t deg = (int64_t)m.dout[r] + m.din[r];
+  const double li_r = m.ldin[r];
+  for (int32_t i = grp; i < np; i += NG) {
+    const int64_t idx = (int64_t)r * np + i;
+    const Philox x = philox((uint32_t)idx, phase, key.chain, TAG_MERGE, key.k0, key.k1);
+    int32_t s;
+    if (deg == 0) {
+      s = (int32_t)mulhi(x.x0, B);
+    } else {
+      // u: first entry of row r then column r (staged) whose running weight exceeds target
+      const uint64_t target = mulhi(x.x1, deg);
+      int32_t u = 0;
+      uint64_t base = 0;
+      for (int32_t c = 0; c < n; c += G) {
+        const int32_t k = c + q;
+        uint64_t incl = k < n ? (uint64_t)sv[k] : 0;
+#pragma unroll
+        for (int off = 1; off < G; off <<= 1) {
+          const uint64_t y = __shfl_up(incl, off, G);
+          if (q >= off) incl += y;
+        }
+        const uint64_t bal = (__ballot(k < n && base + incl > target) >> gbase) & gmask;
+        if (bal) {
+          u = sk[c + __ffsll((long long)bal) - 1];
+          break;
+        }
+        base += __shfl(incl, G - 1, G);
+      }
+      const int64_t degu = (int64_t)m.dout[u] + m.din[u];
+      if (coin_random(x.x3, degu, B)) s = (int32_t)mulhi(x.x0, B);
+      else s = group_walk<G>(m, u, mulhi(x.x2, degu), r, 0);
+    }
+    if (exact && s == r) {
+      if (q == 0) {
+        prop[idx] = s;
+        dSout[idx] = 1.7976931348623157e308;
+      }
+      continue;
+    }
+    wave_lds_sync();  // the previous proposal's reads of the match array are done
+    for (int32_t k = q; k < n; k += G) mw[k] = s == r ? sv[k] : 0;
+    wave_lds_sync();
+    const double ldo = clog2_s((double)((int64_t)m.dout[r] + m.dout[s]));
+    const double ldi = clog2_s((double)((int64_t)m.din[r] + m.din[s]));
+    const double li_s = m.ldin[s];
+    // row s: the terms of entries absent from row r, and M_st at row r's positions
+    double b_ = 0.0;
+    if (s != r) {
+      CAcc<G> nb;
+      nb.init();
+      const int32_t s0 = m.rowptr[s], ns = m.rowptr[s + 1] - s0;
+      seg_for3<G, P>(
+          ns, [&](int32_t k) { return make_int2(m.col[s0 + k], m.val[s0 + k]); },
+          [&](int2 e) { return m.ldin[e.x]; },
+          [&](int2 e, double l, int j) {
+            const int32_t pos = lds_find(rk, nr, e.x);
+            if (pos >= 0) mw[pos] = e.y;
+            else if (!(exact && (e.x == r || e.x == s))) nb.p[j] += term_s(e.y, l, ldo);
+          });
+      b_ = nb.reduce();
+    }
+    // column s: its old terms (t not in {r, s}), the terms of entries absent from column r,
+    // and M_ts at column r's positions
+    double inx_s, ib_;
+    {
+      CAcc<G> ox, nb;
+      ox.init();
+      nb.init();
+      const int32_t cs0 = m.colptr[s], ncs = m.colptr[s + 1] - cs0;
+      seg_for3<G, P>(
+          ncs, [&](int32_t k) { return make_int2(m.row[cs0 + k], m.cval[cs0 + k]); },
+          [&](int2 e) { return m.ldout[e.x]; },
+          [&](int2 e, double l, int j) {
+            const bool ex = e.x == r || e.x == s;
+            if (!ex) ox.p[j] += term_s(e.y, l, li_s);
+            if (s != r) {
+              const int32_t pos = lds_find(ck, nc, e.x);
+              if (pos >= 0) mw[nr + pos] = e.y;
+              else if (!ex) nb.p[j] += term_s(e.y, l, ldi);
+            }
+          });
+      inx_s = ox.reduce();
+      ib_ = nb.reduce();
+    }
+    wave_lds_sync();
+    // merged row over row r's entries, old column r and merged column over column r's
+    double a_;
+    {
+      CAcc<G> na;
+      na.init();
+      seg_for3<G, P>(
+          nr, [&](int32_t k) { return make_int2(rk[k], k); },
+          [&](int2 e) { return m.ldin[e.x]; },
+          [&](int2 e, double l, int j) {
+            if (exact && (e.x == r || e.x == s)) return;
+            na.p[j] += term_s((int64_t)sv[e.y] + mw[e.y], l, ldo);
+          });
+      a_ = na.reduce();
+    }
+    double inx_r, ia_;
+    {
+      CAcc<G> ox, na;
+      ox.init();
+      na.init();
+      seg_for3<G, P>(
+          nc, [&](int32_t k) { return make_int2(ck[k], k); },
+          [&](int2 e) { return m.ldout[e.x]; },
+          [&](int2 e, double l, int j) {
+            if (e.x == r || e.x == s) return;
+            const int32_t v = sv[nr + e.y];
+            ox.p[j] += term_s(v, l, li_r);
+            na.p[j] += term_s((int64_t)v + mw[nr + e.y], l, ldi);
+          });
+      inx_r = ox.reduce();
+      ia_ = na.reduce();
+    }
+    if (q == 0) {
+      double new_out = a_ + b_;
+      if (exact) {
+        const int64_t X = (int64_t)bm_row_get(m, r, r) + bm_row_get(m, r, s) +
+                          bm_row_get(m, s, r) + bm_row_get(m, s, s);
+        new_out = new_out + (X > 0 ? term_s(X, ldi, ldo) : 0.0);
+      }
+      const double new_in = ia_ + ib_;
+      double d = OUT[r] + OUT[s];
+      d = d + (inx_r + inx_s);
+      d = d - new_out;
+      d = d - new_in;
+      prop[idx] = s;
+      dSout[idx] = d;
+    }
+  }
+}
+
 void launch_merge(const BMView &m, const double *OUT, RngKey key, uint32_t phase, int32_t np,
                   int32_t *prop, double *dS, int32_t avg_row, bool exact, int32_t *big_list,
                   uint32_t *big_count, hipStream_t st) {
@@ -2043,13 +2248,26 @@ void launch_merge(const BMView &m, const double *OUT, RngKey key, uint32_t phase
   const int ex = exact ? 1 : 0;
   if (avg_row > 12 && m.B >= 4096 && big_list) {
     // long rows and enough blocks to fill the device: a workgroup per block with its row and
-    // column in LDS; the few blocks beyond the staging arrays get a wave per proposal (their
-    // count comes back to the host to size that launch)
-    const int32_t cap = avg_row <= 96 ? 512 : 1024;
-    if (cap == 512)
-      k_merge_blk<512><<<(unsigned)m.B, 256, 0, st>>>(m, OUT, key, phase, np, prop, dS, ex);
-    else
-      k_merge_blk<1024><<<(unsigned)m.B, 256, 0, st>>>(m, OUT, key, phase, np, prop, dS, ex);
+    // column in LDS (short rows: groups of 16 lanes, twelve proposals in flight per
+    // workgroup, 80 registers at six waves per SIMD; longer rows: a wave per proposal); the
+    // few blocks beyond the staging arrays get a wave per proposal (their count comes back to
+    // the host to size that launch)
+    int32_t cap;
+    if (avg_row <= 32) {
+      cap = 256;
+      k_merge_blk_g<16, 256, 192, 1, 6><<<(unsigned)m.B, 192, 0, st>>>(m, OUT, key, phase, np,
+                                                                        prop, dS, ex);
+    } else if (avg_row <= 40) {
+      cap = 384;
+      k_merge_blk_g<16, 384, 192, 1, 6><<<(unsigned)m.B, 192, 0, st>>>(m, OUT, key, phase, np,
+                                                                        prop, dS, ex);
+    } else if (avg_row <= 96) {
+      cap = 512;
+      k_merge_blk_w<512><<<(unsigned)m.B, 256, 0, st>>>(m, OUT, key, phase, np, prop, dS, ex);
+    } else {
+      cap = 1024;
+      k_merge_blk_w<1024><<<(unsigned)m.B, 256, 0, st>>>(m, OUT, key, phase, np, prop, dS, ex);
+    }
     uint32_t nbig = 0;
     if (hipMemsetAsync(big_count, 0, 4, st) != hipSuccess) return;
     k_big_blocks<<<grid_for(m.B, 256), 256, 0, st>>>(m, cap, big_list, big_count);
