@@ -105,7 +105,8 @@ EXPORTS = [
     "gsap_pairwise_i32", "gsap_log2", "gsap_log2_tables", "gsap_partition_batch_i32",
     "gsap_partition_chains_i32", "gsap_log2f_n", "gsap_quality_i32", "gsap_save_csr_i32",
     "gsap_load_csr_i32", "gsap_autotune_i32", "gsap_golden_init", "gsap_golden_step",
-    "gsap_converged", "gsap_mdl_noalloc_i32",
+    "gsap_converged", "gsap_mdl_noalloc_i32", "gsap_upload_graph_i32",
+    "gsap_partition_resident_i32",
 ]
 
 # the option fields the autotuner writes
@@ -132,6 +133,9 @@ def lib() -> C.CDLL:
     L.gsap_last_error.restype = C.c_char_p
     L.gsap_partition_i32.argtypes = [vp, C.POINTER(CGraph), C.POINTER(COptions), _i32p,
                                      C.POINTER(CReport)]
+    L.gsap_upload_graph_i32.argtypes = [vp, C.POINTER(CGraph)]
+    L.gsap_partition_resident_i32.argtypes = [vp, C.POINTER(COptions), C.c_int64, _i32p,
+                                              C.POINTER(CReport)]
     L.gsap_get_trace.argtypes = [vp, C.POINTER(CTrace), i64, C.POINTER(i64)]
     L.gsap_partition_batch_i32.argtypes = [vp, C.POINTER(CGraph), i32, C.POINTER(COptions),
                                            C.POINTER(C.c_void_p), C.POINTER(CReport)]
@@ -337,6 +341,22 @@ class Context:
         rep = CReport()
         cg, co = g.cstruct(), opts.cstruct()
         _check(lib().gsap_partition_i32(self._h, C.byref(cg), C.byref(co), out, C.byref(rep)),
+               self._h)
+        return out, Report.from_c(rep)
+
+    def upload_graph(self, g: Graph) -> None:
+        """Copies g into the context's device memory (the first step of partition())."""
+        cg = g.cstruct()
+        _check(lib().gsap_upload_graph_i32(self._h, C.byref(cg)), self._h)
+
+    def partition_resident(self, N: int, opts: Options = Options()):
+        """Whole partition of the N-vertex graph resident on this context (left by
+        upload_graph() or by any call that took a graph), with no host-to-device copy of
+        it; the same result as partition() on that graph."""
+        out = np.empty(N, dtype=np.int32)
+        rep = CReport()
+        co = opts.cstruct()
+        _check(lib().gsap_partition_resident_i32(self._h, C.byref(co), N, out, C.byref(rep)),
                self._h)
         return out, Report.from_c(rep)
 

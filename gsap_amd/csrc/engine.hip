@@ -232,6 +232,10 @@ struct gsap_ctx {
   uint32_t pub_seq = 0;
   // pinned staging of the merge phase's arrays (delta-S, best partner, order, merge forest)
   HPin stage;
+  // the uploaded graph's row pointers on the host (the batch lists are built from them) and
+  // whether a graph is resident for gsap_partition_resident_i32
+  std::vector<int32_t> hptr_out, hptr_in;
+  bool resident = false;
 };
 
 namespace {
@@ -332,8 +336,11 @@ void h2d(DBuf &b, const T *src, int64_t n, hipStream_t st) {
 
 void upload_graph(gsap_ctx *c, const gsap_graph_i32 *g) {
   validate(g);
+  c->resident = false;
   const int64_t N = g->N, E = g->E;
   c->N = N; c->E = E;
+  c->hptr_out.assign(g->out_ptr, g->out_ptr + N + 1);
+  c->hptr_in.assign(g->in_ptr, g->in_ptr + N + 1);
   hipStream_t st = c->st;
   std::vector<int32_t> kout(N, 0), kin(N, 0), sloop(N, 0);
   const int nt = host_threads();
@@ -395,6 +402,17 @@ void upload_graph(gsap_ctx *c, const gsap_graph_i32 *g) {
   d.wmax = *std::max_element(wmx.begin(), wmx.end());
   d.hoff = c->hoff.as<int64_t>(); d.hbits_o = c->hbo.as<uint8_t>(); d.hbits_i = c->hbi.as<uint8_t>();
   d.hpool = c->hpool.as<int32_t>();
+  c->resident = true;
+}
+
+// The resident graph's row pointers as a host graph view (what build_batch_lists reads)
+gsap_graph_i32 resident_ptrs(const gsap_ctx *c) {
+  gsap_graph_i32 h{};
+  h.N = c->N;
+  h.E = c->E;
+  h.out_ptr = c->hptr_out.data();
+  h.in_ptr = c->hptr_in.data();
+  return h;
 }
 
 // Vertex lists of every nodal batch by class of max(out, in) neighbour count: <= 16, <= 32,
@@ -925,8 +943,11 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
   check_options(o);
   const double t0 = now();
   set_device(c);
-  upload_graph(c, gin);
-  build_batch_lists(c, gin, o.num_batches, resolve_nodal(o));
+  // gin == nullptr: the graph a previous upload left on the device
+  if (gin) upload_graph(c, gin);
+  else if (!c->resident) throw Err(GSAP_ERR_INVALID, "no graph uploaded to this context");
+  const gsap_graph_i32 hp = resident_ptrs(c);
+  build_batch_lists(c, &hp, o.num_batches, resolve_nodal(o));
   alloc_work(c, o.num_proposals, resolve_dense(o));
   apply_seg_classes(c, o);
   c->tile_max_B = resolve_tile(o);
@@ -1292,6 +1313,27 @@ int gsap_partition_i32(gsap_ctx *c, const gsap_graph_i32 *g, const gsap_options 
     if (!partitions_out) throw Err(GSAP_ERR_INVALID, "partitions_out is NULL");
     gsap_options oo = o ? *o : defaults();
     partition_impl(c, g, oo, partitions_out, report);
+  });
+}
+
+int gsap_upload_graph_i32(gsap_ctx *c, const gsap_graph_i32 *g) {
+  return guarded(c, [&] {
+    if (!g) throw Err(GSAP_ERR_INVALID, "graph is NULL");
+    set_device(c);
+    upload_graph(c, g);
+  });
+}
+
+int gsap_partition_resident_i32(gsap_ctx *c, const gsap_options *o, int64_t N,
+                                int32_t *partitions_out, gsap_report *report) {
+  return guarded(c, [&] {
+    if (!partitions_out) throw Err(GSAP_ERR_INVALID, "partitions_out is NULL");
+    if (!c->resident) throw Err(GSAP_ERR_INVALID, "no graph uploaded to this context");
+    if (N != c->N)
+      throw Err(GSAP_ERR_INVALID, "N (" + std::to_string(N) + ") is not the resident graph's " +
+                                      std::to_string(c->N) + " vertices");
+    gsap_options oo = o ? *o : defaults();
+    partition_impl(c, nullptr, oo, partitions_out, report);
   });
 }
 

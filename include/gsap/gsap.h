@@ -154,7 +154,8 @@ typedef struct gsap_report {
                           /* 6 nodal proposal pass (unit-weight graphs)                  */
   int64_t fp64_evals;     /* mixed form: nodal proposals evaluated in FP64 (rechecks plus
                              every accepted move, whose exact delta-S enters the batch sum) */
-  double t_upload;        /* graph validation, host preparation and upload */
+  double t_upload;        /* graph validation, host preparation and upload (resident
+                           * graph: the batch lists and workspace only) */
   int64_t nodal_evals;    /* options.profile = 1 on a unit-weight graph: nodal proposals of
                              another block (the moves kernel evaluates these), else -1 */
   int64_t nodal_eval_adj; /* their out+in adjacency entries, summed over batches, else -1 */
@@ -184,6 +185,20 @@ const char *gsap_last_error(const gsap_ctx *ctx);
  * 0-based block ids.  report may be NULL. */
 int gsap_partition_i32(gsap_ctx *ctx, const gsap_graph_i32 *g, const gsap_options *o,
                        int32_t *partitions_out, gsap_report *report);
+
+/* Device-resident graph.  No counterpart in the reference, whose partition() copies the
+ * graph to the device inside the call (partition.hpp:852-870, allocate_device.hpp:4-194);
+ * for callers that partition a graph already in device memory.  gsap_upload_graph_i32
+ * validates g and copies it into the context (the first step of gsap_partition_i32);
+ * gsap_partition_resident_i32 then runs the whole partition on that copy, with no
+ * host-to-device transfer of the graph, and gives the result gsap_partition_i32 gives on
+ * the same graph and options.  N is the length of partitions_out and must equal the
+ * resident graph's vertex count.  Every call on the context that takes a graph replaces
+ * the resident copy; without one, gsap_partition_resident_i32 fails with
+ * GSAP_ERR_INVALID. */
+int gsap_upload_graph_i32(gsap_ctx *ctx, const gsap_graph_i32 *g);
+int gsap_partition_resident_i32(gsap_ctx *ctx, const gsap_options *o, int64_t N,
+                                int32_t *partitions_out, gsap_report *report);
 
 /* Batched form: n_graphs independent graphs (uneven sizes allowed).  They run on a pool of
  * o->workers sub-contexts, each with its own HIP stream and workspace, so the kernels of

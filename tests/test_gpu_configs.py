@@ -3,7 +3,8 @@
 
   C3  the 200k-vertex graph: the whole trace (every merge phase, nodal batch, sweep and
       golden-section step: proposals, labels, block matrix, delta-S sums, S) and the final
-      labels, B and S, bit for bit; the untraced call gives the same result
+      labels, B and S, bit for bit; the untraced call and the partition of the resident
+      graph (no second upload) give the same result
   C4  the 256 batched graphs through gsap_partition_batch_i32: every graph's labels, B, S,
       levels and sweeps equal the oracle's; S equals gsap_mdl_i32 of the final labels
   C5  the 2M-vertex graph with chains 0 and 1 through gsap_partition_chains_i32: chain 0's
@@ -55,6 +56,9 @@ def test_c3_partition_matches_oracle(ctx):
     print("C3 untraced partition", flush=True)
     lab2, rep2 = ctx.partition(g, Options(seed=1))
     assert labels_digest(lab2) == d["labels"] and f64_hex(rep2.S) == d["S"]
+    print("C3 partition of the resident graph", flush=True)
+    lab3, rep3 = ctx.partition_resident(g.N, Options(seed=1))
+    assert labels_digest(lab3) == d["labels"] and f64_hex(rep3.S) == d["S"]
 
 
 def test_c4_batch_matches_oracle(ctx):

@@ -432,6 +432,24 @@ def test_rejects_malformed_graphs(ctx):
     assert np.array_equal(lab, rlab)
 
 
+def test_partition_resident(ctx):
+    """gsap_upload_graph_i32 + gsap_partition_resident_i32: the oracle's partition, twice in
+    a row from the one upload; a call that takes another graph replaces the resident copy;
+    a wrong N is an error and the context stays usable."""
+    g, h = graph("LL1k"), graph("hubs")
+    o = Options(seed=4)
+    rlab, rrep, _ = O.partition(g, o)
+    ctx.upload_graph(g)
+    for _ in range(2):
+        lab, rep = ctx.partition_resident(g.N, o)
+        assert np.array_equal(lab, rlab) and eq_bits(rep.S, rrep.S) and rep.B == rrep.B
+    with pytest.raises(gsap_amd.GsapError, match="resident graph"):
+        ctx.partition_resident(g.N + 1, o)
+    hlab, hrep = ctx.partition(h, o)
+    lab, rep = ctx.partition_resident(h.N, o)
+    assert np.array_equal(lab, hlab) and eq_bits(rep.S, hrep.S)
+
+
 def test_partition_batch(ctx):
     graphs = [graph("tiny_w"), graph("hubs"), graph("LL1k"),
               gsap_amd.generate_dcsbm(1000, 11, 0.0, 4.0, 20.0, seed=1), graph("rand")]
