@@ -229,7 +229,7 @@ void launch_mdl_sum(const BMView &m, const int32_t *erow, int64_t n_max, double 
 // serve the workgroup-per-block form at large B (it synchronizes the stream once); without
 // them every proposal gets a group of lanes.
 void launch_merge(const BMView &m, const double *OUT, RngKey key, uint32_t phase, int32_t np,
-                  int32_t *prop, double *dS, int32_t avg_row, bool exact, int32_t *big_list,
+                  int32_t *prop, double *dS, int64_t nnz, bool exact, int32_t *big_list,
                   uint32_t *big_count, hipStream_t st);
 void launch_argmin(const double *dS, const int32_t *prop, int32_t B, int32_t np, int32_t *best,
                    double *dS_best, hipStream_t st);

@@ -725,14 +725,15 @@ int32_t merge_phase(gsap_ctx *c, const gsap_options &o, int32_t B_to_merge, uint
   const int32_t B = c->bm.B, np = o.num_proposals;
   hipStream_t st = c->st;
   BMView m = c->bm.view();
-  int32_t ar = avg_row(read_nnz(c), B);
+  const int64_t nnz = read_nnz(c);
+  int32_t ar = avg_row(nnz, B);
   int p = c->prof.begin(2, st);
   launch_block_terms(m, c->OUT.as<double>(), nullptr, nullptr, rng_key(o), PhaseRef{nullptr, phase},
                      false, ar, st);
   c->prof.end(p, st);
   p = c->prof.begin(3, st);
   launch_merge(m, c->OUT.as<double>(), rng_key(o), phase, np, c->prop.as<int32_t>(),
-               c->dS.as<double>(), ar, o.delta_mode == GSAP_DELTA_EXACT, c->sof.as<int32_t>(),
+               c->dS.as<double>(), nnz, o.delta_mode == GSAP_DELTA_EXACT, c->sof.as<int32_t>(),
                c->ctr.as<uint32_t>() + 1, st);
   c->prof.end(p, st);
   launch_argmin(c->dS.as<double>(), c->prop.as<int32_t>(), B, np, c->best.as<int32_t>(),

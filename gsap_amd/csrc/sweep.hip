@@ -1912,16 +1912,20 @@ __global__ void k_big_blocks(BMView m, int32_t cap, int32_t *list, uint32_t *cou
     list[atomicAdd(count, 1u)] = r;
 }
 
-// one wave per proposal of the listed blocks
+// one wave per proposal of the listed blocks (*count of them, written by k_big_blocks on the
+// same stream), waves striding over the proposals
 __global__ __launch_bounds__(256) void k_merge_list(BMView m, const double *OUT, RngKey key,
                                                     uint32_t phase, int32_t np, int32_t *prop,
                                                     double *dSout, int exact,
-                                                    const int32_t *list, int32_t nlist) {
+                                                    const int32_t *list, const uint32_t *count) {
+  const int64_t n = (int64_t)*count * np;
+  if ((int64_t)blockIdx.x * 4 >= n) return;
   log2_tables_to_lds();
-  const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (j >= (int64_t)nlist * np) return;
-  const int32_t r = list[j / np];
-  merge_one<64>(m, OUT, key, phase, np, (int64_t)r * np + j % np, prop, dSout, exact);
+  for (int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); j < n;
+       j += (int64_t)gridDim.x * 4) {
+    const int32_t r = list[j / np];
+    merge_one<64>(m, OUT, key, phase, np, (int64_t)r * np + j % np, prop, dSout, exact);
+  }
 }
 
 // position of t in the sorted keys[0, n) (LDS), or -1
@@ -2242,16 +2246,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) void k_
 }
 
 void launch_merge(const BMView &m, const double *OUT, RngKey key, uint32_t phase, int32_t np,
-                  int32_t *prop, double *dS, int32_t avg_row, bool exact, int32_t *big_list,
+                  int32_t *prop, double *dS, int64_t nnz, bool exact, int32_t *big_list,
                   uint32_t *big_count, hipStream_t st) {
   const int64_t n = (int64_t)m.B * np;
+  const int32_t avg_row = (int32_t)(m.B > 0 ? (nnz + m.B - 1) / m.B : 0);
   const int ex = exact ? 1 : 0;
   if (avg_row > 12 && m.B >= 4096 && big_list) {
     // long rows and enough blocks to fill the device: a workgroup per block with its row and
     // column in LDS (short rows: groups of 16 lanes, twelve proposals in flight per
     // workgroup, 80 registers at six waves per SIMD; longer rows: a wave per proposal); the
-    // few blocks beyond the staging arrays get a wave per proposal (their count comes back to
-    // the host to size that launch)
+    // few blocks beyond the staging arrays get a wave per proposal, in a grid sized for the
+    // most such blocks the entry count allows (row + column > cap entries each), whose
+    // waves read the listed count on the device
     int32_t cap;
     if (avg_row <= 32) {
       cap = 256;
@@ -2268,15 +2274,12 @@ void launch_merge(const BMView &m, const double *OUT, RngKey key, uint32_t phase
       cap = 1024;
       k_merge_blk_w<1024><<<(unsigned)m.B, 256, 0, st>>>(m, OUT, key, phase, np, prop, dS, ex);
     }
-    uint32_t nbig = 0;
     if (hipMemsetAsync(big_count, 0, 4, st) != hipSuccess) return;
     k_big_blocks<<<grid_for(m.B, 256), 256, 0, st>>>(m, cap, big_list, big_count);
-    if (hipMemcpyAsync(&nbig, big_count, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-      return;  // the caller's error check reports it
-    if (nbig > 0)
-      k_merge_list<<<grid_for((int64_t)nbig * np, 4), 256, 0, st>>>(
-          m, OUT, key, phase, np, prop, dS, ex, big_list, (int32_t)nbig);
+    // (the waves stride over all listed proposals, so any grid size is correct)
+    const int64_t most = std::min<int64_t>(m.B, 2 * nnz / (cap + 1));
+    k_merge_list<<<grid_for(std::min<int64_t>(most * np, 65536), 4), 256, 0, st>>>(
+        m, OUT, key, phase, np, prop, dS, ex, big_list, big_count);
     return;
   }
   if (avg_row <= 12)
