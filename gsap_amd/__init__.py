@@ -60,7 +60,8 @@ class COptions(C.Structure):
                 ("tile_max_B", C.c_int32), ("delta_mode", C.c_int32), ("accept", C.c_int32),
                 ("beta", C.c_double), ("nodal_max16", C.c_int32), ("nodal_max32", C.c_int32),
                 ("nodal_maxwave", C.c_int32), ("seg_reg_max", C.c_int32),
-                ("seg_count_min", C.c_int32), ("sparse_update", C.c_int32)]
+                ("seg_count_min", C.c_int32), ("sparse_update", C.c_int32),
+                ("csr_every_batch", C.c_int32)]
 
 
 class CReport(C.Structure):
@@ -261,6 +262,10 @@ class Options:
     # after a batch without the dense block matrix: 0 rebuilds from every edge, 1 merges the
     # moves into the current CSR/CSC (identical results)
     sparse_update: int = 0
+    # at a dense B: 0 rebuilds the CSR/CSC after a sweep's last batch only (the batches
+    # before it update the dense matrix and the block degrees), 1 after every batch
+    # (identical results)
+    csr_every_batch: int = 0
 
     def cstruct(self) -> COptions:
         o = COptions()
@@ -269,7 +274,7 @@ class Options:
                   "chain", "trace", "dense_max_B", "profile", "model", "workers",
                   "profile_classes", "graph", "tile_max_B", "delta_mode", "accept", "beta",
                   "nodal_max16", "nodal_max32", "nodal_maxwave", "seg_reg_max",
-                  "seg_count_min", "sparse_update"):
+                  "seg_count_min", "sparse_update", "csr_every_batch"):
             setattr(o, f, getattr(self, f))
         return o
 

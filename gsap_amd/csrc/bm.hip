@@ -1412,8 +1412,52 @@ __global__ __launch_bounds__(512) void k_dense_moves_tile(const int32_t *optr, c
   }
 }
 
+// Block degrees after a nodal batch [lo, lo + BS): a moved vertex's weighted out- and
+// in-degree leave its old block (olab) and enter its new one (kout = kin = nullptr: unit
+// weights, the adjacency counts).  LDS: a workgroup sums its moves per block in LDS (2B
+// words) and adds each nonzero total with one global atomic, so that few blocks do not
+// serialize many atomics on one address.
+template <bool LDS>
+__global__ __launch_bounds__(256) void k_deg_moves(const int32_t *optr, const int32_t *iptr,
+                                                   const int32_t *kout, const int32_t *kin,
+                                                   const int32_t *lab, const int32_t *olab,
+                                                   int64_t lo, int64_t BS, int32_t B,
+                                                   int32_t *dout, int32_t *din) {
+  extern __shared__ int32_t sdeg[];
+  if constexpr (LDS) {
+    for (int32_t k = threadIdx.x; k < 2 * B; k += 256) sdeg[k] = 0;
+    __syncthreads();
+  }
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < BS;
+       i += (int64_t)gridDim.x * 256) {
+    const int64_t x = lo + i;
+    const int32_t xo = olab[i], xn = lab[x];
+    if (xo == xn) continue;
+    const int32_t ko = kout ? kout[x] : optr[x + 1] - optr[x];
+    const int32_t ki = kin ? kin[x] : iptr[x + 1] - iptr[x];
+    int32_t *po = LDS ? sdeg : dout, *pi = LDS ? sdeg + B : din;
+    if (ko) {
+      atomicSub(&po[xo], ko);
+      atomicAdd(&po[xn], ko);
+    }
+    if (ki) {
+      atomicSub(&pi[xo], ki);
+      atomicAdd(&pi[xn], ki);
+    }
+  }
+  if constexpr (LDS) {
+    __syncthreads();
+    for (int32_t b = threadIdx.x; b < B; b += 256) {
+      if (sdeg[b]) atomicAdd(&dout[b], sdeg[b]);
+      if (sdeg[B + b]) atomicAdd(&din[b], sdeg[B + b]);
+    }
+  }
+}
+constexpr int DEG_LDS_B = 4096;  // largest B whose degree moves are summed in LDS
+
 bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t *olab, int64_t lo,
-                         int64_t BS, int32_t B, BMBuf &bm, RebuildScratch &rs, hipStream_t st) {
+                         int64_t BS, int32_t B, BMBuf &bm, RebuildScratch &rs, hipStream_t st,
+                         bool defer_csr) {
   // the dense buffer must hold the block matrix of the labels before the batch; the moves
   // are summed in an LDS copy of the matrix (B <= SMALL_B) or of a tile of its rows
   // (B <= INC_TILE_B, where global atomics would contend on few cells), or added with global
@@ -1421,7 +1465,12 @@ bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t 
   if (!(bm.dense_valid && bm.dense_B == B && bm.B == B && B <= INC_MAX_B &&
         (int64_t)B * B <= rs.partials_cap))
     return false;
-  const unsigned grid = (unsigned)std::min<int64_t>(512, (BS + 31) / 32);
+  if (defer_csr && B > INC_TILE_B && bm.dt_B != B) {
+    // the transpose follows the moves from here on: the deferred samples walk its rows
+    const unsigned tb = (unsigned)((B + 31) / 32);
+    k_transpose<<<dim3(tb, tb), 256, 0, st>>>(bm.dense, B, rs.partials);
+    bm.dt_B = B;
+  }
   if (BS > 0 && B <= SMALL_B)
     k_dense_moves_lds<<<(unsigned)std::min<int64_t>(512, (BS + 255) / 256), 512,
                         (size_t)B * B * 4, st>>>(
@@ -1437,12 +1486,28 @@ bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t 
         g.optr, g.oadj, g.ow, g.iptr, g.iadj, g.iw, lab32, olab, lo, BS, B, bm.dense,
         bm.dt_B == B ? rs.partials : nullptr);
   if (B <= INC_TILE_B) bm.dt_B = 0;  // the LDS forms change D alone
+  if (defer_csr) {
+    if (BS > 0) {
+      const bool unit = g.wmax == 1;
+      const int32_t *ko = unit ? nullptr : g.kout, *ki = unit ? nullptr : g.kin;
+      if (B <= DEG_LDS_B)
+        k_deg_moves<true><<<(unsigned)std::min<int64_t>(256, (BS + 255) / 256), 256,
+                            (size_t)8 * B, st>>>(g.optr, g.iptr, ko, ki, lab32, olab, lo, BS, B,
+                                                 bm.dout, bm.din);
+      else
+        k_deg_moves<false><<<(unsigned)((BS + 255) / 256), 256, 0, st>>>(
+            g.optr, g.iptr, ko, ki, lab32, olab, lo, BS, B, bm.dout, bm.din);
+    }
+    bm.csr_stale = true;
+    return true;
+  }
   // (the 16-bit label copy was kept current by the batch's apply)
   if (B <= SMALL_B)
     k_small_bm<<<1, 1024, 0, st>>>(bm.dense, B, bm.rowptr, bm.col, bm.val, bm.erow, bm.colptr,
                                    bm.row, bm.cval, bm.dout, bm.din, bm.ldout, bm.ldin);
   else
     dense_to_csr(B, bm, rs, st);
+  bm.csr_stale = false;
   return true;
 }
 
@@ -2032,7 +2097,8 @@ __global__ __launch_bounds__(256) void k_merge_long(BMOld o, const uint32_t *off
 bool rebuild_after_moves_sparse(const GraphDev &g, const int32_t *lab32, const int32_t *olab,
                                 int64_t lo, int64_t BS, int32_t B, int64_t bound, BMBuf &bm,
                                 RebuildScratch &rs, hipStream_t st) {
-  if (!(rs.sparse_on && !bm.dense_valid && bm.dense_B == 0 && bm.B == B && B > SMALL_B &&
+  if (!(rs.sparse_on && !bm.csr_stale && !bm.dense_valid && bm.dense_B == 0 && bm.B == B &&
+        B > SMALL_B &&
         bm.a_rowptr && rs.rk && rs.mwork && bound <= rs.kcap && bound <= rs.rcap &&
         BS <= rs.mwork_list))
     return false;
@@ -2150,6 +2216,7 @@ void rebuild_blockmodel(const GraphDev &g, const int32_t *lab32, int32_t B, BMBu
   else seg_path(g, lab, B, bm, rs, dense, st);
   bm.dense_valid = dense;
   bm.dense_B = dense ? B : 0;
+  bm.csr_stale = false;
   // lookups without the dense copy search long rows through the coarse index (the
   // segmented path, the only one without it, writes the index with the entries)
   bm.index_valid = !dense && bm.gcol && bm.grow;

@@ -45,6 +45,10 @@ struct BMBuf {
   int32_t dt_B = 0;
   // B of the entries the dense buffer holds (the current CSR's); 0 = all zero
   int32_t dense_B = 0;
+  // the dense matrix and the block degrees are current but the CSR/CSC (and nnz in
+  // rowptr[B]) and the log2 degrees still hold an earlier labelling's matrix: set by a
+  // deferred update between the batches of a sweep, cleared by every full CSR build
+  bool csr_stale = false;
   // The alternate CSR/CSC buffers (same sizes): the sparse update after a nodal batch merges
   // the current matrix and the batch's delta into them, then the two sets change roles;
   // flipped records that the alternate set holds the context's primary buffers.
@@ -111,9 +115,13 @@ void rebuild_blockmodel(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf 
 // After a nodal batch over [lo, lo + BS) (olab: the batch's labels before it), the block
 // matrix by moving the moved vertices' edges between cells of the dense matrix, when that
 // holds the matrix of the labels before the batch (dense B, previous rebuild at this B).
-// Returns false (nothing launched) otherwise.
+// Returns false (nothing launched) otherwise.  defer_csr: only the dense matrix (its
+// transpose too above 512 blocks, kept in the partial-plane buffer with bm.dt_B = B) and
+// the block degrees follow the batch; the CSR, CSC and log2 degrees are left for a later
+// rebuild (bm.csr_stale).
 bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t *olab, int64_t lo,
-                         int64_t BS, int32_t B, BMBuf &bm, RebuildScratch &rs, hipStream_t st);
+                         int64_t BS, int32_t B, BMBuf &bm, RebuildScratch &rs, hipStream_t st,
+                         bool defer_csr = false);
 
 // After a nodal batch over [lo, lo + BS) at a B without the dense copy: the batch's edge
 // moves as a signed delta matrix, sorted per row and column by the segmented rebuild's
@@ -156,6 +164,13 @@ struct PhaseRef {
 void launch_block_terms(const BMView &m, double *OUT, int32_t *sof, uint8_t *coin, RngKey key,
                         PhaseRef phase, bool sample, int32_t avg_row, hipStream_t st,
                         int32_t *zero = nullptr);
+// The per-block samples and coins from the dense matrix D (B x B, row-major) and, when not
+// null, its transpose DT, for batches whose CSR/CSC lags the labels; also writes the log2
+// block degrees of dout / din.
+void launch_block_terms_dense(const int32_t *D, const int32_t *DT, int32_t B, const int32_t *dout,
+                              const int32_t *din, double *ldout, double *ldin, int32_t *sof,
+                              uint8_t *coin, RngKey key, PhaseRef phase, hipStream_t st,
+                              int32_t *zero);
 // Nodal kernels by the larger of a vertex's out- and in-neighbour counts d: d <= 16 and 32
 // use register deduplication in groups of 16 and 32 lanes; d <= 128 a wave with 256-slot
 // LDS hash tables; larger d a 256-thread workgroup with tables sized for the vertex (LDS

@@ -608,13 +608,20 @@ void rebuild(gsap_ctx *c, int32_t B) {
 // Without the dense matrix and with options.sparse_update = 1, the sparse update merges the
 // batch's delta into the current CSR/CSC (a captured sweep of an odd number of batches would
 // end on the other buffer set, so it rebuilds instead).
-void rebuild_after_batch(gsap_ctx *c, int32_t B, const gsap_options &o, int32_t b) {
+//
+// defer_csr (a sweep's batches before its last, unless options.csr_every_batch): where the
+// dense update applies, only the dense matrix and the block degrees follow the batch; the
+// next batch samples from the dense matrix and the sweep's last batch rebuilds the CSR/CSC
+// that the description length and the merges read.
+void rebuild_after_batch(gsap_ctx *c, int32_t B, const gsap_options &o, int32_t b,
+                         bool defer_csr = false) {
   int p = c->prof.begin(1, c->st);
   const int64_t BS = c->N / o.num_batches, lo = (int64_t)b * BS;
   const int32_t *lab = c->lab.as<int32_t>(), *olab = c->olab.as<int32_t>();
   c->rs.sparse_on = o.sparse_update == 1 && !(c->capturing && (o.num_batches & 1));
   const int64_t bound = b < (int32_t)c->delta_bound.size() ? c->delta_bound[b] : INT64_MAX;
-  if (!rebuild_after_moves(c->g, lab, olab, lo, BS, B, c->bm, c->rs, c->st) &&
+  if (!rebuild_after_moves(c->g, lab, olab, lo, BS, B, c->bm, c->rs, c->st,
+                           defer_csr && o.csr_every_batch == 0) &&
       !rebuild_after_moves_sparse(c->g, lab, olab, lo, BS, B, bound, c->bm, c->rs, c->st))
     rebuild_blockmodel(c->g, lab, B, c->bm, c->rs, c->dense_max_B, c->tile_max_B, c->st);
   c->prof.end(p, c->st);
@@ -626,7 +633,15 @@ void rebuild_after_batch(gsap_ctx *c, int32_t B, const gsap_options &o, int32_t 
 // by a sequence number, which the host polls (no interrupt-driven wake-up between a sweep
 // and the next).  The stream is queried now and then, so a failed launch ends the wait with
 // its error.
+// Readers of the CSR/CSC (the entry count, the description length, the merges) need the
+// current labels' matrix: no deferred update may be pending.
+void require_csr(const gsap_ctx *c) {
+  if (c->bm.csr_stale)
+    throw Err(GSAP_ERR_INTERNAL, "block-matrix CSR read while a deferred update is pending");
+}
+
 uint32_t publish(gsap_ctx *c, const double *res, int32_t n) {
+  require_csr(c);
   const uint32_t seq = ++c->pub_seq;
   launch_publish(res, n, c->bm.rowptr + c->bm.B, c->pub_dev->v, &c->pub_dev->nnz,
                  &c->pub_dev->seq, seq, c->st);
@@ -684,6 +699,7 @@ int64_t batch_tiles_off(const gsap_ctx *c) { return std::max<int64_t>(c->E, 1) /
 int64_t batch_tiles_stride(const gsap_ctx *c, int32_t nb) { return c->N / nb / 1024 + 1; }
 
 void launch_mdl(gsap_ctx *c, double *out, uint32_t *ctr = nullptr, uint32_t add = 0) {
+  require_csr(c);
   int p = c->prof.begin(4, c->st);
   BMView m = c->bm.view();
   launch_mdl_sum(m, c->bm.erow, c->E, c->tiles.as<double>(), out, ctr, add, c->st);
@@ -693,6 +709,7 @@ void launch_mdl(gsap_ctx *c, double *out, uint32_t *ctr = nullptr, uint32_t add 
 // the sweep's closing sums: the description length's data term into res[0], each batch's
 // deferred delta-S total into res[1 + b]; the batch counter advanced for the next sweep
 void sweep_sums(gsap_ctx *c, const gsap_options &o, double *res, uint32_t *ctr) {
+  require_csr(c);
   int p = c->prof.begin(4, c->st);
   BMView m = c->bm.view();
   launch_sweep_sums(m, c->bm.erow, c->E, c->tiles.as<double>(), batch_tiles_off(c),
@@ -797,8 +814,15 @@ int32_t merge_phase(gsap_ctx *c, const gsap_options &o, int32_t B_to_merge, uint
 // matrix and the batch counter, and clear the proposal pass's class counters.
 void batch_terms(gsap_ctx *c, const gsap_options &o, PhaseRef phase, int32_t ar) {
   int p = c->prof.begin(2, c->st);
-  launch_block_terms(c->bm.view(), nullptr, c->sof.as<int32_t>(), c->coin.as<uint8_t>(),
-                     rng_key(o), phase, true, ar, c->st, c->dcnt.as<int32_t>());
+  const BMBuf &bm = c->bm;
+  if (bm.csr_stale)  // the samples from the dense matrix (and the log2 degrees with them)
+    launch_block_terms_dense(bm.dense, bm.dt_B == bm.B ? c->rs.partials : nullptr, bm.B,
+                             bm.dout, bm.din, bm.ldout, bm.ldin, c->sof.as<int32_t>(),
+                             c->coin.as<uint8_t>(), rng_key(o), phase, c->st,
+                             c->dcnt.as<int32_t>());
+  else
+    launch_block_terms(bm.view(), nullptr, c->sof.as<int32_t>(), c->coin.as<uint8_t>(),
+                       rng_key(o), phase, true, ar, c->st, c->dcnt.as<int32_t>());
   c->prof.end(p, c->st);
 }
 
@@ -890,6 +914,8 @@ void check_options(const gsap_options &o) {
     throw Err(GSAP_ERR_INVALID, "seg_count_min must be 0 or in 512..4096");
   if (o.sparse_update != 0 && o.sparse_update != 1)
     throw Err(GSAP_ERR_INVALID, "sparse_update must be 0 or 1");
+  if (o.csr_every_batch != 0 && o.csr_every_batch != 1)
+    throw Err(GSAP_ERR_INVALID, "csr_every_batch must be 0 or 1");
 
   if (o.delta_mode != GSAP_DELTA_REFERENCE && o.delta_mode != GSAP_DELTA_EXACT)
     throw Err(GSAP_ERR_INVALID, "unknown delta_mode");
@@ -1027,7 +1053,7 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
         if (!gg.exec) gg.capture([&] {
           for (int b = 0; b < o.num_batches; ++b) {
             nodal_batch(c, o, b, PhaseRef{ctr, (uint32_t)b}, ar_level, nullptr);
-            rebuild_after_batch(c, B, o, b);
+            rebuild_after_batch(c, B, o, b, b + 1 < o.num_batches);
           }
           sweep_sums(c, o, res, ctr);
         });
@@ -1044,7 +1070,7 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
         // traced: every batch's sum at once (read below); else totalled at the sweep's end
         nodal_batch(c, o, b, PhaseRef{ctr, (uint32_t)b}, ar, tr ? res + 1 + b : nullptr,
                     b == 0 && terms_queued);
-        rebuild_after_batch(c, B, o, b);
+        rebuild_after_batch(c, B, o, b, b + 1 < o.num_batches);
         if (tr) {
           double e = 0;
           hprop.resize(BS);

@@ -354,6 +354,110 @@ __global__ __launch_bounds__(1024) void k_block_walk_big(BMView m, int32_t *sof,
   }
 }
 
+// sum over the G lanes of a group, in every lane
+template <int G, class T>
+GSAP_D T group_allsum(T x) {
+#pragma unroll
+  for (int off = G / 2; off >= 1; off >>= 1) x += __shfl_xor(x, off, G);
+  return x;
+}
+
+// First entry of p[0], p[stride], ..., p[(n - 1) stride] whose running weight exceeds
+// target, in rounds of G x U entries: every lane sums its U loads (all in flight), the group
+// total decides whether the crossing lies in this round, and only that round is scanned in
+// order.  A zero entry never holds the crossing, so this equals the walk over the same row's
+// nonzero entries in the CSR (or CSC) order.  fallback when the weights never exceed target.
+template <int G, int U>
+GSAP_D int32_t dense_walk(const int32_t *p, int64_t stride, int32_t n, uint64_t target,
+                          int32_t fallback) {
+  const int q = threadIdx.x & (G - 1);
+  const int gbase = (threadIdx.x & 63) & ~(G - 1);
+  const uint64_t gm = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+  uint64_t base = 0;
+  for (int32_t c = 0; c < n; c += G * U) {
+    uint32_t w[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int32_t k = c + u * G + q;
+      w[u] = k < n ? (uint32_t)p[(int64_t)k * stride] : 0u;
+    }
+    uint64_t s = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) s += w[u];
+    const uint64_t tot = group_allsum<G>(s);
+    if (base + tot > target) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        uint64_t incl = w[u];
+#pragma unroll
+        for (int off = 1; off < G; off <<= 1) {
+          const uint64_t y = __shfl_up(incl, off, G);
+          if (q >= off) incl += y;
+        }
+        const uint64_t bal = (__ballot(base + incl > target) >> gbase) & gm;
+        if (bal) return c + u * G + (__ffsll((long long)bal) - 1);
+        base += __shfl(incl, G - 1, G);
+      }
+    }
+    base += tot;
+  }
+  return fallback;
+}
+
+// The nodal per-block neighbour sample and coin (kernels.hpp:942-997, 1024-1029) while the
+// CSR/CSC lags the labels (between the batches of a sweep at a dense B): the walk over row x
+// and then column x of the dense matrix D, column x read as row x of its transpose DT when
+// that is current (else down D's column).  The target lies in the row when it is below the
+// row's total d_out[x], else in the column at target - d_out[x], so only that part is walked.
+// Also writes the log2 block degrees of the degrees the batch update left in dout / din.
+template <int G, int U>
+__global__ __launch_bounds__(256) void k_block_terms_dense(const int32_t *D, const int32_t *DT,
+                                                           int32_t B, const int32_t *dout,
+                                                           const int32_t *din, double *ldout,
+                                                           double *ldin, int32_t *sof,
+                                                           uint8_t *coin, RngKey key,
+                                                           PhaseRef pr, int32_t *zero) {
+  if (zero && blockIdx.x == 0 && threadIdx.x < NCLS) zero[threadIdx.x] = 0;
+  const int32_t x = (int32_t)(blockIdx.x * (256 / G) + threadIdx.x / G);
+  if (x >= B) return;
+  const int q = threadIdx.x & (G - 1);
+  const int32_t dox = dout[x], dix = din[x];
+  const uint32_t phase = (pr.dev ? *pr.dev : 0u) + pr.add;
+  const Philox y = philox((uint32_t)x, phase, key.chain, TAG_BLOCK, key.k0, key.k1);
+  const int64_t deg = (int64_t)dox + dix;
+  int32_t so = 0;
+  if (deg > 0) {
+    const uint64_t target = mulhi(y.x0, deg);
+    if (target < (uint64_t)dox)
+      so = dense_walk<G, U>(D + (int64_t)x * B, 1, B, target, 0);
+    else if (DT)
+      so = dense_walk<G, U>(DT + (int64_t)x * B, 1, B, target - (uint64_t)dox, 0);
+    else
+      so = dense_walk<G, U>(D + x, B, B, target - (uint64_t)dox, 0);
+  }
+  if (q == 0) {
+    sof[x] = so;
+    coin[x] = coin_random(y.x1, deg, B) ? 1 : 0;
+    ldout[x] = clog2((double)dox);
+    ldin[x] = clog2((double)dix);
+  }
+}
+
+void launch_block_terms_dense(const int32_t *D, const int32_t *DT, int32_t B, const int32_t *dout,
+                              const int32_t *din, double *ldout, double *ldin, int32_t *sof,
+                              uint8_t *coin, RngKey key, PhaseRef phase, hipStream_t st,
+                              int32_t *zero) {
+  if (B <= 128)
+    k_block_terms_dense<16, 4><<<grid_for(B, 16), 256, 0, st>>>(D, DT, B, dout, din, ldout,
+                                                               ldin, sof, coin, key, phase, zero);
+  else if (B <= 1024)
+    k_block_terms_dense<32, 8><<<grid_for(B, 8), 256, 0, st>>>(D, DT, B, dout, din, ldout, ldin,
+                                                              sof, coin, key, phase, zero);
+  else
+    k_block_terms_dense<64, 16><<<grid_for(B, 4), 256, 0, st>>>(D, DT, B, dout, din, ldout,
+                                                               ldin, sof, coin, key, phase, zero);
+}
+
 void launch_block_terms(const BMView &m, double *OUT, int32_t *sof, uint8_t *coin, RngKey key,
                         PhaseRef phase, bool sample, int32_t avg_row, hipStream_t st,
                         int32_t *zero) {
@@ -1095,13 +1199,6 @@ GSAP_D void nodal_big(const NodalArgs &a, int64_t v, double *lds_d, uint32_t lds
 }
 
 // Group sums over the G lanes of a group (every lane gets the result).
-template <int G, class T>
-GSAP_D T group_allsum(T x) {
-#pragma unroll
-  for (int off = G / 2; off >= 1; off >>= 1) x += __shfl_xor(x, off, G);
-  return x;
-}
-
 // Same computation as k_nodal for a vertex with at most G out- and G in-neighbours, G lanes
 // per vertex: lane q holds out-entry q and in-entry q; the neighbour blocks are deduplicated
 // in registers (shuffles inside the group) instead of hash tables, and the block of the

@@ -128,6 +128,11 @@ typedef struct gsap_options {
    * block matrix from every edge; 1 merges the batch's moves into the current CSR/CSC as a
    * sorted signed delta (work follows the moved vertices' edges).  Identical results. */
   int32_t sparse_update;
+  /* At a B with the dense block matrix: 0 (default) updates only the dense matrix and the
+   * block degrees after each nodal batch but a sweep's last, samples the next batch's block
+   * neighbours from the dense matrix, and rebuilds the CSR/CSC after the sweep's last batch;
+   * 1 rebuilds the CSR/CSC after every batch.  Identical results. */
+  int32_t csr_every_batch;
 } gsap_options;
 
 /* Per-call report.  Times are seconds of host wall clock around device work. */

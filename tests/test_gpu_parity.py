@@ -107,12 +107,16 @@ def test_nodal_batch(ctx, name, Bfrac, dense):
         assert np.array_equal(got["labels"], ref["labels"])
 
 
-@pytest.mark.parametrize("name,model", [("tiny_w", 0), ("hubs", 0), ("LL1k", 0), ("LL1k", 1),
-                                        ("HH5k", 1)])
-def test_partition_trace(ctx, name, model):
+@pytest.mark.parametrize("name,model,csr", [("tiny_w", 0, 0), ("hubs", 0, 0), ("LL1k", 0, 0),
+                                            ("LL1k", 1, 0), ("HH5k", 1, 0), ("tiny_w", 0, 1),
+                                            ("HH5k", 1, 1)])
+def test_partition_trace(ctx, name, model, csr):
+    """Whole traces against the oracle: by default the sweeps' batches before the last
+    update only the dense block matrix and the block degrees (samples walk the dense
+    matrix); csr = 1 rebuilds the CSR/CSC after every batch."""
     g = graph(name)
-    o = Options(seed=3, trace=1, model=model)
-    print(f"partition {name}", flush=True)
+    o = Options(seed=3, trace=1, model=model, csr_every_batch=csr)
+    print(f"partition {name} csr_every_batch={csr}", flush=True)
     lab, rep = ctx.partition(g, o)
     tr = ctx.trace()
     rlab, rrep, rtr = O.partition(g, o, trace_cap=len(tr) + 10)
