@@ -486,11 +486,14 @@ __global__ __launch_bounds__(256) void k_seg_fill(LabView lab, const int32_t *es
 }
 
 // Nonempty segments into per-class lists: per-wave ballots, per-workgroup LDS counts, one
-// global atomic per class and workgroup round (1024 segments).
+// global atomic per class and workgroup round (1024 segments).  first (when not null): the
+// segment holding position 64 c of the segment array, for every chunk c (the thread of a
+// segment writes the chunks that start inside it), read by the compaction.
 __global__ __launch_bounds__(256) void k_seg_classify(const uint32_t *off, int64_t nseg,
                                                       int one_range, uint32_t reg_max,
                                                       uint32_t count_min, uint32_t *lists,
-                                                      int64_t lcap, uint32_t *ncl) {
+                                                      int64_t lcap, uint32_t *ncl,
+                                                      uint32_t *first) {
   __shared__ uint32_t wcnt[SEG_NCL], wbase[SEG_NCL];
   const int lane = threadIdx.x & 63;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -504,8 +507,10 @@ __global__ __launch_bounds__(256) void k_seg_classify(const uint32_t *off, int64
       const int64_t x = base + j * 256 + threadIdx.x;
       c[j] = -1;
       if (x < nseg) {
-        const uint32_t n = off[x + 1] - off[x];
+        const uint32_t s0 = off[x], s1 = off[x + 1], n = s1 - s0;
         if (n) c[j] = seg_class(n, one_range != 0, reg_max, count_min);
+        if (first)
+          for (uint32_t ch = (s0 + 63) >> 6; (ch << 6) < s1; ++ch) first[ch] = (uint32_t)x;
       }
 #pragma unroll
       for (int k = 0; k < SEG_NCL; ++k) {
@@ -998,17 +1003,6 @@ __global__ __launch_bounds__(512) void k_seg_c_write(const uint32_t *off, const 
   }
 }
 
-// The segment holding position 64 c of the segment array, for every chunk c (one thread per
-// segment writes the chunks that start inside it)
-__global__ __launch_bounds__(256) void k_seg_chunks(const uint32_t *off, int64_t nseg,
-                                                    uint32_t *first) {
-  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nseg;
-       x += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t s0 = off[x], s1 = off[x + 1];
-    for (uint32_t c = (s0 + 63) >> 6; (c << 6) < s1; ++c) first[c] = (uint32_t)x;
-  }
-}
-
 // Row/column pointers from the scanned run counts, then entry-parallel: the runs of every
 // segment (except two-pass class C, written at their place already) copied from the
 // segment's start to its final position, four entries per thread.  An entry's segment is
@@ -1123,7 +1117,8 @@ static void seg_path_t(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
                                                B, rs.soff, rs.vrel, ekey, ew, wb);
   const uint32_t reg_max = (uint32_t)rs.seg_reg_max, count_min = (uint32_t)rs.seg_count_min;
   k_seg_classify<<<grid_for(nseg, T, SEG_GRID), T, 0, st>>>(rs.soff, nseg, one ? 1 : 0, reg_max,
-                                                            count_min, rs.lists, rs.lcap, rs.ncl);
+                                                            count_min, rs.lists, rs.lcap, rs.ncl,
+                                                            rs.chunk_seg);
   if (one) {
     k_seg_c1<PK><<<SEG_GRID, 512, (size_t)RC * 4, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
                                                         rs.ncl, B, rs.su, wb);
@@ -1151,7 +1146,6 @@ static void seg_path_t(const GraphDev &g, LabView lab, int32_t B, BMBuf &bm,
   k_seg_sort_reg<PK><<<SEG_GRID * 2, 256, 0, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap, rs.ncl,
                                                    rs.su, wb);
   scan_exclusive(rs.su, rs.suoff, nseg, rs.suoff + nseg, rs.tmp, st);
-  k_seg_chunks<<<grid_for(nseg, T), T, 0, st>>>(rs.soff, nseg, rs.chunk_seg);
   k_seg_compact<<<grid_for((2 * E + 3) / 4, T), T, 0, st>>>(
       rs.soff, ekey, ew, rs.chunk_seg, rs.suoff, B, one ? 1 : 0, 2 * E, bm.rowptr, bm.col,
       bm.val, bm.erow, bm.colptr, bm.row, bm.cval, D, gcol, grow);
@@ -2128,7 +2122,8 @@ bool rebuild_after_moves_sparse(const GraphDev &g, const int32_t *lab32, const i
   }
   const uint32_t reg_max = (uint32_t)rs.seg_reg_max, count_min = (uint32_t)rs.seg_count_min;
   k_seg_classify<<<grid_for(nseg, T, SEG_GRID), T, 0, st>>>(rs.soff, nseg, one ? 1 : 0, reg_max,
-                                                            count_min, rs.lists, rs.lcap, rs.ncl);
+                                                            count_min, rs.lists, rs.lcap, rs.ncl,
+                                                            nullptr);
   if (one) {
     k_seg_c1<false><<<SEG_GRID, 512, (size_t)RC * 4, st>>>(rs.soff, ekey, ew, rs.lists, rs.lcap,
                                                            rs.ncl, B, rs.su, 0);
