@@ -177,7 +177,8 @@ __global__ __launch_bounds__(1024) void k_small_bm(const int32_t *D, int32_t B, 
 // ---- vertices grouped by block (counting sort; order inside a block is irrelevant)
 constexpr int VT = 2048;  // vertices per workgroup
 constexpr int TILE_MAX_B = 8192;  // largest B of the row-tile path
-constexpr int INC_MAX_B = 8192;   // largest B of the incremental dense update
+constexpr int INC_MAX_B = 16384;  // largest B of the incremental dense update
+constexpr int TAIL_MAX_B = 8192;  // above it the CSR after an update comes from the edges
 constexpr int INC_TILE_B = 512;   // its LDS row-tile form runs up to this B, global atomics above
 
 __global__ __launch_bounds__(256) void k_label_hist(LabView lab, int64_t N, int32_t B,
@@ -1499,8 +1500,10 @@ bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t 
   if (B <= SMALL_B)
     k_small_bm<<<1, 1024, 0, st>>>(bm.dense, B, bm.rowptr, bm.col, bm.val, bm.erow, bm.colptr,
                                    bm.row, bm.cval, bm.dout, bm.din, bm.ldout, bm.ldin);
-  else
+  else if (B <= TAIL_MAX_B)
     dense_to_csr(B, bm, rs, st);
+  else  // the dense tail reads 4 B^2 words: the segmented rebuild's CSR/CSC instead (D kept)
+    seg_path(g, LabView{lab32, B <= 65535 ? rs.lab16 : nullptr}, B, bm, rs, false, st);
   bm.csr_stale = false;
   return true;
 }

@@ -222,7 +222,7 @@ struct gsap_ctx {
   BMBuf bm;
   RebuildScratch rs;
   int64_t N = 0, E = 0;
-  int32_t dense_max_B = 8192;
+  int32_t dense_max_B = 16384;
   int32_t tile_max_B = 2048;
   // sub-contexts of batch / chain calls (one per worker thread)
   std::vector<gsap_ctx *> workers;
@@ -523,8 +523,8 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   // counters) and of the row-tile histogram (up to 8M counters), then the transpose
   const int64_t sb = std::min<int64_t>(dB, 128);
   // (and the transpose of the dense matrix, which the row-tile and incremental paths
-  // build up to 8192 blocks: at most 8192 x 8192 words)
-  const int64_t tB = std::min<int64_t>(dB, 8192);
+  // keep up to 16384 blocks: at most 16384 x 16384 words)
+  const int64_t tB = std::min<int64_t>(dB, 16384);
   const int64_t pc = std::max<int64_t>(
       std::max<int64_t>(1024 * sb * sb, tB * tB),
       dB > 128 ? std::min<int64_t>(E / 2, (int64_t)32 << 20) : 0);
@@ -576,7 +576,7 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
 }
 
 constexpr int32_t SMALL_B_TUNE = 128;  // the small rebuild path's limit (never tuned)
-int32_t resolve_dense(const gsap_options &o) { return o.dense_max_B > 0 ? o.dense_max_B : 8192; }
+int32_t resolve_dense(const gsap_options &o) { return o.dense_max_B > 0 ? o.dense_max_B : 16384; }
 NodalBounds resolve_nodal(const gsap_options &o) {
   NodalBounds b;
   if (o.nodal_max16) b.max16 = o.nodal_max16;

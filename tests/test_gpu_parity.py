@@ -155,7 +155,7 @@ def c3():
 
 @pytest.mark.parametrize("B,dense", [(71, 0), (5000, 0), (200000, 0), (20000, 0), (3000, 1)])
 def test_c3_blockmodel_and_batch(ctx, c3, B, dense):
-    """dense = 1 (and B > 8192) look the block matrix up in CSR/CSC rows of hundreds of
+    """dense = 1 (and B > 16384) look the block matrix up in CSR/CSC rows of hundreds of
     entries through the coarse index."""
     g = c3
     lab = labels_for(g, B, 9) if B < g.N else np.arange(g.N, dtype=np.int32)
