@@ -1407,48 +1407,11 @@ __global__ __launch_bounds__(512) void k_dense_moves_tile(const int32_t *optr, c
   }
 }
 
-// Block degrees after a nodal batch [lo, lo + BS): a moved vertex's weighted out- and
-// in-degree leave its old block (olab) and enter its new one (kout = kin = nullptr: unit
-// weights, the adjacency counts).  LDS: a workgroup sums its moves per block in LDS (2B
-// words) and adds each nonzero total with one global atomic, so that few blocks do not
-// serialize many atomics on one address.
-template <bool LDS>
-__global__ __launch_bounds__(256) void k_deg_moves(const int32_t *optr, const int32_t *iptr,
-                                                   const int32_t *kout, const int32_t *kin,
-                                                   const int32_t *lab, const int32_t *olab,
-                                                   int64_t lo, int64_t BS, int32_t B,
-                                                   int32_t *dout, int32_t *din) {
-  extern __shared__ int32_t sdeg[];
-  if constexpr (LDS) {
-    for (int32_t k = threadIdx.x; k < 2 * B; k += 256) sdeg[k] = 0;
-    __syncthreads();
-  }
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < BS;
-       i += (int64_t)gridDim.x * 256) {
-    const int64_t x = lo + i;
-    const int32_t xo = olab[i], xn = lab[x];
-    if (xo == xn) continue;
-    const int32_t ko = kout ? kout[x] : optr[x + 1] - optr[x];
-    const int32_t ki = kin ? kin[x] : iptr[x + 1] - iptr[x];
-    int32_t *po = LDS ? sdeg : dout, *pi = LDS ? sdeg + B : din;
-    if (ko) {
-      atomicSub(&po[xo], ko);
-      atomicAdd(&po[xn], ko);
-    }
-    if (ki) {
-      atomicSub(&pi[xo], ki);
-      atomicAdd(&pi[xn], ki);
-    }
-  }
-  if constexpr (LDS) {
-    __syncthreads();
-    for (int32_t b = threadIdx.x; b < B; b += 256) {
-      if (sdeg[b]) atomicAdd(&dout[b], sdeg[b]);
-      if (sdeg[B + b]) atomicAdd(&din[b], sdeg[B + b]);
-    }
-  }
+
+bool dense_update_ok(const BMBuf &bm, const RebuildScratch &rs, int32_t B) {
+  return bm.dense_valid && bm.dense_B == B && bm.B == B && B <= INC_MAX_B &&
+         (int64_t)B * B <= rs.partials_cap;
 }
-constexpr int DEG_LDS_B = 4096;  // largest B whose degree moves are summed in LDS
 
 bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t *olab, int64_t lo,
                          int64_t BS, int32_t B, BMBuf &bm, RebuildScratch &rs, hipStream_t st,
@@ -1457,9 +1420,7 @@ bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t 
   // are summed in an LDS copy of the matrix (B <= SMALL_B) or of a tile of its rows
   // (B <= INC_TILE_B, where global atomics would contend on few cells), or added with global
   // atomics (spread over many cells at larger B)
-  if (!(bm.dense_valid && bm.dense_B == B && bm.B == B && B <= INC_MAX_B &&
-        (int64_t)B * B <= rs.partials_cap))
-    return false;
+  if (!dense_update_ok(bm, rs, B)) return false;
   if (defer_csr && B > INC_TILE_B && bm.dt_B != B) {
     // the transpose follows the moves from here on: the deferred samples walk its rows
     const unsigned tb = (unsigned)((B + 31) / 32);
@@ -1482,17 +1443,6 @@ bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t 
         bm.dt_B == B ? rs.partials : nullptr);
   if (B <= INC_TILE_B) bm.dt_B = 0;  // the LDS forms change D alone
   if (defer_csr) {
-    if (BS > 0) {
-      const bool unit = g.wmax == 1;
-      const int32_t *ko = unit ? nullptr : g.kout, *ki = unit ? nullptr : g.kin;
-      if (B <= DEG_LDS_B)
-        k_deg_moves<true><<<(unsigned)std::min<int64_t>(256, (BS + 255) / 256), 256,
-                            (size_t)8 * B, st>>>(g.optr, g.iptr, ko, ki, lab32, olab, lo, BS, B,
-                                                 bm.dout, bm.din);
-      else
-        k_deg_moves<false><<<(unsigned)((BS + 255) / 256), 256, 0, st>>>(
-            g.optr, g.iptr, ko, ki, lab32, olab, lo, BS, B, bm.dout, bm.din);
-    }
     bm.csr_stale = true;
     return true;
   }

@@ -115,10 +115,11 @@ void rebuild_blockmodel(const GraphDev &g, const int32_t *lab, int32_t B, BMBuf 
 // After a nodal batch over [lo, lo + BS) (olab: the batch's labels before it), the block
 // matrix by moving the moved vertices' edges between cells of the dense matrix, when that
 // holds the matrix of the labels before the batch (dense B, previous rebuild at this B).
-// Returns false (nothing launched) otherwise.  defer_csr: only the dense matrix (its
-// transpose too above 512 blocks, kept in the partial-plane buffer with bm.dt_B = B) and
-// the block degrees follow the batch; the CSR, CSC and log2 degrees are left for a later
-// rebuild (bm.csr_stale).
+// Returns false (nothing launched) otherwise (dense_update_ok).  defer_csr (the batch's
+// apply has moved the block degrees already): only the dense matrix (its transpose too above
+// 512 blocks, kept in the partial-plane buffer with bm.dt_B = B) follows the batch; the CSR,
+// CSC and log2 degrees are left for a later rebuild (bm.csr_stale).
+bool dense_update_ok(const BMBuf &bm, const RebuildScratch &rs, int32_t B);
 bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t *olab, int64_t lo,
                          int64_t BS, int32_t B, BMBuf &bm, RebuildScratch &rs, hipStream_t st,
                          bool defer_csr = false);
@@ -223,9 +224,17 @@ void launch_mix_stats(const float *err, const uint8_t *near, int64_t n, uint32_t
 // prop holds every vertex's new label; olab keeps the old ones, lab16 when given follows)
 // and the canonical sum of the batch's dS into *out (out = nullptr: tile sums only, the
 // total left to launch_sweep_sums).
+// With dm.dout set (a batch whose CSR rebuild is deferred), a moved vertex's weighted out-
+// and in-degree also leave its old block and enter its new one in dm.dout / dm.din.
+struct DegMoves {
+  const int32_t *optr = nullptr, *iptr = nullptr;  // adjacency counts (unit weights), or
+  const int32_t *kout = nullptr, *kin = nullptr;   // weighted degrees
+  int32_t B = 0;
+  int32_t *dout = nullptr, *din = nullptr;
+};
 void launch_apply_sum(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo, int64_t BS,
                       bool all, int32_t *olab, uint16_t *lab16, double *tiles, double *out,
-                      hipStream_t st);
+                      const DegMoves &dm, hipStream_t st);
 // The end of a sweep: the description-length data term (tiles at tiles[0..)) into out[0] and
 // the deferred delta-S sums of the nb batches (tiles at tiles + boff + b * bstride) into
 // out[1 + b], one launch; ctr advanced by add afterwards.

@@ -603,16 +603,22 @@ void rebuild(gsap_ctx *c, int32_t B) {
   HIPX(hipGetLastError());
 }
 
+// Whether the rebuild after batch b of a sweep defers the CSR/CSC (a sweep's batches before
+// its last, at a B whose dense matrix the incremental update keeps, unless
+// options.csr_every_batch).
+bool defers_csr(const gsap_ctx *c, const gsap_options &o, int32_t B, int32_t b) {
+  return b + 1 < o.num_batches && o.csr_every_batch == 0 && dense_update_ok(c->bm, c->rs, B);
+}
+
 // The rebuild after nodal batch b: the moved vertices' edges moved between cells of the
 // dense matrix when it holds the matrix of the labels before the batch, else a full rebuild.
 // Without the dense matrix and with options.sparse_update = 1, the sparse update merges the
 // batch's delta into the current CSR/CSC (a captured sweep of an odd number of batches would
 // end on the other buffer set, so it rebuilds instead).
 //
-// defer_csr (a sweep's batches before its last, unless options.csr_every_batch): where the
-// dense update applies, only the dense matrix and the block degrees follow the batch; the
-// next batch samples from the dense matrix and the sweep's last batch rebuilds the CSR/CSC
-// that the description length and the merges read.
+// defer_csr (from defers_csr: the batch's apply moved the block degrees): only the dense
+// matrix follows the batch; the next batch samples from the dense matrix and the sweep's
+// last batch rebuilds the CSR/CSC that the description length and the merges read.
 void rebuild_after_batch(gsap_ctx *c, int32_t B, const gsap_options &o, int32_t b,
                          bool defer_csr = false) {
   int p = c->prof.begin(1, c->st);
@@ -620,8 +626,9 @@ void rebuild_after_batch(gsap_ctx *c, int32_t B, const gsap_options &o, int32_t 
   const int32_t *lab = c->lab.as<int32_t>(), *olab = c->olab.as<int32_t>();
   c->rs.sparse_on = o.sparse_update == 1 && !(c->capturing && (o.num_batches & 1));
   const int64_t bound = b < (int32_t)c->delta_bound.size() ? c->delta_bound[b] : INT64_MAX;
-  if (!rebuild_after_moves(c->g, lab, olab, lo, BS, B, c->bm, c->rs, c->st,
-                           defer_csr && o.csr_every_batch == 0) &&
+  if (defer_csr && !dense_update_ok(c->bm, c->rs, B))
+    throw Err(GSAP_ERR_INTERNAL, "deferred block-matrix update without the dense matrix");
+  if (!rebuild_after_moves(c->g, lab, olab, lo, BS, B, c->bm, c->rs, c->st, defer_csr) &&
       !rebuild_after_moves_sparse(c->g, lab, olab, lo, BS, B, bound, c->bm, c->rs, c->st))
     rebuild_blockmodel(c->g, lab, B, c->bm, c->rs, c->dense_max_B, c->tile_max_B, c->st);
   c->prof.end(p, c->st);
@@ -840,8 +847,10 @@ NodalPre nodal_pre(gsap_ctx *c) {
 // One nodal batch on the current block matrix; the accepted delta-S sum lands in *dE_dev
 // (nullptr: only the batch's tile sums, totalled by the sweep's launch_sweep_sums).
 // terms_queued: the batch's block terms are already on the stream.
+// defer_csr: the rebuild after this batch defers the CSR (rebuild_after_batch with the
+// same flag), so the apply moves the block degrees.
 void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, PhaseRef phase, int32_t ar,
-                 double *dE_dev, bool terms_queued = false) {
+                 double *dE_dev, bool terms_queued = false, bool defer_csr = false) {
   const int64_t BS = c->N / o.num_batches, lo = (int64_t)b * BS;
   hipStream_t st = c->st;
   BMView m = c->bm.view();
@@ -878,12 +887,19 @@ void nodal_batch(gsap_ctx *c, const gsap_options &o, int32_t b, PhaseRef phase, 
   p = c->prof.begin(5, st);
   // the 16-bit label copy follows the batch's changes (the rebuild after the batch then
   // finds it current)
+  DegMoves dm;
+  if (defer_csr) {
+    const bool unit = c->g.wmax == 1;
+    dm.optr = c->g.optr; dm.iptr = c->g.iptr;
+    dm.kout = unit ? nullptr : c->g.kout; dm.kin = unit ? nullptr : c->g.kin;
+    dm.B = c->bm.B; dm.dout = c->bm.dout; dm.din = c->bm.din;
+  }
   launch_apply_sum(c->lab.as<int32_t>(), c->prop.as<int32_t>(), c->dS.as<double>(), lo, BS,
                    o.accept == GSAP_ACCEPT_MH, c->olab.as<int32_t>(),
                    c->bm.B <= 65535 ? c->rs.lab16 : nullptr,
                    c->tiles.as<double>() + batch_tiles_off(c) +
                        (int64_t)b * batch_tiles_stride(c, o.num_batches),
-                   dE_dev, st);
+                   dE_dev, dm, st);
   c->prof.end(p, st);
   HIPX(hipGetLastError());
 }
@@ -1052,8 +1068,9 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
       if (use_graph) {
         if (!gg.exec) gg.capture([&] {
           for (int b = 0; b < o.num_batches; ++b) {
-            nodal_batch(c, o, b, PhaseRef{ctr, (uint32_t)b}, ar_level, nullptr);
-            rebuild_after_batch(c, B, o, b, b + 1 < o.num_batches);
+            const bool defer = defers_csr(c, o, B, b);
+            nodal_batch(c, o, b, PhaseRef{ctr, (uint32_t)b}, ar_level, nullptr, false, defer);
+            rebuild_after_batch(c, B, o, b, defer);
           }
           sweep_sums(c, o, res, ctr);
         });
@@ -1068,9 +1085,10 @@ void partition_impl(gsap_ctx *c, const gsap_graph_i32 *gin, const gsap_options &
           HIPX(hipMemcpyAsync(hbefore.data(), lab_b, 4 * (size_t)BS, hipMemcpyDeviceToHost, st));
         }
         // traced: every batch's sum at once (read below); else totalled at the sweep's end
+        const bool defer = defers_csr(c, o, B, b);
         nodal_batch(c, o, b, PhaseRef{ctr, (uint32_t)b}, ar, tr ? res + 1 + b : nullptr,
-                    b == 0 && terms_queued);
-        rebuild_after_batch(c, B, o, b, b + 1 < o.num_batches);
+                    b == 0 && terms_queued, defer);
+        rebuild_after_batch(c, B, o, b, defer);
         if (tr) {
           double e = 0;
           hprop.resize(BS);

@@ -1757,11 +1757,20 @@ __global__ __launch_bounds__(64) void k_final_sum(const double *tiles, const int
 // 256-thread workgroup (four entries per thread, then wave 0 sums the tile from LDS in the
 // canonical order): greedy (all = 0) takes prop where dS < 0, Metropolis-Hastings (all = 1)
 // takes prop; olab keeps the previous labels, lab16 (when given) follows the changed ones.
+// DEG (a batch whose CSR rebuild is deferred): a moved vertex's weighted out- and in-degree
+// leave its old block and enter its new one, summed per workgroup in LDS (DEG = 1, 2B words,
+// one global atomic per touched block) or added with global atomics (DEG = 2, large B).
+template <int DEG>
 __global__ __launch_bounds__(256) void k_apply_tiles(int32_t *lab, const int32_t *prop,
                                                      const double *dS, int64_t lo, int64_t BS,
                                                      int all, int32_t *olab, uint16_t *lab16,
-                                                     double *tiles) {
+                                                     double *tiles, DegMoves dm) {
   __shared__ double xs[1024];
+  extern __shared__ int32_t sdeg[];
+  if constexpr (DEG == 1) {
+    for (int32_t k = threadIdx.x; k < 2 * dm.B; k += 256) sdeg[k] = 0;
+    __syncthreads();
+  }
   const int64_t t0 = (int64_t)blockIdx.x * 1024;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -1776,11 +1785,31 @@ __global__ __launch_bounds__(256) void k_apply_tiles(int32_t *lab, const int32_t
         const int32_t nl = prop[i];
         lab[lo + i] = nl;
         if (lab16) lab16[lo + i] = (uint16_t)nl;
+        if (DEG && nl != old) {
+          const int64_t x = lo + i;
+          const int32_t ko = dm.kout ? dm.kout[x] : dm.optr[x + 1] - dm.optr[x];
+          const int32_t ki = dm.kin ? dm.kin[x] : dm.iptr[x + 1] - dm.iptr[x];
+          int32_t *po = DEG == 1 ? sdeg : dm.dout, *pi = DEG == 1 ? sdeg + dm.B : dm.din;
+          if (ko) {
+            atomicSub(&po[old], ko);
+            atomicAdd(&po[nl], ko);
+          }
+          if (ki) {
+            atomicSub(&pi[old], ki);
+            atomicAdd(&pi[nl], ki);
+          }
+        }
       }
     }
     xs[k] = d;
   }
   __syncthreads();
+  if constexpr (DEG == 1) {
+    for (int32_t b = threadIdx.x; b < dm.B; b += 256) {
+      if (sdeg[b]) atomicAdd(&dm.dout[b], sdeg[b]);
+      if (sdeg[dm.B + b]) atomicAdd(&dm.din[b], sdeg[dm.B + b]);
+    }
+  }
   if (threadIdx.x >= 64) return;
   const int lane = threadIdx.x;
   double p = 0.0;
@@ -1791,13 +1820,24 @@ __global__ __launch_bounds__(256) void k_apply_tiles(int32_t *lab, const int32_t
   if (lane == 0) tiles[blockIdx.x] = p;
 }
 
+constexpr int DEG_LDS_B = 4096;  // largest B whose degree moves are summed in LDS
+
 void launch_apply_sum(int32_t *lab, const int32_t *prop, const double *dS, int64_t lo, int64_t BS,
                       bool all, int32_t *olab, uint16_t *lab16, double *tiles, double *out,
-                      hipStream_t st) {
+                      const DegMoves &dm, hipStream_t st) {
   const int64_t nt = (BS + 1023) / 1024;
-  if (nt > 0)
-    k_apply_tiles<<<(unsigned)nt, 256, 0, st>>>(lab, prop, dS, lo, BS, all ? 1 : 0, olab, lab16,
-                                                tiles);
+  const int a = all ? 1 : 0;
+  if (nt > 0) {
+    if (!dm.dout)
+      k_apply_tiles<0><<<(unsigned)nt, 256, 0, st>>>(lab, prop, dS, lo, BS, a, olab, lab16, tiles,
+                                                     dm);
+    else if (dm.B <= DEG_LDS_B)
+      k_apply_tiles<1><<<(unsigned)nt, 256, (size_t)8 * dm.B, st>>>(lab, prop, dS, lo, BS, a,
+                                                                     olab, lab16, tiles, dm);
+    else
+      k_apply_tiles<2><<<(unsigned)nt, 256, 0, st>>>(lab, prop, dS, lo, BS, a, olab, lab16, tiles,
+                                                     dm);
+  }
   if (out) k_final_sum<<<1, 64, 0, st>>>(tiles, nullptr, BS, out, nullptr, 0);
 }
 
