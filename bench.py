@@ -12,7 +12,8 @@ timing.  The roofline object is for the largest
 single kernel, the fused nodal-move kernel (one launch per batch): its average launch time
 comes from HIP events on its stream in as many profiled partitions of the same workload.
 The CPU baseline times the oracle on the host's cores (and on one thread) on a bounded
-sample of the same workload.
+sample of the same workload: a few seconds of nodal batches on one labelling per
+block-count regime, combined over the GPU partition's own batches per regime.
 """
 import argparse
 import dataclasses
@@ -135,29 +136,74 @@ def cpu_threads():
     return max(1, min(16, n))
 
 
-def cpu_baseline(g, labels, B, seconds, threads):
-    """The oracle's nodal batches (each with its block-matrix rebuild) from the GPU's final
-    labelling, for about `seconds` (at most 1024 batches), on `threads` host threads."""
+# block-count regimes of a partition's nodal batches (the rebuild and lookup forms differ)
+REGIMES = [(16385, 1 << 30), (8193, 16384), (513, 8192), (129, 512), (1, 128)]
+
+
+def regime_of(B):
+    return next(k for k, (lo, hi) in enumerate(REGIMES) if lo <= B <= hi)
+
+
+def batch_mix(ctx, g, thr):
+    """Nodal batches per block-count regime in one traced partition of the workload (every
+    batch record of the trace carries its B)."""
+    ctx.partition(g, gsap_amd.Options(seed=1, trace=1, **thr))
+    n = [0] * len(REGIMES)
+    for rec in ctx.trace():
+        if rec[0] == 2:
+            n[regime_of(rec[1])] += 1
+    return n
+
+
+def regime_labellings(ctx, g, final_lab, final_B):
+    """A labelling per regime: above 128 blocks the first block count of the regime that the
+    merge phases alone reach from one block per vertex (on the GPU, the partition's merge
+    kernels and reduction rate, no sweeps in between); at most 128 blocks the partition's
+    final labelling."""
+    o = gsap_amd.Options(seed=1)
+    lab, B, m = np.arange(g.N, dtype=np.int32), g.N, 0
+    out = {regime_of(final_B): (final_lab, final_B)}
+    while B > 128:
+        r = ctx.merge_phase(g, o, lab, B, int(np.float32(B) * np.float32(o.reduction_rate)), m)
+        lab, B, m = r["labels"], r["B"], m + 1
+        out.setdefault(regime_of(B), (lab.copy(), B))
+    return out
+
+
+def cpu_baseline(g, labs, mix, seconds, threads):
+    """The oracle's nodal batches (each with its block-matrix rebuild), seconds / 5 on the
+    labelling of each block-count regime, on `threads` host threads; the regime rates are
+    combined over the GPU partition's own batch mix (the time the oracle would take for those
+    batches), so CPU and GPU values cover the same work."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     opts = gsap_amd.Options(seed=1)
-    lab = labels.copy()
-    t0 = time.perf_counter()
-    nbat = 0
-    while True:
-        if threads == 1:
-            O.nodal_batch(g, opts, lab, B, nbat % opts.num_batches, 100000 + nbat)
-        else:
-            O.nodal_batch_mt(g, opts, lab, B, nbat % opts.num_batches, 100000 + nbat, threads)
-        nbat += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or nbat >= 1024:
-            break
-    sweeps = nbat / opts.num_batches
-    return dict(value=g.E * sweeps / el, unit="edges/s", cores=threads, kind="port",
-                sample=f"{nbat} nodal batches (each with its block-matrix rebuild) of the CPU "
-                       f"oracle on {threads} thread(s), same 200k-vertex graph from the final "
-                       f"labelling (B={B}), {el:.1f} s")
+    per_batch, parts = {}, []
+    for k, (lab, B) in sorted(labs.items()):
+        t0 = time.perf_counter()
+        nbat = 0
+        while True:
+            if threads == 1:
+                O.nodal_batch(g, opts, lab, B, nbat % opts.num_batches, 100000 + nbat)
+            else:
+                O.nodal_batch_mt(g, opts, lab, B, nbat % opts.num_batches, 100000 + nbat, threads)
+            nbat += 1
+            el = time.perf_counter() - t0
+            if el >= seconds / len(labs) or nbat >= 256:
+                break
+        per_batch[k] = el / nbat
+        parts.append(f"B={B}: {nbat} batches in {el:.1f} s")
+    edges_per_batch = g.E / opts.num_batches
+    t_cpu = sum(mix[k] * per_batch[k] for k in per_batch)
+    n_all = sum(mix[k] for k in per_batch)
+    return dict(value=n_all * edges_per_batch / t_cpu, unit="edges/s", cores=threads,
+                kind="port",
+                sample=f"nodal batches (each with its block-matrix rebuild) of the CPU oracle on "
+                       f"{threads} thread(s), same 200k-vertex graph, one labelling per "
+                       f"block-count regime ({'; '.join(parts)}), combined over the GPU "
+                       f"partition's batches per regime {dict(zip(map(str, REGIMES), mix))}",
+                regime_edges_per_s={str(REGIMES[k]): edges_per_batch / per_batch[k]
+                                    for k in per_batch})
 
 
 def compare_forms(ctx, g, lab, prec, thr):
@@ -505,8 +551,10 @@ def main():
         out["precision_forms"] = compare_forms(ctx, g, lab, prec, thr)
     if not a.no_cpu_baseline:
         nt = cpu_threads()
-        out["cpu_baseline"] = cpu_baseline(g, lab, rep.B, a.cpu_seconds, nt)
-        out["cpu_baseline_1thread"] = cpu_baseline(g, lab, rep.B, a.cpu_seconds, 1)
+        mix = batch_mix(ctx, g, thr)
+        labs = regime_labellings(ctx, g, lab, rep.B)
+        out["cpu_baseline"] = cpu_baseline(g, labs, mix, a.cpu_seconds, nt)
+        out["cpu_baseline_1thread"] = cpu_baseline(g, labs, mix, a.cpu_seconds, 1)
     ctx.close()
     print(json.dumps(out))
 
