@@ -404,6 +404,49 @@ GSAP_D int32_t dense_walk(const int32_t *p, int64_t stride, int32_t n, uint64_t 
   return fallback;
 }
 
+// The same entry found from the end: with total = the weights' sum (> target), it is the
+// last entry whose suffix sum (itself included) reaches total - target, so a target in the
+// upper half of the mass is found reading the row's tail instead of its head.  Rounds of
+// G x U entries from the end; inside the round holding the crossing, chunks from the last
+// and, in a chunk, the highest lane whose suffix reaches the need.
+template <int G, int U>
+GSAP_D int32_t dense_walk_back(const int32_t *p, int64_t stride, int32_t n, uint64_t target,
+                               uint64_t total, int32_t fallback) {
+  const int q = threadIdx.x & (G - 1);
+  const int gbase = (threadIdx.x & 63) & ~(G - 1);
+  const uint64_t gm = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+  const uint64_t need = total - target;
+  uint64_t base = 0;  // weight after the current round
+  for (int32_t c = n - G * U; c > -G * U; c -= G * U) {
+    uint32_t w[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int32_t k = c + u * G + q;
+      w[u] = k >= 0 ? (uint32_t)p[(int64_t)k * stride] : 0u;
+    }
+    uint64_t s = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) s += w[u];
+    const uint64_t tot = group_allsum<G>(s);
+    if (base + tot >= need) {
+#pragma unroll
+      for (int u = U - 1; u >= 0; --u) {
+        uint64_t suf = w[u];  // this lane's weight and those of the higher lanes
+#pragma unroll
+        for (int off = 1; off < G; off <<= 1) {
+          const uint64_t y = __shfl_down(suf, off, G);
+          if (q + off < G) suf += y;
+        }
+        const uint64_t bal = (__ballot(base + suf >= need) >> gbase) & gm;
+        if (bal) return c + u * G + (63 - __clzll((long long)bal));
+        base += __shfl(suf, 0, G);
+      }
+    }
+    base += tot;
+  }
+  return fallback;
+}
+
 // The nodal per-block neighbour sample and coin (kernels.hpp:942-997, 1024-1029) while the
 // CSR/CSC lags the labels (between the batches of a sweep at a dense B): the walk over row x
 // and then column x of the dense matrix D, column x read as row x of its transpose DT when
@@ -428,12 +471,15 @@ __global__ __launch_bounds__(256) void k_block_terms_dense(const int32_t *D, con
   int32_t so = 0;
   if (deg > 0) {
     const uint64_t target = mulhi(y.x0, deg);
-    if (target < (uint64_t)dox)
-      so = dense_walk<G, U>(D + (int64_t)x * B, 1, B, target, 0);
-    else if (DT)
-      so = dense_walk<G, U>(DT + (int64_t)x * B, 1, B, target - (uint64_t)dox, 0);
-    else
-      so = dense_walk<G, U>(D + x, B, B, target - (uint64_t)dox, 0);
+    // the part holding the target (row x below d_out[x], else column x), walked from the
+    // end nearer to it in weight
+    const bool in_row = target < (uint64_t)dox;
+    const uint64_t t = in_row ? target : target - (uint64_t)dox;
+    const uint64_t tot = in_row ? (uint64_t)dox : (uint64_t)dix;
+    const int32_t *p = in_row ? D + (int64_t)x * B : DT ? DT + (int64_t)x * B : D + x;
+    const int64_t stride = in_row || DT ? 1 : B;
+    so = 2 * t < tot ? dense_walk<G, U>(p, stride, B, t, 0)
+                     : dense_walk_back<G, U>(p, stride, B, t, tot, 0);
   }
   if (q == 0) {
     sof[x] = so;
