@@ -157,51 +157,6 @@ GSAP_D void seg_for3(int32_t n, A &&a, Bf &&b, C &&c) {
   }
 }
 
-// Group-parallel form of bm_walk: entries of row x then column x in chunks of G lanes, an
-// inclusive prefix sum of the weights inside the group, and the first lane whose running
-// weight exceeds target with an id other than skip.  Same result as the sequential walk.
-// Four chunks' loads are issued together, then the chunks are scanned in order.
-template <int G>
-GSAP_D int32_t group_walk(const BMView &m, int32_t x, uint64_t target, int32_t skip,
-                          int32_t fallback) {
-  constexpr int U = 4;
-  const int q = threadIdx.x & (G - 1);
-  const int gbase = (threadIdx.x & 63) & ~(G - 1);
-  const int32_t r0 = m.rowptr[x], nr = m.rowptr[x + 1] - r0;
-  const int32_t c0 = m.colptr[x], n = nr + (m.colptr[x + 1] - c0);
-  uint64_t base = 0;
-  for (int32_t c = 0; c < n; c += U * G) {
-    uint32_t w[U];
-    int32_t id[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int32_t k = c + u * G + q;
-      w[u] = 0;
-      id[u] = skip;
-      if (k < nr) { w[u] = (uint32_t)m.val[r0 + k]; id[u] = m.col[r0 + k]; }
-      else if (k < n) { w[u] = (uint32_t)m.cval[c0 + k - nr]; id[u] = m.row[c0 + k - nr]; }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int32_t k = c + u * G + q;
-      uint64_t incl = w[u];
-#pragma unroll
-      for (int off = 1; off < G; off <<= 1) {
-        uint64_t y = __shfl_up(incl, off, G);
-        if (q >= off) incl += y;
-      }
-      const bool pred = k < n && base + incl > target && id[u] != skip;
-      const uint64_t bal = (__ballot(pred) >> gbase) & (G == 64 ? ~0ull : ((1ull << G) - 1ull));
-      if (bal) {
-        const int first = __ffsll((long long)bal) - 1;
-        return __shfl(id[u], first, G);
-      }
-      base += __shfl(incl, G - 1, G);
-    }
-  }
-  return fallback;
-}
-
 GSAP_D int64_t wave_sum_i64(int64_t x) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) x += __shfl_down(x, off, 64);
@@ -211,6 +166,131 @@ GSAP_D int64_t wave_sum_i64(int64_t x) {
 static unsigned grid_for(int64_t n, int threads) {
   int64_t g = (n + threads - 1) / threads;
   return (unsigned)(g < 1 ? 1 : g);
+}
+
+// sum over the G lanes of a group, in every lane
+template <int G, class T>
+GSAP_D T group_allsum(T x) {
+#pragma unroll
+  for (int off = G / 2; off >= 1; off >>= 1) x += __shfl_xor(x, off, G);
+  return x;
+}
+
+// First entry of p[0], p[stride], ..., p[(n - 1) stride] whose running weight exceeds
+// target, in rounds of G x U entries: every lane sums its U loads (all in flight), the group
+// total decides whether the crossing lies in this round, and only that round is scanned in
+// order.  A zero entry never holds the crossing, so this equals the walk over the same row's
+// nonzero entries in the CSR (or CSC) order.  fallback when the weights never exceed target.
+template <int G, int U>
+GSAP_D int32_t dense_walk(const int32_t *p, int64_t stride, int32_t n, uint64_t target,
+                          int32_t fallback) {
+  const int q = threadIdx.x & (G - 1);
+  const int gbase = (threadIdx.x & 63) & ~(G - 1);
+  const uint64_t gm = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+  uint64_t base = 0;
+  for (int32_t c = 0; c < n; c += G * U) {
+    uint32_t w[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int32_t k = c + u * G + q;
+      w[u] = k < n ? (uint32_t)p[(int64_t)k * stride] : 0u;
+    }
+    uint64_t s = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) s += w[u];
+    const uint64_t tot = group_allsum<G>(s);
+    if (base + tot > target) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        uint64_t incl = w[u];
+#pragma unroll
+        for (int off = 1; off < G; off <<= 1) {
+          const uint64_t y = __shfl_up(incl, off, G);
+          if (q >= off) incl += y;
+        }
+        const uint64_t bal = (__ballot(base + incl > target) >> gbase) & gm;
+        if (bal) return c + u * G + (__ffsll((long long)bal) - 1);
+        base += __shfl(incl, G - 1, G);
+      }
+    }
+    base += tot;
+  }
+  return fallback;
+}
+
+// The same entry found from the end: with total = the weights' sum (> target), it is the
+// last entry whose suffix sum (itself included) reaches total - target, so a target in the
+// upper half of the mass is found reading the row's tail instead of its head.  Rounds of
+// G x U entries from the end; inside the round holding the crossing, chunks from the last
+// and, in a chunk, the highest lane whose suffix reaches the need.
+template <int G, int U>
+GSAP_D int32_t dense_walk_back(const int32_t *p, int64_t stride, int32_t n, uint64_t target,
+                               uint64_t total, int32_t fallback) {
+  const int q = threadIdx.x & (G - 1);
+  const int gbase = (threadIdx.x & 63) & ~(G - 1);
+  const uint64_t gm = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+  const uint64_t need = total - target;
+  uint64_t base = 0;  // weight after the current round
+  for (int32_t c = n - G * U; c > -G * U; c -= G * U) {
+    uint32_t w[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int32_t k = c + u * G + q;
+      w[u] = k >= 0 ? (uint32_t)p[(int64_t)k * stride] : 0u;
+    }
+    uint64_t s = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) s += w[u];
+    const uint64_t tot = group_allsum<G>(s);
+    if (base + tot >= need) {
+#pragma unroll
+      for (int u = U - 1; u >= 0; --u) {
+        uint64_t suf = w[u];  // this lane's weight and those of the higher lanes
+#pragma unroll
+        for (int off = 1; off < G; off <<= 1) {
+          const uint64_t y = __shfl_down(suf, off, G);
+          if (q + off < G) suf += y;
+        }
+        const uint64_t bal = (__ballot(base + suf >= need) >> gbase) & gm;
+        if (bal) return c + u * G + (63 - __clzll((long long)bal));
+        base += __shfl(suf, 0, G);
+      }
+    }
+    base += tot;
+  }
+  return fallback;
+}
+
+// Group-parallel form of bm_walk (row x then column x; the first entry whose running weight
+// exceeds target, skipping id skip with its mass carried forward): the target lies in the row
+// when it is below the row's total d_out[x], else in the column at target - d_out[x]; that
+// part's weights alone are walked (from the end nearer to the target in weight), which finds
+// the crossing entry; the answer is the first entry from there whose id is not skip.  Same
+// result as the sequential walk (every stored entry weighs at least 1).
+template <int G>
+GSAP_D int32_t group_walk(const BMView &m, int32_t x, uint64_t target, int32_t skip,
+                          int32_t fallback) {
+  const int32_t r0 = m.rowptr[x], nr = m.rowptr[x + 1] - r0;
+  const int32_t c0 = m.colptr[x], nc = m.colptr[x + 1] - c0;
+  const uint64_t dox = (uint64_t)m.dout[x], dix = (uint64_t)m.din[x];
+  if (target >= dox + dix) return fallback;
+  bool row = target < dox;
+  const uint64_t t = row ? target : target - dox;
+  const uint64_t tot = row ? dox : dix;
+  const int32_t *w = row ? m.val + r0 : m.cval + c0;
+  const int32_t n = row ? nr : nc;
+  int32_t k = 2 * t < tot ? dense_walk<G, 4>(w, 1, n, t, 0)
+                          : dense_walk_back<G, 4>(w, 1, n, t, tot, 0);
+  while (true) {
+    const int32_t id = row ? m.col[r0 + k] : m.row[c0 + k];
+    if (id != skip) return id;
+    ++k;
+    if (row && k == nr) {
+      row = false;
+      k = 0;
+    }
+    if (!row && k >= nc) return fallback;
+  }
 }
 
 // ------------------------------------------------ per-block terms and samples
@@ -352,99 +432,6 @@ __global__ __launch_bounds__(1024) void k_block_walk_big(BMView m, int32_t *sof,
     }
     __syncthreads();  // the list is rewritten by the next range of blocks
   }
-}
-
-// sum over the G lanes of a group, in every lane
-template <int G, class T>
-GSAP_D T group_allsum(T x) {
-#pragma unroll
-  for (int off = G / 2; off >= 1; off >>= 1) x += __shfl_xor(x, off, G);
-  return x;
-}
-
-// First entry of p[0], p[stride], ..., p[(n - 1) stride] whose running weight exceeds
-// target, in rounds of G x U entries: every lane sums its U loads (all in flight), the group
-// total decides whether the crossing lies in this round, and only that round is scanned in
-// order.  A zero entry never holds the crossing, so this equals the walk over the same row's
-// nonzero entries in the CSR (or CSC) order.  fallback when the weights never exceed target.
-template <int G, int U>
-GSAP_D int32_t dense_walk(const int32_t *p, int64_t stride, int32_t n, uint64_t target,
-                          int32_t fallback) {
-  const int q = threadIdx.x & (G - 1);
-  const int gbase = (threadIdx.x & 63) & ~(G - 1);
-  const uint64_t gm = G == 64 ? ~0ull : ((1ull << G) - 1ull);
-  uint64_t base = 0;
-  for (int32_t c = 0; c < n; c += G * U) {
-    uint32_t w[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int32_t k = c + u * G + q;
-      w[u] = k < n ? (uint32_t)p[(int64_t)k * stride] : 0u;
-    }
-    uint64_t s = 0;
-#pragma unroll
-    for (int u = 0; u < U; ++u) s += w[u];
-    const uint64_t tot = group_allsum<G>(s);
-    if (base + tot > target) {
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        uint64_t incl = w[u];
-#pragma unroll
-        for (int off = 1; off < G; off <<= 1) {
-          const uint64_t y = __shfl_up(incl, off, G);
-          if (q >= off) incl += y;
-        }
-        const uint64_t bal = (__ballot(base + incl > target) >> gbase) & gm;
-        if (bal) return c + u * G + (__ffsll((long long)bal) - 1);
-        base += __shfl(incl, G - 1, G);
-      }
-    }
-    base += tot;
-  }
-  return fallback;
-}
-
-// The same entry found from the end: with total = the weights' sum (> target), it is the
-// last entry whose suffix sum (itself included) reaches total - target, so a target in the
-// upper half of the mass is found reading the row's tail instead of its head.  Rounds of
-// G x U entries from the end; inside the round holding the crossing, chunks from the last
-// and, in a chunk, the highest lane whose suffix reaches the need.
-template <int G, int U>
-GSAP_D int32_t dense_walk_back(const int32_t *p, int64_t stride, int32_t n, uint64_t target,
-                               uint64_t total, int32_t fallback) {
-  const int q = threadIdx.x & (G - 1);
-  const int gbase = (threadIdx.x & 63) & ~(G - 1);
-  const uint64_t gm = G == 64 ? ~0ull : ((1ull << G) - 1ull);
-  const uint64_t need = total - target;
-  uint64_t base = 0;  // weight after the current round
-  for (int32_t c = n - G * U; c > -G * U; c -= G * U) {
-    uint32_t w[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int32_t k = c + u * G + q;
-      w[u] = k >= 0 ? (uint32_t)p[(int64_t)k * stride] : 0u;
-    }
-    uint64_t s = 0;
-#pragma unroll
-    for (int u = 0; u < U; ++u) s += w[u];
-    const uint64_t tot = group_allsum<G>(s);
-    if (base + tot >= need) {
-#pragma unroll
-      for (int u = U - 1; u >= 0; --u) {
-        uint64_t suf = w[u];  // this lane's weight and those of the higher lanes
-#pragma unroll
-        for (int off = 1; off < G; off <<= 1) {
-          const uint64_t y = __shfl_down(suf, off, G);
-          if (q + off < G) suf += y;
-        }
-        const uint64_t bal = (__ballot(base + suf >= need) >> gbase) & gm;
-        if (bal) return c + u * G + (63 - __clzll((long long)bal));
-        base += __shfl(suf, 0, G);
-      }
-    }
-    base += tot;
-  }
-  return fallback;
 }
 
 // The nodal per-block neighbour sample and coin (kernels.hpp:942-997, 1024-1029) while the
