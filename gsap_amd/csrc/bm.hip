@@ -1256,7 +1256,8 @@ __global__ __launch_bounds__(256) void k_dense_moves(const int32_t *optr, const 
                                                      const int32_t *iadj, const int32_t *iw,
                                                      const int32_t *lab, const int32_t *olab,
                                                      int64_t lo, int64_t BS, int32_t B,
-                                                     int32_t *D, int32_t *DT) {
+                                                     int32_t *D, int32_t *DT, int32_t *CS,
+                                                     int32_t nch) {
   const int64_t i = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
   const int q = threadIdx.x & 15;
   if (i >= BS) return;
@@ -1274,6 +1275,10 @@ __global__ __launch_bounds__(256) void k_dense_moves(const int32_t *optr, const 
       atomicSub(&DT[(int64_t)yo * B + xo], w);
       atomicAdd(&DT[(int64_t)yn * B + xn], w);
     }
+    if (CS) {
+      atomicSub(&CS[(int64_t)yo * nch + (xo >> 6)], w);
+      atomicAdd(&CS[(int64_t)yn * nch + (xn >> 6)], w);
+    }
   }
   for (int32_t j = iptr[x] + q; j < iptr[x + 1]; j += 16) {
     const int32_t y = iadj[j], w = iw[j];  // edge y -> x
@@ -1286,7 +1291,31 @@ __global__ __launch_bounds__(256) void k_dense_moves(const int32_t *optr, const 
       atomicSub(&DT[(int64_t)xo * B + yl], w);
       atomicAdd(&DT[(int64_t)xn * B + yl], w);
     }
+    if (CS) {
+      atomicSub(&CS[(int64_t)xo * nch + (yl >> 6)], w);
+      atomicAdd(&CS[(int64_t)xn * nch + (yl >> 6)], w);
+    }
   }
+}
+
+// Column chunk sums of the block matrix from its CSC: CS[x * nch + c] = the weight of
+// column x in rows 64 c .. 64 c + 63.  A wave per column sums its entries into LDS and
+// writes all nch words of the column (no clearing pass).
+__global__ __launch_bounds__(256) void k_colsum_build(const int32_t *colptr, const int32_t *row,
+                                                      const int32_t *cval, int32_t B, int32_t nch,
+                                                      int32_t *CS) {
+  extern __shared__ int32_t scs[];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int32_t *cs = scs + w * nch;
+  const int32_t x = (int32_t)(blockIdx.x * 4 + w);
+  for (int32_t c = lane; c < nch; c += 64) cs[c] = 0;
+  wave_lds_sync();
+  if (x < B)
+    for (int32_t k = colptr[x] + lane; k < colptr[x + 1]; k += 64)
+      atomicAdd(&cs[row[k] >> 6], cval[k]);
+  wave_lds_sync();
+  if (x < B)
+    for (int32_t c = lane; c < nch; c += 64) CS[(int64_t)x * nch + c] = cs[c];
 }
 
 // B <= SMALL_B: the same edge moves summed in an LDS copy of the B x B matrix per workgroup,
@@ -1410,7 +1439,7 @@ __global__ __launch_bounds__(512) void k_dense_moves_tile(const int32_t *optr, c
 
 bool dense_update_ok(const BMBuf &bm, const RebuildScratch &rs, int32_t B) {
   return bm.dense_valid && bm.dense_B == B && bm.B == B && B <= INC_MAX_B &&
-         (int64_t)B * B <= rs.partials_cap;
+         (B > TAIL_MAX_B || (int64_t)B * B <= rs.partials_cap);
 }
 
 bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t *olab, int64_t lo,
@@ -1421,8 +1450,18 @@ bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t 
   // (B <= INC_TILE_B, where global atomics would contend on few cells), or added with global
   // atomics (spread over many cells at larger B)
   if (!dense_update_ok(bm, rs, B)) return false;
-  if (defer_csr && B > INC_TILE_B && bm.dt_B != B) {
-    // the transpose follows the moves from here on: the deferred samples walk its rows
+  // above TAIL_MAX_B no transpose is kept: the column walks read D's column chunk sums,
+  // built here from the CSC (current: the level's full rebuild or a sweep's last batch
+  // preceded) and kept by the moves; below, the transpose follows the moves from the first
+  // deferred batch on
+  const bool cs_form = B > TAIL_MAX_B;
+  const int32_t nch = (B + 63) / 64;
+  if (cs_form && bm.cs_B != B) {
+    if (bm.csr_stale || (int64_t)B * nch > rs.colsum_cap) return false;
+    k_colsum_build<<<(unsigned)((B + 3) / 4), 256, (size_t)16 * nch, st>>>(
+        bm.colptr, bm.row, bm.cval, B, nch, rs.colsum);
+    bm.cs_B = B;
+  } else if (!cs_form && defer_csr && B > INC_TILE_B && bm.dt_B != B) {
     const unsigned tb = (unsigned)((B + 31) / 32);
     k_transpose<<<dim3(tb, tb), 256, 0, st>>>(bm.dense, B, rs.partials);
     bm.dt_B = B;
@@ -1437,10 +1476,11 @@ bool rebuild_after_moves(const GraphDev &g, const int32_t *lab32, const int32_t 
     k_dense_moves_tile<<<(unsigned)(tiles * nchunk), 512, 0, st>>>(
         g.optr, g.oadj, g.ow, g.iptr, g.iadj, g.iw, lab32, olab, lo, BS, B, nchunk, bm.dense);
   }
-  else if (BS > 0)  // the transpose follows when it is current for this B
+  else if (BS > 0)  // the transpose (or the column chunk sums) follow when current
     k_dense_moves<<<(unsigned)((BS + 15) / 16), 256, 0, st>>>(
         g.optr, g.oadj, g.ow, g.iptr, g.iadj, g.iw, lab32, olab, lo, BS, B, bm.dense,
-        bm.dt_B == B ? rs.partials : nullptr);
+        !cs_form && bm.dt_B == B ? rs.partials : nullptr,
+        cs_form && bm.cs_B == B ? rs.colsum : nullptr, nch);
   if (B <= INC_TILE_B) bm.dt_B = 0;  // the LDS forms change D alone
   if (defer_csr) {
     bm.csr_stale = true;
@@ -2165,6 +2205,7 @@ void rebuild_blockmodel(const GraphDev &g, const int32_t *lab32, int32_t B, BMBu
   bm.dense_valid = dense;
   bm.dense_B = dense ? B : 0;
   bm.csr_stale = false;
+  bm.cs_B = 0;  // D rewritten: the column chunk sums are rebuilt from the new CSC
   // lookups without the dense copy search long rows through the coarse index (the
   // segmented path, the only one without it, writes the index with the entries)
   bm.index_valid = !dense && bm.gcol && bm.grow;

@@ -43,6 +43,8 @@ struct BMBuf {
   // the transpose of the dense matrix in the rebuild scratch's partial-plane buffer is
   // current for B = dt_B (0: not current); the incremental update keeps it so
   int32_t dt_B = 0;
+  // the dense matrix's column chunk sums (rebuild scratch colsum) are current for B = cs_B
+  int32_t cs_B = 0;
   // B of the entries the dense buffer holds (the current CSR's); 0 = all zero
   int32_t dense_B = 0;
   // the dense matrix and the block degrees are current but the CSR/CSC (and nnz in
@@ -82,6 +84,10 @@ struct RebuildScratch {
   int64_t tmp_words = 0;
   int32_t *partials = nullptr;        // partial planes of the small-B histogram
   int64_t partials_cap = 0;           // words
+  // above 8192 blocks (dense): colsum[x * ceil(B / 64) + c] = sum of column x of the dense
+  // matrix over rows 64 c .. 64 c + 63 (the column walks' index; no transpose is kept there)
+  int32_t *colsum = nullptr;
+  int64_t colsum_cap = 0;             // words
   uint32_t *soff = nullptr;           // 2 Bmax + 1: segment sizes, then offsets
   uint32_t *su = nullptr;             // 2 Bmax: runs per segment
   uint32_t *vrel = nullptr;           // 2N: each vertex's slice start inside its row / column segment
@@ -168,10 +174,11 @@ void launch_block_terms(const BMView &m, double *OUT, int32_t *sof, uint8_t *coi
 // The per-block samples and coins from the dense matrix D (B x B, row-major) and, when not
 // null, its transpose DT, for batches whose CSR/CSC lags the labels; also writes the log2
 // block degrees of dout / din.
-void launch_block_terms_dense(const int32_t *D, const int32_t *DT, int32_t B, const int32_t *dout,
-                              const int32_t *din, double *ldout, double *ldin, int32_t *sof,
-                              uint8_t *coin, RngKey key, PhaseRef phase, hipStream_t st,
-                              int32_t *zero);
+// (without DT, CS: the column chunk sums of D, 64 rows per chunk, for the column walks)
+void launch_block_terms_dense(const int32_t *D, const int32_t *DT, const int32_t *CS, int32_t B,
+                              const int32_t *dout, const int32_t *din, double *ldout,
+                              double *ldin, int32_t *sof, uint8_t *coin, RngKey key,
+                              PhaseRef phase, hipStream_t st, int32_t *zero);
 // Nodal kernels by the larger of a vertex's out- and in-neighbour counts d: d <= 16 and 32
 // use register deduplication in groups of 16 and 32 lanes; d <= 128 a wave with 256-slot
 // LDS hash tables; larger d a 256-thread workgroup with tables sized for the vertex (LDS

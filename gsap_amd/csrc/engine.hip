@@ -196,7 +196,7 @@ struct gsap_ctx {
   // runs
   DBuf rowptr2, colptr2, col2, val2, erow2, row2, cval2, gcol2, grow2, rk, rw, mwork;
   // rebuild scratch
-  DBuf k0, k1, tmp, partials, soff, su, suoff, lists, ncl, cntc, vrel, chunk_seg, lab16;
+  DBuf k0, k1, tmp, partials, soff, su, suoff, lists, ncl, cntc, vrel, chunk_seg, lab16, colsum;
   // sweep / merge
   DBuf OUT, sof, coin, prop, dS, best, dSbest, map, tiles, res;
   // device batch counter (Philox phase of the sweep's first batch); merge-order sort
@@ -523,12 +523,15 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   // counters) and of the row-tile histogram (up to 8M counters), then the transpose
   const int64_t sb = std::min<int64_t>(dB, 128);
   // (and the transpose of the dense matrix, which the row-tile and incremental paths
-  // keep up to 16384 blocks: at most 16384 x 16384 words)
-  const int64_t tB = std::min<int64_t>(dB, 16384);
+  // build up to 8192 blocks: at most 8192 x 8192 words)
+  const int64_t tB = std::min<int64_t>(dB, 8192);
   const int64_t pc = std::max<int64_t>(
       std::max<int64_t>(1024 * sb * sb, tB * tB),
       dB > 128 ? std::min<int64_t>(E / 2, (int64_t)32 << 20) : 0);
   c->partials.ensure(4 * pc);
+  // column chunk sums of the dense matrix above 8192 blocks
+  const int64_t csw = dB > 8192 ? dB * ((dB + 63) / 64) : 1;
+  c->colsum.ensure(4 * csw);
   c->soff.ensure(4 * (2 * N + 1));
   c->chunk_seg.ensure(4 * (2 * E / 64 + 2));
   c->su.ensure(4 * 2 * N); c->suoff.ensure(4 * (2 * N + 1)); c->vrel.ensure(4 * 2 * N);
@@ -565,6 +568,7 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
   r.k0 = c->k0.p; r.k1 = c->k1.p; r.tmp = c->tmp.as<uint32_t>();
   r.tmp_words = (int64_t)(c->tmp.cap / 4);
   r.partials = c->partials.as<int32_t>(); r.partials_cap = pc;
+  r.colsum = c->colsum.as<int32_t>(); r.colsum_cap = (int64_t)(c->colsum.cap / 4);
   r.soff = c->soff.as<uint32_t>();
   r.su = c->su.as<uint32_t>(); r.suoff = c->suoff.as<uint32_t>(); r.vrel = c->vrel.as<uint32_t>(); r.chunk_seg = c->chunk_seg.as<uint32_t>(); r.lab16 = c->lab16.as<uint16_t>();
   r.lists = c->lists.as<uint32_t>(); r.lcap = 2 * N; r.ncl = c->ncl.as<uint32_t>();
@@ -823,7 +827,8 @@ void batch_terms(gsap_ctx *c, const gsap_options &o, PhaseRef phase, int32_t ar)
   int p = c->prof.begin(2, c->st);
   const BMBuf &bm = c->bm;
   if (bm.csr_stale)  // the samples from the dense matrix (and the log2 degrees with them)
-    launch_block_terms_dense(bm.dense, bm.dt_B == bm.B ? c->rs.partials : nullptr, bm.B,
+    launch_block_terms_dense(bm.dense, bm.dt_B == bm.B ? c->rs.partials : nullptr,
+                             bm.cs_B == bm.B ? c->rs.colsum : nullptr, bm.B,
                              bm.dout, bm.din, bm.ldout, bm.ldin, c->sof.as<int32_t>(),
                              c->coin.as<uint8_t>(), rng_key(o), phase, c->st,
                              c->dcnt.as<int32_t>());
@@ -1337,7 +1342,7 @@ int gsap_destroy(gsap_ctx *c) {
                  &c->erow2, &c->row2, &c->cval2, &c->gcol2, &c->grow2, &c->rk, &c->rw, &c->mwork,
                  &c->k0,
                  &c->k1, &c->tmp, &c->partials, &c->soff, &c->su, &c->suoff,
-                 &c->lists, &c->ncl, &c->cntc, &c->vrel, &c->chunk_seg, &c->lab16, &c->vlist,
+                 &c->lists, &c->ncl, &c->cntc, &c->vrel, &c->chunk_seg, &c->lab16, &c->colsum, &c->vlist,
                  &c->OUT, &c->sof, &c->coin, &c->prop, &c->dS, &c->best, &c->dSbest, &c->map,
                  &c->tiles, &c->res, &c->ctr, &c->mk0, &c->mk1, &c->mv0,
                  &c->mv1, &c->errv, &c->evcls, &c->mixstat, &c->evstat, &c->olab};

@@ -183,7 +183,7 @@ GSAP_D T group_allsum(T x) {
 // nonzero entries in the CSR (or CSC) order.  fallback when the weights never exceed target.
 template <int G, int U>
 GSAP_D int32_t dense_walk(const int32_t *p, int64_t stride, int32_t n, uint64_t target,
-                          int32_t fallback) {
+                          int32_t fallback, uint64_t *before = nullptr) {
   const int q = threadIdx.x & (G - 1);
   const int gbase = (threadIdx.x & 63) & ~(G - 1);
   const uint64_t gm = G == 64 ? ~0ull : ((1ull << G) - 1ull);
@@ -209,7 +209,12 @@ GSAP_D int32_t dense_walk(const int32_t *p, int64_t stride, int32_t n, uint64_t 
           if (q >= off) incl += y;
         }
         const uint64_t bal = (__ballot(base + incl > target) >> gbase) & gm;
-        if (bal) return c + u * G + (__ffsll((long long)bal) - 1);
+        if (bal) {
+          const int first = __ffsll((long long)bal) - 1;
+          // (before: the weight ahead of the entry found)
+          if (before) *before = base + __shfl(incl - w[u], first, G);
+          return c + u * G + first;
+        }
         base += __shfl(incl, G - 1, G);
       }
     }
@@ -442,6 +447,7 @@ __global__ __launch_bounds__(1024) void k_block_walk_big(BMView m, int32_t *sof,
 // Also writes the log2 block degrees of the degrees the batch update left in dout / din.
 template <int G, int U>
 __global__ __launch_bounds__(256) void k_block_terms_dense(const int32_t *D, const int32_t *DT,
+                                                           const int32_t *CS,
                                                            int32_t B, const int32_t *dout,
                                                            const int32_t *din, double *ldout,
                                                            double *ldin, int32_t *sof,
@@ -463,10 +469,19 @@ __global__ __launch_bounds__(256) void k_block_terms_dense(const int32_t *D, con
     const bool in_row = target < (uint64_t)dox;
     const uint64_t t = in_row ? target : target - (uint64_t)dox;
     const uint64_t tot = in_row ? (uint64_t)dox : (uint64_t)dix;
-    const int32_t *p = in_row ? D + (int64_t)x * B : DT ? DT + (int64_t)x * B : D + x;
-    const int64_t stride = in_row || DT ? 1 : B;
-    so = 2 * t < tot ? dense_walk<G, U>(p, stride, B, t, 0)
-                     : dense_walk_back<G, U>(p, stride, B, t, tot, 0);
+    if (!in_row && !DT && CS) {
+      // column x through its chunk sums: the chunk holding the crossing, then its 64 rows
+      const int32_t nch = (B + 63) / 64;
+      uint64_t ahead = 0;
+      const int32_t c = dense_walk<G, U>(CS + (int64_t)x * nch, 1, nch, t, 0, &ahead);
+      const int32_t r0 = c * 64, nr = B - r0 < 64 ? B - r0 : 64;
+      so = r0 + dense_walk<G, U>(D + (int64_t)r0 * B + x, B, nr, t - ahead, 0);
+    } else {
+      const int32_t *p = in_row ? D + (int64_t)x * B : DT ? DT + (int64_t)x * B : D + x;
+      const int64_t stride = in_row || DT ? 1 : B;
+      so = 2 * t < tot ? dense_walk<G, U>(p, stride, B, t, 0)
+                       : dense_walk_back<G, U>(p, stride, B, t, tot, 0);
+    }
   }
   if (q == 0) {
     sof[x] = so;
@@ -476,18 +491,18 @@ __global__ __launch_bounds__(256) void k_block_terms_dense(const int32_t *D, con
   }
 }
 
-void launch_block_terms_dense(const int32_t *D, const int32_t *DT, int32_t B, const int32_t *dout,
-                              const int32_t *din, double *ldout, double *ldin, int32_t *sof,
-                              uint8_t *coin, RngKey key, PhaseRef phase, hipStream_t st,
-                              int32_t *zero) {
+void launch_block_terms_dense(const int32_t *D, const int32_t *DT, const int32_t *CS, int32_t B,
+                              const int32_t *dout, const int32_t *din, double *ldout,
+                              double *ldin, int32_t *sof, uint8_t *coin, RngKey key,
+                              PhaseRef phase, hipStream_t st, int32_t *zero) {
   if (B <= 128)
-    k_block_terms_dense<16, 4><<<grid_for(B, 16), 256, 0, st>>>(D, DT, B, dout, din, ldout,
+    k_block_terms_dense<16, 4><<<grid_for(B, 16), 256, 0, st>>>(D, DT, CS, B, dout, din, ldout,
                                                                ldin, sof, coin, key, phase, zero);
   else if (B <= 1024)
-    k_block_terms_dense<32, 8><<<grid_for(B, 8), 256, 0, st>>>(D, DT, B, dout, din, ldout, ldin,
-                                                              sof, coin, key, phase, zero);
+    k_block_terms_dense<32, 8><<<grid_for(B, 8), 256, 0, st>>>(D, DT, CS, B, dout, din, ldout,
+                                                              ldin, sof, coin, key, phase, zero);
   else
-    k_block_terms_dense<64, 16><<<grid_for(B, 4), 256, 0, st>>>(D, DT, B, dout, din, ldout,
+    k_block_terms_dense<64, 16><<<grid_for(B, 4), 256, 0, st>>>(D, DT, CS, B, dout, din, ldout,
                                                                ldin, sof, coin, key, phase, zero);
 }
 
