@@ -129,6 +129,32 @@ def test_partition_trace(ctx, name, model, csr):
     assert eq_bits(rep.S, ctx.mdl(g, lab, rep.B, Options(model=model)))
 
 
+def weighted_dcsbm():
+    """20000 vertices, 120k edges of weight 1..3: the search passes B = 13000 and 8450 (dense
+    block matrix with the CSR deferred inside sweeps and column chunk sums) and the dense
+    ranges below with weighted degrees."""
+    g = gsap_amd.generate_dcsbm(20000, 20, 1.0, 2.0, 6.0, seed=13)
+    src = np.repeat(np.arange(g.N), np.diff(g.out_ptr))
+    w = np.random.default_rng(13).integers(1, 4, g.E).astype(np.int32)
+    return gsap_amd.Graph.from_edges(g.N, src, g.out_adj, w)
+
+
+def test_partition_trace_weighted_dense_range(ctx):
+    g = weighted_dcsbm()
+    o = Options(seed=7, trace=1)
+    print("partition weighted 20k", flush=True)
+    lab, rep = ctx.partition(g, o)
+    tr = ctx.trace()
+    assert any(r[0] == 2 and 8192 < r[1] <= 16384 for r in tr)
+    rlab, rrep, rtr = O.partition(g, o, trace_cap=len(tr) + 10, threads=8)
+    assert len(tr) == rrep.n_trace
+    for k, (a, b) in enumerate(zip(tr, rtr)):
+        assert a[:6] == b[:6], (k, a, b)
+        assert eq_bits(a[6], b[6]) and eq_bits(a[7], b[7]), (k, a, b)
+    assert np.array_equal(lab, rlab)
+    assert rep.B == rrep.B and eq_bits(rep.S, rrep.S)
+
+
 def test_partition_generated_c1(ctx):
     g = gsap_amd.generate_dcsbm(1000, 11, 0.0, 4.0, 20.0, seed=1)
     o = Options(seed=1)
