@@ -177,7 +177,7 @@ __global__ __launch_bounds__(1024) void k_small_bm(const int32_t *D, int32_t B, 
 // ---- vertices grouped by block (counting sort; order inside a block is irrelevant)
 constexpr int VT = 2048;  // vertices per workgroup
 constexpr int TILE_MAX_B = 8192;  // largest B of the row-tile path
-constexpr int INC_MAX_B = 16384;  // largest B of the incremental dense update
+constexpr int INC_MAX_B = 24576;  // largest B of the incremental dense update
 constexpr int TAIL_MAX_B = 8192;  // above it the CSR after an update comes from the edges
 constexpr int INC_TILE_B = 512;   // its LDS row-tile form runs up to this B, global atomics above
 

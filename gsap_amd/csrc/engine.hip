@@ -222,7 +222,7 @@ struct gsap_ctx {
   BMBuf bm;
   RebuildScratch rs;
   int64_t N = 0, E = 0;
-  int32_t dense_max_B = 16384;
+  int32_t dense_max_B = 24576;
   int32_t tile_max_B = 2048;
   // sub-contexts of batch / chain calls (one per worker thread)
   std::vector<gsap_ctx *> workers;
@@ -580,7 +580,7 @@ void alloc_work(gsap_ctx *c, int32_t np, int32_t dense_max_B) {
 }
 
 constexpr int32_t SMALL_B_TUNE = 128;  // the small rebuild path's limit (never tuned)
-int32_t resolve_dense(const gsap_options &o) { return o.dense_max_B > 0 ? o.dense_max_B : 16384; }
+int32_t resolve_dense(const gsap_options &o) { return o.dense_max_B > 0 ? o.dense_max_B : 24576; }
 NodalBounds resolve_nodal(const gsap_options &o) {
   NodalBounds b;
   if (o.nodal_max16) b.max16 = o.nodal_max16;

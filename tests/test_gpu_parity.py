@@ -181,7 +181,7 @@ def c3():
 
 @pytest.mark.parametrize("B,dense", [(71, 0), (5000, 0), (200000, 0), (20000, 0), (3000, 1)])
 def test_c3_blockmodel_and_batch(ctx, c3, B, dense):
-    """dense = 1 (and B > 16384) look the block matrix up in CSR/CSC rows of hundreds of
+    """dense = 1 (and B > 24576) look the block matrix up in CSR/CSC rows of hundreds of
     entries through the coarse index."""
     g = c3
     lab = labels_for(g, B, 9) if B < g.N else np.arange(g.N, dtype=np.int32)
@@ -315,7 +315,8 @@ def test_c3_merge_phase(ctx, c3, B, big, dense, exact):
 BM_KEYS = ("nnz", "rowptr", "col", "val", "colptr", "row", "cval", "dout", "din")
 
 
-@pytest.mark.parametrize("name,B,big,dense", [("c3", 20000, 0.3, 0), ("c3", 3000, 0.05, 1),
+@pytest.mark.parametrize("name,B,big,dense", [("c3", 20000, 0.3, 0), ("c3", 20000, 0.3, 1),
+                                              ("c3", 3000, 0.05, 1),
                                               ("c3", 200000, 0.0, 0), ("c3", 9000, 0.0, 0),
                                               ("heavy", 9000, 0.3, 1), ("heavy", 20000, 0.0, 1),
                                               ("hubs", 150, 0.0, 1), ("tiny_w", 16, 0.0, 1),
