@@ -39,7 +39,9 @@ KERNELS_OF = {
     "nodal_propose": ["k_nodal_propose"],
     "nodal": ["k_nodal"],
     "rebuild": ["k_label_", "k_tile_hist", "k_edge_hist", "k_hist_reduce", "k_dense_",
-                "k_small_bm", "k_transpose", "k_logs", "k_seg_", "k_scan_", "k_lab16"],
+                "k_small_bm", "k_transpose", "k_logs", "k_seg_", "k_scan_", "k_lab16",
+                "k_colsum"],
+    "accept_sum": ["k_apply_tiles", "k_final_sum"],
     "block_terms": ["k_block_terms"],
     "merge": ["k_merge", "k_argmin", "k_order_keys", "k_radix_", "k_big_blocks", "k_jump",
               "k_relabel_roots"],
